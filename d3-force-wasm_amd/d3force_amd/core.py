@@ -1,0 +1,274 @@
+"""Loader and sync layer: the host-side mirror of the reference's src/core.js.
+
+A *core* is a shared library exporting the tick engine's C ABI
+(include/d3fgpu.h): the MI355X library ``libd3fgpu.so`` by default. Each node
+table gets one ``Context`` (core.js:89-164) holding the core's simulation
+handle and numpy views of its x, y, vx, vy, fx, fy, fixed arrays; each native
+force holds a ``Slot`` (core.js:174-232) with its force handle.
+
+Differences from core.js, all forced by the GPU boundary:
+  * ``copy_in`` ends with ``sim_sync(h, SYNC_H2D)`` and ``copy_out`` starts with
+    ``sim_sync(h, SYNC_D2H)``: typed-array writes cannot be intercepted, so the
+    host says when the host arrays are authoritative.
+  * the random source is a C callback (``core_set_random``) in place of the
+    wasm import ``env.random`` (core.js:11-14); ``Slot.run`` swaps it per run
+    exactly as core.js:223-229 swaps ``random``.
+"""
+
+import ctypes
+import os
+import weakref
+
+import numpy as np
+
+# Force kinds, scalar slots and array slots (src/core/force.zig:8, :13, :21;
+# core.js:33-37).
+CENTER, COLLIDE, LINK, MANY_BODY, RADIAL, X, Y = range(7)
+(STRENGTH, CX, CY, THETA2, DISTANCE_MIN2, DISTANCE_MAX2, ITERATIONS,
+ PRECISION, ORDER, OPENING, SEED) = range(11)
+F0, F1, F2, SOURCE, TARGET = range(5)
+SCALARS = 11
+# Float parameter arrays per force kind (force.zig:54-61, core.js:40).
+FLOATS = [0, 1, 3, 1, 2, 2, 2]
+
+OK, GROWN = 0, 1
+BAD_HANDLE, OUT_OF_MEMORY, BAD_ARGUMENT, WORKER_FAILED = -1, -2, -3, -4
+_STATUSES = {-1: "BAD_HANDLE", -2: "OUT_OF_MEMORY", -3: "BAD_ARGUMENT", -4: "WORKER_FAILED"}
+
+SYNC_H2D, SYNC_D2H = 0, 1
+
+# sim_ptr slots (src/core/sim.zig:14).
+P_X, P_Y, P_VX, P_VY, P_FX, P_FY, P_FIXED, P_PERM, P_RNG, P_SNAP = range(10)
+
+
+class CoreError(RuntimeError):
+    """A negative core status, named as core.js:42-48 names it."""
+
+    def __init__(self, status):
+        self.status = status
+        super().__init__(_STATUSES.get(status, "core status %d" % status))
+
+
+def check(status):
+    """core.js:45-48: throw on a negative status, else pass it through."""
+    if status < 0:
+        raise CoreError(status)
+    return status
+
+
+_RANDOM_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p)
+
+_SIGNATURES = {
+    "core_set_random": (None, [_RANDOM_FN, ctypes.c_void_p]),
+    "sim_create": (ctypes.c_uint32, []),
+    "sim_destroy": (ctypes.c_int32, [ctypes.c_uint32]),
+    "sim_set_node_count": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32]),
+    "sim_ptr": (ctypes.c_void_p, [ctypes.c_uint32, ctypes.c_uint32]),
+    "sim_sync": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32]),
+    "sim_snapshot": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32]),
+    "sim_integrate": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_double]),
+    "tree_build": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32]),
+    "tree_info": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32]),
+    "tree_order": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
+    "force_create": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
+    "force_destroy": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32]),
+    "force_set_count": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    "force_ptr": (ctypes.c_void_p, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    "force_set_scalar": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double]),
+    "force_get_scalar": (ctypes.c_double, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    "force_run": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double]),
+}
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+GPU_LIBRARY = os.path.join(os.path.dirname(_HERE), "libd3fgpu.so")
+
+
+class Core:
+    """One loaded core library and its exported ABI.
+
+    ``prefix`` is prepended to every export name, so a library that exports
+    the same ABI under another prefix can be driven by the same facade.
+    """
+
+    def __init__(self, path, prefix=""):
+        if not os.path.exists(path):
+            raise OSError("core library not built: %s (run __graft_entry__.build())" % path)
+        self.path = path
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(self.lib, prefix + name)
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, name, fn)
+        self.random = None  # the running force's random source (core.js:12)
+        self._callback = _RANDOM_FN(self._draw)
+        self.core_set_random(self._callback, None)
+        self.contexts = weakref.WeakKeyDictionary()
+
+    def _draw(self, _ctx):
+        return float(self.random())
+
+    def bind(self, name, res, args):
+        """Binds an extra export (library-specific extensions)."""
+        fn = getattr(self.lib, self.prefix + name)
+        fn.restype = res
+        fn.argtypes = args
+        return fn
+
+
+_gpu_core = None
+
+
+def gpu_core():
+    """The MI355X core, loaded once. Raises when the library is missing: there
+    is no CPU fallback on the product path."""
+    global _gpu_core
+    if _gpu_core is None:
+        _gpu_core = Core(GPU_LIBRARY)
+    return _gpu_core
+
+
+def _view(ptr, dtype, n):
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    ctype = np.ctypeslib.as_ctypes_type(np.dtype(dtype))
+    buf = (ctype * n).from_address(ptr)
+    return np.ctypeslib.as_array(buf)
+
+
+class Context:
+    """One simulation region in a core for one node table (core.js:89-164)."""
+
+    def __init__(self, core):
+        self.core = core
+        self.h = core.sim_create()
+        if not self.h:
+            raise CoreError(OUT_OF_MEMORY)
+        self.n = 0
+        self.live = False  # true while a tick owns the arrays
+        self.ptrs = None
+        self.precision = None  # None = exact; dict(order, opening) = fast
+        self.slots = set()
+        self._finalizer = weakref.finalize(self, core.sim_destroy, self.h)
+
+    def resize(self, n):  # core.js:102-104
+        if n != self.n:
+            check(self.core.sim_set_node_count(self.h, n))
+            self.n = n
+            self.ptrs = None
+
+    def views(self):  # core.js:106-119, refreshed when an array moved
+        core, h, n = self.core, self.h, self.n
+        ptrs = tuple(core.sim_ptr(h, w) for w in range(7))
+        if ptrs == self.ptrs:
+            return
+        self.ptrs = ptrs
+        self.x = _view(ptrs[0], np.float64, n)
+        self.y = _view(ptrs[1], np.float64, n)
+        self.vx = _view(ptrs[2], np.float64, n)
+        self.vy = _view(ptrs[3], np.float64, n)
+        self.fx = _view(ptrs[4], np.float64, n)
+        self.fy = _view(ptrs[5], np.float64, n)
+        self.fixed = _view(ptrs[6], np.uint8, n)
+
+    def copy_in(self, nodes):  # core.js:121-131
+        n = len(nodes)
+        self.resize(n)
+        self.views()
+        if n:
+            self.x[:] = nodes.x
+            self.y[:] = nodes.y
+            self.vx[:] = nodes.vx
+            self.vy[:] = nodes.vy
+            self.fx[:] = nodes.fx
+            self.fy[:] = nodes.fy
+            self.fixed[:] = nodes.fx_set.astype(np.uint8) | (nodes.fy_set.astype(np.uint8) << 1)
+        check(self.core.sim_sync(self.h, SYNC_H2D))
+
+    def copy_out(self, nodes):  # core.js:133-140
+        check(self.core.sim_sync(self.h, SYNC_D2H))
+        self.views()
+        if self.n:
+            nodes.x[:] = self.x
+            nodes.y[:] = self.y
+            nodes.vx[:] = self.vx
+            nodes.vy[:] = self.vy
+
+    def step(self, slot, arg):  # core.js:155-159
+        if slot is not None:
+            check(self.core.force_run(self.h, slot.fh, arg))
+        else:
+            check(self.core.sim_integrate(self.h, arg))
+
+    def integrate(self, velocity_decay):  # core.js:161-163
+        self.step(None, velocity_decay)
+
+
+def context(nodes):
+    """core.js:166-170: the node table's Context in the table's core."""
+    core = nodes.core
+    if core is None:
+        core = nodes.core = gpu_core()
+    c = core.contexts.get(nodes)
+    if c is None:
+        c = Context(core)
+        core.contexts[nodes] = c
+    return c
+
+
+class Slot:
+    """A native force's handle in the core (core.js:174-232)."""
+
+    def __init__(self, kind):
+        self.kind = kind
+        self.ctx = None
+        self.fh = 0
+        self.m = 0
+
+    def bind(self, nodes):  # core.js:182-191
+        ctx = context(nodes)
+        if ctx is not self.ctx:
+            if self.ctx is not None:
+                self.ctx.core.force_destroy(self.ctx.h, self.fh)
+                self.ctx.slots.discard(self)
+            self.fh = ctx.core.force_create(ctx.h, self.kind)
+            if not self.fh:
+                raise CoreError(OUT_OF_MEMORY)
+            self.ctx, self.m = ctx, 0
+            ctx.slots.add(self)
+        ctx.resize(len(nodes))
+
+    def count(self, m):  # core.js:193-196
+        check(self.ctx.core.force_set_count(self.ctx.h, self.fh, m))
+        self.m = m
+
+    def f64(self, which, m):  # core.js:199-202
+        return _view(self.ctx.core.force_ptr(self.ctx.h, self.fh, which), np.float64, m)
+
+    def u32(self, which, m):  # core.js:204-207
+        return _view(self.ctx.core.force_ptr(self.ctx.h, self.fh, which), np.uint32, m)
+
+    def scalar(self, which, v):  # core.js:209-211
+        self.ctx.core.force_set_scalar(self.ctx.h, self.fh, which, float(v))
+
+    def run(self, nodes, alpha, rand=None):  # core.js:213-231
+        ctx = self.ctx
+        fast = ctx.precision
+        if not ctx.live:
+            ctx.copy_in(nodes)
+        self.scalar(PRECISION, 1 if fast else 0)
+        if fast:
+            self.scalar(ORDER, fast["order"])
+            self.scalar(OPENING, fast["opening"])
+            if self.kind in (MANY_BODY, LINK, COLLIDE):
+                self.scalar(SEED, rand())
+        core = ctx.core
+        previous = core.random
+        core.random = rand
+        try:
+            ctx.step(self, alpha)
+        finally:
+            core.random = previous
+            if not ctx.live:
+                ctx.copy_out(nodes)
