@@ -1,0 +1,108 @@
+// forceLink, exact mode, on gfx950 (reference: src/core/link.zig, d3's
+// Gauss-Seidel pass over edges in user order).
+//
+// Dependency-level schedule: level(e) = 1 + max(level of the previous edge at
+// its source, level of the previous edge at its target). Edges of one level
+// share no node, so they commute bit for bit, and running the levels in
+// order gives every node its edges in user order with the same operands as
+// the serial pass. The schedule depends only on source/target and is built
+// when the links change (runtime.hip); edge arrays are stored level-major so
+// each level is one coalesced launch.
+//
+// A zero or NaN delta makes d3 draw from random in edge order
+// (link.zig:23-27). The level pass flags it; the host then restores the
+// velocities and runs k_link_serial, which stops before each such edge so the
+// host can draw in order and resume.
+#include "common.h"
+
+namespace d3f {
+namespace {
+
+constexpr int LINK_THREADS = 256;
+
+__global__ __launch_bounds__(LINK_THREADS) void k_link_level(
+    const uint32_t* __restrict__ src, const uint32_t* __restrict__ tgt, const double* __restrict__ strength,
+    const double* __restrict__ distance, const double* __restrict__ bias, uint32_t lo, uint32_t hi,
+    const double* __restrict__ xs, const double* __restrict__ ys, double* __restrict__ vxs,
+    double* __restrict__ vys, double alpha, uint32_t* __restrict__ zero_flag) {
+  const uint32_t e = lo + blockIdx.x * LINK_THREADS + threadIdx.x;
+  if (e >= hi) return;
+  const uint32_t s = src[e], t = tgt[e];
+  double x = xs[t] + vxs[t] - xs[s] - vxs[s];
+  double y = ys[t] + vys[t] - ys[s] - vys[s];
+  if (x == 0 || x != x || y == 0 || y != y) {
+    atomicOr(zero_flag, 1u);
+    return;
+  }
+  double l = __builtin_sqrt(x * x + y * y);
+  l = (l - distance[e]) / l * alpha * strength[e];
+  x *= l;
+  y *= l;
+  double b = bias[e];
+  vxs[t] -= x * b;
+  vys[t] -= y * b;
+  b = 1 - b;
+  vxs[s] += x * b;
+  vys[s] += y * b;
+}
+
+// The serial pass from state->edge, stopping before an edge that needs a
+// draw unless state carries its jiggle values (one thread).
+__global__ void k_link_serial(const uint32_t* __restrict__ src, const uint32_t* __restrict__ tgt,
+                              const double* __restrict__ strength, const double* __restrict__ distance,
+                              const double* __restrict__ bias, uint32_t m, const double* __restrict__ xs,
+                              const double* __restrict__ ys, double* __restrict__ vxs, double* __restrict__ vys,
+                              double alpha, LinkSerialState* __restrict__ st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t e = st->edge;
+  bool has_jig = st->has_jig != 0;
+  for (; e < m; ++e) {
+    const uint32_t s = src[e], t = tgt[e];
+    double x = xs[t] + vxs[t] - xs[s] - vxs[s];
+    double y = ys[t] + vys[t] - ys[s] - vys[s];
+    const bool nx = x == 0 || x != x, ny = y == 0 || y != y;
+    if (nx || ny) {
+      if (!has_jig) {
+        st->edge = e;
+        st->need = (nx ? 1u : 0u) | (ny ? 2u : 0u);
+        st->has_jig = 0;
+        return;
+      }
+      if (nx) x = st->jig[0];
+      if (ny) y = st->jig[1];
+    }
+    has_jig = false;
+    double l = __builtin_sqrt(x * x + y * y);
+    l = (l - distance[e]) / l * alpha * strength[e];
+    x *= l;
+    y *= l;
+    double b = bias[e];
+    vxs[t] -= x * b;
+    vys[t] -= y * b;
+    b = 1 - b;
+    vxs[s] += x * b;
+    vys[s] += y * b;
+  }
+  st->edge = m;
+  st->need = 0;
+  st->has_jig = 0;
+}
+
+}  // namespace
+
+hipError_t link_level(const LinkLevelArgs& a, uint32_t lo, uint32_t hi, hipStream_t stream) {
+  if (hi <= lo) return hipSuccess;
+  hipLaunchKernelGGL(k_link_level, dim3(ceil_div(hi - lo, LINK_THREADS)), dim3(LINK_THREADS), 0, stream, a.src,
+                     a.tgt, a.strength, a.distance, a.bias, lo, hi, a.x, a.y, a.vx, a.vy, a.alpha, a.zero_flag);
+  return hipGetLastError();
+}
+
+hipError_t link_serial(const uint32_t* src, const uint32_t* tgt, const double* strength, const double* distance,
+                       const double* bias, uint32_t m, const double* x, const double* y, double* vx, double* vy,
+                       double alpha, LinkSerialState* state, hipStream_t stream) {
+  hipLaunchKernelGGL(k_link_serial, dim3(1), dim3(64), 0, stream, src, tgt, strength, distance, bias, m, x, y, vx,
+                     vy, alpha, state);
+  return hipGetLastError();
+}
+
+}  // namespace d3f

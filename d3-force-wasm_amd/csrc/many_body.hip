@@ -1,0 +1,322 @@
+// forceManyBody, exact mode, on gfx950 (reference: src/core/many_body.zig,
+// itself d3-force's manyBody.js over the quadtree).
+//
+// Accumulate (visitAfter): one thread per leaf computes the leaf's charge
+// (its chain summed from the head down, many_body.zig:125-132), then climbs:
+// each parent's arrival counter is bumped with acquire-release ordering and
+// the last child to arrive computes the parent from children 0..3 in order
+// (many_body.zig:112-121), so every sum has d3's operands in d3's order.
+//
+// Traversal (visit): one lane per node, 64 nodes per wave in Morton order.
+// The wave walks the union of its lanes' d3 traversals depth first with one
+// shared LDS stack of (cell, lane mask) entries: a lane processes a cell only
+// if every ancestor on the path was opened by that lane, so each lane sees
+// exactly its own serial visit sequence (children 0..3, many_body.zig:190-259)
+// and accumulates its velocity in d3's order. Cells are read with uniform
+// (scalar) loads of one 64-byte record.
+//
+// Jiggle (many_body.zig:65-83): with RECORD a lane that meets a zero delta
+// stops without writing and is listed; the host then sorts the list, COUNT
+// gives each listed node's number of draws (fixed by positions and the tree
+// alone), the host draws them from the JS random source in ascending node
+// order, and REPLAY recomputes those nodes consuming their draws in order.
+#include "common.h"
+
+namespace d3f {
+namespace {
+
+constexpr int ACC_THREADS = 256;
+constexpr int TRAV_THREADS = 256;
+constexpr int TRAV_WAVES = TRAV_THREADS / 64;
+
+__device__ inline bool truthy(double v) { return v != 0 && v == v; }
+
+__device__ inline void put(TCell* __restrict__ tc, Cell* __restrict__ cells, uint32_t c, const Cell& cell, double qx,
+                           double qy, double value, double theta2, uint32_t head, uint32_t flags) {
+  TCell t;
+  t.qx = qx;
+  t.qy = qy;
+  t.value = value;
+  const double w = cell.x1 - cell.x0;
+  t.w2t = w * w / theta2;  // many_body.zig:202, identical operands per cell
+  for (int k = 0; k < 4; ++k) t.child[k] = cell.child[k];
+  t.first = cell.first;
+  t.count = cell.count;
+  t.head = head;
+  t.flags = flags;
+  tc[c] = t;
+  cells[c].x = qx;
+  cells[c].y = qy;
+  cells[c].value = value;
+}
+
+__global__ __launch_bounds__(ACC_THREADS) void k_accumulate(Cell* __restrict__ cells, TCell* __restrict__ tc,
+                                                            const uint32_t* __restrict__ parent,
+                                                            uint32_t* __restrict__ arrive,
+                                                            const uint32_t* __restrict__ perm,
+                                                            const double* __restrict__ xs,
+                                                            const double* __restrict__ ys,
+                                                            const double* __restrict__ s, double theta2,
+                                                            uint32_t len, uint32_t max_climb) {
+  const uint32_t c = blockIdx.x * ACC_THREADS + threadIdx.x;
+  if (c >= len) return;
+  Cell cell = cells[c];
+  if (cell.child[0] != NONE || cell.child[1] != NONE || cell.child[2] != NONE || cell.child[3] != NONE) return;
+  // Leaf: the chain is perm[first + count - 1] down to perm[first].
+  const uint32_t head = perm[cell.first + cell.count - 1];
+  double strength = 0;
+  for (uint32_t k = cell.first + cell.count; k > cell.first;) {
+    --k;
+    strength += s[perm[k]];
+  }
+  put(tc, cells, c, cell, xs[head], ys[head], strength, theta2, head, TC_LEAF | (truthy(strength) ? TC_REACH : 0));
+  uint32_t p = parent[c];
+  for (uint32_t step = 0; p < len && step < max_climb; ++step) {
+    const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const Cell pc = cells[p];
+    uint32_t nch = 0;
+    for (int k = 0; k < 4; ++k) nch += pc.child[k] != NONE;
+    if (old + 1 != nch) return;
+    double st = 0, weight = 0, x = 0, y = 0;
+    bool reach = false;
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t ch = pc.child[k];
+      if (ch == NONE) continue;
+      const double v = __hip_atomic_load(&tc[ch].value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const double qx = __hip_atomic_load(&tc[ch].qx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const double qy = __hip_atomic_load(&tc[ch].qy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t fl = __hip_atomic_load(&tc[ch].flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const double w = fabs(v);
+      if (w > 0) {  // `c = Math.abs(q.value)` is truthy
+        st += v;
+        weight += w;
+        x += w * qx;
+        y += w * qy;
+      }
+      reach = reach || (fl & TC_REACH);
+    }
+    put(tc, cells, p, pc, x / weight, y / weight, st, theta2, NONE, (reach && truthy(st)) ? TC_REACH : 0);
+    p = parent[p];
+  }
+}
+
+// d3's jiggle: (random() - 0.5) * 1e-6 (rng.zig:7-11).
+__device__ inline double jiggle(double r) { return (r - 0.5) * 1e-6; }
+
+template <int MODE, bool GLOBAL_STACK>
+__global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
+    const TCell* __restrict__ tc, const uint32_t* __restrict__ order, uint32_t count,
+    const uint32_t* __restrict__ perm, const double* __restrict__ xs, const double* __restrict__ ys,
+    const double* __restrict__ s, double* __restrict__ vxs, double* __restrict__ vys, double dmin2, double dmax2,
+    double alpha, uint32_t* __restrict__ flagged, uint32_t* __restrict__ n_flagged,
+    uint32_t* __restrict__ draws_count, const uint32_t* __restrict__ draw_off, const double* __restrict__ draws,
+    uint32_t stack_cap, uint32_t* __restrict__ gstack_cells, uint64_t* __restrict__ gstack_masks) {
+  extern __shared__ uint64_t lds[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t* st_mask;
+  uint32_t* st_cell;
+  if (GLOBAL_STACK) {
+    const size_t w = (size_t)blockIdx.x * TRAV_WAVES + wave;
+    st_mask = gstack_masks + w * stack_cap;
+    st_cell = gstack_cells + w * stack_cap;
+  } else {
+    st_mask = lds + (size_t)wave * stack_cap;
+    st_cell = (uint32_t*)(lds + (size_t)TRAV_WAVES * stack_cap) + (size_t)wave * stack_cap;
+  }
+  const uint32_t slot = blockIdx.x * TRAV_THREADS + threadIdx.x;
+  const bool active = slot < count;
+  const uint32_t i = active ? order[slot] : 0;
+  double nx = 0, ny = 0, vx = 0, vy = 0;
+  if (active) {
+    nx = xs[i];
+    ny = ys[i];
+    vx = vxs[i];
+    vy = vys[i];
+  }
+  uint64_t alive = __ballot(active);
+  if (alive == 0) return;
+  uint32_t ndraw = 0;
+  uint32_t dptr = (MODE == TRAV_REPLAY && active) ? draw_off[slot] : 0;
+  bool aborted = false;
+  if (lane == 0) {
+    st_cell[0] = 0;
+    st_mask[0] = alive;
+  }
+  int sp = 1;
+  while (sp > 0) {
+    --sp;
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t c = __builtin_amdgcn_readfirstlane(st_cell[sp]);
+    const uint64_t mraw = st_mask[sp];
+    const uint32_t mlo = __builtin_amdgcn_readfirstlane((uint32_t)mraw);
+    const uint32_t mhi = __builtin_amdgcn_readfirstlane((uint32_t)(mraw >> 32));
+    const uint64_t M = ((uint64_t)mhi << 32 | mlo) & alive;
+    if (M == 0) continue;
+    const TCell* q = tc + c;
+    const double value = q->value;
+    if (!truthy(value)) continue;
+    const bool in = (M >> lane) & 1;
+    double dx = 0, dy = 0, l = 0;
+    bool open = false;
+    if (in) {
+      dx = q->qx - nx;
+      dy = q->qy - ny;
+      l = dx * dx + dy * dy;
+      if (q->w2t < l) {  // Barnes-Hut approximation
+        if (l < dmax2) {
+          const bool jx = dx == 0, jy = dy == 0;
+          if (MODE == TRAV_RECORD && (jx || jy)) {
+            aborted = true;
+          } else {
+            if (jx) {
+              if (MODE == TRAV_COUNT) { ++ndraw; dx = 1; } else dx = jiggle(draws[dptr++]);
+              l += dx * dx;
+            }
+            if (jy) {
+              if (MODE == TRAV_COUNT) { ++ndraw; dy = 1; } else dy = jiggle(draws[dptr++]);
+              l += dy * dy;
+            }
+            if (l < dmin2) l = __builtin_sqrt(dmin2 * l);
+            vx += dx * value * alpha / l;
+            vy += dy * value * alpha / l;
+          }
+        }
+      } else {
+        open = true;
+      }
+    }
+    if (MODE == TRAV_RECORD) alive &= ~__ballot(aborted);
+    const uint64_t O = __ballot(open);
+    if (O == 0) continue;
+    if (!(q->flags & TC_LEAF)) {
+      if (sp + 4 > (int)stack_cap) break;  // cannot happen with cap = 3 depth + 4
+      // Descend: push children 3..0 so 0 is visited first.
+      for (int k = 3; k >= 0; --k) {
+        const uint32_t ch = q->child[k];
+        if (ch != NONE) {
+          if (lane == 0) {
+            st_cell[sp] = ch;
+            st_mask[sp] = O;
+          }
+          ++sp;
+        }
+      }
+      continue;
+    }
+    // Leaf: points directly (many_body.zig:208-258).
+    if (open && !(l >= dmax2)) {
+      const uint32_t first = q->first, cnt = q->count, head = q->head;
+      bool go = true;
+      if (head != i || cnt > 1) {
+        const bool jx = dx == 0, jy = dy == 0;
+        if (MODE == TRAV_RECORD && (jx || jy)) {
+          aborted = true;
+          go = false;
+        } else {
+          if (jx) {
+            if (MODE == TRAV_COUNT) { ++ndraw; dx = 1; } else dx = jiggle(draws[dptr++]);
+            l += dx * dx;
+          }
+          if (jy) {
+            if (MODE == TRAV_COUNT) { ++ndraw; dy = 1; } else dy = jiggle(draws[dptr++]);
+            l += dy * dy;
+          }
+          if (l < dmin2) l = __builtin_sqrt(dmin2 * l);
+        }
+      }
+      if (go) {
+        for (uint32_t k = first + cnt; k > first;) {
+          --k;
+          const uint32_t j = perm[k];
+          if (j != i) {
+            const double wj = s[j] * alpha / l;
+            vx += dx * wj;
+            vy += dy * wj;
+          }
+        }
+      }
+    }
+    if (MODE == TRAV_RECORD) alive &= ~__ballot(aborted);
+  }
+  if (!active) return;
+  if (MODE == TRAV_RECORD && aborted) {
+    const uint32_t k = atomicAdd(n_flagged, 1u);
+    flagged[k] = i;
+  } else if (MODE == TRAV_COUNT) {
+    draws_count[slot] = ndraw;
+  } else {
+    vxs[i] = vx;
+    vys[i] = vy;
+  }
+}
+
+__global__ void k_nan_nodes(const uint32_t* __restrict__ rest, uint32_t count, const TCell* __restrict__ tc,
+                            double* __restrict__ vx, double* __restrict__ vy) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  if (!(tc[0].flags & TC_REACH)) return;
+  const uint32_t i = rest[k];
+  const double nan = __builtin_nan("");
+  vx[i] = vx[i] + nan;
+  vy[i] = vy[i] + nan;
+}
+
+}  // namespace
+
+hipError_t mb_accumulate(TreeBuffers& t, const double* x, const double* y, const double* strengths, double theta2,
+                         hipStream_t stream) {
+  const uint32_t len = t.meta_host->len;
+  if (len == 0) return hipSuccess;
+  HIPCHECK(hipMemsetAsync(t.arrive, 0, (size_t)len * sizeof(uint32_t), stream));
+  hipLaunchKernelGGL(k_accumulate, dim3(ceil_div(len, ACC_THREADS)), dim3(ACC_THREADS), 0, stream, t.cells,
+                     t.tcells, t.parent, t.arrive, t.perm, x, y, strengths, theta2, len, t.meta_host->depth + 2);
+  return hipGetLastError();
+}
+
+static uint32_t* g_stack_cells = nullptr;
+static uint64_t* g_stack_masks = nullptr;
+static size_t g_stack_entries = 0;
+
+hipError_t mb_traverse(const TravArgs& a, int mode, hipStream_t stream) {
+  if (a.count == 0) return hipSuccess;
+  const uint32_t blocks = ceil_div(a.count, TRAV_THREADS);
+  const size_t lds = (size_t)TRAV_WAVES * a.stack_cap * (sizeof(uint64_t) + sizeof(uint32_t));
+  const bool global = lds > 48 * 1024;
+  if (global) {
+    const size_t need = (size_t)blocks * TRAV_WAVES * a.stack_cap;
+    if (need > g_stack_entries) {
+      if (g_stack_cells) HIPCHECK(hipFree(g_stack_cells));
+      if (g_stack_masks) HIPCHECK(hipFree(g_stack_masks));
+      g_stack_cells = nullptr;
+      g_stack_masks = nullptr;
+      HIPCHECK(hipMalloc(&g_stack_cells, need * sizeof(uint32_t)));
+      HIPCHECK(hipMalloc(&g_stack_masks, need * sizeof(uint64_t)));
+      g_stack_entries = need;
+    }
+  }
+#define TRAV_LAUNCH(M, G)                                                                                        \
+  hipLaunchKernelGGL((k_traverse<M, G>), dim3(blocks), dim3(TRAV_THREADS), G ? 0 : lds, stream, a.tc, a.order,   \
+                     a.count, a.perm, a.x, a.y, a.s, a.vx, a.vy, a.dmin2, a.dmax2, a.alpha, a.flagged,          \
+                     a.n_flagged, a.draws_count, a.draw_off, a.draws, a.stack_cap, g_stack_cells, g_stack_masks)
+  if (global) {
+    if (mode == TRAV_RECORD) TRAV_LAUNCH(TRAV_RECORD, true);
+    else if (mode == TRAV_COUNT) TRAV_LAUNCH(TRAV_COUNT, true);
+    else TRAV_LAUNCH(TRAV_REPLAY, true);
+  } else {
+    if (mode == TRAV_RECORD) TRAV_LAUNCH(TRAV_RECORD, false);
+    else if (mode == TRAV_COUNT) TRAV_LAUNCH(TRAV_COUNT, false);
+    else TRAV_LAUNCH(TRAV_REPLAY, false);
+  }
+#undef TRAV_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t mb_nan_nodes(const TreeBuffers& t, uint32_t n, double* vx, double* vy, hipStream_t stream) {
+  const uint32_t m = t.meta_host->m;
+  if (!t.meta_host->valid || n <= m) return hipSuccess;
+  const uint32_t count = n - m;
+  hipLaunchKernelGGL(k_nan_nodes, dim3(ceil_div(count, 256)), dim3(256), 0, stream, t.rest, count, t.tcells, vx, vy);
+  return hipGetLastError();
+}
+
+}  // namespace d3f

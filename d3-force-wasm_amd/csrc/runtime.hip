@@ -1,0 +1,769 @@
+// Host runtime and C ABI of the d3fgpu library (include/d3fgpu.h): the
+// replacement for the reference core's exported ABI (src/core/root.zig) with
+// the simulation state resident in HBM.
+//
+// Ownership: each simulation owns pinned host arrays (what sim_ptr returns;
+// the facade reads and writes them) and their device mirrors; sim_sync moves
+// node arrays between the two. Each force owns pinned parameter arrays and
+// device copies, uploaded on the first force_run after the facade could have
+// written them (any force_ptr or force_set_count call).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/d3fgpu.h"
+#include "common.h"
+
+#define EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace d3f {
+namespace {
+
+enum Status : int32_t { OK = 0, GROWN = 1, BAD_HANDLE = -1, OUT_OF_MEMORY = -2, BAD_ARGUMENT = -3, WORKER_FAILED = -4 };
+enum Kind : uint32_t { K_CENTER, K_COLLIDE, K_LINK, K_MANY_BODY, K_RADIAL, K_X, K_Y, K_COUNT };
+enum Scalar : uint32_t { S_STRENGTH, S_X, S_Y, S_THETA2, S_DMIN2, S_DMAX2, S_ITERATIONS, S_PRECISION, S_ORDER,
+                         S_OPENING, S_SEED, S_COUNT };
+constexpr uint32_t MIN_N = 2000;  // fmm.zig:43, the fast-mode crossover
+
+d3f_random_fn g_random = nullptr;
+void* g_random_ctx = nullptr;
+bool g_timing = false;
+
+double draw() { return g_random ? g_random(g_random_ctx) : 0.5; }
+double jiggle_host(double r) { return (r - 0.5) * 1e-6; }  // rng.zig:7-11
+
+int32_t status_of(hipError_t e) {
+  if (e == hipSuccess) return OK;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return OUT_OF_MEMORY;
+  return WORKER_FAILED;
+}
+
+// A pinned host array with capacity; realloc keeps the contents.
+template <class T>
+struct Pinned {
+  T* p = nullptr;
+  size_t cap = 0;
+  hipError_t reserve(size_t n, size_t keep) {
+    if (n <= cap) return hipSuccess;
+    T* q = nullptr;
+    HIPCHECK(hipHostMalloc((void**)&q, n * sizeof(T), hipHostMallocDefault));
+    if (p) {
+      if (keep) std::memcpy(q, p, std::min(keep, cap) * sizeof(T));
+      HIPCHECK(hipHostFree(p));
+    }
+    p = q;
+    cap = n;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+template <class T>
+struct Device {
+  T* p = nullptr;
+  size_t cap = 0;
+  hipError_t reserve(size_t n, size_t keep = 0, hipStream_t stream = nullptr) {
+    if (n <= cap) return hipSuccess;
+    T* q = nullptr;
+    HIPCHECK(hipMalloc((void**)&q, n * sizeof(T)));
+    if (p) {
+      if (keep) HIPCHECK(hipMemcpyAsync(q, p, std::min(keep, cap) * sizeof(T), hipMemcpyDeviceToDevice, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      HIPCHECK(hipFree(p));
+    }
+    p = q;
+    cap = n;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct Force {
+  Kind kind;
+  double scalars[S_COUNT] = {0};
+  uint32_t count = 0;
+  Pinned<double> f[3];
+  Pinned<uint32_t> source, target;
+  Device<double> df[3];
+  Device<uint32_t> dsrc, dtgt;
+  bool dirty = true;
+  // exact link schedule, level-major
+  bool sched_valid = false;
+  uint32_t sched_n = 0;
+  std::vector<uint32_t> level_off;
+  Device<uint32_t> lsrc, ltgt;
+  Device<double> lf[3];
+
+  explicit Force(Kind k) : kind(k) {}
+  ~Force() {
+    for (auto& a : f) a.release();
+    source.release();
+    target.release();
+    for (auto& a : df) a.release();
+    dsrc.release();
+    dtgt.release();
+    lsrc.release();
+    ltgt.release();
+    for (auto& a : lf) a.release();
+  }
+  bool node_sized() const { return kind != K_LINK && kind != K_CENTER; }  // force.zig:50-52
+  int float_arrays() const {                                              // force.zig:54-61
+    switch (kind) {
+      case K_CENTER: return 0;
+      case K_COLLIDE: case K_MANY_BODY: return 1;
+      case K_RADIAL: case K_X: case K_Y: return 2;
+      default: return 3;
+    }
+  }
+  // force.zig:65-82: new float slots NaN, new index slots 0.
+  hipError_t set_count(uint32_t c) {
+    const int k = float_arrays();
+    for (int i = 0; i < k; ++i) {
+      HIPCHECK(f[i].reserve(c > 0 ? c : 1, count));
+      for (uint32_t j = count; j < c; ++j) f[i].p[j] = __builtin_nan("");
+    }
+    if (kind == K_LINK) {
+      HIPCHECK(source.reserve(c > 0 ? c : 1, count));
+      HIPCHECK(target.reserve(c > 0 ? c : 1, count));
+      for (uint32_t j = count; j < c; ++j) source.p[j] = target.p[j] = 0;
+    }
+    count = c;
+    dirty = true;
+    sched_valid = false;
+    return hipSuccess;
+  }
+  double scalar(Scalar s) const { return scalars[s]; }
+};
+
+struct Sim {
+  uint32_t n = 0, cap = 0;
+  Pinned<double> x, y, vx, vy, fx, fy, snap;
+  Pinned<uint8_t> fixed;
+  Device<double> dx, dy, dvx, dvy, dfx, dfy, dpx, dpy, dsvx, dsvy;
+  Device<uint8_t> dfixed;
+  TreeBuffers tree;
+  hipStream_t stream = nullptr;
+  std::vector<Force*> forces;
+  // scratch
+  Device<uint32_t> flagged, order, draws_count, draw_off;
+  Device<double> draws;
+  uint32_t* dscalar_u = nullptr;   // [0] flagged count, [1] link zero flag
+  double* dscalar_d = nullptr;     // [0..1] center shift
+  LinkSerialState* dlink = nullptr;
+  uint32_t* hscalar_u = nullptr;   // pinned mirrors
+  LinkSerialState* hlink = nullptr;
+  // host copies for tree introspection
+  Pinned<Cell> hcells;
+  Pinned<uint32_t> hperm;
+  double hextent[4];
+  d3f_stats stats{};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+
+  ~Sim() {
+    for (Force* f : forces) delete f;
+    for (auto* a : {&x, &y, &vx, &vy, &fx, &fy, &snap}) a->release();
+    fixed.release();
+    for (auto* a : {&dx, &dy, &dvx, &dvy, &dfx, &dfy, &dpx, &dpy, &dsvx, &dsvy}) a->release();
+    dfixed.release();
+    tree_free(tree);
+    flagged.release();
+    order.release();
+    draws_count.release();
+    draw_off.release();
+    draws.release();
+    if (dscalar_u) (void)hipFree(dscalar_u);
+    if (dscalar_d) (void)hipFree(dscalar_d);
+    if (dlink) (void)hipFree(dlink);
+    if (hscalar_u) (void)hipHostFree(hscalar_u);
+    if (hlink) (void)hipHostFree(hlink);
+    hcells.release();
+    hperm.release();
+    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  hipError_t init() {
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(hipMalloc(&dscalar_u, 16 * sizeof(uint32_t)));
+    HIPCHECK(hipMalloc(&dscalar_d, 16 * sizeof(double)));
+    HIPCHECK(hipMalloc(&dlink, sizeof(LinkSerialState)));
+    HIPCHECK(hipHostMalloc((void**)&hscalar_u, 16 * sizeof(uint32_t), hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc((void**)&hlink, sizeof(LinkSerialState), hipHostMallocDefault));
+    HIPCHECK(hipEventCreate(&ev[0]));
+    HIPCHECK(hipEventCreate(&ev[1]));
+    for (int k = 0; k < 4; ++k) hextent[k] = __builtin_nan("");
+    return hipSuccess;
+  }
+
+  // sim.zig:59-73
+  hipError_t set_node_count(uint32_t nn, bool* grown) {
+    const uint32_t old_n = n;
+    n = nn;
+    *grown = false;
+    for (Force* f : forces)
+      if (f && f->node_sized() && f->count < nn) {
+        HIPCHECK(f->set_count(nn));
+        *grown = true;
+      }
+    if (nn <= cap) return hipSuccess;
+    const uint32_t c = std::max(std::max(nn, cap * 2), 16u);
+    for (auto* a : {&x, &y, &vx, &vy, &fx, &fy}) HIPCHECK(a->reserve(c, old_n));
+    HIPCHECK(fixed.reserve(c, old_n));
+    for (auto* a : {&dx, &dy, &dvx, &dvy, &dfx, &dfy}) HIPCHECK(a->reserve(c, old_n, stream));
+    for (auto* a : {&dpx, &dpy, &dsvx, &dsvy}) HIPCHECK(a->reserve(c, 0, stream));
+    HIPCHECK(dfixed.reserve(c, old_n, stream));
+    HIPCHECK(flagged.reserve(c, 0, stream));
+    HIPCHECK(order.reserve(c, 0, stream));
+    HIPCHECK(draws_count.reserve(c, 0, stream));
+    HIPCHECK(draw_off.reserve(c, 0, stream));
+    cap = c;
+    *grown = true;
+    return hipSuccess;
+  }
+
+  void timer_start() {
+    if (g_timing) (void)hipEventRecord(ev[0], stream);
+  }
+  double timer_stop() {
+    if (!g_timing) return 0;
+    float ms = 0;
+    (void)hipEventRecord(ev[1], stream);
+    (void)hipEventSynchronize(ev[1]);
+    (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+    return ms;
+  }
+};
+
+std::vector<Sim*> g_sims;
+
+Sim* get(uint32_t h) {
+  if (h == 0 || h > g_sims.size()) return nullptr;
+  return g_sims[h - 1];
+}
+Force* get_force(Sim* s, uint32_t fh) {
+  if (!s || fh == 0 || fh > s->forces.size()) return nullptr;
+  return s->forces[fh - 1];
+}
+
+bool fast_mode(const Sim& s, const Force& f) {  // root.zig:246-248
+  return f.scalar(S_PRECISION) == 1 && s.n >= MIN_N;
+}
+
+// Uploads a force's parameter arrays when the facade may have written them.
+hipError_t upload(Sim& s, Force& f) {
+  if (!f.dirty) return hipSuccess;
+  const size_t c = f.count > 0 ? f.count : 1;
+  for (int i = 0; i < f.float_arrays(); ++i) {
+    HIPCHECK(f.df[i].reserve(c, 0, s.stream));
+    if (f.count) HIPCHECK(hipMemcpyAsync(f.df[i].p, f.f[i].p, f.count * sizeof(double), hipMemcpyHostToDevice, s.stream));
+  }
+  if (f.kind == K_LINK) {
+    HIPCHECK(f.dsrc.reserve(c, 0, s.stream));
+    HIPCHECK(f.dtgt.reserve(c, 0, s.stream));
+    if (f.count) {
+      HIPCHECK(hipMemcpyAsync(f.dsrc.p, f.source.p, f.count * sizeof(uint32_t), hipMemcpyHostToDevice, s.stream));
+      HIPCHECK(hipMemcpyAsync(f.dtgt.p, f.target.p, f.count * sizeof(uint32_t), hipMemcpyHostToDevice, s.stream));
+    }
+  }
+  HIPCHECK(hipStreamSynchronize(s.stream));
+  f.dirty = false;
+  f.sched_valid = false;
+  return hipSuccess;
+}
+
+// ---------------------------------------------------------------- many-body (exact)
+int32_t run_many_body_exact(Sim& s, Force& f, double alpha) {
+  TreeBuffers& t = s.tree;
+  s.timer_start();
+  hipError_t e = tree_build_exact(t, s.dx.p, s.dy.p, s.n, s.stream);
+  if (e != hipSuccess) return status_of(e);
+  s.stats.tree_ms = s.timer_stop();
+  if (!t.meta_host->valid) return OK;  // many_body.zig:33
+  s.timer_start();
+  e = mb_accumulate(t, s.dx.p, s.dy.p, f.df[0].p, f.scalar(S_THETA2), s.stream);
+  if (e != hipSuccess) return status_of(e);
+  s.stats.accumulate_ms = s.timer_stop();
+  TravArgs a{};
+  a.tc = t.tcells;
+  a.order = t.perm;
+  a.count = t.meta_host->m;
+  a.perm = t.perm;
+  a.x = s.dx.p;
+  a.y = s.dy.p;
+  a.s = f.df[0].p;
+  a.vx = s.dvx.p;
+  a.vy = s.dvy.p;
+  a.dmin2 = f.scalar(S_DMIN2);
+  a.dmax2 = f.scalar(S_DMAX2);
+  a.alpha = alpha;
+  a.flagged = s.flagged.p;
+  a.n_flagged = s.dscalar_u;
+  a.stack_cap = 3 * t.meta_host->depth + 4;
+  s.timer_start();
+  if ((e = hipMemsetAsync(s.dscalar_u, 0, sizeof(uint32_t), s.stream)) != hipSuccess) return status_of(e);
+  if ((e = mb_traverse(a, TRAV_RECORD, s.stream)) != hipSuccess) return status_of(e);
+  if ((e = mb_nan_nodes(t, s.n, s.dvx.p, s.dvy.p, s.stream)) != hipSuccess) return status_of(e);
+  if ((e = hipMemcpyAsync(s.hscalar_u, s.dscalar_u, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream)) != hipSuccess)
+    return status_of(e);
+  if ((e = hipStreamSynchronize(s.stream)) != hipSuccess) return status_of(e);
+  s.stats.traversal_ms = s.timer_stop();
+  const uint32_t nf = s.hscalar_u[0];
+  s.stats.flagged = nf;
+  if (nf == 0) return OK;
+  // Jiggle replay (many_body.zig:77-83): ascending node order, real draws.
+  std::vector<uint32_t> list(nf), counts(nf), offs(nf);
+  if ((e = hipMemcpy(list.data(), s.flagged.p, nf * sizeof(uint32_t), hipMemcpyDeviceToHost)) != hipSuccess)
+    return status_of(e);
+  std::sort(list.begin(), list.end());
+  if ((e = hipMemcpy(s.order.p, list.data(), nf * sizeof(uint32_t), hipMemcpyHostToDevice)) != hipSuccess)
+    return status_of(e);
+  a.order = s.order.p;
+  a.count = nf;
+  a.draws_count = s.draws_count.p;
+  if ((e = mb_traverse(a, TRAV_COUNT, s.stream)) != hipSuccess) return status_of(e);
+  if ((e = hipMemcpyAsync(counts.data(), s.draws_count.p, nf * sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream)) !=
+      hipSuccess)
+    return status_of(e);
+  if ((e = hipStreamSynchronize(s.stream)) != hipSuccess) return status_of(e);
+  uint32_t total = 0;
+  for (uint32_t k = 0; k < nf; ++k) {
+    offs[k] = total;
+    total += counts[k];
+  }
+  std::vector<double> values(total > 0 ? total : 1);
+  for (uint32_t k = 0; k < total; ++k) values[k] = draw();
+  if ((e = s.draws.reserve(values.size(), 0, s.stream)) != hipSuccess) return status_of(e);
+  if ((e = hipMemcpy(s.draws.p, values.data(), values.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+    return status_of(e);
+  if ((e = hipMemcpy(s.draw_off.p, offs.data(), nf * sizeof(uint32_t), hipMemcpyHostToDevice)) != hipSuccess)
+    return status_of(e);
+  a.draw_off = s.draw_off.p;
+  a.draws = s.draws.p;
+  if ((e = mb_traverse(a, TRAV_REPLAY, s.stream)) != hipSuccess) return status_of(e);
+  return status_of(hipStreamSynchronize(s.stream));
+}
+
+// ---------------------------------------------------------------- link (exact)
+// The dependency-level schedule (see link.hip), built on the host when the
+// links change: a serial O(m) scan, as the reference's CSR build is.
+hipError_t build_schedule(Sim& s, Force& f) {
+  const uint32_t m = f.count, n = s.n;
+  std::vector<uint32_t> last(n, 0), level(m);
+  uint32_t levels = 0;
+  for (uint32_t e = 0; e < m; ++e) {
+    const uint32_t a = f.source.p[e], b = f.target.p[e];
+    const uint32_t l = std::max(last[a], last[b]) + 1;
+    last[a] = last[b] = l;
+    level[e] = l;
+    levels = std::max(levels, l);
+  }
+  f.level_off.assign(levels + 2, 0);
+  for (uint32_t e = 0; e < m; ++e) f.level_off[level[e] + 1]++;
+  for (uint32_t l = 1; l < levels + 2; ++l) f.level_off[l] += f.level_off[l - 1];
+  std::vector<uint32_t> pos(f.level_off.begin(), f.level_off.end());
+  std::vector<uint32_t> ls(m), lt(m);
+  std::vector<double> l0(m), l1(m), l2(m);
+  for (uint32_t e = 0; e < m; ++e) {
+    const uint32_t p = pos[level[e]]++;
+    ls[p] = f.source.p[e];
+    lt[p] = f.target.p[e];
+    l0[p] = f.f[0].p[e];
+    l1[p] = f.f[1].p[e];
+    l2[p] = f.f[2].p[e];
+  }
+  const size_t c = m > 0 ? m : 1;
+  HIPCHECK(f.lsrc.reserve(c, 0, s.stream));
+  HIPCHECK(f.ltgt.reserve(c, 0, s.stream));
+  for (auto& a : f.lf) HIPCHECK(a.reserve(c, 0, s.stream));
+  if (m) {
+    HIPCHECK(hipMemcpy(f.lsrc.p, ls.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(f.ltgt.p, lt.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(f.lf[0].p, l0.data(), m * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(f.lf[1].p, l1.data(), m * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(f.lf[2].p, l2.data(), m * sizeof(double), hipMemcpyHostToDevice));
+  }
+  f.sched_valid = true;
+  f.sched_n = n;
+  return hipSuccess;
+}
+
+int32_t run_link_exact(Sim& s, Force& f, double alpha) {
+  const uint32_t m = f.count;
+  if (!f.sched_valid || f.sched_n != s.n) {
+    for (uint32_t e = 0; e < m; ++e)  // link.zig:10-12
+      if (f.source.p[e] >= s.n || f.target.p[e] >= s.n) return BAD_ARGUMENT;
+    hipError_t e = build_schedule(s, f);
+    if (e != hipSuccess) return status_of(e);
+  }
+  s.stats.link_levels = f.level_off.size() >= 2 ? f.level_off.size() - 2 : 0;
+  const double iterations = f.scalar(S_ITERATIONS);
+  if (m == 0 || !(0 < iterations)) return OK;
+  hipError_t e;
+  s.timer_start();
+  const size_t bytes = (size_t)s.n * sizeof(double);
+  if ((e = hipMemcpyAsync(s.dsvx.p, s.dvx.p, bytes, hipMemcpyDeviceToDevice, s.stream)) != hipSuccess) return status_of(e);
+  if ((e = hipMemcpyAsync(s.dsvy.p, s.dvy.p, bytes, hipMemcpyDeviceToDevice, s.stream)) != hipSuccess) return status_of(e);
+  if ((e = hipMemsetAsync(s.dscalar_u + 1, 0, sizeof(uint32_t), s.stream)) != hipSuccess) return status_of(e);
+  LinkLevelArgs a{f.lsrc.p, f.ltgt.p, f.lf[0].p, f.lf[1].p, f.lf[2].p, s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, alpha,
+                  s.dscalar_u + 1};
+  const uint32_t levels = (uint32_t)f.level_off.size() - 2;
+  for (double k = 0; k < iterations; k += 1) {
+    for (uint32_t l = 1; l <= levels; ++l) {
+      if ((e = link_level(a, f.level_off[l], f.level_off[l + 1], s.stream)) != hipSuccess) return status_of(e);
+    }
+  }
+  if ((e = hipMemcpyAsync(s.hscalar_u + 1, s.dscalar_u + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream)) !=
+      hipSuccess)
+    return status_of(e);
+  if ((e = hipStreamSynchronize(s.stream)) != hipSuccess) return status_of(e);
+  s.stats.link_ms = s.timer_stop();
+  if (s.hscalar_u[1] == 0) return OK;
+  // A zero or NaN delta: d3 draws in edge order. Restore and run serially.
+  if ((e = hipMemcpyAsync(s.dvx.p, s.dsvx.p, bytes, hipMemcpyDeviceToDevice, s.stream)) != hipSuccess) return status_of(e);
+  if ((e = hipMemcpyAsync(s.dvy.p, s.dsvy.p, bytes, hipMemcpyDeviceToDevice, s.stream)) != hipSuccess) return status_of(e);
+  for (double k = 0; k < iterations; k += 1) {
+    LinkSerialState* st = s.hlink;
+    st->edge = 0;
+    st->need = 0;
+    st->has_jig = 0;
+    for (;;) {
+      if ((e = hipMemcpyAsync(s.dlink, st, sizeof(LinkSerialState), hipMemcpyHostToDevice, s.stream)) != hipSuccess)
+        return status_of(e);
+      if ((e = link_serial(f.dsrc.p, f.dtgt.p, f.df[0].p, f.df[1].p, f.df[2].p, m, s.dx.p, s.dy.p, s.dvx.p, s.dvy.p,
+                           alpha, s.dlink, s.stream)) != hipSuccess)
+        return status_of(e);
+      if ((e = hipMemcpyAsync(st, s.dlink, sizeof(LinkSerialState), hipMemcpyDeviceToHost, s.stream)) != hipSuccess)
+        return status_of(e);
+      if ((e = hipStreamSynchronize(s.stream)) != hipSuccess) return status_of(e);
+      if (st->edge >= m) break;
+      if (st->need & 1) st->jig[0] = jiggle_host(draw());
+      if (st->need & 2) st->jig[1] = jiggle_host(draw());
+      st->has_jig = 1;
+    }
+  }
+  return OK;
+}
+
+int32_t run_center(Sim& s, Force& f) {
+  s.timer_start();
+  hipError_t e = center_sum(s.dx.p, s.dy.p, s.n, f.scalar(S_X), f.scalar(S_Y), f.scalar(S_STRENGTH), s.dscalar_d,
+                            s.stream);
+  if (e == hipSuccess) e = center_apply(s.dx.p, s.dy.p, s.n, s.dscalar_d, s.stream);
+  s.stats.center_ms = s.timer_stop();
+  return status_of(e);
+}
+
+hipError_t refresh_tree_host(Sim& s) {
+  TreeBuffers& t = s.tree;
+  if (!t.meta) {
+    for (int k = 0; k < 4; ++k) s.hextent[k] = __builtin_nan("");
+    return hipSuccess;
+  }
+  HIPCHECK(hipMemcpy(t.meta_host, t.meta, sizeof(TreeMeta), hipMemcpyDeviceToHost));
+  if (!t.meta_host->valid) t.meta_host->len = 0;
+  for (int k = 0; k < 4; ++k) s.hextent[k] = t.meta_host->extent[k];
+  const uint32_t len = t.meta_host->len, m = t.meta_host->m;
+  HIPCHECK(s.hcells.reserve(len > 0 ? len : 1, 0));
+  HIPCHECK(s.hperm.reserve(m > 0 ? m : 1, 0));
+  if (len) HIPCHECK(hipMemcpy(s.hcells.p, t.cells, (size_t)len * sizeof(Cell), hipMemcpyDeviceToHost));
+  if (m) HIPCHECK(hipMemcpy(s.hperm.p, t.perm, (size_t)m * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return hipSuccess;
+}
+
+}  // namespace
+}  // namespace d3f
+
+using namespace d3f;
+
+// ---------------------------------------------------------------- exported ABI
+
+EXPORT void core_set_random(d3f_random_fn fn, void* ctx) {
+  g_random = fn;
+  g_random_ctx = ctx;
+}
+
+EXPORT void core_set_timing(int on) { g_timing = on != 0; }
+
+EXPORT uint32_t sim_create(void) {  // root.zig:65-78
+  int devices = 0;
+  if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) return 0;
+  Sim* s = new (std::nothrow) Sim();
+  if (!s) return 0;
+  if (s->init() != hipSuccess) {
+    delete s;
+    return 0;
+  }
+  for (size_t i = 0; i < g_sims.size(); ++i)
+    if (!g_sims[i]) {
+      g_sims[i] = s;
+      return (uint32_t)i + 1;
+    }
+  g_sims.push_back(s);
+  return (uint32_t)g_sims.size();
+}
+
+EXPORT int32_t sim_destroy(uint32_t h) {  // root.zig:80-86
+  Sim* s = get(h);
+  if (!s) return BAD_HANDLE;
+  (void)hipStreamSynchronize(s->stream);
+  delete s;
+  g_sims[h - 1] = nullptr;
+  return OK;
+}
+
+EXPORT int32_t sim_set_node_count(uint32_t h, uint32_t n) {  // root.zig:88-92
+  Sim* s = get(h);
+  if (!s) return BAD_HANDLE;
+  bool grown = false;
+  hipError_t e = s->set_node_count(n, &grown);
+  if (e != hipSuccess) return OUT_OF_MEMORY;
+  return grown ? GROWN : OK;
+}
+
+EXPORT void* sim_ptr(uint32_t h, uint32_t which) {  // root.zig:94-98, sim.zig:75-88
+  Sim* s = get(h);
+  if (!s) return nullptr;
+  switch (which) {
+    case D3F_PX: return s->cap ? s->x.p : nullptr;
+    case D3F_PY: return s->cap ? s->y.p : nullptr;
+    case D3F_PVX: return s->cap ? s->vx.p : nullptr;
+    case D3F_PVY: return s->cap ? s->vy.p : nullptr;
+    case D3F_PFX: return s->cap ? s->fx.p : nullptr;
+    case D3F_PFY: return s->cap ? s->fy.p : nullptr;
+    case D3F_PFIXED: return s->cap ? s->fixed.p : nullptr;
+    case D3F_PPERM:
+      if (refresh_tree_host(*s) != hipSuccess) return nullptr;
+      return s->tree.meta_host && s->tree.meta_host->m ? s->hperm.p : nullptr;
+    case D3F_PSNAP: return s->snap.cap ? s->snap.p : nullptr;
+    default: return nullptr;
+  }
+}
+
+EXPORT int32_t sim_sync(uint32_t h, uint32_t direction) {
+  Sim* s = get(h);
+  if (!s) return BAD_HANDLE;
+  const uint32_t n = s->n;
+  if (n == 0) return OK;
+  hipError_t e = hipSuccess;
+  const size_t b = (size_t)n * sizeof(double);
+  if (direction == D3F_SYNC_H2D) {
+    struct { double* d; double* h; } pairs[] = {{s->dx.p, s->x.p}, {s->dy.p, s->y.p}, {s->dvx.p, s->vx.p},
+                                                {s->dvy.p, s->vy.p}, {s->dfx.p, s->fx.p}, {s->dfy.p, s->fy.p}};
+    for (auto& p : pairs)
+      if (e == hipSuccess) e = hipMemcpyAsync(p.d, p.h, b, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->dfixed.p, s->fixed.p, n, hipMemcpyHostToDevice, s->stream);
+  } else if (direction == D3F_SYNC_D2H) {
+    struct { double* h; double* d; } pairs[] = {{s->x.p, s->dx.p}, {s->y.p, s->dy.p}, {s->vx.p, s->dvx.p},
+                                                {s->vy.p, s->dvy.p}};
+    for (auto& p : pairs)
+      if (e == hipSuccess) e = hipMemcpyAsync(p.h, p.d, b, hipMemcpyDeviceToHost, s->stream);
+  } else {
+    return BAD_ARGUMENT;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  return status_of(e);
+}
+
+EXPORT int32_t sim_snapshot(uint32_t h, uint32_t positions) {  // root.zig:205-209, sim.zig:91-100
+  Sim* s = get(h);
+  if (!s) return BAD_HANDLE;
+  const uint32_t n = s->n;
+  if (s->snap.reserve(4 * (size_t)(s->cap > 0 ? s->cap : 1), 0) != hipSuccess) return OUT_OF_MEMORY;
+  const size_t b = (size_t)n * sizeof(double);
+  hipError_t e = hipSuccess;
+  if (n) {
+    e = hipMemcpyAsync(s->snap.p, s->dvx.p, b, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->snap.p + n, s->dvy.p, b, hipMemcpyDeviceToHost, s->stream);
+    if (positions) {
+      if (e == hipSuccess) e = hipMemcpyAsync(s->snap.p + 2 * n, s->dx.p, b, hipMemcpyDeviceToHost, s->stream);
+      if (e == hipSuccess) e = hipMemcpyAsync(s->snap.p + 3 * n, s->dy.p, b, hipMemcpyDeviceToHost, s->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  }
+  return status_of(e);
+}
+
+EXPORT int32_t sim_integrate(uint32_t h, double velocity_decay) {  // root.zig:250-254
+  Sim* s = get(h);
+  if (!s) return BAD_HANDLE;
+  s->timer_start();
+  hipError_t e = integrate(s->dx.p, s->dy.p, s->dvx.p, s->dvy.p, s->dfx.p, s->dfy.p, s->dfixed.p, s->n,
+                           velocity_decay, s->stream);
+  s->stats.integrate_ms = s->timer_stop();
+  return status_of(e);
+}
+
+EXPORT int32_t tree_build(uint32_t h, uint32_t shifted) {  // root.zig:102-108
+  Sim* s = get(h);
+  if (!s) return BAD_HANDLE;
+  const void* c0 = s->hcells.p;
+  const void* p0 = s->hperm.p;
+  hipError_t e;
+  if (shifted) {
+    e = shift_positions(s->dx.p, s->dy.p, s->dvx.p, s->dvy.p, s->dpx.p, s->dpy.p, s->n, s->stream);
+    if (e == hipSuccess) e = tree_build_exact(s->tree, s->dpx.p, s->dpy.p, s->n, s->stream);
+  } else {
+    e = tree_build_exact(s->tree, s->dx.p, s->dy.p, s->n, s->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = refresh_tree_host(*s);
+  if (e != hipSuccess) return status_of(e);
+  return (s->hcells.p != c0 || s->hperm.p != p0) ? GROWN : OK;
+}
+
+EXPORT uint64_t tree_info(uint32_t h, uint32_t which) {  // root.zig:113-126
+  Sim* s = get(h);
+  if (!s) return 0;
+  if (refresh_tree_host(*s) != hipSuccess) return 0;
+  const TreeMeta* m = s->tree.meta_host;
+  const bool valid = m && m->valid;
+  switch (which) {
+    case 0: return valid ? m->len : 0;
+    case 1: return valid ? 0 : NONE;
+    case 2: return (uint64_t)(uintptr_t)s->hcells.p;
+    case 3: return (uint64_t)(uintptr_t)s->hextent;
+    case 4: return (uint64_t)(uintptr_t)s->hperm.p;
+    case 5: return sizeof(Cell);
+    case 6: return valid ? m->m : 0;
+    default: return 0;
+  }
+}
+
+EXPORT int32_t tree_order(uint32_t h, uint32_t after, uint32_t* out) {  // root.zig:130-137, tree.zig:652-684
+  Sim* s = get(h);
+  if (!s) return BAD_HANDLE;
+  if (!out) return BAD_ARGUMENT;
+  if (refresh_tree_host(*s) != hipSuccess) return WORKER_FAILED;
+  const TreeMeta* m = s->tree.meta_host;
+  if (!m || !m->valid) return OK;
+  std::vector<uint32_t> stack;
+  stack.push_back(0);
+  size_t w = 0;
+  while (!stack.empty()) {
+    const uint32_t c = stack.back();
+    stack.pop_back();
+    out[w++] = c;
+    const uint32_t* ch = s->hcells.p[c].child;
+    if (after) {
+      for (int i = 0; i < 4; ++i) if (ch[i] != NONE) stack.push_back(ch[i]);
+    } else {
+      for (int i = 3; i >= 0; --i) if (ch[i] != NONE) stack.push_back(ch[i]);
+    }
+  }
+  if (after) std::reverse(out, out + w);
+  return OK;
+}
+
+EXPORT uint32_t force_create(uint32_t h, uint32_t kind) {  // root.zig:147-161
+  Sim* s = get(h);
+  if (!s || kind >= K_COUNT) return 0;
+  Force* f = new (std::nothrow) Force((Kind)kind);
+  if (!f) return 0;
+  if (f->node_sized() && f->set_count(s->n) != hipSuccess) {
+    delete f;
+    return 0;
+  }
+  for (size_t i = 0; i < s->forces.size(); ++i)
+    if (!s->forces[i]) {
+      s->forces[i] = f;
+      return (uint32_t)i + 1;
+    }
+  s->forces.push_back(f);
+  return (uint32_t)s->forces.size();
+}
+
+EXPORT int32_t force_destroy(uint32_t h, uint32_t fh) {  // root.zig:163-170
+  Sim* s = get(h);
+  Force* f = get_force(s, fh);
+  if (!f) return BAD_HANDLE;
+  (void)hipStreamSynchronize(s->stream);
+  delete f;
+  s->forces[fh - 1] = nullptr;
+  return OK;
+}
+
+EXPORT int32_t force_set_count(uint32_t h, uint32_t fh, uint32_t count) {  // root.zig:174-179
+  Sim* s = get(h);
+  Force* f = get_force(s, fh);
+  if (!f) return BAD_HANDLE;
+  if (f->set_count(count) != hipSuccess) return OUT_OF_MEMORY;
+  return GROWN;
+}
+
+EXPORT void* force_ptr(uint32_t h, uint32_t fh, uint32_t which) {  // root.zig:181-186
+  Sim* s = get(h);
+  Force* f = get_force(s, fh);
+  if (!f || f->count == 0) return nullptr;
+  f->dirty = true;
+  f->sched_valid = false;
+  switch (which) {
+    case D3F_F0: return f->float_arrays() > 0 ? f->f[0].p : nullptr;
+    case D3F_F1: return f->float_arrays() > 1 ? f->f[1].p : nullptr;
+    case D3F_F2: return f->float_arrays() > 2 ? f->f[2].p : nullptr;
+    case D3F_SOURCE: return f->kind == K_LINK ? f->source.p : nullptr;
+    case D3F_TARGET: return f->kind == K_LINK ? f->target.p : nullptr;
+    default: return nullptr;
+  }
+}
+
+EXPORT int32_t force_set_scalar(uint32_t h, uint32_t fh, uint32_t which, double v) {  // root.zig:188-194
+  Sim* s = get(h);
+  Force* f = get_force(s, fh);
+  if (!f) return BAD_HANDLE;
+  if (which >= S_COUNT) return BAD_ARGUMENT;
+  f->scalars[which] = v;
+  return OK;
+}
+
+EXPORT double force_get_scalar(uint32_t h, uint32_t fh, uint32_t which) {  // root.zig:196-200
+  Sim* s = get(h);
+  Force* f = get_force(s, fh);
+  if (!f) return 0;
+  return which < S_COUNT ? f->scalars[which] : 0;
+}
+
+EXPORT int32_t force_run(uint32_t h, uint32_t fh, double alpha) {  // root.zig:211-242
+  Sim* s = get(h);
+  Force* f = get_force(s, fh);
+  if (!f) return BAD_HANDLE;
+  if (f->node_sized() && f->count < s->n) return BAD_ARGUMENT;
+  hipError_t e = upload(*s, *f);
+  if (e != hipSuccess) return status_of(e);
+  switch (f->kind) {
+    case K_CENTER: return run_center(*s, *f);
+    case K_X:
+      return status_of(axis_force(s->dx.p, s->dvx.p, f->df[0].p, f->df[1].p, s->n, alpha, s->stream));
+    case K_Y:
+      return status_of(axis_force(s->dy.p, s->dvy.p, f->df[0].p, f->df[1].p, s->n, alpha, s->stream));
+    case K_RADIAL:
+      return status_of(radial_force(s->dx.p, s->dy.p, s->dvx.p, s->dvy.p, f->df[0].p, f->df[1].p, s->n,
+                                    f->scalar(S_X), f->scalar(S_Y), alpha, s->stream));
+    case K_LINK:
+      if (fast_mode(*s, *f)) return BAD_ARGUMENT;  // fast link: not yet on this build
+      return run_link_exact(*s, *f, alpha);
+    case K_MANY_BODY:
+      if (fast_mode(*s, *f)) return BAD_ARGUMENT;  // fast many-body: not yet on this build
+      return run_many_body_exact(*s, *f, alpha);
+    case K_COLLIDE:
+      return BAD_ARGUMENT;  // collide: not on this build yet
+    default:
+      return BAD_ARGUMENT;
+  }
+}
+
+EXPORT int32_t sim_stats(uint32_t h, d3f_stats* out) {
+  Sim* s = get(h);
+  if (!s || !out) return BAD_HANDLE;
+  *out = s->stats;
+  return OK;
+}

@@ -1,0 +1,159 @@
+// center, x, y, radial, integrate and the shifted positions on gfx950
+// (reference: src/core/simple.zig, src/core/sim.zig:129-146). All but center
+// are per node. forceCenter's mean is d3's serial sum over nodes 0..n-1
+// (simple.zig:18-22), which no reassociation may change: one workgroup stages
+// x and y through LDS and one lane per axis adds them in index order.
+#include "common.h"
+
+namespace d3f {
+namespace {
+
+constexpr int EW = 256;
+constexpr int CENTER_THREADS = 256;
+constexpr int CENTER_CHUNK = 2048;
+
+__global__ __launch_bounds__(CENTER_THREADS) void k_center_sum(const double* __restrict__ xs,
+                                                               const double* __restrict__ ys, uint32_t n, double cx,
+                                                               double cy, double strength,
+                                                               double* __restrict__ out) {
+  __shared__ double bx[CENTER_CHUNK], by[CENTER_CHUNK];
+  __shared__ double sy_shared;
+  double sx = 0, sy = 0;
+  for (uint32_t base = 0; base < n; base += CENTER_CHUNK) {
+    const uint32_t cnt = n - base < (uint32_t)CENTER_CHUNK ? n - base : (uint32_t)CENTER_CHUNK;
+    for (uint32_t k = threadIdx.x; k < cnt; k += CENTER_THREADS) {
+      bx[k] = xs[base + k];
+      by[k] = ys[base + k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (uint32_t k = 0; k < cnt; ++k) sx += bx[k];
+    } else if (threadIdx.x == 64) {
+      for (uint32_t k = 0; k < cnt; ++k) sy += by[k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 64) sy_shared = sy;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double nf = (double)n;
+    out[0] = (sx / nf - cx) * strength;
+    out[1] = (sy_shared / nf - cy) * strength;
+  }
+}
+
+__global__ __launch_bounds__(EW) void k_center_apply(double* __restrict__ xs, double* __restrict__ ys, uint32_t n,
+                                                     const double* __restrict__ shift) {
+  const uint32_t i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n) return;
+  const double sx = shift[0], sy = shift[1];
+  xs[i] -= sx;
+  ys[i] -= sy;
+}
+
+// simple.zig:90-123: `x += vx *= decay`, or `x = fx, vx = 0` on a fixed axis.
+__global__ __launch_bounds__(EW) void k_integrate(double* __restrict__ xs, double* __restrict__ ys,
+                                                  double* __restrict__ vxs, double* __restrict__ vys,
+                                                  const double* __restrict__ fxs, const double* __restrict__ fys,
+                                                  const uint8_t* __restrict__ fixed, uint32_t n, double decay) {
+  const uint32_t i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n) return;
+  const uint8_t f = fixed[i];
+  if ((f & 1) == 0) {
+    const double v = vxs[i] * decay;
+    vxs[i] = v;
+    xs[i] = xs[i] + v;
+  } else {
+    xs[i] = fxs[i];
+    vxs[i] = 0;
+  }
+  if ((f & 2) == 0) {
+    const double v = vys[i] * decay;
+    vys[i] = v;
+    ys[i] = ys[i] + v;
+  } else {
+    ys[i] = fys[i];
+    vys[i] = 0;
+  }
+}
+
+// simple.zig:51-62: v += (target - p) * strength * alpha.
+__global__ __launch_bounds__(EW) void k_axis(const double* __restrict__ pos, double* __restrict__ v,
+                                             const double* __restrict__ strength,
+                                             const double* __restrict__ target, uint32_t n, double alpha) {
+  const uint32_t i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n) return;
+  v[i] += (target[i] - pos[i]) * strength[i] * alpha;
+}
+
+// simple.zig:64-86 (`a || 1e-6`, never jiggle).
+__global__ __launch_bounds__(EW) void k_radial(const double* __restrict__ xs, const double* __restrict__ ys,
+                                               double* __restrict__ vxs, double* __restrict__ vys,
+                                               const double* __restrict__ strength,
+                                               const double* __restrict__ radius, uint32_t n, double cx, double cy,
+                                               double alpha) {
+  const uint32_t i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n) return;
+  double dx = xs[i] - cx;
+  if (dx == 0 || dx != dx) dx = 1e-6;
+  double dy = ys[i] - cy;
+  if (dy == 0 || dy != dy) dy = 1e-6;
+  const double l = __builtin_sqrt(dx * dx + dy * dy);
+  const double k = (radius[i] - l) * strength[i] * alpha / l;
+  vxs[i] += dx * k;
+  vys[i] += dy * k;
+}
+
+__global__ __launch_bounds__(EW) void k_shift(const double* __restrict__ xs, const double* __restrict__ ys,
+                                              const double* __restrict__ vxs, const double* __restrict__ vys,
+                                              double* __restrict__ px, double* __restrict__ py, uint32_t n) {
+  const uint32_t i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n) return;
+  px[i] = xs[i] + vxs[i];
+  py[i] = ys[i] + vys[i];
+}
+
+}  // namespace
+
+hipError_t center_sum(const double* x, const double* y, uint32_t n, double cx, double cy, double strength,
+                      double* shift_out, hipStream_t stream) {
+  hipLaunchKernelGGL(k_center_sum, dim3(1), dim3(CENTER_THREADS), 0, stream, x, y, n, cx, cy, strength, shift_out);
+  return hipGetLastError();
+}
+
+hipError_t center_apply(double* x, double* y, uint32_t n, const double* shift, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_center_apply, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, x, y, n, shift);
+  return hipGetLastError();
+}
+
+hipError_t integrate(double* x, double* y, double* vx, double* vy, const double* fx, const double* fy,
+                     const uint8_t* fixed, uint32_t n, double decay, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_integrate, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, x, y, vx, vy, fx, fy, fixed, n, decay);
+  return hipGetLastError();
+}
+
+hipError_t axis_force(const double* pos, double* v, const double* strength, const double* target, uint32_t n,
+                      double alpha, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_axis, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, pos, v, strength, target, n, alpha);
+  return hipGetLastError();
+}
+
+hipError_t radial_force(const double* x, const double* y, double* vx, double* vy, const double* strength,
+                        const double* radius, uint32_t n, double cx, double cy, double alpha, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_radial, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, x, y, vx, vy, strength, radius, n, cx,
+                     cy, alpha);
+  return hipGetLastError();
+}
+
+hipError_t shift_positions(const double* x, const double* y, const double* vx, const double* vy, double* px,
+                           double* py, uint32_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_shift, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, x, y, vx, vy, px, py, n);
+  return hipGetLastError();
+}
+
+}  // namespace d3f

@@ -1,0 +1,115 @@
+/* d3fgpu: the d3-force tick engine on AMD MI355X (gfx950), as a C ABI.
+ *
+ * This is the drop-in boundary for the reference's core: every entry point
+ * below replaces one export of the reference's Zig/wasm core
+ * (src/core/root.zig), with the same handles, enums, status codes and
+ * semantics. Differences forced by the boundary are marked "GPU:".
+ *
+ *  - Handles are 1-based u32; 0 is never valid (root.zig:29-35).
+ *  - Status codes (root.zig:16-25): 0 OK, 1 GROWN, -1 BAD_HANDLE,
+ *    -2 OUT_OF_MEMORY, -3 BAD_ARGUMENT, -4 WORKER_FAILED (a GPU error).
+ *  - Arrays live in pinned host memory owned by the library; sim_ptr and
+ *    force_ptr return their addresses (root.zig returned byte offsets into
+ *    wasm linear memory). An address stays valid until the next call that
+ *    returns GROWN for that simulation or force.
+ *  - GPU: node arrays are mirrored in HBM. sim_sync(h, D3F_SYNC_H2D) makes
+ *    the device copy current after the host wrote x, y, vx, vy, fx, fy,
+ *    fixed; sim_sync(h, D3F_SYNC_D2H) brings x, y, vx, vy back and waits.
+ *    Force parameter arrays are uploaded on the next force_run after any
+ *    force_ptr or force_set_count call (the facade writes them only through
+ *    pointers obtained right after those calls).
+ *  - Every call is synchronous with respect to the random callback: jiggle
+ *    draws happen inside force_run, in d3's order.
+ */
+#ifndef D3FGPU_H
+#define D3FGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Force kinds, src/core/force.zig:8. */
+enum { D3F_CENTER = 0, D3F_COLLIDE = 1, D3F_LINK = 2, D3F_MANY_BODY = 3, D3F_RADIAL = 4, D3F_X = 5, D3F_Y = 6 };
+/* Scalars, force.zig:13. */
+enum {
+  D3F_STRENGTH = 0, D3F_CX = 1, D3F_CY = 2, D3F_THETA2 = 3, D3F_DISTANCE_MIN2 = 4, D3F_DISTANCE_MAX2 = 5,
+  D3F_ITERATIONS = 6, D3F_PRECISION = 7, D3F_ORDER = 8, D3F_OPENING = 9, D3F_SEED = 10
+};
+/* Force arrays, force.zig:21 (f0 strength/radius, f1 target/distance, f2 bias,
+ * source, target). */
+enum { D3F_F0 = 0, D3F_F1 = 1, D3F_F2 = 2, D3F_SOURCE = 3, D3F_TARGET = 4 };
+/* sim_ptr slots, src/core/sim.zig:14 (x, y, vx, vy, fx, fy, fixed, perm,
+ * rng_buf, snap). */
+enum { D3F_PX = 0, D3F_PY = 1, D3F_PVX = 2, D3F_PVY = 3, D3F_PFX = 4, D3F_PFY = 5, D3F_PFIXED = 6,
+       D3F_PPERM = 7, D3F_PRNG = 8, D3F_PSNAP = 9 };
+enum { D3F_SYNC_H2D = 0, D3F_SYNC_D2H = 1 };
+
+/* The running force's random source: replaces the wasm import env.random
+ * (root.zig:139-145, core.js:11-14). Called on the calling thread, inside
+ * force_run, once per jiggle draw, in d3's order. */
+typedef double (*d3f_random_fn)(void* ctx);
+void core_set_random(d3f_random_fn fn, void* ctx);
+
+/* root.zig:65-78. Returns a handle, or 0 when out of memory or no GPU. */
+uint32_t sim_create(void);
+/* root.zig:80-86. */
+int32_t sim_destroy(uint32_t h);
+/* root.zig:88-92: 0, or GROWN when arrays moved (refresh pointers). */
+int32_t sim_set_node_count(uint32_t h, uint32_t n);
+/* root.zig:94-98: address of array `which` (D3F_P*), NULL when empty. */
+void* sim_ptr(uint32_t h, uint32_t which);
+/* GPU: host<->HBM synchronisation of the node arrays (no reference
+ * counterpart; core.js copyIn / copyOut at core.js:121-140 call it). */
+int32_t sim_sync(uint32_t h, uint32_t direction);
+/* root.zig:205-209: copies vx, vy (and x, y when positions != 0) into the
+ * snapshot array, sim_ptr(h, D3F_PSNAP). */
+int32_t sim_snapshot(uint32_t h, uint32_t positions);
+/* root.zig:250-254: the end of d3's tick (velocity decay, fixed nodes). */
+int32_t sim_integrate(uint32_t h, double velocity_decay);
+
+/* root.zig:102-108: builds the exact quadtree on x, y (shifted = 0) or on
+ * x + vx, y + vy (1). Returns GROWN when tree arrays moved. */
+int32_t tree_build(uint32_t h, uint32_t shifted);
+/* root.zig:113-126: 0 cell count, 1 root, 2 cells address (88-byte cells,
+ * tree.zig:46-62), 3 extent address (4 f64), 4 perm address, 5 sizeof(Cell),
+ * 6 points in tree. Addresses are host copies made by the call. */
+uint64_t tree_info(uint32_t h, uint32_t which);
+/* root.zig:130-137: cell indices in d3-quadtree visit (after = 0) or
+ * visitAfter (after = 1) order into out[cell count]. */
+int32_t tree_order(uint32_t h, uint32_t after, uint32_t* out);
+
+/* root.zig:147-161: a force of `kind` (D3F_CENTER ..), or 0. */
+uint32_t force_create(uint32_t h, uint32_t kind);
+/* root.zig:163-170. */
+int32_t force_destroy(uint32_t h, uint32_t fh);
+/* root.zig:174-179: resizes the force's arrays (new float slots NaN);
+ * always GROWN. */
+int32_t force_set_count(uint32_t h, uint32_t fh, uint32_t count);
+/* root.zig:181-186: address of array `which` (D3F_F0 ..), NULL when empty. */
+void* force_ptr(uint32_t h, uint32_t fh, uint32_t which);
+/* root.zig:188-194 / 196-200. */
+int32_t force_set_scalar(uint32_t h, uint32_t fh, uint32_t which, double v);
+double force_get_scalar(uint32_t h, uint32_t fh, uint32_t which);
+/* root.zig:211-242: runs the force on the device arrays. Fast mode applies
+ * when D3F_PRECISION is 1 and the node count is at least 2000
+ * (root.zig:246-248). */
+int32_t force_run(uint32_t h, uint32_t fh, double alpha);
+
+/* Library introspection (no reference counterpart): per-kernel timing of
+ * the last force_run of each kind, for the benchmark's roofline. */
+typedef struct {
+  double tree_ms, accumulate_ms, traversal_ms, link_ms, center_ms, integrate_ms;
+  uint64_t flagged;   /* many-body nodes replayed for jiggle in the last run */
+  uint64_t link_levels;
+} d3f_stats;
+int32_t sim_stats(uint32_t h, d3f_stats* out);
+/* Enables HIP-event timing of phases (off by default: events cost syncs). */
+void core_set_timing(int on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
