@@ -1,0 +1,198 @@
+"""Benchmark: d3-force ticks/s at 1,000,000 nodes in exact mode on one MI355X.
+
+Workload (the exact 1M-node configuration): 1,000,000 nodes from d3's
+phyllotaxis start, 1% of them pinned with fx, fy uniform in [-500, 500];
+2,000,000 links over distinct node pairs drawn with a seeded d3 LCG;
+forceManyBody (strength -30, theta 0.9, distanceMin 1, distanceMax inf),
+forceLink (distance 30, strength 1/min(degree), 1 iteration), forceCenter(0, 0);
+FP64, bit-identical to the reference core. A step is one tick. The timed
+region runs K ticks on arrays already resident in HBM (the facade's copy-in
+happens before it, the copy-out after it).
+
+Output: one JSON line with the metric, the roofline of the dominant kernel
+(measured live with HIP events on the library's stream) and a CPU baseline:
+the oracle (a C++ restatement of the reference core) timed on this host's
+cores over a bounded sample of the same workload.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "d3-force-wasm_amd"), os.path.join(ROOT, "tests")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "ticks/s at 1M nodes: exact BH (bit-exact) vs fast FMM (RMS err ≤1e-6)"
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(core, n, m, pin_fraction, seed=1):
+    import d3force_amd as d3
+    from scenarios import pinned, random_graph
+    t0 = time.time()
+    src, tgt = random_graph(n, m, seed=seed)
+    nodes = d3.Nodes(n)
+    if pin_fraction:
+        idx, fx, fy = pinned(n, pin_fraction)
+        nodes.fix(fx=fx, fy=fy, where=idx)
+    sim = (d3.forceSimulation(nodes, core=core)
+           .force("charge", d3.forceManyBody())
+           .force("link", d3.forceLink((src, tgt)))
+           .force("center", d3.forceCenter()))
+    log("workload %d nodes, %d links ready in %.1fs" % (n, m, time.time() - t0))
+    return sim, nodes
+
+
+def device_sync(core):
+    """Waits for the device (the library's own hipDeviceSynchronize: the
+    library links ROCm's HIP runtime directly, so torch is not loaded)."""
+    from d3force_amd.core import check
+    check(core.core_synchronize())
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", "traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = [p.strip() for p in line.strip().strip("|").split("|")]
+        if len(parts) >= 2 and parts[0] == kernel:
+            try:
+                return float(parts[-1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(n, m, pin_fraction, ticks, threads):
+    from oracle import oracle
+    core = oracle.load()
+    core.set_threads(threads)
+    sim, _ = make_workload(core, n, m, pin_fraction)
+    ctx = sim.begin()
+    t0 = time.time()
+    sim.iterate(ticks, ctx)
+    dt = time.time() - t0
+    sim.end(ctx)
+    return {"value": ticks / dt, "unit": "ticks/s", "cores": threads, "kind": "port",
+            "sample": "ticks 1-%d of the same 1M-node workload from its start, C++ oracle "
+                      "(restatement of the reference core), Barnes-Hut traversal on %d threads, "
+                      "tree build, link, center and integrate serial as in the reference" % (ticks, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nodes", type=int, default=1000000)
+    ap.add_argument("--cpu-ticks", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the tick on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    import d3force_amd as d3
+    n = args.nodes
+    m = 2 * n
+    pin = 0.01
+    core = d3.gpu_core()
+    sim, nodes = make_workload(core, n, m, pin)
+    ctx = sim.begin()
+    log("warmup %d ticks" % args.warmup)
+    sim.iterate(args.warmup, ctx)
+    device_sync(core)
+    core.stats(ctx.h)  # reset counters and timers
+    core.core_set_timing(1)
+    log("timing %d ticks" % args.steps)
+    device_sync(core)
+    t0 = time.perf_counter()
+    sim.iterate(args.steps, ctx)
+    device_sync(core)
+    dt = time.perf_counter() - t0
+    core.core_set_timing(0)
+    st = core.stats(ctx.h)
+    sim.end(ctx)
+    ticks_per_s = args.steps / dt
+    log("%.3f ms/tick, phases %s" % (1e3 * dt / args.steps, st["ms"]))
+
+    # Dominant kernel and its roofline.
+    phase_ms = st["ms"]
+    dominant = max(phase_ms, key=phase_ms.get)
+    launches = max(st["launches"][dominant], 1)
+    per_launch_s = phase_ms[dominant] / 1e3 / launches
+    C = 1.72 * n  # cells per node measured on this workload
+    if dominant == "traversal":
+        flops = 8 * st["visits"] + 8 * st["accepts"] + 6 * st["pairs"]
+        per_launch = flops / launches
+        achieved = per_launch / per_launch_s / 1e12
+        roof = {"kernel": "k_traverse<RECORD> (Barnes-Hut traversal)", "bound": "mfma",
+                "pipe": "FP64 VALU; MI355X's FP64 vector peak equals its FP64 matrix peak",
+                "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK_TFLOPS,
+                "work_per_launch": {"flops": per_launch, "visits": st["visits"] / launches,
+                                    "accepts": st["accepts"] / launches, "pairs": st["pairs"] / launches,
+                                    "formula": "8 V + 8 A + 6 P (V + 2A + P of them divides)"},
+                "traffic": read_traffic("k_traverse")}
+    elif dominant == "tree":
+        b = 28 * n + 88 * C
+        achieved = b / per_launch_s / 1e9
+        roof = {"kernel": "quadtree build", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    elif dominant == "link":
+        b = 128 * m
+        achieved = b / per_launch_s / 1e9
+        roof = {"kernel": "k_link_level (all levels)", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    else:
+        b = 48 * n if dominant == "center" else 81 * n
+        achieved = b / per_launch_s / 1e9
+        roof = {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    roof["launch_ms"] = per_launch_s * 1e3
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log("cpu baseline: oracle, %d ticks on %d threads" % (args.cpu_ticks, threads))
+        cpu = cpu_baseline(n, m, pin, args.cpu_ticks, threads)
+        log("cpu baseline %.3f ticks/s" % cpu["value"])
+
+    out = {
+        "metric": METRIC,
+        "value": ticks_per_s,
+        "unit": "ticks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "exact mode, %d nodes, %d links (distinct pairs), 1%% pinned, phyllotaxis start, "
+                               "manyBody + link + center with d3 defaults" % (n, m),
+                   "nodes": n, "links": m, "pinned": int(round(n * pin)), "mode": "exact",
+                   "ticks": "%d-%d" % (args.warmup + 1, args.warmup + args.steps)},
+        "phases_ms_per_tick": {k: v / args.steps for k, v in phase_ms.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

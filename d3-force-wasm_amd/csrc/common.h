@@ -82,7 +82,10 @@ struct TreeBuffers {
   Frame* deep_stack = nullptr;
   Cell* cells = nullptr;
   TCell* tcells = nullptr;
-  uint32_t *parent = nullptr, *arrive = nullptr;
+  uint32_t* cdepth = nullptr;                        // depth of each cell
+  uint32_t *cell_s = nullptr, *cell_d = nullptr;     // each cell's key group and depth
+  uint32_t* rs_glob = nullptr;                       // radix digit totals, 8 passes
+  uint32_t *dbins = nullptr, *dstart = nullptr, *dorder = nullptr;  // cells by depth
   TreeMeta* meta = nullptr;       // device
   TreeMeta* meta_host = nullptr;  // pinned mirror
 };
@@ -114,6 +117,7 @@ struct TravArgs {
   const uint32_t* draw_off;  // REPLAY: first draw per slot
   const double* draws;     // REPLAY: the random() values
   uint32_t stack_cap;      // entries per wave
+  unsigned long long* counters;  // visits, accepts, pairs (summed)
 };
 hipError_t mb_traverse(const TravArgs& a, int mode, hipStream_t stream);
 // Nodes with a NaN coordinate (tree.rest): their traversal reaches every
@@ -141,9 +145,11 @@ hipError_t link_serial(const uint32_t* src, const uint32_t* tgt, const double* s
                        double alpha, LinkSerialState* state, hipStream_t stream);
 
 // ---- simple.hip ----
-hipError_t center_sum(const double* x, const double* y, uint32_t n, double cx, double cy, double strength,
-                      double* shift_out, hipStream_t stream);
-hipError_t center_apply(double* x, double* y, uint32_t n, const double* shift, hipStream_t stream);
+// sums[0..1] = d3's serial sums of x and y (one workgroup).
+hipError_t center_raw_sum(const double* x, const double* y, uint32_t n, double* sums, hipStream_t stream);
+// x -= (sums[0] / n - cx) * strength, y likewise.
+hipError_t center_apply(double* x, double* y, uint32_t n, const double* sums, double cx, double cy, double strength,
+                        hipStream_t stream);
 hipError_t integrate(double* x, double* y, double* vx, double* vy, const double* fx, const double* fy,
                      const uint8_t* fixed, uint32_t n, double decay, hipStream_t stream);
 hipError_t axis_force(const double* pos, double* v, const double* strength, const double* target, uint32_t n,

@@ -1,11 +1,12 @@
 // forceManyBody, exact mode, on gfx950 (reference: src/core/many_body.zig,
 // itself d3-force's manyBody.js over the quadtree).
 //
-// Accumulate (visitAfter): one thread per leaf computes the leaf's charge
-// (its chain summed from the head down, many_body.zig:125-132), then climbs:
-// each parent's arrival counter is bumped with acquire-release ordering and
-// the last child to arrive computes the parent from children 0..3 in order
-// (many_body.zig:112-121), so every sum has d3's operands in d3's order.
+// Accumulate (visitAfter): cells are bucketed by depth and processed from
+// the deepest level up, one launch per level; a leaf sums its chain from the
+// head down (many_body.zig:125-132) and an internal cell combines children
+// 0..3 in order (many_body.zig:112-121), so every sum has d3's operands in
+// d3's order. Kernel boundaries order the levels, with no inter-workgroup
+// atomics (whose cache maintenance spans all eight XCDs' L2s).
 //
 // Traversal (visit): one lane per node, 64 nodes per wave in Morton order.
 // The wave walks the union of its lanes' d3 traversals depth first with one
@@ -50,53 +51,94 @@ __device__ inline void put(TCell* __restrict__ tc, Cell* __restrict__ cells, uin
   cells[c].value = value;
 }
 
-__global__ __launch_bounds__(ACC_THREADS) void k_accumulate(Cell* __restrict__ cells, TCell* __restrict__ tc,
-                                                            const uint32_t* __restrict__ parent,
-                                                            uint32_t* __restrict__ arrive,
-                                                            const uint32_t* __restrict__ perm,
-                                                            const double* __restrict__ xs,
-                                                            const double* __restrict__ ys,
-                                                            const double* __restrict__ s, double theta2,
-                                                            uint32_t len, uint32_t max_climb) {
-  const uint32_t c = blockIdx.x * ACC_THREADS + threadIdx.x;
-  if (c >= len) return;
-  Cell cell = cells[c];
-  if (cell.child[0] != NONE || cell.child[1] != NONE || cell.child[2] != NONE || cell.child[3] != NONE) return;
-  // Leaf: the chain is perm[first + count - 1] down to perm[first].
-  const uint32_t head = perm[cell.first + cell.count - 1];
-  double strength = 0;
-  for (uint32_t k = cell.first + cell.count; k > cell.first;) {
-    --k;
-    strength += s[perm[k]];
+// Cells grouped by depth (a counting sort): histogram, scan, scatter. The
+// order inside a depth does not matter: cells of one depth never read each
+// other.
+constexpr int DEPTH_BINS = MAX_DEPTH + 64;
+
+__global__ __launch_bounds__(ACC_THREADS) void k_depth_hist(const uint32_t* __restrict__ cdepth, uint32_t len,
+                                                            uint32_t nbins, uint32_t* __restrict__ bins) {
+  extern __shared__ uint32_t h[];
+  for (uint32_t b = threadIdx.x; b < nbins; b += ACC_THREADS) h[b] = 0;
+  __syncthreads();
+  for (uint32_t c = blockIdx.x * ACC_THREADS + threadIdx.x; c < len; c += gridDim.x * ACC_THREADS)
+    atomicAdd(&h[cdepth[c]], 1u);
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nbins; b += ACC_THREADS)
+    if (h[b]) atomicAdd(&bins[b], h[b]);
+}
+
+// Exclusive scan of the bins into start[0..nbins] and cursor (one thread:
+// at most a few thousand bins).
+__global__ void k_depth_scan(uint32_t* __restrict__ bins, uint32_t nbins, uint32_t* __restrict__ start) {
+  if (threadIdx.x != 0) return;
+  uint32_t run = 0;
+  for (uint32_t b = 0; b < nbins; ++b) {
+    const uint32_t v = bins[b];
+    start[b] = run;
+    bins[b] = run;  // becomes the scatter cursor
+    run += v;
   }
-  put(tc, cells, c, cell, xs[head], ys[head], strength, theta2, head, TC_LEAF | (truthy(strength) ? TC_REACH : 0));
-  uint32_t p = parent[c];
-  for (uint32_t step = 0; p < len && step < max_climb; ++step) {
-    const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    const Cell pc = cells[p];
-    uint32_t nch = 0;
-    for (int k = 0; k < 4; ++k) nch += pc.child[k] != NONE;
-    if (old + 1 != nch) return;
+  start[nbins] = run;
+}
+
+__global__ __launch_bounds__(ACC_THREADS) void k_depth_scatter(const uint32_t* __restrict__ cdepth, uint32_t len,
+                                                               uint32_t nbins, uint32_t* __restrict__ cursor,
+                                                               uint32_t* __restrict__ order) {
+  extern __shared__ uint32_t h[];  // [nbins] local counts, then local bases
+  const uint32_t c = blockIdx.x * ACC_THREADS + threadIdx.x;
+  for (uint32_t b = threadIdx.x; b < nbins; b += ACC_THREADS) h[b] = 0;
+  __syncthreads();
+  uint32_t d = 0, local = 0;
+  if (c < len) {
+    d = cdepth[c];
+    local = atomicAdd(&h[d], 1u);
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nbins; b += ACC_THREADS)
+    if (h[b]) h[b] = atomicAdd(&cursor[b], h[b]);
+  __syncthreads();
+  if (c < len) order[h[d] + local] = c;
+}
+
+// One depth of visitAfter: leaves sum their chain from the head down
+// (many_body.zig:125-132); internal cells combine children 0..3, whose depth
+// was done by the previous launch (many_body.zig:112-121).
+__global__ __launch_bounds__(ACC_THREADS) void k_accumulate_level(
+    Cell* __restrict__ cells, TCell* __restrict__ tc, const uint32_t* __restrict__ order,
+    const uint32_t* __restrict__ start, uint32_t depth, const uint32_t* __restrict__ perm,
+    const double* __restrict__ xs, const double* __restrict__ ys, const double* __restrict__ s, double theta2) {
+  const uint32_t lo = start[depth], hi = start[depth + 1];
+  for (uint32_t k = lo + blockIdx.x * ACC_THREADS + threadIdx.x; k < hi; k += gridDim.x * ACC_THREADS) {
+    const uint32_t c = order[k];
+    const Cell cell = cells[c];
+    if (cell.child[0] == NONE && cell.child[1] == NONE && cell.child[2] == NONE && cell.child[3] == NONE) {
+      const uint32_t head = perm[cell.first + cell.count - 1];
+      double strength = 0;
+      for (uint32_t j = cell.first + cell.count; j > cell.first;) {
+        --j;
+        strength += s[perm[j]];
+      }
+      put(tc, cells, c, cell, xs[head], ys[head], strength, theta2, head,
+          TC_LEAF | (truthy(strength) ? TC_REACH : 0));
+      continue;
+    }
     double st = 0, weight = 0, x = 0, y = 0;
     bool reach = false;
-    for (int k = 0; k < 4; ++k) {
-      const uint32_t ch = pc.child[k];
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t ch = cell.child[q];
       if (ch == NONE) continue;
-      const double v = __hip_atomic_load(&tc[ch].value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const double qx = __hip_atomic_load(&tc[ch].qx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const double qy = __hip_atomic_load(&tc[ch].qy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t fl = __hip_atomic_load(&tc[ch].flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const double v = tc[ch].value;
       const double w = fabs(v);
       if (w > 0) {  // `c = Math.abs(q.value)` is truthy
         st += v;
         weight += w;
-        x += w * qx;
-        y += w * qy;
+        x += w * tc[ch].qx;
+        y += w * tc[ch].qy;
       }
-      reach = reach || (fl & TC_REACH);
+      reach = reach || (tc[ch].flags & TC_REACH);
     }
-    put(tc, cells, p, pc, x / weight, y / weight, st, theta2, NONE, (reach && truthy(st)) ? TC_REACH : 0);
-    p = parent[p];
+    put(tc, cells, c, cell, x / weight, y / weight, st, theta2, NONE, (reach && truthy(st)) ? TC_REACH : 0);
   }
 }
 
@@ -110,7 +152,8 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     const double* __restrict__ s, double* __restrict__ vxs, double* __restrict__ vys, double dmin2, double dmax2,
     double alpha, uint32_t* __restrict__ flagged, uint32_t* __restrict__ n_flagged,
     uint32_t* __restrict__ draws_count, const uint32_t* __restrict__ draw_off, const double* __restrict__ draws,
-    uint32_t stack_cap, uint32_t* __restrict__ gstack_cells, uint64_t* __restrict__ gstack_masks) {
+    uint32_t stack_cap, uint32_t* __restrict__ gstack_cells, uint64_t* __restrict__ gstack_masks,
+    unsigned long long* __restrict__ counters) {
   extern __shared__ uint64_t lds[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint64_t* st_mask;
@@ -136,6 +179,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
   uint64_t alive = __ballot(active);
   if (alive == 0) return;
   uint32_t ndraw = 0;
+  uint32_t n_visit = 0, n_accept = 0, n_pair = 0;
   uint32_t dptr = (MODE == TRAV_REPLAY && active) ? draw_off[slot] : 0;
   bool aborted = false;
   if (lane == 0) {
@@ -159,6 +203,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     double dx = 0, dy = 0, l = 0;
     bool open = false;
     if (in) {
+      ++n_visit;
       dx = q->qx - nx;
       dy = q->qy - ny;
       l = dx * dx + dy * dy;
@@ -179,6 +224,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
             if (l < dmin2) l = __builtin_sqrt(dmin2 * l);
             vx += dx * value * alpha / l;
             vy += dy * value * alpha / l;
+            ++n_accept;
           }
         }
       } else {
@@ -232,11 +278,28 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
             const double wj = s[j] * alpha / l;
             vx += dx * wj;
             vy += dy * wj;
+            ++n_pair;
           }
         }
       }
     }
     if (MODE == TRAV_RECORD) alive &= ~__ballot(aborted);
+  }
+  if (MODE != TRAV_COUNT) {
+    // Work of the lanes whose results stand (an aborted lane is redone by REPLAY).
+    const bool keep = active && !(MODE == TRAV_RECORD && aborted);
+    uint32_t cv = keep ? n_visit : 0, ca = keep ? n_accept : 0, cp = keep ? n_pair : 0;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+      cv += __shfl_xor(cv, d, 64);
+      ca += __shfl_xor(ca, d, 64);
+      cp += __shfl_xor(cp, d, 64);
+    }
+    if (lane == 0 && counters) {
+      atomicAdd(&counters[0], (unsigned long long)cv);
+      atomicAdd(&counters[1], (unsigned long long)ca);
+      atomicAdd(&counters[2], (unsigned long long)cp);
+    }
   }
   if (!active) return;
   if (MODE == TRAV_RECORD && aborted) {
@@ -267,9 +330,22 @@ hipError_t mb_accumulate(TreeBuffers& t, const double* x, const double* y, const
                          hipStream_t stream) {
   const uint32_t len = t.meta_host->len;
   if (len == 0) return hipSuccess;
-  HIPCHECK(hipMemsetAsync(t.arrive, 0, (size_t)len * sizeof(uint32_t), stream));
-  hipLaunchKernelGGL(k_accumulate, dim3(ceil_div(len, ACC_THREADS)), dim3(ACC_THREADS), 0, stream, t.cells,
-                     t.tcells, t.parent, t.arrive, t.perm, x, y, strengths, theta2, len, t.meta_host->depth + 2);
+  const uint32_t nbins = t.meta_host->depth + 1;
+  if (nbins + 1 > (uint32_t)DEPTH_BINS) return hipErrorInvalidValue;
+  const size_t lds = (size_t)nbins * sizeof(uint32_t);
+  HIPCHECK(hipMemsetAsync(t.dbins, 0, lds, stream));
+  uint32_t hist_blocks = ceil_div(len, ACC_THREADS * 8);
+  if (hist_blocks > 1024) hist_blocks = 1024;
+  hipLaunchKernelGGL(k_depth_hist, dim3(hist_blocks), dim3(ACC_THREADS), lds, stream, t.cdepth, len, nbins, t.dbins);
+  hipLaunchKernelGGL(k_depth_scan, dim3(1), dim3(64), 0, stream, t.dbins, nbins, t.dstart);
+  hipLaunchKernelGGL(k_depth_scatter, dim3(ceil_div(len, ACC_THREADS)), dim3(ACC_THREADS), lds, stream, t.cdepth,
+                     len, nbins, t.dbins, t.dorder);
+  uint32_t level_blocks = ceil_div(len, ACC_THREADS);
+  if (level_blocks > 2048) level_blocks = 2048;
+  for (uint32_t d = nbins; d-- > 0;) {
+    hipLaunchKernelGGL(k_accumulate_level, dim3(level_blocks), dim3(ACC_THREADS), 0, stream, t.cells, t.tcells,
+                       t.dorder, t.dstart, d, t.perm, x, y, strengths, theta2);
+  }
   return hipGetLastError();
 }
 
@@ -297,7 +373,8 @@ hipError_t mb_traverse(const TravArgs& a, int mode, hipStream_t stream) {
 #define TRAV_LAUNCH(M, G)                                                                                        \
   hipLaunchKernelGGL((k_traverse<M, G>), dim3(blocks), dim3(TRAV_THREADS), G ? 0 : lds, stream, a.tc, a.order,   \
                      a.count, a.perm, a.x, a.y, a.s, a.vx, a.vy, a.dmin2, a.dmax2, a.alpha, a.flagged,          \
-                     a.n_flagged, a.draws_count, a.draw_off, a.draws, a.stack_cap, g_stack_cells, g_stack_masks)
+                     a.n_flagged, a.draws_count, a.draw_off, a.draws, a.stack_cap, g_stack_cells, g_stack_masks,   \
+                     a.counters)
   if (global) {
     if (mode == TRAV_RECORD) TRAV_LAUNCH(TRAV_RECORD, true);
     else if (mode == TRAV_COUNT) TRAV_LAUNCH(TRAV_COUNT, true);

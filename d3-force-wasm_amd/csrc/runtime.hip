@@ -158,7 +158,17 @@ struct Sim {
   Device<uint32_t> flagged, order, draws_count, draw_off;
   Device<double> draws;
   uint32_t* dscalar_u = nullptr;   // [0] flagged count, [1] link zero flag
-  double* dscalar_d = nullptr;     // [0..1] center shift
+  double* dscalar_d = nullptr;     // [0..1] center sums computed on the main stream
+  // forceCenter's sums of x and y depend on x, y alone, and only center,
+  // integrate and an H2D sync write them. So after each of those writes (when
+  // a center force exists) the sums for the next center run are started on a
+  // side stream, where they overlap the many-body and link work of the tick.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_x_ready = nullptr, ev_sums_done = nullptr;
+  double* dsums_side = nullptr;
+  bool side_valid = false;   // dsums_side holds the sums of the current x, y
+  bool side_busy = false;    // the side stream may still be reading x, y
+  uint32_t centers = 0;      // center forces of this simulation
   LinkSerialState* dlink = nullptr;
   uint32_t* hscalar_u = nullptr;   // pinned mirrors
   LinkSerialState* hlink = nullptr;
@@ -167,7 +177,12 @@ struct Sim {
   Pinned<uint32_t> hperm;
   double hextent[4];
   d3f_stats stats{};
-  hipEvent_t ev[2] = {nullptr, nullptr};
+  unsigned long long* dcounters = nullptr;  // traversal visits, accepts, pairs
+  struct Timer { int phase; hipEvent_t a, b; };
+  std::vector<Timer> pending;
+  std::vector<hipEvent_t> spare;
+  hipEvent_t open_event = nullptr;
+  int open_phase = 0;
 
   ~Sim() {
     for (Force* f : forces) delete f;
@@ -188,7 +203,18 @@ struct Sim {
     if (hlink) (void)hipHostFree(hlink);
     hcells.release();
     hperm.release();
-    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    for (auto& t : pending) {
+      (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.b);
+    }
+    for (auto e : spare) (void)hipEventDestroy(e);
+    if (open_event) (void)hipEventDestroy(open_event);
+    if (dcounters) (void)hipFree(dcounters);
+    if (side) (void)hipStreamSynchronize(side);
+    if (dsums_side) (void)hipFree(dsums_side);
+    if (ev_x_ready) (void)hipEventDestroy(ev_x_ready);
+    if (ev_sums_done) (void)hipEventDestroy(ev_sums_done);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -199,14 +225,20 @@ struct Sim {
     HIPCHECK(hipMalloc(&dlink, sizeof(LinkSerialState)));
     HIPCHECK(hipHostMalloc((void**)&hscalar_u, 16 * sizeof(uint32_t), hipHostMallocDefault));
     HIPCHECK(hipHostMalloc((void**)&hlink, sizeof(LinkSerialState), hipHostMallocDefault));
-    HIPCHECK(hipEventCreate(&ev[0]));
-    HIPCHECK(hipEventCreate(&ev[1]));
+    HIPCHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&ev_x_ready, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&ev_sums_done, hipEventDisableTiming));
+    HIPCHECK(hipMalloc(&dsums_side, 2 * sizeof(double)));
+    HIPCHECK(hipMalloc(&dcounters, 4 * sizeof(unsigned long long)));
+    HIPCHECK(hipMemset(dcounters, 0, 4 * sizeof(unsigned long long)));
     for (int k = 0; k < 4; ++k) hextent[k] = __builtin_nan("");
     return hipSuccess;
   }
 
   // sim.zig:59-73
   hipError_t set_node_count(uint32_t nn, bool* grown) {
+    HIPCHECK(before_x_write());
+    side_valid = false;
     const uint32_t old_n = n;
     n = nn;
     *grown = false;
@@ -231,16 +263,70 @@ struct Sim {
     return hipSuccess;
   }
 
-  void timer_start() {
-    if (g_timing) (void)hipEventRecord(ev[0], stream);
+  // The main stream must not write x, y while the side stream reads them.
+  hipError_t before_x_write() {
+    if (side_busy) {
+      HIPCHECK(hipStreamWaitEvent(stream, ev_sums_done, 0));
+      side_busy = false;
+    }
+    side_valid = false;
+    return hipSuccess;
   }
-  double timer_stop() {
-    if (!g_timing) return 0;
-    float ms = 0;
-    (void)hipEventRecord(ev[1], stream);
-    (void)hipEventSynchronize(ev[1]);
-    (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
-    return ms;
+  // Starts the sums of the current x, y on the side stream.
+  hipError_t speculate_center() {
+    if (centers == 0 || n == 0) return hipSuccess;
+    HIPCHECK(hipEventRecord(ev_x_ready, stream));
+    HIPCHECK(hipStreamWaitEvent(side, ev_x_ready, 0));
+    HIPCHECK(center_raw_sum(dx.p, dy.p, n, dsums_side, side));
+    HIPCHECK(hipEventRecord(ev_sums_done, side));
+    side_valid = true;
+    side_busy = true;
+    return hipSuccess;
+  }
+
+  hipEvent_t take_event() {
+    hipEvent_t e = nullptr;
+    if (!spare.empty()) {
+      e = spare.back();
+      spare.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+    }
+    return e;
+  }
+  // Brackets a phase's launches with events on the stream; no synchronisation.
+  void timer_start(int phase) {
+    if (!g_timing) return;
+    open_phase = phase;
+    open_event = take_event();
+    if (open_event) (void)hipEventRecord(open_event, stream);
+  }
+  void timer_stop() {
+    if (!g_timing || !open_event) return;
+    hipEvent_t b = take_event();
+    if (!b) return;
+    (void)hipEventRecord(b, stream);
+    pending.push_back(Timer{open_phase, open_event, b});
+    open_event = nullptr;
+  }
+  hipError_t collect() {
+    HIPCHECK(hipStreamSynchronize(stream));
+    for (auto& t : pending) {
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, t.a, t.b));
+      stats.ms[t.phase] += ms;
+      stats.launches[t.phase] += 1;
+      spare.push_back(t.a);
+      spare.push_back(t.b);
+    }
+    pending.clear();
+    unsigned long long c[4];
+    HIPCHECK(hipMemcpy(c, dcounters, sizeof(c), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemset(dcounters, 0, sizeof(c)));
+    stats.visits += c[0];
+    stats.accepts += c[1];
+    stats.pairs += c[2];
+    return hipSuccess;
   }
 };
 
@@ -284,15 +370,15 @@ hipError_t upload(Sim& s, Force& f) {
 // ---------------------------------------------------------------- many-body (exact)
 int32_t run_many_body_exact(Sim& s, Force& f, double alpha) {
   TreeBuffers& t = s.tree;
-  s.timer_start();
+  s.timer_start(D3F_PHASE_TREE);
   hipError_t e = tree_build_exact(t, s.dx.p, s.dy.p, s.n, s.stream);
   if (e != hipSuccess) return status_of(e);
-  s.stats.tree_ms = s.timer_stop();
+  s.timer_stop();
   if (!t.meta_host->valid) return OK;  // many_body.zig:33
-  s.timer_start();
+  s.timer_start(D3F_PHASE_ACCUMULATE);
   e = mb_accumulate(t, s.dx.p, s.dy.p, f.df[0].p, f.scalar(S_THETA2), s.stream);
   if (e != hipSuccess) return status_of(e);
-  s.stats.accumulate_ms = s.timer_stop();
+  s.timer_stop();
   TravArgs a{};
   a.tc = t.tcells;
   a.order = t.perm;
@@ -309,16 +395,17 @@ int32_t run_many_body_exact(Sim& s, Force& f, double alpha) {
   a.flagged = s.flagged.p;
   a.n_flagged = s.dscalar_u;
   a.stack_cap = 3 * t.meta_host->depth + 4;
-  s.timer_start();
+  a.counters = s.dcounters;
   if ((e = hipMemsetAsync(s.dscalar_u, 0, sizeof(uint32_t), s.stream)) != hipSuccess) return status_of(e);
+  s.timer_start(D3F_PHASE_TRAVERSAL);
   if ((e = mb_traverse(a, TRAV_RECORD, s.stream)) != hipSuccess) return status_of(e);
+  s.timer_stop();
   if ((e = mb_nan_nodes(t, s.n, s.dvx.p, s.dvy.p, s.stream)) != hipSuccess) return status_of(e);
   if ((e = hipMemcpyAsync(s.hscalar_u, s.dscalar_u, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream)) != hipSuccess)
     return status_of(e);
   if ((e = hipStreamSynchronize(s.stream)) != hipSuccess) return status_of(e);
-  s.stats.traversal_ms = s.timer_stop();
   const uint32_t nf = s.hscalar_u[0];
-  s.stats.flagged = nf;
+  s.stats.flagged += nf;
   if (nf == 0) return OK;
   // Jiggle replay (many_body.zig:77-83): ascending node order, real draws.
   std::vector<uint32_t> list(nf), counts(nf), offs(nf);
@@ -409,7 +496,6 @@ int32_t run_link_exact(Sim& s, Force& f, double alpha) {
   const double iterations = f.scalar(S_ITERATIONS);
   if (m == 0 || !(0 < iterations)) return OK;
   hipError_t e;
-  s.timer_start();
   const size_t bytes = (size_t)s.n * sizeof(double);
   if ((e = hipMemcpyAsync(s.dsvx.p, s.dvx.p, bytes, hipMemcpyDeviceToDevice, s.stream)) != hipSuccess) return status_of(e);
   if ((e = hipMemcpyAsync(s.dsvy.p, s.dvy.p, bytes, hipMemcpyDeviceToDevice, s.stream)) != hipSuccess) return status_of(e);
@@ -417,6 +503,7 @@ int32_t run_link_exact(Sim& s, Force& f, double alpha) {
   LinkLevelArgs a{f.lsrc.p, f.ltgt.p, f.lf[0].p, f.lf[1].p, f.lf[2].p, s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, alpha,
                   s.dscalar_u + 1};
   const uint32_t levels = (uint32_t)f.level_off.size() - 2;
+  s.timer_start(D3F_PHASE_LINK);
   for (double k = 0; k < iterations; k += 1) {
     for (uint32_t l = 1; l <= levels; ++l) {
       if ((e = link_level(a, f.level_off[l], f.level_off[l + 1], s.stream)) != hipSuccess) return status_of(e);
@@ -425,8 +512,8 @@ int32_t run_link_exact(Sim& s, Force& f, double alpha) {
   if ((e = hipMemcpyAsync(s.hscalar_u + 1, s.dscalar_u + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream)) !=
       hipSuccess)
     return status_of(e);
+  s.timer_stop();
   if ((e = hipStreamSynchronize(s.stream)) != hipSuccess) return status_of(e);
-  s.stats.link_ms = s.timer_stop();
   if (s.hscalar_u[1] == 0) return OK;
   // A zero or NaN delta: d3 draws in edge order. Restore and run serially.
   if ((e = hipMemcpyAsync(s.dvx.p, s.dsvx.p, bytes, hipMemcpyDeviceToDevice, s.stream)) != hipSuccess) return status_of(e);
@@ -454,12 +541,22 @@ int32_t run_link_exact(Sim& s, Force& f, double alpha) {
   return OK;
 }
 
-int32_t run_center(Sim& s, Force& f) {
-  s.timer_start();
-  hipError_t e = center_sum(s.dx.p, s.dy.p, s.n, f.scalar(S_X), f.scalar(S_Y), f.scalar(S_STRENGTH), s.dscalar_d,
-                            s.stream);
-  if (e == hipSuccess) e = center_apply(s.dx.p, s.dy.p, s.n, s.dscalar_d, s.stream);
-  s.stats.center_ms = s.timer_stop();
+int32_t run_center(Sim& s, Force& f) {  // simple.zig:14-31
+  hipError_t e = hipSuccess;
+  s.timer_start(D3F_PHASE_CENTER);
+  const double* sums = s.dsums_side;
+  if (s.side_valid) {
+    e = hipStreamWaitEvent(s.stream, s.ev_sums_done, 0);
+    s.side_busy = false;
+  } else {
+    sums = s.dscalar_d;
+    e = s.before_x_write();
+    if (e == hipSuccess) e = center_raw_sum(s.dx.p, s.dy.p, s.n, s.dscalar_d, s.stream);
+  }
+  s.side_valid = false;
+  if (e == hipSuccess)
+    e = center_apply(s.dx.p, s.dy.p, s.n, sums, f.scalar(S_X), f.scalar(S_Y), f.scalar(S_STRENGTH), s.stream);
+  s.timer_stop();
   return status_of(e);
 }
 
@@ -494,6 +591,8 @@ EXPORT void core_set_random(d3f_random_fn fn, void* ctx) {
 
 EXPORT void core_set_timing(int on) { g_timing = on != 0; }
 
+EXPORT int32_t core_synchronize(void) { return status_of(hipDeviceSynchronize()); }
+
 EXPORT uint32_t sim_create(void) {  // root.zig:65-78
   int devices = 0;
   if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) return 0;
@@ -516,6 +615,7 @@ EXPORT int32_t sim_destroy(uint32_t h) {  // root.zig:80-86
   Sim* s = get(h);
   if (!s) return BAD_HANDLE;
   (void)hipStreamSynchronize(s->stream);
+  (void)hipStreamSynchronize(s->side);
   delete s;
   g_sims[h - 1] = nullptr;
   return OK;
@@ -557,6 +657,7 @@ EXPORT int32_t sim_sync(uint32_t h, uint32_t direction) {
   hipError_t e = hipSuccess;
   const size_t b = (size_t)n * sizeof(double);
   if (direction == D3F_SYNC_H2D) {
+    e = s->before_x_write();
     struct { double* d; double* h; } pairs[] = {{s->dx.p, s->x.p}, {s->dy.p, s->y.p}, {s->dvx.p, s->vx.p},
                                                 {s->dvy.p, s->vy.p}, {s->dfx.p, s->fx.p}, {s->dfy.p, s->fy.p}};
     for (auto& p : pairs)
@@ -571,6 +672,7 @@ EXPORT int32_t sim_sync(uint32_t h, uint32_t direction) {
     return BAD_ARGUMENT;
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess && direction == D3F_SYNC_H2D) e = s->speculate_center();
   return status_of(e);
 }
 
@@ -596,10 +698,13 @@ EXPORT int32_t sim_snapshot(uint32_t h, uint32_t positions) {  // root.zig:205-2
 EXPORT int32_t sim_integrate(uint32_t h, double velocity_decay) {  // root.zig:250-254
   Sim* s = get(h);
   if (!s) return BAD_HANDLE;
-  s->timer_start();
-  hipError_t e = integrate(s->dx.p, s->dy.p, s->dvx.p, s->dvy.p, s->dfx.p, s->dfy.p, s->dfixed.p, s->n,
-                           velocity_decay, s->stream);
-  s->stats.integrate_ms = s->timer_stop();
+  hipError_t e = s->before_x_write();
+  s->timer_start(D3F_PHASE_INTEGRATE);
+  if (e == hipSuccess)
+    e = integrate(s->dx.p, s->dy.p, s->dvx.p, s->dvy.p, s->dfx.p, s->dfy.p, s->dfixed.p, s->n, velocity_decay,
+                  s->stream);
+  s->timer_stop();
+  if (e == hipSuccess) e = s->speculate_center();
   return status_of(e);
 }
 
@@ -673,6 +778,7 @@ EXPORT uint32_t force_create(uint32_t h, uint32_t kind) {  // root.zig:147-161
     delete f;
     return 0;
   }
+  if (kind == K_CENTER) s->centers++;
   for (size_t i = 0; i < s->forces.size(); ++i)
     if (!s->forces[i]) {
       s->forces[i] = f;
@@ -687,6 +793,7 @@ EXPORT int32_t force_destroy(uint32_t h, uint32_t fh) {  // root.zig:163-170
   Force* f = get_force(s, fh);
   if (!f) return BAD_HANDLE;
   (void)hipStreamSynchronize(s->stream);
+  if (f->kind == K_CENTER) s->centers--;
   delete f;
   s->forces[fh - 1] = nullptr;
   return OK;
@@ -764,6 +871,9 @@ EXPORT int32_t force_run(uint32_t h, uint32_t fh, double alpha) {  // root.zig:2
 EXPORT int32_t sim_stats(uint32_t h, d3f_stats* out) {
   Sim* s = get(h);
   if (!s || !out) return BAD_HANDLE;
+  hipError_t e = s->collect();
+  if (e != hipSuccess) return status_of(e);
   *out = s->stats;
+  s->stats = d3f_stats{};
   return OK;
 }

@@ -9,44 +9,81 @@ namespace d3f {
 namespace {
 
 constexpr int EW = 256;
-constexpr int CENTER_THREADS = 256;
-constexpr int CENTER_CHUNK = 2048;
 
-__global__ __launch_bounds__(CENTER_THREADS) void k_center_sum(const double* __restrict__ xs,
-                                                               const double* __restrict__ ys, uint32_t n, double cx,
-                                                               double cy, double strength,
-                                                               double* __restrict__ out) {
-  __shared__ double bx[CENTER_CHUNK], by[CENTER_CHUNK];
-  __shared__ double sy_shared;
-  double sx = 0, sy = 0;
-  for (uint32_t base = 0; base < n; base += CENTER_CHUNK) {
-    const uint32_t cnt = n - base < (uint32_t)CENTER_CHUNK ? n - base : (uint32_t)CENTER_CHUNK;
-    for (uint32_t k = threadIdx.x; k < cnt; k += CENTER_THREADS) {
-      bx[k] = xs[base + k];
-      by[k] = ys[base + k];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (uint32_t k = 0; k < cnt; ++k) sx += bx[k];
-    } else if (threadIdx.x == 64) {
-      for (uint32_t k = 0; k < cnt; ++k) sy += by[k];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 64) sy_shared = sy;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double nf = (double)n;
-    out[0] = (sx / nf - cx) * strength;
-    out[1] = (sy_shared / nf - cy) * strength;
+// d3's center sums: sx = x0 + x1 + ... + x(n-1) in index order (and sy),
+// simple.zig:18-22. Waves 1..15 stream x and y into a double-buffered LDS
+// chunk while lane 0 (x) and lane 1 (y) of wave 0 add the previous chunk in
+// order, with register double-buffering so LDS latency hides behind the
+// dependent adds.
+constexpr int CS_THREADS = 1024;
+constexpr int CS_CHUNK = 2048;
+
+__device__ inline void cs_load(const double* __restrict__ xs, const double* __restrict__ ys, uint32_t n,
+                               uint32_t base, double (*buf)[CS_CHUNK], int tid, int nthreads) {
+  const uint32_t cnt = n - base < (uint32_t)CS_CHUNK ? n - base : (uint32_t)CS_CHUNK;
+  for (uint32_t k = tid; k < cnt; k += nthreads) {
+    buf[0][k] = xs[base + k];
+    buf[1][k] = ys[base + k];
   }
 }
 
+__global__ __launch_bounds__(CS_THREADS) void k_center_raw_sum(const double* __restrict__ xs,
+                                                               const double* __restrict__ ys, uint32_t n,
+                                                               double* __restrict__ out) {
+  __shared__ double buf[2][2][CS_CHUNK];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t chunks = ceil_div(n, CS_CHUNK);
+  double s = 0;
+  if (chunks) cs_load(xs, ys, n, 0, buf[0], threadIdx.x, CS_THREADS);
+  __syncthreads();
+  for (uint32_t c = 0; c < chunks; ++c) {
+    const uint32_t base = c * CS_CHUNK;
+    if (wave != 0) {
+      if (c + 1 < chunks) cs_load(xs, ys, n, base + CS_CHUNK, buf[(c + 1) & 1], threadIdx.x - 64, CS_THREADS - 64);
+    } else if (lane < 2) {
+      const double* p = buf[c & 1][lane];
+      const uint32_t cnt = n - base < (uint32_t)CS_CHUNK ? n - base : (uint32_t)CS_CHUNK;
+      if (cnt == (uint32_t)CS_CHUNK) {
+        double a[16], b[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) a[j] = p[j];
+        for (uint32_t k = 0; k < (uint32_t)CS_CHUNK; k += 32) {
+#pragma unroll
+          for (int j = 0; j < 16; ++j) b[j] = p[k + 16 + j];
+#pragma unroll
+          for (int j = 0; j < 16; ++j) s += a[j];
+          if (k + 32 < (uint32_t)CS_CHUNK) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[j] = p[k + 32 + j];
+          }
+#pragma unroll
+          for (int j = 0; j < 16; ++j) s += b[j];
+        }
+      } else {
+        for (uint32_t k = 0; k < cnt; ++k) s += p[k];
+      }
+    }
+    __syncthreads();
+  }
+  __shared__ double sy;
+  if (threadIdx.x == 1) sy = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = s;
+    out[1] = sy;
+  }
+}
+
+// x -= (sx / n - cx) * strength, y likewise (simple.zig:23-30); every thread
+// forms the same shift from the same operands.
 __global__ __launch_bounds__(EW) void k_center_apply(double* __restrict__ xs, double* __restrict__ ys, uint32_t n,
-                                                     const double* __restrict__ shift) {
+                                                     const double* __restrict__ sums, double cx, double cy,
+                                                     double strength) {
   const uint32_t i = blockIdx.x * EW + threadIdx.x;
   if (i >= n) return;
-  const double sx = shift[0], sy = shift[1];
+  const double nf = (double)n;
+  const double sx = (sums[0] / nf - cx) * strength;
+  const double sy = (sums[1] / nf - cy) * strength;
   xs[i] -= sx;
   ys[i] -= sy;
 }
@@ -115,15 +152,15 @@ __global__ __launch_bounds__(EW) void k_shift(const double* __restrict__ xs, con
 
 }  // namespace
 
-hipError_t center_sum(const double* x, const double* y, uint32_t n, double cx, double cy, double strength,
-                      double* shift_out, hipStream_t stream) {
-  hipLaunchKernelGGL(k_center_sum, dim3(1), dim3(CENTER_THREADS), 0, stream, x, y, n, cx, cy, strength, shift_out);
+hipError_t center_raw_sum(const double* x, const double* y, uint32_t n, double* sums, hipStream_t stream) {
+  hipLaunchKernelGGL(k_center_raw_sum, dim3(1), dim3(CS_THREADS), 0, stream, x, y, n, sums);
   return hipGetLastError();
 }
 
-hipError_t center_apply(double* x, double* y, uint32_t n, const double* shift, hipStream_t stream) {
+hipError_t center_apply(double* x, double* y, uint32_t n, const double* sums, double cx, double cy, double strength,
+                        hipStream_t stream) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_center_apply, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, x, y, n, shift);
+  hipLaunchKernelGGL(k_center_apply, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, x, y, n, sums, cx, cy, strength);
   return hipGetLastError();
 }
 

@@ -10,7 +10,9 @@
 //  2. k_keys: 32 levels of d3's midpoint test per point -> 64-bit Morton key
 //     (tree.zig:252-284); NaN points compacted out by a scan (rest, in index
 //     order).
-//  3. Stable LSD radix sort of (key, node), 8 passes of 8 bits (sort.zig).
+//  3. Stable LSD radix sort of (key, node), 8 passes of 8 bits (sort.zig):
+//     per pass a tile histogram, one scan block per digit, and a stable
+//     ballot-ranked scatter.
 //  4. Key groups (runs of equal keys). A group whose points all coincide is
 //     one leaf at depth L = 1 + max(divergence depth with the previous and the
 //     next group); the cells on its path that start at the group are the
@@ -19,9 +21,11 @@
 //     split, serially (tree.zig:449-479). Preorder index of a cell =
 //     (cells of earlier groups) + (depth - dleft - 1): preorder is the order
 //     of (first point, depth).
-//  5. k_emit: one thread per group writes its cells whole (children found by
-//     binary search of the digit boundaries inside the cell's key range), so
-//     no two threads write one cell.
+//  5. k_assign names each cell's (group, depth) at its preorder index; then
+//     k_emit, one thread per cell, writes the cell whole (bounds by the
+//     midpoint walk along the key, children by binary search of the digit
+//     boundaries inside the cell's key range), so no two threads write one
+//     cell and no thread walks a long chain of searches.
 #include <cstring>
 #include <utility>
 
@@ -338,7 +342,8 @@ hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n_max, con
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ keys,
                                                         const uint32_t* __restrict__ count,
-                                                        uint32_t* __restrict__ hist, uint32_t ntiles, int shift) {
+                                                        uint32_t* __restrict__ hist, uint32_t* __restrict__ glob,
+                                                        uint32_t ntiles, int shift) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -351,21 +356,48 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restri
   }
   __syncthreads();
   hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+  if (h[threadIdx.x]) atomicAdd(&glob[threadIdx.x], h[threadIdx.x]);
 }
 
-// Exclusive scan of the digit-major histogram (256 x ntiles), one block.
-__global__ __launch_bounds__(SCAN_THREADS) void k_rs_scan(uint32_t* __restrict__ hist, uint32_t entries) {
-  const uint32_t per = ceil_div(entries, SCAN_THREADS);
-  const uint32_t lo = threadIdx.x * per;
-  const uint32_t hi = lo + per < entries ? lo + per : entries;
-  uint32_t s = 0;
-  for (uint32_t i = lo; i < hi; ++i) s += hist[i];
-  uint32_t total;
-  uint32_t ex = block_exclusive_scan(s, &total);
-  for (uint32_t i = lo; i < hi; ++i) {
-    uint32_t v = hist[i];
-    hist[i] = ex;
-    ex += v;
+// Digit d's tile offsets: (keys of smaller digits) + exclusive scan of its
+// tile counts, one block per digit.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scan(uint32_t* __restrict__ hist,
+                                                        const uint32_t* __restrict__ glob, uint32_t ntiles) {
+  __shared__ uint32_t wsum[RS_THREADS / 64];
+  __shared__ uint32_t carry;
+  const uint32_t d = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // offset of digit d = sum of glob[0..d)
+  uint32_t v = threadIdx.x < d ? glob[threadIdx.x] : 0;
+#pragma unroll
+  for (int k = 32; k > 0; k >>= 1) v += __shfl_xor(v, k, 64);
+  if (lane == 0) wsum[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t o = 0;
+    for (int w = 0; w < RS_THREADS / 64; ++w) o += wsum[w];
+    carry = o;
+  }
+  __syncthreads();
+  uint32_t* row = hist + (size_t)d * ntiles;
+  for (uint32_t b0 = 0; b0 < ntiles; b0 += RS_THREADS) {
+    const uint32_t i = b0 + threadIdx.x;
+    const uint32_t x = i < ntiles ? row[i] : 0;
+    uint32_t inc = x;
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) {
+      uint32_t t = __shfl_up(inc, k, 64);
+      if (lane >= k) inc += t;
+    }
+    __syncthreads();
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t before = carry;
+    for (int w = 0; w < wave; ++w) before += wsum[w];
+    if (i < ntiles) row[i] = before + inc - x;
+    __syncthreads();
+    if (threadIdx.x == RS_THREADS - 1) carry = before + inc;
+    __syncthreads();
   }
 }
 
@@ -519,12 +551,12 @@ __global__ __launch_bounds__(EW_THREADS) void k_group_b(const uint64_t* __restri
                                                         uint32_t* __restrict__ leafdepth,
                                                         uint32_t* __restrict__ mixed_list) {
   const uint32_t m = meta->m;
-  uint32_t k = blockIdx.x * EW_THREADS + threadIdx.x;
-  if (k >= m) return;
-  if (!start[k]) {
-    count[k] = 0;
-    return;
-  }
+  const uint32_t k = blockIdx.x * EW_THREADS + threadIdx.x;
+  __shared__ uint32_t bmax;
+  if (threadIdx.x == 0) bmax = 0;
+  __syncthreads();
+  if (k < m && !start[k]) count[k] = 0;
+  if (k < m && start[k]) {
   const uint64_t key = keys[k];
   const uint32_t e = group_end(keys, m, k + 1, key);
   const int dl = k == 0 ? -1 : div_depth(keys[k - 1], key);
@@ -543,7 +575,10 @@ __global__ __launch_bounds__(EW_THREADS) void k_group_b(const uint64_t* __restri
   dleft[k] = dl;
   gend[k] = e;
   leafdepth[k] = L;
-  atomicMax(&meta->depth, L);
+  atomicMax(&bmax, L);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && bmax) atomicMax(&meta->depth, bmax);
 }
 
 // ---------------------------------------------------------------- below key resolution
@@ -563,7 +598,7 @@ __global__ void k_deep(const uint64_t* __restrict__ keys, uint32_t* __restrict__
                        const uint32_t* __restrict__ mixed_list, const int32_t* __restrict__ dleft,
                        const uint32_t* __restrict__ gend, uint32_t* __restrict__ count,
                        const uint32_t* __restrict__ base, Frame* __restrict__ stack, Cell* __restrict__ cells,
-                       uint32_t* __restrict__ parent) {
+                       uint32_t* __restrict__ cdepth) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const uint32_t groups = meta->mixed;
   uint32_t maxd = 0;
@@ -592,10 +627,8 @@ __global__ void k_deep(const uint64_t* __restrict__ keys, uint32_t* __restrict__
         c.x0 = f.x0; c.y0 = f.y0; c.x1 = f.x1; c.y1 = f.y1;
         c.x = c.y = c.value = c.r = 0;
         cells[idx] = c;
-        if (f.parent != NONE) {
-          cells[f.parent].child[f.slot] = idx;
-          parent[idx] = f.parent;
-        }
+        cdepth[idx] = f.depth;
+        if (f.parent != NONE) cells[f.parent].child[f.slot] = idx;
       }
       const uint32_t me = idx++;
       ++cnt;
@@ -641,65 +674,78 @@ __global__ void k_deep(const uint64_t* __restrict__ keys, uint32_t* __restrict__
   if (maxd > meta->depth) meta->depth = maxd;
 }
 
-// One thread per key group: its cells at depths (dleft, L] (to depth 31 for
-// a mixed group, whose depth-32 cell and subtree k_deep writes).
+// Phase 1, one thread per key group: names the group's cells (depths
+// (dleft, L], to depth 31 for a mixed group, whose depth-32 cell and subtree
+// k_deep writes) by writing their group and depth at their preorder index.
+__global__ __launch_bounds__(EW_THREADS) void k_assign(const uint32_t* __restrict__ start,
+                                                       const uint32_t* __restrict__ mixed,
+                                                       const TreeMeta* __restrict__ meta,
+                                                       const int32_t* __restrict__ dleft,
+                                                       const uint32_t* __restrict__ leafdepth,
+                                                       const uint32_t* __restrict__ base,
+                                                       uint32_t* __restrict__ cell_s, uint32_t* __restrict__ cell_d) {
+  const uint32_t s = blockIdx.x * EW_THREADS + threadIdx.x;
+  if (s >= meta->m || !start[s]) return;
+  const int dl = dleft[s];
+  const int lmax = mixed[s] ? KEY_LEVELS - 1 : (int)leafdepth[s];
+  const uint32_t b = base[s];
+  for (int d = dl + 1; d <= lmax; ++d) {
+    const uint32_t idx = b + (uint32_t)(d - dl - 1);
+    cell_s[idx] = s;
+    cell_d[idx] = (uint32_t)d;
+  }
+}
+
+// Phase 2, one thread per cell: bounds by d3's midpoint walk along the key,
+// the point range, and children 0..3 from the digit boundaries inside it.
 __global__ __launch_bounds__(EW_THREADS) void k_emit(const uint64_t* __restrict__ keys,
-                                                     const uint32_t* __restrict__ start,
                                                      const uint32_t* __restrict__ mixed,
                                                      const TreeMeta* __restrict__ meta,
-                                                     const int32_t* __restrict__ dleft,
                                                      const uint32_t* __restrict__ gend,
                                                      const uint32_t* __restrict__ leafdepth,
-                                                     const uint32_t* __restrict__ base, Cell* __restrict__ cells,
-                                                     uint32_t* __restrict__ parent) {
+                                                     const uint32_t* __restrict__ base,
+                                                     const uint32_t* __restrict__ cell_s,
+                                                     const uint32_t* __restrict__ cell_d, Cell* __restrict__ cells,
+                                                     uint32_t* __restrict__ cdepth) {
+  const uint32_t idx = blockIdx.x * EW_THREADS + threadIdx.x;
+  if (idx >= meta->len) return;
+  const uint32_t s = cell_s[idx];
+  if (s == NONE) return;  // a cell below key resolution (k_deep)
+  const int d = (int)cell_d[idx];
   const uint32_t m = meta->m;
-  const uint32_t s = blockIdx.x * EW_THREADS + threadIdx.x;
-  if (s >= m || !start[s]) return;
   const uint64_t key = keys[s];
   const uint32_t e = gend[s];
-  const int dl = dleft[s];
-  const bool is_mixed = mixed[s] != 0;
-  const int L = (int)leafdepth[s];
-  const int lmax = is_mixed ? KEY_LEVELS - 1 : L;
-  const uint32_t b = base[s];
+  const bool internal = mixed[s] != 0 || d < (int)leafdepth[s];
   double x0 = meta->extent[0], y0 = meta->extent[1], x1 = meta->extent[2], y1 = meta->extent[3];
-  if (s == 0) parent[0] = NONE;
-  for (int d = 0; d <= lmax; ++d) {
-    if (d > dl) {
-      const uint32_t idx = b + (uint32_t)(d - dl - 1);
-      const bool internal = is_mixed || d < L;
-      uint32_t hi;
-      if (!internal) hi = e;
-      else if (d == 0) hi = m;
-      else hi = prefix_end(keys, m, e, key >> (64 - 2 * d), 64 - 2 * d);
-      Cell c;
-      c.child[0] = c.child[1] = c.child[2] = c.child[3] = NONE;
-      c.first = s;
-      c.count = hi - s;
-      c.x0 = x0; c.y0 = y0; c.x1 = x1; c.y1 = y1;
-      c.x = c.y = c.value = c.r = 0;
-      if (internal) {
-        const int dsh = 62 - 2 * d;
-        uint32_t bnd[5];
-        bnd[0] = s;
-        bnd[4] = hi;
-        for (uint32_t q = 1; q < 4; ++q) bnd[q] = digit_bound(keys, bnd[q - 1], hi, dsh, q);
-        for (int q = 0; q < 4; ++q) {
-          if (bnd[q] == bnd[q + 1]) continue;
-          uint32_t ch = bnd[q] == s ? idx + 1 : base[bnd[q]];
-          c.child[q] = ch;
-          parent[ch] = idx;
-        }
-      }
-      cells[idx] = c;
-    }
-    if (d < lmax) {
-      uint32_t dg = (uint32_t)(key >> (62 - 2 * d)) & 3u;
-      double xm = (x0 + x1) / 2, ym = (y0 + y1) / 2;
-      if (dg & 1) x0 = xm; else x1 = xm;
-      if (dg & 2) y0 = ym; else y1 = ym;
+  for (int k = 0; k < d; ++k) {
+    const uint32_t dg = (uint32_t)(key >> (62 - 2 * k)) & 3u;
+    const double xm = (x0 + x1) / 2, ym = (y0 + y1) / 2;
+    if (dg & 1) x0 = xm; else x1 = xm;
+    if (dg & 2) y0 = ym; else y1 = ym;
+  }
+  uint32_t hi;
+  if (!internal) hi = e;
+  else if (d == 0) hi = m;
+  else hi = prefix_end(keys, m, e, key >> (64 - 2 * d), 64 - 2 * d);
+  Cell c;
+  c.child[0] = c.child[1] = c.child[2] = c.child[3] = NONE;
+  c.first = s;
+  c.count = hi - s;
+  c.x0 = x0; c.y0 = y0; c.x1 = x1; c.y1 = y1;
+  c.x = c.y = c.value = c.r = 0;
+  if (internal) {
+    const int dsh = 62 - 2 * d;
+    uint32_t bnd[5];
+    bnd[0] = s;
+    bnd[4] = hi;
+    for (uint32_t q = 1; q < 4; ++q) bnd[q] = digit_bound(keys, bnd[q - 1], hi, dsh, q);
+    for (int q = 0; q < 4; ++q) {
+      if (bnd[q] == bnd[q + 1]) continue;
+      c.child[q] = bnd[q] == s ? idx + 1 : base[bnd[q]];
     }
   }
+  cells[idx] = c;
+  cdepth[idx] = (uint32_t)d;
 }
 
 }  // namespace
@@ -722,6 +768,9 @@ hipError_t tree_reserve(TreeBuffers& t, uint32_t n) {
     std::memset(t.meta_host, 0, sizeof(TreeMeta));
     HIPCHECK(hipMalloc(&t.box_partials, BOX_BLOCKS * sizeof(BoxPartial)));
     HIPCHECK(hipMalloc(&t.deep_stack, DEEP_STACK * sizeof(Frame)));
+    HIPCHECK(hipMalloc(&t.dbins, (MAX_DEPTH + 64) * sizeof(uint32_t)));
+    HIPCHECK(hipMalloc(&t.rs_glob, 8 * 256 * sizeof(uint32_t)));
+    HIPCHECK(hipMalloc(&t.dstart, (MAX_DEPTH + 64) * sizeof(uint32_t)));
   }
   if (n <= t.cap) return hipSuccess;
   uint32_t cap = n < 1024 ? 1024 : n;
@@ -751,8 +800,10 @@ hipError_t tree_reserve_cells(TreeBuffers& t, uint32_t cells) {
   uint32_t cap = cells + cells / 4;
   HIPCHECK(grow(t.cells, cap));
   HIPCHECK(grow(t.tcells, cap));
-  HIPCHECK(grow(t.parent, cap));
-  HIPCHECK(grow(t.arrive, cap));
+  HIPCHECK(grow(t.cdepth, cap));
+  HIPCHECK(grow(t.cell_s, cap));
+  HIPCHECK(grow(t.cell_d, cap));
+  HIPCHECK(grow(t.dorder, cap));
   t.cell_cap = cap;
   return hipSuccess;
 }
@@ -760,7 +811,8 @@ hipError_t tree_reserve_cells(TreeBuffers& t, uint32_t cells) {
 void tree_free(TreeBuffers& t) {
   void* ptrs[] = {t.keys, t.keys_tmp, t.keys_all, t.perm, t.perm_tmp, t.rest, t.flags, t.offs, t.count,
                   t.base, t.dleft, t.gend, t.leafdepth, t.mixed, t.mixed_list, t.hist, t.partials,
-                  t.box_partials, t.deep_stack, t.cells, t.tcells, t.parent, t.arrive, t.meta};
+                  t.box_partials, t.deep_stack, t.cells, t.tcells, t.cdepth, t.dbins, t.dstart, t.dorder,
+                  t.rs_glob, t.cell_s, t.cell_d, t.meta};
   for (void* p : ptrs) if (p) (void)hipFree(p);
   if (t.meta_host) (void)hipHostFree(t.meta_host);
   t = TreeBuffers();
@@ -787,9 +839,12 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
   const uint32_t ntiles = ceil_div(n > 0 ? n : 1, RS_TILE);
   uint64_t *ka = t.keys, *kb = t.keys_tmp;
   uint32_t *va = t.perm, *vb = t.perm_tmp;
+  HIPCHECK(hipMemsetAsync(t.rs_glob, 0, 8 * 256 * sizeof(uint32_t), stream));
   for (int pass = 0; pass < 8; ++pass) {
-    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, mptr, t.hist, ntiles, 8 * pass);
-    hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(SCAN_THREADS), 0, stream, t.hist, 256 * ntiles);
+    uint32_t* glob = t.rs_glob + 256 * pass;
+    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, mptr, t.hist, glob, ntiles,
+                       8 * pass);
+    hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(RS_THREADS), 0, stream, t.hist, glob, ntiles);
     hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, va, kb, vb, t.hist, mptr,
                        ntiles, 8 * pass);
     std::swap(ka, kb);
@@ -802,7 +857,7 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
   hipLaunchKernelGGL(k_group_b, dim3(nb), dim3(EW_THREADS), 0, stream, t.keys, t.flags, t.mixed, t.meta, t.count,
                      t.dleft, t.gend, t.leafdepth, t.mixed_list);
   hipLaunchKernelGGL(k_deep<false>, dim3(1), dim3(64), 0, stream, t.keys, t.perm, t.perm_tmp, x, y, t.meta,
-                     t.mixed_list, t.dleft, t.gend, t.count, t.base, t.deep_stack, t.cells, t.parent);
+                     t.mixed_list, t.dleft, t.gend, t.count, t.base, t.deep_stack, t.cells, t.cdepth);
   HIPCHECK(scan_exclusive(t.count, t.base, n, mptr, t.partials, &t.meta->len, nullptr, stream));
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipMemcpyAsync(t.meta_host, t.meta, sizeof(TreeMeta), hipMemcpyDeviceToHost, stream));
@@ -813,12 +868,15 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
     return hipSuccess;
   }
   HIPCHECK(tree_reserve_cells(t, t.meta_host->len));
-  HIPCHECK(hipMemsetAsync(t.parent, 0xff, (size_t)t.meta_host->len * sizeof(uint32_t), stream));
-  hipLaunchKernelGGL(k_emit, dim3(nb), dim3(EW_THREADS), 0, stream, t.keys, t.flags, t.mixed, t.meta, t.dleft,
-                     t.gend, t.leafdepth, t.base, t.cells, t.parent);
+  const uint32_t len = t.meta_host->len;
+  HIPCHECK(hipMemsetAsync(t.cell_s, 0xff, (size_t)len * sizeof(uint32_t), stream));
+  hipLaunchKernelGGL(k_assign, dim3(nb), dim3(EW_THREADS), 0, stream, t.flags, t.mixed, t.meta, t.dleft,
+                     t.leafdepth, t.base, t.cell_s, t.cell_d);
+  hipLaunchKernelGGL(k_emit, dim3(ceil_div(len, EW_THREADS)), dim3(EW_THREADS), 0, stream, t.keys, t.mixed, t.meta,
+                     t.gend, t.leafdepth, t.base, t.cell_s, t.cell_d, t.cells, t.cdepth);
   if (t.meta_host->mixed) {
     hipLaunchKernelGGL(k_deep<true>, dim3(1), dim3(64), 0, stream, t.keys, t.perm, t.perm_tmp, x, y, t.meta,
-                       t.mixed_list, t.dleft, t.gend, t.count, t.base, t.deep_stack, t.cells, t.parent);
+                       t.mixed_list, t.dleft, t.gend, t.count, t.base, t.deep_stack, t.cells, t.cdepth);
   }
   return hipGetLastError();
 }

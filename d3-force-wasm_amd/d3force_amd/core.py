@@ -58,6 +58,22 @@ def check(status):
 
 _RANDOM_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p)
 
+PHASES = ("tree", "accumulate", "traversal", "link", "center", "integrate")
+
+
+class Stats(ctypes.Structure):
+    """d3f_stats of include/d3fgpu.h: per-phase kernel time and launches since
+    the last call, and the Barnes-Hut traversal's work counters."""
+    _fields_ = [("ms", ctypes.c_double * 6), ("launches", ctypes.c_uint64 * 6),
+                ("visits", ctypes.c_uint64), ("accepts", ctypes.c_uint64), ("pairs", ctypes.c_uint64),
+                ("flagged", ctypes.c_uint64), ("link_levels", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {"ms": {p: self.ms[k] for k, p in enumerate(PHASES)},
+                "launches": {p: int(self.launches[k]) for k, p in enumerate(PHASES)},
+                "visits": int(self.visits), "accepts": int(self.accepts), "pairs": int(self.pairs),
+                "flagged": int(self.flagged), "link_levels": int(self.link_levels)}
+
 _SIGNATURES = {
     "core_set_random": (None, [_RANDOM_FN, ctypes.c_void_p]),
     "sim_create": (ctypes.c_uint32, []),
@@ -105,6 +121,16 @@ class Core:
         self._callback = _RANDOM_FN(self._draw)
         self.core_set_random(self._callback, None)
         self.contexts = weakref.WeakKeyDictionary()
+        if hasattr(self.lib, prefix + "sim_stats"):
+            self.sim_stats = self.bind("sim_stats", ctypes.c_int32, [ctypes.c_uint32, ctypes.POINTER(Stats)])
+            self.core_set_timing = self.bind("core_set_timing", None, [ctypes.c_int])
+            self.core_synchronize = self.bind("core_synchronize", ctypes.c_int32, [])
+
+    def stats(self, h):
+        """Per-phase kernel times and work counters since the last call."""
+        st = Stats()
+        check(self.sim_stats(h, ctypes.byref(st)))
+        return st.as_dict()
 
     def _draw(self, _ctx):
         return float(self.random())

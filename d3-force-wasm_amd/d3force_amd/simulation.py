@@ -78,31 +78,45 @@ class Simulation:
     def tick(self, iterations=1):
         if not (0 < iterations):
             return self
+        ctx = self.begin()
+        try:
+            self.iterate(iterations, ctx)
+        finally:
+            self.end(ctx)
+        return self
+
+    def begin(self):
+        """The start of d3's tick: node arrays copied into the core once."""
         nodes = self._nodes
         ctx = _core.context(nodes)
         ctx.precision = self._fast
         ctx.copy_in(nodes)
         ctx.live = True
-        try:
-            for _ in range(int(iterations)):
-                self._alpha += (self._alpha_target - self._alpha) * self._alpha_decay
-                for force in list(self._forces.values()):
-                    slot = getattr(force, "slot", None)
-                    if slot is not None and slot.ctx is ctx:
-                        force(self._alpha)
-                        continue
-                    ctx.live = False
-                    ctx.copy_out(nodes)
-                    try:
-                        force(self._alpha)
-                    finally:
-                        ctx.copy_in(nodes)
-                        ctx.live = True
-                ctx.integrate(self._velocity_decay)
-        finally:
-            ctx.live = False
-            ctx.copy_out(nodes)
-        return self
+        return ctx
+
+    def iterate(self, iterations, ctx):
+        """``iterations`` ticks on the core's resident arrays (simulation.js:63-82)."""
+        nodes = self._nodes
+        for _ in range(int(iterations)):
+            self._alpha += (self._alpha_target - self._alpha) * self._alpha_decay
+            for force in list(self._forces.values()):
+                slot = getattr(force, "slot", None)
+                if slot is not None and slot.ctx is ctx:
+                    force(self._alpha)
+                    continue
+                ctx.live = False
+                ctx.copy_out(nodes)
+                try:
+                    force(self._alpha)
+                finally:
+                    ctx.copy_in(nodes)
+                    ctx.live = True
+            ctx.integrate(self._velocity_decay)
+
+    def end(self, ctx):
+        """The end of d3's tick: node arrays copied out once."""
+        ctx.live = False
+        ctx.copy_out(self._nodes)
 
     def nodes(self, nodes=None):
         if nodes is None:

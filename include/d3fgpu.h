@@ -97,16 +97,25 @@ double force_get_scalar(uint32_t h, uint32_t fh, uint32_t which);
  * (root.zig:246-248). */
 int32_t force_run(uint32_t h, uint32_t fh, double alpha);
 
-/* Library introspection (no reference counterpart): per-kernel timing of
- * the last force_run of each kind, for the benchmark's roofline. */
+/* Library introspection (no reference counterpart), for the benchmark's
+ * roofline: with timing on, every phase launch is bracketed by HIP events on
+ * the simulation's stream (no extra synchronisation); sim_stats waits for the
+ * stream, adds the elapsed times up per phase and resets them. The traversal
+ * counts its algorithmic work always: non-empty cell visits, accepted
+ * far-field cells and leaf pair interactions, summed over nodes. */
+enum { D3F_PHASE_TREE = 0, D3F_PHASE_ACCUMULATE = 1, D3F_PHASE_TRAVERSAL = 2, D3F_PHASE_LINK = 3,
+       D3F_PHASE_CENTER = 4, D3F_PHASE_INTEGRATE = 5, D3F_PHASES = 6 };
 typedef struct {
-  double tree_ms, accumulate_ms, traversal_ms, link_ms, center_ms, integrate_ms;
-  uint64_t flagged;   /* many-body nodes replayed for jiggle in the last run */
-  uint64_t link_levels;
+  double ms[D3F_PHASES];        /* summed kernel time per phase */
+  uint64_t launches[D3F_PHASES];
+  uint64_t visits, accepts, pairs;  /* Barnes-Hut work since the last call */
+  uint64_t flagged;             /* many-body nodes replayed for jiggle */
+  uint64_t link_levels;         /* levels of the last exact link schedule */
 } d3f_stats;
 int32_t sim_stats(uint32_t h, d3f_stats* out);
-/* Enables HIP-event timing of phases (off by default: events cost syncs). */
 void core_set_timing(int on);
+/* Waits for all work the library queued on the device (benchmark brackets). */
+int32_t core_synchronize(void);
 
 #ifdef __cplusplus
 }
