@@ -61,12 +61,12 @@ def device_sync(core):
 
 def read_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "traffic.md")
+    path = os.path.join(ROOT, "profiles", "exact_1m_tick_hbm_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
         parts = [p.strip() for p in line.strip().strip("|").split("|")]
-        if len(parts) >= 2 and parts[0] == kernel:
+        if len(parts) >= 2 and parts[0].split("<")[0] == kernel:
             try:
                 return float(parts[-1])
             except ValueError:
@@ -128,40 +128,47 @@ def main():
     ticks_per_s = args.steps / dt
     log("%.3f ms/tick, phases %s" % (1e3 * dt / args.steps, st["ms"]))
 
-    # Dominant kernel and its roofline.
+    # Per-kernel rooflines from the live event timing; the dominant kernel is
+    # the one with the most device time per tick. Algorithmic work per tick
+    # (C = cells, measured 1.72 per node on this workload):
+    #   traversal  8 V + 8 A + 6 P FP64 flops (V visits, A accepted cells,
+    #              P leaf pairs, counted by the kernel itself)
+    #   center_sum 16 B/node read; 2n dependent FP64 adds in d3's order
+    #   tree       28 B/node + 88 B/cell;  accumulate 88 B/cell + 24 B/node
+    #   link       128 B/link;  integrate 81 B/node
     phase_ms = st["ms"]
-    dominant = max(phase_ms, key=phase_ms.get)
-    launches = max(st["launches"][dominant], 1)
-    per_launch_s = phase_ms[dominant] / 1e3 / launches
-    C = 1.72 * n  # cells per node measured on this workload
-    if dominant == "traversal":
-        flops = 8 * st["visits"] + 8 * st["accepts"] + 6 * st["pairs"]
-        per_launch = flops / launches
-        achieved = per_launch / per_launch_s / 1e12
-        roof = {"kernel": "k_traverse<RECORD> (Barnes-Hut traversal)", "bound": "mfma",
-                "pipe": "FP64 VALU; MI355X's FP64 vector peak equals its FP64 matrix peak",
-                "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS,
-                "work_per_launch": {"flops": per_launch, "visits": st["visits"] / launches,
-                                    "accepts": st["accepts"] / launches, "pairs": st["pairs"] / launches,
-                                    "formula": "8 V + 8 A + 6 P (V + 2A + P of them divides)"},
-                "traffic": read_traffic("k_traverse")}
-    elif dominant == "tree":
-        b = 28 * n + 88 * C
-        achieved = b / per_launch_s / 1e9
-        roof = {"kernel": "quadtree build", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
-    elif dominant == "link":
-        b = 128 * m
-        achieved = b / per_launch_s / 1e9
-        roof = {"kernel": "k_link_level (all levels)", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
-    else:
-        b = 48 * n if dominant == "center" else 81 * n
-        achieved = b / per_launch_s / 1e9
-        roof = {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
-    roof["launch_ms"] = per_launch_s * 1e3
+    C = 1.72 * n
+    hbm_bytes = {"tree": 28 * n + 88 * C, "accumulate": 88 * C + 24 * n, "link": 128 * m,
+                 "integrate": 81 * n, "center_sum": 16 * n}
+    names = {"traversal": "k_traverse<RECORD> (Barnes-Hut traversal)", "center_sum": "k_center_raw_sum (serial sums)",
+             "tree": "quadtree build (11 kernels)", "accumulate": "k_accumulate_level (all levels)",
+             "link": "k_link_level (all levels)", "integrate": "k_integrate"}
+    kernels = {}
+    for ph, ms in phase_ms.items():
+        if ph == "center" or ms <= 0:
+            continue
+        launches = max(st["launches"][ph], 1)
+        t = ms / 1e3 / launches
+        if ph == "traversal":
+            flops = (8 * st["visits"] + 8 * st["accepts"] + 6 * st["pairs"]) / launches
+            a = flops / t / 1e12
+            k = {"kernel": names[ph], "bound": "mfma",
+                 "pipe": "FP64 VALU (MI355X FP64 vector peak = FP64 matrix peak)",
+                 "achieved": a, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": a / FP64_PEAK_TFLOPS,
+                 "work_per_launch": {"flops": flops, "visits": st["visits"] / launches,
+                                     "accepts": st["accepts"] / launches, "pairs": st["pairs"] / launches}}
+        else:
+            a = hbm_bytes[ph] / t / 1e9
+            k = {"kernel": names[ph], "bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": a / HBM_PEAK_GBS, "bytes_per_launch": hbm_bytes[ph]}
+            if ph == "center_sum":
+                k["note"] = ("serial dependency chain: n FP64 adds per axis in index order, both axes in one "
+                             "instruction stream; %.2f ns per dependent add" % (t * 1e9 / n))
+        k["launch_ms"] = t * 1e3
+        k["traffic"] = read_traffic(names[ph].split(" ")[0].split("<")[0])
+        kernels[ph] = k
+    dominant = max(kernels, key=lambda ph: kernels[ph]["launch_ms"] * max(st["launches"][ph], 1))
+    roof = dict(kernels[dominant])
 
     cpu = None
     if not args.no_cpu:
@@ -189,6 +196,7 @@ def main():
                    "ticks": "%d-%d" % (args.warmup + 1, args.warmup + args.steps)},
         "phases_ms_per_tick": {k: v / args.steps for k, v in phase_ms.items()},
         "roofline": roof,
+        "rooflines": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -179,6 +179,7 @@ struct Sim {
   d3f_stats stats{};
   unsigned long long* dcounters = nullptr;  // traversal visits, accepts, pairs
   struct Timer { int phase; hipEvent_t a, b; };
+  hipEvent_t side_open = nullptr;
   std::vector<Timer> pending;
   std::vector<hipEvent_t> spare;
   hipEvent_t open_event = nullptr;
@@ -277,7 +278,16 @@ struct Sim {
     if (centers == 0 || n == 0) return hipSuccess;
     HIPCHECK(hipEventRecord(ev_x_ready, stream));
     HIPCHECK(hipStreamWaitEvent(side, ev_x_ready, 0));
+    hipEvent_t a = g_timing ? take_event() : nullptr;
+    if (a) HIPCHECK(hipEventRecord(a, side));
     HIPCHECK(center_raw_sum(dx.p, dy.p, n, dsums_side, side));
+    if (a) {
+      hipEvent_t b = take_event();
+      if (b) {
+        HIPCHECK(hipEventRecord(b, side));
+        pending.push_back(Timer{D3F_PHASE_CENTER_SUM, a, b});
+      }
+    }
     HIPCHECK(hipEventRecord(ev_sums_done, side));
     side_valid = true;
     side_busy = true;
@@ -311,6 +321,7 @@ struct Sim {
   }
   hipError_t collect() {
     HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipStreamSynchronize(side));
     for (auto& t : pending) {
       float ms = 0;
       HIPCHECK(hipEventElapsedTime(&ms, t.a, t.b));

@@ -11,20 +11,54 @@ namespace {
 constexpr int EW = 256;
 
 // d3's center sums: sx = x0 + x1 + ... + x(n-1) in index order (and sy),
-// simple.zig:18-22. Waves 1..15 stream x and y into a double-buffered LDS
-// chunk while lane 0 (x) and lane 1 (y) of wave 0 add the previous chunk in
-// order, with register double-buffering so LDS latency hides behind the
-// dependent adds.
-constexpr int CS_THREADS = 1024;
+// simple.zig:18-22. The chain of dependent FP64 adds is the whole cost, so
+// each axis gets its own SIMD: of the 8 waves (wave w issues on SIMD w % 4)
+// wave 0 adds x and wave 1 adds y, one lane each, from a double-buffered LDS
+// chunk that waves 2, 3, 6 and 7 (SIMDs 2 and 3) fill with the next chunk
+// meanwhile; waves 4 and 5 only keep the barrier. Each lane loads 32 values
+// at once, then adds them in order.
+constexpr int CS_THREADS = 512;
 constexpr int CS_CHUNK = 2048;
+constexpr int CS_PRODUCERS = 256;  // threads of waves 2, 3, 6, 7
+constexpr int CS_BLOCK = 32;
+
+__device__ inline bool cs_producer(int wave) { return (wave & 3) >= 2; }
+__device__ inline int cs_slot(int wave, int lane) { return ((wave >> 2) * 2 + ((wave & 3) - 2)) * 64 + lane; }
 
 __device__ inline void cs_load(const double* __restrict__ xs, const double* __restrict__ ys, uint32_t n,
-                               uint32_t base, double (*buf)[CS_CHUNK], int tid, int nthreads) {
+                               uint32_t base, double (*buf)[CS_CHUNK], int tid) {
   const uint32_t cnt = n - base < (uint32_t)CS_CHUNK ? n - base : (uint32_t)CS_CHUNK;
-  for (uint32_t k = tid; k < cnt; k += nthreads) {
-    buf[0][k] = xs[base + k];
-    buf[1][k] = ys[base + k];
+  constexpr int U = CS_CHUNK / CS_PRODUCERS;
+  double vx[U], vy[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint32_t k = tid + u * CS_PRODUCERS;
+    vx[u] = k < cnt ? xs[base + k] : 0;
+    vy[u] = k < cnt ? ys[base + k] : 0;
   }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint32_t k = tid + u * CS_PRODUCERS;
+    if (k < cnt) {
+      buf[0][k] = vx[u];
+      buf[1][k] = vy[u];
+    }
+  }
+}
+
+__device__ inline double cs_add(const double* p, uint32_t cnt, double s) {
+  if (cnt == (uint32_t)CS_CHUNK) {
+    for (uint32_t k = 0; k < (uint32_t)CS_CHUNK; k += CS_BLOCK) {
+      double a[CS_BLOCK];
+#pragma unroll
+      for (int j = 0; j < CS_BLOCK; ++j) a[j] = p[k + j];
+#pragma unroll
+      for (int j = 0; j < CS_BLOCK; ++j) s += a[j];
+    }
+  } else {
+    for (uint32_t k = 0; k < cnt; ++k) s += p[k];
+  }
+  return s;
 }
 
 __global__ __launch_bounds__(CS_THREADS) void k_center_raw_sum(const double* __restrict__ xs,
@@ -34,44 +68,19 @@ __global__ __launch_bounds__(CS_THREADS) void k_center_raw_sum(const double* __r
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t chunks = ceil_div(n, CS_CHUNK);
   double s = 0;
-  if (chunks) cs_load(xs, ys, n, 0, buf[0], threadIdx.x, CS_THREADS);
+  if (chunks && cs_producer(wave)) cs_load(xs, ys, n, 0, buf[0], cs_slot(wave, lane));
   __syncthreads();
   for (uint32_t c = 0; c < chunks; ++c) {
     const uint32_t base = c * CS_CHUNK;
-    if (wave != 0) {
-      if (c + 1 < chunks) cs_load(xs, ys, n, base + CS_CHUNK, buf[(c + 1) & 1], threadIdx.x - 64, CS_THREADS - 64);
-    } else if (lane < 2) {
-      const double* p = buf[c & 1][lane];
-      const uint32_t cnt = n - base < (uint32_t)CS_CHUNK ? n - base : (uint32_t)CS_CHUNK;
-      if (cnt == (uint32_t)CS_CHUNK) {
-        double a[16], b[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) a[j] = p[j];
-        for (uint32_t k = 0; k < (uint32_t)CS_CHUNK; k += 32) {
-#pragma unroll
-          for (int j = 0; j < 16; ++j) b[j] = p[k + 16 + j];
-#pragma unroll
-          for (int j = 0; j < 16; ++j) s += a[j];
-          if (k + 32 < (uint32_t)CS_CHUNK) {
-#pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = p[k + 32 + j];
-          }
-#pragma unroll
-          for (int j = 0; j < 16; ++j) s += b[j];
-        }
-      } else {
-        for (uint32_t k = 0; k < cnt; ++k) s += p[k];
-      }
+    const uint32_t cnt = n - base < (uint32_t)CS_CHUNK ? n - base : (uint32_t)CS_CHUNK;
+    if (cs_producer(wave)) {
+      if (c + 1 < chunks) cs_load(xs, ys, n, base + CS_CHUNK, buf[(c + 1) & 1], cs_slot(wave, lane));
+    } else if (wave < 2 && lane == 0) {
+      s = cs_add(buf[c & 1][wave], cnt, s);
     }
     __syncthreads();
   }
-  __shared__ double sy;
-  if (threadIdx.x == 1) sy = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    out[0] = s;
-    out[1] = sy;
-  }
+  if (wave < 2 && lane == 0) out[wave] = s;
 }
 
 // x -= (sx / n - cx) * strength, y likewise (simple.zig:23-30); every thread

@@ -58,13 +58,13 @@ def check(status):
 
 _RANDOM_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p)
 
-PHASES = ("tree", "accumulate", "traversal", "link", "center", "integrate")
+PHASES = ("tree", "accumulate", "traversal", "link", "center", "integrate", "center_sum")
 
 
 class Stats(ctypes.Structure):
     """d3f_stats of include/d3fgpu.h: per-phase kernel time and launches since
     the last call, and the Barnes-Hut traversal's work counters."""
-    _fields_ = [("ms", ctypes.c_double * 6), ("launches", ctypes.c_uint64 * 6),
+    _fields_ = [("ms", ctypes.c_double * 7), ("launches", ctypes.c_uint64 * 7),
                 ("visits", ctypes.c_uint64), ("accepts", ctypes.c_uint64), ("pairs", ctypes.c_uint64),
                 ("flagged", ctypes.c_uint64), ("link_levels", ctypes.c_uint64)]
 

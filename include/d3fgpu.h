@@ -104,7 +104,10 @@ int32_t force_run(uint32_t h, uint32_t fh, double alpha);
  * counts its algorithmic work always: non-empty cell visits, accepted
  * far-field cells and leaf pair interactions, summed over nodes. */
 enum { D3F_PHASE_TREE = 0, D3F_PHASE_ACCUMULATE = 1, D3F_PHASE_TRAVERSAL = 2, D3F_PHASE_LINK = 3,
-       D3F_PHASE_CENTER = 4, D3F_PHASE_INTEGRATE = 5, D3F_PHASES = 6 };
+       D3F_PHASE_CENTER = 4, D3F_PHASE_INTEGRATE = 5, D3F_PHASE_CENTER_SUM = 6, D3F_PHASES = 7 };
+/* D3F_PHASE_CENTER is the center force on the tick's stream (including any
+ * wait for its sums); D3F_PHASE_CENTER_SUM is the serial sum kernel itself,
+ * which runs on a second stream overlapping the tick. */
 typedef struct {
   double ms[D3F_PHASES];        /* summed kernel time per phase */
   uint64_t launches[D3F_PHASES];
