@@ -52,6 +52,98 @@ def make_workload(core, n, m, pin_fraction, seed=1):
     return sim, nodes
 
 
+def make_fast_workload(core, n, seed=1):
+    """The fast-mode workload: phyllotaxis start, Barabasi-Albert links (m0 = 3,
+    3 per new node), forceManyBody (FMM order 8, opening 1.5), forceLink
+    (adjacency sweep), forceCollide with radius uniform in [2, 6]."""
+    import numpy as np
+    import d3force_amd as d3
+    from scenarios import ba_graph
+    t0 = time.time()
+    src, tgt = ba_graph(n, seed=seed)
+    radii = 2 + 4 * np.random.default_rng(seed).random(n)
+    nodes = d3.Nodes(n)
+    sim = (d3.forceSimulation(nodes, core=core)
+           .force("charge", d3.forceManyBody())
+           .force("link", d3.forceLink((src, tgt)))
+           .force("collide", d3.forceCollide(radii)))
+    sim.precision("fast", {"order": 8, "opening": 1.5})
+    log("fast workload %d nodes, %d links ready in %.1fs" % (n, len(src), time.time() - t0))
+    return sim, nodes, len(src)
+
+
+def fast_rms(core, nodes, sample_size=10000, seed=3):
+    """Relative RMS error of the fast many-body force on the current layout
+    against the library's FP64 direct sum, over a fixed sample of nodes: one
+    manyBody-only tick with velocityDecay 0 from v = 0 (the form of the
+    reference's test/fast/error-test.js), so v after it is the force."""
+    import ctypes
+    import numpy as np
+    import d3force_amd as d3
+    n = len(nodes)
+    probe = d3.Nodes(n, x=nodes.x.copy(), y=nodes.y.copy(), vx=np.zeros(n), vy=np.zeros(n))
+    charge = d3.forceManyBody()
+    sim = d3.forceSimulation(probe, core=core).velocity_decay(0).force("charge", charge).stop()
+    sim.precision("fast", {"order": 8, "opening": 1.5})
+    sample = np.sort(np.random.default_rng(seed).choice(n, min(n, sample_size), replace=False)).astype(np.uint32)
+    ref = np.empty(2 * len(sample))
+    ctx = sim.begin()
+    charge.push_scalars()
+    from d3force_amd.core import check
+    check(core.force_direct_sum(ctx.h, charge.slot.fh, 1.0, sample.ctypes.data_as(ctypes.c_void_p), len(sample),
+                                ref.ctypes.data_as(ctypes.c_void_p)))
+    sim.iterate(1, ctx)
+    sim.end(ctx)
+    ref *= sim.alpha()  # the sum is linear in alpha; the tick used this alpha
+    fx, fy = ref[0::2], ref[1::2]
+    ex, ey = probe.vx[sample] - fx, probe.vy[sample] - fy
+    return float(np.sqrt(np.sum(ex * ex + ey * ey) / np.sum(fx * fx + fy * fy))), len(sample)
+
+
+def run_fast(core, n, steps, warmup):
+    sim, nodes, m = make_fast_workload(core, n)
+    ctx = sim.begin()
+    log("fast: warmup %d ticks" % warmup)
+    sim.iterate(warmup, ctx)
+    device_sync(core)
+    core.stats(ctx.h)
+    core.core_set_timing(1)
+    log("fast: timing %d ticks" % steps)
+    device_sync(core)
+    t0 = time.perf_counter()
+    sim.iterate(steps, ctx)
+    device_sync(core)
+    dt = time.perf_counter() - t0
+    core.core_set_timing(0)
+    st = core.stats(ctx.h)
+    sim.end(ctx)
+    log("fast: %.3f ms/tick, phases %s" % (1e3 * dt / steps, st["ms"]))
+    err, k = fast_rms(core, nodes)
+    log("fast: rms vs FP64 direct sum %.3e on %d nodes" % (err, k))
+    p = 8
+    launches = max(st["launches"]["fmm"], 1)
+    t_fmm = st["ms"]["fmm"] / 1e3 / launches
+    flops = (8 * (p + 1) ** 2 * st["m2l"] + 11 * st["p2p_pairs"]) / launches
+    return {
+        "value": steps / dt, "unit": "ticks/s", "ms_per_step": 1e3 * dt / steps, "steps": steps, "warmup": warmup,
+        "dtype": "f64 positions and centres, f32 expansions / M2L / P2P",
+        "config": {"workload": "fast mode, %d nodes, Barabasi-Albert graph (m0 = 3, 3 links per new node, %d links), "
+                               "phyllotaxis start, manyBody FMM order 8 opening 1.5 + link sweep + grid collide "
+                               "radius U[2, 6]" % (n, m), "nodes": n, "links": m, "mode": "fast",
+                   "ticks": "%d-%d" % (warmup + 1, warmup + steps)},
+        "rms_force_error_vs_fp64_direct_sum": err, "rms_sample_nodes": k,
+        "phases_ms_per_tick": {ph: v / steps for ph, v in st["ms"].items() if v > 0},
+        "fmm_work_per_tick": {"m2l_pairs": st["m2l"] / launches, "p2p_point_pairs": st["p2p_pairs"] / launches,
+                              "cells": st["fmm_cells"]},
+        "fmm_roofline": {"kernel": "fast many-body (all FMM kernels)", "bound": "mfma",
+                         "pipe": "FP32 (MI355X FP32 matrix peak 157.3 TFLOP/s)",
+                         "achieved": flops / t_fmm / 1e12, "peak": 157.3, "unit": "TFLOP/s",
+                         "frac": flops / t_fmm / 1e12 / 157.3, "launch_ms": t_fmm * 1e3,
+                         "work_per_launch": {"flops": flops, "m2l_flops_per_pair": 8 * (p + 1) ** 2,
+                                             "p2p_flops_per_pair": 11}},
+    }
+
+
 def device_sync(core):
     """Waits for the device (the library's own hipDeviceSynchronize: the
     library links ROCm's HIP runtime directly, so torch is not loaded)."""
@@ -98,6 +190,8 @@ def main():
     ap.add_argument("--nodes", type=int, default=1000000)
     ap.add_argument("--cpu-ticks", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--fast-nodes", type=int, default=1000000, help="nodes of the fast-mode leg (0: skip it)")
+    ap.add_argument("--fast-only", action="store_true", help="run only the fast-mode leg (prints its object)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the tick on a single MI355X "
@@ -105,6 +199,9 @@ def main():
         sys.exit(2)
 
     import d3force_amd as d3
+    if args.fast_only:
+        print(json.dumps(run_fast(d3.gpu_core(), args.fast_nodes, args.steps, args.warmup)))
+        return
     n = args.nodes
     m = 2 * n
     pin = 0.01
@@ -170,6 +267,8 @@ def main():
     dominant = max(kernels, key=lambda ph: kernels[ph]["launch_ms"] * max(st["launches"][ph], 1))
     roof = dict(kernels[dominant])
 
+    fast = run_fast(core, args.fast_nodes, args.steps, args.warmup) if args.fast_nodes else None
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -198,6 +297,7 @@ def main():
         "roofline": roof,
         "rooflines": kernels,
         "cpu_baseline": cpu,
+        "fast": fast,
     }
     print(json.dumps(out))
 

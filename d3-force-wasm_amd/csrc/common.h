@@ -41,6 +41,7 @@ constexpr uint32_t TC_LEAF = 1, TC_REACH = 2;
 // Device-resident scalars of the last tree build.
 struct TreeMeta {
   double extent[4];
+  double box[4];    // bounding box of the non-NaN points (x0, y0, x1, y1)
   uint32_t n;       // nodes
   uint32_t m;       // points in the tree (no NaN coordinate)
   uint32_t len;     // cells
@@ -65,6 +66,17 @@ constexpr uint32_t DEEP_STACK = 3 * MAX_DEPTH + 64;
   } while (0)
 
 __host__ __device__ inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+// Fast mode's counter-based jiggle (hash.zig:10-25): the splitmix64
+// finaliser of (seed, i, j, salt) -> 53 uniform bits -> (u - 0.5) * 1e-6.
+__host__ __device__ inline double hash_jiggle(uint64_t seed, uint32_t i, uint32_t j, uint32_t salt) {
+  uint64_t h = seed ^ ((uint64_t)i << 32 | j) ^ ((uint64_t)salt * 0x9E3779B97F4A7C15ull);
+  h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+  h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+  h = h ^ (h >> 31);
+  const double u = (double)(h >> 11) * 0x1.0p-53;
+  return (u - 0.5) * 1e-6;
+}
 
 // ---- tree.hip ----
 struct TreeBuffers {
@@ -97,6 +109,20 @@ void tree_free(TreeBuffers& t);
 // meta_host holds the tree's scalars (the call synchronises once, after the
 // cell count is known, to size the cell buffers).
 hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, uint32_t n, hipStream_t stream);
+// Fast mode's keys (tree.zig:346-364): box into meta->box, 16-level Morton
+// keys of the non-NaN points sorted with their nodes into keys/perm (m =
+// meta->m), NaN nodes into rest. No synchronisation.
+hipError_t tree_prepare_fast(TreeBuffers& t, const double* x, const double* y, uint32_t n, hipStream_t stream);
+// Stable LSD radix sort of (keys, vals)[0 .. *count) on the 8-bit digits
+// [first_digit, first_digit + digits), digits even; n_max bounds *count.
+hipError_t sort_digits(uint32_t* hist, uint32_t* glob, uint64_t* keys, uint32_t* vals, uint64_t* keys_tmp,
+                       uint32_t* vals_tmp, uint32_t n_max, const uint32_t* count, int first_digit, int digits,
+                       hipStream_t stream);
+size_t sort_hist_words(uint32_t n_max);  // hist size sort_digits needs
+// Exclusive scan of in[0 .. *count_dev) (n_max bounds the count; partials
+// holds ceil(n_max / 4096) + 1 words); the total goes to *total_out.
+hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t n_max, const uint32_t* count_dev, uint32_t* partials,
+                    uint32_t* total_out, hipStream_t stream);
 
 // ---- many_body.hip ----
 // visitAfter charge sums (many_body.zig:101-135) into cells and tcells.
@@ -143,6 +169,103 @@ struct LinkSerialState {
 hipError_t link_serial(const uint32_t* src, const uint32_t* tgt, const double* strength, const double* distance,
                        const double* bias, uint32_t m, const double* x, const double* y, double* vx, double* vy,
                        double alpha, LinkSerialState* state, hipStream_t stream);
+
+// ---- shared scratch of the fast-mode kernels ----
+struct SortBuffers {
+  uint64_t *k0 = nullptr, *k1 = nullptr;  // keys and their ping-pong partner
+  uint32_t *v0 = nullptr, *v1 = nullptr;  // values likewise
+  uint32_t *hist = nullptr, *glob = nullptr, *count = nullptr;
+  uint32_t cap = 0;
+};
+
+// ---- fast.hip ----
+struct GridParams {
+  double x0, y0, inv_h;
+  uint32_t gx, gy, cells, m, sort_n, active;
+};
+struct GridBuffers {
+  double *px = nullptr, *py = nullptr;  // x + vx, y + vy
+  uint32_t *cell_of = nullptr, *start = nullptr;
+  void* partials = nullptr;
+  GridParams* params = nullptr;
+};
+size_t grid_partials_bytes();
+// Adjacency rows of the links (link_fast.zig:105-130): row[n + 1], adj[2m],
+// and the nodes with long rows, heavy[*heavy_count] (heavy_count is a host
+// pointer; the call synchronises).
+hipError_t link_csr(const uint32_t* src, const uint32_t* tgt, uint32_t m, uint32_t n, uint32_t* row, uint32_t* adj,
+                    uint32_t* heavy, uint32_t* heavy_count, const SortBuffers& sb, hipStream_t stream);
+// One Jacobi sweep over the rows on px = x + vx (link_fast.zig:60-99).
+hipError_t link_sweep(uint32_t n, const uint32_t* row, const uint32_t* adj, const uint32_t* heavy, uint32_t heavy_count,
+                      const uint32_t* src, const uint32_t* tgt, const double* strength, const double* distance,
+                      const double* bias, const double* px, const double* py, double* vx, double* vy, double alpha,
+                      uint64_t seed, hipStream_t stream);
+// One iteration of the uniform-grid collide (grid.zig:221-285).
+hipError_t grid_collide_iteration(const double* x, const double* y, double* vx, double* vy, const double* radii,
+                                  uint32_t n, double strength, uint64_t seed, const GridBuffers& g,
+                                  const SortBuffers& sb, hipStream_t stream);
+
+// ---- fmm.hip ----
+constexpr uint32_t FMM_LEAF = 32, FMM_LEVELS = 16;  // fmm.zig:43-47
+constexpr uint32_t FMM_MAX_ORDER = 24;
+constexpr uint32_t FMM_MAX_STEPS = 2 * FMM_LEVELS + 2;  // dual-tree rounds
+
+// A cell of the fast tree (fmm.zig:68-78): level order, children
+// contiguous; centre and radius in the unit frame u = (x - ox) / w.
+struct FCell {
+  double cx, cy, r;
+  uint32_t first, count, child0, nchild, depth;
+};
+static_assert(sizeof(FCell) == 48, "fast cell layout");
+
+// Device-resident scalars of one fast many-body run.
+struct FmmParams {
+  double extent[4];
+  double ox, oy, w, inv_w;
+  uint32_t m, ok;                          // ok: w finite and positive
+  uint32_t cell_overflow;
+  uint32_t level_n[FMM_LEVELS + 2];        // cells per depth
+  uint32_t level_off[FMM_LEVELS + 3];      // first cell of each depth
+  uint32_t front[FMM_MAX_STEPS + 2];       // dual-tree frontier sizes
+  uint32_t m2l, p2p;                       // list lengths
+  uint32_t list_overflow;
+  unsigned long long pairs;                // point pairs of the P2P list
+};
+
+struct FmmBuffers {
+  uint32_t cell_cap = 0, point_cap = 0, front_cap = 0, m2l_cap = 0, p2p_cap = 0, cellarr_cap = 0;
+  size_t exp_cap = 0;                      // complex coefficients mexp and lexp hold
+  FCell* cells = nullptr;
+  double* bbox = nullptr;                  // 4 per cell
+  uint32_t *split = nullptr, *split_off = nullptr;  // per level: child counts, offsets
+  uint32_t* bounds = nullptr;              // 5 per cell of a level
+  float *mexp = nullptr, *lexp = nullptr;  // 2(p+1) per cell
+  double *px = nullptr, *py = nullptr, *ux = nullptr, *uy = nullptr;
+  float* ps = nullptr;
+  uint2 *front_a = nullptr, *front_b = nullptr;
+  uint64_t *m2l = nullptr, *m2l_tmp = nullptr, *p2p = nullptr, *p2p_tmp = nullptr;
+  uint32_t *seg_vals = nullptr, *seg_vals_tmp = nullptr;
+  uint32_t *m2l_lo = nullptr, *m2l_hi = nullptr, *p2p_lo = nullptr, *p2p_hi = nullptr;
+  uint32_t* hist = nullptr;
+  FmmParams* params = nullptr;       // device
+  FmmParams* params_host = nullptr;  // pinned
+};
+void fmm_free(FmmBuffers& f);
+struct FmmArgs {
+  const double *x, *y, *strength;
+  double *vx, *vy;
+  uint32_t n, order;
+  double alpha, opening, dmin2, dmax2;
+  uint64_t seed;
+};
+// Fast many-body (fmm.zig:247-354). Returns hipErrorNotSupported when the
+// FMM declines (a non-finite extent), and the caller then runs the exact
+// kernel, as the reference does.
+hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t stream);
+// out[2q], out[2q + 1] = FP64 direct-sum many-body velocity increment of node
+// sample[q] (device pointers).
+hipError_t direct_sum(const double* x, const double* y, const double* strength, uint32_t n, const uint32_t* sample,
+                      uint32_t k, double alpha, double dmin2, double dmax2, double* out, hipStream_t stream);
 
 // ---- simple.hip ----
 // sums[0..1] = d3's serial sums of x and y (one workgroup).

@@ -1,0 +1,413 @@
+// Fast-mode link and collide on gfx950 (src/core/link_fast.zig,
+// src/core/grid.zig), in FP64 and bit-identical to the reference: both are
+// Jacobi sweeps in which every node sums its own terms in a fixed order and
+// writes only its own velocity, so one thread per node reproduces them.
+// Nothing here synchronises with the host: the grid's shape is computed on
+// the device and read by the kernels after it.
+#include "common.h"
+
+namespace d3f {
+namespace {
+
+constexpr int EW = 256;
+constexpr uint32_t TARGET_BIT = 0x80000000u;  // link_fast.zig:20
+constexpr uint32_t NO_CELL = 0xffffffffu;     // grid.zig:58
+constexpr uint32_t HEAVY_ROW = 64;            // rows longer than this get a wave
+
+// ---------------------------------------------------------------- link sweep
+
+// The serial CSR fill (link_fast.zig:105-130) walks the edges in order and
+// fills each row from its end, so a row lists its entries by descending
+// edge, and a self-link's target entry precedes its source entry. Position
+// q of the sort input holds entry q of that order; a stable sort by node
+// then yields the rows.
+__global__ __launch_bounds__(EW) void k_csr_entries(const uint32_t* __restrict__ src,
+                                                    const uint32_t* __restrict__ tgt, uint32_t m,
+                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t q = blockIdx.x * EW + threadIdx.x;
+  if (q >= 2 * m) return;
+  const uint32_t e = m - 1 - q / 2;
+  if (q & 1) {
+    keys[q] = src[e];
+    vals[q] = e;
+  } else {
+    keys[q] = tgt[e];
+    vals[q] = e | TARGET_BIT;
+  }
+}
+
+// row[i] = first sorted position whose node is >= i; row[n] = 2m.
+__global__ __launch_bounds__(EW) void k_csr_rows(const uint64_t* __restrict__ keys, uint32_t total, uint32_t n,
+                                                 uint32_t* __restrict__ row) {
+  const uint32_t i = blockIdx.x * EW + threadIdx.x;
+  if (i > n) return;
+  uint32_t lo = 0, hi = total;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (keys[mid] < i) lo = mid + 1; else hi = mid;
+  }
+  row[i] = i == n ? total : lo;
+}
+
+// One entry of node i's row (link_fast.zig:70-97) as the signed term its
+// velocity adds: vx -= x * b on the target side is vx += -(x * b), which
+// rounds identically, so a row's terms can be formed in parallel and added
+// in row order.
+__device__ inline void link_term(uint32_t entry, const uint32_t* __restrict__ src, const uint32_t* __restrict__ tgt,
+                                 const double* __restrict__ strength, const double* __restrict__ distance,
+                                 const double* __restrict__ bias, const double* __restrict__ px,
+                                 const double* __restrict__ py, double pxi, double pyi, double alpha, uint64_t seed,
+                                 double& tx, double& ty) {
+  const uint32_t e = entry & ~TARGET_BIT;
+  const bool is_target = (entry & TARGET_BIT) != 0;
+  const uint32_t j = is_target ? src[e] : tgt[e];
+  double x = is_target ? pxi - px[j] : px[j] - pxi;
+  double y = is_target ? pyi - py[j] : py[j] - pyi;
+  if (x == 0 || x != x) x = hash_jiggle(seed, e, 0, 0);
+  if (y == 0 || y != y) y = hash_jiggle(seed, e, 0, 1);
+  double l = __builtin_sqrt(x * x + y * y);
+  l = (l - distance[e]) / l * alpha * strength[e];
+  x *= l;
+  y *= l;
+  const double b = bias[e];
+  if (is_target) {
+    tx = -(x * b);
+    ty = -(y * b);
+  } else {
+    tx = x * (1 - b);
+    ty = y * (1 - b);
+  }
+}
+
+// link_fast.zig:60-99: node i walks its row on the snapshot px = x + vx and
+// applies its own side of each spring. Rows longer than HEAVY_ROW are left
+// to k_link_sweep_heavy.
+__global__ __launch_bounds__(EW) void k_link_sweep(uint32_t n, const uint32_t* __restrict__ row,
+                                                   const uint32_t* __restrict__ adj,
+                                                   const uint32_t* __restrict__ src,
+                                                   const uint32_t* __restrict__ tgt,
+                                                   const double* __restrict__ strength,
+                                                   const double* __restrict__ distance,
+                                                   const double* __restrict__ bias, const double* __restrict__ px,
+                                                   const double* __restrict__ py, double* __restrict__ vxs,
+                                                   double* __restrict__ vys, double alpha, uint64_t seed) {
+  const uint32_t i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t a0 = row[i], a1 = row[i + 1];
+  if (a0 == a1 || a1 - a0 > HEAVY_ROW) return;
+  double vx = vxs[i], vy = vys[i];
+  const double pxi = px[i], pyi = py[i];
+  for (uint32_t a = a0; a < a1; ++a) {
+    double tx, ty;
+    link_term(adj[a], src, tgt, strength, distance, bias, px, py, pxi, pyi, alpha, seed, tx, ty);
+    vx += tx;
+    vy += ty;
+  }
+  vxs[i] = vx;
+  vys[i] = vy;
+}
+
+// A long row (a hub of the graph): the wave forms 64 terms at a time and
+// lane 0 adds them in row order.
+__global__ __launch_bounds__(64) void k_link_sweep_heavy(const uint32_t* __restrict__ heavy,
+                                                         const uint32_t* __restrict__ row,
+                                                         const uint32_t* __restrict__ adj,
+                                                         const uint32_t* __restrict__ src,
+                                                         const uint32_t* __restrict__ tgt,
+                                                         const double* __restrict__ strength,
+                                                         const double* __restrict__ distance,
+                                                         const double* __restrict__ bias,
+                                                         const double* __restrict__ px, const double* __restrict__ py,
+                                                         double* __restrict__ vxs, double* __restrict__ vys,
+                                                         double alpha, uint64_t seed) {
+  __shared__ double tx[64], ty[64];
+  const uint32_t i = heavy[blockIdx.x];
+  const int lane = threadIdx.x;
+  const uint32_t a0 = row[i], a1 = row[i + 1];
+  const double pxi = px[i], pyi = py[i];
+  double vx = vxs[i], vy = vys[i];
+  for (uint32_t base = a0; base < a1; base += 64) {
+    const uint32_t a = base + lane;
+    double x = 0, y = 0;
+    if (a < a1) link_term(adj[a], src, tgt, strength, distance, bias, px, py, pxi, pyi, alpha, seed, x, y);
+    tx[lane] = x;
+    ty[lane] = y;
+    __syncthreads();
+    if (lane == 0) {
+      const uint32_t k = a1 - base < 64 ? a1 - base : 64;
+      for (uint32_t q = 0; q < k; ++q) {
+        vx += tx[q];
+        vy += ty[q];
+      }
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    vxs[i] = vx;
+    vys[i] = vy;
+  }
+}
+
+// Nodes whose row is longer than HEAVY_ROW (any order: each is one block).
+__global__ __launch_bounds__(EW) void k_heavy_rows(const uint32_t* __restrict__ row, uint32_t n,
+                                                   uint32_t* __restrict__ heavy, uint32_t* __restrict__ count) {
+  const uint32_t i = blockIdx.x * EW + threadIdx.x;
+  if (i < n && row[i + 1] - row[i] > HEAVY_ROW) heavy[atomicAdd(count, 1u)] = i;
+}
+
+// ---------------------------------------------------------------- grid collide
+
+struct GridPartial {
+  double x0, y0, x1, y1, rmax;
+  uint32_t ix0, iy0, ix1, iy1, m, pad;
+};
+
+// The serial scan keeps the first of equal values (it replaces on a strict
+// < or >), so ties go to the lower node.
+__device__ inline void gmin(double& v, uint32_t& i, double bv, uint32_t bi) {
+  if (bv < v || (bv == v && bi < i)) { v = bv; i = bi; }
+}
+__device__ inline void gmax(double& v, uint32_t& i, double bv, uint32_t bi) {
+  if (bv > v || (bv == v && bi < i)) { v = bv; i = bi; }
+}
+__device__ inline void gcombine(GridPartial& a, const GridPartial& b) {
+  gmin(a.x0, a.ix0, b.x0, b.ix0);
+  gmin(a.y0, a.iy0, b.y0, b.iy0);
+  gmax(a.x1, a.ix1, b.x1, b.ix1);
+  gmax(a.y1, a.iy1, b.y1, b.iy1);
+  if (b.rmax > a.rmax) a.rmax = b.rmax;
+  a.m += b.m;
+}
+__device__ inline GridPartial gempty() {
+  const double inf = __builtin_huge_val();
+  return GridPartial{inf, inf, -inf, -inf, 0, NONE, NONE, NONE, NONE, 0, 0};
+}
+
+__device__ inline GridPartial block_combine(GridPartial p) {
+  __shared__ GridPartial sh[EW];
+  sh[threadIdx.x] = p;
+  __syncthreads();
+  for (int s = EW / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) gcombine(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// grid.zig:66-91: shifted positions, their bounds and the largest radius
+// over the points without a NaN coordinate.
+__global__ __launch_bounds__(EW) void k_grid_bounds(const double* __restrict__ xs, const double* __restrict__ ys,
+                                                    const double* __restrict__ vxs, const double* __restrict__ vys,
+                                                    const double* __restrict__ radii, uint32_t n,
+                                                    double* __restrict__ px, double* __restrict__ py,
+                                                    GridPartial* __restrict__ partials) {
+  GridPartial p = gempty();
+  for (uint32_t i = blockIdx.x * EW + threadIdx.x; i < n; i += gridDim.x * EW) {
+    const double x = xs[i] + vxs[i], y = ys[i] + vys[i];
+    px[i] = x;
+    py[i] = y;
+    if (x != x || y != y) continue;
+    gmin(p.x0, p.ix0, x, i);
+    gmin(p.y0, p.iy0, y, i);
+    gmax(p.x1, p.ix1, x, i);
+    gmax(p.y1, p.iy1, y, i);
+    const double rad = radii[i];
+    if (rad > p.rmax) p.rmax = rad;
+    p.m++;
+  }
+  p = block_combine(p);
+  if (threadIdx.x == 0) partials[blockIdx.x] = p;
+}
+
+// grid.zig:224-253: the grid's shape. Cells are 2 r_max wide, doubled until
+// there are at most 4n + 16 of them; `active` is 0 when the iteration does
+// nothing (fewer than two points or no positive radius).
+__global__ __launch_bounds__(EW) void k_grid_setup(const GridPartial* __restrict__ partials, uint32_t blocks,
+                                                   uint32_t n, GridParams* __restrict__ gp) {
+  GridPartial q = gempty();
+  for (uint32_t b = threadIdx.x; b < blocks; b += EW) gcombine(q, partials[b]);
+  q = block_combine(q);
+  if (threadIdx.x != 0) return;
+  GridParams r{};
+  r.m = q.m;
+  r.active = q.m >= 2 && q.rmax > 0;
+  if (r.active) {
+    double h = 2 * q.rmax;
+    uint32_t gx = (uint32_t)(floor((q.x1 - q.x0) / h) + 1);
+    uint32_t gy = (uint32_t)(floor((q.y1 - q.y0) / h) + 1);
+    while ((uint64_t)gx * gy > 4 * (uint64_t)n + 16) {
+      h *= 2;
+      gx = (uint32_t)(floor((q.x1 - q.x0) / h) + 1);
+      gy = (uint32_t)(floor((q.y1 - q.y0) / h) + 1);
+    }
+    r.x0 = q.x0;
+    r.y0 = q.y0;
+    r.inv_h = 1 / h;
+    r.gx = gx;
+    r.gy = gy;
+    r.cells = gx * gy;
+  }
+  r.sort_n = r.active ? n : 0;
+  *gp = r;
+}
+
+// grid.zig:95-111: sort input in descending node order keyed by cell
+// (`cells`, sorting last, for a NaN position).
+__global__ __launch_bounds__(EW) void k_grid_bin(const double* __restrict__ px, const double* __restrict__ py,
+                                                 uint32_t n, const GridParams* __restrict__ gp,
+                                                 uint32_t* __restrict__ cell_of, uint64_t* __restrict__ keys,
+                                                 uint32_t* __restrict__ items) {
+  const uint32_t k = blockIdx.x * EW + threadIdx.x;
+  if (k >= n || !gp->active) return;
+  const uint32_t i = n - 1 - k;
+  const uint32_t cells = gp->cells;
+  uint32_t cell = cells;
+  if (px[i] == px[i] && py[i] == py[i]) {
+    const uint32_t cx = (uint32_t)floor((px[i] - gp->x0) * gp->inv_h);
+    const uint32_t cy = (uint32_t)floor((py[i] - gp->y0) * gp->inv_h);
+    cell = cy * gp->gx + cx;
+  }
+  cell_of[i] = cell == cells ? NO_CELL : cell;
+  keys[k] = cell;
+  items[k] = i;
+}
+
+// grid.zig:115-127 and 255-258: start[c] = first sorted position whose cell
+// is >= c (m for the cells after the last binned node, and for `cells`).
+__global__ __launch_bounds__(EW) void k_grid_starts(const uint64_t* __restrict__ keys,
+                                                    const GridParams* __restrict__ gp, uint32_t* __restrict__ start) {
+  const uint32_t c = blockIdx.x * EW + threadIdx.x;
+  if (!gp->active) return;
+  const uint32_t cells = gp->cells;
+  if (c > cells) return;
+  if (c == cells) {
+    start[c] = gp->m;
+    return;
+  }
+  uint32_t lo = 0, hi = gp->m;  // keys[m..n) hold `cells`
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (keys[mid] < c) lo = mid + 1; else hi = mid;
+  }
+  start[c] = lo;
+}
+
+// grid.zig:129-195: node i scans its 3 x 3 cells (rows, then columns, then
+// each cell's nodes high index first) and adds its share of each push.
+__global__ __launch_bounds__(EW) void k_grid_sweep(uint32_t n, const GridParams* __restrict__ gp,
+                                                   const uint32_t* __restrict__ cell_of,
+                                                   const uint32_t* __restrict__ start,
+                                                   const uint32_t* __restrict__ items,
+                                                   const double* __restrict__ radii, const double* __restrict__ px,
+                                                   const double* __restrict__ py, double* __restrict__ vxs,
+                                                   double* __restrict__ vys, double strength, uint64_t seed) {
+  const uint32_t i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n || !gp->active) return;
+  const uint32_t ci = cell_of[i];
+  if (ci == NO_CELL) return;
+  const uint32_t gx = gp->gx, gy = gp->gy;
+  const double ri = radii[i];
+  const double ri2 = ri * ri;
+  const double xi = px[i], yi = py[i];
+  double vx = vxs[i], vy = vys[i];
+  const uint32_t cxi = ci % gx, cyi = ci / gx;
+  const uint32_t xa = cxi > 0 ? cxi - 1 : 0, xb = cxi + 1 < gx - 1 ? cxi + 1 : gx - 1;
+  const uint32_t ya = cyi > 0 ? cyi - 1 : 0, yb = cyi + 1 < gy - 1 ? cyi + 1 : gy - 1;
+  for (uint32_t cy = ya; cy <= yb; ++cy) {
+    // The cells xa..xb of a row are consecutive ids, so their nodes are one
+    // run of the sorted items, in the reference's cell-then-node order.
+    const uint32_t a1 = start[cy * gx + xb + 1];
+    for (uint32_t a = start[cy * gx + xa]; a < a1; ++a) {
+      const uint32_t j = items[a];
+      if (j == i) continue;
+      const double rj = radii[j];
+      double rr = ri + rj;
+      double x = xi - px[j], y = yi - py[j];
+      double l = x * x + y * y;
+      if (l < rr * rr) {
+        if (x == 0 || y == 0) {
+          const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+          const double sign = i == lo ? 1.0 : -1.0;
+          if (x == 0) {
+            x = sign * hash_jiggle(seed, lo, hi, 0);
+            l += x * x;
+          }
+          if (y == 0) {
+            y = sign * hash_jiggle(seed, lo, hi, 1);
+            l += y * y;
+          }
+        }
+        l = __builtin_sqrt(l);
+        l = (rr - l) / l * strength;
+        x *= l;
+        y *= l;
+        rr = rj * rj / (ri2 + rj * rj);
+        vx += x * rr;
+        vy += y * rr;
+      }
+    }
+  }
+  vxs[i] = vx;
+  vys[i] = vy;
+}
+
+constexpr uint32_t GRID_BLOCKS = 512;
+
+}  // namespace
+
+hipError_t link_csr(const uint32_t* src, const uint32_t* tgt, uint32_t m, uint32_t n, uint32_t* row, uint32_t* adj,
+                    uint32_t* heavy, uint32_t* heavy_count, const SortBuffers& sb, hipStream_t stream) {
+  const uint32_t total = 2 * m;
+  if (total) {
+    hipLaunchKernelGGL(k_csr_entries, dim3(ceil_div(total, EW)), dim3(EW), 0, stream, src, tgt, m, sb.k0, adj);
+    HIPCHECK(hipMemcpyAsync(sb.count, &total, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    int digits = 0;
+    for (uint64_t v = n; v > 0; v >>= 8) ++digits;
+    digits += digits & 1;
+    if (digits == 0) digits = 2;
+    HIPCHECK(sort_digits(sb.hist, sb.glob, sb.k0, adj, sb.k1, sb.v1, total, sb.count, 0, digits, stream));
+  }
+  hipLaunchKernelGGL(k_csr_rows, dim3(ceil_div(n + 1, EW)), dim3(EW), 0, stream, sb.k0, total, n, row);
+  HIPCHECK(hipMemsetAsync(sb.count, 0, sizeof(uint32_t), stream));
+  if (n) hipLaunchKernelGGL(k_heavy_rows, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, row, n, heavy, sb.count);
+  HIPCHECK(hipMemcpyAsync(heavy_count, sb.count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  HIPCHECK(hipGetLastError());
+  return hipStreamSynchronize(stream);  // `total` and *heavy_count live on the host
+}
+
+hipError_t link_sweep(uint32_t n, const uint32_t* row, const uint32_t* adj, const uint32_t* heavy, uint32_t heavy_count,
+                      const uint32_t* src, const uint32_t* tgt, const double* strength, const double* distance,
+                      const double* bias, const double* px, const double* py, double* vx, double* vy, double alpha,
+                      uint64_t seed, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_link_sweep, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, n, row, adj, src, tgt, strength,
+                     distance, bias, px, py, vx, vy, alpha, seed);
+  if (heavy_count)
+    hipLaunchKernelGGL(k_link_sweep_heavy, dim3(heavy_count), dim3(64), 0, stream, heavy, row, adj, src, tgt, strength,
+                       distance, bias, px, py, vx, vy, alpha, seed);
+  return hipGetLastError();
+}
+
+size_t grid_partials_bytes() { return GRID_BLOCKS * sizeof(GridPartial); }
+
+hipError_t grid_collide_iteration(const double* x, const double* y, double* vx, double* vy, const double* radii,
+                                  uint32_t n, double strength, uint64_t seed, const GridBuffers& g,
+                                  const SortBuffers& sb, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  uint32_t blocks = ceil_div(n, EW * 4);
+  if (blocks > GRID_BLOCKS) blocks = GRID_BLOCKS;
+  const uint32_t nb = ceil_div(n, EW);
+  hipLaunchKernelGGL(k_grid_bounds, dim3(blocks), dim3(EW), 0, stream, x, y, vx, vy, radii, n, g.px, g.py,
+                     (GridPartial*)g.partials);
+  hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(EW), 0, stream, (const GridPartial*)g.partials, blocks, n,
+                     g.params);
+  hipLaunchKernelGGL(k_grid_bin, dim3(nb), dim3(EW), 0, stream, g.px, g.py, n, g.params, g.cell_of, sb.k0, sb.v0);
+  // Cell ids fit 32 bits (at most 4n + 16 cells); the count is 0 when inactive.
+  HIPCHECK(sort_digits(sb.hist, sb.glob, sb.k0, sb.v0, sb.k1, sb.v1, n, &g.params->sort_n, 0, 4, stream));
+  hipLaunchKernelGGL(k_grid_starts, dim3(ceil_div(4 * n + 17, EW)), dim3(EW), 0, stream, sb.k0, g.params, g.start);
+  hipLaunchKernelGGL(k_grid_sweep, dim3(nb), dim3(EW), 0, stream, n, g.params, g.cell_of, g.start, sb.v0, radii,
+                     g.px, g.py, vx, vy, strength, seed);
+  return hipGetLastError();
+}
+
+}  // namespace d3f

@@ -1,0 +1,887 @@
+// Fast many-body on gfx950: the reference's adaptive FMM (src/core/fmm.zig)
+// with its tree, geometry and interaction lists reproduced exactly in FP64,
+// and the expansions, M2L and near field computed in FP32.
+//
+//  1. Keys and sort (tree.hip, tree_prepare_fast), then the level-order
+//     cells (fmm.zig:685-711) one depth at a time: each splitting cell finds
+//     its quadrant boundaries by binary search in its key range, a scan of
+//     the child counts places the children contiguously in the next level.
+//  2. Gather in Morton order (fmm.zig:668-682), geometry (fmm.zig:721-754)
+//     and P2M / M2M (fmm.zig:757-806) one depth at a time, bottom up. Bounds,
+//     centres and radii are FP64 with the reference's operations, so the
+//     opening test sees the same numbers.
+//  3. The dual-tree interaction (fmm.zig:400-453) breadth first: one round
+//     per pair depth expands the frontier of cell pairs into children pairs,
+//     M2L pairs and P2P pairs (wave-aggregated appends). The recursion's
+//     decisions depend on the pair alone, so the lists hold exactly the
+//     reference's pairs.
+//  4. Both lists are sorted by (target, source), so each target reads its
+//     sources in one fixed order (the FP32 sums do not depend on the
+//     schedule).
+//  5. M2L, one wave per target cell, one lane per source cell; L2L top down;
+//     then P2P and L2P fused, one wave per target leaf, and the sum is added
+//     to the FP64 velocity once per node.
+//
+// Expansions are scaled by their cell's radius rho (M_k / rho^k, L_l rho^l),
+// so every FP32 quantity stays near unit size at any depth. With w = 1/z0
+// (z0 = centre_A - centre_B), a = rho_B w and b = rho_A w:
+//   L^_l(A) += (-1)^l b^l w sum_k C(k+l, k) M^_k(B) a^k,
+// which is the reference's T_{k+l} = (k+l)! / z0^{k+l+1} M2L (fmm.zig:561-610)
+// with its 1/k! and (-1)^l / l! scalings (fmm.zig:629-657) folded in.
+#include <cmath>
+#include <cstring>
+#include <utility>
+#include <vector>
+
+#include "common.h"
+
+namespace d3f {
+namespace {
+
+constexpr int EWF = 256;
+constexpr uint32_t GRID_BLOCKS = 1024;  // grid-stride kernels over device counts
+constexpr double MIN_SEP2 = 1e-28;      // fmm.zig:49
+constexpr int BN = 2 * FMM_MAX_ORDER + 1;
+
+__constant__ float c_binom[BN * BN];  // C(n, k), n, k <= 2 * FMM_MAX_ORDER
+
+__device__ inline float binom(int n, int k) { return c_binom[n * BN + k]; }
+
+// Zig's @min / @max on floats as the reference uses them: the first operand
+// unless the second is strictly smaller (larger); a NaN first operand yields
+// the second.
+__device__ inline double zmin(double a, double b) { return (b < a || a != a) ? b : a; }
+__device__ inline double zmax(double a, double b) { return (b > a || a != a) ? b : a; }
+
+__device__ inline float2 cmul(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ inline float2 cfma(float2 a, float2 b, float2 c) {  // a * b + c
+  return make_float2(fmaf(a.x, b.x, fmaf(-a.y, b.y, c.x)), fmaf(a.x, b.y, fmaf(a.y, b.x, c.y)));
+}
+__device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// ---------------------------------------------------------------- build
+
+__global__ void k_fmm_setup(TreeMeta* __restrict__ meta, FmmParams* __restrict__ fp, FCell* __restrict__ cells) {
+  FmmParams p;
+  memset(&p, 0, sizeof(p));
+  const double nan = __builtin_nan(""), inf = __builtin_huge_val();
+  for (int k = 0; k < 4; ++k) p.extent[k] = nan;
+  p.m = meta->valid ? meta->m : 0;
+  if (p.m) {  // tree.zig:357-359
+    const double b0 = meta->box[0], b1 = meta->box[1], b2 = meta->box[2], b3 = meta->box[3];
+    double side = zmax(b2 - b0, b3 - b1);
+    if (!(side > 0)) side = 1;
+    p.extent[0] = b0;
+    p.extent[1] = b1;
+    p.extent[2] = b0 + side;
+    p.extent[3] = b1 + side;
+    p.w = p.extent[2] - p.extent[0];  // fmm.zig:255-257
+    p.ok = p.w > 0 && p.w < inf;
+    p.inv_w = 1 / p.w;
+    p.ox = (p.extent[0] + p.extent[2]) / 2;
+    p.oy = (p.extent[1] + p.extent[3]) / 2;
+  }
+  p.level_n[0] = (p.m && p.ok) ? 1 : 0;
+  p.level_off[0] = 0;
+  p.level_off[1] = p.level_n[0];
+  *fp = p;
+  if (p.level_n[0]) cells[0] = FCell{0, 0, 0, 0, p.m, 0, 0, 0};
+  // prepareFast leaves no exact tree behind (tree.zig:351-356).
+  for (int k = 0; k < 4; ++k) meta->extent[k] = p.extent[k];
+  meta->len = 0;
+  meta->valid = 0;
+}
+
+// Quadrant boundaries of each splitting cell of depth d (fmm.zig:694-708):
+// a cell of more than FMM_LEAF points above the last level splits into its
+// non-empty quadrants, in digit order.
+__global__ __launch_bounds__(EWF) void k_fmm_split(const uint64_t* __restrict__ keys, const FCell* __restrict__ cells,
+                                                   const FmmParams* __restrict__ fp, uint32_t d,
+                                                   uint32_t* __restrict__ split, uint32_t* __restrict__ bounds,
+                                                   uint32_t cell_cap) {
+  const uint32_t lo = fp->level_off[d], cnt = fp->level_n[d];
+  const int shift = 2 * (int)(FMM_LEVELS - 1 - d);
+  for (uint32_t t = blockIdx.x * EWF + threadIdx.x; t < cnt; t += gridDim.x * EWF) {
+    uint32_t nc = 0;
+    if (lo + t < cell_cap) {
+      const FCell c = cells[lo + t];
+      if (c.count > FMM_LEAF && d < FMM_LEVELS) {
+        uint32_t b[5];
+        b[0] = c.first;
+        b[4] = c.first + c.count;
+        for (uint32_t q = 1; q < 4; ++q) {
+          uint32_t l = b[q - 1], h = b[4];
+          while (l < h) {
+            const uint32_t mid = l + (h - l) / 2;
+            if (((uint32_t)(keys[mid] >> shift) & 3u) < q) l = mid + 1; else h = mid;
+          }
+          b[q] = l;
+        }
+        for (int q = 0; q < 5; ++q) bounds[5 * (size_t)t + q] = b[q];
+        for (int q = 0; q < 4; ++q) nc += b[q + 1] > b[q];
+      }
+    }
+    split[t] = nc;
+  }
+}
+
+__global__ __launch_bounds__(EWF) void k_fmm_children(FCell* __restrict__ cells, FmmParams* __restrict__ fp,
+                                                      uint32_t d, const uint32_t* __restrict__ split,
+                                                      const uint32_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ bounds, uint32_t cell_cap) {
+  const uint32_t lo = fp->level_off[d], cnt = fp->level_n[d];
+  const uint32_t base = fp->level_off[d + 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) fp->level_off[d + 2] = base + fp->level_n[d + 1];
+  for (uint32_t t = blockIdx.x * EWF + threadIdx.x; t < cnt; t += gridDim.x * EWF) {
+    const uint32_t nc = split[t];
+    if (nc == 0) continue;
+    const uint32_t c0 = base + off[t];
+    cells[lo + t].child0 = c0;
+    cells[lo + t].nchild = nc;
+    uint32_t k = 0;
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t b0 = bounds[5 * (size_t)t + q], b1 = bounds[5 * (size_t)t + q + 1];
+      if (b1 == b0) continue;
+      const uint32_t idx = c0 + k++;
+      if (idx < cell_cap) cells[idx] = FCell{0, 0, 0, b0, b1 - b0, 0, 0, d + 1};
+      else fp->cell_overflow = 1;
+    }
+  }
+}
+
+// fmm.zig:668-682: Morton-ordered raw positions, unit-frame positions and
+// strength * alpha (NaN -> 0).
+__global__ __launch_bounds__(EWF) void k_fmm_gather(const uint32_t* __restrict__ perm, const double* __restrict__ xs,
+                                                    const double* __restrict__ ys, const double* __restrict__ st,
+                                                    const FmmParams* __restrict__ fp, double alpha, uint32_t n,
+                                                    double* __restrict__ px, double* __restrict__ py,
+                                                    double* __restrict__ ux, double* __restrict__ uy,
+                                                    float* __restrict__ ps) {
+  const uint32_t j = blockIdx.x * EWF + threadIdx.x;
+  if (j >= n || j >= fp->m || !fp->ok) return;
+  const uint32_t i = perm[j];
+  const double x = xs[i], y = ys[i];
+  px[j] = x;
+  py[j] = y;
+  const double s = st[i] * alpha;
+  ps[j] = s != s ? 0.0f : (float)s;
+  ux[j] = (x - fp->ox) * fp->inv_w;
+  uy[j] = (y - fp->oy) * fp->inv_w;
+}
+
+// ---------------------------------------------------------------- upward
+
+// Geometry (fmm.zig:721-754) and the radius-scaled multipole of the cells
+// [lo, lo + cnt) of one depth: P2M for a leaf, M2M from the children.
+template <int Q>
+__global__ __launch_bounds__(EWF) void k_fmm_up(FCell* __restrict__ cells, double* __restrict__ bbox,
+                                                const double* __restrict__ ux, const double* __restrict__ uy,
+                                                const float* __restrict__ ps, float2* __restrict__ mexp, uint32_t lo,
+                                                uint32_t cnt) {
+  const uint32_t t = blockIdx.x * EWF + threadIdx.x;
+  if (t >= cnt) return;
+  const uint32_t c = lo + t;
+  const FCell cell = cells[c];
+  const double inf = __builtin_huge_val();
+  double x0 = inf, y0 = inf, x1 = -inf, y1 = -inf;
+  if (cell.nchild == 0) {
+    for (uint32_t k = cell.first; k < cell.first + cell.count; ++k) {
+      x0 = zmin(x0, ux[k]);
+      x1 = zmax(x1, ux[k]);
+      y0 = zmin(y0, uy[k]);
+      y1 = zmax(y1, uy[k]);
+    }
+  } else {
+    for (uint32_t ch = cell.child0; ch < cell.child0 + cell.nchild; ++ch) {
+      x0 = zmin(x0, bbox[4 * (size_t)ch + 0]);
+      x1 = zmax(x1, bbox[4 * (size_t)ch + 2]);
+      y0 = zmin(y0, bbox[4 * (size_t)ch + 1]);
+      y1 = zmax(y1, bbox[4 * (size_t)ch + 3]);
+    }
+  }
+  bbox[4 * (size_t)c + 0] = x0;
+  bbox[4 * (size_t)c + 1] = y0;
+  bbox[4 * (size_t)c + 2] = x1;
+  bbox[4 * (size_t)c + 3] = y1;
+  const double cx = (x0 + x1) / 2, cy = (y0 + y1) / 2;
+  const double dx = x1 - x0, dy = y1 - y0;
+  const double r = 0.5 * __builtin_sqrt(dx * dx + dy * dy);
+  cells[c].cx = cx;
+  cells[c].cy = cy;
+  cells[c].r = r;
+
+  float2 m[Q];
+#pragma unroll
+  for (int k = 0; k < Q; ++k) m[k] = make_float2(0.f, 0.f);
+  if (cell.nchild == 0) {
+    const double inv = r > 0 ? 1 / r : 0;
+    for (uint32_t k = cell.first; k < cell.first + cell.count; ++k) {
+      const float s = ps[k];
+      const float2 z = make_float2((float)((ux[k] - cx) * inv), (float)((uy[k] - cy) * inv));
+      float2 zp = make_float2(s, 0.f);
+      m[0].x += s;
+#pragma unroll
+      for (int o = 1; o < Q; ++o) {
+        zp = cmul(zp, z);
+        m[o].x += zp.x;
+        m[o].y += zp.y;
+      }
+    }
+  } else {
+    const double inv = r > 0 ? 1 / r : 0;
+    for (uint32_t ch = cell.child0; ch < cell.child0 + cell.nchild; ++ch) {
+      const FCell cc = cells[ch];
+      const float tr = (float)(cc.r * inv);
+      const float2 e = make_float2((float)((cc.cx - cx) * inv), (float)((cc.cy - cy) * inv));
+      float2 mc[Q], ep[Q];
+      float tk = 1.f;
+      ep[0] = make_float2(1.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < Q; ++k) {
+        mc[k] = cscale(mexp[(size_t)ch * Q + k], tk);
+        tk *= tr;
+        if (k > 0) ep[k] = cmul(ep[k - 1], e);
+      }
+      // M^_n(P) += sum_k C(n, k) (rho_c / rho_P)^k M^_k(c) (d / rho_P)^(n-k)
+#pragma unroll
+      for (int nn = 0; nn < Q; ++nn) {
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k <= nn; ++k) acc = cfma(cscale(mc[k], binom(nn, k)), ep[nn - k], acc);
+        m[nn].x += acc.x;
+        m[nn].y += acc.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < Q; ++k) mexp[(size_t)c * Q + k] = m[k];
+}
+
+// ---------------------------------------------------------------- dual tree
+
+__device__ inline uint32_t wave_exclusive(uint32_t v, uint32_t& total) {
+  const int lane = threadIdx.x & 63;
+  uint32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += t;
+  }
+  total = __shfl(inc, 63, 64);
+  return inc - v;
+}
+
+// Reserves `v` slots per lane in *counter with one atomic per wave; every
+// lane of the wave must call it.
+__device__ inline uint32_t wave_append(uint32_t* counter, uint32_t v) {
+  uint32_t total;
+  const uint32_t off = wave_exclusive(v, total);
+  uint32_t base = 0;
+  if ((threadIdx.x & 63) == 0 && total) base = atomicAdd(counter, total);
+  return __shfl(base, 0, 64) + off;
+}
+
+struct MacArgs {
+  double mac_far, dmin_s, dmax_s;
+  int ring;
+  uint32_t nbits;
+};
+
+// One round of fmm.zig:400-453 over the frontier fp->front[step].
+__global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cells, FmmParams* __restrict__ fp,
+                                                   uint32_t step, const uint2* __restrict__ in,
+                                                   uint2* __restrict__ out, uint32_t front_cap,
+                                                   uint64_t* __restrict__ m2l, uint32_t m2l_cap,
+                                                   uint64_t* __restrict__ p2p, uint32_t p2p_cap, MacArgs mac) {
+  uint32_t cnt = fp->front[step];
+  if (cnt > front_cap) cnt = front_cap;
+  for (uint32_t base = blockIdx.x * EWF; base < cnt; base += gridDim.x * EWF) {
+    const uint32_t idx = base + threadIdx.x;
+    // kind: 0 none, 1 M2L, 2 P2P; push: 0 none, 1 children x children,
+    // 2 (A, children of B), 3 (children of A, B)
+    uint32_t A = 0, B = 0, kind = 0, push = 0, npush = 0, c0 = 0, nca = 0, ncb = 0;
+    unsigned long long pairs = 0;
+    if (idx < cnt) {
+      const uint2 pr = in[idx];
+      A = pr.x;
+      B = pr.y;
+      const FCell ca = cells[A];
+      if (A == B) {
+        if (ca.nchild == 0) {
+          kind = 2;
+          pairs = (unsigned long long)ca.count * ca.count;
+        } else {
+          push = 1;
+          c0 = ca.child0;
+          nca = ca.nchild;
+          npush = nca * nca;
+        }
+      } else {
+        const FCell cb = cells[B];
+        const double ra = ca.r, rb = cb.r;
+        const double dx = ca.cx - cb.cx, dy = ca.cy - cb.cy;
+        const double d2 = dx * dx + dy * dy;
+        const double rmax = zmax(ra, rb), rmin = zmin(ra, rb);
+        const double thr = mac.mac_far * rmax + rmin;
+        bool done = false;
+        if (d2 > MIN_SEP2 && d2 >= thr * thr) {
+          bool separated = true;
+          if (mac.ring) {
+            const double gap = __builtin_sqrt(d2) - ra - rb;
+            if (gap >= mac.dmax_s) done = true;
+            else if (gap + 2 * (ra + rb) > mac.dmax_s || gap < mac.dmin_s) separated = false;
+          }
+          if (!done && separated) {
+            kind = 1;
+            done = true;
+          }
+        }
+        if (!done) {
+          const bool la = ca.nchild == 0, lb = cb.nchild == 0;
+          if (la && lb) {
+            kind = 2;
+            pairs = (unsigned long long)ca.count * cb.count;
+          } else if (la || (!lb && rb > ra)) {
+            push = 2;
+            c0 = cb.child0;
+            ncb = cb.nchild;
+            npush = ncb;
+          } else {
+            push = 3;
+            c0 = ca.child0;
+            nca = ca.nchild;
+            npush = nca;
+          }
+        }
+      }
+    }
+    const uint32_t pos_front = wave_append(&fp->front[step + 1], npush);
+    const uint32_t pos_m2l = wave_append(&fp->m2l, kind == 1);
+    const uint32_t pos_p2p = wave_append(&fp->p2p, kind == 2);
+    unsigned long long ps = pairs;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) ps += __shfl_xor(ps, d, 64);
+    if ((threadIdx.x & 63) == 0 && ps) atomicAdd(&fp->pairs, ps);
+    const uint64_t key = (uint64_t)A << mac.nbits | B;
+    if (kind == 1) {
+      if (pos_m2l < m2l_cap) m2l[pos_m2l] = key;
+      else atomicOr(&fp->list_overflow, 2u);
+    } else if (kind == 2) {
+      if (pos_p2p < p2p_cap) p2p[pos_p2p] = key;
+      else atomicOr(&fp->list_overflow, 4u);
+    }
+    if (npush) {
+      if (pos_front + npush > front_cap) {
+        atomicOr(&fp->list_overflow, 1u);
+      } else {
+        uint32_t o = pos_front;
+        if (push == 1) {
+          for (uint32_t i = 0; i < nca; ++i)
+            for (uint32_t j = 0; j < nca; ++j) out[o++] = make_uint2(c0 + i, c0 + j);
+        } else if (push == 2) {
+          for (uint32_t j = 0; j < ncb; ++j) out[o++] = make_uint2(A, c0 + j);
+        } else {
+          for (uint32_t i = 0; i < nca; ++i) out[o++] = make_uint2(c0 + i, B);
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_fmm_lists_reset(FmmParams* __restrict__ fp, uint2* __restrict__ front) {
+  const uint32_t t = threadIdx.x;
+  if (t < FMM_MAX_STEPS + 2) fp->front[t] = t == 0 ? 1u : 0u;
+  if (t == 0) {
+    fp->m2l = 0;
+    fp->p2p = 0;
+    fp->list_overflow = 0;
+    fp->pairs = 0;
+    front[0] = make_uint2(0, 0);
+  }
+}
+
+// First and end position of each target's run in a (target, source) list.
+__global__ __launch_bounds__(EWF) void k_segments(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ count,
+                                                  uint32_t nbits, uint32_t* __restrict__ lo, uint32_t* __restrict__ hi) {
+  const uint32_t k = blockIdx.x * EWF + threadIdx.x;
+  const uint32_t cnt = *count;
+  if (k >= cnt) return;
+  const uint64_t a = keys[k] >> nbits;
+  if (k == 0 || (keys[k - 1] >> nbits) != a) lo[a] = k;
+  if (k == cnt - 1 || (keys[k + 1] >> nbits) != a) hi[a] = k + 1;
+}
+
+// ---------------------------------------------------------------- M2L
+
+template <int Q>
+__global__ __launch_bounds__(64) void k_m2l(const FCell* __restrict__ cells, const uint64_t* __restrict__ list,
+                                            const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
+                                            uint32_t nbits, const float2* __restrict__ mexp,
+                                            float2* __restrict__ lexp) {
+  const uint32_t A = blockIdx.x;
+  const int lane = threadIdx.x;
+  const uint32_t s0 = lo[A], s1 = hi[A];
+  const uint64_t mask = ((uint64_t)1 << nbits) - 1;
+  float2 acc[Q];
+#pragma unroll
+  for (int l = 0; l < Q; ++l) acc[l] = make_float2(0.f, 0.f);
+  if (s1 > s0) {
+    const FCell ca = cells[A];
+    for (uint32_t base = s0; base < s1; base += 64) {
+      const uint32_t s = base + lane;
+      if (s >= s1) continue;
+      const uint32_t B = (uint32_t)(list[s] & mask);
+      const FCell cb = cells[B];
+      const double dx = ca.cx - cb.cx, dy = ca.cy - cb.cy;
+      const double inv = 1 / (dx * dx + dy * dy);
+      const double wr = dx * inv, wi = -dy * inv;
+      const float2 w = make_float2((float)wr, (float)wi);
+      const float2 a = make_float2((float)(cb.r * wr), (float)(cb.r * wi));
+      const float2 b = make_float2((float)(ca.r * wr), (float)(ca.r * wi));
+      float2 u[Q];
+      float2 ap = make_float2(1.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < Q; ++k) {
+        u[k] = cmul(mexp[(size_t)B * Q + k], ap);
+        ap = cmul(ap, a);
+      }
+      float2 coef = w;  // (-1)^l b^l w
+#pragma unroll
+      for (int l = 0; l < Q; ++l) {
+        float2 v = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+          const float g = binom(k + l, k);
+          v.x = fmaf(g, u[k].x, v.x);
+          v.y = fmaf(g, u[k].y, v.y);
+        }
+        acc[l] = cfma(coef, v, acc[l]);
+        coef = cmul(coef, b);
+        coef.x = -coef.x;
+        coef.y = -coef.y;
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) {
+        acc[l].x += __shfl_xor(acc[l].x, d, 64);
+        acc[l].y += __shfl_xor(acc[l].y, d, 64);
+      }
+    }
+  }
+  if (lane < Q) {
+    float2 v = acc[0];
+#pragma unroll
+    for (int l = 1; l < Q; ++l)
+      if (lane == l) v = acc[l];
+    lexp[(size_t)A * Q + lane] = v;
+  }
+}
+
+// ---------------------------------------------------------------- downward
+
+// L2L (fmm.zig:823-849) from each cell of one depth into its children.
+template <int Q>
+__global__ __launch_bounds__(EWF) void k_fmm_down(const FCell* __restrict__ cells, float2* __restrict__ lexp,
+                                                  uint32_t lo, uint32_t cnt) {
+  const uint32_t t = blockIdx.x * EWF + threadIdx.x;
+  if (t >= cnt) return;
+  const uint32_t c = lo + t;
+  const FCell cell = cells[c];
+  if (cell.nchild == 0) return;
+  float2 lp[Q];
+#pragma unroll
+  for (int l = 0; l < Q; ++l) lp[l] = lexp[(size_t)c * Q + l];
+  const double inv = cell.r > 0 ? 1 / cell.r : 0;
+  for (uint32_t ch = cell.child0; ch < cell.child0 + cell.nchild; ++ch) {
+    const FCell cc = cells[ch];
+    const float tr = (float)(cc.r * inv);
+    const float2 e = make_float2((float)((cc.cx - cell.cx) * inv), (float)((cc.cy - cell.cy) * inv));
+    float2 ep[Q];
+    ep[0] = make_float2(1.f, 0.f);
+#pragma unroll
+    for (int k = 1; k < Q; ++k) ep[k] = cmul(ep[k - 1], e);
+    float tm = 1.f;
+    // L^_m(c) += (rho_c / rho_P)^m sum_{l >= m} C(l, m) L^_l(P) (d / rho_P)^(l-m)
+#pragma unroll
+    for (int mm = 0; mm < Q; ++mm) {
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int l = mm; l < Q; ++l) acc = cfma(cscale(lp[l], binom(l, mm)), ep[l - mm], acc);
+      float2 old = lexp[(size_t)ch * Q + mm];
+      old.x = fmaf(acc.x, tm, old.x);
+      old.y = fmaf(acc.y, tm, old.y);
+      lexp[(size_t)ch * Q + mm] = old;
+      tm *= tr;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- near field + L2P
+
+// One wave per target leaf: lane groups of S lanes per target point split
+// the sources; the S partial sums are added in lane order, then L2P
+// (fmm.zig:853-886) and the FP64 velocity update. The pair term is d3's
+// (fmm.zig:529-553): skip beyond dmax, hash jiggle on an exact zero delta,
+// sqrt(dmin2 * l) below dmin.
+template <int Q>
+__global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, const uint64_t* __restrict__ list,
+                                             const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
+                                             uint32_t nbits, const float2* __restrict__ lexp,
+                                             const double* __restrict__ px, const double* __restrict__ py,
+                                             const float* __restrict__ ps, const double* __restrict__ ux,
+                                             const double* __restrict__ uy, const uint32_t* __restrict__ perm,
+                                             double* __restrict__ vxs, double* __restrict__ vys, double inv_w,
+                                             float dmin2, float dmax2, uint64_t seed) {
+  const uint32_t A = blockIdx.x;
+  const FCell ca = cells[A];
+  if (ca.nchild != 0) return;
+  const int lane = threadIdx.x;
+  const uint32_t s0 = lo[A], s1 = hi[A];
+  const uint64_t mask = ((uint64_t)1 << nbits) - 1;
+  __shared__ float2 part[64];
+  float2 L[Q];
+#pragma unroll
+  for (int l = 0; l < Q; ++l) L[l] = lexp[(size_t)A * Q + l];
+  const double inv_r = ca.r > 0 ? 1 / ca.r : 0;
+  for (uint32_t tb = 0; tb < ca.count; tb += 64) {
+    const uint32_t T = ca.count - tb < 64 ? ca.count - tb : 64;
+    const uint32_t S = 64 / T;
+    const uint32_t t = lane / S, sub = lane % S;
+    const bool active = t < T;
+    const uint32_t ti = ca.first + tb + (active ? t : 0);
+    const double xi = px[ti], yi = py[ti];
+    const uint32_t i = perm[ti];
+    float ax = 0.f, ay = 0.f;
+    for (uint32_t s = s0; s < s1; ++s) {
+      const uint32_t B = (uint32_t)(list[s] & mask);
+      const uint32_t f0 = cells[B].first, f1 = f0 + cells[B].count;
+      if (!active) continue;
+      for (uint32_t j = f0 + sub; j < f1; j += S) {
+        if (j == ti) continue;
+        float dx = (float)(px[j] - xi), dy = (float)(py[j] - yi);
+        float l = fmaf(dx, dx, dy * dy);
+        if (!(l < dmax2)) continue;
+        if (dx == 0.f || dy == 0.f) {
+          const uint32_t jj = perm[j];
+          if (dx == 0.f) {
+            dx = (float)hash_jiggle(seed, i, jj, 0);
+            l = fmaf(dx, dx, l);
+          }
+          if (dy == 0.f) {
+            dy = (float)hash_jiggle(seed, i, jj, 1);
+            l = fmaf(dy, dy, l);
+          }
+        }
+        if (l < dmin2) l = __builtin_sqrtf(dmin2 * l);
+        const float wj = ps[j] / l;
+        ax = fmaf(dx, wj, ax);
+        ay = fmaf(dy, wj, ay);
+      }
+    }
+    part[lane] = make_float2(ax, ay);
+    __syncthreads();
+    if (active && sub == 0) {
+      float2 near = part[lane];
+      for (uint32_t q = 1; q < S; ++q) {
+        near.x += part[lane + q].x;
+        near.y += part[lane + q].y;
+      }
+      const float2 z = make_float2((float)((ux[ti] - ca.cx) * inv_r), (float)((uy[ti] - ca.cy) * inv_r));
+      float2 f = L[Q - 1];
+#pragma unroll
+      for (int l = Q - 2; l >= 0; --l) f = cfma(f, z, L[l]);
+      vxs[i] += (double)near.x - (double)f.x * inv_w;
+      vys[i] += (double)near.y + (double)f.y * inv_w;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- direct sum
+
+// FP64 reference force for the error measurement: d3's pair term summed over
+// every other node in index order, one block per sampled target
+// (fmm.zig:946-971, test/fast/error-test.js:44-58): coincident pairs and
+// NaN terms are skipped, l < dmin2 -> sqrt(dmin2 * l), l >= dmax2 skipped.
+__global__ __launch_bounds__(256) void k_direct_sum(const double* __restrict__ xs, const double* __restrict__ ys,
+                                                    const double* __restrict__ st, uint32_t n,
+                                                    const uint32_t* __restrict__ sample, double alpha, double dmin2,
+                                                    double dmax2, double* __restrict__ out) {
+  const uint32_t i = sample[blockIdx.x];
+  const double xi = xs[i], yi = ys[i];
+  double vx = 0, vy = 0;
+  for (uint32_t j = threadIdx.x; j < n; j += 256) {
+    if (j == i) continue;
+    const double dx = xs[j] - xi, dy = ys[j] - yi;
+    double l = dx * dx + dy * dy;
+    if (!(l > 0) || !(l < dmax2)) continue;
+    const double s = st[j] * alpha;
+    if (s != s) continue;
+    if (l < dmin2) l = __builtin_sqrt(dmin2 * l);
+    const double w = s / l;
+    vx += dx * w;
+    vy += dy * w;
+  }
+  __shared__ double sx[256], sy[256];
+  sx[threadIdx.x] = vx;
+  sy[threadIdx.x] = vy;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      sx[threadIdx.x] += sx[threadIdx.x + k];
+      sy[threadIdx.x] += sy[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = sx[0];
+    out[2 * blockIdx.x + 1] = sy[0];
+  }
+}
+
+// ---------------------------------------------------------------- host
+
+template <class T>
+hipError_t grow(T*& p, size_t n) {
+  if (p) HIPCHECK(hipFree(p));
+  p = nullptr;
+  return hipMalloc((void**)&p, (n > 0 ? n : 1) * sizeof(T));
+}
+
+hipError_t upload_binomials() {
+  static bool done = false;
+  if (done) return hipSuccess;
+  std::vector<double> b((size_t)BN * BN, 0.0);
+  for (int n = 0; n < BN; ++n) {
+    b[(size_t)n * BN] = 1;
+    for (int k = 1; k <= n; ++k) b[(size_t)n * BN + k] = b[(size_t)(n - 1) * BN + k - 1] + (k < n ? b[(size_t)(n - 1) * BN + k] : 0.0);
+  }
+  std::vector<float> f(b.begin(), b.end());
+  HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_binom), f.data(), f.size() * sizeof(float)));
+  done = true;
+  return hipSuccess;
+}
+
+hipError_t reserve_points(FmmBuffers& f, uint32_t n) {
+  if (!f.params) {
+    HIPCHECK(hipMalloc((void**)&f.params, sizeof(FmmParams)));
+    HIPCHECK(hipHostMalloc((void**)&f.params_host, sizeof(FmmParams), hipHostMallocDefault));
+  }
+  if (n <= f.point_cap) return hipSuccess;
+  const uint32_t cap = n < 4096 ? 4096 : n;
+  HIPCHECK(grow(f.px, cap));
+  HIPCHECK(grow(f.py, cap));
+  HIPCHECK(grow(f.ux, cap));
+  HIPCHECK(grow(f.uy, cap));
+  HIPCHECK(grow(f.ps, cap));
+  HIPCHECK(grow(f.split, cap));
+  HIPCHECK(grow(f.split_off, cap));
+  HIPCHECK(grow(f.bounds, (size_t)5 * cap));
+  f.point_cap = cap;
+  if (f.cell_cap < cap / 4 + 4096) {
+    HIPCHECK(grow(f.cells, cap / 4 + 4096));
+    f.cell_cap = cap / 4 + 4096;
+  }
+  return hipSuccess;
+}
+
+// Per-cell arrays and the first list capacities, kept across runs.
+hipError_t reserve_cells(FmmBuffers& f, uint32_t nc, uint32_t q) {
+  if (nc > f.cellarr_cap) {
+    const uint32_t cap = nc + nc / 4 + 1024;
+    HIPCHECK(grow(f.bbox, (size_t)4 * cap));
+    HIPCHECK(grow(f.m2l_lo, cap));
+    HIPCHECK(grow(f.m2l_hi, cap));
+    HIPCHECK(grow(f.p2p_lo, cap));
+    HIPCHECK(grow(f.p2p_hi, cap));
+    f.cellarr_cap = cap;
+  }
+  if ((size_t)nc * q > f.exp_cap) {
+    const size_t cap = (size_t)(nc + nc / 4 + 1024) * q;
+    HIPCHECK(grow(f.mexp, 2 * cap));
+    HIPCHECK(grow(f.lexp, 2 * cap));
+    f.exp_cap = cap;
+  }
+  const uint32_t want_front = std::max<uint32_t>(1u << 16, 16 * nc);
+  if (want_front > f.front_cap) {
+    HIPCHECK(grow(f.front_a, want_front));
+    HIPCHECK(grow(f.front_b, want_front));
+    f.front_cap = want_front;
+  }
+  const uint32_t want_list = std::max<uint32_t>(1u << 16, 40 * nc);
+  if (want_list > f.m2l_cap) {
+    HIPCHECK(grow(f.m2l, want_list));
+    HIPCHECK(grow(f.m2l_tmp, want_list));
+    HIPCHECK(grow(f.p2p, want_list));
+    HIPCHECK(grow(f.p2p_tmp, want_list));
+    HIPCHECK(grow(f.seg_vals, want_list));
+    HIPCHECK(grow(f.seg_vals_tmp, want_list));
+    HIPCHECK(grow(f.hist, sort_hist_words(want_list)));
+    f.m2l_cap = f.p2p_cap = want_list;
+  }
+  return hipSuccess;
+}
+
+template <int Q>
+hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t stream) {
+  const FmmParams& P = *f.params_host;
+  const uint32_t nc = P.level_off[FMM_LEVELS + 1];
+  uint32_t maxd = 0;
+  for (uint32_t d = 0; d <= FMM_LEVELS; ++d)
+    if (P.level_n[d]) maxd = d;
+  if (nc > f.cell_cap || nc > f.cellarr_cap || (size_t)nc * Q > f.exp_cap) return hipErrorInvalidValue;
+  // upward
+  for (int d = (int)maxd; d >= 0; --d) {
+    const uint32_t cnt = P.level_n[d];
+    hipLaunchKernelGGL(k_fmm_up<Q>, dim3(ceil_div(cnt, EWF)), dim3(EWF), 0, stream, f.cells, f.bbox, f.ux, f.uy, f.ps,
+                       (float2*)f.mexp, P.level_off[d], cnt);
+  }
+  // interaction lists
+  uint32_t nbits = 1;
+  while (((uint64_t)1 << nbits) < nc) ++nbits;
+  MacArgs mac;
+  double kappa = a.opening;
+  if (!(kappa > 1)) kappa = 1.4;  // fmm.zig:262-263
+  mac.mac_far = 2 * kappa - 1;
+  const double inf = __builtin_huge_val();
+  mac.dmax_s = a.dmax2 < inf ? std::sqrt(a.dmax2) / P.w : inf;
+  mac.dmin_s = a.dmin2 > 0 ? std::sqrt(a.dmin2) / P.w : 0;
+  mac.ring = mac.dmin_s > 0 || mac.dmax_s < inf;
+  mac.nbits = nbits;
+  const uint32_t steps = 2 * maxd + 1;
+  for (;;) {
+    hipLaunchKernelGGL(k_fmm_lists_reset, dim3(1), dim3(64), 0, stream, f.params, f.front_a);
+    uint2 *in = f.front_a, *out = f.front_b;
+    for (uint32_t s = 0; s < steps; ++s) {
+      hipLaunchKernelGGL(k_dual_step, dim3(GRID_BLOCKS), dim3(EWF), 0, stream, f.cells, f.params, s, in, out,
+                         f.front_cap, f.m2l, f.m2l_cap, f.p2p, f.p2p_cap, mac);
+      std::swap(in, out);
+    }
+    HIPCHECK(hipMemcpyAsync(f.params_host, f.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (P.front[steps] != 0) return hipErrorUnknown;  // the rounds must drain the recursion
+    if (!P.list_overflow) break;
+    uint32_t fmax = 0;
+    for (uint32_t s = 0; s <= steps; ++s) fmax = std::max(fmax, P.front[s]);
+    if (P.list_overflow & 1u) {
+      const uint32_t c = std::max(2 * f.front_cap, fmax + fmax / 4);
+      HIPCHECK(grow(f.front_a, c));
+      HIPCHECK(grow(f.front_b, c));
+      f.front_cap = c;
+    }
+    if (P.list_overflow & 6u) {
+      const uint32_t c = std::max(P.m2l, P.p2p);
+      const uint32_t cap = c + c / 4 + 1024;
+      if (cap > f.m2l_cap) {
+        HIPCHECK(grow(f.m2l, cap));
+        HIPCHECK(grow(f.m2l_tmp, cap));
+        HIPCHECK(grow(f.p2p, cap));
+        HIPCHECK(grow(f.p2p_tmp, cap));
+        HIPCHECK(grow(f.seg_vals, cap));
+        HIPCHECK(grow(f.seg_vals_tmp, cap));
+        HIPCHECK(grow(f.hist, sort_hist_words(cap)));
+        f.m2l_cap = f.p2p_cap = cap;
+      }
+    }
+  }
+  // sorted lists and their segments
+  int digits = (int)((2 * nbits + 7) / 8);
+  digits += digits & 1;
+  uint32_t* glob = t.rs_glob;
+  if (P.m2l) HIPCHECK(sort_digits(f.hist, glob, f.m2l, f.seg_vals, f.m2l_tmp, f.seg_vals_tmp, P.m2l, &f.params->m2l, 0, digits, stream));
+  if (P.p2p) HIPCHECK(sort_digits(f.hist, glob, f.p2p, f.seg_vals, f.p2p_tmp, f.seg_vals_tmp, P.p2p, &f.params->p2p, 0, digits, stream));
+  HIPCHECK(hipMemsetAsync(f.m2l_lo, 0, (size_t)nc * sizeof(uint32_t), stream));
+  HIPCHECK(hipMemsetAsync(f.m2l_hi, 0, (size_t)nc * sizeof(uint32_t), stream));
+  HIPCHECK(hipMemsetAsync(f.p2p_lo, 0, (size_t)nc * sizeof(uint32_t), stream));
+  HIPCHECK(hipMemsetAsync(f.p2p_hi, 0, (size_t)nc * sizeof(uint32_t), stream));
+  if (P.m2l)
+    hipLaunchKernelGGL(k_segments, dim3(ceil_div(P.m2l, EWF)), dim3(EWF), 0, stream, f.m2l, &f.params->m2l, nbits,
+                       f.m2l_lo, f.m2l_hi);
+  if (P.p2p)
+    hipLaunchKernelGGL(k_segments, dim3(ceil_div(P.p2p, EWF)), dim3(EWF), 0, stream, f.p2p, &f.params->p2p, nbits,
+                       f.p2p_lo, f.p2p_hi);
+  // far field
+  hipLaunchKernelGGL(k_m2l<Q>, dim3(nc), dim3(64), 0, stream, f.cells, f.m2l, f.m2l_lo, f.m2l_hi, nbits,
+                     (const float2*)f.mexp, (float2*)f.lexp);
+  for (uint32_t d = 0; d < maxd; ++d) {
+    const uint32_t cnt = P.level_n[d];
+    hipLaunchKernelGGL(k_fmm_down<Q>, dim3(ceil_div(cnt, EWF)), dim3(EWF), 0, stream, f.cells, (float2*)f.lexp,
+                       P.level_off[d], cnt);
+  }
+  // near field and evaluation
+  const float dmin2 = (float)a.dmin2, dmax2 = (float)a.dmax2;
+  hipLaunchKernelGGL(k_near<Q>, dim3(nc), dim3(64), 0, stream, f.cells, f.p2p, f.p2p_lo, f.p2p_hi, nbits,
+                     (const float2*)f.lexp, f.px, f.py, f.ps, f.ux, f.uy, t.perm, a.vx, a.vy, P.inv_w, dmin2, dmax2,
+                     a.seed);
+  return hipGetLastError();
+}
+
+using OrderFn = hipError_t (*)(FmmBuffers&, TreeBuffers&, const FmmArgs&, hipStream_t);
+
+template <int... Qs>
+struct OrderTable {
+  static constexpr OrderFn fns[] = {&run_order<Qs>...};
+};
+template <int... Qs>
+constexpr OrderFn OrderTable<Qs...>::fns[];
+
+using Orders = OrderTable<2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25>;
+
+}  // namespace
+
+void fmm_free(FmmBuffers& f) {
+  void* ptrs[] = {f.cells, f.bbox, f.split, f.split_off, f.bounds, f.mexp, f.lexp, f.px, f.py, f.ux, f.uy, f.ps,
+                  f.front_a, f.front_b, f.m2l, f.m2l_tmp, f.p2p, f.p2p_tmp, f.seg_vals, f.seg_vals_tmp, f.m2l_lo,
+                  f.m2l_hi, f.p2p_lo, f.p2p_hi, f.hist, f.params};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (f.params_host) (void)hipHostFree(f.params_host);
+  f = FmmBuffers();
+}
+
+hipError_t direct_sum(const double* x, const double* y, const double* strength, uint32_t n, const uint32_t* sample,
+                      uint32_t k, double alpha, double dmin2, double dmax2, double* out, hipStream_t stream) {
+  if (k == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_direct_sum, dim3(k), dim3(256), 0, stream, x, y, strength, n, sample, alpha, dmin2, dmax2, out);
+  return hipGetLastError();
+}
+
+hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t stream) {
+  HIPCHECK(upload_binomials());
+  const uint32_t n = a.n;
+  uint32_t p = a.order;  // fmm.zig:258-261
+  if (p < 1 || p > FMM_MAX_ORDER) p = 8;
+  HIPCHECK(tree_prepare_fast(t, a.x, a.y, n, stream));
+  HIPCHECK(reserve_points(f, n));
+  const uint32_t nb = ceil_div(n > 0 ? n : 1, EWF);
+  for (;;) {
+    hipLaunchKernelGGL(k_fmm_setup, dim3(1), dim3(1), 0, stream, t.meta, f.params, f.cells);
+    for (uint32_t d = 0; d < FMM_LEVELS; ++d) {
+      hipLaunchKernelGGL(k_fmm_split, dim3(GRID_BLOCKS), dim3(EWF), 0, stream, t.keys, f.cells, f.params, d, f.split,
+                         f.bounds, f.cell_cap);
+      HIPCHECK(scan_u32(f.split, f.split_off, n > 0 ? n : 1, &f.params->level_n[d], t.partials,
+                        &f.params->level_n[d + 1], stream));
+      hipLaunchKernelGGL(k_fmm_children, dim3(GRID_BLOCKS), dim3(EWF), 0, stream, f.cells, f.params, d, f.split,
+                         f.split_off, f.bounds, f.cell_cap);
+    }
+    HIPCHECK(hipMemcpyAsync(f.params_host, f.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (!f.params_host->cell_overflow) break;
+    const uint32_t cap = std::max(2 * f.cell_cap, f.params_host->level_off[FMM_LEVELS + 1] + 4096);
+    HIPCHECK(grow(f.cells, cap));
+    f.cell_cap = cap;
+  }
+  const FmmParams& P = *f.params_host;
+  if (P.m == 0) return hipSuccess;      // no point: nothing to do (fmm.zig:251-252)
+  if (!P.ok) return hipErrorNotSupported;  // fmm.zig:255-257: the exact kernel runs instead
+  HIPCHECK(reserve_cells(f, P.level_off[FMM_LEVELS + 1], p + 1));
+  hipLaunchKernelGGL(k_fmm_gather, dim3(nb), dim3(EWF), 0, stream, t.perm, a.x, a.y, a.strength, f.params, a.alpha, n,
+                     f.px, f.py, f.ux, f.uy, f.ps);
+  return Orders::fns[p - 1](f, t, a, stream);
+}
+
+}  // namespace d3f

@@ -103,6 +103,11 @@ struct Force {
   std::vector<uint32_t> level_off;
   Device<uint32_t> lsrc, ltgt;
   Device<double> lf[3];
+  // fast link adjacency rows (link_fast.zig:105-130)
+  bool csr_valid = false;
+  uint32_t csr_n = 0;
+  Device<uint32_t> row, adj, heavy;
+  uint32_t heavy_count = 0;
 
   explicit Force(Kind k) : kind(k) {}
   ~Force() {
@@ -115,6 +120,9 @@ struct Force {
     lsrc.release();
     ltgt.release();
     for (auto& a : lf) a.release();
+    row.release();
+    adj.release();
+    heavy.release();
   }
   bool node_sized() const { return kind != K_LINK && kind != K_CENTER; }  // force.zig:50-52
   int float_arrays() const {                                              // force.zig:54-61
@@ -140,6 +148,7 @@ struct Force {
     count = c;
     dirty = true;
     sched_valid = false;
+    csr_valid = false;
     return hipSuccess;
   }
   double scalar(Scalar s) const { return scalars[s]; }
@@ -152,6 +161,13 @@ struct Sim {
   Device<double> dx, dy, dvx, dvy, dfx, dfy, dpx, dpy, dsvx, dsvy;
   Device<uint8_t> dfixed;
   TreeBuffers tree;
+  FmmBuffers fmm;
+  // fast-mode scratch: radix sort ping-pong, grid of the fast collide
+  Device<uint64_t> sk0, sk1;
+  Device<uint32_t> sv0, sv1, shist, sglob, scount;
+  Device<uint32_t> gcell_of, gstart;
+  Device<uint8_t> gpartials;
+  GridParams* gparams = nullptr;
   hipStream_t stream = nullptr;
   std::vector<Force*> forces;
   // scratch
@@ -192,6 +208,11 @@ struct Sim {
     for (auto* a : {&dx, &dy, &dvx, &dvy, &dfx, &dfy, &dpx, &dpy, &dsvx, &dsvy}) a->release();
     dfixed.release();
     tree_free(tree);
+    fmm_free(fmm);
+    for (auto* a : {&sk0, &sk1}) a->release();
+    for (auto* a : {&sv0, &sv1, &shist, &sglob, &scount, &gcell_of, &gstart}) a->release();
+    gpartials.release();
+    if (gparams) (void)hipFree(gparams);
     flagged.release();
     order.release();
     draws_count.release();
@@ -231,6 +252,9 @@ struct Sim {
     HIPCHECK(hipEventCreateWithFlags(&ev_sums_done, hipEventDisableTiming));
     HIPCHECK(hipMalloc(&dsums_side, 2 * sizeof(double)));
     HIPCHECK(hipMalloc(&dcounters, 4 * sizeof(unsigned long long)));
+    HIPCHECK(hipMalloc(&gparams, sizeof(GridParams)));
+    HIPCHECK(sglob.reserve(8 * 256));
+    HIPCHECK(scount.reserve(4));
     HIPCHECK(hipMemset(dcounters, 0, 4 * sizeof(unsigned long long)));
     for (int k = 0; k < 4; ++k) hextent[k] = __builtin_nan("");
     return hipSuccess;
@@ -291,6 +315,25 @@ struct Sim {
     HIPCHECK(hipEventRecord(ev_sums_done, side));
     side_valid = true;
     side_busy = true;
+    return hipSuccess;
+  }
+
+  // Radix-sort scratch for `items` entries.
+  hipError_t sort_buffers(size_t items, SortBuffers* out) {
+    const size_t c = items > 0 ? items : 1;
+    HIPCHECK(sk0.reserve(c, 0, stream));
+    HIPCHECK(sk1.reserve(c, 0, stream));
+    HIPCHECK(sv0.reserve(c, 0, stream));
+    HIPCHECK(sv1.reserve(c, 0, stream));
+    HIPCHECK(shist.reserve(sort_hist_words((uint32_t)c), 0, stream));
+    out->k0 = sk0.p;
+    out->k1 = sk1.p;
+    out->v0 = sv0.p;
+    out->v1 = sv1.p;
+    out->hist = shist.p;
+    out->glob = sglob.p;
+    out->count = scount.p;
+    out->cap = (uint32_t)c;
     return hipSuccess;
   }
 
@@ -549,6 +592,100 @@ int32_t run_link_exact(Sim& s, Force& f, double alpha) {
       st->has_jig = 1;
     }
   }
+  return OK;
+}
+
+// ---------------------------------------------------------------- fast mode
+uint64_t seed_of(const Force& f) {  // hash.zig:28-30: the seed's bits
+  const double v = f.scalar(S_SEED);
+  uint64_t u;
+  std::memcpy(&u, &v, 8);
+  return u;
+}
+
+int32_t run_link_fast(Sim& s, Force& f, double alpha) {  // link_fast.zig:22-99
+  const uint32_t m = f.count, n = s.n;
+  hipError_t e;
+  if (!f.csr_valid || f.csr_n != n) {
+    for (uint32_t k = 0; k < m; ++k)  // link_fast.zig:24-26
+      if (f.source.p[k] >= n || f.target.p[k] >= n) return BAD_ARGUMENT;
+    SortBuffers sb;
+    if ((e = s.sort_buffers(2 * (size_t)m, &sb)) != hipSuccess) return status_of(e);
+    if ((e = f.row.reserve((size_t)n + 1, 0, s.stream)) != hipSuccess) return status_of(e);
+    if ((e = f.adj.reserve(2 * (size_t)m > 0 ? 2 * (size_t)m : 1, 0, s.stream)) != hipSuccess) return status_of(e);
+    if ((e = f.heavy.reserve(n > 0 ? n : 1, 0, s.stream)) != hipSuccess) return status_of(e);
+    if ((e = link_csr(f.dsrc.p, f.dtgt.p, m, n, f.row.p, f.adj.p, f.heavy.p, &f.heavy_count, sb, s.stream)) !=
+        hipSuccess)
+      return status_of(e);
+    f.csr_valid = true;
+    f.csr_n = n;
+  }
+  const double iterations = f.scalar(S_ITERATIONS);
+  const uint64_t seed = seed_of(f);
+  s.timer_start(D3F_PHASE_LINK);
+  for (double k = 0; k < iterations; k += 1) {
+    e = shift_positions(s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, s.dpx.p, s.dpy.p, n, s.stream);
+    if (e == hipSuccess)
+      e = link_sweep(n, f.row.p, f.adj.p, f.heavy.p, f.heavy_count, f.dsrc.p, f.dtgt.p, f.df[0].p, f.df[1].p,
+                     f.df[2].p, s.dpx.p, s.dpy.p, s.dvx.p, s.dvy.p, alpha, seed, s.stream);
+    if (e != hipSuccess) return status_of(e);
+  }
+  s.timer_stop();
+  return OK;
+}
+
+int32_t run_collide_fast(Sim& s, Force& f) {  // grid.zig:198-286
+  const uint32_t n = s.n;
+  hipError_t e;
+  SortBuffers sb;
+  if ((e = s.sort_buffers(n, &sb)) != hipSuccess) return status_of(e);
+  const size_t c = n > 0 ? n : 1;
+  if ((e = s.gcell_of.reserve(c, 0, s.stream)) != hipSuccess) return status_of(e);
+  if ((e = s.gstart.reserve(4 * c + 17, 0, s.stream)) != hipSuccess) return status_of(e);
+  if ((e = s.gpartials.reserve(grid_partials_bytes(), 0, s.stream)) != hipSuccess) return status_of(e);
+  GridBuffers g;
+  g.px = s.dpx.p;
+  g.py = s.dpy.p;
+  g.cell_of = s.gcell_of.p;
+  g.start = s.gstart.p;
+  g.partials = s.gpartials.p;
+  g.params = s.gparams;
+  const double iterations = f.scalar(S_ITERATIONS);
+  const uint64_t seed = seed_of(f);
+  s.timer_start(D3F_PHASE_COLLIDE);
+  for (double k = 0; k < iterations; k += 1) {
+    e = grid_collide_iteration(s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, f.df[0].p, n, f.scalar(S_STRENGTH), seed, g, sb,
+                               s.stream);
+    if (e != hipSuccess) return status_of(e);
+  }
+  s.timer_stop();
+  return OK;
+}
+
+int32_t run_many_body_fast(Sim& s, Force& f, double alpha) {  // fmm.zig:247-354
+  FmmArgs a;
+  a.x = s.dx.p;
+  a.y = s.dy.p;
+  a.strength = f.df[0].p;
+  a.vx = s.dvx.p;
+  a.vy = s.dvy.p;
+  a.n = s.n;
+  const double order = f.scalar(S_ORDER);
+  a.order = order >= 1 && order <= (double)FMM_MAX_ORDER ? (uint32_t)order : 8;
+  a.alpha = alpha;
+  a.opening = f.scalar(S_OPENING);
+  a.dmin2 = f.scalar(S_DMIN2);
+  a.dmax2 = f.scalar(S_DMAX2);
+  a.seed = seed_of(f);
+  s.timer_start(D3F_PHASE_FMM);
+  hipError_t e = fmm_run(s.fmm, s.tree, a, s.stream);
+  s.timer_stop();
+  if (e == hipErrorNotSupported) return run_many_body_exact(s, f, alpha);  // fmm.zig:255-257
+  if (e != hipSuccess) return status_of(e);
+  const FmmParams& P = *s.fmm.params_host;
+  s.stats.m2l += P.m2l;
+  s.stats.p2p_pairs += P.pairs;
+  s.stats.fmm_cells = P.level_off[FMM_LEVELS + 1];
   return OK;
 }
 
@@ -824,6 +961,7 @@ EXPORT void* force_ptr(uint32_t h, uint32_t fh, uint32_t which) {  // root.zig:1
   if (!f || f->count == 0) return nullptr;
   f->dirty = true;
   f->sched_valid = false;
+  f->csr_valid = false;
   switch (which) {
     case D3F_F0: return f->float_arrays() > 0 ? f->f[0].p : nullptr;
     case D3F_F1: return f->float_arrays() > 1 ? f->f[1].p : nullptr;
@@ -867,16 +1005,40 @@ EXPORT int32_t force_run(uint32_t h, uint32_t fh, double alpha) {  // root.zig:2
       return status_of(radial_force(s->dx.p, s->dy.p, s->dvx.p, s->dvy.p, f->df[0].p, f->df[1].p, s->n,
                                     f->scalar(S_X), f->scalar(S_Y), alpha, s->stream));
     case K_LINK:
-      if (fast_mode(*s, *f)) return BAD_ARGUMENT;  // fast link: not yet on this build
-      return run_link_exact(*s, *f, alpha);
+      return fast_mode(*s, *f) ? run_link_fast(*s, *f, alpha) : run_link_exact(*s, *f, alpha);
     case K_MANY_BODY:
-      if (fast_mode(*s, *f)) return BAD_ARGUMENT;  // fast many-body: not yet on this build
-      return run_many_body_exact(*s, *f, alpha);
+      return fast_mode(*s, *f) ? run_many_body_fast(*s, *f, alpha) : run_many_body_exact(*s, *f, alpha);
     case K_COLLIDE:
-      return BAD_ARGUMENT;  // collide: not on this build yet
+      // Exact collide (collide.zig, Gauss-Seidel pushes onto other nodes) is
+      // not on this build: only the fast grid collide runs.
+      return fast_mode(*s, *f) ? run_collide_fast(*s, *f) : BAD_ARGUMENT;
     default:
       return BAD_ARGUMENT;
   }
+}
+
+EXPORT int32_t force_direct_sum(uint32_t h, uint32_t fh, double alpha, const uint32_t* sample, uint32_t k,
+                                double* out) {
+  Sim* s = get(h);
+  Force* f = get_force(s, fh);
+  if (!f) return BAD_HANDLE;
+  if (f->kind != K_MANY_BODY || f->count < s->n || (k && (!sample || !out))) return BAD_ARGUMENT;
+  for (uint32_t q = 0; q < k; ++q)
+    if (sample[q] >= s->n) return BAD_ARGUMENT;
+  hipError_t e = upload(*s, *f);
+  Device<uint32_t> dsample;
+  Device<double> dout;
+  if (e == hipSuccess) e = dsample.reserve(k > 0 ? k : 1);
+  if (e == hipSuccess) e = dout.reserve(2 * (size_t)(k > 0 ? k : 1));
+  if (e == hipSuccess && k) e = hipMemcpyAsync(dsample.p, sample, k * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess)
+    e = direct_sum(s->dx.p, s->dy.p, f->df[0].p, s->n, dsample.p, k, alpha, f->scalar(S_DMIN2), f->scalar(S_DMAX2),
+                   dout.p, s->stream);
+  if (e == hipSuccess && k) e = hipMemcpyAsync(out, dout.p, 2 * k * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dsample.release();
+  dout.release();
+  return status_of(e);
 }
 
 EXPORT int32_t sim_stats(uint32_t h, d3f_stats* out) {

@@ -159,6 +159,10 @@ __global__ __launch_bounds__(BOX_THREADS) void k_box(const double* __restrict__ 
       meta->valid = 1;
     }
     for (int k = 0; k < 4; ++k) meta->extent[k] = e[k];
+    meta->box[0] = r.x0;
+    meta->box[1] = r.y0;
+    meta->box[2] = r.x1;
+    meta->box[3] = r.y1;
     meta->ticket = 0;
   }
 }
@@ -179,16 +183,50 @@ __device__ inline uint64_t exact_key(double x, double y, const double* e) {
   return key;
 }
 
+// Bits 0..15 of v moved to the even bit positions (tree.zig:688-695).
+__device__ inline uint32_t spread16(uint32_t v) {
+  v &= 0xffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+// Fast mode's 16-level key (tree.zig:346-364): the square of side
+// max(width, height) (1 when that is not positive) over the points' box,
+// quantised to 65536 steps per axis and interleaved, y above x.
+__device__ inline uint64_t fast_key(double x, double y, const double* b) {
+  const double w = b[2] - b[0], h = b[3] - b[1];
+  double side = (h > w || w != w) ? h : w;  // Zig @max, first operand kept on ties
+  if (!(side > 0)) side = 1;
+  const double scale = 65536.0 / side;
+  double fx = floor((x - b[0]) * scale), fy = floor((y - b[1]) * scale);
+  fx = (65535.0 < fx || fx != fx) ? 65535.0 : fx;
+  fy = (65535.0 < fy || fy != fy) ? 65535.0 : fy;
+  const uint32_t qx = (uint32_t)fx, qy = (uint32_t)fy;
+  return (uint64_t)(spread16(qx) | spread16(qy) << 1);
+}
+
+template <bool FAST>
 __global__ __launch_bounds__(EW_THREADS) void k_keys(const double* __restrict__ xs, const double* __restrict__ ys,
                                                      uint32_t n, const TreeMeta* __restrict__ meta,
                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ perm,
                                                      uint64_t* __restrict__ keys_all, uint32_t* __restrict__ flags) {
   uint32_t i = blockIdx.x * EW_THREADS + threadIdx.x;
   if (i >= n || !meta->valid) return;
-  double e[4] = {meta->extent[0], meta->extent[1], meta->extent[2], meta->extent[3]};
   double x = xs[i], y = ys[i];
   bool valid = !(x != x || y != y);
-  uint64_t key = valid ? exact_key(x, y, e) : 0;
+  uint64_t key = 0;
+  if (valid) {
+    if (FAST) {
+      double b[4] = {meta->box[0], meta->box[1], meta->box[2], meta->box[3]};
+      key = fast_key(x, y, b);
+    } else {
+      double e[4] = {meta->extent[0], meta->extent[1], meta->extent[2], meta->extent[3]};
+      key = exact_key(x, y, e);
+    }
+  }
   if (meta->m == n) {
     keys[i] = key;
     perm[i] = i;
@@ -752,6 +790,38 @@ __global__ __launch_bounds__(EW_THREADS) void k_emit(const uint64_t* __restrict_
 
 // ---------------------------------------------------------------- host side
 
+// Stable sort of (keys, vals)[0 .. *count) by the 8-bit digits first_digit ..
+// first_digit + digits - 1 (digits even, so the result ends in keys/vals).
+// hist holds 256 x ceil(n_max / RS_TILE) counts, glob 256 x digits.
+hipError_t sort_digits(uint32_t* hist, uint32_t* glob, uint64_t* keys, uint32_t* vals, uint64_t* keys_tmp,
+                       uint32_t* vals_tmp, uint32_t n_max, const uint32_t* count, int first_digit, int digits,
+                       hipStream_t stream) {
+  if (digits & 1) return hipErrorInvalidValue;
+  const uint32_t ntiles = ceil_div(n_max > 0 ? n_max : 1, RS_TILE);
+  uint64_t *ka = keys, *kb = keys_tmp;
+  uint32_t *va = vals, *vb = vals_tmp;
+  HIPCHECK(hipMemsetAsync(glob, 0, (size_t)digits * 256 * sizeof(uint32_t), stream));
+  for (int pass = 0; pass < digits; ++pass) {
+    uint32_t* g = glob + 256 * pass;
+    const int shift = 8 * (first_digit + pass);
+    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, count, hist, g, ntiles, shift);
+    hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(RS_THREADS), 0, stream, hist, g, ntiles);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, va, kb, vb, hist, count,
+                       ntiles, shift);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  return hipGetLastError();
+}
+
+hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t n_max, const uint32_t* count_dev, uint32_t* partials,
+                    uint32_t* total_out, hipStream_t stream) {
+  return scan_exclusive(in, out, n_max, count_dev, partials, total_out, nullptr, stream);
+}
+
+size_t sort_hist_words(uint32_t n_max) { return (size_t)256 * ceil_div(n_max > 0 ? n_max : 1, RS_TILE); }
+
+
 template <class T>
 static hipError_t grow(T*& p, size_t n) {
   if (p) HIPCHECK(hipFree(p));
@@ -825,7 +895,7 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
   if (box_blocks > (uint32_t)BOX_BLOCKS) box_blocks = BOX_BLOCKS;
   hipLaunchKernelGGL(k_box, dim3(box_blocks), dim3(BOX_THREADS), 0, stream, x, y, n,
                      (BoxPartial*)t.box_partials, t.meta);
-  hipLaunchKernelGGL(k_keys, dim3(nb), dim3(EW_THREADS), 0, stream, x, y, n, t.meta, t.keys, t.perm, t.keys_all,
+  hipLaunchKernelGGL(k_keys<false>, dim3(nb), dim3(EW_THREADS), 0, stream, x, y, n, t.meta, t.keys, t.perm, t.keys_all,
                      t.flags);
   // Compaction of NaN points; skipped on the device when m == n.
   const uint32_t* nptr = &t.meta->n;
@@ -836,20 +906,8 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
   hipLaunchKernelGGL(k_rest_all, dim3(nb), dim3(EW_THREADS), 0, stream, n, t.meta, t.rest);
   // Radix sort over m = meta->m keys, 8 passes (even: the result ends in keys/perm).
   const uint32_t* mptr = &t.meta->m;
-  const uint32_t ntiles = ceil_div(n > 0 ? n : 1, RS_TILE);
-  uint64_t *ka = t.keys, *kb = t.keys_tmp;
-  uint32_t *va = t.perm, *vb = t.perm_tmp;
-  HIPCHECK(hipMemsetAsync(t.rs_glob, 0, 8 * 256 * sizeof(uint32_t), stream));
-  for (int pass = 0; pass < 8; ++pass) {
-    uint32_t* glob = t.rs_glob + 256 * pass;
-    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, mptr, t.hist, glob, ntiles,
-                       8 * pass);
-    hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(RS_THREADS), 0, stream, t.hist, glob, ntiles);
-    hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, va, kb, vb, t.hist, mptr,
-                       ntiles, 8 * pass);
-    std::swap(ka, kb);
-    std::swap(va, vb);
-  }
+  HIPCHECK(sort_digits(t.hist, t.rs_glob, t.keys, t.perm, t.keys_tmp, t.perm_tmp, n > 0 ? n : 1, mptr, 0, 8,
+                       stream));
   // Key groups.
   HIPCHECK(hipMemsetAsync(t.mixed, 0, (size_t)(n > 0 ? n : 1) * sizeof(uint32_t), stream));
   hipLaunchKernelGGL(k_group_a, dim3(nb), dim3(EW_THREADS), 0, stream, t.keys, t.perm, x, y, t.meta, t.flags,
@@ -879,6 +937,26 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
                        t.mixed_list, t.dleft, t.gend, t.count, t.base, t.deep_stack, t.cells, t.cdepth);
   }
   return hipGetLastError();
+}
+
+hipError_t tree_prepare_fast(TreeBuffers& t, const double* x, const double* y, uint32_t n, hipStream_t stream) {
+  HIPCHECK(tree_reserve(t, n > 0 ? n : 1));
+  const uint32_t nb = ceil_div(n > 0 ? n : 1, EW_THREADS);
+  uint32_t box_blocks = ceil_div(n > 0 ? n : 1, BOX_THREADS * 4);
+  if (box_blocks > (uint32_t)BOX_BLOCKS) box_blocks = BOX_BLOCKS;
+  hipLaunchKernelGGL(k_box, dim3(box_blocks), dim3(BOX_THREADS), 0, stream, x, y, n,
+                     (BoxPartial*)t.box_partials, t.meta);
+  hipLaunchKernelGGL(k_keys<true>, dim3(nb), dim3(EW_THREADS), 0, stream, x, y, n, t.meta, t.keys, t.perm,
+                     t.keys_all, t.flags);
+  const uint32_t* nptr = &t.meta->n;
+  const uint32_t* skip = &t.meta->n;
+  HIPCHECK(scan_exclusive(t.flags, t.offs, n, nptr, t.partials, nullptr, skip, stream));
+  hipLaunchKernelGGL(k_compact, dim3(nb), dim3(EW_THREADS), 0, stream, n, t.meta, t.keys_all, t.flags, t.offs,
+                     t.keys, t.perm, t.rest);
+  hipLaunchKernelGGL(k_rest_all, dim3(nb), dim3(EW_THREADS), 0, stream, n, t.meta, t.rest);
+  // 32-bit keys: 4 passes (sort.zig via tree.zig:363).
+  return sort_digits(t.hist, t.rs_glob, t.keys, t.perm, t.keys_tmp, t.perm_tmp, n > 0 ? n : 1, &t.meta->m, 0, 4,
+                     stream);
 }
 
 }  // namespace d3f

@@ -58,21 +58,23 @@ def check(status):
 
 _RANDOM_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p)
 
-PHASES = ("tree", "accumulate", "traversal", "link", "center", "integrate", "center_sum")
+PHASES = ("tree", "accumulate", "traversal", "link", "center", "integrate", "center_sum", "fmm", "collide")
 
 
 class Stats(ctypes.Structure):
     """d3f_stats of include/d3fgpu.h: per-phase kernel time and launches since
     the last call, and the Barnes-Hut traversal's work counters."""
-    _fields_ = [("ms", ctypes.c_double * 7), ("launches", ctypes.c_uint64 * 7),
+    _fields_ = [("ms", ctypes.c_double * len(PHASES)), ("launches", ctypes.c_uint64 * len(PHASES)),
                 ("visits", ctypes.c_uint64), ("accepts", ctypes.c_uint64), ("pairs", ctypes.c_uint64),
-                ("flagged", ctypes.c_uint64), ("link_levels", ctypes.c_uint64)]
+                ("flagged", ctypes.c_uint64), ("link_levels", ctypes.c_uint64),
+                ("m2l", ctypes.c_uint64), ("p2p_pairs", ctypes.c_uint64), ("fmm_cells", ctypes.c_uint64)]
 
     def as_dict(self):
         return {"ms": {p: self.ms[k] for k, p in enumerate(PHASES)},
                 "launches": {p: int(self.launches[k]) for k, p in enumerate(PHASES)},
                 "visits": int(self.visits), "accepts": int(self.accepts), "pairs": int(self.pairs),
-                "flagged": int(self.flagged), "link_levels": int(self.link_levels)}
+                "flagged": int(self.flagged), "link_levels": int(self.link_levels),
+                "m2l": int(self.m2l), "p2p_pairs": int(self.p2p_pairs), "fmm_cells": int(self.fmm_cells)}
 
 _SIGNATURES = {
     "core_set_random": (None, [_RANDOM_FN, ctypes.c_void_p]),
@@ -125,6 +127,10 @@ class Core:
             self.sim_stats = self.bind("sim_stats", ctypes.c_int32, [ctypes.c_uint32, ctypes.POINTER(Stats)])
             self.core_set_timing = self.bind("core_set_timing", None, [ctypes.c_int])
             self.core_synchronize = self.bind("core_synchronize", ctypes.c_int32, [])
+        if hasattr(self.lib, prefix + "force_direct_sum"):
+            self.force_direct_sum = self.bind("force_direct_sum", ctypes.c_int32,
+                                              [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_void_p,
+                                               ctypes.c_uint32, ctypes.c_void_p])
 
     def stats(self, h):
         """Per-phase kernel times and work counters since the last call."""

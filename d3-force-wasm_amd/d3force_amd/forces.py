@@ -63,11 +63,15 @@ class ManyBody(_Force):
         self._theta2 = 0.81
 
     def __call__(self, alpha):  # manyBody.js:14-17
+        self.push_scalars()
+        self.slot.run(self.nodes, alpha, self.random)
+
+    def push_scalars(self):
+        """theta^2, distanceMin^2 and distanceMax^2 into the core's force."""
         s = self.slot
         s.scalar(THETA2, self._theta2)
         s.scalar(DISTANCE_MIN2, self._distance_min2)
         s.scalar(DISTANCE_MAX2, self._distance_max2)
-        s.run(self.nodes, alpha, self.random)
 
     def _initialize(self):  # manyBody.js:20-27
         if self.nodes is None:
@@ -149,6 +153,14 @@ class Link(_Force):
             return
         v = _values(self._distance, len(self.links.source))
         self.slot.f64(F1, self.size)[:] = v[: self.size]
+
+    def set_links(self, links):  # link.js:98-100, d3's link.links(_)
+        if not hasattr(links, "source"):
+            from .nodes import Links
+            links = Links(*links)
+        self.links = links
+        self._initialize()
+        return self
 
     def iterations(self, k):
         self._iterations = float(k)

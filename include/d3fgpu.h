@@ -104,7 +104,8 @@ int32_t force_run(uint32_t h, uint32_t fh, double alpha);
  * counts its algorithmic work always: non-empty cell visits, accepted
  * far-field cells and leaf pair interactions, summed over nodes. */
 enum { D3F_PHASE_TREE = 0, D3F_PHASE_ACCUMULATE = 1, D3F_PHASE_TRAVERSAL = 2, D3F_PHASE_LINK = 3,
-       D3F_PHASE_CENTER = 4, D3F_PHASE_INTEGRATE = 5, D3F_PHASE_CENTER_SUM = 6, D3F_PHASES = 7 };
+       D3F_PHASE_CENTER = 4, D3F_PHASE_INTEGRATE = 5, D3F_PHASE_CENTER_SUM = 6, D3F_PHASE_FMM = 7,
+       D3F_PHASE_COLLIDE = 8, D3F_PHASES = 9 };
 /* D3F_PHASE_CENTER is the center force on the tick's stream (including any
  * wait for its sums); D3F_PHASE_CENTER_SUM is the serial sum kernel itself,
  * which runs on a second stream overlapping the tick. */
@@ -114,8 +115,19 @@ typedef struct {
   uint64_t visits, accepts, pairs;  /* Barnes-Hut work since the last call */
   uint64_t flagged;             /* many-body nodes replayed for jiggle */
   uint64_t link_levels;         /* levels of the last exact link schedule */
+  uint64_t m2l, p2p_pairs;      /* fast many-body: M2L cell pairs, P2P point pairs */
+  uint64_t fmm_cells;           /* cells of the last fast tree */
 } d3f_stats;
 int32_t sim_stats(uint32_t h, d3f_stats* out);
+/* FP64 direct sum for the fast mode's error measurement (no reference
+ * counterpart; the reference's fmm.zig:946-971 test and
+ * test/fast/error-test.js:44-58 compute the same sum on the CPU): for each
+ * sample[q] < n, out[2q] and out[2q+1] receive sum over j != sample[q] of
+ * d3's pair term x * strength[j] * alpha / l (coincident pairs and NaN terms
+ * skipped, l < distanceMin^2 -> sqrt(distanceMin^2 * l), l >= distanceMax^2
+ * skipped) on the current positions, using many-body force fh's strengths
+ * and distance scalars. sample and out are host pointers. */
+int32_t force_direct_sum(uint32_t h, uint32_t fh, double alpha, const uint32_t* sample, uint32_t k, double* out);
 void core_set_timing(int on);
 /* Waits for all work the library queued on the device (benchmark brackets). */
 int32_t core_synchronize(void);

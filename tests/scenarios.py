@@ -147,3 +147,57 @@ def pinned(n, fraction, seed=7):
     fx = np.array([1000 * (r() - 0.5) for _ in idx])
     fy = np.array([1000 * (r() - 0.5) for _ in idx])
     return idx, fx, fy
+
+
+def ba_graph(n, m0=3, k=3, seed=1):
+    """Barabasi-Albert graph: m0 fully connected seed nodes, then each new
+    node v links to k distinct earlier nodes chosen with probability
+    proportional to degree (uniform over the endpoints of the earlier
+    links). Returns (source, target) with source = the new node. numpy
+    Generator(PCG64) seeded with `seed`."""
+    rng = np.random.default_rng(seed)
+    s0 = [i for i in range(m0) for j in range(i + 1, m0)]
+    t0 = [j for i in range(m0) for j in range(i + 1, m0)]
+    e0 = len(s0)
+    if n <= m0:
+        keep = [(a, b) for a, b in zip(s0, t0) if b < n]
+        return (np.array([a for a, _ in keep], dtype=np.int64), np.array([b for _, b in keep], dtype=np.int64))
+    new = np.repeat(np.arange(m0, n, dtype=np.int64), k)
+    e = np.arange(e0, e0 + len(new), dtype=np.int64)
+    before = e0 + k * (new - m0)           # links that exist when node `new` arrives
+    src = np.concatenate([np.array(s0, dtype=np.int64), new])
+    tgt = np.empty(len(src), dtype=np.int64)
+    tgt[:e0] = t0
+
+    def resolve(r):
+        # endpoint r of the link list: even = source of link r/2, odd = target
+        out = np.empty(len(r), dtype=np.int64)
+        idx = np.arange(len(r))
+        r = r.copy()
+        while len(idx):
+            even = r % 2 == 0
+            out[idx[even]] = src[r[even] // 2]
+            idx, r = idx[~even], r[~even]
+            link = (r - 1) // 2
+            init = link < e0
+            out[idx[init]] = tgt[link[init]]
+            idx, r = idx[~init], link[~init]
+            r = draws[r - e0]
+        return out
+
+    draws = (rng.random(len(new)) * (2 * before)).astype(np.int64)
+    tgt[e0:] = resolve(draws)
+    # Redraw repeated targets of one new node (links are distinct pairs).
+    for _ in range(64):
+        t = tgt[e0:].reshape(-1, k)
+        dup = np.zeros(t.shape, dtype=bool)
+        for a in range(k):
+            for b in range(a):
+                dup[:, a] |= t[:, a] == t[:, b]
+        dup = dup.reshape(-1)
+        if not dup.any():
+            break
+        which = np.nonzero(dup)[0]
+        draws[which] = (rng.random(len(which)) * (2 * before[which])).astype(np.int64)
+        tgt[e0 + which] = resolve(draws[which])
+    return src, tgt

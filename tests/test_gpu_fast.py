@@ -1,0 +1,380 @@
+"""GPU fast mode (precision "fast", n >= 2000) against the oracle's FP64
+restatement of the reference's fast kernels.
+
+- Link sweep and grid collide run in FP64 with the reference's order of
+  operations: bit-identical (Object.is) after every tick.
+- The FMM reproduces the reference's tree, geometry and opening test in FP64,
+  so its interaction lists equal the oracle's: the same number of M2L cell
+  pairs, P2P point pairs and cells. Its expansions, M2L and near field run in
+  FP32, so its velocities are compared by relative RMS error: against the
+  oracle's FP64 FMM (tolerance below) and against an FP64 direct sum with
+  d3's pair formula (fmm.zig:946-994, test/fast/error-test.js:44-70), where
+  it must stay within the reference's own gate (1e-3) and close to the
+  oracle's FP64 error.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import d3force_amd as d3
+from parity import same
+from scenarios import FAST_LAYOUTS, fast_scenario, generator, random_graph
+
+pytestmark = pytest.mark.gpu
+
+# FP32 expansions, offsets and near field against the FP64 FMM: relative RMS
+# of the many-body velocities after one tick.
+FP32_RMS = 2e-6
+
+
+def nodes_of(xs, ys):
+    return d3.Nodes(len(xs), x=np.array(xs, dtype=np.float64), y=np.array(ys, dtype=np.float64))
+
+
+def fast(sim, **options):
+    sim.precision("fast", options or None)
+    return sim
+
+
+def rms(a, b):
+    """sqrt(sum |a - b|^2 / sum |b|^2) over the rows' (vx, vy)."""
+    e2 = np.sum((a[2] - b[2]) ** 2 + (a[3] - b[3]) ** 2)
+    r2 = np.sum(b[2] ** 2 + b[3] ** 2)
+    return math.sqrt(e2 / r2)
+
+
+# ---------------------------------------------------------------- link sweep
+
+def link_case(name):
+    xs, ys, src, tgt = fast_scenario(name)
+    return xs, ys, (src, tgt)
+
+
+def special_links(n=3000, seed=3):
+    """Duplicates, self-links, coincident endpoints (zero deltas jiggle by
+    hash) and a node with a NaN position."""
+    r = generator(seed)
+    xs = np.array([round(40 * r()) for _ in range(n)], dtype=np.float64)
+    ys = np.array([round(40 * r()) for _ in range(n)], dtype=np.float64)
+    xs[17] = np.nan
+    src = np.array([math.floor(r() * n) for _ in range(3 * n)])
+    tgt = np.array([math.floor(r() * n) for _ in range(3 * n)])
+    src[:50] = tgt[:50]           # self-links
+    src[50:100] = src[100:150]    # duplicates
+    tgt[50:100] = tgt[100:150]
+    return xs, ys, (src, tgt)
+
+
+def hub_links(n=4000):
+    """A Barabasi-Albert graph plus a star: rows of thousands of entries (the
+    long-row kernel) next to ordinary ones."""
+    from scenarios import ba_graph
+    xs, ys = fast_scenario("uniform 4000")[:2]
+    src, tgt = ba_graph(n, seed=4)
+    star = np.arange(1, 2500)
+    return xs, ys, (np.concatenate([src, np.zeros(len(star), dtype=np.int64)]), np.concatenate([tgt, star]))
+
+
+@pytest.mark.parametrize("name", list(FAST_LAYOUTS) + ["special", "hubs"])
+@pytest.mark.parametrize("iterations", [1, 2])
+def test_fast_link_sweep_bit_identical(gpu_core, oracle_core, name, iterations):
+    print("fast link", name, iterations, flush=True)
+    cases = {"special": special_links, "hubs": hub_links}
+    xs, ys, links = cases[name]() if name in cases else link_case(name)
+    sims = []
+    for core in (oracle_core, gpu_core):
+        nodes = nodes_of(xs, ys)
+        sim = fast(d3.forceSimulation(nodes, core=core)
+                   .force("link", d3.forceLink(links).iterations(iterations)).stop())
+        sims.append((sim, nodes))
+    for t in range(12):
+        for sim, _ in sims:
+            sim.tick()
+        assert same(sims[0][1].state(), sims[1][1].state()), "%s tick %d" % (name, t + 1)
+
+
+def test_fast_link_rows_follow_link_changes(gpu_core, oracle_core):
+    # The adjacency rows are rebuilt when the links or the node count change
+    # (force.zig:41-42, 81).
+    xs, ys, links = link_case("uniform 4000")
+    out = []
+    for core in (oracle_core, gpu_core):
+        nodes = nodes_of(xs, ys)
+        link = d3.forceLink(links)
+        sim = fast(d3.forceSimulation(nodes, core=core).force("link", link).stop())
+        sim.tick(3)
+        src, tgt = random_graph(4000, 5000, seed=9)
+        link.set_links((src, tgt))
+        sim.tick(3)
+        out.append(nodes.state())
+    assert same(out[0], out[1])
+
+
+# ---------------------------------------------------------------- grid collide
+
+def collide_case(name, seed=5):
+    xs, ys, _, _ = fast_scenario(name)
+    r = generator(seed)
+    radii = np.array([2 + 4 * r() for _ in range(len(xs))])
+    return xs, ys, radii
+
+
+@pytest.mark.parametrize("name", list(FAST_LAYOUTS) + ["coincident", "nan"])
+@pytest.mark.parametrize("iterations", [1, 3])
+def test_fast_collide_bit_identical(gpu_core, oracle_core, name, iterations):
+    print("fast collide", name, iterations, flush=True)
+    if name == "coincident":
+        n = 2500
+        xs = np.array([float(i % 7) for i in range(n)])
+        ys = np.array([float(i % 5) for i in range(n)])
+        radii = np.full(n, 3.0)
+    elif name == "nan":
+        xs, ys, radii = collide_case("uniform 4000")
+        xs = xs.copy()
+        xs[::97] = np.nan
+        radii = radii.copy()
+        radii[5] = np.nan
+    else:
+        xs, ys, radii = collide_case(name)
+    sims = []
+    for core in (oracle_core, gpu_core):
+        nodes = nodes_of(xs, ys)
+        sim = fast(d3.forceSimulation(nodes, core=core)
+                   .force("collide", d3.forceCollide(radii).iterations(iterations)).stop())
+        sims.append((sim, nodes))
+    for t in range(10):
+        for sim, _ in sims:
+            sim.tick()
+        assert same(sims[0][1].state(), sims[1][1].state()), "%s tick %d" % (name, t + 1)
+
+
+def test_fast_collide_tiny_radii_coarsen_the_grid(gpu_core, oracle_core):
+    # r_max far below the extent: the cell size doubles until gx * gy <= 4n + 16.
+    xs, ys, _ = collide_case("uniform 4000")
+    xs = xs * 1e4
+    radii = np.full(len(xs), 0.01)
+    radii[::5] = 1e-6
+    out = []
+    for core in (oracle_core, gpu_core):
+        nodes = nodes_of(xs, ys)
+        sim = fast(d3.forceSimulation(nodes, core=core).force("collide", d3.forceCollide(radii)).stop())
+        sim.tick(3)
+        out.append(nodes.state())
+    assert same(out[0], out[1])
+
+
+# ---------------------------------------------------------------- FMM
+
+def many_body_once(core, xs, ys, **options):
+    """One tick of manyBody alone with velocityDecay 0 (vx, vy = the force),
+    the form of test/fast/error-test.js:37-42. Returns the state and the
+    core's FMM counters."""
+    nodes = nodes_of(xs, ys)
+    sim = fast(d3.forceSimulation(nodes, core=core).velocity_decay(0)
+               .force("charge", d3.forceManyBody()).stop(), **options)
+    sim.tick()
+    return nodes.state(), sim
+
+
+def fmm_counts(core, sim):
+    h = d3.core.context(sim.nodes()).h
+    if hasattr(core, "fmm_stats"):
+        import ctypes
+        st = (ctypes.c_uint64 * 3)()
+        core.fmm_stats(h, st)
+        return int(st[0]), int(st[1]), int(st[2])
+    s = core.stats(h)
+    return s["m2l"], s["p2p_pairs"], s["fmm_cells"]
+
+
+def direct_sum(xs, ys, alpha, sample, strength=-30.0):
+    """FP64 direct sum with d3's pair formula (distanceMin 1, no distanceMax;
+    coincident pairs skipped as error-test.js:52 does)."""
+    fx = np.empty(len(sample))
+    fy = np.empty(len(sample))
+    for k, i in enumerate(sample):
+        dx = xs - xs[i]
+        dy = ys - ys[i]
+        l = dx * dx + dy * dy
+        keep = l > 0
+        dx, dy, l = dx[keep], dy[keep], l[keep]
+        l = np.where(l < 1, np.sqrt(l), l)
+        w = strength * alpha / l
+        fx[k] = np.sum(dx * w)
+        fy[k] = np.sum(dy * w)
+    return fx, fy
+
+
+def rms_vs_direct(state, ref, sample, zero_delta=False):
+    ex = 0 if zero_delta else state[2][sample] - ref[0]
+    ey = state[3][sample] - ref[1]
+    return math.sqrt(np.sum(ex * ex + ey * ey) / np.sum(ref[0] ** 2 + ref[1] ** 2))
+
+
+FMM_CASES = {name: (lambda name=name: fast_scenario(name)[:2]) for name in FAST_LAYOUTS}
+FMM_CASES["phyllotaxis 100000"] = lambda: d3.simulation.phyllotaxis(100000)
+
+
+@pytest.mark.parametrize("name", list(FMM_CASES))
+def test_fmm_interaction_lists_and_accuracy(gpu_core, oracle_core, name):
+    print("fmm", name, flush=True)
+    xs, ys = FMM_CASES[name]()
+    a, sa = many_body_once(oracle_core, xs, ys)
+    b, sb = many_body_once(gpu_core, xs, ys)
+    assert fmm_counts(oracle_core, sa) == fmm_counts(gpu_core, sb), "interaction lists"
+    err = rms(b, a)
+    print("  rms vs FP64 FMM %.2e" % err, flush=True)
+    assert err < FP32_RMS
+    alpha = sa.alpha()
+    rng = np.random.default_rng(0)
+    sample = rng.choice(len(xs), min(len(xs), 2000), replace=False)
+    ref = direct_sum(np.asarray(xs), np.asarray(ys), alpha, sample)
+    grid = name.startswith("grid")
+    e_oracle = rms_vs_direct(a, ref, sample, grid)
+    e_gpu = rms_vs_direct(b, ref, sample, grid)
+    print("  rms vs direct: oracle %.2e gpu %.2e" % (e_oracle, e_gpu), flush=True)
+    assert e_gpu <= 1e-3
+    assert e_gpu <= e_oracle + 2 * FP32_RMS
+
+
+@pytest.mark.parametrize("options", [{"order": 4}, {"order": 12}, {"order": 24}, {"opening": 1.1},
+                                     {"opening": 3.0}])
+def test_fmm_orders_and_openings(gpu_core, oracle_core, options):
+    xs, ys = fast_scenario("three clusters 6000")[:2]
+    a, sa = many_body_once(oracle_core, xs, ys, **options)
+    b, sb = many_body_once(gpu_core, xs, ys, **options)
+    assert fmm_counts(oracle_core, sa) == fmm_counts(gpu_core, sb)
+    err = rms(b, a)
+    print("  %s: rms vs FP64 FMM %.2e" % (options, err), flush=True)
+    assert err < FP32_RMS
+
+
+def test_fmm_distance_ring(gpu_core, oracle_core):
+    # distanceMax finite and distanceMin 0 both change the opening test
+    # (fmm.zig:423-431) and the pair term.
+    xs, ys = fast_scenario("uniform 4000")[:2]
+    out = []
+    for core in (oracle_core, gpu_core):
+        nodes = nodes_of(xs, ys)
+        sim = fast(d3.forceSimulation(nodes, core=core).velocity_decay(0)
+                   .force("charge", d3.forceManyBody().distance_max(120).distance_min(0)).stop())
+        sim.tick()
+        out.append((nodes.state(), fmm_counts(core, sim)))
+    assert out[0][1] == out[1][1]
+    assert rms(out[1][0], out[0][0]) < FP32_RMS
+
+
+def test_fmm_declines_a_non_finite_extent(gpu_core, oracle_core):
+    # An infinite coordinate makes the extent non-finite: the exact kernel
+    # runs instead (fmm.zig:255-257), so the result is bit-identical.
+    xs, ys = fast_scenario("uniform 4000")[:2]
+    xs = xs.copy()
+    xs[3] = np.inf
+    out = []
+    for core in (oracle_core, gpu_core):
+        nodes = nodes_of(xs, ys)
+        sim = fast(d3.forceSimulation(nodes, core=core).force("charge", d3.forceManyBody()).stop())
+        sim.tick()
+        out.append(nodes.state())
+    assert same(out[0], out[1])
+
+
+def test_fmm_skips_nan_nodes(gpu_core, oracle_core):
+    # d3 places NaN starting positions (simulation.js:91-105), so NaN
+    # positions come from fx = NaN: integrate copies it into x at the first
+    # tick, and from the second tick on the FMM leaves those nodes out of the
+    # tree (tree.zig:275-279): their y velocity gets nothing.
+    xs, ys = fast_scenario("uniform 4000")[:2]
+    out = []
+    for core in (oracle_core, gpu_core):
+        nodes = nodes_of(xs, ys)
+        idx = np.arange(0, len(xs), 50)
+        nodes.fix(fx=np.full(len(idx), np.nan), where=idx)
+        sim = fast(d3.forceSimulation(nodes, core=core).velocity_decay(0)
+                   .force("charge", d3.forceManyBody()).stop())
+        sim.tick()
+        vy1 = nodes.vy[idx].copy()
+        sim.tick()
+        out.append((nodes.state(), vy1, nodes.vy[idx].copy()))
+    (a, a1, a2), (b, b1, b2) = out
+    assert same(np.isnan(a), np.isnan(b))
+    assert same(a1, a2) and same(b1, b2)  # nothing added at the second tick
+    finite = ~np.isnan(a[0])
+    assert rms(b[:, finite], a[:, finite]) < FP32_RMS
+
+
+def _tick_drift(gpu_core, oracle_core, forces, ticks, name="three clusters 6000"):
+    xs, ys, src, tgt = fast_scenario(name)
+    sims = []
+    for core in (oracle_core, gpu_core):
+        nodes = nodes_of(xs, ys)
+        sim = d3.forceSimulation(nodes, core=core)
+        for key, make in forces(src, tgt):
+            sim.force(key, make())
+        sims.append((fast(sim.stop()), nodes))
+    drift = []
+    for _ in range(ticks):
+        for sim, _ in sims:
+            sim.tick()
+        a, b = sims[0][1].state(), sims[1][1].state()
+        drift.append(math.sqrt(np.sum((a[0] - b[0]) ** 2 + (a[1] - b[1]) ** 2) / np.sum(a[0] ** 2 + a[1] ** 2)))
+    print("  relative position rms by tick", " ".join("%.1e" % d for d in drift), flush=True)
+    return drift
+
+
+def test_fast_tick_starts_at_fp32_level(gpu_core, oracle_core):
+    # The whole fast tick (FMM + link sweep + grid collide + center): the
+    # first ticks differ from the oracle only by the FMM's FP32 rounding.
+    # Collisions in dense clusters are discontinuous in the positions, so the
+    # layouts then separate chaotically (about x1.7 per tick here); the
+    # error test's accuracy gate is per tick.
+    drift = _tick_drift(gpu_core, oracle_core, lambda s, t: [
+        ("charge", d3.forceManyBody), ("link", lambda: d3.forceLink((s, t))),
+        ("collide", lambda: d3.forceCollide(3)), ("center", d3.forceCenter)], 6)
+    assert drift[0] < 3e-7
+    assert drift[2] < 3e-6
+
+
+def test_fast_tick_without_collide_tracks_the_oracle(gpu_core, oracle_core):
+    # Without collide the separation grows smoothly until a point crosses a
+    # cell boundary of the fast tree in one run and not the other: from then
+    # on the two runs use different interaction lists, whose results differ by
+    # the FMM's truncation error (about 1e-6 of the force).
+    drift = _tick_drift(gpu_core, oracle_core, lambda s, t: [
+        ("charge", d3.forceManyBody), ("link", lambda: d3.forceLink((s, t))), ("center", d3.forceCenter)], 20)
+    assert drift[0] < 3e-7
+    assert drift[4] < 1e-5
+    assert drift[-1] < 1e-3
+
+
+def test_direct_sum_kernel_matches_numpy(gpu_core):
+    # The library's FP64 direct sum (the error measurement's reference) against
+    # the same sum in numpy, with a zero delta, a NaN node and distanceMin.
+    import ctypes
+    xs, ys = fast_scenario("grid 2500 with exact zero deltas")[:2]
+    xs = xs.copy()
+    ys = ys.copy()
+    xs[7] = xs[8]
+    ys[7] = ys[8]
+    nodes = nodes_of(xs, ys)
+    charge = d3.forceManyBody().strength(lambda i: -30.0 - (i % 7))
+    sim = fast(d3.forceSimulation(nodes, core=gpu_core).force("charge", charge).stop())
+    ctx = sim.begin()
+    charge.push_scalars()
+    sample = np.array([0, 7, 8, 1234, 2499], dtype=np.uint32)
+    out = np.empty(2 * len(sample))
+    from d3force_amd.core import check
+    check(gpu_core.force_direct_sum(ctx.h, charge.slot.fh, 0.5, sample.ctypes.data_as(ctypes.c_void_p), len(sample),
+                                    out.ctypes.data_as(ctypes.c_void_p)))
+    sim.end(ctx)
+    s = -30.0 - (np.arange(len(xs)) % 7)
+    for q, i in enumerate(sample):
+        dx, dy = xs - xs[i], ys - ys[i]
+        l = dx * dx + dy * dy
+        keep = (l > 0) & (np.arange(len(xs)) != i)
+        l = np.where(l < 1, np.sqrt(l), l)
+        w = np.where(keep, s * 0.5 / np.where(keep, l, 1), 0)
+        assert np.isclose(out[2 * q], np.sum(dx * w), rtol=1e-12, atol=1e-9)
+        assert np.isclose(out[2 * q + 1], np.sum(dy * w), rtol=1e-12, atol=1e-9)
