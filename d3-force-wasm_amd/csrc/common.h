@@ -114,7 +114,9 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
 // meta->m), NaN nodes into rest. No synchronisation.
 hipError_t tree_prepare_fast(TreeBuffers& t, const double* x, const double* y, uint32_t n, hipStream_t stream);
 // Stable LSD radix sort of (keys, vals)[0 .. *count) on the 8-bit digits
-// [first_digit, first_digit + digits), digits even; n_max bounds *count.
+// [first_digit, first_digit + digits); n_max bounds *count. The result is in
+// keys / vals (after an odd number of passes it is copied back from the
+// ping-pong buffers).
 hipError_t sort_digits(uint32_t* hist, uint32_t* glob, uint64_t* keys, uint32_t* vals, uint64_t* keys_tmp,
                        uint32_t* vals_tmp, uint32_t n_max, const uint32_t* count, int first_digit, int digits,
                        hipStream_t stream);
@@ -186,20 +188,34 @@ struct GridParams {
 struct GridBuffers {
   double *px = nullptr, *py = nullptr;  // x + vx, y + vy
   uint32_t *cell_of = nullptr, *start = nullptr;
+  double4* cand = nullptr;              // binned nodes: x, y, radius, node bits
   void* partials = nullptr;
   GridParams* params = nullptr;
 };
 size_t grid_partials_bytes();
-// Adjacency rows of the links (link_fast.zig:105-130): row[n + 1], adj[2m],
-// and the nodes with long rows, heavy[*heavy_count] (heavy_count is a host
-// pointer; the call synchronises).
-hipError_t link_csr(const uint32_t* src, const uint32_t* tgt, uint32_t m, uint32_t n, uint32_t* row, uint32_t* adj,
-                    uint32_t* heavy, uint32_t* heavy_count, const SortBuffers& sb, hipStream_t stream);
+// One adjacency entry in row order: the other node, the edge (with bit 31
+// set on the target side), and the edge's strength, distance and bias
+// factor (b on the target side, 1 - b on the source side).
+struct LinkEntry {
+  uint32_t j, e;
+  double strength, distance, bf;
+};
+static_assert(sizeof(LinkEntry) == 32, "link entry layout");
+struct LinkCsrArgs {
+  const uint32_t *src, *tgt;
+  const double *strength, *distance, *bias;
+  uint32_t m, n;
+  uint32_t *row, *adj, *owner, *rows_long;  // row[n + 1], adj[2m], owner[2m], rows_long[n]
+  LinkEntry* entries;                       // [2m]
+  double* terms;                            // [4m] scratch of a sweep
+};
+// Adjacency rows of the links (link_fast.zig:105-130) and their packed
+// entries; *long_count (host) receives the number of rows longer than the
+// per-thread limit. Synchronises.
+hipError_t link_csr(const LinkCsrArgs& a, const SortBuffers& sb, uint32_t* long_count, hipStream_t stream);
 // One Jacobi sweep over the rows on px = x + vx (link_fast.zig:60-99).
-hipError_t link_sweep(uint32_t n, const uint32_t* row, const uint32_t* adj, const uint32_t* heavy, uint32_t heavy_count,
-                      const uint32_t* src, const uint32_t* tgt, const double* strength, const double* distance,
-                      const double* bias, const double* px, const double* py, double* vx, double* vy, double alpha,
-                      uint64_t seed, hipStream_t stream);
+hipError_t link_sweep(const LinkCsrArgs& a, uint32_t long_count, const double* px, const double* py, double* vx,
+                      double* vy, double alpha, uint64_t seed, hipStream_t stream);
 // One iteration of the uniform-grid collide (grid.zig:221-285).
 hipError_t grid_collide_iteration(const double* x, const double* y, double* vx, double* vy, const double* radii,
                                   uint32_t n, double strength, uint64_t seed, const GridBuffers& g,
@@ -229,6 +245,7 @@ struct FmmParams {
   uint32_t front[FMM_MAX_STEPS + 2];       // dual-tree frontier sizes
   uint32_t m2l, p2p;                       // list lengths
   uint32_t list_overflow;
+  uint32_t unsorted;                       // target segments too long to order by source
   unsigned long long pairs;                // point pairs of the P2P list
 };
 
@@ -243,10 +260,10 @@ struct FmmBuffers {
   double *px = nullptr, *py = nullptr, *ux = nullptr, *uy = nullptr;
   float* ps = nullptr;
   uint2 *front_a = nullptr, *front_b = nullptr;
-  uint64_t *m2l = nullptr, *m2l_tmp = nullptr, *p2p = nullptr, *p2p_tmp = nullptr;
-  uint32_t *seg_vals = nullptr, *seg_vals_tmp = nullptr;
-  uint32_t *m2l_lo = nullptr, *m2l_hi = nullptr, *p2p_lo = nullptr, *p2p_hi = nullptr;
-  uint32_t* hist = nullptr;
+  uint64_t *m2l = nullptr, *p2p = nullptr;         // target << nbits | source, as emitted
+  uint32_t *m2l_src = nullptr, *p2p_src = nullptr;  // sources grouped by target
+  uint32_t *m2l_off = nullptr, *m2l_cnt = nullptr, *p2p_off = nullptr, *p2p_cnt = nullptr, *cursor = nullptr;
+  uint32_t* scan_partials = nullptr;
   FmmParams* params = nullptr;       // device
   FmmParams* params_host = nullptr;  // pinned
 };

@@ -12,7 +12,7 @@ namespace {
 constexpr int EW = 256;
 constexpr uint32_t TARGET_BIT = 0x80000000u;  // link_fast.zig:20
 constexpr uint32_t NO_CELL = 0xffffffffu;     // grid.zig:58
-constexpr uint32_t HEAVY_ROW = 64;            // rows longer than this get a wave
+constexpr uint32_t LONG_ROW = 64;             // rows longer than this are summed by a wave
 
 // ---------------------------------------------------------------- link sweep
 
@@ -52,92 +52,114 @@ __global__ __launch_bounds__(EW) void k_csr_rows(const uint64_t* __restrict__ ke
 // One entry of node i's row (link_fast.zig:70-97) as the signed term its
 // velocity adds: vx -= x * b on the target side is vx += -(x * b), which
 // rounds identically, so a row's terms can be formed in parallel and added
-// in row order.
-__device__ inline void link_term(uint32_t entry, const uint32_t* __restrict__ src, const uint32_t* __restrict__ tgt,
-                                 const double* __restrict__ strength, const double* __restrict__ distance,
-                                 const double* __restrict__ bias, const double* __restrict__ px,
-                                 const double* __restrict__ py, double pxi, double pyi, double alpha, uint64_t seed,
-                                 double& tx, double& ty) {
-  const uint32_t e = entry & ~TARGET_BIT;
-  const bool is_target = (entry & TARGET_BIT) != 0;
-  const uint32_t j = is_target ? src[e] : tgt[e];
-  double x = is_target ? pxi - px[j] : px[j] - pxi;
-  double y = is_target ? pyi - py[j] : py[j] - pyi;
+// in row order. bf is b on the target side and 1 - b on the source side.
+__device__ inline void link_term(const LinkEntry& le, const double* __restrict__ px, const double* __restrict__ py,
+                                 double pxi, double pyi, double alpha, uint64_t seed, double& tx, double& ty) {
+  const uint32_t e = le.e & ~TARGET_BIT;
+  const bool is_target = (le.e & TARGET_BIT) != 0;
+  double x = is_target ? pxi - px[le.j] : px[le.j] - pxi;
+  double y = is_target ? pyi - py[le.j] : py[le.j] - pyi;
   if (x == 0 || x != x) x = hash_jiggle(seed, e, 0, 0);
   if (y == 0 || y != y) y = hash_jiggle(seed, e, 0, 1);
   double l = __builtin_sqrt(x * x + y * y);
-  l = (l - distance[e]) / l * alpha * strength[e];
+  l = (l - le.distance) / l * alpha * le.strength;
   x *= l;
   y *= l;
-  const double b = bias[e];
   if (is_target) {
-    tx = -(x * b);
-    ty = -(y * b);
+    tx = -(x * le.bf);
+    ty = -(y * le.bf);
   } else {
-    tx = x * (1 - b);
-    ty = y * (1 - b);
+    tx = x * le.bf;
+    ty = y * le.bf;
   }
 }
 
-// link_fast.zig:60-99: node i walks its row on the snapshot px = x + vx and
-// applies its own side of each spring. Rows longer than HEAVY_ROW are left
-// to k_link_sweep_heavy.
-__global__ __launch_bounds__(EW) void k_link_sweep(uint32_t n, const uint32_t* __restrict__ row,
-                                                   const uint32_t* __restrict__ adj,
-                                                   const uint32_t* __restrict__ src,
-                                                   const uint32_t* __restrict__ tgt,
-                                                   const double* __restrict__ strength,
-                                                   const double* __restrict__ distance,
-                                                   const double* __restrict__ bias, const double* __restrict__ px,
-                                                   const double* __restrict__ py, double* __restrict__ vxs,
-                                                   double* __restrict__ vys, double alpha, uint64_t seed) {
+// Each adjacency entry's parameters in row order (rebuilt with the rows).
+__global__ __launch_bounds__(EW) void k_csr_pack(uint32_t total, const uint64_t* __restrict__ keys,
+                                                 const uint32_t* __restrict__ adj, const uint32_t* __restrict__ src,
+                                                 const uint32_t* __restrict__ tgt,
+                                                 const double* __restrict__ strength,
+                                                 const double* __restrict__ distance,
+                                                 const double* __restrict__ bias, uint32_t* __restrict__ owner,
+                                                 LinkEntry* __restrict__ entries) {
+  const uint32_t a = blockIdx.x * EW + threadIdx.x;
+  if (a >= total) return;
+  const uint32_t entry = adj[a];
+  const uint32_t e = entry & ~TARGET_BIT;
+  const bool is_target = (entry & TARGET_BIT) != 0;
+  const double b = bias[e];
+  entries[a] = LinkEntry{is_target ? src[e] : tgt[e], entry, strength[e], distance[e], is_target ? b : 1 - b};
+  owner[a] = (uint32_t)keys[a];
+}
+
+// Phase 1 of a sweep (link_fast.zig:60-99 on the snapshot px = x + vx):
+// every adjacency entry's term for its row's node, all in parallel.
+__global__ __launch_bounds__(EW) void k_link_terms(uint32_t total, const uint32_t* __restrict__ owner,
+                                                   const LinkEntry* __restrict__ entries,
+                                                   const double* __restrict__ px, const double* __restrict__ py,
+                                                   double2* __restrict__ terms, double alpha, uint64_t seed) {
+  const uint32_t a = blockIdx.x * EW + threadIdx.x;
+  if (a >= total) return;
+  const uint32_t i = owner[a];
+  double tx, ty;
+  link_term(entries[a], px, py, px[i], py[i], alpha, seed, tx, ty);
+  terms[a] = make_double2(tx, ty);
+}
+
+// Phase 2: each node adds its row's terms in row order, as the serial walk
+// does. Rows longer than LONG_ROW are left to k_link_rows_long.
+__global__ __launch_bounds__(EW) void k_link_rows(uint32_t n, const uint32_t* __restrict__ row,
+                                                  const double2* __restrict__ terms, double* __restrict__ vxs,
+                                                  double* __restrict__ vys) {
   const uint32_t i = blockIdx.x * EW + threadIdx.x;
   if (i >= n) return;
   const uint32_t a0 = row[i], a1 = row[i + 1];
-  if (a0 == a1 || a1 - a0 > HEAVY_ROW) return;
+  if (a0 == a1 || a1 - a0 > LONG_ROW) return;
   double vx = vxs[i], vy = vys[i];
-  const double pxi = px[i], pyi = py[i];
   for (uint32_t a = a0; a < a1; ++a) {
-    double tx, ty;
-    link_term(adj[a], src, tgt, strength, distance, bias, px, py, pxi, pyi, alpha, seed, tx, ty);
-    vx += tx;
-    vy += ty;
+    const double2 t = terms[a];
+    vx += t.x;
+    vy += t.y;
   }
   vxs[i] = vx;
   vys[i] = vy;
 }
 
-// A long row (a hub of the graph): the wave forms 64 terms at a time and
-// lane 0 adds them in row order.
-__global__ __launch_bounds__(64) void k_link_sweep_heavy(const uint32_t* __restrict__ heavy,
-                                                         const uint32_t* __restrict__ row,
-                                                         const uint32_t* __restrict__ adj,
-                                                         const uint32_t* __restrict__ src,
-                                                         const uint32_t* __restrict__ tgt,
-                                                         const double* __restrict__ strength,
-                                                         const double* __restrict__ distance,
-                                                         const double* __restrict__ bias,
-                                                         const double* __restrict__ px, const double* __restrict__ py,
-                                                         double* __restrict__ vxs, double* __restrict__ vys,
-                                                         double alpha, uint64_t seed) {
-  __shared__ double tx[64], ty[64];
-  const uint32_t i = heavy[blockIdx.x];
+// A long row (a hub of the graph): the wave loads 64 terms at a time into
+// LDS, the next chunk's loads in flight while lane 0 adds the current one in
+// row order.
+__global__ __launch_bounds__(64) void k_link_rows_long(const uint32_t* __restrict__ rows_long,
+                                                       const uint32_t* __restrict__ row,
+                                                       const double2* __restrict__ terms, double* __restrict__ vxs,
+                                                       double* __restrict__ vys) {
+  __shared__ double2 buf[64];
+  const uint32_t i = rows_long[blockIdx.x];
   const int lane = threadIdx.x;
   const uint32_t a0 = row[i], a1 = row[i + 1];
-  const double pxi = px[i], pyi = py[i];
   double vx = vxs[i], vy = vys[i];
+  double2 next = a0 + lane < a1 ? terms[a0 + lane] : make_double2(0, 0);
   for (uint32_t base = a0; base < a1; base += 64) {
-    const uint32_t a = base + lane;
-    double x = 0, y = 0;
-    if (a < a1) link_term(adj[a], src, tgt, strength, distance, bias, px, py, pxi, pyi, alpha, seed, x, y);
-    tx[lane] = x;
-    ty[lane] = y;
+    buf[lane] = next;
     __syncthreads();
+    const uint32_t nb = base + 64 + lane;
+    if (nb < a1) next = terms[nb];
     if (lane == 0) {
       const uint32_t k = a1 - base < 64 ? a1 - base : 64;
-      for (uint32_t q = 0; q < k; ++q) {
-        vx += tx[q];
-        vy += ty[q];
+      uint32_t q = 0;
+      for (; q + 4 <= k; q += 4) {
+        const double2 t0 = buf[q], t1 = buf[q + 1], t2 = buf[q + 2], t3 = buf[q + 3];
+        vx += t0.x;
+        vy += t0.y;
+        vx += t1.x;
+        vy += t1.y;
+        vx += t2.x;
+        vy += t2.y;
+        vx += t3.x;
+        vy += t3.y;
+      }
+      for (; q < k; ++q) {
+        vx += buf[q].x;
+        vy += buf[q].y;
       }
     }
     __syncthreads();
@@ -148,11 +170,11 @@ __global__ __launch_bounds__(64) void k_link_sweep_heavy(const uint32_t* __restr
   }
 }
 
-// Nodes whose row is longer than HEAVY_ROW (any order: each is one block).
-__global__ __launch_bounds__(EW) void k_heavy_rows(const uint32_t* __restrict__ row, uint32_t n,
-                                                   uint32_t* __restrict__ heavy, uint32_t* __restrict__ count) {
+// Nodes whose row is longer than LONG_ROW (any order: each is one block).
+__global__ __launch_bounds__(EW) void k_long_rows(const uint32_t* __restrict__ row, uint32_t n,
+                                                  uint32_t* __restrict__ rows_long, uint32_t* __restrict__ count) {
   const uint32_t i = blockIdx.x * EW + threadIdx.x;
-  if (i < n && row[i + 1] - row[i] > HEAVY_ROW) heavy[atomicAdd(count, 1u)] = i;
+  if (i < n && row[i + 1] - row[i] > LONG_ROW) rows_long[atomicAdd(count, 1u)] = i;
 }
 
 // ---------------------------------------------------------------- grid collide
@@ -292,23 +314,37 @@ __global__ __launch_bounds__(EW) void k_grid_starts(const uint64_t* __restrict__
   start[c] = lo;
 }
 
+// The binned nodes' positions and radii in sorted (cell) order, so a cell's
+// candidates are contiguous.
+__global__ __launch_bounds__(EW) void k_grid_gather(const GridParams* __restrict__ gp,
+                                                    const uint32_t* __restrict__ items, const double* __restrict__ px,
+                                                    const double* __restrict__ py, const double* __restrict__ radii,
+                                                    double4* __restrict__ cand) {
+  const uint32_t k = blockIdx.x * EW + threadIdx.x;
+  if (!gp->active || k >= gp->m) return;
+  const uint32_t j = items[k];
+  cand[k] = make_double4(px[j], py[j], radii[j], __longlong_as_double((long long)j));
+}
+
 // grid.zig:129-195: node i scans its 3 x 3 cells (rows, then columns, then
 // each cell's nodes high index first) and adds its share of each push.
-__global__ __launch_bounds__(EW) void k_grid_sweep(uint32_t n, const GridParams* __restrict__ gp,
-                                                   const uint32_t* __restrict__ cell_of,
+// Threads run in cell order (thread k takes the k-th binned node), so a
+// wave's nodes share their candidate cells; each node's own sum is the
+// reference's, term for term.
+__global__ __launch_bounds__(EW) void k_grid_sweep(const GridParams* __restrict__ gp,
+                                                   const uint64_t* __restrict__ keys,
                                                    const uint32_t* __restrict__ start,
-                                                   const uint32_t* __restrict__ items,
-                                                   const double* __restrict__ radii, const double* __restrict__ px,
-                                                   const double* __restrict__ py, double* __restrict__ vxs,
+                                                   const double4* __restrict__ cand, double* __restrict__ vxs,
                                                    double* __restrict__ vys, double strength, uint64_t seed) {
-  const uint32_t i = blockIdx.x * EW + threadIdx.x;
-  if (i >= n || !gp->active) return;
-  const uint32_t ci = cell_of[i];
-  if (ci == NO_CELL) return;
+  const uint32_t k = blockIdx.x * EW + threadIdx.x;
+  if (!gp->active || k >= gp->m) return;
+  const uint32_t ci = (uint32_t)keys[k];
+  const double4 me = cand[k];
+  const uint32_t i = (uint32_t)__double_as_longlong(me.w);
   const uint32_t gx = gp->gx, gy = gp->gy;
-  const double ri = radii[i];
+  const double ri = me.z;
   const double ri2 = ri * ri;
-  const double xi = px[i], yi = py[i];
+  const double xi = me.x, yi = me.y;
   double vx = vxs[i], vy = vys[i];
   const uint32_t cxi = ci % gx, cyi = ci / gx;
   const uint32_t xa = cxi > 0 ? cxi - 1 : 0, xb = cxi + 1 < gx - 1 ? cxi + 1 : gx - 1;
@@ -318,11 +354,12 @@ __global__ __launch_bounds__(EW) void k_grid_sweep(uint32_t n, const GridParams*
     // run of the sorted items, in the reference's cell-then-node order.
     const uint32_t a1 = start[cy * gx + xb + 1];
     for (uint32_t a = start[cy * gx + xa]; a < a1; ++a) {
-      const uint32_t j = items[a];
+      const double4 c = cand[a];
+      const uint32_t j = (uint32_t)__double_as_longlong(c.w);
       if (j == i) continue;
-      const double rj = radii[j];
+      const double rj = c.z;
       double rr = ri + rj;
-      double x = xi - px[j], y = yi - py[j];
+      double x = xi - c.x, y = yi - c.y;
       double l = x * x + y * y;
       if (l < rr * rr) {
         if (x == 0 || y == 0) {
@@ -355,36 +392,37 @@ constexpr uint32_t GRID_BLOCKS = 512;
 
 }  // namespace
 
-hipError_t link_csr(const uint32_t* src, const uint32_t* tgt, uint32_t m, uint32_t n, uint32_t* row, uint32_t* adj,
-                    uint32_t* heavy, uint32_t* heavy_count, const SortBuffers& sb, hipStream_t stream) {
-  const uint32_t total = 2 * m;
+hipError_t link_csr(const LinkCsrArgs& a, const SortBuffers& sb, uint32_t* long_count, hipStream_t stream) {
+  const uint32_t total = 2 * a.m, n = a.n;
   if (total) {
-    hipLaunchKernelGGL(k_csr_entries, dim3(ceil_div(total, EW)), dim3(EW), 0, stream, src, tgt, m, sb.k0, adj);
+    hipLaunchKernelGGL(k_csr_entries, dim3(ceil_div(total, EW)), dim3(EW), 0, stream, a.src, a.tgt, a.m, sb.k0,
+                       a.adj);
     HIPCHECK(hipMemcpyAsync(sb.count, &total, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     int digits = 0;
     for (uint64_t v = n; v > 0; v >>= 8) ++digits;
-    digits += digits & 1;
-    if (digits == 0) digits = 2;
-    HIPCHECK(sort_digits(sb.hist, sb.glob, sb.k0, adj, sb.k1, sb.v1, total, sb.count, 0, digits, stream));
+    HIPCHECK(sort_digits(sb.hist, sb.glob, sb.k0, a.adj, sb.k1, sb.v1, total, sb.count, 0, digits, stream));
+    hipLaunchKernelGGL(k_csr_pack, dim3(ceil_div(total, EW)), dim3(EW), 0, stream, total, sb.k0, a.adj, a.src, a.tgt,
+                       a.strength, a.distance, a.bias, a.owner, a.entries);
   }
-  hipLaunchKernelGGL(k_csr_rows, dim3(ceil_div(n + 1, EW)), dim3(EW), 0, stream, sb.k0, total, n, row);
+  hipLaunchKernelGGL(k_csr_rows, dim3(ceil_div(n + 1, EW)), dim3(EW), 0, stream, sb.k0, total, n, a.row);
   HIPCHECK(hipMemsetAsync(sb.count, 0, sizeof(uint32_t), stream));
-  if (n) hipLaunchKernelGGL(k_heavy_rows, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, row, n, heavy, sb.count);
-  HIPCHECK(hipMemcpyAsync(heavy_count, sb.count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  if (n) hipLaunchKernelGGL(k_long_rows, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, a.row, n, a.rows_long, sb.count);
+  HIPCHECK(hipMemcpyAsync(long_count, sb.count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
   HIPCHECK(hipGetLastError());
-  return hipStreamSynchronize(stream);  // `total` and *heavy_count live on the host
+  return hipStreamSynchronize(stream);  // `total` and *long_count live on the host
 }
 
-hipError_t link_sweep(uint32_t n, const uint32_t* row, const uint32_t* adj, const uint32_t* heavy, uint32_t heavy_count,
-                      const uint32_t* src, const uint32_t* tgt, const double* strength, const double* distance,
-                      const double* bias, const double* px, const double* py, double* vx, double* vy, double alpha,
-                      uint64_t seed, hipStream_t stream) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_link_sweep, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, n, row, adj, src, tgt, strength,
-                     distance, bias, px, py, vx, vy, alpha, seed);
-  if (heavy_count)
-    hipLaunchKernelGGL(k_link_sweep_heavy, dim3(heavy_count), dim3(64), 0, stream, heavy, row, adj, src, tgt, strength,
-                       distance, bias, px, py, vx, vy, alpha, seed);
+hipError_t link_sweep(const LinkCsrArgs& a, uint32_t long_count, const double* px, const double* py, double* vx,
+                      double* vy, double alpha, uint64_t seed, hipStream_t stream) {
+  const uint32_t total = 2 * a.m;
+  if (a.n == 0 || total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_link_terms, dim3(ceil_div(total, EW)), dim3(EW), 0, stream, total, a.owner, a.entries, px, py,
+                     (double2*)a.terms, alpha, seed);
+  hipLaunchKernelGGL(k_link_rows, dim3(ceil_div(a.n, EW)), dim3(EW), 0, stream, a.n, a.row, (const double2*)a.terms,
+                     vx, vy);
+  if (long_count)
+    hipLaunchKernelGGL(k_link_rows_long, dim3(long_count), dim3(64), 0, stream, a.rows_long, a.row,
+                       (const double2*)a.terms, vx, vy);
   return hipGetLastError();
 }
 
@@ -402,11 +440,15 @@ hipError_t grid_collide_iteration(const double* x, const double* y, double* vx, 
   hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(EW), 0, stream, (const GridPartial*)g.partials, blocks, n,
                      g.params);
   hipLaunchKernelGGL(k_grid_bin, dim3(nb), dim3(EW), 0, stream, g.px, g.py, n, g.params, g.cell_of, sb.k0, sb.v0);
-  // Cell ids fit 32 bits (at most 4n + 16 cells); the count is 0 when inactive.
-  HIPCHECK(sort_digits(sb.hist, sb.glob, sb.k0, sb.v0, sb.k1, sb.v1, n, &g.params->sort_n, 0, 4, stream));
+  // Cell ids are at most 4n + 16 (grid.zig:243-250); the count is 0 when
+  // the iteration is inactive.
+  int digits = 0;
+  for (uint64_t v = 4 * (uint64_t)n + 16; v > 0; v >>= 8) ++digits;
+  HIPCHECK(sort_digits(sb.hist, sb.glob, sb.k0, sb.v0, sb.k1, sb.v1, n, &g.params->sort_n, 0, digits, stream));
   hipLaunchKernelGGL(k_grid_starts, dim3(ceil_div(4 * n + 17, EW)), dim3(EW), 0, stream, sb.k0, g.params, g.start);
-  hipLaunchKernelGGL(k_grid_sweep, dim3(nb), dim3(EW), 0, stream, n, g.params, g.cell_of, g.start, sb.v0, radii,
-                     g.px, g.py, vx, vy, strength, seed);
+  hipLaunchKernelGGL(k_grid_gather, dim3(nb), dim3(EW), 0, stream, g.params, sb.v0, g.px, g.py, radii, g.cand);
+  hipLaunchKernelGGL(k_grid_sweep, dim3(nb), dim3(EW), 0, stream, g.params, sb.k0, g.start, g.cand, vx, vy, strength,
+                     seed);
   return hipGetLastError();
 }
 

@@ -15,9 +15,9 @@
 //     M2L pairs and P2P pairs (wave-aggregated appends). The recursion's
 //     decisions depend on the pair alone, so the lists hold exactly the
 //     reference's pairs.
-//  4. Both lists are sorted by (target, source), so each target reads its
-//     sources in one fixed order (the FP32 sums do not depend on the
-//     schedule).
+//  4. Both lists are grouped by target (count, scan, scatter), and each
+//     target's group is ordered by source in LDS before it is read, so every
+//     FP32 sum runs in one fixed order whatever the schedule.
 //  5. M2L, one wave per target cell, one lane per source cell; L2L top down;
 //     then P2P and L2P fused, one wave per target leaf, and the sum is added
 //     to the FP64 velocity once per node.
@@ -40,6 +40,8 @@ namespace {
 
 constexpr int EWF = 256;
 constexpr uint32_t GRID_BLOCKS = 1024;  // grid-stride kernels over device counts
+constexpr uint32_t LEVEL_BLOCKS = 256;  // per-depth build kernels (grid-stride)
+constexpr uint32_t SEG_MAX = 1024;      // sources of one target ordered in LDS
 constexpr double MIN_SEP2 = 1e-28;      // fmm.zig:49
 constexpr int BN = 2 * FMM_MAX_ORDER + 1;
 
@@ -261,130 +263,169 @@ __global__ __launch_bounds__(EWF) void k_fmm_up(FCell* __restrict__ cells, doubl
 
 // ---------------------------------------------------------------- dual tree
 
-__device__ inline uint32_t wave_exclusive(uint32_t v, uint32_t& total) {
-  const int lane = threadIdx.x & 63;
-  uint32_t inc = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t t = __shfl_up(inc, d, 64);
-    if (lane >= d) inc += t;
-  }
-  total = __shfl(inc, 63, 64);
-  return inc - v;
-}
-
-// Reserves `v` slots per lane in *counter with one atomic per wave; every
-// lane of the wave must call it.
-__device__ inline uint32_t wave_append(uint32_t* counter, uint32_t v) {
-  uint32_t total;
-  const uint32_t off = wave_exclusive(v, total);
-  uint32_t base = 0;
-  if ((threadIdx.x & 63) == 0 && total) base = atomicAdd(counter, total);
-  return __shfl(base, 0, 64) + off;
-}
-
 struct MacArgs {
   double mac_far, dmin_s, dmax_s;
   int ring;
   uint32_t nbits;
 };
 
-// One round of fmm.zig:400-453 over the frontier fp->front[step].
+constexpr int STEP_PER = 4;                   // pairs per thread per tile
+constexpr int STEP_TILE = EWF * STEP_PER;     // pairs per block per tile
+
+// Inclusive scan of v over the block (EWF threads); wsum holds EWF / 64 words.
+__device__ inline uint32_t block_inclusive(uint32_t v, uint32_t* wsum, uint32_t& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += t;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  uint32_t before = 0;
+  total = 0;
+#pragma unroll
+  for (int w = 0; w < EWF / 64; ++w) {
+    const uint32_t x = wsum[w];
+    if (w < wave) before += x;
+    total += x;
+  }
+  __syncthreads();
+  return before + inc;
+}
+
+// One round of fmm.zig:400-453 over the frontier fp->front[step]. A block
+// classifies a tile of STEP_TILE pairs, scans its output counts and reserves
+// room in each list with one atomic per list and tile.
 __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cells, FmmParams* __restrict__ fp,
                                                    uint32_t step, const uint2* __restrict__ in,
                                                    uint2* __restrict__ out, uint32_t front_cap,
                                                    uint64_t* __restrict__ m2l, uint32_t m2l_cap,
                                                    uint64_t* __restrict__ p2p, uint32_t p2p_cap, MacArgs mac) {
+  __shared__ uint32_t wsum[3][EWF / 64];
+  __shared__ uint32_t base_sh[3];
+  __shared__ unsigned long long pairs_sh[EWF / 64];
   uint32_t cnt = fp->front[step];
   if (cnt > front_cap) cnt = front_cap;
-  for (uint32_t base = blockIdx.x * EWF; base < cnt; base += gridDim.x * EWF) {
-    const uint32_t idx = base + threadIdx.x;
+  for (uint32_t tile = blockIdx.x * STEP_TILE; tile < cnt; tile += gridDim.x * STEP_TILE) {
     // kind: 0 none, 1 M2L, 2 P2P; push: 0 none, 1 children x children,
     // 2 (A, children of B), 3 (children of A, B)
-    uint32_t A = 0, B = 0, kind = 0, push = 0, npush = 0, c0 = 0, nca = 0, ncb = 0;
+    uint32_t pa[STEP_PER], pb[STEP_PER], pc0[STEP_PER], code[STEP_PER];
+    uint32_t n_front = 0, n_m2l = 0, n_p2p = 0;
     unsigned long long pairs = 0;
-    if (idx < cnt) {
-      const uint2 pr = in[idx];
-      A = pr.x;
-      B = pr.y;
-      const FCell ca = cells[A];
-      if (A == B) {
-        if (ca.nchild == 0) {
-          kind = 2;
-          pairs = (unsigned long long)ca.count * ca.count;
-        } else {
-          push = 1;
-          c0 = ca.child0;
-          nca = ca.nchild;
-          npush = nca * nca;
-        }
-      } else {
-        const FCell cb = cells[B];
-        const double ra = ca.r, rb = cb.r;
-        const double dx = ca.cx - cb.cx, dy = ca.cy - cb.cy;
-        const double d2 = dx * dx + dy * dy;
-        const double rmax = zmax(ra, rb), rmin = zmin(ra, rb);
-        const double thr = mac.mac_far * rmax + rmin;
-        bool done = false;
-        if (d2 > MIN_SEP2 && d2 >= thr * thr) {
-          bool separated = true;
-          if (mac.ring) {
-            const double gap = __builtin_sqrt(d2) - ra - rb;
-            if (gap >= mac.dmax_s) done = true;
-            else if (gap + 2 * (ra + rb) > mac.dmax_s || gap < mac.dmin_s) separated = false;
-          }
-          if (!done && separated) {
-            kind = 1;
-            done = true;
-          }
-        }
-        if (!done) {
-          const bool la = ca.nchild == 0, lb = cb.nchild == 0;
-          if (la && lb) {
+#pragma unroll
+    for (int q = 0; q < STEP_PER; ++q) {
+      const uint32_t idx = tile + q * EWF + threadIdx.x;
+      uint32_t A = 0, B = 0, kind = 0, push = 0, c0 = 0, nch = 0;
+      if (idx < cnt) {
+        const uint2 pr = in[idx];
+        A = pr.x;
+        B = pr.y;
+        const FCell ca = cells[A];
+        if (A == B) {
+          if (ca.nchild == 0) {
             kind = 2;
-            pairs = (unsigned long long)ca.count * cb.count;
-          } else if (la || (!lb && rb > ra)) {
-            push = 2;
-            c0 = cb.child0;
-            ncb = cb.nchild;
-            npush = ncb;
+            pairs += (unsigned long long)ca.count * ca.count;
           } else {
-            push = 3;
+            push = 1;
             c0 = ca.child0;
-            nca = ca.nchild;
-            npush = nca;
+            nch = ca.nchild;
+          }
+        } else {
+          const FCell cb = cells[B];
+          const double ra = ca.r, rb = cb.r;
+          const double dx = ca.cx - cb.cx, dy = ca.cy - cb.cy;
+          const double d2 = dx * dx + dy * dy;
+          const double rmax = zmax(ra, rb), rmin = zmin(ra, rb);
+          const double thr = mac.mac_far * rmax + rmin;
+          bool done = false;
+          if (d2 > MIN_SEP2 && d2 >= thr * thr) {
+            bool separated = true;
+            if (mac.ring) {
+              const double gap = __builtin_sqrt(d2) - ra - rb;
+              if (gap >= mac.dmax_s) done = true;
+              else if (gap + 2 * (ra + rb) > mac.dmax_s || gap < mac.dmin_s) separated = false;
+            }
+            if (!done && separated) {
+              kind = 1;
+              done = true;
+            }
+          }
+          if (!done) {
+            const bool la = ca.nchild == 0, lb = cb.nchild == 0;
+            if (la && lb) {
+              kind = 2;
+              pairs += (unsigned long long)ca.count * cb.count;
+            } else if (la || (!lb && rb > ra)) {
+              push = 2;
+              c0 = cb.child0;
+              nch = cb.nchild;
+            } else {
+              push = 3;
+              c0 = ca.child0;
+              nch = ca.nchild;
+            }
           }
         }
       }
+      pa[q] = A;
+      pb[q] = B;
+      pc0[q] = c0;
+      code[q] = kind | push << 2 | nch << 4;
+      n_front += push == 1 ? nch * nch : (push ? nch : 0);
+      n_m2l += kind == 1;
+      n_p2p += kind == 2;
     }
-    const uint32_t pos_front = wave_append(&fp->front[step + 1], npush);
-    const uint32_t pos_m2l = wave_append(&fp->m2l, kind == 1);
-    const uint32_t pos_p2p = wave_append(&fp->p2p, kind == 2);
-    unsigned long long ps = pairs;
+    uint32_t t_front, t_m2l, t_p2p;
+    uint32_t o_front = block_inclusive(n_front, wsum[0], t_front) - n_front;
+    uint32_t o_m2l = block_inclusive(n_m2l, wsum[1], t_m2l) - n_m2l;
+    uint32_t o_p2p = block_inclusive(n_p2p, wsum[2], t_p2p) - n_p2p;
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) ps += __shfl_xor(ps, d, 64);
-    if ((threadIdx.x & 63) == 0 && ps) atomicAdd(&fp->pairs, ps);
-    const uint64_t key = (uint64_t)A << mac.nbits | B;
-    if (kind == 1) {
-      if (pos_m2l < m2l_cap) m2l[pos_m2l] = key;
-      else atomicOr(&fp->list_overflow, 2u);
-    } else if (kind == 2) {
-      if (pos_p2p < p2p_cap) p2p[pos_p2p] = key;
-      else atomicOr(&fp->list_overflow, 4u);
+    for (int d = 32; d > 0; d >>= 1) pairs += __shfl_xor(pairs, d, 64);
+    if ((threadIdx.x & 63) == 0) pairs_sh[threadIdx.x >> 6] = pairs;
+    if (threadIdx.x == 0) {
+      base_sh[0] = t_front ? atomicAdd(&fp->front[step + 1], t_front) : 0;
+      base_sh[1] = t_m2l ? atomicAdd(&fp->m2l, t_m2l) : 0;
+      base_sh[2] = t_p2p ? atomicAdd(&fp->p2p, t_p2p) : 0;
     }
-    if (npush) {
-      if (pos_front + npush > front_cap) {
-        atomicOr(&fp->list_overflow, 1u);
-      } else {
-        uint32_t o = pos_front;
-        if (push == 1) {
-          for (uint32_t i = 0; i < nca; ++i)
-            for (uint32_t j = 0; j < nca; ++j) out[o++] = make_uint2(c0 + i, c0 + j);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long ps = 0;
+      for (int w = 0; w < EWF / 64; ++w) ps += pairs_sh[w];
+      if (ps) atomicAdd(&fp->pairs, ps);
+    }
+    o_front += base_sh[0];
+    o_m2l += base_sh[1];
+    o_p2p += base_sh[2];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < STEP_PER; ++q) {
+      const uint32_t kind = code[q] & 3, push = (code[q] >> 2) & 3, nch = code[q] >> 4;
+      const uint32_t A = pa[q], B = pb[q], c0 = pc0[q];
+      const uint64_t key = (uint64_t)A << mac.nbits | B;
+      if (kind == 1) {
+        if (o_m2l < m2l_cap) m2l[o_m2l] = key;
+        else atomicOr(&fp->list_overflow, 2u);
+        ++o_m2l;
+      } else if (kind == 2) {
+        if (o_p2p < p2p_cap) p2p[o_p2p] = key;
+        else atomicOr(&fp->list_overflow, 4u);
+        ++o_p2p;
+      }
+      if (push) {
+        const uint32_t np = push == 1 ? nch * nch : nch;
+        if (o_front + np > front_cap) {
+          atomicOr(&fp->list_overflow, 1u);
+        } else if (push == 1) {
+          for (uint32_t i = 0; i < nch; ++i)
+            for (uint32_t j = 0; j < nch; ++j) out[o_front + i * nch + j] = make_uint2(c0 + i, c0 + j);
         } else if (push == 2) {
-          for (uint32_t j = 0; j < ncb; ++j) out[o++] = make_uint2(A, c0 + j);
+          for (uint32_t j = 0; j < nch; ++j) out[o_front + j] = make_uint2(A, c0 + j);
         } else {
-          for (uint32_t i = 0; i < nca; ++i) out[o++] = make_uint2(c0 + i, B);
+          for (uint32_t i = 0; i < nch; ++i) out[o_front + i] = make_uint2(c0 + i, B);
         }
+        o_front += np;
       }
     }
   }
@@ -397,70 +438,121 @@ __global__ void k_fmm_lists_reset(FmmParams* __restrict__ fp, uint2* __restrict_
     fp->m2l = 0;
     fp->p2p = 0;
     fp->list_overflow = 0;
+    fp->unsorted = 0;
     fp->pairs = 0;
     front[0] = make_uint2(0, 0);
   }
 }
 
-// First and end position of each target's run in a (target, source) list.
-__global__ __launch_bounds__(EWF) void k_segments(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ count,
-                                                  uint32_t nbits, uint32_t* __restrict__ lo, uint32_t* __restrict__ hi) {
+// Interaction pairs grouped by target: a count per target, a scan, and a
+// scatter through per-target cursors. Within a target the scatter's order is
+// arbitrary; the kernels that read a group first order it by source
+// (sort_segment), so every sum runs in one fixed order.
+__global__ __launch_bounds__(EWF) void k_target_count(const uint64_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ count, uint32_t nbits,
+                                                      uint32_t* __restrict__ cnt) {
   const uint32_t k = blockIdx.x * EWF + threadIdx.x;
-  const uint32_t cnt = *count;
-  if (k >= cnt) return;
-  const uint64_t a = keys[k] >> nbits;
-  if (k == 0 || (keys[k - 1] >> nbits) != a) lo[a] = k;
-  if (k == cnt - 1 || (keys[k + 1] >> nbits) != a) hi[a] = k + 1;
+  if (k < *count) atomicAdd(&cnt[list[k] >> nbits], 1u);
+}
+
+__global__ __launch_bounds__(EWF) void k_target_scatter(const uint64_t* __restrict__ list,
+                                                        const uint32_t* __restrict__ count, uint32_t nbits,
+                                                        const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
+                                                        uint32_t* __restrict__ src) {
+  const uint32_t k = blockIdx.x * EWF + threadIdx.x;
+  if (k >= *count) return;
+  const uint64_t key = list[k];
+  const uint32_t a = (uint32_t)(key >> nbits);
+  src[off[a] + atomicAdd(&cursor[a], 1u)] = (uint32_t)(key & (((uint64_t)1 << nbits) - 1));
+}
+
+// One wave loads the sources of window w (SEG_MAX at a time) of a target
+// with k sources into out[0 .. min(SEG_MAX, k - w)): ordered by source when
+// the target has at most SEG_MAX of them, in scatter order otherwise.
+__device__ inline uint32_t load_window(const uint32_t* __restrict__ src, uint32_t off, uint32_t k, uint32_t w,
+                                       uint32_t (&in)[SEG_MAX], uint32_t (&out)[SEG_MAX]);
+
+// One wave orders the k <= SEG_MAX distinct sources src[off .. off + k) by
+// rank into out[0 .. k).
+__device__ inline void sort_segment(const uint32_t* __restrict__ src, uint32_t off, uint32_t k,
+                                    uint32_t (&in)[SEG_MAX], uint32_t (&out)[SEG_MAX]) {
+  const int lane = threadIdx.x & 63;
+  for (uint32_t q = lane; q < k; q += 64) in[q] = src[off + q];
+  __syncthreads();
+  for (uint32_t q = lane; q < k; q += 64) {
+    const uint32_t b = in[q];
+    uint32_t r = 0;
+    for (uint32_t t = 0; t < k; ++t) r += in[t] < b;
+    out[r] = b;
+  }
+  __syncthreads();
+}
+
+__device__ inline uint32_t load_window(const uint32_t* __restrict__ src, uint32_t off, uint32_t k, uint32_t w,
+                                       uint32_t (&in)[SEG_MAX], uint32_t (&out)[SEG_MAX]) {
+  if (k <= SEG_MAX) {
+    sort_segment(src, off, k, in, out);
+    return k;
+  }
+  const uint32_t wk = k - w < SEG_MAX ? k - w : SEG_MAX;
+  __syncthreads();
+  for (uint32_t q = threadIdx.x & 63; q < wk; q += 64) out[q] = src[off + w + q];
+  __syncthreads();
+  return wk;
 }
 
 // ---------------------------------------------------------------- M2L
 
 template <int Q>
-__global__ __launch_bounds__(64) void k_m2l(const FCell* __restrict__ cells, const uint64_t* __restrict__ list,
-                                            const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
-                                            uint32_t nbits, const float2* __restrict__ mexp,
-                                            float2* __restrict__ lexp) {
+__global__ __launch_bounds__(64) void k_m2l(const FCell* __restrict__ cells, const uint32_t* __restrict__ srcs,
+                                            const uint32_t* __restrict__ offs, const uint32_t* __restrict__ cnts,
+                                            const float2* __restrict__ mexp, float2* __restrict__ lexp,
+                                            FmmParams* __restrict__ fp) {
+  __shared__ uint32_t seg_in[SEG_MAX], seg[SEG_MAX];
   const uint32_t A = blockIdx.x;
   const int lane = threadIdx.x;
-  const uint32_t s0 = lo[A], s1 = hi[A];
-  const uint64_t mask = ((uint64_t)1 << nbits) - 1;
+  const uint32_t K = cnts[A], off = offs[A];
+  if (K > SEG_MAX && lane == 0) atomicAdd(&fp->unsorted, 1u);
   float2 acc[Q];
 #pragma unroll
   for (int l = 0; l < Q; ++l) acc[l] = make_float2(0.f, 0.f);
-  if (s1 > s0) {
+  if (K > 0) {
     const FCell ca = cells[A];
-    for (uint32_t base = s0; base < s1; base += 64) {
-      const uint32_t s = base + lane;
-      if (s >= s1) continue;
-      const uint32_t B = (uint32_t)(list[s] & mask);
-      const FCell cb = cells[B];
-      const double dx = ca.cx - cb.cx, dy = ca.cy - cb.cy;
-      const double inv = 1 / (dx * dx + dy * dy);
-      const double wr = dx * inv, wi = -dy * inv;
-      const float2 w = make_float2((float)wr, (float)wi);
-      const float2 a = make_float2((float)(cb.r * wr), (float)(cb.r * wi));
-      const float2 b = make_float2((float)(ca.r * wr), (float)(ca.r * wi));
-      float2 u[Q];
-      float2 ap = make_float2(1.f, 0.f);
-#pragma unroll
-      for (int k = 0; k < Q; ++k) {
-        u[k] = cmul(mexp[(size_t)B * Q + k], ap);
-        ap = cmul(ap, a);
-      }
-      float2 coef = w;  // (-1)^l b^l w
-#pragma unroll
-      for (int l = 0; l < Q; ++l) {
-        float2 v = make_float2(0.f, 0.f);
+    for (uint32_t w0 = 0; w0 < K; w0 += SEG_MAX) {
+      const uint32_t wk = load_window(srcs, off, K, w0, seg_in, seg);
+      for (uint32_t base = 0; base < wk; base += 64) {
+        const uint32_t s = base + lane;
+        if (s >= wk) continue;
+        const uint32_t B = seg[s];
+        const FCell cb = cells[B];
+        const double dx = ca.cx - cb.cx, dy = ca.cy - cb.cy;
+        const double inv = 1 / (dx * dx + dy * dy);
+        const double wr = dx * inv, wi = -dy * inv;
+        const float2 w = make_float2((float)wr, (float)wi);
+        const float2 a = make_float2((float)(cb.r * wr), (float)(cb.r * wi));
+        const float2 b = make_float2((float)(ca.r * wr), (float)(ca.r * wi));
+        float2 u[Q];
+        float2 ap = make_float2(1.f, 0.f);
 #pragma unroll
         for (int k = 0; k < Q; ++k) {
-          const float g = binom(k + l, k);
-          v.x = fmaf(g, u[k].x, v.x);
-          v.y = fmaf(g, u[k].y, v.y);
+          u[k] = cmul(mexp[(size_t)B * Q + k], ap);
+          ap = cmul(ap, a);
         }
-        acc[l] = cfma(coef, v, acc[l]);
-        coef = cmul(coef, b);
-        coef.x = -coef.x;
-        coef.y = -coef.y;
+        float2 coef = w;  // (-1)^l b^l w
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+          float2 v = make_float2(0.f, 0.f);
+#pragma unroll
+          for (int k = 0; k < Q; ++k) {
+            const float g = binom(k + l, k);
+            v.x = fmaf(g, u[k].x, v.x);
+            v.y = fmaf(g, u[k].y, v.y);
+          }
+          acc[l] = cfma(coef, v, acc[l]);
+          coef = cmul(coef, b);
+          coef.x = -coef.x;
+          coef.y = -coef.y;
+        }
       }
     }
 #pragma unroll
@@ -524,13 +616,15 @@ __global__ __launch_bounds__(EWF) void k_fmm_down(const FCell* __restrict__ cell
 
 // One wave per target leaf: lane groups of S lanes per target point split
 // the sources; the S partial sums are added in lane order, then L2P
-// (fmm.zig:853-886) and the FP64 velocity update. The pair term is d3's
-// (fmm.zig:529-553): skip beyond dmax, hash jiggle on an exact zero delta,
-// sqrt(dmin2 * l) below dmin.
+// (fmm.zig:853-886) and the FP64 velocity update. Each source leaf is staged
+// in LDS as FP32 offsets from an anchor (the target leaf's first point,
+// FP64), so two points with equal FP64 coordinates have an exactly zero
+// FP32 delta. The pair term is d3's (fmm.zig:529-553): skip beyond dmax,
+// hash jiggle on a zero delta, sqrt(dmin2 * l) below dmin.
 template <int Q>
-__global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, const uint64_t* __restrict__ list,
-                                             const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
-                                             uint32_t nbits, const float2* __restrict__ lexp,
+__global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, const uint32_t* __restrict__ srcs,
+                                             const uint32_t* __restrict__ offs, const uint32_t* __restrict__ cnts,
+                                             FmmParams* __restrict__ fp, const float2* __restrict__ lexp,
                                              const double* __restrict__ px, const double* __restrict__ py,
                                              const float* __restrict__ ps, const double* __restrict__ ux,
                                              const double* __restrict__ uy, const uint32_t* __restrict__ perm,
@@ -540,12 +634,12 @@ __global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, co
   const FCell ca = cells[A];
   if (ca.nchild != 0) return;
   const int lane = threadIdx.x;
-  const uint32_t s0 = lo[A], s1 = hi[A];
-  const uint64_t mask = ((uint64_t)1 << nbits) - 1;
+  __shared__ uint32_t seg_in[SEG_MAX], seg[SEG_MAX];
+  const uint32_t K = cnts[A], off = offs[A];
+  if (K > SEG_MAX && lane == 0) atomicAdd(&fp->unsorted, 1u);
+  __shared__ float4 src[64];
   __shared__ float2 part[64];
-  float2 L[Q];
-#pragma unroll
-  for (int l = 0; l < Q; ++l) L[l] = lexp[(size_t)A * Q + l];
+  const double ax = px[ca.first], ay = py[ca.first];
   const double inv_r = ca.r > 0 ? 1 / ca.r : 0;
   for (uint32_t tb = 0; tb < ca.count; tb += 64) {
     const uint32_t T = ca.count - tb < 64 ? ca.count - tb : 64;
@@ -553,36 +647,51 @@ __global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, co
     const uint32_t t = lane / S, sub = lane % S;
     const bool active = t < T;
     const uint32_t ti = ca.first + tb + (active ? t : 0);
-    const double xi = px[ti], yi = py[ti];
+    const float xi = (float)(px[ti] - ax), yi = (float)(py[ti] - ay);
     const uint32_t i = perm[ti];
-    float ax = 0.f, ay = 0.f;
-    for (uint32_t s = s0; s < s1; ++s) {
-      const uint32_t B = (uint32_t)(list[s] & mask);
-      const uint32_t f0 = cells[B].first, f1 = f0 + cells[B].count;
-      if (!active) continue;
-      for (uint32_t j = f0 + sub; j < f1; j += S) {
-        if (j == ti) continue;
-        float dx = (float)(px[j] - xi), dy = (float)(py[j] - yi);
-        float l = fmaf(dx, dx, dy * dy);
-        if (!(l < dmax2)) continue;
-        if (dx == 0.f || dy == 0.f) {
-          const uint32_t jj = perm[j];
-          if (dx == 0.f) {
-            dx = (float)hash_jiggle(seed, i, jj, 0);
-            l = fmaf(dx, dx, l);
+    float accx = 0.f, accy = 0.f;
+    for (uint32_t w0 = 0; w0 < K; w0 += SEG_MAX) {
+      const uint32_t wk = load_window(srcs, off, K, w0, seg_in, seg);
+      for (uint32_t s = 0; s < wk; ++s) {
+        const uint32_t B = seg[s];
+        const uint32_t f0 = cells[B].first, cnt = cells[B].count;
+        for (uint32_t p0 = 0; p0 < cnt; p0 += 64) {
+          const uint32_t np = cnt - p0 < 64 ? cnt - p0 : 64;
+          if ((uint32_t)lane < np) {
+            const uint32_t j = f0 + p0 + lane;
+            src[lane] = make_float4((float)(px[j] - ax), (float)(py[j] - ay), ps[j], __uint_as_float(j));
           }
-          if (dy == 0.f) {
-            dy = (float)hash_jiggle(seed, i, jj, 1);
-            l = fmaf(dy, dy, l);
+          __syncthreads();
+          if (active) {
+            for (uint32_t q = sub; q < np; q += S) {
+              const float4 c = src[q];
+              const uint32_t j = f0 + p0 + q;
+              if (j == ti) continue;
+              float dx = c.x - xi, dy = c.y - yi;
+              float l = fmaf(dx, dx, dy * dy);
+              if (!(l < dmax2)) continue;
+              if (dx == 0.f || dy == 0.f) {
+                const uint32_t jj = perm[j];
+                if (dx == 0.f) {
+                  dx = (float)hash_jiggle(seed, i, jj, 0);
+                  l = fmaf(dx, dx, l);
+                }
+                if (dy == 0.f) {
+                  dy = (float)hash_jiggle(seed, i, jj, 1);
+                  l = fmaf(dy, dy, l);
+                }
+              }
+              if (l < dmin2) l = __builtin_sqrtf(dmin2 * l);
+              const float wj = c.z * __builtin_amdgcn_rcpf(l);
+              accx = fmaf(dx, wj, accx);
+              accy = fmaf(dy, wj, accy);
+            }
           }
+          __syncthreads();
         }
-        if (l < dmin2) l = __builtin_sqrtf(dmin2 * l);
-        const float wj = ps[j] / l;
-        ax = fmaf(dx, wj, ax);
-        ay = fmaf(dy, wj, ay);
       }
     }
-    part[lane] = make_float2(ax, ay);
+    part[lane] = make_float2(accx, accy);
     __syncthreads();
     if (active && sub == 0) {
       float2 near = part[lane];
@@ -591,9 +700,9 @@ __global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, co
         near.y += part[lane + q].y;
       }
       const float2 z = make_float2((float)((ux[ti] - ca.cx) * inv_r), (float)((uy[ti] - ca.cy) * inv_r));
-      float2 f = L[Q - 1];
+      float2 f = lexp[(size_t)A * Q + Q - 1];
 #pragma unroll
-      for (int l = Q - 2; l >= 0; --l) f = cfma(f, z, L[l]);
+      for (int l = Q - 2; l >= 0; --l) f = cfma(f, z, lexp[(size_t)A * Q + l]);
       vxs[i] += (double)near.x - (double)f.x * inv_w;
       vys[i] += (double)near.y + (double)f.y * inv_w;
     }
@@ -694,10 +803,12 @@ hipError_t reserve_cells(FmmBuffers& f, uint32_t nc, uint32_t q) {
   if (nc > f.cellarr_cap) {
     const uint32_t cap = nc + nc / 4 + 1024;
     HIPCHECK(grow(f.bbox, (size_t)4 * cap));
-    HIPCHECK(grow(f.m2l_lo, cap));
-    HIPCHECK(grow(f.m2l_hi, cap));
-    HIPCHECK(grow(f.p2p_lo, cap));
-    HIPCHECK(grow(f.p2p_hi, cap));
+    HIPCHECK(grow(f.m2l_off, cap));
+    HIPCHECK(grow(f.m2l_cnt, cap));
+    HIPCHECK(grow(f.p2p_off, cap));
+    HIPCHECK(grow(f.p2p_cnt, cap));
+    HIPCHECK(grow(f.cursor, cap));
+    HIPCHECK(grow(f.scan_partials, (size_t)cap / 4096 + 64));
     f.cellarr_cap = cap;
   }
   if ((size_t)nc * q > f.exp_cap) {
@@ -715,12 +826,9 @@ hipError_t reserve_cells(FmmBuffers& f, uint32_t nc, uint32_t q) {
   const uint32_t want_list = std::max<uint32_t>(1u << 16, 40 * nc);
   if (want_list > f.m2l_cap) {
     HIPCHECK(grow(f.m2l, want_list));
-    HIPCHECK(grow(f.m2l_tmp, want_list));
+    HIPCHECK(grow(f.m2l_src, want_list));
     HIPCHECK(grow(f.p2p, want_list));
-    HIPCHECK(grow(f.p2p_tmp, want_list));
-    HIPCHECK(grow(f.seg_vals, want_list));
-    HIPCHECK(grow(f.seg_vals_tmp, want_list));
-    HIPCHECK(grow(f.hist, sort_hist_words(want_list)));
+    HIPCHECK(grow(f.p2p_src, want_list));
     f.m2l_cap = f.p2p_cap = want_list;
   }
   return hipSuccess;
@@ -778,35 +886,38 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
       const uint32_t cap = c + c / 4 + 1024;
       if (cap > f.m2l_cap) {
         HIPCHECK(grow(f.m2l, cap));
-        HIPCHECK(grow(f.m2l_tmp, cap));
+        HIPCHECK(grow(f.m2l_src, cap));
         HIPCHECK(grow(f.p2p, cap));
-        HIPCHECK(grow(f.p2p_tmp, cap));
-        HIPCHECK(grow(f.seg_vals, cap));
-        HIPCHECK(grow(f.seg_vals_tmp, cap));
-        HIPCHECK(grow(f.hist, sort_hist_words(cap)));
+        HIPCHECK(grow(f.p2p_src, cap));
         f.m2l_cap = f.p2p_cap = cap;
       }
     }
   }
-  // sorted lists and their segments
-  int digits = (int)((2 * nbits + 7) / 8);
-  digits += digits & 1;
-  uint32_t* glob = t.rs_glob;
-  if (P.m2l) HIPCHECK(sort_digits(f.hist, glob, f.m2l, f.seg_vals, f.m2l_tmp, f.seg_vals_tmp, P.m2l, &f.params->m2l, 0, digits, stream));
-  if (P.p2p) HIPCHECK(sort_digits(f.hist, glob, f.p2p, f.seg_vals, f.p2p_tmp, f.seg_vals_tmp, P.p2p, &f.params->p2p, 0, digits, stream));
-  HIPCHECK(hipMemsetAsync(f.m2l_lo, 0, (size_t)nc * sizeof(uint32_t), stream));
-  HIPCHECK(hipMemsetAsync(f.m2l_hi, 0, (size_t)nc * sizeof(uint32_t), stream));
-  HIPCHECK(hipMemsetAsync(f.p2p_lo, 0, (size_t)nc * sizeof(uint32_t), stream));
-  HIPCHECK(hipMemsetAsync(f.p2p_hi, 0, (size_t)nc * sizeof(uint32_t), stream));
+  // lists grouped by target
+  const uint32_t* ncp = &f.params->level_off[FMM_LEVELS + 1];
+  HIPCHECK(hipMemsetAsync(f.m2l_cnt, 0, (size_t)nc * sizeof(uint32_t), stream));
+  HIPCHECK(hipMemsetAsync(f.p2p_cnt, 0, (size_t)nc * sizeof(uint32_t), stream));
   if (P.m2l)
-    hipLaunchKernelGGL(k_segments, dim3(ceil_div(P.m2l, EWF)), dim3(EWF), 0, stream, f.m2l, &f.params->m2l, nbits,
-                       f.m2l_lo, f.m2l_hi);
+    hipLaunchKernelGGL(k_target_count, dim3(ceil_div(P.m2l, EWF)), dim3(EWF), 0, stream, f.m2l, &f.params->m2l, nbits,
+                       f.m2l_cnt);
   if (P.p2p)
-    hipLaunchKernelGGL(k_segments, dim3(ceil_div(P.p2p, EWF)), dim3(EWF), 0, stream, f.p2p, &f.params->p2p, nbits,
-                       f.p2p_lo, f.p2p_hi);
+    hipLaunchKernelGGL(k_target_count, dim3(ceil_div(P.p2p, EWF)), dim3(EWF), 0, stream, f.p2p, &f.params->p2p, nbits,
+                       f.p2p_cnt);
+  HIPCHECK(scan_u32(f.m2l_cnt, f.m2l_off, nc, ncp, f.scan_partials, nullptr, stream));
+  HIPCHECK(scan_u32(f.p2p_cnt, f.p2p_off, nc, ncp, f.scan_partials, nullptr, stream));
+  if (P.m2l) {
+    HIPCHECK(hipMemsetAsync(f.cursor, 0, (size_t)nc * sizeof(uint32_t), stream));
+    hipLaunchKernelGGL(k_target_scatter, dim3(ceil_div(P.m2l, EWF)), dim3(EWF), 0, stream, f.m2l, &f.params->m2l,
+                       nbits, f.m2l_off, f.cursor, f.m2l_src);
+  }
+  if (P.p2p) {
+    HIPCHECK(hipMemsetAsync(f.cursor, 0, (size_t)nc * sizeof(uint32_t), stream));
+    hipLaunchKernelGGL(k_target_scatter, dim3(ceil_div(P.p2p, EWF)), dim3(EWF), 0, stream, f.p2p, &f.params->p2p,
+                       nbits, f.p2p_off, f.cursor, f.p2p_src);
+  }
   // far field
-  hipLaunchKernelGGL(k_m2l<Q>, dim3(nc), dim3(64), 0, stream, f.cells, f.m2l, f.m2l_lo, f.m2l_hi, nbits,
-                     (const float2*)f.mexp, (float2*)f.lexp);
+  hipLaunchKernelGGL(k_m2l<Q>, dim3(nc), dim3(64), 0, stream, f.cells, f.m2l_src, f.m2l_off, f.m2l_cnt,
+                     (const float2*)f.mexp, (float2*)f.lexp, f.params);
   for (uint32_t d = 0; d < maxd; ++d) {
     const uint32_t cnt = P.level_n[d];
     hipLaunchKernelGGL(k_fmm_down<Q>, dim3(ceil_div(cnt, EWF)), dim3(EWF), 0, stream, f.cells, (float2*)f.lexp,
@@ -814,7 +925,7 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   }
   // near field and evaluation
   const float dmin2 = (float)a.dmin2, dmax2 = (float)a.dmax2;
-  hipLaunchKernelGGL(k_near<Q>, dim3(nc), dim3(64), 0, stream, f.cells, f.p2p, f.p2p_lo, f.p2p_hi, nbits,
+  hipLaunchKernelGGL(k_near<Q>, dim3(nc), dim3(64), 0, stream, f.cells, f.p2p_src, f.p2p_off, f.p2p_cnt, f.params,
                      (const float2*)f.lexp, f.px, f.py, f.ps, f.ux, f.uy, t.perm, a.vx, a.vy, P.inv_w, dmin2, dmax2,
                      a.seed);
   return hipGetLastError();
@@ -835,8 +946,8 @@ using Orders = OrderTable<2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17
 
 void fmm_free(FmmBuffers& f) {
   void* ptrs[] = {f.cells, f.bbox, f.split, f.split_off, f.bounds, f.mexp, f.lexp, f.px, f.py, f.ux, f.uy, f.ps,
-                  f.front_a, f.front_b, f.m2l, f.m2l_tmp, f.p2p, f.p2p_tmp, f.seg_vals, f.seg_vals_tmp, f.m2l_lo,
-                  f.m2l_hi, f.p2p_lo, f.p2p_hi, f.hist, f.params};
+                  f.front_a, f.front_b, f.m2l, f.p2p, f.m2l_src, f.p2p_src, f.m2l_off, f.m2l_cnt, f.p2p_off,
+                  f.p2p_cnt, f.cursor, f.scan_partials, f.params};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (f.params_host) (void)hipHostFree(f.params_host);
@@ -861,11 +972,11 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
   for (;;) {
     hipLaunchKernelGGL(k_fmm_setup, dim3(1), dim3(1), 0, stream, t.meta, f.params, f.cells);
     for (uint32_t d = 0; d < FMM_LEVELS; ++d) {
-      hipLaunchKernelGGL(k_fmm_split, dim3(GRID_BLOCKS), dim3(EWF), 0, stream, t.keys, f.cells, f.params, d, f.split,
+      hipLaunchKernelGGL(k_fmm_split, dim3(LEVEL_BLOCKS), dim3(EWF), 0, stream, t.keys, f.cells, f.params, d, f.split,
                          f.bounds, f.cell_cap);
       HIPCHECK(scan_u32(f.split, f.split_off, n > 0 ? n : 1, &f.params->level_n[d], t.partials,
                         &f.params->level_n[d + 1], stream));
-      hipLaunchKernelGGL(k_fmm_children, dim3(GRID_BLOCKS), dim3(EWF), 0, stream, f.cells, f.params, d, f.split,
+      hipLaunchKernelGGL(k_fmm_children, dim3(LEVEL_BLOCKS), dim3(EWF), 0, stream, f.cells, f.params, d, f.split,
                          f.split_off, f.bounds, f.cell_cap);
     }
     HIPCHECK(hipMemcpyAsync(f.params_host, f.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream));

@@ -106,8 +106,27 @@ struct Force {
   // fast link adjacency rows (link_fast.zig:105-130)
   bool csr_valid = false;
   uint32_t csr_n = 0;
-  Device<uint32_t> row, adj, heavy;
-  uint32_t heavy_count = 0;
+  Device<uint32_t> row, adj, owner, rows_long;
+  Device<LinkEntry> entries;
+  Device<double> terms;
+  uint32_t long_count = 0;
+  LinkCsrArgs csr_args(uint32_t n) const {
+    LinkCsrArgs a;
+    a.src = dsrc.p;
+    a.tgt = dtgt.p;
+    a.strength = df[0].p;
+    a.distance = df[1].p;
+    a.bias = df[2].p;
+    a.m = count;
+    a.n = n;
+    a.row = row.p;
+    a.adj = adj.p;
+    a.owner = owner.p;
+    a.rows_long = rows_long.p;
+    a.entries = entries.p;
+    a.terms = terms.p;
+    return a;
+  }
 
   explicit Force(Kind k) : kind(k) {}
   ~Force() {
@@ -122,7 +141,10 @@ struct Force {
     for (auto& a : lf) a.release();
     row.release();
     adj.release();
-    heavy.release();
+    owner.release();
+    rows_long.release();
+    entries.release();
+    terms.release();
   }
   bool node_sized() const { return kind != K_LINK && kind != K_CENTER; }  // force.zig:50-52
   int float_arrays() const {                                              // force.zig:54-61
@@ -166,6 +188,7 @@ struct Sim {
   Device<uint64_t> sk0, sk1;
   Device<uint32_t> sv0, sv1, shist, sglob, scount;
   Device<uint32_t> gcell_of, gstart;
+  Device<double4> gcand;
   Device<uint8_t> gpartials;
   GridParams* gparams = nullptr;
   hipStream_t stream = nullptr;
@@ -211,6 +234,7 @@ struct Sim {
     fmm_free(fmm);
     for (auto* a : {&sk0, &sk1}) a->release();
     for (auto* a : {&sv0, &sv1, &shist, &sglob, &scount, &gcell_of, &gstart}) a->release();
+    gcand.release();
     gpartials.release();
     if (gparams) (void)hipFree(gparams);
     flagged.release();
@@ -613,10 +637,12 @@ int32_t run_link_fast(Sim& s, Force& f, double alpha) {  // link_fast.zig:22-99
     if ((e = s.sort_buffers(2 * (size_t)m, &sb)) != hipSuccess) return status_of(e);
     if ((e = f.row.reserve((size_t)n + 1, 0, s.stream)) != hipSuccess) return status_of(e);
     if ((e = f.adj.reserve(2 * (size_t)m > 0 ? 2 * (size_t)m : 1, 0, s.stream)) != hipSuccess) return status_of(e);
-    if ((e = f.heavy.reserve(n > 0 ? n : 1, 0, s.stream)) != hipSuccess) return status_of(e);
-    if ((e = link_csr(f.dsrc.p, f.dtgt.p, m, n, f.row.p, f.adj.p, f.heavy.p, &f.heavy_count, sb, s.stream)) !=
-        hipSuccess)
-      return status_of(e);
+    const size_t c = m > 0 ? m : 1;
+    if ((e = f.owner.reserve(2 * c, 0, s.stream)) != hipSuccess) return status_of(e);
+    if ((e = f.entries.reserve(2 * c, 0, s.stream)) != hipSuccess) return status_of(e);
+    if ((e = f.terms.reserve(4 * c, 0, s.stream)) != hipSuccess) return status_of(e);
+    if ((e = f.rows_long.reserve(n > 0 ? n : 1, 0, s.stream)) != hipSuccess) return status_of(e);
+    if ((e = link_csr(f.csr_args(n), sb, &f.long_count, s.stream)) != hipSuccess) return status_of(e);
     f.csr_valid = true;
     f.csr_n = n;
   }
@@ -626,8 +652,7 @@ int32_t run_link_fast(Sim& s, Force& f, double alpha) {  // link_fast.zig:22-99
   for (double k = 0; k < iterations; k += 1) {
     e = shift_positions(s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, s.dpx.p, s.dpy.p, n, s.stream);
     if (e == hipSuccess)
-      e = link_sweep(n, f.row.p, f.adj.p, f.heavy.p, f.heavy_count, f.dsrc.p, f.dtgt.p, f.df[0].p, f.df[1].p,
-                     f.df[2].p, s.dpx.p, s.dpy.p, s.dvx.p, s.dvy.p, alpha, seed, s.stream);
+      e = link_sweep(f.csr_args(n), f.long_count, s.dpx.p, s.dpy.p, s.dvx.p, s.dvy.p, alpha, seed, s.stream);
     if (e != hipSuccess) return status_of(e);
   }
   s.timer_stop();
@@ -642,12 +667,14 @@ int32_t run_collide_fast(Sim& s, Force& f) {  // grid.zig:198-286
   const size_t c = n > 0 ? n : 1;
   if ((e = s.gcell_of.reserve(c, 0, s.stream)) != hipSuccess) return status_of(e);
   if ((e = s.gstart.reserve(4 * c + 17, 0, s.stream)) != hipSuccess) return status_of(e);
+  if ((e = s.gcand.reserve(c, 0, s.stream)) != hipSuccess) return status_of(e);
   if ((e = s.gpartials.reserve(grid_partials_bytes(), 0, s.stream)) != hipSuccess) return status_of(e);
   GridBuffers g;
   g.px = s.dpx.p;
   g.py = s.dpy.p;
   g.cell_of = s.gcell_of.p;
   g.start = s.gstart.p;
+  g.cand = s.gcand.p;
   g.partials = s.gpartials.p;
   g.params = s.gparams;
   const double iterations = f.scalar(S_ITERATIONS);

@@ -791,12 +791,13 @@ __global__ __launch_bounds__(EW_THREADS) void k_emit(const uint64_t* __restrict_
 // ---------------------------------------------------------------- host side
 
 // Stable sort of (keys, vals)[0 .. *count) by the 8-bit digits first_digit ..
-// first_digit + digits - 1 (digits even, so the result ends in keys/vals).
-// hist holds 256 x ceil(n_max / RS_TILE) counts, glob 256 x digits.
+// first_digit + digits - 1. hist holds 256 x ceil(n_max / RS_TILE) counts,
+// glob 256 x digits. After an odd number of passes the first n_max entries of
+// the ping-pong buffers are copied back.
 hipError_t sort_digits(uint32_t* hist, uint32_t* glob, uint64_t* keys, uint32_t* vals, uint64_t* keys_tmp,
                        uint32_t* vals_tmp, uint32_t n_max, const uint32_t* count, int first_digit, int digits,
                        hipStream_t stream) {
-  if (digits & 1) return hipErrorInvalidValue;
+  if (digits < 0 || digits > 8) return hipErrorInvalidValue;
   const uint32_t ntiles = ceil_div(n_max > 0 ? n_max : 1, RS_TILE);
   uint64_t *ka = keys, *kb = keys_tmp;
   uint32_t *va = vals, *vb = vals_tmp;
@@ -810,6 +811,10 @@ hipError_t sort_digits(uint32_t* hist, uint32_t* glob, uint64_t* keys, uint32_t*
                        ntiles, shift);
     std::swap(ka, kb);
     std::swap(va, vb);
+  }
+  if (digits & 1) {
+    HIPCHECK(hipMemcpyAsync(keys, ka, (size_t)n_max * sizeof(uint64_t), hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(vals, va, (size_t)n_max * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
   }
   return hipGetLastError();
 }
