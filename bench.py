@@ -144,6 +144,79 @@ def run_fast(core, n, steps, warmup):
     }
 
 
+def run_batch(core, graphs, steps, warmup, seed=1):
+    """The batched workload: `graphs` stochastic-block-model graphs (n
+    log-uniform in [50, 20000], 2-16 blocks, p_in 0.05 / p_out 0.002 scaled to
+    mean degree 6), each from d3's phyllotaxis start, ticked together with
+    manyBody (FMM order 8, opening 1.5) + link + center. A step is one tick of
+    every graph."""
+    import math
+    import numpy as np
+    from d3force_amd.batch import (Batch, BS_ALPHA, BS_CENTER_STRENGTH, BS_LINK_ITERATIONS, BS_VELOCITY_DECAY)
+    from scenarios import sbm_batch
+    t0 = time.time()
+    gs = sbm_batch(graphs, seed=seed)
+    b = Batch(gs, core=core)
+    n, m = b.n, b.m
+    log("batch: %d graphs, %d nodes, %d links ready in %.1fs" % (graphs, n, m, time.time() - t0))
+    b.tick(warmup)
+    device_sync(core)
+    b.stats()
+    core.batch_set_timing(1)
+    t0 = time.perf_counter()
+    b.tick(steps)
+    device_sync(core)
+    dt = time.perf_counter() - t0
+    core.batch_set_timing(0)
+    st = b.stats()
+    log("batch: %.3f ms/tick, phases %s" % (1e3 * dt / steps, st["ms"]))
+    b.close()
+    # Per-graph many-body error on the start layout of a sample of graphs: one
+    # manyBody-only tick from v = 0, against an FP64 direct sum (numpy) on up
+    # to 300 nodes of each sampled graph.
+    rng = np.random.default_rng(7)
+    pick = sorted(rng.choice(graphs, min(graphs, 16), replace=False).tolist())
+    probe = Batch([gs[g] for g in pick], core=core, seed_ids=pick)
+    probe.scalar(BS_LINK_ITERATIONS, 0).scalar(BS_CENTER_STRENGTH, 0).scalar(BS_VELOCITY_DECAY, 1)
+    s0 = probe.state()
+    probe.tick(1)
+    alpha = probe.scalar(BS_ALPHA)
+    s1 = probe.state()
+    errs = []
+    for k in range(len(pick)):
+        a, c = probe.graph_state(k, s0), probe.graph_state(k, s1)
+        sample = rng.choice(a.shape[1], min(a.shape[1], 300), replace=False)
+        e2 = r2 = 0.0
+        for i in sample:
+            dx, dy = a[0] - a[0][i], a[1] - a[1][i]
+            l = dx * dx + dy * dy
+            keep = l > 0
+            dx, dy, l = dx[keep], dy[keep], l[keep]
+            l = np.where(l < 1, np.sqrt(l), l)
+            w = -30.0 * alpha / l
+            fx, fy = np.sum(dx * w), np.sum(dy * w)
+            e2 += (c[2][i] - fx) ** 2 + (c[3][i] - fy) ** 2
+            r2 += fx * fx + fy * fy
+        errs.append(math.sqrt(e2 / r2))
+    probe.close()
+    sizes = [gs[g][0] for g in pick]
+    return {
+        "value": steps / dt, "unit": "ticks/s (every graph one tick)", "ms_per_step": 1e3 * dt / steps,
+        "graph_ticks_per_s": graphs * steps / dt, "node_ticks_per_s": n * steps / dt, "steps": steps,
+        "warmup": warmup, "dtype": "f64 positions and centres, f32 expansions / M2L / P2P",
+        "config": {"workload": "batched fast mode, %d stochastic-block-model graphs (n log-uniform in [50, 20000], "
+                               "2-16 blocks, p_in 0.05 / p_out 0.002 scaled to mean degree 6), %d nodes, %d links, "
+                               "phyllotaxis start per graph, manyBody FMM order 8 opening 1.5 + link + center"
+                               % (graphs, n, m), "graphs": graphs, "nodes": n, "links": m,
+                   "graphs_below_2000_nodes": int(sum(1 for g in gs if g[0] < 2000))},
+        "phases_ms_per_tick": {ph: v / steps for ph, v in st["ms"].items() if v > 0},
+        "fmm_work_per_tick": {"m2l_pairs": st["m2l"] / steps, "p2p_point_pairs": st["p2p_pairs"] / steps,
+                              "cells": st["fmm_cells"]},
+        "rms_force_error_vs_fp64_direct_sum": {"sampled_graphs": len(pick), "nodes": sizes,
+                                               "per_graph": errs, "max": max(errs)},
+    }
+
+
 def device_sync(core):
     """Waits for the device (the library's own hipDeviceSynchronize: the
     library links ROCm's HIP runtime directly, so torch is not loaded)."""
@@ -192,6 +265,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fast-nodes", type=int, default=1000000, help="nodes of the fast-mode leg (0: skip it)")
     ap.add_argument("--fast-only", action="store_true", help="run only the fast-mode leg (prints its object)")
+    ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs of the batched leg (0: skip it)")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched leg (prints its object)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the tick on a single MI355X "
@@ -201,6 +276,9 @@ def main():
     import d3force_amd as d3
     if args.fast_only:
         print(json.dumps(run_fast(d3.gpu_core(), args.fast_nodes, args.steps, args.warmup)))
+        return
+    if args.batch_only:
+        print(json.dumps(run_batch(d3.gpu_core(), args.batch_graphs, args.steps, args.warmup)))
         return
     n = args.nodes
     m = 2 * n
@@ -268,6 +346,7 @@ def main():
     roof = dict(kernels[dominant])
 
     fast = run_fast(core, args.fast_nodes, args.steps, args.warmup) if args.fast_nodes else None
+    batch = run_batch(core, args.batch_graphs, args.steps, args.warmup) if args.batch_graphs else None
 
     cpu = None
     if not args.no_cpu:
@@ -298,6 +377,7 @@ def main():
         "rooflines": kernels,
         "cpu_baseline": cpu,
         "fast": fast,
+        "batch": batch,
     }
     print(json.dumps(out))
 

@@ -1,0 +1,412 @@
+// Batched fast mode: many independent d3 simulations ("graphs") in one node
+// arena and one link arena, ticked together on one MI355X. The reference has
+// no counterpart (it runs one simulation per core instance); per graph, a
+// tick is the facade's fast tick with forceManyBody, forceLink and
+// forceCenter in that order, then integrate (simulation.js:49-89):
+//  - many-body: one forest FMM over every graph (fmm.hip): one key sort, one
+//    level-by-level build and one dual-tree traversal started from every
+//    root; a graph's cells, interaction lists and sums are the ones a run of
+//    that graph alone builds;
+//  - link: the adjacency sweep (link_fast.zig) over the arena's rows;
+//  - center: each graph's serial sums (simple.zig:14-31), one wave a graph;
+//  - integrate: per node.
+// Jiggle seeds are per graph and per tick, mixed from the graph's seed id, and
+// hash indices within the graph, so a graph's result does not depend on the
+// other graphs of the batch.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/d3fgpu.h"
+#include "buffers.h"
+#include "common.h"
+
+#define EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace d3f {
+namespace {
+
+enum BatchStatus : int32_t { B_OK = 0, B_BAD_HANDLE = -1, B_OUT_OF_MEMORY = -2, B_BAD_ARGUMENT = -3, B_FAILED = -4 };
+
+int32_t bstatus(hipError_t e) {
+  if (e == hipSuccess) return B_OK;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return B_OUT_OF_MEMORY;
+  return B_FAILED;
+}
+
+// splitmix64 of (id, tick, salt): the graph's seed for one force and tick.
+uint64_t mix_seed(uint64_t id, uint64_t tick, uint64_t salt) {
+  uint64_t h = id * 0x9E3779B97F4A7C15ull ^ (tick << 8 | salt);
+  h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+  h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+  return h ^ (h >> 31);
+}
+
+struct Batch {
+  uint32_t graphs = 0, n = 0, m = 0;
+  std::vector<uint32_t> node_off, link_off, node_graph;
+  Pinned<double> x, y, vx, vy, charge, lstrength, ldistance, lbias;
+  Pinned<uint32_t> source, target;
+  Pinned<uint64_t> seed_ids, seeds_mb, seeds_link;
+  Device<double> dx, dy, dvx, dvy, dcharge, dls, dld, dlb, dpx, dpy, dzero, dsums;
+  Device<uint8_t> dfixed;
+  Device<uint32_t> dsrc, dtgt, dnode_graph, dnode_off, dlink_off;
+  Device<uint64_t> dseeds_link;
+  // link rows
+  bool csr_valid = false;
+  Device<uint32_t> row, adj, owner, rows_long;
+  Device<LinkEntry> entries;
+  Device<double> terms;
+  uint32_t long_count = 0;
+  Device<uint64_t> sk0, sk1;
+  Device<uint32_t> sv0, sv1, shist, sglob, scount;
+  TreeBuffers tree;
+  FmmBuffers fmm;
+  hipStream_t stream = nullptr;
+  double scalars[D3F_BS_COUNT] = {0};
+  d3f_stats stats{};
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+
+  ~Batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto* a : {&x, &y, &vx, &vy, &charge, &lstrength, &ldistance, &lbias}) a->release();
+    for (auto* a : {&source, &target}) a->release();
+    for (auto* a : {&seed_ids, &seeds_mb, &seeds_link}) a->release();
+    for (auto* a : {&dx, &dy, &dvx, &dvy, &dcharge, &dls, &dld, &dlb, &dpx, &dpy, &dzero, &dsums, &terms}) a->release();
+    dfixed.release();
+    for (auto* a : {&dsrc, &dtgt, &dnode_graph, &dnode_off, &dlink_off, &row, &adj, &owner, &rows_long, &sv0, &sv1,
+                    &shist, &sglob, &scount})
+      a->release();
+    for (auto* a : {&dseeds_link, &sk0, &sk1}) a->release();
+    entries.release();
+    tree_free(tree);
+    fmm_free(fmm);
+    for (auto& p : pending) {
+      (void)hipEventDestroy(p.second.first);
+      (void)hipEventDestroy(p.second.second);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  hipError_t init() {
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(sglob.reserve(8 * 256));
+    HIPCHECK(scount.reserve(4));
+    // d3's defaults (simulation.js:18-31, manyBody.js:5-12, center.js:4-7)
+    scalars[D3F_BS_ALPHA] = 1;
+    scalars[D3F_BS_ALPHA_DECAY] = 1 - std::pow(0.001, 1.0 / 300);
+    scalars[D3F_BS_ALPHA_TARGET] = 0;
+    scalars[D3F_BS_VELOCITY_DECAY] = 0.6;
+    scalars[D3F_BS_DISTANCE_MIN2] = 1;
+    scalars[D3F_BS_DISTANCE_MAX2] = __builtin_huge_val();
+    scalars[D3F_BS_ORDER] = 8;
+    scalars[D3F_BS_OPENING] = 1.5;
+    scalars[D3F_BS_CENTER_STRENGTH] = 1;
+    scalars[D3F_BS_LINK_ITERATIONS] = 1;
+    return hipSuccess;
+  }
+
+  hipError_t set_graphs(uint32_t g, const uint32_t* noff, const uint32_t* loff) {
+    HIPCHECK(hipStreamSynchronize(stream));
+    graphs = g;
+    node_off.assign(noff, noff + g + 1);
+    link_off.assign(loff, loff + g + 1);
+    n = node_off[g];
+    m = link_off[g];
+    node_graph.resize(n);
+    for (uint32_t k = 0; k < g; ++k)
+      for (uint32_t i = node_off[k]; i < node_off[k + 1]; ++i) node_graph[i] = k;
+    const size_t cn = n > 0 ? n : 1, cm = m > 0 ? m : 1, cg = g > 0 ? g : 1;
+    for (auto* a : {&x, &y, &vx, &vy, &charge}) HIPCHECK(a->reserve(cn, 0));
+    for (auto* a : {&lstrength, &ldistance, &lbias}) HIPCHECK(a->reserve(cm, 0));
+    for (auto* a : {&source, &target}) HIPCHECK(a->reserve(cm, 0));
+    for (auto* a : {&seed_ids, &seeds_mb, &seeds_link}) HIPCHECK(a->reserve(cg, 0));
+    for (uint32_t k = 0; k < g; ++k) seed_ids.p[k] = k;
+    for (auto* a : {&dx, &dy, &dvx, &dvy, &dcharge, &dpx, &dpy, &dzero}) HIPCHECK(a->reserve(cn, 0, stream));
+    for (auto* a : {&dls, &dld, &dlb}) HIPCHECK(a->reserve(cm, 0, stream));
+    HIPCHECK(dsums.reserve(2 * cg, 0, stream));
+    HIPCHECK(dfixed.reserve(cn, 0, stream));
+    HIPCHECK(dsrc.reserve(cm, 0, stream));
+    HIPCHECK(dtgt.reserve(cm, 0, stream));
+    HIPCHECK(dnode_graph.reserve(cn, 0, stream));
+    HIPCHECK(dnode_off.reserve(cg + 1, 0, stream));
+    HIPCHECK(dlink_off.reserve(cg + 1, 0, stream));
+    HIPCHECK(dseeds_link.reserve(cg, 0, stream));
+    HIPCHECK(hipMemsetAsync(dfixed.p, 0, cn, stream));
+    HIPCHECK(hipMemsetAsync(dzero.p, 0, cn * sizeof(double), stream));
+    if (n) HIPCHECK(hipMemcpyAsync(dnode_graph.p, node_graph.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(dnode_off.p, node_off.data(), (g + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(dlink_off.p, link_off.data(), (g + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    csr_valid = false;
+    return hipStreamSynchronize(stream);
+  }
+
+  LinkCsrArgs csr_args() {
+    LinkCsrArgs a;
+    a.src = dsrc.p;
+    a.tgt = dtgt.p;
+    a.strength = dls.p;
+    a.distance = dld.p;
+    a.bias = dlb.p;
+    a.m = m;
+    a.n = n;
+    a.row = row.p;
+    a.adj = adj.p;
+    a.owner = owner.p;
+    a.rows_long = rows_long.p;
+    a.entries = entries.p;
+    a.terms = terms.p;
+    a.node_graph = dnode_graph.p;
+    a.link_base = dlink_off.p;
+    a.seeds = dseeds_link.p;
+    return a;
+  }
+
+  hipError_t build_rows() {
+    const size_t c = m > 0 ? m : 1;
+    for (auto* a : {&sk0, &sk1}) HIPCHECK(a->reserve(2 * c, 0, stream));
+    for (auto* a : {&sv0, &sv1}) HIPCHECK(a->reserve(2 * c, 0, stream));
+    HIPCHECK(shist.reserve(sort_hist_words((uint32_t)(2 * c)), 0, stream));
+    HIPCHECK(row.reserve((size_t)n + 1, 0, stream));
+    HIPCHECK(adj.reserve(2 * c, 0, stream));
+    HIPCHECK(owner.reserve(2 * c, 0, stream));
+    HIPCHECK(entries.reserve(2 * c, 0, stream));
+    HIPCHECK(terms.reserve(4 * c, 0, stream));
+    HIPCHECK(rows_long.reserve(n > 0 ? n : 1, 0, stream));
+    SortBuffers sb;
+    sb.k0 = sk0.p;
+    sb.k1 = sk1.p;
+    sb.v0 = sv0.p;
+    sb.v1 = sv1.p;
+    sb.hist = shist.p;
+    sb.glob = sglob.p;
+    sb.count = scount.p;
+    sb.cap = (uint32_t)(2 * c);
+    HIPCHECK(link_csr(csr_args(), sb, &long_count, stream));
+    csr_valid = true;
+    return hipSuccess;
+  }
+
+  void timer(int phase, hipEvent_t a) {
+    hipEvent_t b = nullptr;
+    if (!a || hipEventCreate(&b) != hipSuccess) return;
+    (void)hipEventRecord(b, stream);
+    pending.push_back({phase, {a, b}});
+  }
+  hipEvent_t mark() {
+    hipEvent_t a = nullptr;
+    if (hipEventCreate(&a) != hipSuccess) return nullptr;
+    (void)hipEventRecord(a, stream);
+    return a;
+  }
+
+  int32_t tick() {
+    hipError_t e;
+    double& alpha = scalars[D3F_BS_ALPHA];
+    alpha += (scalars[D3F_BS_ALPHA_TARGET] - alpha) * scalars[D3F_BS_ALPHA_DECAY];  // simulation.js:64
+    const uint64_t t = (uint64_t)scalars[D3F_BS_TICK];
+    for (uint32_t g = 0; g < graphs; ++g) {
+      seeds_mb.p[g] = mix_seed(seed_ids.p[g], t, 0);
+      seeds_link.p[g] = mix_seed(seed_ids.p[g], t, 1);
+    }
+    // many-body
+    FmmArgs a;
+    a.x = dx.p;
+    a.y = dy.p;
+    a.strength = dcharge.p;
+    a.vx = dvx.p;
+    a.vy = dvy.p;
+    a.n = n;
+    const double order = scalars[D3F_BS_ORDER];
+    a.order = order >= 1 && order <= (double)FMM_MAX_ORDER ? (uint32_t)order : 8;
+    a.alpha = alpha;
+    a.opening = scalars[D3F_BS_OPENING];
+    a.dmin2 = scalars[D3F_BS_DISTANCE_MIN2];
+    a.dmax2 = scalars[D3F_BS_DISTANCE_MAX2];
+    a.graphs = graphs;
+    a.node_off = node_off.data();
+    a.node_graph = dnode_graph.p;
+    a.seeds = seeds_mb.p;
+    hipEvent_t ev = g_batch_timing ? mark() : nullptr;
+    if ((e = fmm_run(fmm, tree, a, stream)) != hipSuccess) return bstatus(e);
+    timer(D3F_PHASE_FMM, ev);
+    if (fmm.params_host->declined) return B_BAD_ARGUMENT;  // a graph with a non-finite extent
+    stats.m2l += fmm.params_host->m2l;
+    stats.p2p_pairs += fmm.params_host->pairs;
+    stats.fmm_cells = fmm.params_host->level_off[FMM_LEVELS + 1];
+    // link
+    ev = g_batch_timing ? mark() : nullptr;
+    if (!csr_valid && (e = build_rows()) != hipSuccess) return bstatus(e);
+    if ((e = hipMemcpyAsync(dseeds_link.p, seeds_link.p, graphs * sizeof(uint64_t), hipMemcpyHostToDevice, stream)) !=
+        hipSuccess)
+      return bstatus(e);
+    for (double k = 0; k < scalars[D3F_BS_LINK_ITERATIONS]; k += 1) {
+      e = shift_positions(dx.p, dy.p, dvx.p, dvy.p, dpx.p, dpy.p, n, stream);
+      if (e == hipSuccess) e = link_sweep(csr_args(), long_count, dpx.p, dpy.p, dvx.p, dvy.p, alpha, 0, stream);
+      if (e != hipSuccess) return bstatus(e);
+    }
+    timer(D3F_PHASE_LINK, ev);
+    // center
+    ev = g_batch_timing ? mark() : nullptr;
+    if ((e = batch_center(dx.p, dy.p, dnode_off.p, dnode_graph.p, graphs, n, dsums.p, scalars[D3F_BS_CENTER_X],
+                          scalars[D3F_BS_CENTER_Y], scalars[D3F_BS_CENTER_STRENGTH], stream)) != hipSuccess)
+      return bstatus(e);
+    timer(D3F_PHASE_CENTER, ev);
+    // integrate
+    ev = g_batch_timing ? mark() : nullptr;
+    if ((e = integrate(dx.p, dy.p, dvx.p, dvy.p, dzero.p, dzero.p, dfixed.p, n, scalars[D3F_BS_VELOCITY_DECAY],
+                       stream)) != hipSuccess)
+      return bstatus(e);
+    timer(D3F_PHASE_INTEGRATE, ev);
+    scalars[D3F_BS_TICK] += 1;
+    return B_OK;
+  }
+
+  static bool g_batch_timing;
+};
+bool Batch::g_batch_timing = false;
+
+std::vector<Batch*> g_batches;
+
+Batch* bget(uint32_t h) {
+  if (h == 0 || h > g_batches.size()) return nullptr;
+  return g_batches[h - 1];
+}
+
+}  // namespace
+}  // namespace d3f
+
+using namespace d3f;
+
+EXPORT uint32_t batch_create(void) {
+  int devices = 0;
+  if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) return 0;
+  Batch* b = new (std::nothrow) Batch();
+  if (!b) return 0;
+  if (b->init() != hipSuccess) {
+    delete b;
+    return 0;
+  }
+  for (size_t i = 0; i < g_batches.size(); ++i)
+    if (!g_batches[i]) {
+      g_batches[i] = b;
+      return (uint32_t)i + 1;
+    }
+  g_batches.push_back(b);
+  return (uint32_t)g_batches.size();
+}
+
+EXPORT int32_t batch_destroy(uint32_t h) {
+  Batch* b = bget(h);
+  if (!b) return B_BAD_HANDLE;
+  delete b;
+  g_batches[h - 1] = nullptr;
+  return B_OK;
+}
+
+EXPORT int32_t batch_set_graphs(uint32_t h, uint32_t graphs, const uint32_t* node_off, const uint32_t* link_off) {
+  Batch* b = bget(h);
+  if (!b) return B_BAD_HANDLE;
+  if (graphs == 0 || !node_off || !link_off || node_off[0] != 0 || link_off[0] != 0) return B_BAD_ARGUMENT;
+  for (uint32_t g = 0; g < graphs; ++g)
+    if (node_off[g + 1] < node_off[g] || link_off[g + 1] < link_off[g]) return B_BAD_ARGUMENT;
+  return bstatus(b->set_graphs(graphs, node_off, link_off));
+}
+
+EXPORT void* batch_ptr(uint32_t h, uint32_t which) {
+  Batch* b = bget(h);
+  if (!b || b->graphs == 0) return nullptr;
+  switch (which) {
+    case D3F_B_X: return b->x.p;
+    case D3F_B_Y: return b->y.p;
+    case D3F_B_VX: return b->vx.p;
+    case D3F_B_VY: return b->vy.p;
+    case D3F_B_CHARGE: return b->charge.p;
+    case D3F_B_SOURCE: b->csr_valid = false; return b->source.p;
+    case D3F_B_TARGET: b->csr_valid = false; return b->target.p;
+    case D3F_B_LINK_STRENGTH: b->csr_valid = false; return b->lstrength.p;
+    case D3F_B_LINK_DISTANCE: b->csr_valid = false; return b->ldistance.p;
+    case D3F_B_LINK_BIAS: b->csr_valid = false; return b->lbias.p;
+    case D3F_B_SEED: return b->seed_ids.p;
+    default: return nullptr;
+  }
+}
+
+EXPORT int32_t batch_sync(uint32_t h, uint32_t direction) {
+  Batch* b = bget(h);
+  if (!b) return B_BAD_HANDLE;
+  const size_t bn = (size_t)b->n * sizeof(double), bm = (size_t)b->m * sizeof(double);
+  hipError_t e = hipSuccess;
+  hipStream_t s = b->stream;
+  if (direction == D3F_SYNC_H2D) {
+    for (uint32_t g = 0; g < b->graphs; ++g)  // link endpoints stay inside their graph
+      for (uint32_t k = b->link_off[g]; k < b->link_off[g + 1]; ++k)
+        if (b->source.p[k] < b->node_off[g] || b->source.p[k] >= b->node_off[g + 1] ||
+            b->target.p[k] < b->node_off[g] || b->target.p[k] >= b->node_off[g + 1])
+          return B_BAD_ARGUMENT;
+    struct { void* d; const void* p; size_t bytes; } copies[] = {
+        {b->dx.p, b->x.p, bn}, {b->dy.p, b->y.p, bn}, {b->dvx.p, b->vx.p, bn}, {b->dvy.p, b->vy.p, bn},
+        {b->dcharge.p, b->charge.p, bn}, {b->dls.p, b->lstrength.p, bm}, {b->dld.p, b->ldistance.p, bm},
+        {b->dlb.p, b->lbias.p, bm}, {b->dsrc.p, b->source.p, (size_t)b->m * 4}, {b->dtgt.p, b->target.p, (size_t)b->m * 4}};
+    for (auto& c : copies)
+      if (e == hipSuccess && c.bytes) e = hipMemcpyAsync(c.d, c.p, c.bytes, hipMemcpyHostToDevice, s);
+    b->csr_valid = false;
+  } else if (direction == D3F_SYNC_D2H) {
+    struct { void* p; const void* d; } copies[] = {{b->x.p, b->dx.p}, {b->y.p, b->dy.p}, {b->vx.p, b->dvx.p},
+                                                   {b->vy.p, b->dvy.p}};
+    for (auto& c : copies)
+      if (e == hipSuccess && bn) e = hipMemcpyAsync(c.p, c.d, bn, hipMemcpyDeviceToHost, s);
+  } else {
+    return B_BAD_ARGUMENT;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return bstatus(e);
+}
+
+EXPORT int32_t batch_set_scalar(uint32_t h, uint32_t which, double v) {
+  Batch* b = bget(h);
+  if (!b) return B_BAD_HANDLE;
+  if (which >= D3F_BS_COUNT) return B_BAD_ARGUMENT;
+  b->scalars[which] = v;
+  return B_OK;
+}
+
+EXPORT double batch_get_scalar(uint32_t h, uint32_t which) {
+  Batch* b = bget(h);
+  if (!b || which >= D3F_BS_COUNT) return 0;
+  return b->scalars[which];
+}
+
+EXPORT int32_t batch_tick(uint32_t h, uint32_t ticks) {
+  Batch* b = bget(h);
+  if (!b) return B_BAD_HANDLE;
+  if (b->graphs == 0) return B_BAD_ARGUMENT;
+  for (uint32_t k = 0; k < ticks; ++k) {
+    const int32_t st = b->tick();
+    if (st != B_OK) return st;
+  }
+  return B_OK;
+}
+
+EXPORT void batch_set_timing(int on) { Batch::g_batch_timing = on != 0; }
+
+EXPORT int32_t batch_stats(uint32_t h, d3f_stats* out) {
+  Batch* b = bget(h);
+  if (!b || !out) return B_BAD_HANDLE;
+  hipError_t e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return bstatus(e);
+  for (auto& p : b->pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+      b->stats.ms[p.first] += ms;
+      b->stats.launches[p.first] += 1;
+    }
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  b->pending.clear();
+  *out = b->stats;
+  b->stats = d3f_stats{};
+  return B_OK;
+}

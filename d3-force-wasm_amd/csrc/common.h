@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace d3f {
 
 constexpr uint32_t NONE = 0xffffffffu;
@@ -208,6 +210,11 @@ struct LinkCsrArgs {
   uint32_t *row, *adj, *owner, *rows_long;  // row[n + 1], adj[2m], owner[2m], rows_long[n]
   LinkEntry* entries;                       // [2m]
   double* terms;                            // [4m] scratch of a sweep
+  // A batch of graphs (null for one simulation): each node's graph, each
+  // graph's first link and jiggle seed.
+  const uint32_t* node_graph = nullptr;
+  const uint32_t* link_base = nullptr;
+  const uint64_t* seeds = nullptr;
 };
 // Adjacency rows of the links (link_fast.zig:105-130) and their packed
 // entries; *long_count (host) receives the number of rows longer than the
@@ -230,15 +237,26 @@ constexpr uint32_t FMM_MAX_STEPS = 2 * FMM_LEVELS + 2;  // dual-tree rounds
 // contiguous; centre and radius in the unit frame u = (x - ox) / w.
 struct FCell {
   double cx, cy, r;
-  uint32_t first, count, child0, nchild, depth;
+  uint32_t first, count, child0, nchild, depth, graph;
 };
 static_assert(sizeof(FCell) == 48, "fast cell layout");
 
+// One graph of a fast many-body run (the runtime's simulation is a forest of
+// one): its box and the reference's fast extent and unit frame
+// (tree.zig:357-359, fmm.zig:253-270), its scaled distance limits, its
+// jiggle seed, its root cell and its points' range in Morton order.
+struct FmmGraph {
+  double box[4], extent[4];
+  uint32_t node_base, node_end;      // the graph's nodes [node_base, node_end)
+  double scale;                      // 65536 / side (tree.zig:360)
+  double ox, oy, w, inv_w, dmin_s, dmax_s;
+  uint64_t seed;
+  uint32_t m, ok, ring, root, point_off, pad;
+};
+
 // Device-resident scalars of one fast many-body run.
 struct FmmParams {
-  double extent[4];
-  double ox, oy, w, inv_w;
-  uint32_t m, ok;                          // ok: w finite and positive
+  uint32_t n, m_total, roots, declined;    // nodes, points in trees, graphs with a tree, non-finite extents
   uint32_t cell_overflow;
   uint32_t level_n[FMM_LEVELS + 2];        // cells per depth
   uint32_t level_off[FMM_LEVELS + 3];      // first cell of each depth
@@ -251,6 +269,14 @@ struct FmmParams {
 
 struct FmmBuffers {
   uint32_t cell_cap = 0, point_cap = 0, front_cap = 0, m2l_cap = 0, p2p_cap = 0, cellarr_cap = 0;
+  uint32_t graph_cap = 0, chunk_cap = 0;
+  FmmGraph* graphs = nullptr;              // per graph
+  uint64_t* seeds = nullptr;
+  uint32_t* node_off_dev = nullptr;
+  uint32_t *chunk_lo = nullptr, *chunk_hi = nullptr, *chunk_first = nullptr;
+  void* box_parts = nullptr;
+  std::vector<uint32_t> node_off_host;     // the forest the chunk arrays describe
+  uint32_t chunks = 0;
   size_t exp_cap = 0;                      // complex coefficients mexp and lexp hold
   FCell* cells = nullptr;
   double* bbox = nullptr;                  // 4 per cell
@@ -273,11 +299,20 @@ struct FmmArgs {
   double *vx, *vy;
   uint32_t n, order;
   double alpha, opening, dmin2, dmax2;
-  uint64_t seed;
+  // The forest: graphs g own nodes [node_off[g], node_off[g + 1]) (host
+  // array of graphs + 1 entries); node_graph (device, n entries) maps a node
+  // to its graph and may be null for one graph; seeds (host) holds each
+  // graph's jiggle seed.
+  uint32_t graphs;
+  const uint32_t* node_off;
+  const uint32_t* node_graph;
+  const uint64_t* seeds;
 };
-// Fast many-body (fmm.zig:247-354). Returns hipErrorNotSupported when the
-// FMM declines (a non-finite extent), and the caller then runs the exact
-// kernel, as the reference does.
+// Fast many-body (fmm.zig:247-354) over every graph of the forest at once.
+// With one graph, returns hipErrorNotSupported when the FMM declines (a
+// non-finite extent) and the caller then runs the exact kernel, as the
+// reference does; with more, declined graphs are counted in
+// params_host->declined and left unchanged.
 hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t stream);
 // out[2q], out[2q + 1] = FP64 direct-sum many-body velocity increment of node
 // sample[q] (device pointers).
@@ -298,5 +333,10 @@ hipError_t radial_force(const double* x, const double* y, double* vx, double* vy
                         const double* radius, uint32_t n, double cx, double cy, double alpha, hipStream_t stream);
 hipError_t shift_positions(const double* x, const double* y, const double* vx, const double* vy, double* px,
                            double* py, uint32_t n, hipStream_t stream);
+// A batch of graphs: sums[2g], sums[2g + 1] = d3's serial sums of graph g's
+// x and y (nodes node_off[g] .. node_off[g + 1] - 1 in order), then
+// x -= (sx / n_g - cx) * strength per node (simple.zig:14-31 per graph).
+hipError_t batch_center(double* x, double* y, const uint32_t* node_off, const uint32_t* node_graph, uint32_t graphs,
+                        uint32_t n, double* sums, double cx, double cy, double strength, hipStream_t stream);
 
 }  // namespace d3f

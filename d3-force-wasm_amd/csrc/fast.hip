@@ -80,7 +80,9 @@ __global__ __launch_bounds__(EW) void k_csr_pack(uint32_t total, const uint64_t*
                                                  const uint32_t* __restrict__ tgt,
                                                  const double* __restrict__ strength,
                                                  const double* __restrict__ distance,
-                                                 const double* __restrict__ bias, uint32_t* __restrict__ owner,
+                                                 const double* __restrict__ bias,
+                                                 const uint32_t* __restrict__ node_graph,
+                                                 const uint32_t* __restrict__ link_base, uint32_t* __restrict__ owner,
                                                  LinkEntry* __restrict__ entries) {
   const uint32_t a = blockIdx.x * EW + threadIdx.x;
   if (a >= total) return;
@@ -88,8 +90,13 @@ __global__ __launch_bounds__(EW) void k_csr_pack(uint32_t total, const uint64_t*
   const uint32_t e = entry & ~TARGET_BIT;
   const bool is_target = (entry & TARGET_BIT) != 0;
   const double b = bias[e];
-  entries[a] = LinkEntry{is_target ? src[e] : tgt[e], entry, strength[e], distance[e], is_target ? b : 1 - b};
-  owner[a] = (uint32_t)keys[a];
+  const uint32_t i = (uint32_t)keys[a];
+  // The jiggle hashes the edge's index within its graph (the whole link
+  // array for one simulation).
+  const uint32_t local = node_graph ? e - link_base[node_graph[i]] : e;
+  entries[a] = LinkEntry{is_target ? src[e] : tgt[e], local | (entry & TARGET_BIT), strength[e], distance[e],
+                         is_target ? b : 1 - b};
+  owner[a] = i;
 }
 
 // Phase 1 of a sweep (link_fast.zig:60-99 on the snapshot px = x + vx):
@@ -97,10 +104,13 @@ __global__ __launch_bounds__(EW) void k_csr_pack(uint32_t total, const uint64_t*
 __global__ __launch_bounds__(EW) void k_link_terms(uint32_t total, const uint32_t* __restrict__ owner,
                                                    const LinkEntry* __restrict__ entries,
                                                    const double* __restrict__ px, const double* __restrict__ py,
-                                                   double2* __restrict__ terms, double alpha, uint64_t seed) {
+                                                   double2* __restrict__ terms, double alpha, uint64_t seed,
+                                                   const uint32_t* __restrict__ node_graph,
+                                                   const uint64_t* __restrict__ seeds) {
   const uint32_t a = blockIdx.x * EW + threadIdx.x;
   if (a >= total) return;
   const uint32_t i = owner[a];
+  if (node_graph) seed = seeds[node_graph[i]];
   double tx, ty;
   link_term(entries[a], px, py, px[i], py[i], alpha, seed, tx, ty);
   terms[a] = make_double2(tx, ty);
@@ -402,7 +412,7 @@ hipError_t link_csr(const LinkCsrArgs& a, const SortBuffers& sb, uint32_t* long_
     for (uint64_t v = n; v > 0; v >>= 8) ++digits;
     HIPCHECK(sort_digits(sb.hist, sb.glob, sb.k0, a.adj, sb.k1, sb.v1, total, sb.count, 0, digits, stream));
     hipLaunchKernelGGL(k_csr_pack, dim3(ceil_div(total, EW)), dim3(EW), 0, stream, total, sb.k0, a.adj, a.src, a.tgt,
-                       a.strength, a.distance, a.bias, a.owner, a.entries);
+                       a.strength, a.distance, a.bias, a.node_graph, a.link_base, a.owner, a.entries);
   }
   hipLaunchKernelGGL(k_csr_rows, dim3(ceil_div(n + 1, EW)), dim3(EW), 0, stream, sb.k0, total, n, a.row);
   HIPCHECK(hipMemsetAsync(sb.count, 0, sizeof(uint32_t), stream));
@@ -417,7 +427,7 @@ hipError_t link_sweep(const LinkCsrArgs& a, uint32_t long_count, const double* p
   const uint32_t total = 2 * a.m;
   if (a.n == 0 || total == 0) return hipSuccess;
   hipLaunchKernelGGL(k_link_terms, dim3(ceil_div(total, EW)), dim3(EW), 0, stream, total, a.owner, a.entries, px, py,
-                     (double2*)a.terms, alpha, seed);
+                     (double2*)a.terms, alpha, seed, a.node_graph, a.seeds);
   hipLaunchKernelGGL(k_link_rows, dim3(ceil_div(a.n, EW)), dim3(EW), 0, stream, a.n, a.row, (const double2*)a.terms,
                      vx, vy);
   if (long_count)

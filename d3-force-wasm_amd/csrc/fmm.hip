@@ -28,6 +28,7 @@
 //   L^_l(A) += (-1)^l b^l w sum_k C(k+l, k) M^_k(B) a^k,
 // which is the reference's T_{k+l} = (k+l)! / z0^{k+l+1} M2L (fmm.zig:561-610)
 // with its 1/k! and (-1)^l / l! scalings (fmm.zig:629-657) folded in.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <utility>
@@ -65,35 +66,222 @@ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s,
 
 // ---------------------------------------------------------------- build
 
-__global__ void k_fmm_setup(TreeMeta* __restrict__ meta, FmmParams* __restrict__ fp, FCell* __restrict__ cells) {
-  FmmParams p;
-  memset(&p, 0, sizeof(p));
-  const double nan = __builtin_nan(""), inf = __builtin_huge_val();
-  for (int k = 0; k < 4; ++k) p.extent[k] = nan;
-  p.m = meta->valid ? meta->m : 0;
-  if (p.m) {  // tree.zig:357-359
-    const double b0 = meta->box[0], b1 = meta->box[1], b2 = meta->box[2], b3 = meta->box[3];
-    double side = zmax(b2 - b0, b3 - b1);
-    if (!(side > 0)) side = 1;
-    p.extent[0] = b0;
-    p.extent[1] = b1;
-    p.extent[2] = b0 + side;
-    p.extent[3] = b1 + side;
-    p.w = p.extent[2] - p.extent[0];  // fmm.zig:255-257
-    p.ok = p.w > 0 && p.w < inf;
-    p.inv_w = 1 / p.w;
-    p.ox = (p.extent[0] + p.extent[2]) / 2;
-    p.oy = (p.extent[1] + p.extent[3]) / 2;
+// A chunk's box of its non-NaN points; equal values go to the lower node,
+// as the serial scan keeps the first (tree.zig:237-244).
+struct BoxPart {
+  double x0, y0, x1, y1;
+  uint32_t ix0, iy0, ix1, iy1, m, pad;
+};
+__device__ inline void bmin(double& v, uint32_t& i, double bv, uint32_t bi) {
+  if (bv < v || (bv == v && bi < i)) { v = bv; i = bi; }
+}
+__device__ inline void bmax(double& v, uint32_t& i, double bv, uint32_t bi) {
+  if (bv > v || (bv == v && bi < i)) { v = bv; i = bi; }
+}
+__device__ inline void bcombine(BoxPart& a, const BoxPart& b) {
+  bmin(a.x0, a.ix0, b.x0, b.ix0);
+  bmin(a.y0, a.iy0, b.y0, b.iy0);
+  bmax(a.x1, a.ix1, b.x1, b.ix1);
+  bmax(a.y1, a.iy1, b.y1, b.iy1);
+  a.m += b.m;
+}
+__device__ inline BoxPart bempty() {
+  const double inf = __builtin_huge_val();
+  return BoxPart{inf, inf, -inf, -inf, NONE, NONE, NONE, NONE, 0, 0};
+}
+
+__global__ void k_forest_reset_overflow(FmmParams* __restrict__ fp) {
+  if (threadIdx.x == 0) fp->cell_overflow = 0;
+}
+
+__global__ void k_forest_reset(FmmParams* __restrict__ fp, uint32_t n) {
+  if (threadIdx.x == 0) {
+    fp->n = n;
+    fp->m_total = 0;
+    fp->roots = 0;
+    fp->declined = 0;
+    fp->cell_overflow = 0;
   }
-  p.level_n[0] = (p.m && p.ok) ? 1 : 0;
-  p.level_off[0] = 0;
-  p.level_off[1] = p.level_n[0];
-  *fp = p;
-  if (p.level_n[0]) cells[0] = FCell{0, 0, 0, 0, p.m, 0, 0, 0};
-  // prepareFast leaves no exact tree behind (tree.zig:351-356).
-  for (int k = 0; k < 4; ++k) meta->extent[k] = p.extent[k];
-  meta->len = 0;
-  meta->valid = 0;
+}
+
+// One block per chunk of a graph's nodes (chunks never straddle graphs).
+__global__ __launch_bounds__(EWF) void k_forest_box(const double* __restrict__ xs, const double* __restrict__ ys,
+                                                    const uint32_t* __restrict__ chunk_lo,
+                                                    const uint32_t* __restrict__ chunk_hi,
+                                                    BoxPart* __restrict__ parts) {
+  const uint32_t lo = chunk_lo[blockIdx.x], hi = chunk_hi[blockIdx.x];
+  BoxPart p = bempty();
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += EWF) {
+    const double x = xs[i], y = ys[i];
+    if (x != x || y != y) continue;
+    bmin(p.x0, p.ix0, x, i);
+    bmin(p.y0, p.iy0, y, i);
+    bmax(p.x1, p.ix1, x, i);
+    bmax(p.y1, p.iy1, y, i);
+    p.m++;
+  }
+  __shared__ BoxPart sh[EWF];
+  sh[threadIdx.x] = p;
+  __syncthreads();
+  for (int k = EWF / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) bcombine(sh[threadIdx.x], sh[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) parts[blockIdx.x] = sh[0];
+}
+
+// One thread per graph: its box, the fast extent (tree.zig:357-359), the unit
+// frame (fmm.zig:253-270) and the scaled distance limits (fmm.zig:272-281).
+__global__ __launch_bounds__(64) void k_forest_graphs(const BoxPart* __restrict__ parts,
+                                                      const uint32_t* __restrict__ chunk_first, uint32_t graphs,
+                                                      double dmin2, double dmax2, const uint64_t* __restrict__ seeds,
+                                                      const uint32_t* __restrict__ node_off,
+                                                      FmmGraph* __restrict__ gp, FmmParams* __restrict__ fp) {
+  const uint32_t g = blockIdx.x * 64 + threadIdx.x;
+  if (g >= graphs) return;
+  BoxPart b = bempty();
+  for (uint32_t c = chunk_first[g]; c < chunk_first[g + 1]; ++c) bcombine(b, parts[c]);
+  FmmGraph r;
+  memset(&r, 0, sizeof(r));
+  const double nan = __builtin_nan(""), inf = __builtin_huge_val();
+  r.box[0] = b.x0;
+  r.box[1] = b.y0;
+  r.box[2] = b.x1;
+  r.box[3] = b.y1;
+  for (int k = 0; k < 4; ++k) r.extent[k] = nan;
+  r.m = b.m;
+  r.seed = seeds[g];
+  r.root = NONE;
+  r.node_base = node_off[g];
+  r.node_end = node_off[g + 1];
+  if (b.m) {
+    double side = zmax(b.x1 - b.x0, b.y1 - b.y0);
+    if (!(side > 0)) side = 1;
+    r.extent[0] = b.x0;
+    r.extent[1] = b.y0;
+    r.extent[2] = b.x0 + side;
+    r.extent[3] = b.y0 + side;
+    r.scale = 65536.0 / side;
+    r.w = r.extent[2] - r.extent[0];
+    r.ok = r.w > 0 && r.w < inf;
+    r.inv_w = 1 / r.w;
+    r.ox = (r.extent[0] + r.extent[2]) / 2;
+    r.oy = (r.extent[1] + r.extent[3]) / 2;
+    r.dmax_s = dmax2 < inf ? __builtin_sqrt(dmax2) / r.w : inf;
+    r.dmin_s = dmin2 > 0 ? __builtin_sqrt(dmin2) / r.w : 0;
+    r.ring = r.dmin_s > 0 || r.dmax_s < inf;
+    if (!r.ok) atomicAdd(&fp->declined, 1u);
+  }
+  gp[g] = r;
+  if (b.m) atomicAdd(&fp->m_total, b.m);
+}
+
+// Bits 0..15 of v moved to the even bit positions (tree.zig:688-695).
+__device__ inline uint32_t spread16(uint32_t v) {
+  v &= 0xffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+// Fast keys (tree.zig:346-364) with the graph above the 32 Morton bits, so
+// one sort orders every graph's points, graph by graph.
+__global__ __launch_bounds__(EWF) void k_forest_keys(const double* __restrict__ xs, const double* __restrict__ ys,
+                                                     uint32_t n, const uint32_t* __restrict__ node_graph,
+                                                     const FmmGraph* __restrict__ gp,
+                                                     const FmmParams* __restrict__ fp, uint64_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ perm, uint64_t* __restrict__ keys_all,
+                                                     uint32_t* __restrict__ flags) {
+  const uint32_t i = blockIdx.x * EWF + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t g = node_graph ? node_graph[i] : 0;
+  const double x = xs[i], y = ys[i];
+  const bool valid = !(x != x || y != y);
+  uint64_t key = 0;
+  if (valid) {
+    const FmmGraph& G = gp[g];
+    double fx = floor((x - G.box[0]) * G.scale), fy = floor((y - G.box[1]) * G.scale);
+    fx = (65535.0 < fx || fx != fx) ? 65535.0 : fx;
+    fy = (65535.0 < fy || fy != fy) ? 65535.0 : fy;
+    key = (uint64_t)g << 32 | (uint64_t)(spread16((uint32_t)fx) | spread16((uint32_t)fy) << 1);
+  }
+  if (fp->m_total == n) {
+    keys[i] = key;
+    perm[i] = i;
+  } else {
+    keys_all[i] = key;
+    flags[i] = valid ? 1u : 0u;
+  }
+}
+
+__global__ __launch_bounds__(EWF) void k_forest_compact(uint32_t n, const FmmParams* __restrict__ fp,
+                                                        const uint64_t* __restrict__ keys_all,
+                                                        const uint32_t* __restrict__ flags,
+                                                        const uint32_t* __restrict__ offs, uint64_t* __restrict__ keys,
+                                                        uint32_t* __restrict__ perm) {
+  const uint32_t i = blockIdx.x * EWF + threadIdx.x;
+  if (i >= n || fp->m_total == n || !flags[i]) return;
+  keys[offs[i]] = keys_all[i];
+  perm[offs[i]] = i;
+}
+
+// One block: each graph's first point in Morton order (the sorted keys hold
+// the graphs in order) and a root cell for every graph with points and a
+// finite extent, roots numbered in graph order (level 0 of the forest).
+__global__ __launch_bounds__(1024) void k_forest_roots(FmmGraph* __restrict__ gp, uint32_t graphs,
+                                                       FmmParams* __restrict__ fp, FCell* __restrict__ cells,
+                                                       uint32_t cell_cap, TreeMeta* __restrict__ meta) {
+  __shared__ uint32_t wsum[2][16];
+  __shared__ uint32_t carry[2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+  __syncthreads();
+  for (uint32_t g0 = 0; g0 < graphs; g0 += 1024) {
+    const uint32_t g = g0 + threadIdx.x;
+    const bool in = g < graphs;
+    const uint32_t m = in ? gp[g].m : 0;
+    const uint32_t root = in && m && gp[g].ok ? 1u : 0u;
+    uint32_t v[2] = {m, root}, inc[2];
+    for (int k = 0; k < 2; ++k) {
+      inc[k] = v[k];
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(inc[k], d, 64);
+        if (lane >= d) inc[k] += t;
+      }
+      if (lane == 63) wsum[k][wave] = inc[k];
+    }
+    __syncthreads();
+    uint32_t before[2] = {carry[0], carry[1]}, total[2] = {0, 0};
+    for (int w = 0; w < 16; ++w)
+      for (int k = 0; k < 2; ++k) {
+        if (w < wave) before[k] += wsum[k][w];
+        total[k] += wsum[k][w];
+      }
+    if (in) {
+      const uint32_t first = before[0] + inc[0] - m, idx = before[1] + inc[1] - root;
+      gp[g].point_off = first;
+      gp[g].root = root ? idx : NONE;
+      if (root && idx < cell_cap) cells[idx] = FCell{0, 0, 0, first, m, 0, 0, 0, g};
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      carry[0] += total[0];
+      carry[1] += total[1];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    fp->roots = carry[1];
+    fp->level_n[0] = carry[1];
+    fp->level_off[0] = 0;
+    fp->level_off[1] = carry[1];
+    // prepareFast leaves no exact tree behind (tree.zig:351-356).
+    for (int k = 0; k < 4; ++k) meta->extent[k] = gp[0].extent[k];
+    meta->len = 0;
+    meta->valid = 0;
+  }
 }
 
 // Quadrant boundaries of each splitting cell of depth d (fmm.zig:694-708):
@@ -142,35 +330,39 @@ __global__ __launch_bounds__(EWF) void k_fmm_children(FCell* __restrict__ cells,
     const uint32_t c0 = base + off[t];
     cells[lo + t].child0 = c0;
     cells[lo + t].nchild = nc;
+    const uint32_t graph = cells[lo + t].graph;
     uint32_t k = 0;
     for (int q = 0; q < 4; ++q) {
       const uint32_t b0 = bounds[5 * (size_t)t + q], b1 = bounds[5 * (size_t)t + q + 1];
       if (b1 == b0) continue;
       const uint32_t idx = c0 + k++;
-      if (idx < cell_cap) cells[idx] = FCell{0, 0, 0, b0, b1 - b0, 0, 0, d + 1};
+      if (idx < cell_cap) cells[idx] = FCell{0, 0, 0, b0, b1 - b0, 0, 0, d + 1, graph};
       else fp->cell_overflow = 1;
     }
   }
 }
 
 // fmm.zig:668-682: Morton-ordered raw positions, unit-frame positions and
-// strength * alpha (NaN -> 0).
-__global__ __launch_bounds__(EWF) void k_fmm_gather(const uint32_t* __restrict__ perm, const double* __restrict__ xs,
-                                                    const double* __restrict__ ys, const double* __restrict__ st,
+// strength * alpha (NaN -> 0), each point in its graph's frame.
+__global__ __launch_bounds__(EWF) void k_fmm_gather(const uint32_t* __restrict__ perm, const uint64_t* __restrict__ keys,
+                                                    const double* __restrict__ xs, const double* __restrict__ ys,
+                                                    const double* __restrict__ st, const FmmGraph* __restrict__ gp,
                                                     const FmmParams* __restrict__ fp, double alpha, uint32_t n,
                                                     double* __restrict__ px, double* __restrict__ py,
                                                     double* __restrict__ ux, double* __restrict__ uy,
                                                     float* __restrict__ ps) {
   const uint32_t j = blockIdx.x * EWF + threadIdx.x;
-  if (j >= n || j >= fp->m || !fp->ok) return;
+  if (j >= n || j >= fp->m_total) return;
+  const FmmGraph& G = gp[keys[j] >> 32];
+  if (!G.ok) return;
   const uint32_t i = perm[j];
   const double x = xs[i], y = ys[i];
   px[j] = x;
   py[j] = y;
   const double s = st[i] * alpha;
   ps[j] = s != s ? 0.0f : (float)s;
-  ux[j] = (x - fp->ox) * fp->inv_w;
-  uy[j] = (y - fp->oy) * fp->inv_w;
+  ux[j] = (x - G.ox) * G.inv_w;
+  uy[j] = (y - G.oy) * G.inv_w;
 }
 
 // ---------------------------------------------------------------- upward
@@ -264,9 +456,9 @@ __global__ __launch_bounds__(EWF) void k_fmm_up(FCell* __restrict__ cells, doubl
 // ---------------------------------------------------------------- dual tree
 
 struct MacArgs {
-  double mac_far, dmin_s, dmax_s;
-  int ring;
+  double mac_far;  // 2 kappa - 1 (fmm.zig:266)
   uint32_t nbits;
+  const FmmGraph* graphs;
 };
 
 constexpr int STEP_PER = 4;                   // pairs per thread per tile
@@ -342,10 +534,11 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
           bool done = false;
           if (d2 > MIN_SEP2 && d2 >= thr * thr) {
             bool separated = true;
-            if (mac.ring) {
+            const FmmGraph& G = mac.graphs[ca.graph];
+            if (G.ring) {
               const double gap = __builtin_sqrt(d2) - ra - rb;
-              if (gap >= mac.dmax_s) done = true;
-              else if (gap + 2 * (ra + rb) > mac.dmax_s || gap < mac.dmin_s) separated = false;
+              if (gap >= G.dmax_s) done = true;
+              else if (gap + 2 * (ra + rb) > G.dmax_s || gap < G.dmin_s) separated = false;
             }
             if (!done && separated) {
               kind = 1;
@@ -431,16 +624,20 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
   }
 }
 
-__global__ void k_fmm_lists_reset(FmmParams* __restrict__ fp, uint2* __restrict__ front) {
-  const uint32_t t = threadIdx.x;
-  if (t < FMM_MAX_STEPS + 2) fp->front[t] = t == 0 ? 1u : 0u;
-  if (t == 0) {
-    fp->m2l = 0;
-    fp->p2p = 0;
-    fp->list_overflow = 0;
-    fp->unsorted = 0;
-    fp->pairs = 0;
-    front[0] = make_uint2(0, 0);
+// The first frontier: every root paired with itself.
+__global__ __launch_bounds__(EWF) void k_fmm_lists_reset(FmmParams* __restrict__ fp, uint2* __restrict__ front,
+                                                         uint32_t roots) {
+  const uint32_t t = blockIdx.x * EWF + threadIdx.x;
+  if (t < roots) front[t] = make_uint2(t, t);
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < FMM_MAX_STEPS + 2) fp->front[threadIdx.x] = threadIdx.x == 0 ? roots : 0u;
+    if (threadIdx.x == 0) {
+      fp->m2l = 0;
+      fp->p2p = 0;
+      fp->list_overflow = 0;
+      fp->unsorted = 0;
+      fp->pairs = 0;
+    }
   }
 }
 
@@ -628,8 +825,8 @@ __global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, co
                                              const double* __restrict__ px, const double* __restrict__ py,
                                              const float* __restrict__ ps, const double* __restrict__ ux,
                                              const double* __restrict__ uy, const uint32_t* __restrict__ perm,
-                                             double* __restrict__ vxs, double* __restrict__ vys, double inv_w,
-                                             float dmin2, float dmax2, uint64_t seed) {
+                                             double* __restrict__ vxs, double* __restrict__ vys,
+                                             const FmmGraph* __restrict__ gp, float dmin2, float dmax2) {
   const uint32_t A = blockIdx.x;
   const FCell ca = cells[A];
   if (ca.nchild != 0) return;
@@ -641,6 +838,9 @@ __global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, co
   __shared__ float2 part[64];
   const double ax = px[ca.first], ay = py[ca.first];
   const double inv_r = ca.r > 0 ? 1 / ca.r : 0;
+  const double inv_w = gp[ca.graph].inv_w;
+  const uint64_t seed = gp[ca.graph].seed;
+  const uint32_t base = gp[ca.graph].node_base;  // the hash takes indices within the graph
   for (uint32_t tb = 0; tb < ca.count; tb += 64) {
     const uint32_t T = ca.count - tb < 64 ? ca.count - tb : 64;
     const uint32_t S = 64 / T;
@@ -649,6 +849,7 @@ __global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, co
     const uint32_t ti = ca.first + tb + (active ? t : 0);
     const float xi = (float)(px[ti] - ax), yi = (float)(py[ti] - ay);
     const uint32_t i = perm[ti];
+    const uint32_t il = i - base;
     float accx = 0.f, accy = 0.f;
     for (uint32_t w0 = 0; w0 < K; w0 += SEG_MAX) {
       const uint32_t wk = load_window(srcs, off, K, w0, seg_in, seg);
@@ -671,13 +872,13 @@ __global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, co
               float l = fmaf(dx, dx, dy * dy);
               if (!(l < dmax2)) continue;
               if (dx == 0.f || dy == 0.f) {
-                const uint32_t jj = perm[j];
+                const uint32_t jl = perm[j] - base;
                 if (dx == 0.f) {
-                  dx = (float)hash_jiggle(seed, i, jj, 0);
+                  dx = (float)hash_jiggle(seed, il, jl, 0);
                   l = fmaf(dx, dx, l);
                 }
                 if (dy == 0.f) {
-                  dy = (float)hash_jiggle(seed, i, jj, 1);
+                  dy = (float)hash_jiggle(seed, il, jl, 1);
                   l = fmaf(dy, dy, l);
                 }
               }
@@ -855,14 +1056,14 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   double kappa = a.opening;
   if (!(kappa > 1)) kappa = 1.4;  // fmm.zig:262-263
   mac.mac_far = 2 * kappa - 1;
-  const double inf = __builtin_huge_val();
-  mac.dmax_s = a.dmax2 < inf ? std::sqrt(a.dmax2) / P.w : inf;
-  mac.dmin_s = a.dmin2 > 0 ? std::sqrt(a.dmin2) / P.w : 0;
-  mac.ring = mac.dmin_s > 0 || mac.dmax_s < inf;
   mac.nbits = nbits;
+  mac.graphs = f.graphs;
   const uint32_t steps = 2 * maxd + 1;
+  const uint32_t roots = P.roots;
+  if (roots > f.front_cap) return hipErrorInvalidValue;
   for (;;) {
-    hipLaunchKernelGGL(k_fmm_lists_reset, dim3(1), dim3(64), 0, stream, f.params, f.front_a);
+    hipLaunchKernelGGL(k_fmm_lists_reset, dim3(ceil_div(roots > 0 ? roots : 1, EWF)), dim3(EWF), 0, stream, f.params,
+                       f.front_a, roots);
     uint2 *in = f.front_a, *out = f.front_b;
     for (uint32_t s = 0; s < steps; ++s) {
       hipLaunchKernelGGL(k_dual_step, dim3(GRID_BLOCKS), dim3(EWF), 0, stream, f.cells, f.params, s, in, out,
@@ -926,8 +1127,7 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   // near field and evaluation
   const float dmin2 = (float)a.dmin2, dmax2 = (float)a.dmax2;
   hipLaunchKernelGGL(k_near<Q>, dim3(nc), dim3(64), 0, stream, f.cells, f.p2p_src, f.p2p_off, f.p2p_cnt, f.params,
-                     (const float2*)f.lexp, f.px, f.py, f.ps, f.ux, f.uy, t.perm, a.vx, a.vy, P.inv_w, dmin2, dmax2,
-                     a.seed);
+                     (const float2*)f.lexp, f.px, f.py, f.ps, f.ux, f.uy, t.perm, a.vx, a.vy, f.graphs, dmin2, dmax2);
   return hipGetLastError();
 }
 
@@ -945,7 +1145,7 @@ using Orders = OrderTable<2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17
 }  // namespace
 
 void fmm_free(FmmBuffers& f) {
-  void* ptrs[] = {f.cells, f.bbox, f.split, f.split_off, f.bounds, f.mexp, f.lexp, f.px, f.py, f.ux, f.uy, f.ps,
+  void* ptrs[] = {f.graphs, f.seeds, f.node_off_dev, f.chunk_lo, f.chunk_hi, f.chunk_first, f.box_parts, f.cells, f.bbox, f.split, f.split_off, f.bounds, f.mexp, f.lexp, f.px, f.py, f.ux, f.uy, f.ps,
                   f.front_a, f.front_b, f.m2l, f.p2p, f.m2l_src, f.p2p_src, f.m2l_off, f.m2l_cnt, f.p2p_off,
                   f.p2p_cnt, f.cursor, f.scan_partials, f.params};
   for (void* p : ptrs)
@@ -961,16 +1161,90 @@ hipError_t direct_sum(const double* x, const double* y, const double* strength, 
   return hipGetLastError();
 }
 
+// Chunks of at most FOREST_CHUNK nodes that never straddle a graph, for the
+// per-graph boxes; rebuilt when the forest changes.
+constexpr uint32_t FOREST_CHUNK = 16384;
+
+hipError_t reserve_forest(FmmBuffers& f, const FmmArgs& a, hipStream_t stream) {
+  const uint32_t G = a.graphs;
+  if (G > f.graph_cap) {
+    const uint32_t cap = G + G / 4 + 16;
+    HIPCHECK(grow(f.graphs, cap));
+    HIPCHECK(grow(f.seeds, cap));
+    HIPCHECK(grow(f.chunk_first, cap + 1));
+    HIPCHECK(grow(f.node_off_dev, cap + 1));
+    f.graph_cap = cap;
+  }
+  const bool same = f.node_off_host.size() == (size_t)G + 1 &&
+                    std::equal(f.node_off_host.begin(), f.node_off_host.end(), a.node_off);
+  if (!same) {
+    std::vector<uint32_t> lo, hi, first(G + 1);
+    for (uint32_t g = 0; g < G; ++g) {
+      first[g] = (uint32_t)lo.size();
+      for (uint32_t b = a.node_off[g]; b < a.node_off[g + 1]; b += FOREST_CHUNK) {
+        lo.push_back(b);
+        hi.push_back(std::min(b + FOREST_CHUNK, a.node_off[g + 1]));
+      }
+    }
+    first[G] = (uint32_t)lo.size();
+    const uint32_t C = (uint32_t)lo.size();
+    if (C > f.chunk_cap) {
+      const uint32_t cap = C + C / 4 + 16;
+      HIPCHECK(grow(f.chunk_lo, cap));
+      HIPCHECK(grow(f.chunk_hi, cap));
+      if (f.box_parts) HIPCHECK(hipFree(f.box_parts));
+      f.box_parts = nullptr;
+      HIPCHECK(hipMalloc(&f.box_parts, (size_t)cap * sizeof(BoxPart)));
+      f.chunk_cap = cap;
+    }
+    if (C) {
+      HIPCHECK(hipMemcpyAsync(f.chunk_lo, lo.data(), C * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+      HIPCHECK(hipMemcpyAsync(f.chunk_hi, hi.data(), C * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    }
+    HIPCHECK(hipMemcpyAsync(f.chunk_first, first.data(), (G + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(f.node_off_dev, a.node_off, (G + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipStreamSynchronize(stream));  // the staging vectors end here
+    f.node_off_host.assign(a.node_off, a.node_off + G + 1);
+    f.chunks = C;
+  }
+  HIPCHECK(hipMemcpyAsync(f.seeds, a.seeds, G * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+  return hipSuccess;
+}
+
 hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t stream) {
   HIPCHECK(upload_binomials());
   const uint32_t n = a.n;
   uint32_t p = a.order;  // fmm.zig:258-261
   if (p < 1 || p > FMM_MAX_ORDER) p = 8;
-  HIPCHECK(tree_prepare_fast(t, a.x, a.y, n, stream));
+  if (a.graphs == 0 || a.node_off[a.graphs] != n) return hipErrorInvalidValue;
+  HIPCHECK(tree_reserve(t, n > 0 ? n : 1));
   HIPCHECK(reserve_points(f, n));
+  HIPCHECK(reserve_forest(f, a, stream));
+  if (f.cell_cap < a.graphs + 4096) {
+    HIPCHECK(grow(f.cells, a.graphs + 4096));
+    f.cell_cap = a.graphs + 4096;
+  }
   const uint32_t nb = ceil_div(n > 0 ? n : 1, EWF);
+  // Boxes, frames and keys of every graph; NaN points compacted out; one sort.
+  hipLaunchKernelGGL(k_forest_reset, dim3(1), dim3(64), 0, stream, f.params, n);
+  if (f.chunks)
+    hipLaunchKernelGGL(k_forest_box, dim3(f.chunks), dim3(EWF), 0, stream, a.x, a.y, f.chunk_lo, f.chunk_hi,
+                       (BoxPart*)f.box_parts);
+  hipLaunchKernelGGL(k_forest_graphs, dim3(ceil_div(a.graphs, 64)), dim3(64), 0, stream, (const BoxPart*)f.box_parts,
+                     f.chunk_first, a.graphs, a.dmin2, a.dmax2, f.seeds, f.node_off_dev, f.graphs, f.params);
+  hipLaunchKernelGGL(k_forest_keys, dim3(nb), dim3(EWF), 0, stream, a.x, a.y, n, a.node_graph, f.graphs, f.params,
+                     t.keys, t.perm, t.keys_all, t.flags);
+  HIPCHECK(scan_u32(t.flags, t.offs, n > 0 ? n : 1, &f.params->n, t.partials, nullptr, stream));
+  hipLaunchKernelGGL(k_forest_compact, dim3(nb), dim3(EWF), 0, stream, n, f.params, t.keys_all, t.flags, t.offs,
+                     t.keys, t.perm);
+  int digits = 4;  // 32 Morton bits (tree.zig:363), then the graph's bytes
+  for (uint32_t v = a.graphs - 1; v > 0; v >>= 8) ++digits;
+  HIPCHECK(sort_digits(t.hist, t.rs_glob, t.keys, t.perm, t.keys_tmp, t.perm_tmp, n > 0 ? n : 1, &f.params->m_total,
+                       0, digits, stream));
+  // Cells, one depth at a time; grown and rebuilt if they outgrow the buffer.
   for (;;) {
-    hipLaunchKernelGGL(k_fmm_setup, dim3(1), dim3(1), 0, stream, t.meta, f.params, f.cells);
+    hipLaunchKernelGGL(k_forest_roots, dim3(1), dim3(1024), 0, stream, f.graphs, a.graphs, f.params, f.cells,
+                       f.cell_cap, t.meta);
     for (uint32_t d = 0; d < FMM_LEVELS; ++d) {
       hipLaunchKernelGGL(k_fmm_split, dim3(LEVEL_BLOCKS), dim3(EWF), 0, stream, t.keys, f.cells, f.params, d, f.split,
                          f.bounds, f.cell_cap);
@@ -985,13 +1259,15 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
     const uint32_t cap = std::max(2 * f.cell_cap, f.params_host->level_off[FMM_LEVELS + 1] + 4096);
     HIPCHECK(grow(f.cells, cap));
     f.cell_cap = cap;
+    hipLaunchKernelGGL(k_forest_reset_overflow, dim3(1), dim3(64), 0, stream, f.params);
   }
   const FmmParams& P = *f.params_host;
-  if (P.m == 0) return hipSuccess;      // no point: nothing to do (fmm.zig:251-252)
-  if (!P.ok) return hipErrorNotSupported;  // fmm.zig:255-257: the exact kernel runs instead
+  // fmm.zig:255-257: one graph with a non-finite extent runs the exact kernel.
+  if (a.graphs == 1 && P.declined) return hipErrorNotSupported;
+  if (P.roots == 0) return hipSuccess;  // no point in any tree (fmm.zig:251-252)
   HIPCHECK(reserve_cells(f, P.level_off[FMM_LEVELS + 1], p + 1));
-  hipLaunchKernelGGL(k_fmm_gather, dim3(nb), dim3(EWF), 0, stream, t.perm, a.x, a.y, a.strength, f.params, a.alpha, n,
-                     f.px, f.py, f.ux, f.uy, f.ps);
+  hipLaunchKernelGGL(k_fmm_gather, dim3(nb), dim3(EWF), 0, stream, t.perm, t.keys, a.x, a.y, a.strength, f.graphs,
+                     f.params, a.alpha, n, f.px, f.py, f.ux, f.uy, f.ps);
   return Orders::fns[p - 1](f, t, a, stream);
 }
 

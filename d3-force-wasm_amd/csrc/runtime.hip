@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/d3fgpu.h"
+#include "buffers.h"
 #include "common.h"
 
 #define EXPORT extern "C" __attribute__((visibility("default")))
@@ -39,54 +40,6 @@ int32_t status_of(hipError_t e) {
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return OUT_OF_MEMORY;
   return WORKER_FAILED;
 }
-
-// A pinned host array with capacity; realloc keeps the contents.
-template <class T>
-struct Pinned {
-  T* p = nullptr;
-  size_t cap = 0;
-  hipError_t reserve(size_t n, size_t keep) {
-    if (n <= cap) return hipSuccess;
-    T* q = nullptr;
-    HIPCHECK(hipHostMalloc((void**)&q, n * sizeof(T), hipHostMallocDefault));
-    if (p) {
-      if (keep) std::memcpy(q, p, std::min(keep, cap) * sizeof(T));
-      HIPCHECK(hipHostFree(p));
-    }
-    p = q;
-    cap = n;
-    return hipSuccess;
-  }
-  void release() {
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    cap = 0;
-  }
-};
-
-template <class T>
-struct Device {
-  T* p = nullptr;
-  size_t cap = 0;
-  hipError_t reserve(size_t n, size_t keep = 0, hipStream_t stream = nullptr) {
-    if (n <= cap) return hipSuccess;
-    T* q = nullptr;
-    HIPCHECK(hipMalloc((void**)&q, n * sizeof(T)));
-    if (p) {
-      if (keep) HIPCHECK(hipMemcpyAsync(q, p, std::min(keep, cap) * sizeof(T), hipMemcpyDeviceToDevice, stream));
-      HIPCHECK(hipStreamSynchronize(stream));
-      HIPCHECK(hipFree(p));
-    }
-    p = q;
-    cap = n;
-    return hipSuccess;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-  }
-};
 
 struct Force {
   Kind kind;
@@ -703,7 +656,12 @@ int32_t run_many_body_fast(Sim& s, Force& f, double alpha) {  // fmm.zig:247-354
   a.opening = f.scalar(S_OPENING);
   a.dmin2 = f.scalar(S_DMIN2);
   a.dmax2 = f.scalar(S_DMAX2);
-  a.seed = seed_of(f);
+  const uint64_t seed = seed_of(f);
+  const uint32_t node_off[2] = {0, s.n};
+  a.graphs = 1;
+  a.node_off = node_off;
+  a.node_graph = nullptr;
+  a.seeds = &seed;
   s.timer_start(D3F_PHASE_FMM);
   hipError_t e = fmm_run(s.fmm, s.tree, a, s.stream);
   s.timer_stop();

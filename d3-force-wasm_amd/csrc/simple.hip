@@ -83,6 +83,48 @@ __global__ __launch_bounds__(CS_THREADS) void k_center_raw_sum(const double* __r
   if (wave < 2 && lane == 0) out[wave] = s;
 }
 
+// A batch of graphs: one wave per graph; lane 0 adds x and lane 1 adds y in
+// node order (simple.zig:18-22) from LDS, the next 64 values already in
+// flight.
+__global__ __launch_bounds__(64) void k_batch_center_sums(const double* __restrict__ xs,
+                                                          const double* __restrict__ ys,
+                                                          const uint32_t* __restrict__ node_off,
+                                                          double* __restrict__ sums) {
+  __shared__ double buf[2][64];
+  const uint32_t g = blockIdx.x;
+  const int lane = threadIdx.x, axis = lane & 1;
+  const uint32_t lo = node_off[g], hi = node_off[g + 1];
+  double s = 0;
+  double nx = lo + lane < hi ? xs[lo + lane] : 0, ny = lo + lane < hi ? ys[lo + lane] : 0;
+  for (uint32_t base = lo; base < hi; base += 64) {
+    buf[0][lane] = nx;
+    buf[1][lane] = ny;
+    __syncthreads();
+    const uint32_t k = base + 64 + lane;
+    nx = k < hi ? xs[k] : 0;
+    ny = k < hi ? ys[k] : 0;
+    if (lane < 2) {
+      const uint32_t cnt = hi - base < 64 ? hi - base : 64;
+      for (uint32_t q = 0; q < cnt; ++q) s += buf[axis][q];
+    }
+    __syncthreads();
+  }
+  if (lane < 2) sums[2 * g + axis] = s;
+}
+
+__global__ __launch_bounds__(EW) void k_batch_center_apply(double* __restrict__ xs, double* __restrict__ ys,
+                                                           uint32_t n, const uint32_t* __restrict__ node_off,
+                                                           const uint32_t* __restrict__ node_graph,
+                                                           const double* __restrict__ sums, double cx, double cy,
+                                                           double strength) {
+  const uint32_t i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t g = node_graph[i];
+  const double nf = (double)(node_off[g + 1] - node_off[g]);
+  xs[i] -= (sums[2 * g] / nf - cx) * strength;
+  ys[i] -= (sums[2 * g + 1] / nf - cy) * strength;
+}
+
 // x -= (sx / n - cx) * strength, y likewise (simple.zig:23-30); every thread
 // forms the same shift from the same operands.
 __global__ __launch_bounds__(EW) void k_center_apply(double* __restrict__ xs, double* __restrict__ ys, uint32_t n,
@@ -160,6 +202,15 @@ __global__ __launch_bounds__(EW) void k_shift(const double* __restrict__ xs, con
 }
 
 }  // namespace
+
+hipError_t batch_center(double* x, double* y, const uint32_t* node_off, const uint32_t* node_graph, uint32_t graphs,
+                        uint32_t n, double* sums, double cx, double cy, double strength, hipStream_t stream) {
+  if (graphs == 0 || n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_center_sums, dim3(graphs), dim3(64), 0, stream, x, y, node_off, sums);
+  hipLaunchKernelGGL(k_batch_center_apply, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, x, y, n, node_off, node_graph,
+                     sums, cx, cy, strength);
+  return hipGetLastError();
+}
 
 hipError_t center_raw_sum(const double* x, const double* y, uint32_t n, double* sums, hipStream_t stream) {
   hipLaunchKernelGGL(k_center_raw_sum, dim3(1), dim3(CS_THREADS), 0, stream, x, y, n, sums);
