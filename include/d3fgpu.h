@@ -169,6 +169,28 @@ int32_t batch_tick(uint32_t b, uint32_t ticks);
 void batch_set_timing(int on);
 int32_t batch_stats(uint32_t b, d3f_stats* out);
 
+/* ---- Learned warm start (no reference counterpart) ----
+ * A message-passing network predicting a layout from node features: 8
+ * features -> 64, 4 layers h' = relu(h Ws + mean_nbr(h) Wn + b), -> 2
+ * positions; BF16 weights and activations with FP32 accumulation on the
+ * matrix cores, forward and backward. gnn_param_count() FP32 parameters in
+ * the layout W0[16][64] b0[64] {Ws[64][64] Wn[64][64] b[64]} x 4 Wo[64][16]
+ * bo[16] (padding rows / columns zero). Status codes are the simulation's. */
+uint32_t gnn_create(uint64_t seed);
+int32_t gnn_destroy(uint32_t h);
+uint32_t gnn_param_count(void);
+float* gnn_params(uint32_t h); /* pinned host copy of the parameters */
+int32_t gnn_sync_params(uint32_t h, uint32_t direction); /* H2D also resets the optimizer */
+/* row[n + 1], col[row[n]]: symmetric adjacency rows; features: n x 8 FP32. */
+int32_t gnn_set_graph(uint32_t h, uint32_t n, const uint32_t* row, const uint32_t* col, const float* features);
+int32_t gnn_forward(uint32_t h, float* positions); /* n x 2 */
+/* Stress pairs for training: |p_i - p_j| should be d. */
+int32_t gnn_set_pairs(uint32_t h, uint32_t pairs, const uint32_t* i, const uint32_t* j, const float* d);
+/* Gradient of sum(dp * positions) into grad (gnn_param_count() floats). */
+int32_t gnn_gradient(uint32_t h, const float* dp, float* grad);
+/* One Adam step on the stress; *loss receives the stress before the step. */
+int32_t gnn_train_step(uint32_t h, float lr, float* loss);
+
 #ifdef __cplusplus
 }
 #endif
