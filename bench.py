@@ -217,6 +217,42 @@ def run_batch(core, graphs, steps, warmup, seed=1):
     }
 
 
+def run_warm(core, nodes=100000):
+    """The learned warm start with the committed network on held-out graphs of
+    `nodes` nodes (two stochastic block models, one Barabasi-Albert graph):
+    GPU time for the warm start (prediction + refinement from alpha 0.1) to
+    come within 1% of the energy of 300 plain fast-mode ticks, against those
+    300 ticks, and the energy gap of both at equal tick counts."""
+    from d3force_amd import warmstart as ws
+    from d3force_amd.graphs import ba_graph, sbm_batch
+    net = ws.Network(core=core)
+    net.load(ws.WEIGHTS)
+    graphs = [("sbm-a", sbm_batch(1, seed=778, n_min=nodes, n_max=nodes)[0]),
+              ("sbm-b", sbm_batch(1, seed=779, n_min=nodes, n_max=nodes)[0]),
+              ("ba", (nodes,) + ba_graph(nodes, seed=777))]
+    out = []
+    for name, (n, src, tgt) in graphs:
+        r = ws.compare(net, n, src, tgt, core=core, seed=99, short=(25, 50, 75, 100))
+        hit = next((run for run in r["runs"] if abs(run["warm"]["relative_to_plain"]) <= 0.01), None)
+        row = {"graph": name, "nodes": n, "links": len(src), "plain_ticks": 300,
+               "plain_gpu_seconds": r["plain_seconds"], "predict_seconds": r["predict_seconds"],
+               "predicted_layout_energy_gap": r["predicted_energy_gap"],
+               "within_1pct": None if hit is None else {
+                   "ticks": hit["ticks"], "gpu_seconds": hit["warm"]["seconds"],
+                   "speedup": r["plain_seconds"] / hit["warm"]["seconds"]},
+               "equal_ticks": [{"ticks": run["ticks"], "warm_gap": run["warm"]["gap"],
+                                "plain_gap": run["plain_same_ticks"]["gap"],
+                                "warm_stress": run["warm"]["stress"], "plain_stress": run["plain_same_ticks"]["stress"]}
+                               for run in r["runs"]],
+               "plain_300_stress": r["plain_stress"]}
+        log("warm start %s: %s" % (name, json.dumps(row["within_1pct"])))
+        out.append(row)
+    net.close()
+    return {"graphs": out, "network": "4 message-passing layers, width 64, BF16 on matrix cores",
+            "energy": "d3 potential: links 1/2 k (r - 30)^2 + 30 sum over node pairs log(1 / max(r, 1)), "
+                      "pair sum from a fixed 1M-pair sample; gap = (E - E_plain300) / (E_start - E_plain300)"}
+
+
 def device_sync(core):
     """Waits for the device (the library's own hipDeviceSynchronize: the
     library links ROCm's HIP runtime directly, so torch is not loaded)."""
@@ -267,6 +303,8 @@ def main():
     ap.add_argument("--fast-only", action="store_true", help="run only the fast-mode leg (prints its object)")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs of the batched leg (0: skip it)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched leg (prints its object)")
+    ap.add_argument("--warm-nodes", type=int, default=100000, help="nodes of the warm-start graphs (0: skip)")
+    ap.add_argument("--warm-only", action="store_true", help="run only the warm-start leg (prints its object)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the tick on a single MI355X "
@@ -279,6 +317,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps(run_batch(d3.gpu_core(), args.batch_graphs, args.steps, args.warmup)))
+        return
+    if args.warm_only:
+        print(json.dumps(run_warm(d3.gpu_core(), args.warm_nodes)))
         return
     n = args.nodes
     m = 2 * n
@@ -347,6 +388,7 @@ def main():
 
     fast = run_fast(core, args.fast_nodes, args.steps, args.warmup) if args.fast_nodes else None
     batch = run_batch(core, args.batch_graphs, args.steps, args.warmup) if args.batch_graphs else None
+    warm = run_warm(core, args.warm_nodes) if args.warm_nodes else None
 
     cpu = None
     if not args.no_cpu:
@@ -378,6 +420,7 @@ def main():
         "cpu_baseline": cpu,
         "fast": fast,
         "batch": batch,
+        "warm_start": warm,
     }
     print(json.dumps(out))
 

@@ -216,3 +216,236 @@ class Network:
 
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npy")
+
+
+# ---------------------------------------------------------------- pipeline
+
+def fast_simulation(n, src, tgt, core=None, x=None, y=None):
+    """The fast-mode simulation the warm start is trained on and judged by:
+    manyBody + link + center with d3's defaults, FMM order 8, opening 1.5."""
+    from . import forceSimulation, forceManyBody, forceLink, forceCenter, Nodes
+    nodes = Nodes(n) if x is None else Nodes(n, x=x, y=y)
+    sim = (forceSimulation(nodes, core=core or gpu_core())
+           .force("charge", forceManyBody())
+           .force("link", forceLink((src, tgt)))
+           .force("center", forceCenter()).stop())
+    sim.precision("fast", {"order": 8, "opening": 1.5})
+    return sim, nodes
+
+
+def timed_ticks(sim, core, ticks):
+    """Runs `ticks` ticks (one copy in and out); returns the GPU seconds of
+    their kernels, from the library's event timing of each phase (the center
+    sums overlap the tick on a second stream and are not added)."""
+    from .core import context
+    h = context(sim.nodes()).h
+    check(core.core_synchronize())
+    core.stats(h)
+    core.core_set_timing(1)
+    sim.tick(ticks)
+    check(core.core_synchronize())
+    core.core_set_timing(0)
+    ms = core.stats(h)["ms"]
+    return sum(v for k, v in ms.items() if k != "center_sum") / 1e3
+
+
+def ticks_to_alpha(alpha0, alpha1, decay):
+    """d3's alpha schedule (simulation.js:64, target 0): ticks from alpha0 to
+    reach alpha1 or below."""
+    return int(math.ceil(math.log(alpha1 / alpha0) / math.log(1 - decay)))
+
+
+def training_graphs(count, seed=1, n_min=1000, n_max=20000):
+    """Alternating Barabasi-Albert (m0 = 3, 3 links per node) and stochastic
+    block model graphs with n log-uniform in [n_min, n_max]."""
+    from .graphs import ba_graph, sbm_batch
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        if k % 2 == 0:
+            s, t = ba_graph(n, seed=seed * 1000 + k)
+            out.append((n, s, t))
+        else:
+            g = sbm_batch(1, seed=seed * 1000 + k, n_min=n, n_max=n)[0]
+            out.append(g)
+    return out
+
+
+class Prepared:
+    """One graph's rows, features and (for training) normalised target layout."""
+
+    def __init__(self, n, src, tgt, seed=0):
+        self.n, self.src, self.tgt = n, np.asarray(src), np.asarray(tgt)
+        self.row, self.col = csr_undirected(n, src, tgt)
+        self.x = features(n, self.row, self.col, seed=seed)
+        self.scale = math.sqrt(n)
+
+
+def target_layout(g, core=None, alpha=0.1):
+    """The simulator's own layout at `alpha` (fast mode from phyllotaxis)."""
+    sim, nodes = fast_simulation(g.n, g.src, g.tgt, core=core)
+    decay = 1 - math.pow(0.001, 1 / 300)
+    sim.tick(ticks_to_alpha(1.0, alpha, decay))
+    return np.stack([nodes.x, nodes.y], axis=1)
+
+
+def train(graphs=48, epochs=30, steps=4, lr=2e-3, seed=1, out=WEIGHTS, log=print, core=None):
+    """Trains the network on simulator layouts at alpha = 0.1 of `graphs`
+    training graphs and saves its parameters to `out`."""
+    core = core or gpu_core()
+    gs = training_graphs(graphs, seed=seed)
+    prepared = []
+    for k, (n, s, t) in enumerate(gs):
+        g = Prepared(n, s, t, seed=k)
+        lay = target_layout(g, core=core)
+        lay = (lay - lay.mean(axis=0)) / g.scale
+        g.pairs = stress_pairs(g.row, g.col, lay, seed=k)
+        prepared.append(g)
+    log("warm start: %d training layouts ready" % len(prepared))
+    net = Network(seed=seed, core=core)
+    rng = np.random.default_rng(seed)
+    total = epochs * len(prepared) * steps
+    done = 0
+    for e in range(epochs):
+        losses = []
+        for k in rng.permutation(len(prepared)):
+            g = prepared[k]
+            net.set_graph(g.row, g.col, g.x)
+            net.set_pairs(*g.pairs)
+            for _ in range(steps):
+                rate = lr * 0.5 * (1 + math.cos(math.pi * done / total))
+                losses.append(net.train_step(rate))
+                done += 1
+        log("warm start: epoch %d stress %.4f" % (e + 1, float(np.mean(losses))))
+    if out:
+        net.save(out)
+    return net
+
+
+def energy_scale(p, src, tgt, link_strength, charge=-30.0, distance=30.0):
+    """The factor s minimising the energy of s p (module docstring) over
+    uniform scalings, with every node pair farther apart than distanceMin:
+    dE/ds = sum k (s r - d) r - |charge| n (n - 1) / 2 / s = 0."""
+    n = len(p)
+    r = np.hypot(*(p[src] - p[tgt]).T)
+    a = np.sum(link_strength * r * r)
+    b = np.sum(link_strength * distance * r)
+    c = abs(charge) * n * (n - 1) / 2
+    return (b + math.sqrt(b * b + 4 * a * c)) / (2 * a) if a > 0 else 1.0
+
+
+def predict(net, g, jitter=1.0, seed=0):
+    """Predicted layout (n x 2) in the simulation's units: the network's shape,
+    centred and scaled to the energy-minimising size (the network's global
+    scale is its least reliable output, and a layout much larger or smaller
+    than the simulation's costs refinement ticks), plus a seeded uniform
+    jitter of +-`jitter` so that nodes the network puts at one point do not
+    start coincident."""
+    net.set_graph(g.row, g.col, g.x)
+    p = net.forward().astype(np.float64)
+    p -= p.mean(axis=0)
+    if len(g.src):
+        p *= energy_scale(p, g.src, g.tgt, _link_strength(g.n, g.src, g.tgt))
+    if jitter:
+        p += np.random.default_rng(seed).uniform(-jitter, jitter, p.shape)
+    return p
+
+
+def graph_stress(x, y, row, col, sources=16, seed=5):
+    """Sampled graph-distance stress of a layout: over the pairs (source,
+    node) for `sources` random sources and every node they reach,
+    sum ((|p_i - p_j| - c D_ij) / (c D_ij))^2 / pairs with D the hop
+    distance and c the scale that minimises it."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import shortest_path
+    n = len(row) - 1
+    rows = np.repeat(np.arange(n), np.diff(row.astype(np.int64)))
+    a = sp.csr_matrix((np.ones(len(col)), (rows, col.astype(np.int64))), shape=(n, n))
+    src = np.random.default_rng(seed).choice(n, min(n, sources), replace=False)
+    dist = shortest_path(a, unweighted=True, indices=src)
+    pairs = []
+    for k, s_ in enumerate(src):
+        d = dist[k]
+        ok = np.isfinite(d) & (d > 0)
+        r = np.hypot(x[ok] - x[s_], y[ok] - y[s_])
+        pairs.append((r, d[ok]))
+    # c minimising sum (r / (c D) - 1)^2: c = sum (r / D)^2 / sum (r / D)
+    q = np.concatenate([r / d for r, d in pairs])
+    c = np.sum(q * q) / np.sum(q)
+    return float(np.mean((q / c - 1) ** 2))
+
+
+def compare(net, n, src, tgt, core=None, seed=0, short=(50, 100)):
+    """Plain fast mode from phyllotaxis (300 ticks, d3's schedule) against
+    the warm start (the prediction refined from alpha = 0.1, alpha decaying to
+    0.001 over K ticks) and against plain fast mode squeezed into the same K
+    ticks (alpha 1 -> 0.001 over K ticks: the method without the network),
+    for each K of `short`. All runs reuse one simulation (its buffers are
+    allocated by a first, untimed run); seconds are GPU kernel time.
+    Energies use one sample of node pairs; `gap` is (E - E_plain) /
+    (E_start - E_plain): 0 is as good as 300 plain ticks, 1 no better than
+    the start."""
+    import time
+    core = core or gpu_core()
+    g = Prepared(n, src, tgt, seed=seed)
+    k_link = _link_strength(n, g.src, g.tgt)
+    sim, nodes = fast_simulation(n, src, tgt, core=core)
+    x0, y0 = nodes.x.copy(), nodes.y.copy()
+    plain_decay = sim.alpha_decay()
+
+    def run(x, y, alpha, decay, ticks):
+        nodes.x[:] = x
+        nodes.y[:] = y
+        nodes.vx[:] = 0
+        nodes.vy[:] = 0
+        sim.alpha(alpha).alpha_decay(decay)
+        return timed_ticks(sim, core, ticks)
+
+    run(x0, y0, 1.0, plain_decay, 5)  # allocations, untimed
+    e_start, pairs = energy(x0, y0, g.src, g.tgt, k_link)
+    t_plain = run(x0, y0, 1.0, plain_decay, 300)
+    e_plain, _ = energy(nodes.x, nodes.y, g.src, g.tgt, k_link, pairs=pairs)
+    span = e_start - e_plain
+    out = {"nodes": n, "links": len(src), "start_energy": e_start, "plain_energy": e_plain, "plain_ticks": 300,
+           "plain_seconds": t_plain, "plain_stress": graph_stress(nodes.x, nodes.y, g.row, g.col), "runs": []}
+    predict(net, g)  # first call allocates
+    check(core.core_synchronize())
+    t0 = time.perf_counter()
+    p = predict(net, g)
+    check(core.core_synchronize())
+    t_predict = time.perf_counter() - t0  # wall time of the network's kernels and copies
+    e_pred, _ = energy(p[:, 0], p[:, 1], g.src, g.tgt, k_link, pairs=pairs)
+    out["predict_seconds"] = t_predict
+    out["predicted_energy_gap"] = (e_pred - e_plain) / span
+    run(p[:, 0], p[:, 1], 0.1, 1 - math.pow(0.001 / 0.1, 1 / max(short)), 3)  # buffers for this layout, untimed
+    for k in short:
+        t_warm = t_predict + run(p[:, 0], p[:, 1], 0.1, 1 - math.pow(0.001 / 0.1, 1 / k), k)
+        e_warm, _ = energy(nodes.x, nodes.y, g.src, g.tgt, k_link, pairs=pairs)
+        s_warm = graph_stress(nodes.x, nodes.y, g.row, g.col)
+        t_short = run(x0, y0, 1.0, 1 - math.pow(0.001, 1 / k), k)
+        e_short, _ = energy(nodes.x, nodes.y, g.src, g.tgt, k_link, pairs=pairs)
+        s_short = graph_stress(nodes.x, nodes.y, g.row, g.col)
+        out["runs"].append({
+            "ticks": k,
+            "warm": {"energy": e_warm, "gap": (e_warm - e_plain) / span,
+                     "relative_to_plain": (e_warm - e_plain) / abs(e_plain), "seconds": t_warm, "stress": s_warm},
+            "plain_same_ticks": {"energy": e_short, "gap": (e_short - e_plain) / span,
+                                 "relative_to_plain": (e_short - e_plain) / abs(e_plain), "seconds": t_short,
+                                 "stress": s_short}})
+    return out
+
+
+def _link_strength(n, src, tgt):
+    count = np.bincount(src, minlength=n) + np.bincount(tgt, minlength=n)
+    return 1.0 / np.minimum(count[src], count[tgt])
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="train the warm-start network on the GPU")
+    ap.add_argument("--graphs", type=int, default=48)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--out", default=WEIGHTS)
+    a = ap.parse_args()
+    train(graphs=a.graphs, epochs=a.epochs, out=a.out)
