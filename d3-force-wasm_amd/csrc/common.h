@@ -263,8 +263,10 @@ struct FmmParams {
   uint32_t front[FMM_MAX_STEPS + 2];       // dual-tree frontier sizes
   uint32_t m2l, p2p;                       // list lengths
   uint32_t list_overflow;
-  uint32_t unsorted;                       // target segments too long to order by source
+  uint32_t long_segs;                      // target segments ordered by the long-segment sort
   unsigned long long pairs;                // point pairs of the P2P list
+  uint32_t near_items, near_slots, near_combines;  // near-field work items, partial-sum slots, combines
+  uint32_t pad;
 };
 
 struct FmmBuffers {
@@ -286,6 +288,11 @@ struct FmmBuffers {
   double *px = nullptr, *py = nullptr, *ux = nullptr, *uy = nullptr;
   float* ps = nullptr;
   uint2 *front_a = nullptr, *front_b = nullptr;
+  uint32_t long_cap = 0, slot_cap = 0, item_cap = 0;
+  uint32_t* long_list = nullptr;           // targets whose segments the long-segment sort orders
+  uint4* near_items = nullptr;             // (leaf, target batch, chunk or ALL, slot)
+  uint4* near_combines = nullptr;          // (leaf, target batch, first slot, chunks)
+  float2* near_partial = nullptr;          // slot_cap x 64 per-target partial sums
   uint64_t *m2l = nullptr, *p2p = nullptr;         // target << nbits | source, as emitted
   uint32_t *m2l_src = nullptr, *p2p_src = nullptr;  // sources grouped by target
   uint32_t *m2l_off = nullptr, *m2l_cnt = nullptr, *p2p_off = nullptr, *p2p_cnt = nullptr, *cursor = nullptr;

@@ -43,6 +43,11 @@ constexpr int EWF = 256;
 constexpr uint32_t GRID_BLOCKS = 1024;  // grid-stride kernels over device counts
 constexpr uint32_t LEVEL_BLOCKS = 256;  // per-depth build kernels (grid-stride)
 constexpr uint32_t SEG_MAX = 1024;      // sources of one target ordered in LDS
+constexpr uint32_t NEAR_EXTRA = 256;    // blocks that share the near-field work items
+constexpr uint32_t NEAR_CHUNK = 64;     // source leaves per near-field chunk
+constexpr uint32_t NEAR_ALL = 0xFFFFFFFFu;
+constexpr uint32_t LONG_IDS = 262144;   // cell ids per pass of the long-segment sort (32 KB of LDS bitmap)
+constexpr uint32_t LONG_BLOCKS = 128;
 constexpr double MIN_SEP2 = 1e-28;      // fmm.zig:49
 constexpr int BN = 2 * FMM_MAX_ORDER + 1;
 
@@ -635,8 +640,11 @@ __global__ __launch_bounds__(EWF) void k_fmm_lists_reset(FmmParams* __restrict__
       fp->m2l = 0;
       fp->p2p = 0;
       fp->list_overflow = 0;
-      fp->unsorted = 0;
+      fp->long_segs = 0;
       fp->pairs = 0;
+      fp->near_items = 0;
+      fp->near_slots = 0;
+      fp->near_combines = 0;
     }
   }
 }
@@ -709,7 +717,6 @@ __global__ __launch_bounds__(64) void k_m2l(const FCell* __restrict__ cells, con
   const uint32_t A = blockIdx.x;
   const int lane = threadIdx.x;
   const uint32_t K = cnts[A], off = offs[A];
-  if (K > SEG_MAX && lane == 0) atomicAdd(&fp->unsorted, 1u);
   float2 acc[Q];
 #pragma unroll
   for (int l = 0; l < Q; ++l) acc[l] = make_float2(0.f, 0.f);
@@ -811,103 +818,303 @@ __global__ __launch_bounds__(EWF) void k_fmm_down(const FCell* __restrict__ cell
 
 // ---------------------------------------------------------------- near field + L2P
 
-// One wave per target leaf: lane groups of S lanes per target point split
-// the sources; the S partial sums are added in lane order, then L2P
-// (fmm.zig:853-886) and the FP64 velocity update. Each source leaf is staged
-// in LDS as FP32 offsets from an anchor (the target leaf's first point,
-// FP64), so two points with equal FP64 coordinates have an exactly zero
-// FP32 delta. The pair term is d3's (fmm.zig:529-553): skip beyond dmax,
-// hash jiggle on a zero delta, sqrt(dmin2 * l) below dmin.
+// L2P (fmm.zig:853-886) at one target and the FP64 velocity update with its
+// near-field sum.
 template <int Q>
-__global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, const uint32_t* __restrict__ srcs,
-                                             const uint32_t* __restrict__ offs, const uint32_t* __restrict__ cnts,
-                                             FmmParams* __restrict__ fp, const float2* __restrict__ lexp,
-                                             const double* __restrict__ px, const double* __restrict__ py,
-                                             const float* __restrict__ ps, const double* __restrict__ ux,
-                                             const double* __restrict__ uy, const uint32_t* __restrict__ perm,
-                                             double* __restrict__ vxs, double* __restrict__ vys,
-                                             const FmmGraph* __restrict__ gp, float dmin2, float dmax2) {
-  const uint32_t A = blockIdx.x;
+__device__ __forceinline__ void near_finish(const FCell& ca, uint32_t A, uint32_t ti, float2 near,
+                                            const float2* __restrict__ lexp, const double* __restrict__ ux,
+                                            const double* __restrict__ uy, const uint32_t* __restrict__ perm,
+                                            double* __restrict__ vxs, double* __restrict__ vys, double inv_w) {
+  const double inv_r = ca.r > 0 ? 1 / ca.r : 0;
+  const float2 z = make_float2((float)((ux[ti] - ca.cx) * inv_r), (float)((uy[ti] - ca.cy) * inv_r));
+  float2 f = lexp[(size_t)A * Q + Q - 1];
+#pragma unroll
+  for (int l = Q - 2; l >= 0; --l) f = cfma(f, z, lexp[(size_t)A * Q + l]);
+  const uint32_t i = perm[ti];
+  vxs[i] += (double)near.x - (double)f.x * inv_w;
+  vys[i] += (double)near.y + (double)f.y * inv_w;
+}
+
+// Near field of up to 64 targets of leaf A, starting at target tb, on one
+// wave. A target's sources (its P2P segment, ordered by source leaf) are cut
+// into chunks of NEAR_CHUNK leaves; within a chunk, S lanes per target split
+// the source points and their sums are added in lane order; the chunk sums
+// are added in chunk order. chunk == NEAR_ALL runs every chunk and finishes
+// the targets (L2P, velocity); otherwise the one chunk's sums go to `out`
+// (one float2 per target) and k_near_combine adds them later in the same
+// order, so a split target's result is bit-identical to an unsplit one.
+// Each source leaf is staged in LDS as FP32 offsets from an anchor (the
+// target leaf's first point, FP64), so two points with equal FP64
+// coordinates have an exactly zero FP32 delta. The pair term is d3's
+// (fmm.zig:529-553): skip beyond dmax, hash jiggle on a zero delta,
+// sqrt(dmin2 * l) below dmin.
+template <int Q>
+__device__ __forceinline__ void near_batch(uint32_t A, uint32_t tb, uint32_t chunk, float2* __restrict__ out,
+                                           const FCell* __restrict__ cells, const uint32_t* __restrict__ srcs,
+                                           const uint32_t* __restrict__ offs, const uint32_t* __restrict__ cnts,
+                                           const float2* __restrict__ lexp, const double* __restrict__ px,
+                                           const double* __restrict__ py, const float* __restrict__ ps,
+                                           const double* __restrict__ ux, const double* __restrict__ uy,
+                                           const uint32_t* __restrict__ perm, double* __restrict__ vxs,
+                                           double* __restrict__ vys, const FmmGraph* __restrict__ gp, float dmin2,
+                                           float dmax2) {
   const FCell ca = cells[A];
-  if (ca.nchild != 0) return;
   const int lane = threadIdx.x;
   __shared__ uint32_t seg_in[SEG_MAX], seg[SEG_MAX];
-  const uint32_t K = cnts[A], off = offs[A];
-  if (K > SEG_MAX && lane == 0) atomicAdd(&fp->unsorted, 1u);
   __shared__ float4 src[64];
   __shared__ float2 part[64];
+  const uint32_t K = cnts[A], off = offs[A];
+  const uint32_t nch = K > NEAR_CHUNK ? (K + NEAR_CHUNK - 1) / NEAR_CHUNK : 1;
+  const uint32_t c0 = chunk == NEAR_ALL ? 0 : chunk, c1 = chunk == NEAR_ALL ? nch : chunk + 1;
   const double ax = px[ca.first], ay = py[ca.first];
-  const double inv_r = ca.r > 0 ? 1 / ca.r : 0;
-  const double inv_w = gp[ca.graph].inv_w;
   const uint64_t seed = gp[ca.graph].seed;
   const uint32_t base = gp[ca.graph].node_base;  // the hash takes indices within the graph
-  for (uint32_t tb = 0; tb < ca.count; tb += 64) {
-    const uint32_t T = ca.count - tb < 64 ? ca.count - tb : 64;
-    const uint32_t S = 64 / T;
-    const uint32_t t = lane / S, sub = lane % S;
-    const bool active = t < T;
-    const uint32_t ti = ca.first + tb + (active ? t : 0);
-    const float xi = (float)(px[ti] - ax), yi = (float)(py[ti] - ay);
-    const uint32_t i = perm[ti];
-    const uint32_t il = i - base;
+  const uint32_t T = ca.count - tb < 64 ? ca.count - tb : 64;
+  const uint32_t S = 64 / T;
+  const uint32_t t = lane / S, sub = lane % S;
+  const bool active = t < T;
+  const uint32_t ti = ca.first + tb + (active ? t : 0);
+  const float xi = (float)(px[ti] - ax), yi = (float)(py[ti] - ay);
+  const uint32_t il = perm[ti] - base;
+  float2 total = make_float2(0.f, 0.f);
+  for (uint32_t c = c0; c < c1; ++c) {
+    uint32_t wk;
+    if (K <= NEAR_CHUNK) {
+      sort_segment(srcs, off, K, seg_in, seg);
+      wk = K;
+    } else {  // ordered in place by k_sort_long
+      wk = K - c * NEAR_CHUNK < NEAR_CHUNK ? K - c * NEAR_CHUNK : NEAR_CHUNK;
+      __syncthreads();
+      if ((uint32_t)lane < wk) seg[lane] = srcs[off + c * NEAR_CHUNK + lane];
+      __syncthreads();
+    }
     float accx = 0.f, accy = 0.f;
-    for (uint32_t w0 = 0; w0 < K; w0 += SEG_MAX) {
-      const uint32_t wk = load_window(srcs, off, K, w0, seg_in, seg);
-      for (uint32_t s = 0; s < wk; ++s) {
-        const uint32_t B = seg[s];
-        const uint32_t f0 = cells[B].first, cnt = cells[B].count;
-        for (uint32_t p0 = 0; p0 < cnt; p0 += 64) {
-          const uint32_t np = cnt - p0 < 64 ? cnt - p0 : 64;
-          if ((uint32_t)lane < np) {
-            const uint32_t j = f0 + p0 + lane;
-            src[lane] = make_float4((float)(px[j] - ax), (float)(py[j] - ay), ps[j], __uint_as_float(j));
-          }
-          __syncthreads();
-          if (active) {
-            for (uint32_t q = sub; q < np; q += S) {
-              const float4 c = src[q];
-              const uint32_t j = f0 + p0 + q;
-              if (j == ti) continue;
-              float dx = c.x - xi, dy = c.y - yi;
-              float l = fmaf(dx, dx, dy * dy);
-              if (!(l < dmax2)) continue;
-              if (dx == 0.f || dy == 0.f) {
-                const uint32_t jl = perm[j] - base;
-                if (dx == 0.f) {
-                  dx = (float)hash_jiggle(seed, il, jl, 0);
-                  l = fmaf(dx, dx, l);
-                }
-                if (dy == 0.f) {
-                  dy = (float)hash_jiggle(seed, il, jl, 1);
-                  l = fmaf(dy, dy, l);
-                }
-              }
-              if (l < dmin2) l = __builtin_sqrtf(dmin2 * l);
-              const float wj = c.z * __builtin_amdgcn_rcpf(l);
-              accx = fmaf(dx, wj, accx);
-              accy = fmaf(dy, wj, accy);
-            }
-          }
-          __syncthreads();
+    for (uint32_t s = 0; s < wk; ++s) {
+      const uint32_t B = seg[s];
+      const uint32_t f0 = cells[B].first, cnt = cells[B].count;
+      for (uint32_t p0 = 0; p0 < cnt; p0 += 64) {
+        const uint32_t np = cnt - p0 < 64 ? cnt - p0 : 64;
+        if ((uint32_t)lane < np) {
+          const uint32_t j = f0 + p0 + lane;
+          src[lane] = make_float4((float)(px[j] - ax), (float)(py[j] - ay), ps[j], __uint_as_float(j));
         }
+        __syncthreads();
+        if (active) {
+          for (uint32_t q = sub; q < np; q += S) {
+            const float4 cq = src[q];
+            const uint32_t j = f0 + p0 + q;
+            if (j == ti) continue;
+            float dx = cq.x - xi, dy = cq.y - yi;
+            float l = fmaf(dx, dx, dy * dy);
+            if (!(l < dmax2)) continue;
+            if (dx == 0.f || dy == 0.f) {
+              const uint32_t jl = perm[j] - base;
+              if (dx == 0.f) {
+                dx = (float)hash_jiggle(seed, il, jl, 0);
+                l = fmaf(dx, dx, l);
+              }
+              if (dy == 0.f) {
+                dy = (float)hash_jiggle(seed, il, jl, 1);
+                l = fmaf(dy, dy, l);
+              }
+            }
+            if (l < dmin2) l = __builtin_sqrtf(dmin2 * l);
+            const float wj = cq.z * __builtin_amdgcn_rcpf(l);
+            accx = fmaf(dx, wj, accx);
+            accy = fmaf(dy, wj, accy);
+          }
+        }
+        __syncthreads();
       }
     }
     part[lane] = make_float2(accx, accy);
     __syncthreads();
     if (active && sub == 0) {
-      float2 near = part[lane];
+      float2 p = part[lane];
       for (uint32_t q = 1; q < S; ++q) {
-        near.x += part[lane + q].x;
-        near.y += part[lane + q].y;
+        p.x += part[lane + q].x;
+        p.y += part[lane + q].y;
       }
-      const float2 z = make_float2((float)((ux[ti] - ca.cx) * inv_r), (float)((uy[ti] - ca.cy) * inv_r));
-      float2 f = lexp[(size_t)A * Q + Q - 1];
-#pragma unroll
-      for (int l = Q - 2; l >= 0; --l) f = cfma(f, z, lexp[(size_t)A * Q + l]);
-      vxs[i] += (double)near.x - (double)f.x * inv_w;
-      vys[i] += (double)near.y + (double)f.y * inv_w;
+      if (c == c0) {
+        total = p;
+      } else {
+        total.x += p.x;
+        total.y += p.y;
+      }
     }
     __syncthreads();
+  }
+  if (!(active && sub == 0)) return;
+  if (chunk != NEAR_ALL) {
+    out[t] = total;
+    return;
+  }
+  near_finish<Q>(ca, A, ti, total, lexp, ux, uy, perm, vxs, vys, gp[ca.graph].inv_w);
+}
+
+// Targets of one list whose segments are longer than the in-LDS sort takes
+// (M2L: SEG_MAX) or longer than one near-field chunk (P2P).
+__global__ __launch_bounds__(EWF) void k_long_list(const uint32_t* __restrict__ ncp,
+                                                   const uint32_t* __restrict__ m2l_cnt,
+                                                   const uint32_t* __restrict__ p2p_cnt, FmmParams* __restrict__ fp,
+                                                   uint32_t* __restrict__ list) {
+  const uint32_t c = blockIdx.x * EWF + threadIdx.x;
+  if (c >= *ncp) return;
+  if (p2p_cnt[c] > NEAR_CHUNK) list[atomicAdd(&fp->long_segs, 1u)] = c | 0x80000000u;
+  if (m2l_cnt[c] > SEG_MAX) list[atomicAdd(&fp->long_segs, 1u)] = c;
+}
+
+// Orders each long segment by source in place. A target's sources are
+// distinct cell ids, so a bitmap over the ids orders them: the segment is
+// copied aside (into the pair list, which is no longer read), its ids are
+// set in an LDS bitmap covering LONG_IDS ids per pass, and the set bits are
+// written back in id order (each thread owns 32 consecutive words; a block
+// scan of their counts gives the output positions).
+__global__ __launch_bounds__(256) void k_sort_long(const uint32_t* __restrict__ list,
+                                                   const FmmParams* __restrict__ fp, const uint32_t* __restrict__ ncp,
+                                                   uint32_t* __restrict__ m2l_src, const uint32_t* __restrict__ m2l_off,
+                                                   const uint32_t* __restrict__ m2l_cnt, uint32_t* __restrict__ m2l_tmp,
+                                                   uint32_t* __restrict__ p2p_src, const uint32_t* __restrict__ p2p_off,
+                                                   const uint32_t* __restrict__ p2p_cnt,
+                                                   uint32_t* __restrict__ p2p_tmp) {
+  constexpr uint32_t WORDS = LONG_IDS / 32, PER = WORDS / 256;
+  __shared__ uint32_t bits[WORDS];
+  __shared__ uint32_t scan[256];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nseg = fp->long_segs, nc = *ncp;
+  for (uint32_t k = blockIdx.x; k < nseg; k += gridDim.x) {
+    const uint32_t e = list[k];
+    const bool near = e >> 31;
+    const uint32_t A = e & 0x7FFFFFFFu;
+    uint32_t* src = near ? p2p_src : m2l_src;
+    uint32_t* tmp = near ? p2p_tmp : m2l_tmp;
+    const uint32_t off = (near ? p2p_off : m2l_off)[A], K = (near ? p2p_cnt : m2l_cnt)[A];
+    __syncthreads();
+    for (uint32_t q = tid; q < K; q += 256) tmp[off + q] = src[off + q];
+    __syncthreads();
+    uint32_t done = 0;
+    for (uint32_t lo = 0; lo < nc; lo += LONG_IDS) {
+      for (uint32_t w = tid; w < WORDS; w += 256) bits[w] = 0;
+      __syncthreads();
+      for (uint32_t q = tid; q < K; q += 256) {
+        const uint32_t b = tmp[off + q] - lo;
+        if (b < LONG_IDS) atomicOr(&bits[b >> 5], 1u << (b & 31));
+      }
+      __syncthreads();
+      uint32_t mine = 0;
+      for (uint32_t w = 0; w < PER; ++w) mine += __popc(bits[tid * PER + w]);
+      scan[tid] = mine;
+      __syncthreads();
+      for (uint32_t d = 1; d < 256; d <<= 1) {
+        const uint32_t v = tid >= d ? scan[tid - d] : 0u;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+      }
+      uint32_t pos = off + done + scan[tid] - mine;
+      for (uint32_t w = 0; w < PER; ++w) {
+        uint32_t word = bits[tid * PER + w];
+        while (word) {
+          const uint32_t bit = __ffs(word) - 1;
+          word &= word - 1;
+          src[pos++] = lo + (tid * PER + w) * 32 + bit;
+        }
+      }
+      done += scan[255];
+      __syncthreads();
+    }
+  }
+}
+
+// Near-field work beyond each leaf's first 64 targets and its first
+// unsplit pass: for every (leaf, target batch), either one item that runs
+// every chunk, or (segments of several chunks, while partial-sum slots last)
+// one item per chunk plus a combine. split0[leaf] says whether the leaf's
+// first batch was split (its own block then skips it).
+__global__ __launch_bounds__(EWF) void k_near_items(const FCell* __restrict__ cells, const uint32_t* __restrict__ ncp,
+                                                    const uint32_t* __restrict__ cnts, FmmParams* __restrict__ fp,
+                                                    uint4* __restrict__ items, uint4* __restrict__ combines,
+                                                    uint32_t* __restrict__ split0, uint32_t slot_cap) {
+  const uint32_t c = blockIdx.x * EWF + threadIdx.x;
+  if (c >= *ncp) return;
+  const FCell cell = cells[c];
+  if (cell.nchild != 0) return;
+  const uint32_t K = cnts[c];
+  const uint32_t nch = K > NEAR_CHUNK ? (K + NEAR_CHUNK - 1) / NEAR_CHUNK : 1;
+  uint32_t first = 0;
+  for (uint32_t tb = 0; tb < cell.count; tb += 64) {
+    bool split = false;
+    if (nch > 1) {
+      const uint32_t s0 = atomicAdd(&fp->near_slots, nch);
+      if (s0 <= slot_cap && nch <= slot_cap - s0) {
+        const uint32_t k = atomicAdd(&fp->near_items, nch);
+        for (uint32_t q = 0; q < nch; ++q) items[k + q] = make_uint4(c, tb, q, s0 + q);
+        combines[atomicAdd(&fp->near_combines, 1u)] = make_uint4(c, tb, s0, nch);
+        split = true;
+      }
+    }
+    if (tb == 0)
+      first = split;
+    else if (!split)
+      items[atomicAdd(&fp->near_items, 1u)] = make_uint4(c, tb, NEAR_ALL, 0);
+  }
+  split0[c] = first;
+}
+
+// Blocks [0, extra) share the items (launched first, so the longest work
+// starts first); block extra + c runs the first 64 targets of leaf c unless
+// that batch was split into chunk items.
+template <int Q>
+__global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, const uint32_t* __restrict__ srcs,
+                                             const uint32_t* __restrict__ offs, const uint32_t* __restrict__ cnts,
+                                             const FmmParams* __restrict__ fp, const float2* __restrict__ lexp,
+                                             const double* __restrict__ px, const double* __restrict__ py,
+                                             const float* __restrict__ ps, const double* __restrict__ ux,
+                                             const double* __restrict__ uy, const uint32_t* __restrict__ perm,
+                                             double* __restrict__ vxs, double* __restrict__ vys,
+                                             const FmmGraph* __restrict__ gp, float dmin2, float dmax2,
+                                             const uint4* __restrict__ items, const uint32_t* __restrict__ split0,
+                                             float2* __restrict__ partial, uint32_t extra) {
+  if (blockIdx.x < extra) {
+    const uint32_t count = fp->near_items;
+    for (uint32_t k = blockIdx.x; k < count; k += extra) {
+      const uint4 it = items[k];
+      near_batch<Q>(it.x, it.y, it.z, partial + (size_t)it.w * 64, cells, srcs, offs, cnts, lexp, px, py, ps, ux, uy,
+                    perm, vxs, vys, gp, dmin2, dmax2);
+    }
+    return;
+  }
+  const uint32_t A = blockIdx.x - extra;
+  const FCell ca = cells[A];
+  if (ca.nchild != 0 || ca.count == 0 || split0[A]) return;
+  near_batch<Q>(A, 0, NEAR_ALL, nullptr, cells, srcs, offs, cnts, lexp, px, py, ps, ux, uy, perm, vxs, vys, gp, dmin2,
+                dmax2);
+}
+
+// Chunk sums of a split (leaf, batch) added in chunk order, then L2P and the
+// velocity update, as near_batch does for an unsplit batch.
+template <int Q>
+__global__ __launch_bounds__(64) void k_near_combine(const FCell* __restrict__ cells, const FmmParams* __restrict__ fp,
+                                                     const uint4* __restrict__ combines,
+                                                     const float2* __restrict__ partial,
+                                                     const float2* __restrict__ lexp, const double* __restrict__ ux,
+                                                     const double* __restrict__ uy, const uint32_t* __restrict__ perm,
+                                                     double* __restrict__ vxs, double* __restrict__ vys,
+                                                     const FmmGraph* __restrict__ gp) {
+  const uint32_t count = fp->near_combines;
+  const uint32_t t = threadIdx.x;
+  for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
+    const uint4 it = combines[k];
+    const FCell ca = cells[it.x];
+    const uint32_t T = ca.count - it.y < 64 ? ca.count - it.y : 64;
+    if (t >= T) continue;
+    float2 total = partial[(size_t)it.z * 64 + t];
+    for (uint32_t c = 1; c < it.w; ++c) {
+      const float2 p = partial[(size_t)(it.z + c) * 64 + t];
+      total.x += p.x;
+      total.y += p.y;
+    }
+    near_finish<Q>(ca, it.x, ca.first + it.y + t, total, lexp, ux, uy, perm, vxs, vys, gp[ca.graph].inv_w);
   }
 }
 
@@ -1094,6 +1301,25 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
       }
     }
   }
+  // long-segment list, near-field items and partial-sum slots for these lists
+  const uint32_t long_need = P.p2p / NEAR_CHUNK + P.m2l / SEG_MAX + 2;
+  if (long_need > f.long_cap) {
+    const uint32_t c = long_need + long_need / 4;
+    HIPCHECK(grow(f.long_list, c));
+    f.long_cap = c;
+  }
+  const uint32_t slot_need = 2 * (P.p2p / NEAR_CHUNK) + 1024;
+  if (slot_need > f.slot_cap) {
+    const uint32_t c = slot_need + slot_need / 4;
+    HIPCHECK(grow(f.near_partial, (size_t)c * 64));
+    HIPCHECK(grow(f.near_combines, c / 2 + 1));
+    f.slot_cap = c;
+  }
+  const uint32_t item_need = f.slot_cap + P.m_total / 64 + 1;
+  if (item_need > f.item_cap) {
+    HIPCHECK(grow(f.near_items, item_need));
+    f.item_cap = item_need;
+  }
   // lists grouped by target
   const uint32_t* ncp = &f.params->level_off[FMM_LEVELS + 1];
   HIPCHECK(hipMemsetAsync(f.m2l_cnt, 0, (size_t)nc * sizeof(uint32_t), stream));
@@ -1116,6 +1342,11 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
     hipLaunchKernelGGL(k_target_scatter, dim3(ceil_div(P.p2p, EWF)), dim3(EWF), 0, stream, f.p2p, &f.params->p2p,
                        nbits, f.p2p_off, f.cursor, f.p2p_src);
   }
+  // segments too long for the in-LDS order, ordered in place
+  hipLaunchKernelGGL(k_long_list, dim3(ceil_div(nc, EWF)), dim3(EWF), 0, stream, ncp, f.m2l_cnt, f.p2p_cnt, f.params,
+                     f.long_list);
+  hipLaunchKernelGGL(k_sort_long, dim3(LONG_BLOCKS), dim3(256), 0, stream, f.long_list, f.params, ncp, f.m2l_src,
+                     f.m2l_off, f.m2l_cnt, (uint32_t*)f.m2l, f.p2p_src, f.p2p_off, f.p2p_cnt, (uint32_t*)f.p2p);
   // far field
   hipLaunchKernelGGL(k_m2l<Q>, dim3(nc), dim3(64), 0, stream, f.cells, f.m2l_src, f.m2l_off, f.m2l_cnt,
                      (const float2*)f.mexp, (float2*)f.lexp, f.params);
@@ -1126,8 +1357,14 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   }
   // near field and evaluation
   const float dmin2 = (float)a.dmin2, dmax2 = (float)a.dmax2;
-  hipLaunchKernelGGL(k_near<Q>, dim3(nc), dim3(64), 0, stream, f.cells, f.p2p_src, f.p2p_off, f.p2p_cnt, f.params,
-                     (const float2*)f.lexp, f.px, f.py, f.ps, f.ux, f.uy, t.perm, a.vx, a.vy, f.graphs, dmin2, dmax2);
+  hipLaunchKernelGGL(k_near_items, dim3(ceil_div(nc, EWF)), dim3(EWF), 0, stream, f.cells, ncp, f.p2p_cnt, f.params,
+                     f.near_items, f.near_combines, f.cursor, f.slot_cap);
+  const uint32_t extra = P.m_total / 64 < NEAR_EXTRA ? P.m_total / 64 + 1 : NEAR_EXTRA;
+  hipLaunchKernelGGL(k_near<Q>, dim3(extra + nc), dim3(64), 0, stream, f.cells, f.p2p_src, f.p2p_off, f.p2p_cnt,
+                     f.params, (const float2*)f.lexp, f.px, f.py, f.ps, f.ux, f.uy, t.perm, a.vx, a.vy, f.graphs,
+                     dmin2, dmax2, f.near_items, f.cursor, f.near_partial, extra);
+  hipLaunchKernelGGL(k_near_combine<Q>, dim3(extra), dim3(64), 0, stream, f.cells, f.params, f.near_combines,
+                     f.near_partial, (const float2*)f.lexp, f.ux, f.uy, t.perm, a.vx, a.vy, f.graphs);
   return hipGetLastError();
 }
 
@@ -1146,7 +1383,7 @@ using Orders = OrderTable<2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17
 
 void fmm_free(FmmBuffers& f) {
   void* ptrs[] = {f.graphs, f.seeds, f.node_off_dev, f.chunk_lo, f.chunk_hi, f.chunk_first, f.box_parts, f.cells, f.bbox, f.split, f.split_off, f.bounds, f.mexp, f.lexp, f.px, f.py, f.ux, f.uy, f.ps,
-                  f.front_a, f.front_b, f.m2l, f.p2p, f.m2l_src, f.p2p_src, f.m2l_off, f.m2l_cnt, f.p2p_off,
+                  f.front_a, f.front_b, f.long_list, f.near_items, f.near_combines, f.near_partial, f.m2l, f.p2p, f.m2l_src, f.p2p_src, f.m2l_off, f.m2l_cnt, f.p2p_off,
                   f.p2p_cnt, f.cursor, f.scan_partials, f.params};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

@@ -335,20 +335,25 @@ def energy_scale(p, src, tgt, link_strength, charge=-30.0, distance=30.0):
     return (b + math.sqrt(b * b + 4 * a * c)) / (2 * a) if a > 0 else 1.0
 
 
-def predict(net, g, jitter=1.0, seed=0):
+def predict(net, g, jitter=0.02, seed=0):
     """Predicted layout (n x 2) in the simulation's units: the network's shape,
     centred and scaled to the energy-minimising size (the network's global
     scale is its least reliable output, and a layout much larger or smaller
     than the simulation's costs refinement ticks), plus a seeded uniform
-    jitter of +-`jitter` so that nodes the network puts at one point do not
-    start coincident."""
+    jitter of +-`jitter` times the median predicted link length. Nodes with
+    identical features and neighbourhoods (leaves of one hub, isolated
+    nodes) get identical predictions; the jitter spreads them on the
+    layout's own scale, where a fixed jitter would leave them stacked (far
+    inside distanceMin, where their repulsion is largest)."""
     net.set_graph(g.row, g.col, g.x)
     p = net.forward().astype(np.float64)
     p -= p.mean(axis=0)
+    spread = jitter
     if len(g.src):
         p *= energy_scale(p, g.src, g.tgt, _link_strength(g.n, g.src, g.tgt))
-    if jitter:
-        p += np.random.default_rng(seed).uniform(-jitter, jitter, p.shape)
+        spread = jitter * float(np.median(np.hypot(*(p[g.src] - p[g.tgt]).T)))
+    if spread:
+        p += np.random.default_rng(seed).uniform(-spread, spread, p.shape)
     return p
 
 

@@ -266,6 +266,67 @@ def test_fmm_distance_ring(gpu_core, oracle_core):
     assert rms(out[1][0], out[0][0]) < FP32_RMS
 
 
+def stacked_clumps(seed=6):
+    """Leaves far above the leaf size: at the depth limit a leaf keeps every
+    point it holds. Uniform points over 1000 x 1000 plus a clump of 300
+    coincident points and clumps of 1500 and 700 points inside boxes
+    smaller than a depth-16 cell."""
+    rng = np.random.default_rng(seed)
+    xs = [1000 * (rng.random(3000) - 0.5)]
+    ys = [1000 * (rng.random(3000) - 0.5)]
+    xs.append(np.full(300, 120.25))
+    ys.append(np.full(300, -80.5))
+    for cnt, cx, cy in ((1500, -200.0, 310.0), (700, 401.0, 2.0)):
+        xs.append(cx + 1e-3 * rng.random(cnt))
+        ys.append(cy + 1e-3 * rng.random(cnt))
+    return np.concatenate(xs), np.concatenate(ys)
+
+
+def test_fmm_oversized_leaves(gpu_core, oracle_core):
+    # Such a leaf's targets are spread over several blocks; each target's sum
+    # keeps its order, so the result matches the FP64 FMM as any other leaf.
+    xs, ys = stacked_clumps()
+    a, sa = many_body_once(oracle_core, xs, ys)
+    b, sb = many_body_once(gpu_core, xs, ys)
+    assert fmm_counts(oracle_core, sa) == fmm_counts(gpu_core, sb)
+    err = rms(b, a)
+    print("  rms vs FP64 FMM %.2e" % err, flush=True)
+    assert err < FP32_RMS
+    c, _ = many_body_once(gpu_core, xs, ys)
+    assert same(b, c), "repeatable"
+
+
+def spread_quadrant_leaf(core_nodes, seed=3):
+    """A dense disc near the centre of the box and one quadrant holding a
+    handful of points spread corner to corner: that quadrant is a depth-1
+    leaf whose radius fails the opening test against every leaf of the disc,
+    so its P2P segment holds them all (1840 source leaves for 20000 disc
+    points, 530 for 8000: several near-field chunks, and for the larger one
+    longer than the in-LDS ordering takes)."""
+    rng = np.random.default_rng(seed)
+    r = 50 * np.sqrt(rng.random(core_nodes))
+    t = 2 * np.pi * rng.random(core_nodes)
+    qx = np.array([505.0, 995.0, 750.0, 1000.0, 0.0, 0.0])
+    qy = np.array([495.0, 5.0, 250.0, 1000.0, 1000.0, 0.0])
+    return np.concatenate([450 + r * np.cos(t), qx]), np.concatenate([550 + r * np.sin(t), qy])
+
+
+@pytest.mark.parametrize("core_nodes", [8000, 20000])
+def test_fmm_long_near_segments(gpu_core, oracle_core, core_nodes):
+    # A long segment is ordered in place and cut into chunks whose sums
+    # several blocks compute and a combine adds in chunk order: the result
+    # is repeatable bit for bit and within FP32 rounding of the FP64 FMM.
+    xs, ys = spread_quadrant_leaf(core_nodes)
+    a, sa = many_body_once(oracle_core, xs, ys)
+    b, sb = many_body_once(gpu_core, xs, ys)
+    assert fmm_counts(oracle_core, sa) == fmm_counts(gpu_core, sb)
+    err = rms(b, a)
+    print("  rms vs FP64 FMM %.2e" % err, flush=True)
+    assert err < FP32_RMS
+    c, _ = many_body_once(gpu_core, xs, ys)
+    assert same(b, c), "repeatable"
+
+
 def test_fmm_declines_a_non_finite_extent(gpu_core, oracle_core):
     # An infinite coordinate makes the extent non-finite: the exact kernel
     # runs instead (fmm.zig:255-257), so the result is bit-identical.
