@@ -51,6 +51,8 @@ constexpr uint32_t LONG_BLOCKS = 128;
 constexpr double MIN_SEP2 = 1e-28;      // fmm.zig:49
 constexpr int BN = 2 * FMM_MAX_ORDER + 1;
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 __constant__ float c_binom[BN * BN];  // C(n, k), n, k <= 2 * FMM_MAX_ORDER
 
 __device__ inline float binom(int n, int k) { return c_binom[n * BN + k]; }
@@ -708,72 +710,120 @@ __device__ inline uint32_t load_window(const uint32_t* __restrict__ src, uint32_
 
 // ---------------------------------------------------------------- M2L
 
+// M2L (fmm.zig:561-610) on the FP32 matrix cores. With w = 1/z0,
+// a = rho_B w and b = rho_A w, the radius-scaled translation is
+//   L^_l(A) += (-1)^l b^l w sum_k C(k + l, k) a^k M^_k(B),
+// i.e. U_k = a^k M^_k (per pair), V = P U with the constant Pascal matrix
+// P[l][k] = C(k + l, k), then L^_l += (-1)^l b^l w V_l. The middle step is
+// one batched GEMM whose left operand every pair shares: one wave per target
+// takes its sources (ordered by source) 16 at a time as the 16 columns of
+// v_mfma_f32_16x16x4f32 (real and imaginary parts in two products), rows
+// 16 t + i of P in row tiles of 16 and k in steps of 4. Lane L supplies
+// P[16 t + L % 16][4 s + L / 16] and U of pair L % 16 at k = 4 s + L / 16,
+// and receives V rows 16 t + 4 (L / 16) + r of pair L % 16, so the
+// post-scaling uses the lane's own pair. Each lane sums its pairs in source
+// order; the 16 lanes of a row group are then added by a fixed butterfly.
 template <int Q>
 __global__ __launch_bounds__(64) void k_m2l(const FCell* __restrict__ cells, const uint32_t* __restrict__ srcs,
                                             const uint32_t* __restrict__ offs, const uint32_t* __restrict__ cnts,
-                                            const float2* __restrict__ mexp, float2* __restrict__ lexp,
-                                            FmmParams* __restrict__ fp) {
+                                            const float2* __restrict__ mexp, float2* __restrict__ lexp) {
+  constexpr int RT = (Q + 15) / 16;  // row tiles of local coefficients
+  constexpr int KS = (Q + 3) / 4;    // k steps of multipole coefficients
   __shared__ uint32_t seg_in[SEG_MAX], seg[SEG_MAX];
   const uint32_t A = blockIdx.x;
   const int lane = threadIdx.x;
+  const int col = lane & 15, kq = lane >> 4;
   const uint32_t K = cnts[A], off = offs[A];
-  float2 acc[Q];
+  float pa[RT][KS];
 #pragma unroll
-  for (int l = 0; l < Q; ++l) acc[l] = make_float2(0.f, 0.f);
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int l = 16 * t + col, k = 4 * s + kq;
+      pa[t][s] = (l < Q && k < Q) ? binom(k + l, k) : 0.f;
+    }
+  f32x4 accr[RT], acci[RT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    accr[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    acci[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
   if (K > 0) {
     const FCell ca = cells[A];
     for (uint32_t w0 = 0; w0 < K; w0 += SEG_MAX) {
       const uint32_t wk = load_window(srcs, off, K, w0, seg_in, seg);
-      for (uint32_t base = 0; base < wk; base += 64) {
-        const uint32_t s = base + lane;
-        if (s >= wk) continue;
-        const uint32_t B = seg[s];
-        const FCell cb = cells[B];
-        const double dx = ca.cx - cb.cx, dy = ca.cy - cb.cy;
-        const double inv = 1 / (dx * dx + dy * dy);
-        const double wr = dx * inv, wi = -dy * inv;
-        const float2 w = make_float2((float)wr, (float)wi);
-        const float2 a = make_float2((float)(cb.r * wr), (float)(cb.r * wi));
-        const float2 b = make_float2((float)(ca.r * wr), (float)(ca.r * wi));
-        float2 u[Q];
-        float2 ap = make_float2(1.f, 0.f);
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
-          u[k] = cmul(mexp[(size_t)B * Q + k], ap);
-          ap = cmul(ap, a);
+      for (uint32_t base = 0; base < wk; base += 16) {
+        const uint32_t sp = base + col;
+        float2 w = make_float2(0.f, 0.f), a = w, b = w;
+        uint32_t B = 0;
+        if (sp < wk) {
+          B = seg[sp];
+          const FCell cb = cells[B];
+          const double dx = ca.cx - cb.cx, dy = ca.cy - cb.cy;
+          const double inv = 1 / (dx * dx + dy * dy);
+          const double wr = dx * inv, wi = -dy * inv;
+          w = make_float2((float)wr, (float)wi);
+          a = make_float2((float)(cb.r * wr), (float)(cb.r * wi));
+          b = make_float2((float)(ca.r * wr), (float)(ca.r * wi));
         }
-        float2 coef = w;  // (-1)^l b^l w
+        const bool valid = sp < wk;
+        float2 ak = make_float2(1.f, 0.f);
+        for (int q = 0; q < kq; ++q) ak = cmul(ak, a);
+        const float2 a2 = cmul(a, a), a4 = cmul(a2, a2);
+        f32x4 vr[RT], vi[RT];
 #pragma unroll
-        for (int l = 0; l < Q; ++l) {
-          float2 v = make_float2(0.f, 0.f);
+        for (int t = 0; t < RT; ++t) {
+          vr[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+          vi[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
 #pragma unroll
-          for (int k = 0; k < Q; ++k) {
-            const float g = binom(k + l, k);
-            v.x = fmaf(g, u[k].x, v.x);
-            v.y = fmaf(g, u[k].y, v.y);
+        for (int s = 0; s < KS; ++s) {
+          const int k = 4 * s + kq;
+          const float2 u = (valid && k < Q) ? cmul(mexp[(size_t)B * Q + k], ak) : make_float2(0.f, 0.f);
+          ak = cmul(ak, a4);
+#pragma unroll
+          for (int t = 0; t < RT; ++t) {
+            vr[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[t][s], u.x, vr[t], 0, 0, 0);
+            vi[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[t][s], u.y, vi[t], 0, 0, 0);
           }
-          acc[l] = cfma(coef, v, acc[l]);
-          coef = cmul(coef, b);
-          coef.x = -coef.x;
-          coef.y = -coef.y;
+        }
+        // coefficient (-b)^l w for rows l = 16 t + 4 kq + r
+        const float2 nb = make_float2(-b.x, -b.y);
+        const float2 nb2 = cmul(nb, nb), nb4 = cmul(nb2, nb2);
+        const float2 nb8 = cmul(nb4, nb4), nb16 = cmul(nb8, nb8);
+        float2 c = w;
+        for (int q = 0; q < kq; ++q) c = cmul(c, nb4);
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+          float2 cr = c;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            accr[t][r] = fmaf(cr.x, vr[t][r], fmaf(-cr.y, vi[t][r], accr[t][r]));
+            acci[t][r] = fmaf(cr.x, vi[t][r], fmaf(cr.y, vr[t][r], acci[t][r]));
+            cr = cmul(cr, nb);
+          }
+          c = cmul(c, nb16);
         }
       }
     }
 #pragma unroll
-    for (int l = 0; l < Q; ++l) {
+    for (int t = 0; t < RT; ++t)
 #pragma unroll
-      for (int d = 32; d > 0; d >>= 1) {
-        acc[l].x += __shfl_xor(acc[l].x, d, 64);
-        acc[l].y += __shfl_xor(acc[l].y, d, 64);
-      }
-    }
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int d = 8; d > 0; d >>= 1) {
+          accr[t][r] += __shfl_xor(accr[t][r], d, 64);
+          acci[t][r] += __shfl_xor(acci[t][r], d, 64);
+        }
   }
-  if (lane < Q) {
-    float2 v = acc[0];
+  if (col == 0) {
 #pragma unroll
-    for (int l = 1; l < Q; ++l)
-      if (lane == l) v = acc[l];
-    lexp[(size_t)A * Q + lane] = v;
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int l = 16 * t + 4 * kq + r;
+        if (l < Q) lexp[(size_t)A * Q + l] = make_float2(accr[t][r], acci[t][r]);
+      }
   }
 }
 
@@ -1349,7 +1399,7 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
                      f.m2l_off, f.m2l_cnt, (uint32_t*)f.m2l, f.p2p_src, f.p2p_off, f.p2p_cnt, (uint32_t*)f.p2p);
   // far field
   hipLaunchKernelGGL(k_m2l<Q>, dim3(nc), dim3(64), 0, stream, f.cells, f.m2l_src, f.m2l_off, f.m2l_cnt,
-                     (const float2*)f.mexp, (float2*)f.lexp, f.params);
+                     (const float2*)f.mexp, (float2*)f.lexp);
   for (uint32_t d = 0; d < maxd; ++d) {
     const uint32_t cnt = P.level_n[d];
     hipLaunchKernelGGL(k_fmm_down<Q>, dim3(ceil_div(cnt, EWF)), dim3(EWF), 0, stream, f.cells, (float2*)f.lexp,
