@@ -327,8 +327,29 @@ hipError_t direct_sum(const double* x, const double* y, const double* strength, 
                       uint32_t k, double alpha, double dmin2, double dmax2, double* out, hipStream_t stream);
 
 // ---- simple.hip ----
-// sums[0..1] = d3's serial sums of x and y (one workgroup).
-hipError_t center_raw_sum(const double* x, const double* y, uint32_t n, double* sums, hipStream_t stream);
+// Workspace of the chunked center sums (per axis and chunk of 32 nodes).
+struct CenterWork {
+  uint32_t cap = 0;                 // chunks per axis
+  double* csum = nullptr;           // 2 cap: chunk sums in any order (the binade guesses)
+  double* start = nullptr;          // 2 cap: exclusive scan of csum
+  long long* ct = nullptr;          // 2 cap x 6: chunk transforms in ulp units
+  uint32_t* meta = nullptr;         // 2 cap: binade, sign, valid
+  uint32_t* need = nullptr;         // 2 cap: chunk predicted to need its values
+  uint32_t* fidx = nullptr;         // 2 cap: exclusive scan of need
+  uint32_t* segflag = nullptr;      // 2 cap: a segment starts at the chunk
+  uint32_t* segidx = nullptr;       // 2 cap: exclusive scan of segflag
+  double* segtmp = nullptr;         // 2 cap x 8: segment records at their first chunk
+  double* seglist = nullptr;        // 2 cap x 8: segment records in order
+  uint32_t* round_f = nullptr;      // 2 (cap / 128 + 2): staged index at each staging round
+  uint32_t* segrange = nullptr;     // 4: first segment and end of each axis
+  double* staged = nullptr;         // 2 cap x 32: values of needing chunks, in order
+  uint32_t* partials = nullptr;
+  uint32_t* count = nullptr;
+};
+void center_work_free(CenterWork& w);
+// sums[0..1] = d3's serial sums of x and y, bit for bit.
+hipError_t center_raw_sum(const double* x, const double* y, uint32_t n, double* sums, CenterWork& w,
+                          hipStream_t stream);
 // x -= (sums[0] / n - cx) * strength, y likewise.
 hipError_t center_apply(double* x, double* y, uint32_t n, const double* sums, double cx, double cy, double strength,
                         hipStream_t stream);

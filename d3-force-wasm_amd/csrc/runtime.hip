@@ -158,6 +158,7 @@ struct Sim {
   hipStream_t side = nullptr;
   hipEvent_t ev_x_ready = nullptr, ev_sums_done = nullptr;
   double* dsums_side = nullptr;
+  CenterWork cw_side, cw_main;  // workspaces of the sums on each stream
   bool side_valid = false;   // dsums_side holds the sums of the current x, y
   bool side_busy = false;    // the side stream may still be reading x, y
   uint32_t centers = 0;      // center forces of this simulation
@@ -211,6 +212,8 @@ struct Sim {
     if (dcounters) (void)hipFree(dcounters);
     if (side) (void)hipStreamSynchronize(side);
     if (dsums_side) (void)hipFree(dsums_side);
+    center_work_free(cw_side);
+    center_work_free(cw_main);
     if (ev_x_ready) (void)hipEventDestroy(ev_x_ready);
     if (ev_sums_done) (void)hipEventDestroy(ev_sums_done);
     if (side) (void)hipStreamDestroy(side);
@@ -281,7 +284,7 @@ struct Sim {
     HIPCHECK(hipStreamWaitEvent(side, ev_x_ready, 0));
     hipEvent_t a = g_timing ? take_event() : nullptr;
     if (a) HIPCHECK(hipEventRecord(a, side));
-    HIPCHECK(center_raw_sum(dx.p, dy.p, n, dsums_side, side));
+    HIPCHECK(center_raw_sum(dx.p, dy.p, n, dsums_side, cw_side, side));
     if (a) {
       hipEvent_t b = take_event();
       if (b) {
@@ -684,7 +687,7 @@ int32_t run_center(Sim& s, Force& f) {  // simple.zig:14-31
   } else {
     sums = s.dscalar_d;
     e = s.before_x_write();
-    if (e == hipSuccess) e = center_raw_sum(s.dx.p, s.dy.p, s.n, s.dscalar_d, s.stream);
+    if (e == hipSuccess) e = center_raw_sum(s.dx.p, s.dy.p, s.n, s.dscalar_d, s.cw_main, s.stream);
   }
   s.side_valid = false;
   if (e == hipSuccess)

@@ -83,6 +83,364 @@ __global__ __launch_bounds__(CS_THREADS) void k_center_raw_sum(const double* __r
   if (wave < 2 && lane == 0) out[wave] = s;
 }
 
+// ---- the same sums, by binade chunks and segments (n >= CHUNKED_MIN)
+//
+// Let s be a partial sum in the binade [2^e, 2^(e+1)) (or its negative) with
+// ulp u = 2^(e-52): s = m u for an integer m, and while s + x lands strictly
+// inside the same binade, fl(s + x) = (m + r) u with r = x / u rounded to the
+// nearest integer, ties to the r that makes m + r even. So over a chunk of 32
+// values whose partial sums all stay at least one ulp inside the binade, the
+// chain is exactly m + (r_1 + ... + r_32); the r depend on m only through its
+// parity at the ties. For each start parity p a chunk is a transform "start
+// m in [L_p, H_p] -> m + D_p" (ulp units, int64, exact), and consecutive
+// chunks in one binade compose into one transform of the same form: for
+// T then B, L = max(L_T, L_B' - D_T), H = min(H_T, H_B' - D_T), D = D_T +
+// D_B' with B' the variant for parity (p + D_T) & 1. The binade of each
+// chunk is guessed from an approximate prefix sum (chunk sums scanned in
+// another order); chunks whose guessed start is not safely inside their
+// interval, or that have no transform, are "needing" chunks added one by
+// one (their values are staged in order); runs of the others, within one
+// binade and one 256-chunk block, become segments. The serial kernel walks
+// the records: s += D when lo <= s <= hi (lo = L u, hi = H u, exact
+// doubles), else the values one by one, so a wrong guess costs time only.
+constexpr uint32_t CHUNKED_MIN = 65536;
+constexpr uint32_t CC = 32;        // values per chunk
+constexpr uint32_t SEG_CAP = 256;  // segments do not cross multiples of this many chunks
+constexpr uint32_t CD = 8;         // doubles per record: lo0 hi0 D0 lo1 hi1 D1 (fidx, need) (chunk, len)
+constexpr uint32_t CR = 128;       // records per staging round (two per lane); every needing one is staged
+static_assert(CR == 128, "the chain reads two records per lane");
+constexpr int CK_THREADS = 256;    // wave 0 adds, waves 1-3 stage
+constexpr uint32_t CS_RTAB = 4096; // rounds whose staging indices are kept in LDS
+constexpr uint32_t META_VALID = 1u << 13, META_NEG = 1u << 12;
+
+__global__ __launch_bounds__(EW) void k_cs_chunk_sums(const double* __restrict__ xs, const double* __restrict__ ys,
+                                                      uint32_t n, uint32_t nch, double* __restrict__ csum) {
+  const uint32_t t = blockIdx.x * EW + threadIdx.x;
+  if (t >= 2 * nch) return;
+  const uint32_t axis = t >= nch, j = t - axis * nch;
+  const double* p = axis ? ys : xs;
+  const uint32_t b = j * CC, e = n - b < CC ? n : b + CC;
+  double acc = 0;
+  for (uint32_t k = b; k < e; ++k) acc += p[k];
+  csum[t] = acc;
+}
+
+// Exclusive scan of one axis's chunk sums (block = axis).
+__global__ __launch_bounds__(1024) void k_cs_start(const double* __restrict__ csum, uint32_t nch,
+                                                   double* __restrict__ start) {
+  __shared__ double part[1024];
+  const uint32_t axis = blockIdx.x, tid = threadIdx.x;
+  const uint32_t per = ceil_div(nch, 1024u);
+  const uint32_t b = tid * per, e = b + per < nch ? b + per : nch;
+  const double* c = csum + (size_t)axis * nch;
+  double acc = 0;
+  for (uint32_t k = b; k < e; ++k) acc += c[k];
+  part[tid] = acc;
+  __syncthreads();
+  for (uint32_t d = 1; d < 1024; d <<= 1) {
+    const double v = tid >= d ? part[tid - d] : 0.0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  double run = tid ? part[tid - 1] : 0.0;
+  for (uint32_t k = b; k < e; ++k) {
+    start[(size_t)axis * nch + k] = run;
+    run += c[k];
+  }
+}
+
+__device__ inline double cs_pack(uint32_t lo, uint32_t hi) { return __hiloint2double((int)hi, (int)lo); }
+
+// One chunk: its binade (from the guessed start), its two transforms and
+// whether the guess is safely inside both.
+__global__ __launch_bounds__(EW) void k_cs_desc(const double* __restrict__ xs, const double* __restrict__ ys,
+                                                uint32_t n, uint32_t nch, const double* __restrict__ start,
+                                                long long* __restrict__ ct, uint32_t* __restrict__ meta,
+                                                uint32_t* __restrict__ need, uint32_t* __restrict__ count) {
+  const uint32_t t = blockIdx.x * EW + threadIdx.x;
+  if (t == 0) *count = 2 * nch;
+  if (t >= 2 * nch) return;
+  const uint32_t axis = t >= nch, j = t - axis * nch;
+  const double* p = axis ? ys : xs;
+  const double g = start[t];
+  const uint32_t b = j * CC;
+  uint32_t mt = 0;
+  bool nd = true;
+  long long L[2] = {1, 1}, H[2] = {0, 0}, D[2] = {0, 0};
+  if (b + CC <= n && g != 0 && g - g == 0) {
+    int E;
+    (void)__builtin_frexp(g, &E);
+    const int e = E - 1;  // |g| in [2^e, 2^(e+1))
+    if (e > -960 && e < 1000) {
+      const long long two52 = 4503599627370496LL, two53 = 2 * two52;
+      // start and partial-sum ranges in ulp units
+      const long long in_lo = g > 0 ? two52 : -(two53 - 1), in_hi = g > 0 ? two53 - 1 : -two52;
+      const long long out_lo = g > 0 ? two52 + 1 : -(two53 - 1), out_hi = g > 0 ? two53 - 1 : -(two52 + 1);
+      double v[CC];
+#pragma unroll
+      for (uint32_t k = 0; k < CC; ++k) v[k] = p[b + k];
+      const double lim = 1152921504606846976.0;  // 2^60
+      bool all_ok = true;
+      for (int par0 = 0; par0 < 2; ++par0) {
+        long long P = 0, mn = 0, mx = 0;
+        int par = par0;
+        bool ok = true;
+#pragma unroll
+        for (uint32_t k = 0; k < CC; ++k) {
+          double q = __builtin_ldexp(v[k], 52 - e);  // x / u, exact unless |q| < 2^-1022
+          if (!(__builtin_fabs(q) <= lim)) {
+            ok = false;
+            q = 0;
+          }
+          const double f = __builtin_floor(q), fr = q - f;
+          long long r = (long long)f;
+          if (fr > 0.5 || (fr == 0.5 && ((par + r) & 1))) r += 1;
+          P += r;
+          par = (int)((par + r) & 1);
+          if (k == 0) {
+            mn = P;
+            mx = P;
+          } else {
+            mn = P < mn ? P : mn;
+            mx = P > mx ? P : mx;
+          }
+        }
+        if (!ok || mn < -two53 || mx > two53) {
+          all_ok = false;
+          continue;
+        }
+        L[par0] = in_lo > out_lo - mn ? in_lo : out_lo - mn;
+        H[par0] = in_hi < out_hi - mx ? in_hi : out_hi - mx;
+        D[par0] = P;
+      }
+      if (all_ok) {
+        mt = META_VALID | (g < 0 ? META_NEG : 0u) | (uint32_t)(e + 1100);
+        // the guess is inexact: keep it 2^-36 |g| inside both intervals
+        const double u = __builtin_ldexp(1.0, e - 52);
+        const double gm = g / u, mg = __builtin_fabs(gm) * 1.4551915228366852e-11;
+        nd = !((double)L[0] + mg <= gm && gm <= (double)H[0] - mg && (double)L[1] + mg <= gm &&
+               gm <= (double)H[1] - mg);
+      }
+    }
+  }
+  long long* c = ct + (size_t)t * 6;
+  c[0] = L[0];
+  c[1] = H[0];
+  c[2] = D[0];
+  c[3] = L[1];
+  c[4] = H[1];
+  c[5] = D[1];
+  meta[t] = mt;
+  need[t] = nd ? 1u : 0u;
+}
+
+// Segment starts, and each segment's composed record at its first chunk.
+__global__ __launch_bounds__(EW) void k_cs_segments(uint32_t nch, const long long* __restrict__ ct,
+                                                    const uint32_t* __restrict__ meta,
+                                                    const uint32_t* __restrict__ need, uint32_t* __restrict__ segflag,
+                                                    double* __restrict__ segtmp) {
+  const uint32_t t = blockIdx.x * EW + threadIdx.x;
+  if (t >= 2 * nch) return;
+  const uint32_t j = t - (t >= nch) * nch;
+  const bool st = need[t] || j % SEG_CAP == 0 || need[t - 1] || meta[t] != meta[t - 1];
+  segflag[t] = st;
+  if (!st) return;
+  const double inf = __builtin_huge_val();
+  double* rec = segtmp + (size_t)t * CD;
+  uint32_t len = 1;
+  if (need[t]) {
+    rec[0] = inf;
+    rec[1] = -inf;
+    rec[2] = 0;
+    rec[3] = inf;
+    rec[4] = -inf;
+    rec[5] = 0;
+  } else {
+    const long long* c = ct + (size_t)t * 6;
+    long long L[2] = {c[0], c[3]}, H[2] = {c[1], c[4]}, D[2] = {c[2], c[5]};
+    for (uint32_t k = t + 1; k < t - j + nch && (k - t + j) % SEG_CAP != 0 && !need[k] && meta[k] == meta[t]; ++k) {
+      const long long* cb = ct + (size_t)k * 6;
+#pragma unroll
+      for (int pp = 0; pp < 2; ++pp) {
+        const int q = (int)((pp + D[pp]) & 1);
+        const long long lb = cb[3 * q] - D[pp], hb = cb[3 * q + 1] - D[pp];
+        L[pp] = L[pp] > lb ? L[pp] : lb;
+        H[pp] = H[pp] < hb ? H[pp] : hb;
+        D[pp] += cb[3 * q + 2];
+      }
+      ++len;
+    }
+    const int e = (int)(meta[t] & 0xfff) - 1100;
+    const double u = __builtin_ldexp(1.0, e - 52);
+#pragma unroll
+    for (int pp = 0; pp < 2; ++pp) {
+      const bool okp = L[pp] <= H[pp];
+      rec[3 * pp] = okp ? (double)L[pp] * u : inf;  // |L|, |H| <= 2^53: exact
+      rec[3 * pp + 1] = okp ? (double)H[pp] * u : -inf;
+      rec[3 * pp + 2] = okp ? (double)D[pp] * u : 0.0;  // |D| <= 2^53 when the interval is not empty
+    }
+  }
+  rec[7] = cs_pack(j, len);
+}
+
+// Segment records in order, each with its staging index; per-axis record
+// ranges; the staging index at each round of CR records.
+__global__ __launch_bounds__(EW) void k_cs_seglist(uint32_t nch, const uint32_t* __restrict__ segflag,
+                                                   const uint32_t* __restrict__ segidx,
+                                                   const uint32_t* __restrict__ need,
+                                                   const uint32_t* __restrict__ fidx,
+                                                   const double* __restrict__ segtmp, double* __restrict__ seglist,
+                                                   uint32_t* __restrict__ round_f, uint32_t* __restrict__ segrange,
+                                                   uint32_t rstride) {
+  const uint32_t t = blockIdx.x * EW + threadIdx.x;
+  if (t >= 2 * nch) return;
+  const uint32_t axis = t >= nch, j = t - axis * nch;
+  const uint32_t base = segidx[axis * nch];
+  if (j == 0) segrange[2 * axis] = base;
+  if (j == nch - 1) {
+    const uint32_t end = segidx[t] + segflag[t];
+    segrange[2 * axis + 1] = end;
+    round_f[axis * rstride + ceil_div(end - base, CR)] = fidx[t] + need[t];
+  }
+  if (!segflag[t]) return;
+  const uint32_t k = segidx[t];
+  double* r = seglist + (size_t)k * CD;
+  const double* tmp = segtmp + (size_t)t * CD;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) r[f] = tmp[f];
+  r[6] = cs_pack(fidx[t], need[t]);
+  r[7] = tmp[7];
+  if ((k - base) % CR == 0) round_f[axis * rstride + (k - base) / CR] = fidx[t];
+}
+
+// Copies the values of needing chunks (the last, partial chunk padded with
+// -0, which leaves any sum unchanged) into their staging slots.
+__global__ __launch_bounds__(EW) void k_cs_stage(const double* __restrict__ xs, const double* __restrict__ ys,
+                                                 uint32_t n, uint32_t nch, const uint32_t* __restrict__ need,
+                                                 const uint32_t* __restrict__ fidx, double* __restrict__ staged) {
+  const uint32_t t = blockIdx.x * EW + threadIdx.x;
+  if (t >= 2 * nch || !need[t]) return;
+  const uint32_t axis = t >= nch, j = t - axis * nch;
+  const double* p = axis ? ys : xs;
+  const uint32_t b = j * CC;
+  const uint32_t f = fidx[t];
+  for (uint32_t k = 0; k < CC; ++k) staged[(size_t)f * CC + k] = b + k < n ? p[b + k] : -0.0;
+}
+
+// The serial chain of one axis (block = axis): wave 0 walks the records;
+// waves 1-3 stage the next round's records and staged values in LDS. Lane l
+// of wave 0 reads records l and l + 64 of the round once; record k's fields
+// are taken with readlane into scalar registers, and the interval test is a
+// ballot, so every branch is wave-uniform (every lane holds the same s).
+__global__ __launch_bounds__(CK_THREADS) void k_cs_chain(const double* __restrict__ xs, const double* __restrict__ ys,
+                                                         uint32_t n, const double* __restrict__ seglist,
+                                                         const double* __restrict__ staged,
+                                                         const uint32_t* __restrict__ round_f,
+                                                         const uint32_t* __restrict__ segrange, uint32_t rstride,
+                                                         double* __restrict__ out) {
+  __shared__ double dbuf[2][CR * CD];
+  __shared__ double sbuf[2][CR * CC];
+  __shared__ uint32_t f0buf[2];
+  __shared__ uint32_t rtab[CS_RTAB];
+  const uint32_t axis = blockIdx.x;
+  const double* p = axis ? ys : xs;
+  const uint32_t base = segrange[2 * axis], S = segrange[2 * axis + 1] - base;
+  const double* rec = seglist + (size_t)base * CD;
+  const uint32_t rounds = ceil_div(S, CR);
+  const uint32_t* rf = round_f + axis * rstride;
+  const bool tab = rounds + 1 <= CS_RTAB;
+  if (tab)
+    for (uint32_t q = threadIdx.x; q <= rounds; q += CK_THREADS) rtab[q] = rf[q];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6;
+  const uint32_t ptid = threadIdx.x - 64;  // producer thread 0..191
+  // All loads of a round are issued before any LDS store: one memory round
+  // trip per round.
+  auto stage = [&](uint32_t r, int b) {
+    constexpr uint32_t PT = CK_THREADS - 64;
+    constexpr uint32_t ND = (CR * CD + PT - 1) / PT, NS = (CR * CC + PT - 1) / PT;
+    const uint32_t c0 = r * CR, cnt = S - c0 < CR ? S - c0 : CR;
+    const uint32_t f0 = tab ? rtab[r] : rf[r], m = (tab ? rtab[r + 1] : rf[r + 1]) - f0;  // m <= cnt
+    double vd[ND], vs[NS];
+#pragma unroll
+    for (uint32_t u = 0; u < ND; ++u) {
+      const uint32_t q = ptid + u * PT;
+      vd[u] = q < cnt * CD ? rec[(size_t)c0 * CD + q] : 0.0;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < NS; ++u) {
+      const uint32_t q = ptid + u * PT;
+      vs[u] = q < m * CC ? staged[(size_t)f0 * CC + q] : 0.0;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < ND; ++u) {
+      const uint32_t q = ptid + u * PT;
+      if (q < cnt * CD) dbuf[b][q] = vd[u];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < NS; ++u) {
+      const uint32_t q = ptid + u * PT;
+      if (q < m * CC) sbuf[b][q] = vs[u];
+    }
+    if (ptid == 0) f0buf[b] = f0;
+  };
+  double s = 0;
+  if (wave > 0 && rounds) stage(0, 0);
+  __syncthreads();
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const int b = r & 1;
+    if (wave > 0) {
+      if (r + 1 < rounds) stage(r + 1, b ^ 1);
+    } else {
+      const uint32_t c0 = r * CR, cnt = S - c0 < CR ? S - c0 : CR;
+      const uint32_t f0 = __builtin_amdgcn_readfirstlane(f0buf[b]);
+      const int lane = threadIdx.x & 63;
+      const double* d = dbuf[b];
+      double da[CD], db[CD];
+#pragma unroll
+      for (uint32_t f = 0; f < CD; ++f) {
+        da[f] = d[lane * CD + f];
+        db[f] = d[(lane + 64) * CD + f];
+      }
+      auto rl = [](double v, uint32_t l) {
+        return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                                __builtin_amdgcn_readlane(__double2loint(v), l));
+      };
+      auto record = [&](const double (&v)[CD], uint32_t l) {
+        const double lo0 = rl(v[0], l), hi0 = rl(v[1], l), d0 = rl(v[2], l);
+        const double lo1 = rl(v[3], l), hi1 = rl(v[4], l), d1 = rl(v[5], l);
+        const bool odd = __builtin_amdgcn_readfirstlane(__double2loint(s)) & 1;
+        const double lo = odd ? lo1 : lo0, hi = odd ? hi1 : hi0, dd = odd ? d1 : d0;
+        if (__builtin_amdgcn_ballot_w64(lo <= s && s <= hi)) {
+          s += dd;
+          return;
+        }
+        const uint32_t fi = __builtin_amdgcn_readlane(__double2loint(v[6]), l);
+        const uint32_t nd = __builtin_amdgcn_readlane(__double2hiint(v[6]), l);
+        if (nd) {
+          const double* x = sbuf[b] + (fi - f0) * CC;
+          double a[CC];
+#pragma unroll
+          for (uint32_t e = 0; e < CC; ++e) a[e] = x[e];
+#pragma unroll
+          for (uint32_t e = 0; e < CC; ++e) s += a[e];
+        } else {  // a segment whose guess failed: its values from global memory
+          const uint32_t j = __builtin_amdgcn_readlane(__double2loint(v[7]), l);
+          const uint32_t len = __builtin_amdgcn_readlane(__double2hiint(v[7]), l);
+          const uint32_t lo_e = j * CC, hi_e = n - lo_e < len * CC ? n : lo_e + len * CC;
+          for (uint32_t e = lo_e; e < hi_e; ++e) s += p[e];
+        }
+      };
+      const uint32_t m1 = cnt < 64 ? cnt : 64;
+      for (uint32_t k = 0; k < m1; ++k) record(da, k);
+      for (uint32_t k = 64; k < cnt; ++k) record(db, k - 64);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[axis] = s;
+}
+
 // A batch of graphs: one wave per graph; lane 0 adds x and lane 1 adds y in
 // node order (simple.zig:18-22) from LDS, the next 64 values already in
 // flight.
@@ -212,8 +570,75 @@ hipError_t batch_center(double* x, double* y, const uint32_t* node_off, const ui
   return hipGetLastError();
 }
 
-hipError_t center_raw_sum(const double* x, const double* y, uint32_t n, double* sums, hipStream_t stream) {
-  hipLaunchKernelGGL(k_center_raw_sum, dim3(1), dim3(CS_THREADS), 0, stream, x, y, n, sums);
+void center_work_free(CenterWork& w) {
+  void* ptrs[] = {w.csum, w.start, w.ct, w.meta, w.need, w.fidx, w.segflag, w.segidx, w.segtmp, w.seglist,
+                  w.round_f, w.segrange, w.staged, w.partials, w.count};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  w = CenterWork();
+}
+
+static uint32_t cs_rstride(uint32_t cap) { return cap / CR + 2; }
+
+static hipError_t center_reserve(CenterWork& w, uint32_t nch) {
+  if (nch <= w.cap) return hipSuccess;
+  center_work_free(w);
+  const uint32_t cap = nch + nch / 8 + 64;
+  const size_t c2 = (size_t)2 * cap;
+  HIPCHECK(hipMalloc(&w.csum, c2 * sizeof(double)));
+  HIPCHECK(hipMalloc(&w.start, c2 * sizeof(double)));
+  HIPCHECK(hipMalloc(&w.ct, c2 * 6 * sizeof(long long)));
+  HIPCHECK(hipMalloc(&w.meta, c2 * sizeof(uint32_t)));
+  HIPCHECK(hipMalloc(&w.need, c2 * sizeof(uint32_t)));
+  HIPCHECK(hipMalloc(&w.fidx, c2 * sizeof(uint32_t)));
+  HIPCHECK(hipMalloc(&w.segflag, c2 * sizeof(uint32_t)));
+  HIPCHECK(hipMalloc(&w.segidx, c2 * sizeof(uint32_t)));
+  HIPCHECK(hipMalloc(&w.segtmp, c2 * CD * sizeof(double)));
+  HIPCHECK(hipMalloc(&w.seglist, c2 * CD * sizeof(double)));
+  HIPCHECK(hipMalloc(&w.round_f, (size_t)2 * cs_rstride(cap) * sizeof(uint32_t)));
+  HIPCHECK(hipMalloc(&w.segrange, 4 * sizeof(uint32_t)));
+  HIPCHECK(hipMalloc(&w.staged, c2 * CC * sizeof(double)));
+  HIPCHECK(hipMalloc(&w.partials, (c2 / 4096 + 64) * sizeof(uint32_t)));
+  HIPCHECK(hipMalloc(&w.count, sizeof(uint32_t)));
+  w.cap = cap;
+  return hipSuccess;
+}
+
+hipError_t center_raw_sum(const double* x, const double* y, uint32_t n, double* sums, CenterWork& w,
+                          hipStream_t stream) {
+  if (n < CHUNKED_MIN) {
+    hipLaunchKernelGGL(k_center_raw_sum, dim3(1), dim3(CS_THREADS), 0, stream, x, y, n, sums);
+    return hipGetLastError();
+  }
+  const uint32_t nch = ceil_div(n, CC);
+  HIPCHECK(center_reserve(w, nch));
+  const uint32_t rstride = cs_rstride(w.cap);
+  const uint32_t g = ceil_div(2 * nch, (uint32_t)EW);
+  hipLaunchKernelGGL(k_cs_chunk_sums, dim3(g), dim3(EW), 0, stream, x, y, n, nch, w.csum);
+  hipLaunchKernelGGL(k_cs_start, dim3(2), dim3(1024), 0, stream, w.csum, nch, w.start);
+  hipLaunchKernelGGL(k_cs_desc, dim3(g), dim3(EW), 0, stream, x, y, n, nch, w.start, w.ct, w.meta, w.need, w.count);
+  hipLaunchKernelGGL(k_cs_segments, dim3(g), dim3(EW), 0, stream, nch, w.ct, w.meta, w.need, w.segflag, w.segtmp);
+  HIPCHECK(scan_u32(w.need, w.fidx, 2 * nch, w.count, w.partials, nullptr, stream));
+  HIPCHECK(scan_u32(w.segflag, w.segidx, 2 * nch, w.count, w.partials, nullptr, stream));
+  hipLaunchKernelGGL(k_cs_seglist, dim3(g), dim3(EW), 0, stream, nch, w.segflag, w.segidx, w.need, w.fidx, w.segtmp,
+                     w.seglist, w.round_f, w.segrange, rstride);
+  hipLaunchKernelGGL(k_cs_stage, dim3(g), dim3(EW), 0, stream, x, y, n, nch, w.need, w.fidx, w.staged);
+  // Each chain block takes its CU's whole LDS, so no other kernel's
+  // workgroups (the tick's traversal runs meanwhile) share its SIMDs.
+  static size_t pad = 0;
+  if (!pad) {
+    hipFuncAttributes fa;
+    HIPCHECK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_cs_chain)));
+    int dev = 0, lds = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    pad = lds > (int)fa.sharedSizeBytes ? (size_t)lds - fa.sharedSizeBytes : 1;
+    if (pad > 1)
+      HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cs_chain),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)pad));
+  }
+  hipLaunchKernelGGL(k_cs_chain, dim3(2), dim3(CK_THREADS), pad > 1 ? pad : 0, stream, x, y, n, w.seglist,
+                     w.staged, w.round_f, w.segrange, rstride, sums);
   return hipGetLastError();
 }
 
