@@ -356,7 +356,7 @@ def main():
     C = 1.72 * n
     hbm_bytes = {"tree": 28 * n + 88 * C, "accumulate": 88 * C + 24 * n, "link": 128 * m,
                  "integrate": 81 * n, "center_sum": 16 * n}
-    names = {"traversal": "k_traverse<RECORD> (Barnes-Hut traversal)", "center_sum": "k_center_raw_sum (serial sums)",
+    names = {"traversal": "k_traverse<RECORD> (Barnes-Hut traversal)", "center_sum": "k_cs_chain (center sums, side stream)",
              "tree": "quadtree build (11 kernels)", "accumulate": "k_accumulate_level (all levels)",
              "link": "k_link_level (all levels)", "integrate": "k_integrate"}
     kernels = {}
@@ -378,12 +378,16 @@ def main():
             k = {"kernel": names[ph], "bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": a / HBM_PEAK_GBS, "bytes_per_launch": hbm_bytes[ph]}
             if ph == "center_sum":
-                k["note"] = ("serial dependency chain: n FP64 adds per axis in index order, both axes in one "
-                             "instruction stream; %.2f ns per dependent add" % (t * 1e9 / n))
+                k["note"] = ("second stream, overlapped with the tick: d3's serial sums by exact binade segments "
+                             "(chunk transforms composed in parallel, one serial pass over the segment records); "
+                             "%.2f ns per node from the side stream's start to its end" % (t * 1e9 / n))
         k["launch_ms"] = t * 1e3
         k["traffic"] = read_traffic(names[ph].split(" ")[0].split("<")[0])
         kernels[ph] = k
-    dominant = max(kernels, key=lambda ph: kernels[ph]["launch_ms"] * max(st["launches"][ph], 1))
+    # The dominant kernel is taken on the tick's critical path: the center sums
+    # run on a second stream, overlapped with the tree build and traversal.
+    dominant = max((ph for ph in kernels if ph != "center_sum"),
+                   key=lambda ph: kernels[ph]["launch_ms"] * max(st["launches"][ph], 1))
     roof = dict(kernels[dominant])
 
     fast = run_fast(core, args.fast_nodes, args.steps, args.warmup) if args.fast_nodes else None
