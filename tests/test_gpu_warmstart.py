@@ -23,3 +23,22 @@ def test_warm_start_beats_plain_fast_mode_at_equal_ticks(gpu_core):
     assert run["warm"]["gap"] < run["plain_same_ticks"]["gap"]
     assert r["predicted_energy_gap"] < 1.0
     net.close()
+
+
+def test_warm_start_reaches_plain_energy_in_a_third_of_the_gpu_time(gpu_core):
+    # The criterion of the learned warm start on a held-out 100k-node SBM
+    # graph: within 1% of the energy of 300 plain fast-mode ticks, in at most
+    # a third of their GPU time (the network's time included). Stress is
+    # reported beside it.
+    net = ws.Network(core=gpu_core)
+    net.load(ws.WEIGHTS)
+    n, src, tgt = sbm_batch(1, seed=5151, n_min=100000, n_max=100000)[0]
+    r = ws.compare(net, n, src, tgt, core=gpu_core, seed=7, short=(25, 50, 75))
+    hit = next((run for run in r["runs"] if abs(run["warm"]["relative_to_plain"]) <= 0.01), None)
+    for run in r["runs"]:
+        print("  %d ticks: warm energy %+.4f of plain, %.3f s (stress %.3f); plain 300 ticks %.3f s (stress %.3f)"
+              % (run["ticks"], run["warm"]["relative_to_plain"], run["warm"]["seconds"], run["warm"]["stress"],
+                 r["plain_seconds"], r["plain_stress"]), flush=True)
+    assert hit is not None
+    assert r["plain_seconds"] >= 3 * hit["warm"]["seconds"]
+    net.close()
