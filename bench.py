@@ -159,6 +159,8 @@ def run_batch(core, graphs, steps, warmup, seed=1):
     b = Batch(gs, core=core)
     n, m = b.n, b.m
     log("batch: %d graphs, %d nodes, %d links ready in %.1fs" % (graphs, n, m, time.time() - t0))
+    tuned = b.autotune(2)  # 29 ticks, untimed: launch shapes only, results do not depend on them
+    log("batch: autotuned %s" % tuned)
     b.tick(warmup)
     device_sync(core)
     b.stats()
@@ -210,6 +212,7 @@ def run_batch(core, graphs, steps, warmup, seed=1):
                                % (graphs, n, m), "graphs": graphs, "nodes": n, "links": m,
                    "graphs_below_2000_nodes": int(sum(1 for g in gs if g[0] < 2000))},
         "phases_ms_per_tick": {ph: v / steps for ph, v in st["ms"].items() if v > 0},
+        "autotuned_launch_shapes": tuned,
         "fmm_work_per_tick": {"m2l_pairs": st["m2l"] / steps, "p2p_point_pairs": st["p2p_pairs"] / steps,
                               "cells": st["fmm_cells"]},
         "rms_force_error_vs_fp64_direct_sum": {"sampled_graphs": len(pick), "nodes": sizes,

@@ -389,6 +389,59 @@ EXPORT int32_t batch_tick(uint32_t h, uint32_t ticks) {
   return B_OK;
 }
 
+// Launch-shape autotuner: coordinate descent over the FMM's result-neutral
+// launch shapes (dual-traversal grid, per-depth build grid, near-field item
+// blocks), each candidate timed with HIP events over `ticks` real ticks of
+// the batch. Every candidate computes the same bits, so the tuning ticks are
+// ordinary ticks of the simulation (one untimed tick, then `ticks` per
+// candidate). chosen[3] receives the shapes kept;
+// ms_out (optional, 16 floats) the per-tick time of each candidate in order.
+EXPORT int32_t batch_autotune(uint32_t h, uint32_t ticks, uint32_t* chosen, float* ms_out) {
+  Batch* b = bget(h);
+  if (!b) return B_BAD_HANDLE;
+  if (b->graphs == 0 || ticks == 0) return B_BAD_ARGUMENT;
+  static const uint32_t dual[] = {256, 512, 1024, 2048, 4096};
+  static const uint32_t level[] = {64, 128, 256, 512, 1024};
+  static const uint32_t extra[] = {64, 128, 256, 512};
+  struct Knob { uint32_t* field; const uint32_t* values; int count; };
+  Knob knobs[3] = {{&b->fmm.dual_blocks, dual, 5}, {&b->fmm.level_blocks, level, 5}, {&b->fmm.near_extra, extra, 4}};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return B_FAILED;
+  // one untimed tick first: the first tick also builds the link rows and
+  // grows the buffers, which would bias the first candidate
+  int32_t st = b->tick();
+  int slot = 0;
+  for (Knob& k : knobs) {
+    uint32_t best = *k.field;
+    float best_ms = -1;
+    for (int c = 0; c < k.count && st == B_OK; ++c) {
+      *k.field = k.values[c];
+      if (hipEventRecord(e0, b->stream) != hipSuccess) st = B_FAILED;
+      for (uint32_t t = 0; t < ticks && st == B_OK; ++t) st = b->tick();
+      float ms = 0;
+      if (st == B_OK && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = B_FAILED;
+      ms /= (float)ticks;
+      if (ms_out && slot < 16) ms_out[slot] = ms;
+      ++slot;
+      if (best_ms < 0 || ms < best_ms) {
+        best_ms = ms;
+        best = k.values[c];
+      }
+    }
+    *k.field = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (chosen) {
+    chosen[0] = b->fmm.dual_blocks;
+    chosen[1] = b->fmm.level_blocks;
+    chosen[2] = b->fmm.near_extra;
+  }
+  return st;
+}
+
 EXPORT void batch_set_timing(int on) { Batch::g_batch_timing = on != 0; }
 
 EXPORT int32_t batch_stats(uint32_t h, d3f_stats* out) {

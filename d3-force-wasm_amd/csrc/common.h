@@ -289,6 +289,9 @@ struct FmmBuffers {
   float* ps = nullptr;
   uint2 *front_a = nullptr, *front_b = nullptr;
   uint32_t long_cap = 0, slot_cap = 0, item_cap = 0;
+  // launch shapes (no result depends on them): the dual traversal's grid, the
+  // per-depth build grids, the blocks sharing near-field work items
+  uint32_t dual_blocks = 1024, level_blocks = 256, near_extra = 256;
   uint32_t* long_list = nullptr;           // targets whose segments the long-segment sort orders
   uint4* near_items = nullptr;             // (leaf, target batch, chunk or ALL, slot)
   uint4* near_combines = nullptr;          // (leaf, target batch, first slot, chunks)

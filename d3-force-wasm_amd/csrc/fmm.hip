@@ -40,10 +40,7 @@ namespace d3f {
 namespace {
 
 constexpr int EWF = 256;
-constexpr uint32_t GRID_BLOCKS = 1024;  // grid-stride kernels over device counts
-constexpr uint32_t LEVEL_BLOCKS = 256;  // per-depth build kernels (grid-stride)
 constexpr uint32_t SEG_MAX = 1024;      // sources of one target ordered in LDS
-constexpr uint32_t NEAR_EXTRA = 256;    // blocks that share the near-field work items
 constexpr uint32_t NEAR_CHUNK = 64;     // source leaves per near-field chunk
 constexpr uint32_t NEAR_ALL = 0xFFFFFFFFu;
 constexpr uint32_t LONG_IDS = 262144;   // cell ids per pass of the long-segment sort (32 KB of LDS bitmap)
@@ -1323,7 +1320,7 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
                        f.front_a, roots);
     uint2 *in = f.front_a, *out = f.front_b;
     for (uint32_t s = 0; s < steps; ++s) {
-      hipLaunchKernelGGL(k_dual_step, dim3(GRID_BLOCKS), dim3(EWF), 0, stream, f.cells, f.params, s, in, out,
+      hipLaunchKernelGGL(k_dual_step, dim3(f.dual_blocks), dim3(EWF), 0, stream, f.cells, f.params, s, in, out,
                          f.front_cap, f.m2l, f.m2l_cap, f.p2p, f.p2p_cap, mac);
       std::swap(in, out);
     }
@@ -1409,7 +1406,7 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   const float dmin2 = (float)a.dmin2, dmax2 = (float)a.dmax2;
   hipLaunchKernelGGL(k_near_items, dim3(ceil_div(nc, EWF)), dim3(EWF), 0, stream, f.cells, ncp, f.p2p_cnt, f.params,
                      f.near_items, f.near_combines, f.cursor, f.slot_cap);
-  const uint32_t extra = P.m_total / 64 < NEAR_EXTRA ? P.m_total / 64 + 1 : NEAR_EXTRA;
+  const uint32_t extra = P.m_total / 64 < f.near_extra ? P.m_total / 64 + 1 : f.near_extra;
   hipLaunchKernelGGL(k_near<Q>, dim3(extra + nc), dim3(64), 0, stream, f.cells, f.p2p_src, f.p2p_off, f.p2p_cnt,
                      f.params, (const float2*)f.lexp, f.px, f.py, f.ps, f.ux, f.uy, t.perm, a.vx, a.vy, f.graphs,
                      dmin2, dmax2, f.near_items, f.cursor, f.near_partial, extra);
@@ -1533,11 +1530,11 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
     hipLaunchKernelGGL(k_forest_roots, dim3(1), dim3(1024), 0, stream, f.graphs, a.graphs, f.params, f.cells,
                        f.cell_cap, t.meta);
     for (uint32_t d = 0; d < FMM_LEVELS; ++d) {
-      hipLaunchKernelGGL(k_fmm_split, dim3(LEVEL_BLOCKS), dim3(EWF), 0, stream, t.keys, f.cells, f.params, d, f.split,
+      hipLaunchKernelGGL(k_fmm_split, dim3(f.level_blocks), dim3(EWF), 0, stream, t.keys, f.cells, f.params, d, f.split,
                          f.bounds, f.cell_cap);
       HIPCHECK(scan_u32(f.split, f.split_off, n > 0 ? n : 1, &f.params->level_n[d], t.partials,
                         &f.params->level_n[d + 1], stream));
-      hipLaunchKernelGGL(k_fmm_children, dim3(LEVEL_BLOCKS), dim3(EWF), 0, stream, f.cells, f.params, d, f.split,
+      hipLaunchKernelGGL(k_fmm_children, dim3(f.level_blocks), dim3(EWF), 0, stream, f.cells, f.params, d, f.split,
                          f.split_off, f.bounds, f.cell_cap);
     }
     HIPCHECK(hipMemcpyAsync(f.params_host, f.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream));

@@ -29,6 +29,7 @@ _SIGNATURES = {
     "batch_get_scalar": (ctypes.c_double, [ctypes.c_uint32, ctypes.c_uint32]),
     "batch_tick": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32]),
     "batch_set_timing": (None, [ctypes.c_int]),
+    "batch_autotune": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 
@@ -127,6 +128,18 @@ class Batch:
     def tick(self, k=1):
         check(self.core.batch_tick(self.h, int(k)))
         return self
+
+    def autotune(self, ticks=2):
+        """Times `ticks` ticks per candidate launch shape (include/d3fgpu.h,
+        batch_autotune) and keeps the fastest; returns the shapes chosen
+        (dual-traversal blocks, build blocks, near-field item blocks) and the
+        per-tick milliseconds of the candidates. The ticks (one untimed, then
+        `ticks` per candidate) advance the batch."""
+        chosen = (ctypes.c_uint32 * 3)()
+        ms = (ctypes.c_float * 16)()
+        check(self.core.batch_autotune(self.h, ticks, chosen, ms))
+        return {"dual_blocks": chosen[0], "level_blocks": chosen[1], "near_extra": chosen[2],
+                "candidate_ms": [round(v, 3) for v in ms[:14]]}
 
     def state(self):
         """x, y, vx, vy of every node as a (4, n) float64 array (a copy)."""

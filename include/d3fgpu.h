@@ -166,6 +166,14 @@ int32_t batch_sync(uint32_t b, uint32_t direction);
 int32_t batch_set_scalar(uint32_t b, uint32_t which, double v);
 double batch_get_scalar(uint32_t b, uint32_t which);
 int32_t batch_tick(uint32_t b, uint32_t ticks);
+/* Autotunes the FMM's launch shapes (dual-traversal grid, per-depth build
+ * grid, blocks sharing near-field items; no result depends on them) by timing
+ * `ticks` real ticks per candidate (after one untimed tick), one shape at a
+ * time; keeps the fastest.
+ * chosen[3] receives the shapes kept; ms (optional, 16 floats) the per-tick
+ * time of each candidate. The tuning ticks advance the batch. No reference
+ * counterpart. */
+int32_t batch_autotune(uint32_t b, uint32_t ticks, uint32_t* chosen, float* ms);
 void batch_set_timing(int on);
 int32_t batch_stats(uint32_t b, d3f_stats* out);
 

@@ -111,3 +111,20 @@ def test_batch_rejects_links_across_graphs(gpu_core):
     with pytest.raises(Exception):
         b.sync_in()
     b.close()
+
+
+def test_batch_autotune_keeps_every_bit(gpu_core, graphs):
+    # The tuned launch shapes change no result: the tuning ticks and the ticks
+    # after them equal an untuned batch's, bit for bit.
+    a = Batch(graphs, core=gpu_core)
+    b = Batch(graphs, core=gpu_core)
+    info = a.autotune(1)
+    print("  autotuned", info, flush=True)
+    assert len([v for v in info["candidate_ms"] if v > 0]) == 14
+    b.tick(15)
+    assert same(a.state(), b.state())
+    a.tick(3)
+    b.tick(3)
+    assert same(a.state(), b.state())
+    a.close()
+    b.close()
