@@ -266,7 +266,8 @@ struct FmmParams {
   uint32_t long_segs;                      // target segments ordered by the long-segment sort
   unsigned long long pairs;                // point pairs of the P2P list
   uint32_t near_items, near_slots, near_combines;  // near-field work items, partial-sum slots, combines
-  uint32_t pad;
+  uint32_t abort;                          // a device-side size check failed: later kernels skip the tick
+  uint32_t ncg;                            // cells, or 0 when aborted (the count the per-cell kernels use)
 };
 
 struct FmmBuffers {
@@ -317,6 +318,12 @@ struct FmmArgs {
   const uint32_t* node_off;
   const uint32_t* node_graph;
   const uint64_t* seeds;
+  // Capturable form: no host synchronisation, launches sized by the buffers'
+  // capacities (reserved beforehand by fmm_reserve_capacity) and by counts
+  // read on the device; alpha read from alpha_dev. The caller reads
+  // params_host after the tick and redoes it eagerly when abort is set.
+  bool graph = false;
+  const double* alpha_dev = nullptr;
 };
 // Fast many-body (fmm.zig:247-354) over every graph of the forest at once.
 // With one graph, returns hipErrorNotSupported when the FMM declines (a
@@ -324,6 +331,9 @@ struct FmmArgs {
 // reference does; with more, declined graphs are counted in
 // params_host->declined and left unchanged.
 hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t stream);
+// Sizes every FMM buffer for the capturable form: cells for `cells` cells,
+// the lists, near-field items and expansions for what those cells allow.
+hipError_t fmm_reserve_capacity(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, uint32_t cells, hipStream_t stream);
 // out[2q], out[2q + 1] = FP64 direct-sum many-body velocity increment of node
 // sample[q] (device pointers).
 hipError_t direct_sum(const double* x, const double* y, const double* strength, uint32_t n, const uint32_t* sample,

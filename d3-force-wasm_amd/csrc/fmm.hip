@@ -351,10 +351,12 @@ __global__ __launch_bounds__(EWF) void k_fmm_children(FCell* __restrict__ cells,
 __global__ __launch_bounds__(EWF) void k_fmm_gather(const uint32_t* __restrict__ perm, const uint64_t* __restrict__ keys,
                                                     const double* __restrict__ xs, const double* __restrict__ ys,
                                                     const double* __restrict__ st, const FmmGraph* __restrict__ gp,
-                                                    const FmmParams* __restrict__ fp, double alpha, uint32_t n,
+                                                    const FmmParams* __restrict__ fp, double alpha,
+                                                    const double* __restrict__ alpha_dev, uint32_t n,
                                                     double* __restrict__ px, double* __restrict__ py,
                                                     double* __restrict__ ux, double* __restrict__ uy,
                                                     float* __restrict__ ps) {
+  if (alpha_dev) alpha = *alpha_dev;
   const uint32_t j = blockIdx.x * EWF + threadIdx.x;
   if (j >= n || j >= fp->m_total) return;
   const FmmGraph& G = gp[keys[j] >> 32];
@@ -374,13 +376,9 @@ __global__ __launch_bounds__(EWF) void k_fmm_gather(const uint32_t* __restrict__
 // Geometry (fmm.zig:721-754) and the radius-scaled multipole of the cells
 // [lo, lo + cnt) of one depth: P2M for a leaf, M2M from the children.
 template <int Q>
-__global__ __launch_bounds__(EWF) void k_fmm_up(FCell* __restrict__ cells, double* __restrict__ bbox,
-                                                const double* __restrict__ ux, const double* __restrict__ uy,
-                                                const float* __restrict__ ps, float2* __restrict__ mexp, uint32_t lo,
-                                                uint32_t cnt) {
-  const uint32_t t = blockIdx.x * EWF + threadIdx.x;
-  if (t >= cnt) return;
-  const uint32_t c = lo + t;
+__device__ __forceinline__ void fmm_up_cell(uint32_t c, FCell* __restrict__ cells, double* __restrict__ bbox,
+                                            const double* __restrict__ ux, const double* __restrict__ uy,
+                                            const float* __restrict__ ps, float2* __restrict__ mexp) {
   const FCell cell = cells[c];
   const double inf = __builtin_huge_val();
   double x0 = inf, y0 = inf, x1 = -inf, y1 = -inf;
@@ -457,6 +455,18 @@ __global__ __launch_bounds__(EWF) void k_fmm_up(FCell* __restrict__ cells, doubl
   for (int k = 0; k < Q; ++k) mexp[(size_t)c * Q + k] = m[k];
 }
 
+// P2M / M2M of the cells of depth d (counts read on the device).
+template <int Q>
+__global__ __launch_bounds__(EWF) void k_fmm_up(FCell* __restrict__ cells, double* __restrict__ bbox,
+                                                const double* __restrict__ ux, const double* __restrict__ uy,
+                                                const float* __restrict__ ps, float2* __restrict__ mexp,
+                                                const FmmParams* __restrict__ fp, uint32_t d) {
+  if (fp->abort) return;
+  const uint32_t lo = fp->level_off[d], cnt = fp->level_n[d];
+  for (uint32_t t = blockIdx.x * EWF + threadIdx.x; t < cnt; t += gridDim.x * EWF)
+    fmm_up_cell<Q>(lo + t, cells, bbox, ux, uy, ps, mexp);
+}
+
 // ---------------------------------------------------------------- dual tree
 
 struct MacArgs {
@@ -502,6 +512,7 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
   __shared__ uint32_t wsum[3][EWF / 64];
   __shared__ uint32_t base_sh[3];
   __shared__ unsigned long long pairs_sh[EWF / 64];
+  if (fp->abort) return;
   uint32_t cnt = fp->front[step];
   if (cnt > front_cap) cnt = front_cap;
   for (uint32_t tile = blockIdx.x * STEP_TILE; tile < cnt; tile += gridDim.x * STEP_TILE) {
@@ -630,7 +641,9 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
 
 // The first frontier: every root paired with itself.
 __global__ __launch_bounds__(EWF) void k_fmm_lists_reset(FmmParams* __restrict__ fp, uint2* __restrict__ front,
-                                                         uint32_t roots) {
+                                                         uint32_t front_cap) {
+  if (fp->abort) return;
+  const uint32_t roots = fp->roots <= front_cap ? fp->roots : 0;
   const uint32_t t = blockIdx.x * EWF + threadIdx.x;
   if (t < roots) front[t] = make_uint2(t, t);
   if (blockIdx.x == 0) {
@@ -654,20 +667,22 @@ __global__ __launch_bounds__(EWF) void k_fmm_lists_reset(FmmParams* __restrict__
 // (sort_segment), so every sum runs in one fixed order.
 __global__ __launch_bounds__(EWF) void k_target_count(const uint64_t* __restrict__ list,
                                                       const uint32_t* __restrict__ count, uint32_t nbits,
-                                                      uint32_t* __restrict__ cnt) {
-  const uint32_t k = blockIdx.x * EWF + threadIdx.x;
-  if (k < *count) atomicAdd(&cnt[list[k] >> nbits], 1u);
+                                                      uint32_t* __restrict__ cnt, const FmmParams* __restrict__ fp) {
+  if (fp->abort) return;
+  for (uint32_t k = blockIdx.x * EWF + threadIdx.x; k < *count; k += gridDim.x * EWF)
+    atomicAdd(&cnt[list[k] >> nbits], 1u);
 }
 
 __global__ __launch_bounds__(EWF) void k_target_scatter(const uint64_t* __restrict__ list,
                                                         const uint32_t* __restrict__ count, uint32_t nbits,
                                                         const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
-                                                        uint32_t* __restrict__ src) {
-  const uint32_t k = blockIdx.x * EWF + threadIdx.x;
-  if (k >= *count) return;
-  const uint64_t key = list[k];
-  const uint32_t a = (uint32_t)(key >> nbits);
-  src[off[a] + atomicAdd(&cursor[a], 1u)] = (uint32_t)(key & (((uint64_t)1 << nbits) - 1));
+                                                        uint32_t* __restrict__ src, const FmmParams* __restrict__ fp) {
+  if (fp->abort) return;
+  for (uint32_t k = blockIdx.x * EWF + threadIdx.x; k < *count; k += gridDim.x * EWF) {
+    const uint64_t key = list[k];
+    const uint32_t a = (uint32_t)(key >> nbits);
+    src[off[a] + atomicAdd(&cursor[a], 1u)] = (uint32_t)(key & (((uint64_t)1 << nbits) - 1));
+  }
 }
 
 // One wave loads the sources of window w (SEG_MAX at a time) of a target
@@ -721,13 +736,13 @@ __device__ inline uint32_t load_window(const uint32_t* __restrict__ src, uint32_
 // post-scaling uses the lane's own pair. Each lane sums its pairs in source
 // order; the 16 lanes of a row group are then added by a fixed butterfly.
 template <int Q>
-__global__ __launch_bounds__(64) void k_m2l(const FCell* __restrict__ cells, const uint32_t* __restrict__ srcs,
-                                            const uint32_t* __restrict__ offs, const uint32_t* __restrict__ cnts,
-                                            const float2* __restrict__ mexp, float2* __restrict__ lexp) {
+__device__ __forceinline__ void m2l_target(uint32_t A, const FCell* __restrict__ cells,
+                                           const uint32_t* __restrict__ srcs, const uint32_t* __restrict__ offs,
+                                           const uint32_t* __restrict__ cnts, const float2* __restrict__ mexp,
+                                           float2* __restrict__ lexp) {
   constexpr int RT = (Q + 15) / 16;  // row tiles of local coefficients
   constexpr int KS = (Q + 3) / 4;    // k steps of multipole coefficients
   __shared__ uint32_t seg_in[SEG_MAX], seg[SEG_MAX];
-  const uint32_t A = blockIdx.x;
   const int lane = threadIdx.x;
   const int col = lane & 15, kq = lane >> 4;
   const uint32_t K = cnts[A], off = offs[A];
@@ -824,15 +839,21 @@ __global__ __launch_bounds__(64) void k_m2l(const FCell* __restrict__ cells, con
   }
 }
 
+template <int Q>
+__global__ __launch_bounds__(64) void k_m2l(const FCell* __restrict__ cells, const uint32_t* __restrict__ srcs,
+                                            const uint32_t* __restrict__ offs, const uint32_t* __restrict__ cnts,
+                                            const float2* __restrict__ mexp, float2* __restrict__ lexp,
+                                            const FmmParams* __restrict__ fp) {
+  if (fp->abort) return;
+  for (uint32_t A = blockIdx.x; A < fp->ncg; A += gridDim.x) m2l_target<Q>(A, cells, srcs, offs, cnts, mexp, lexp);
+}
+
 // ---------------------------------------------------------------- downward
 
 // L2L (fmm.zig:823-849) from each cell of one depth into its children.
 template <int Q>
-__global__ __launch_bounds__(EWF) void k_fmm_down(const FCell* __restrict__ cells, float2* __restrict__ lexp,
-                                                  uint32_t lo, uint32_t cnt) {
-  const uint32_t t = blockIdx.x * EWF + threadIdx.x;
-  if (t >= cnt) return;
-  const uint32_t c = lo + t;
+__device__ __forceinline__ void fmm_down_cell(uint32_t c, const FCell* __restrict__ cells,
+                                              float2* __restrict__ lexp) {
   const FCell cell = cells[c];
   if (cell.nchild == 0) return;
   float2 lp[Q];
@@ -861,6 +882,14 @@ __global__ __launch_bounds__(EWF) void k_fmm_down(const FCell* __restrict__ cell
       tm *= tr;
     }
   }
+}
+
+template <int Q>
+__global__ __launch_bounds__(EWF) void k_fmm_down(const FCell* __restrict__ cells, float2* __restrict__ lexp,
+                                                  const FmmParams* __restrict__ fp, uint32_t d) {
+  if (fp->abort) return;
+  const uint32_t lo = fp->level_off[d], cnt = fp->level_n[d];
+  for (uint32_t t = blockIdx.x * EWF + threadIdx.x; t < cnt; t += gridDim.x * EWF) fmm_down_cell<Q>(lo + t, cells, lexp);
 }
 
 // ---------------------------------------------------------------- near field + L2P
@@ -1005,10 +1034,11 @@ __global__ __launch_bounds__(EWF) void k_long_list(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ m2l_cnt,
                                                    const uint32_t* __restrict__ p2p_cnt, FmmParams* __restrict__ fp,
                                                    uint32_t* __restrict__ list) {
-  const uint32_t c = blockIdx.x * EWF + threadIdx.x;
-  if (c >= *ncp) return;
-  if (p2p_cnt[c] > NEAR_CHUNK) list[atomicAdd(&fp->long_segs, 1u)] = c | 0x80000000u;
-  if (m2l_cnt[c] > SEG_MAX) list[atomicAdd(&fp->long_segs, 1u)] = c;
+  if (fp->abort) return;
+  for (uint32_t c = blockIdx.x * EWF + threadIdx.x; c < *ncp; c += gridDim.x * EWF) {
+    if (p2p_cnt[c] > NEAR_CHUNK) list[atomicAdd(&fp->long_segs, 1u)] = c | 0x80000000u;
+    if (m2l_cnt[c] > SEG_MAX) list[atomicAdd(&fp->long_segs, 1u)] = c;
+  }
 }
 
 // Orders each long segment by source in place. A target's sources are
@@ -1028,6 +1058,7 @@ __global__ __launch_bounds__(256) void k_sort_long(const uint32_t* __restrict__ 
   __shared__ uint32_t bits[WORDS];
   __shared__ uint32_t scan[256];
   const uint32_t tid = threadIdx.x;
+  if (fp->abort) return;
   const uint32_t nseg = fp->long_segs, nc = *ncp;
   for (uint32_t k = blockIdx.x; k < nseg; k += gridDim.x) {
     const uint32_t e = list[k];
@@ -1078,12 +1109,10 @@ __global__ __launch_bounds__(256) void k_sort_long(const uint32_t* __restrict__ 
 // every chunk, or (segments of several chunks, while partial-sum slots last)
 // one item per chunk plus a combine. split0[leaf] says whether the leaf's
 // first batch was split (its own block then skips it).
-__global__ __launch_bounds__(EWF) void k_near_items(const FCell* __restrict__ cells, const uint32_t* __restrict__ ncp,
-                                                    const uint32_t* __restrict__ cnts, FmmParams* __restrict__ fp,
-                                                    uint4* __restrict__ items, uint4* __restrict__ combines,
-                                                    uint32_t* __restrict__ split0, uint32_t slot_cap) {
-  const uint32_t c = blockIdx.x * EWF + threadIdx.x;
-  if (c >= *ncp) return;
+__device__ inline void near_items_cell(uint32_t c, const FCell* __restrict__ cells, const uint32_t* __restrict__ cnts,
+                                       FmmParams* __restrict__ fp, uint4* __restrict__ items,
+                                       uint4* __restrict__ combines, uint32_t* __restrict__ split0,
+                                       uint32_t slot_cap) {
   const FCell cell = cells[c];
   if (cell.nchild != 0) return;
   const uint32_t K = cnts[c];
@@ -1108,9 +1137,20 @@ __global__ __launch_bounds__(EWF) void k_near_items(const FCell* __restrict__ ce
   split0[c] = first;
 }
 
+__global__ __launch_bounds__(EWF) void k_near_items(const FCell* __restrict__ cells, const uint32_t* __restrict__ ncp,
+                                                    const uint32_t* __restrict__ cnts, FmmParams* __restrict__ fp,
+                                                    uint4* __restrict__ items, uint4* __restrict__ combines,
+                                                    uint32_t* __restrict__ split0, uint32_t slot_cap) {
+  if (fp->abort) return;
+  for (uint32_t c = blockIdx.x * EWF + threadIdx.x; c < *ncp; c += gridDim.x * EWF)
+    near_items_cell(c, cells, cnts, fp, items, combines, split0, slot_cap);
+}
+
+
 // Blocks [0, extra) share the items (launched first, so the longest work
-// starts first); block extra + c runs the first 64 targets of leaf c unless
-// that batch was split into chunk items.
+// starts first); the other blocks run the first 64 targets of each leaf
+// (blocks extra + c, extra + c + (gridDim - extra), ...) unless that batch
+// was split into chunk items.
 template <int Q>
 __global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, const uint32_t* __restrict__ srcs,
                                              const uint32_t* __restrict__ offs, const uint32_t* __restrict__ cnts,
@@ -1122,6 +1162,7 @@ __global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, co
                                              const FmmGraph* __restrict__ gp, float dmin2, float dmax2,
                                              const uint4* __restrict__ items, const uint32_t* __restrict__ split0,
                                              float2* __restrict__ partial, uint32_t extra) {
+  if (fp->abort) return;
   if (blockIdx.x < extra) {
     const uint32_t count = fp->near_items;
     for (uint32_t k = blockIdx.x; k < count; k += extra) {
@@ -1131,11 +1172,13 @@ __global__ __launch_bounds__(64) void k_near(const FCell* __restrict__ cells, co
     }
     return;
   }
-  const uint32_t A = blockIdx.x - extra;
-  const FCell ca = cells[A];
-  if (ca.nchild != 0 || ca.count == 0 || split0[A]) return;
-  near_batch<Q>(A, 0, NEAR_ALL, nullptr, cells, srcs, offs, cnts, lexp, px, py, ps, ux, uy, perm, vxs, vys, gp, dmin2,
-                dmax2);
+  const uint32_t nc = fp->ncg;
+  for (uint32_t A = blockIdx.x - extra; A < nc; A += gridDim.x - extra) {
+    const FCell ca = cells[A];
+    if (ca.nchild != 0 || ca.count == 0 || split0[A]) continue;
+    near_batch<Q>(A, 0, NEAR_ALL, nullptr, cells, srcs, offs, cnts, lexp, px, py, ps, ux, uy, perm, vxs, vys, gp, dmin2,
+                  dmax2);
+  }
 }
 
 // Chunk sums of a split (leaf, batch) added in chunk order, then L2P and the
@@ -1148,6 +1191,7 @@ __global__ __launch_bounds__(64) void k_near_combine(const FCell* __restrict__ c
                                                      const double* __restrict__ uy, const uint32_t* __restrict__ perm,
                                                      double* __restrict__ vxs, double* __restrict__ vys,
                                                      const FmmGraph* __restrict__ gp) {
+  if (fp->abort) return;
   const uint32_t count = fp->near_combines;
   const uint32_t t = threadIdx.x;
   for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
@@ -1163,6 +1207,29 @@ __global__ __launch_bounds__(64) void k_near_combine(const FCell* __restrict__ c
     }
     near_finish<Q>(ca, it.x, ca.first + it.y + t, total, lexp, ux, uy, perm, vxs, vys, gp[ca.graph].inv_w);
   }
+}
+
+// Device-side checks of a tick whose sizes the host does not read (a captured
+// graph): after the build (stage 0) the cells must fit every per-cell buffer;
+// after the traversal (stage 1) the lists must fit and the rounds must have
+// drained the recursion. A failed check sets abort, which every later FMM
+// kernel tests first, and zeroes the guarded cell count ncg; the host reads
+// the flags after the tick and redoes it eagerly with larger buffers.
+__global__ void k_fmm_guard(FmmParams* __restrict__ fp, uint32_t stage, uint32_t cell_room, uint32_t front_cap,
+                            uint32_t last_step) {
+  if (threadIdx.x != 0) return;
+  const uint32_t nc = fp->level_off[FMM_LEVELS + 1];
+  if (stage == 0) {
+    fp->abort = (fp->cell_overflow || nc > cell_room || fp->roots > front_cap) ? 1u : 0u;
+  } else if (fp->list_overflow || fp->front[last_step] != 0) {
+    fp->abort |= 2u;
+  }
+  fp->ncg = fp->abort ? 0u : nc;
+}
+
+__global__ __launch_bounds__(256) void k_fmm_zero(uint32_t* __restrict__ p, const uint32_t* __restrict__ count) {
+  const uint32_t n = *count;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0;
 }
 
 // ---------------------------------------------------------------- direct sum
@@ -1289,41 +1356,54 @@ hipError_t reserve_cells(FmmBuffers& f, uint32_t nc, uint32_t q) {
   return hipSuccess;
 }
 
+// Grids of the capturable form (grid-stride kernels over device counts).
+constexpr uint32_t CAP_CELL_BLOCKS = 1024;  // per-cell and per-pair elementwise kernels
+constexpr uint32_t CAP_TARGET_BLOCKS = 8192;  // one block per target, looping
+
 template <int Q>
 hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t stream) {
   const FmmParams& P = *f.params_host;
-  const uint32_t nc = P.level_off[FMM_LEVELS + 1];
-  uint32_t maxd = 0;
-  for (uint32_t d = 0; d <= FMM_LEVELS; ++d)
-    if (P.level_n[d]) maxd = d;
-  if (nc > f.cell_cap || nc > f.cellarr_cap || (size_t)nc * Q > f.exp_cap) return hipErrorInvalidValue;
-  // upward
-  for (int d = (int)maxd; d >= 0; --d) {
-    const uint32_t cnt = P.level_n[d];
-    hipLaunchKernelGGL(k_fmm_up<Q>, dim3(ceil_div(cnt, EWF)), dim3(EWF), 0, stream, f.cells, f.bbox, f.ux, f.uy, f.ps,
-                       (float2*)f.mexp, P.level_off[d], cnt);
+  const bool g = a.graph;
+  // The eager form sizes its launches from the counts read back after the
+  // build; the capturable form from the capacities, every kernel reading its
+  // count on the device (both run the same kernels on the same data).
+  const uint32_t nc = g ? f.cell_cap : P.level_off[FMM_LEVELS + 1];
+  uint32_t maxd = FMM_LEVELS;
+  if (!g) {
+    maxd = 0;
+    for (uint32_t d = 0; d <= FMM_LEVELS; ++d)
+      if (P.level_n[d]) maxd = d;
+    if (nc > f.cell_cap || nc > f.cellarr_cap || (size_t)nc * Q > f.exp_cap) return hipErrorInvalidValue;
   }
-  // interaction lists
+  auto grid = [&](uint32_t count, uint32_t per, uint32_t cap_blocks) -> uint32_t {
+    if (g) return cap_blocks;
+    const uint32_t b = ceil_div(count, per);
+    return b > 0 ? b : 1;
+  };
+  // upward
+  for (int d = (int)maxd; d >= 0; --d)
+    hipLaunchKernelGGL(k_fmm_up<Q>, dim3(grid(g ? 0 : P.level_n[d], EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream,
+                       f.cells, f.bbox, f.ux, f.uy, f.ps, (float2*)f.mexp, f.params, (uint32_t)d);
+  // interaction lists (pair keys: target << nbits | source)
   uint32_t nbits = 1;
-  while (((uint64_t)1 << nbits) < nc) ++nbits;
+  while (((uint64_t)1 << nbits) < f.cell_cap) ++nbits;
   MacArgs mac;
   double kappa = a.opening;
   if (!(kappa > 1)) kappa = 1.4;  // fmm.zig:262-263
   mac.mac_far = 2 * kappa - 1;
   mac.nbits = nbits;
   mac.graphs = f.graphs;
-  const uint32_t steps = 2 * maxd + 1;
-  const uint32_t roots = P.roots;
-  if (roots > f.front_cap) return hipErrorInvalidValue;
+  const uint32_t steps = g ? FMM_MAX_STEPS : 2 * maxd + 1;
+  const uint32_t root_blocks = ceil_div(a.graphs > 0 ? a.graphs : 1, EWF);  // one root per graph at most
   for (;;) {
-    hipLaunchKernelGGL(k_fmm_lists_reset, dim3(ceil_div(roots > 0 ? roots : 1, EWF)), dim3(EWF), 0, stream, f.params,
-                       f.front_a, roots);
+    hipLaunchKernelGGL(k_fmm_lists_reset, dim3(root_blocks), dim3(EWF), 0, stream, f.params, f.front_a, f.front_cap);
     uint2 *in = f.front_a, *out = f.front_b;
     for (uint32_t s = 0; s < steps; ++s) {
       hipLaunchKernelGGL(k_dual_step, dim3(f.dual_blocks), dim3(EWF), 0, stream, f.cells, f.params, s, in, out,
                          f.front_cap, f.m2l, f.m2l_cap, f.p2p, f.p2p_cap, mac);
       std::swap(in, out);
     }
+    if (g) break;
     HIPCHECK(hipMemcpyAsync(f.params_host, f.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     if (P.front[steps] != 0) return hipErrorUnknown;  // the rounds must drain the recursion
@@ -1348,68 +1428,70 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
       }
     }
   }
-  // long-segment list, near-field items and partial-sum slots for these lists
-  const uint32_t long_need = P.p2p / NEAR_CHUNK + P.m2l / SEG_MAX + 2;
-  if (long_need > f.long_cap) {
-    const uint32_t c = long_need + long_need / 4;
-    HIPCHECK(grow(f.long_list, c));
-    f.long_cap = c;
+  hipLaunchKernelGGL(k_fmm_guard, dim3(1), dim3(64), 0, stream, f.params, 1u, 0u, 0u, steps);
+  if (!g) {
+    // long-segment list, near-field items and partial-sum slots for these lists
+    const uint32_t long_need = P.p2p / NEAR_CHUNK + P.m2l / SEG_MAX + 2;
+    if (long_need > f.long_cap) {
+      const uint32_t c = long_need + long_need / 4;
+      HIPCHECK(grow(f.long_list, c));
+      f.long_cap = c;
+    }
+    const uint32_t slot_need = 2 * (P.p2p / NEAR_CHUNK) + 1024;
+    if (slot_need > f.slot_cap) {
+      const uint32_t c = slot_need + slot_need / 4;
+      HIPCHECK(grow(f.near_partial, (size_t)c * 64));
+      HIPCHECK(grow(f.near_combines, c / 2 + 1));
+      f.slot_cap = c;
+    }
+    const uint32_t item_need = f.slot_cap + P.m_total / 64 + 1;
+    if (item_need > f.item_cap) {
+      HIPCHECK(grow(f.near_items, item_need));
+      f.item_cap = item_need;
+    }
   }
-  const uint32_t slot_need = 2 * (P.p2p / NEAR_CHUNK) + 1024;
-  if (slot_need > f.slot_cap) {
-    const uint32_t c = slot_need + slot_need / 4;
-    HIPCHECK(grow(f.near_partial, (size_t)c * 64));
-    HIPCHECK(grow(f.near_combines, c / 2 + 1));
-    f.slot_cap = c;
+  // lists grouped by target (counters cleared by kernels: no memset node)
+  const uint32_t* ncp = &f.params->ncg;
+  const uint32_t scan_max = g ? f.cellarr_cap : nc;
+  hipLaunchKernelGGL(k_fmm_zero, dim3(grid(nc, 256, CAP_CELL_BLOCKS)), dim3(256), 0, stream, f.m2l_cnt, ncp);
+  hipLaunchKernelGGL(k_fmm_zero, dim3(grid(nc, 256, CAP_CELL_BLOCKS)), dim3(256), 0, stream, f.p2p_cnt, ncp);
+  if (g || P.m2l)
+    hipLaunchKernelGGL(k_target_count, dim3(grid(P.m2l, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.m2l,
+                       &f.params->m2l, nbits, f.m2l_cnt, f.params);
+  if (g || P.p2p)
+    hipLaunchKernelGGL(k_target_count, dim3(grid(P.p2p, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.p2p,
+                       &f.params->p2p, nbits, f.p2p_cnt, f.params);
+  HIPCHECK(scan_u32(f.m2l_cnt, f.m2l_off, scan_max, ncp, f.scan_partials, nullptr, stream));
+  HIPCHECK(scan_u32(f.p2p_cnt, f.p2p_off, scan_max, ncp, f.scan_partials, nullptr, stream));
+  if (g || P.m2l) {
+    hipLaunchKernelGGL(k_fmm_zero, dim3(grid(nc, 256, CAP_CELL_BLOCKS)), dim3(256), 0, stream, f.cursor, ncp);
+    hipLaunchKernelGGL(k_target_scatter, dim3(grid(P.m2l, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.m2l,
+                       &f.params->m2l, nbits, f.m2l_off, f.cursor, f.m2l_src, f.params);
   }
-  const uint32_t item_need = f.slot_cap + P.m_total / 64 + 1;
-  if (item_need > f.item_cap) {
-    HIPCHECK(grow(f.near_items, item_need));
-    f.item_cap = item_need;
-  }
-  // lists grouped by target
-  const uint32_t* ncp = &f.params->level_off[FMM_LEVELS + 1];
-  HIPCHECK(hipMemsetAsync(f.m2l_cnt, 0, (size_t)nc * sizeof(uint32_t), stream));
-  HIPCHECK(hipMemsetAsync(f.p2p_cnt, 0, (size_t)nc * sizeof(uint32_t), stream));
-  if (P.m2l)
-    hipLaunchKernelGGL(k_target_count, dim3(ceil_div(P.m2l, EWF)), dim3(EWF), 0, stream, f.m2l, &f.params->m2l, nbits,
-                       f.m2l_cnt);
-  if (P.p2p)
-    hipLaunchKernelGGL(k_target_count, dim3(ceil_div(P.p2p, EWF)), dim3(EWF), 0, stream, f.p2p, &f.params->p2p, nbits,
-                       f.p2p_cnt);
-  HIPCHECK(scan_u32(f.m2l_cnt, f.m2l_off, nc, ncp, f.scan_partials, nullptr, stream));
-  HIPCHECK(scan_u32(f.p2p_cnt, f.p2p_off, nc, ncp, f.scan_partials, nullptr, stream));
-  if (P.m2l) {
-    HIPCHECK(hipMemsetAsync(f.cursor, 0, (size_t)nc * sizeof(uint32_t), stream));
-    hipLaunchKernelGGL(k_target_scatter, dim3(ceil_div(P.m2l, EWF)), dim3(EWF), 0, stream, f.m2l, &f.params->m2l,
-                       nbits, f.m2l_off, f.cursor, f.m2l_src);
-  }
-  if (P.p2p) {
-    HIPCHECK(hipMemsetAsync(f.cursor, 0, (size_t)nc * sizeof(uint32_t), stream));
-    hipLaunchKernelGGL(k_target_scatter, dim3(ceil_div(P.p2p, EWF)), dim3(EWF), 0, stream, f.p2p, &f.params->p2p,
-                       nbits, f.p2p_off, f.cursor, f.p2p_src);
+  if (g || P.p2p) {
+    hipLaunchKernelGGL(k_fmm_zero, dim3(grid(nc, 256, CAP_CELL_BLOCKS)), dim3(256), 0, stream, f.cursor, ncp);
+    hipLaunchKernelGGL(k_target_scatter, dim3(grid(P.p2p, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.p2p,
+                       &f.params->p2p, nbits, f.p2p_off, f.cursor, f.p2p_src, f.params);
   }
   // segments too long for the in-LDS order, ordered in place
-  hipLaunchKernelGGL(k_long_list, dim3(ceil_div(nc, EWF)), dim3(EWF), 0, stream, ncp, f.m2l_cnt, f.p2p_cnt, f.params,
-                     f.long_list);
+  hipLaunchKernelGGL(k_long_list, dim3(grid(nc, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, ncp, f.m2l_cnt,
+                     f.p2p_cnt, f.params, f.long_list);
   hipLaunchKernelGGL(k_sort_long, dim3(LONG_BLOCKS), dim3(256), 0, stream, f.long_list, f.params, ncp, f.m2l_src,
                      f.m2l_off, f.m2l_cnt, (uint32_t*)f.m2l, f.p2p_src, f.p2p_off, f.p2p_cnt, (uint32_t*)f.p2p);
   // far field
-  hipLaunchKernelGGL(k_m2l<Q>, dim3(nc), dim3(64), 0, stream, f.cells, f.m2l_src, f.m2l_off, f.m2l_cnt,
-                     (const float2*)f.mexp, (float2*)f.lexp);
-  for (uint32_t d = 0; d < maxd; ++d) {
-    const uint32_t cnt = P.level_n[d];
-    hipLaunchKernelGGL(k_fmm_down<Q>, dim3(ceil_div(cnt, EWF)), dim3(EWF), 0, stream, f.cells, (float2*)f.lexp,
-                       P.level_off[d], cnt);
-  }
+  hipLaunchKernelGGL(k_m2l<Q>, dim3(grid(nc, 1, CAP_TARGET_BLOCKS)), dim3(64), 0, stream, f.cells, f.m2l_src,
+                     f.m2l_off, f.m2l_cnt, (const float2*)f.mexp, (float2*)f.lexp, f.params);
+  for (uint32_t d = 0; d < maxd; ++d)
+    hipLaunchKernelGGL(k_fmm_down<Q>, dim3(grid(g ? 0 : P.level_n[d], EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream,
+                       f.cells, (float2*)f.lexp, f.params, d);
   // near field and evaluation
   const float dmin2 = (float)a.dmin2, dmax2 = (float)a.dmax2;
-  hipLaunchKernelGGL(k_near_items, dim3(ceil_div(nc, EWF)), dim3(EWF), 0, stream, f.cells, ncp, f.p2p_cnt, f.params,
-                     f.near_items, f.near_combines, f.cursor, f.slot_cap);
-  const uint32_t extra = P.m_total / 64 < f.near_extra ? P.m_total / 64 + 1 : f.near_extra;
-  hipLaunchKernelGGL(k_near<Q>, dim3(extra + nc), dim3(64), 0, stream, f.cells, f.p2p_src, f.p2p_off, f.p2p_cnt,
-                     f.params, (const float2*)f.lexp, f.px, f.py, f.ps, f.ux, f.uy, t.perm, a.vx, a.vy, f.graphs,
-                     dmin2, dmax2, f.near_items, f.cursor, f.near_partial, extra);
+  hipLaunchKernelGGL(k_near_items, dim3(grid(nc, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.cells, ncp,
+                     f.p2p_cnt, f.params, f.near_items, f.near_combines, f.cursor, f.slot_cap);
+  const uint32_t extra = g ? f.near_extra : (P.m_total / 64 < f.near_extra ? P.m_total / 64 + 1 : f.near_extra);
+  hipLaunchKernelGGL(k_near<Q>, dim3(extra + grid(nc, 1, CAP_TARGET_BLOCKS)), dim3(64), 0, stream, f.cells,
+                     f.p2p_src, f.p2p_off, f.p2p_cnt, f.params, (const float2*)f.lexp, f.px, f.py, f.ps, f.ux, f.uy,
+                     t.perm, a.vx, a.vy, f.graphs, dmin2, dmax2, f.near_items, f.cursor, f.near_partial, extra);
   hipLaunchKernelGGL(k_near_combine<Q>, dim3(extra), dim3(64), 0, stream, f.cells, f.params, f.near_combines,
                      f.near_partial, (const float2*)f.lexp, f.ux, f.uy, t.perm, a.vx, a.vy, f.graphs);
   return hipGetLastError();
@@ -1537,6 +1619,7 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
       hipLaunchKernelGGL(k_fmm_children, dim3(f.level_blocks), dim3(EWF), 0, stream, f.cells, f.params, d, f.split,
                          f.split_off, f.bounds, f.cell_cap);
     }
+    if (a.graph) break;  // sizes checked on the device (k_fmm_guard)
     HIPCHECK(hipMemcpyAsync(f.params_host, f.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     if (!f.params_host->cell_overflow) break;
@@ -1545,14 +1628,49 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
     f.cell_cap = cap;
     hipLaunchKernelGGL(k_forest_reset_overflow, dim3(1), dim3(64), 0, stream, f.params);
   }
-  const FmmParams& P = *f.params_host;
-  // fmm.zig:255-257: one graph with a non-finite extent runs the exact kernel.
-  if (a.graphs == 1 && P.declined) return hipErrorNotSupported;
-  if (P.roots == 0) return hipSuccess;  // no point in any tree (fmm.zig:251-252)
-  HIPCHECK(reserve_cells(f, P.level_off[FMM_LEVELS + 1], p + 1));
+  if (!a.graph) {
+    const FmmParams& P = *f.params_host;
+    // fmm.zig:255-257: one graph with a non-finite extent runs the exact kernel.
+    if (a.graphs == 1 && P.declined) return hipErrorNotSupported;
+    if (P.roots == 0) return hipSuccess;  // no point in any tree (fmm.zig:251-252)
+    HIPCHECK(reserve_cells(f, P.level_off[FMM_LEVELS + 1], p + 1));
+  }
+  const uint32_t room = std::min<uint32_t>(std::min(f.cell_cap, f.cellarr_cap), (uint32_t)(f.exp_cap / (p + 1)));
+  hipLaunchKernelGGL(k_fmm_guard, dim3(1), dim3(64), 0, stream, f.params, 0u, room, f.front_cap, 0u);
   hipLaunchKernelGGL(k_fmm_gather, dim3(nb), dim3(EWF), 0, stream, t.perm, t.keys, a.x, a.y, a.strength, f.graphs,
-                     f.params, a.alpha, n, f.px, f.py, f.ux, f.uy, f.ps);
+                     f.params, a.alpha, a.alpha_dev, n, f.px, f.py, f.ux, f.uy, f.ps);
   return Orders::fns[p - 1](f, t, a, stream);
+}
+
+hipError_t fmm_reserve_capacity(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, uint32_t cells, hipStream_t stream) {
+  uint32_t p = a.order;
+  if (p < 1 || p > FMM_MAX_ORDER) p = 8;
+  HIPCHECK(upload_binomials());
+  HIPCHECK(tree_reserve(t, a.n > 0 ? a.n : 1));
+  HIPCHECK(reserve_points(f, a.n));
+  HIPCHECK(reserve_forest(f, a, stream));
+  if (f.cell_cap < cells) {
+    HIPCHECK(grow(f.cells, cells));
+    f.cell_cap = cells;
+  }
+  HIPCHECK(reserve_cells(f, f.cell_cap, p + 1));
+  const uint32_t long_need = f.p2p_cap / NEAR_CHUNK + f.m2l_cap / SEG_MAX + 2;
+  if (long_need > f.long_cap) {
+    HIPCHECK(grow(f.long_list, long_need));
+    f.long_cap = long_need;
+  }
+  const uint32_t slot_need = 2 * (f.p2p_cap / NEAR_CHUNK) + 1024;
+  if (slot_need > f.slot_cap) {
+    HIPCHECK(grow(f.near_partial, (size_t)slot_need * 64));
+    HIPCHECK(grow(f.near_combines, slot_need / 2 + 1));
+    f.slot_cap = slot_need;
+  }
+  const uint32_t item_need = f.slot_cap + f.point_cap / 64 + 1;
+  if (item_need > f.item_cap) {
+    HIPCHECK(grow(f.near_items, item_need));
+    f.item_cap = item_need;
+  }
+  return hipSuccess;
 }
 
 }  // namespace d3f

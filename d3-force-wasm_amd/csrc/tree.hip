@@ -378,6 +378,12 @@ hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n_max, con
 // Stable LSD radix sort of (u64 key, u32 value) by 8-bit digits (sort.zig:55-91
 // is a stable LSD radix sort; a stable sort of the same keys has one result).
 
+// Clears `words` counters (a kernel rather than a memset, so that a captured
+// graph of the sort holds no memset node).
+__global__ __launch_bounds__(256) void k_zero_words(uint32_t* __restrict__ p, uint32_t words) {
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < words; i += gridDim.x * 256) p[i] = 0;
+}
+
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ keys,
                                                         const uint32_t* __restrict__ count,
                                                         uint32_t* __restrict__ hist, uint32_t* __restrict__ glob,
@@ -801,7 +807,7 @@ hipError_t sort_digits(uint32_t* hist, uint32_t* glob, uint64_t* keys, uint32_t*
   const uint32_t ntiles = ceil_div(n_max > 0 ? n_max : 1, RS_TILE);
   uint64_t *ka = keys, *kb = keys_tmp;
   uint32_t *va = vals, *vb = vals_tmp;
-  HIPCHECK(hipMemsetAsync(glob, 0, (size_t)digits * 256 * sizeof(uint32_t), stream));
+  hipLaunchKernelGGL(k_zero_words, dim3(8), dim3(256), 0, stream, glob, (uint32_t)(digits * 256));
   for (int pass = 0; pass < digits; ++pass) {
     uint32_t* g = glob + 256 * pass;
     const int shift = 8 * (first_digit + pass);
