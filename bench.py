@@ -172,6 +172,17 @@ def run_batch(core, graphs, steps, warmup, seed=1):
     core.batch_set_timing(0)
     st = b.stats()
     log("batch: %.3f ms/tick, phases %s" % (1e3 * dt / steps, st["ms"]))
+    # The same ticks as one captured HIP graph replayed per tick (an eager
+    # tick and the capture first, then one replay untimed).
+    from d3force_amd.batch import BS_GRAPH
+    b.scalar(BS_GRAPH, 1)
+    b.tick(2)
+    device_sync(core)
+    t0 = time.perf_counter()
+    b.tick(steps)
+    device_sync(core)
+    dt_graph = time.perf_counter() - t0
+    log("batch: graph replay %.3f ms/tick" % (1e3 * dt_graph / steps))
     b.close()
     # Per-graph many-body error on the start layout of a sample of graphs: one
     # manyBody-only tick from v = 0, against an FP64 direct sum (numpy) on up
@@ -213,6 +224,9 @@ def run_batch(core, graphs, steps, warmup, seed=1):
                    "graphs_below_2000_nodes": int(sum(1 for g in gs if g[0] < 2000))},
         "phases_ms_per_tick": {ph: v / steps for ph, v in st["ms"].items() if v > 0},
         "autotuned_launch_shapes": tuned,
+        "graph_replay": {"ms_per_step": 1e3 * dt_graph / steps, "ticks_per_s": steps / dt_graph,
+                         "note": "the same tick captured once as a HIP graph and replayed per tick (bit-identical "
+                                 "to the eager ticks); value above is the eager form"},
         "fmm_work_per_tick": {"m2l_pairs": st["m2l"] / steps, "p2p_point_pairs": st["p2p_pairs"] / steps,
                               "cells": st["fmm_cells"]},
         "rms_force_error_vs_fp64_direct_sum": {"sampled_graphs": len(pick), "nodes": sizes,

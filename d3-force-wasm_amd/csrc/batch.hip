@@ -36,11 +36,37 @@ int32_t bstatus(hipError_t e) {
 }
 
 // splitmix64 of (id, tick, salt): the graph's seed for one force and tick.
-uint64_t mix_seed(uint64_t id, uint64_t tick, uint64_t salt) {
+__host__ __device__ inline uint64_t mix_seed(uint64_t id, uint64_t tick, uint64_t salt) {
   uint64_t h = id * 0x9E3779B97F4A7C15ull ^ (tick << 8 | salt);
   h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
   h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
   return h ^ (h >> 31);
+}
+
+// Per-tick scalars on the device (so that a captured tick reads them):
+// alpha (simulation.js:64) and the tick counter, advanced by k_batch_step at
+// the start of every tick with the host's arithmetic; `used` is the tick the
+// seeds are mixed with.
+struct BatchState {
+  double alpha;
+  unsigned long long tick, used;
+};
+
+__global__ void k_batch_step(BatchState* __restrict__ st, double target, double decay) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  st->alpha += (target - st->alpha) * decay;
+  st->used = st->tick;
+  st->tick += 1;
+}
+
+__global__ __launch_bounds__(256) void k_batch_seeds(const BatchState* __restrict__ st,
+                                                     const uint64_t* __restrict__ ids, uint32_t graphs,
+                                                     uint64_t* __restrict__ mb, uint64_t* __restrict__ link) {
+  const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= graphs) return;
+  const uint64_t t = st->used;
+  mb[g] = mix_seed(ids[g], t, 0);
+  link[g] = mix_seed(ids[g], t, 1);
 }
 
 struct Batch {
@@ -52,7 +78,15 @@ struct Batch {
   Device<double> dx, dy, dvx, dvy, dcharge, dls, dld, dlb, dpx, dpy, dzero, dsums;
   Device<uint8_t> dfixed;
   Device<uint32_t> dsrc, dtgt, dnode_graph, dnode_off, dlink_off;
-  Device<uint64_t> dseeds_link;
+  Device<uint64_t> dseeds_link, dseeds_mb, dseed_ids;
+  BatchState* dstate = nullptr;
+  bool state_dirty = true;     // the host's alpha / tick must be uploaded before the next tick
+  // captured tick (D3F_BS_GRAPH)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_valid = false;
+  Device<double> snap;         // x, y, vx, vy before a replayed tick
+  uint64_t graph_replays = 0, graph_redos = 0;
   // link rows
   bool csr_valid = false;
   Device<uint32_t> row, adj, owner, rows_long;
@@ -78,7 +112,10 @@ struct Batch {
     for (auto* a : {&dsrc, &dtgt, &dnode_graph, &dnode_off, &dlink_off, &row, &adj, &owner, &rows_long, &sv0, &sv1,
                     &shist, &sglob, &scount})
       a->release();
-    for (auto* a : {&dseeds_link, &sk0, &sk1}) a->release();
+    for (auto* a : {&dseeds_link, &dseeds_mb, &dseed_ids, &sk0, &sk1}) a->release();
+    snap.release();
+    drop_graph();
+    if (dstate) (void)hipFree(dstate);
     entries.release();
     tree_free(tree);
     fmm_free(fmm);
@@ -89,8 +126,17 @@ struct Batch {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    graph_valid = false;
+  }
+
   hipError_t init() {
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(hipMalloc(&dstate, sizeof(BatchState)));
     HIPCHECK(sglob.reserve(8 * 256));
     HIPCHECK(scount.reserve(4));
     // d3's defaults (simulation.js:18-31, manyBody.js:5-12, center.js:4-7)
@@ -133,12 +179,16 @@ struct Batch {
     HIPCHECK(dnode_off.reserve(cg + 1, 0, stream));
     HIPCHECK(dlink_off.reserve(cg + 1, 0, stream));
     HIPCHECK(dseeds_link.reserve(cg, 0, stream));
+    HIPCHECK(dseeds_mb.reserve(cg, 0, stream));
+    HIPCHECK(dseed_ids.reserve(cg, 0, stream));
     HIPCHECK(hipMemsetAsync(dfixed.p, 0, cn, stream));
     HIPCHECK(hipMemsetAsync(dzero.p, 0, cn * sizeof(double), stream));
     if (n) HIPCHECK(hipMemcpyAsync(dnode_graph.p, node_graph.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     HIPCHECK(hipMemcpyAsync(dnode_off.p, node_off.data(), (g + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     HIPCHECK(hipMemcpyAsync(dlink_off.p, link_off.data(), (g + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(dseed_ids.p, seed_ids.p, (size_t)g * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
     csr_valid = false;
+    drop_graph();
     return hipStreamSynchronize(stream);
   }
 
@@ -201,16 +251,42 @@ struct Batch {
     return a;
   }
 
-  int32_t tick() {
+  // The device part of a tick. graph: the capturable form (no host
+  // synchronisation; nothing the host reads back).
+  hipError_t device_tick(bool graph) {
     hipError_t e;
-    double& alpha = scalars[D3F_BS_ALPHA];
-    alpha += (scalars[D3F_BS_ALPHA_TARGET] - alpha) * scalars[D3F_BS_ALPHA_DECAY];  // simulation.js:64
-    const uint64_t t = (uint64_t)scalars[D3F_BS_TICK];
-    for (uint32_t g = 0; g < graphs; ++g) {
-      seeds_mb.p[g] = mix_seed(seed_ids.p[g], t, 0);
-      seeds_link.p[g] = mix_seed(seed_ids.p[g], t, 1);
-    }
+    hipLaunchKernelGGL(k_batch_step, dim3(1), dim3(64), 0, stream, dstate, scalars[D3F_BS_ALPHA_TARGET],
+                       scalars[D3F_BS_ALPHA_DECAY]);
+    hipLaunchKernelGGL(k_batch_seeds, dim3(ceil_div(graphs, 256u)), dim3(256), 0, stream, dstate, dseed_ids.p, graphs,
+                       dseeds_mb.p, dseeds_link.p);
     // many-body
+    FmmArgs a = fmm_args();
+    a.graph = graph;
+    const bool timed = g_batch_timing && !graph;
+    hipEvent_t ev = timed ? mark() : nullptr;
+    if ((e = fmm_run(fmm, tree, a, stream)) != hipSuccess) return e;
+    timer(D3F_PHASE_FMM, ev);
+    // link
+    ev = timed ? mark() : nullptr;
+    for (double k = 0; k < scalars[D3F_BS_LINK_ITERATIONS]; k += 1) {
+      HIPCHECK(shift_positions(dx.p, dy.p, dvx.p, dvy.p, dpx.p, dpy.p, n, stream));
+      HIPCHECK(link_sweep(csr_args(), long_count, dpx.p, dpy.p, dvx.p, dvy.p, scalars[D3F_BS_ALPHA], 0, stream,
+                          &dstate->alpha));
+    }
+    timer(D3F_PHASE_LINK, ev);
+    // center
+    ev = timed ? mark() : nullptr;
+    HIPCHECK(batch_center(dx.p, dy.p, dnode_off.p, dnode_graph.p, graphs, n, dsums.p, scalars[D3F_BS_CENTER_X],
+                          scalars[D3F_BS_CENTER_Y], scalars[D3F_BS_CENTER_STRENGTH], stream));
+    timer(D3F_PHASE_CENTER, ev);
+    // integrate
+    ev = timed ? mark() : nullptr;
+    HIPCHECK(integrate(dx.p, dy.p, dvx.p, dvy.p, dzero.p, dzero.p, dfixed.p, n, scalars[D3F_BS_VELOCITY_DECAY], stream));
+    timer(D3F_PHASE_INTEGRATE, ev);
+    return hipSuccess;
+  }
+
+  FmmArgs fmm_args() {
     FmmArgs a;
     a.x = dx.p;
     a.y = dy.p;
@@ -220,45 +296,93 @@ struct Batch {
     a.n = n;
     const double order = scalars[D3F_BS_ORDER];
     a.order = order >= 1 && order <= (double)FMM_MAX_ORDER ? (uint32_t)order : 8;
-    a.alpha = alpha;
+    a.alpha = scalars[D3F_BS_ALPHA];
+    a.alpha_dev = &dstate->alpha;
     a.opening = scalars[D3F_BS_OPENING];
     a.dmin2 = scalars[D3F_BS_DISTANCE_MIN2];
     a.dmax2 = scalars[D3F_BS_DISTANCE_MAX2];
     a.graphs = graphs;
     a.node_off = node_off.data();
     a.node_graph = dnode_graph.p;
-    a.seeds = seeds_mb.p;
-    hipEvent_t ev = g_batch_timing ? mark() : nullptr;
-    if ((e = fmm_run(fmm, tree, a, stream)) != hipSuccess) return bstatus(e);
-    timer(D3F_PHASE_FMM, ev);
+    a.seeds = nullptr;
+    a.seeds_dev = dseeds_mb.p;
+    return a;
+  }
+
+  // Captures one tick (buffers sized for twice the cells of the last eager
+  // tick): the state snapshot, then the device part of a tick.
+  hipError_t capture() {
+    drop_graph();
+    const size_t bn = (size_t)n * sizeof(double);
+    HIPCHECK(snap.reserve(4 * (size_t)(n > 0 ? n : 1), 0, stream));
+    const uint32_t cells = 2 * fmm.params_host->level_off[FMM_LEVELS + 1] + 4096;
+    HIPCHECK(fmm_reserve_capacity(fmm, tree, fmm_args(), cells, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    const double* from[4] = {dx.p, dy.p, dvx.p, dvy.p};
+    for (int k = 0; k < 4 && e == hipSuccess; ++k)
+      e = hipMemcpyAsync(snap.p + (size_t)k * n, from[k], bn, hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess) e = device_tick(true);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return e != hipSuccess ? e : e2;
+    }
+    graph = g;
+    HIPCHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    graph_valid = true;
+    return hipSuccess;
+  }
+
+  int32_t tick() {
+    hipError_t e;
+    if (!csr_valid) {
+      drop_graph();
+      if ((e = build_rows()) != hipSuccess) return bstatus(e);
+    }
+    if (state_dirty) {
+      const BatchState hs{scalars[D3F_BS_ALPHA], (unsigned long long)scalars[D3F_BS_TICK], 0ull};
+      if ((e = hipMemcpy(dstate, &hs, sizeof(hs), hipMemcpyHostToDevice)) != hipSuccess) return bstatus(e);
+      state_dirty = false;
+    }
+    const bool want_graph = scalars[D3F_BS_GRAPH] != 0;
+    bool done = false;
+    if (want_graph && graph_valid) {
+      if ((e = hipGraphLaunch(gexec, stream)) != hipSuccess) return bstatus(e);
+      if ((e = hipMemcpyAsync(fmm.params_host, fmm.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream)) !=
+          hipSuccess)
+        return bstatus(e);
+      if ((e = hipStreamSynchronize(stream)) != hipSuccess) return bstatus(e);
+      ++graph_replays;
+      if (!fmm.params_host->abort) {
+        done = true;
+      } else {  // sizes outgrew the captured buffers: undo the replay, redo the tick eagerly
+        ++graph_redos;
+        const size_t bn = (size_t)n * sizeof(double);
+        double* to[4] = {dx.p, dy.p, dvx.p, dvy.p};
+        for (int k = 0; k < 4; ++k)
+          if ((e = hipMemcpyAsync(to[k], snap.p + (size_t)k * n, bn, hipMemcpyDeviceToDevice, stream)) != hipSuccess)
+            return bstatus(e);
+        const BatchState hs{scalars[D3F_BS_ALPHA], (unsigned long long)scalars[D3F_BS_TICK], 0ull};
+        if ((e = hipMemcpyAsync(dstate, &hs, sizeof(hs), hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(stream)) != hipSuccess)
+          return bstatus(e);
+        drop_graph();
+      }
+    }
+    if (!done) {
+      if ((e = device_tick(false)) != hipSuccess) return bstatus(e);
+      if (want_graph && !graph_valid && !fmm.params_host->declined && (e = capture()) != hipSuccess)
+        return bstatus(e);
+    }
     if (fmm.params_host->declined) return B_BAD_ARGUMENT;  // a graph with a non-finite extent
     stats.m2l += fmm.params_host->m2l;
     stats.p2p_pairs += fmm.params_host->pairs;
     stats.fmm_cells = fmm.params_host->level_off[FMM_LEVELS + 1];
-    // link
-    ev = g_batch_timing ? mark() : nullptr;
-    if (!csr_valid && (e = build_rows()) != hipSuccess) return bstatus(e);
-    if ((e = hipMemcpyAsync(dseeds_link.p, seeds_link.p, graphs * sizeof(uint64_t), hipMemcpyHostToDevice, stream)) !=
-        hipSuccess)
-      return bstatus(e);
-    for (double k = 0; k < scalars[D3F_BS_LINK_ITERATIONS]; k += 1) {
-      e = shift_positions(dx.p, dy.p, dvx.p, dvy.p, dpx.p, dpy.p, n, stream);
-      if (e == hipSuccess) e = link_sweep(csr_args(), long_count, dpx.p, dpy.p, dvx.p, dvy.p, alpha, 0, stream);
-      if (e != hipSuccess) return bstatus(e);
-    }
-    timer(D3F_PHASE_LINK, ev);
-    // center
-    ev = g_batch_timing ? mark() : nullptr;
-    if ((e = batch_center(dx.p, dy.p, dnode_off.p, dnode_graph.p, graphs, n, dsums.p, scalars[D3F_BS_CENTER_X],
-                          scalars[D3F_BS_CENTER_Y], scalars[D3F_BS_CENTER_STRENGTH], stream)) != hipSuccess)
-      return bstatus(e);
-    timer(D3F_PHASE_CENTER, ev);
-    // integrate
-    ev = g_batch_timing ? mark() : nullptr;
-    if ((e = integrate(dx.p, dy.p, dvx.p, dvy.p, dzero.p, dzero.p, dfixed.p, n, scalars[D3F_BS_VELOCITY_DECAY],
-                       stream)) != hipSuccess)
-      return bstatus(e);
-    timer(D3F_PHASE_INTEGRATE, ev);
+    double& alpha = scalars[D3F_BS_ALPHA];
+    alpha += (scalars[D3F_BS_ALPHA_TARGET] - alpha) * scalars[D3F_BS_ALPHA_DECAY];  // simulation.js:64
     scalars[D3F_BS_TICK] += 1;
     return B_OK;
   }
@@ -348,7 +472,8 @@ EXPORT int32_t batch_sync(uint32_t h, uint32_t direction) {
     struct { void* d; const void* p; size_t bytes; } copies[] = {
         {b->dx.p, b->x.p, bn}, {b->dy.p, b->y.p, bn}, {b->dvx.p, b->vx.p, bn}, {b->dvy.p, b->vy.p, bn},
         {b->dcharge.p, b->charge.p, bn}, {b->dls.p, b->lstrength.p, bm}, {b->dld.p, b->ldistance.p, bm},
-        {b->dlb.p, b->lbias.p, bm}, {b->dsrc.p, b->source.p, (size_t)b->m * 4}, {b->dtgt.p, b->target.p, (size_t)b->m * 4}};
+        {b->dlb.p, b->lbias.p, bm}, {b->dsrc.p, b->source.p, (size_t)b->m * 4}, {b->dtgt.p, b->target.p, (size_t)b->m * 4},
+        {b->dseed_ids.p, b->seed_ids.p, (size_t)b->graphs * 8}};
     for (auto& c : copies)
       if (e == hipSuccess && c.bytes) e = hipMemcpyAsync(c.d, c.p, c.bytes, hipMemcpyHostToDevice, s);
     b->csr_valid = false;
@@ -369,6 +494,10 @@ EXPORT int32_t batch_set_scalar(uint32_t h, uint32_t which, double v) {
   if (!b) return B_BAD_HANDLE;
   if (which >= D3F_BS_COUNT) return B_BAD_ARGUMENT;
   b->scalars[which] = v;
+  if (which == D3F_BS_ALPHA || which == D3F_BS_TICK)
+    b->state_dirty = true;   // read by the device each tick
+  else if (which != D3F_BS_GRAPH)
+    b->drop_graph();         // the captured tick holds the others as launch arguments
   return B_OK;
 }
 
@@ -400,6 +529,7 @@ EXPORT int32_t batch_autotune(uint32_t h, uint32_t ticks, uint32_t* chosen, floa
   Batch* b = bget(h);
   if (!b) return B_BAD_HANDLE;
   if (b->graphs == 0 || ticks == 0) return B_BAD_ARGUMENT;
+  b->drop_graph();  // a captured tick holds the launch shapes
   static const uint32_t dual[] = {256, 512, 1024, 2048, 4096};
   static const uint32_t level[] = {64, 128, 256, 512, 1024};
   static const uint32_t extra[] = {64, 128, 256, 512};
@@ -434,6 +564,7 @@ EXPORT int32_t batch_autotune(uint32_t h, uint32_t ticks, uint32_t* chosen, floa
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  b->drop_graph();
   if (chosen) {
     chosen[0] = b->fmm.dual_blocks;
     chosen[1] = b->fmm.level_blocks;

@@ -221,8 +221,9 @@ struct LinkCsrArgs {
 // per-thread limit. Synchronises.
 hipError_t link_csr(const LinkCsrArgs& a, const SortBuffers& sb, uint32_t* long_count, hipStream_t stream);
 // One Jacobi sweep over the rows on px = x + vx (link_fast.zig:60-99).
+// alpha_dev, when given, replaces alpha (read on the device, for captured graphs).
 hipError_t link_sweep(const LinkCsrArgs& a, uint32_t long_count, const double* px, const double* py, double* vx,
-                      double* vy, double alpha, uint64_t seed, hipStream_t stream);
+                      double* vy, double alpha, uint64_t seed, hipStream_t stream, const double* alpha_dev = nullptr);
 // One iteration of the uniform-grid collide (grid.zig:221-285).
 hipError_t grid_collide_iteration(const double* x, const double* y, double* vx, double* vy, const double* radii,
                                   uint32_t n, double strength, uint64_t seed, const GridBuffers& g,
@@ -324,6 +325,7 @@ struct FmmArgs {
   // params_host after the tick and redoes it eagerly when abort is set.
   bool graph = false;
   const double* alpha_dev = nullptr;
+  const uint64_t* seeds_dev = nullptr;  // device seeds, in place of the host ones
 };
 // Fast many-body (fmm.zig:247-354) over every graph of the forest at once.
 // With one graph, returns hipErrorNotSupported when the FMM declines (a

@@ -1357,8 +1357,8 @@ hipError_t reserve_cells(FmmBuffers& f, uint32_t nc, uint32_t q) {
 }
 
 // Grids of the capturable form (grid-stride kernels over device counts).
-constexpr uint32_t CAP_CELL_BLOCKS = 1024;  // per-cell and per-pair elementwise kernels
-constexpr uint32_t CAP_TARGET_BLOCKS = 8192;  // one block per target, looping
+constexpr uint32_t CAP_CELL_BLOCKS = 4096;  // per-cell and per-pair elementwise kernels
+constexpr uint32_t CAP_TARGET_BLOCKS = 65536;  // one wave per target, looping (many, for load balance)
 
 template <int Q>
 hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t stream) {
@@ -1573,7 +1573,7 @@ hipError_t reserve_forest(FmmBuffers& f, const FmmArgs& a, hipStream_t stream) {
     f.node_off_host.assign(a.node_off, a.node_off + G + 1);
     f.chunks = C;
   }
-  HIPCHECK(hipMemcpyAsync(f.seeds, a.seeds, G * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+  if (!a.seeds_dev) HIPCHECK(hipMemcpyAsync(f.seeds, a.seeds, G * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
   return hipSuccess;
 }
 
@@ -1597,7 +1597,8 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
     hipLaunchKernelGGL(k_forest_box, dim3(f.chunks), dim3(EWF), 0, stream, a.x, a.y, f.chunk_lo, f.chunk_hi,
                        (BoxPart*)f.box_parts);
   hipLaunchKernelGGL(k_forest_graphs, dim3(ceil_div(a.graphs, 64)), dim3(64), 0, stream, (const BoxPart*)f.box_parts,
-                     f.chunk_first, a.graphs, a.dmin2, a.dmax2, f.seeds, f.node_off_dev, f.graphs, f.params);
+                     f.chunk_first, a.graphs, a.dmin2, a.dmax2, a.seeds_dev ? a.seeds_dev : f.seeds, f.node_off_dev,
+                     f.graphs, f.params);
   hipLaunchKernelGGL(k_forest_keys, dim3(nb), dim3(EWF), 0, stream, a.x, a.y, n, a.node_graph, f.graphs, f.params,
                      t.keys, t.perm, t.keys_all, t.flags);
   HIPCHECK(scan_u32(t.flags, t.offs, n > 0 ? n : 1, &f.params->n, t.partials, nullptr, stream));

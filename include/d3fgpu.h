@@ -150,11 +150,14 @@ enum { D3F_B_X = 0, D3F_B_Y = 1, D3F_B_VX = 2, D3F_B_VY = 3, D3F_B_CHARGE = 4, D
        D3F_B_LINK_STRENGTH = 7, D3F_B_LINK_DISTANCE = 8, D3F_B_LINK_BIAS = 9, D3F_B_SEED = 10 };
 /* Batch scalars; VELOCITY_DECAY is the multiplier (1 - d3's velocityDecay,
  * default 0.6), DISTANCE_MIN2 / MAX2 the squared distances, TICK counts the
- * ticks run. */
+ * ticks run. GRAPH = 1 replays one captured HIP graph per tick (captured
+ * after an eager tick, with buffers sized for twice the cells that tick had;
+ * a replay whose sizes do not fit is undone and the tick redone eagerly, and
+ * the graph recaptured); results are bit-identical to the eager ticks. */
 enum { D3F_BS_ALPHA = 0, D3F_BS_ALPHA_DECAY = 1, D3F_BS_ALPHA_TARGET = 2, D3F_BS_VELOCITY_DECAY = 3,
        D3F_BS_DISTANCE_MIN2 = 4, D3F_BS_DISTANCE_MAX2 = 5, D3F_BS_ORDER = 6, D3F_BS_OPENING = 7, D3F_BS_CENTER_X = 8,
        D3F_BS_CENTER_Y = 9, D3F_BS_CENTER_STRENGTH = 10, D3F_BS_LINK_ITERATIONS = 11, D3F_BS_TICK = 12,
-       D3F_BS_COUNT = 13 };
+       D3F_BS_GRAPH = 13, D3F_BS_COUNT = 14 };
 uint32_t batch_create(void);
 int32_t batch_destroy(uint32_t b);
 /* node_off, link_off: graphs + 1 host entries each, starting at 0. */

@@ -128,3 +128,20 @@ def test_batch_autotune_keeps_every_bit(gpu_core, graphs):
     assert same(a.state(), b.state())
     a.close()
     b.close()
+
+
+def test_batch_graph_replays_equal_eager_ticks(gpu_core, graphs):
+    # BS_GRAPH = 1: one eager tick, then a captured tick replayed every tick.
+    # Each replay is compared with an eager batch after the same tick (with
+    # the host's reads and synchronisations between replays).
+    from d3force_amd.batch import BS_GRAPH
+    a = Batch(graphs, core=gpu_core)
+    b = Batch(graphs, core=gpu_core)
+    a.scalar(BS_GRAPH, 1)
+    for t in range(8):
+        a.tick(1)
+        b.tick(1)
+        assert same(a.state(), b.state()), "tick %d" % (t + 1)
+    assert a.scalar(BS_ALPHA) == b.scalar(BS_ALPHA)
+    a.close()
+    b.close()
