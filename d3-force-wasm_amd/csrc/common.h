@@ -43,7 +43,6 @@ constexpr uint32_t TC_LEAF = 1, TC_REACH = 2;
 // Device-resident scalars of the last tree build.
 struct TreeMeta {
   double extent[4];
-  double box[4];    // bounding box of the non-NaN points (x0, y0, x1, y1)
   uint32_t n;       // nodes
   uint32_t m;       // points in the tree (no NaN coordinate)
   uint32_t len;     // cells
@@ -111,10 +110,6 @@ void tree_free(TreeBuffers& t);
 // meta_host holds the tree's scalars (the call synchronises once, after the
 // cell count is known, to size the cell buffers).
 hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, uint32_t n, hipStream_t stream);
-// Fast mode's keys (tree.zig:346-364): box into meta->box, 16-level Morton
-// keys of the non-NaN points sorted with their nodes into keys/perm (m =
-// meta->m), NaN nodes into rest. No synchronisation.
-hipError_t tree_prepare_fast(TreeBuffers& t, const double* x, const double* y, uint32_t n, hipStream_t stream);
 // Stable LSD radix sort of (keys, vals)[0 .. *count) on the 8-bit digits
 // [first_digit, first_digit + digits); n_max bounds *count. The result is in
 // keys / vals (after an odd number of passes it is copied back from the
@@ -189,7 +184,7 @@ struct GridParams {
 };
 struct GridBuffers {
   double *px = nullptr, *py = nullptr;  // x + vx, y + vy
-  uint32_t *cell_of = nullptr, *start = nullptr;
+  uint32_t* start = nullptr;
   double4* cand = nullptr;              // binned nodes: x, y, radius, node bits
   void* partials = nullptr;
   GridParams* params = nullptr;

@@ -11,7 +11,6 @@ namespace {
 
 constexpr int EW = 256;
 constexpr uint32_t TARGET_BIT = 0x80000000u;  // link_fast.zig:20
-constexpr uint32_t NO_CELL = 0xffffffffu;     // grid.zig:58
 constexpr uint32_t LONG_ROW = 64;             // rows longer than this are summed by a wave
 
 // ---------------------------------------------------------------- link sweep
@@ -289,8 +288,7 @@ __global__ __launch_bounds__(EW) void k_grid_setup(const GridPartial* __restrict
 // (`cells`, sorting last, for a NaN position).
 __global__ __launch_bounds__(EW) void k_grid_bin(const double* __restrict__ px, const double* __restrict__ py,
                                                  uint32_t n, const GridParams* __restrict__ gp,
-                                                 uint32_t* __restrict__ cell_of, uint64_t* __restrict__ keys,
-                                                 uint32_t* __restrict__ items) {
+                                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ items) {
   const uint32_t k = blockIdx.x * EW + threadIdx.x;
   if (k >= n || !gp->active) return;
   const uint32_t i = n - 1 - k;
@@ -301,7 +299,6 @@ __global__ __launch_bounds__(EW) void k_grid_bin(const double* __restrict__ px, 
     const uint32_t cy = (uint32_t)floor((py[i] - gp->y0) * gp->inv_h);
     cell = cy * gp->gx + cx;
   }
-  cell_of[i] = cell == cells ? NO_CELL : cell;
   keys[k] = cell;
   items[k] = i;
 }
@@ -451,7 +448,7 @@ hipError_t grid_collide_iteration(const double* x, const double* y, double* vx, 
                      (GridPartial*)g.partials);
   hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(EW), 0, stream, (const GridPartial*)g.partials, blocks, n,
                      g.params);
-  hipLaunchKernelGGL(k_grid_bin, dim3(nb), dim3(EW), 0, stream, g.px, g.py, n, g.params, g.cell_of, sb.k0, sb.v0);
+  hipLaunchKernelGGL(k_grid_bin, dim3(nb), dim3(EW), 0, stream, g.px, g.py, n, g.params, sb.k0, sb.v0);
   // Cell ids are at most 4n + 16 (grid.zig:243-250); the count is 0 when
   // the iteration is inactive.
   int digits = 0;

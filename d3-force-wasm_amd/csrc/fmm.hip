@@ -2,25 +2,35 @@
 // with its tree, geometry and interaction lists reproduced exactly in FP64,
 // and the expansions, M2L and near field computed in FP32.
 //
-//  1. Keys and sort (tree.hip, tree_prepare_fast), then the level-order
-//     cells (fmm.zig:685-711) one depth at a time: each splitting cell finds
-//     its quadrant boundaries by binary search in its key range, a scan of
-//     the child counts places the children contiguously in the next level.
+//  1. Keys and sort: every graph of the forest gets its box and frame, its
+//     points 16-level Morton keys (tree.zig:346-364) with the graph id above
+//     them, and one radix sort orders all graphs at once; then the
+//     level-order cells (fmm.zig:685-711) one depth at a time: each
+//     splitting cell finds its quadrant boundaries by binary search in its
+//     key range, a scan of the child counts places the children contiguously
+//     in the next level.
 //  2. Gather in Morton order (fmm.zig:668-682), geometry (fmm.zig:721-754)
 //     and P2M / M2M (fmm.zig:757-806) one depth at a time, bottom up. Bounds,
 //     centres and radii are FP64 with the reference's operations, so the
 //     opening test sees the same numbers.
 //  3. The dual-tree interaction (fmm.zig:400-453) breadth first: one round
 //     per pair depth expands the frontier of cell pairs into children pairs,
-//     M2L pairs and P2P pairs (wave-aggregated appends). The recursion's
+//     M2L pairs and P2P pairs (block-aggregated appends). The recursion's
 //     decisions depend on the pair alone, so the lists hold exactly the
 //     reference's pairs.
 //  4. Both lists are grouped by target (count, scan, scatter), and each
-//     target's group is ordered by source in LDS before it is read, so every
-//     FP32 sum runs in one fixed order whatever the schedule.
-//  5. M2L, one wave per target cell, one lane per source cell; L2L top down;
-//     then P2P and L2P fused, one wave per target leaf, and the sum is added
-//     to the FP64 velocity once per node.
+//     target's group is ordered by source (in LDS, or in place for long
+//     groups) before it is read, so every FP32 sum runs in one fixed order
+//     whatever the schedule.
+//  5. M2L on the matrix cores, one wave per target cell, 16 source cells per
+//     product; L2L top down; then P2P and L2P fused, one wave per batch of 64
+//     target points, long source groups split into chunks over several
+//     blocks, and the sum is added to the FP64 velocity once per node.
+//
+// The launches are sized either from counts read back once per tick (the
+// eager form) or from the buffers' capacities with every kernel reading its
+// count on the device (the form a captured graph replays); both run the same
+// kernels on the same data.
 //
 // Expansions are scaled by their cell's radius rho (M_k / rho^k, L_l rho^l),
 // so every FP32 quantity stays near unit size at any depth. With w = 1/z0

@@ -140,7 +140,7 @@ struct Sim {
   // fast-mode scratch: radix sort ping-pong, grid of the fast collide
   Device<uint64_t> sk0, sk1;
   Device<uint32_t> sv0, sv1, shist, sglob, scount;
-  Device<uint32_t> gcell_of, gstart;
+  Device<uint32_t> gstart;
   Device<double4> gcand;
   Device<uint8_t> gpartials;
   GridParams* gparams = nullptr;
@@ -187,7 +187,7 @@ struct Sim {
     tree_free(tree);
     fmm_free(fmm);
     for (auto* a : {&sk0, &sk1}) a->release();
-    for (auto* a : {&sv0, &sv1, &shist, &sglob, &scount, &gcell_of, &gstart}) a->release();
+    for (auto* a : {&sv0, &sv1, &shist, &sglob, &scount, &gstart}) a->release();
     gcand.release();
     gpartials.release();
     if (gparams) (void)hipFree(gparams);
@@ -621,14 +621,12 @@ int32_t run_collide_fast(Sim& s, Force& f) {  // grid.zig:198-286
   SortBuffers sb;
   if ((e = s.sort_buffers(n, &sb)) != hipSuccess) return status_of(e);
   const size_t c = n > 0 ? n : 1;
-  if ((e = s.gcell_of.reserve(c, 0, s.stream)) != hipSuccess) return status_of(e);
   if ((e = s.gstart.reserve(4 * c + 17, 0, s.stream)) != hipSuccess) return status_of(e);
   if ((e = s.gcand.reserve(c, 0, s.stream)) != hipSuccess) return status_of(e);
   if ((e = s.gpartials.reserve(grid_partials_bytes(), 0, s.stream)) != hipSuccess) return status_of(e);
   GridBuffers g;
   g.px = s.dpx.p;
   g.py = s.dpy.p;
-  g.cell_of = s.gcell_of.p;
   g.start = s.gstart.p;
   g.cand = s.gcand.p;
   g.partials = s.gpartials.p;

@@ -159,10 +159,6 @@ __global__ __launch_bounds__(BOX_THREADS) void k_box(const double* __restrict__ 
       meta->valid = 1;
     }
     for (int k = 0; k < 4; ++k) meta->extent[k] = e[k];
-    meta->box[0] = r.x0;
-    meta->box[1] = r.y0;
-    meta->box[2] = r.x1;
-    meta->box[3] = r.y1;
     meta->ticket = 0;
   }
 }
@@ -183,32 +179,6 @@ __device__ inline uint64_t exact_key(double x, double y, const double* e) {
   return key;
 }
 
-// Bits 0..15 of v moved to the even bit positions (tree.zig:688-695).
-__device__ inline uint32_t spread16(uint32_t v) {
-  v &= 0xffffu;
-  v = (v | (v << 8)) & 0x00ff00ffu;
-  v = (v | (v << 4)) & 0x0f0f0f0fu;
-  v = (v | (v << 2)) & 0x33333333u;
-  v = (v | (v << 1)) & 0x55555555u;
-  return v;
-}
-
-// Fast mode's 16-level key (tree.zig:346-364): the square of side
-// max(width, height) (1 when that is not positive) over the points' box,
-// quantised to 65536 steps per axis and interleaved, y above x.
-__device__ inline uint64_t fast_key(double x, double y, const double* b) {
-  const double w = b[2] - b[0], h = b[3] - b[1];
-  double side = (h > w || w != w) ? h : w;  // Zig @max, first operand kept on ties
-  if (!(side > 0)) side = 1;
-  const double scale = 65536.0 / side;
-  double fx = floor((x - b[0]) * scale), fy = floor((y - b[1]) * scale);
-  fx = (65535.0 < fx || fx != fx) ? 65535.0 : fx;
-  fy = (65535.0 < fy || fy != fy) ? 65535.0 : fy;
-  const uint32_t qx = (uint32_t)fx, qy = (uint32_t)fy;
-  return (uint64_t)(spread16(qx) | spread16(qy) << 1);
-}
-
-template <bool FAST>
 __global__ __launch_bounds__(EW_THREADS) void k_keys(const double* __restrict__ xs, const double* __restrict__ ys,
                                                      uint32_t n, const TreeMeta* __restrict__ meta,
                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ perm,
@@ -219,13 +189,8 @@ __global__ __launch_bounds__(EW_THREADS) void k_keys(const double* __restrict__ 
   bool valid = !(x != x || y != y);
   uint64_t key = 0;
   if (valid) {
-    if (FAST) {
-      double b[4] = {meta->box[0], meta->box[1], meta->box[2], meta->box[3]};
-      key = fast_key(x, y, b);
-    } else {
-      double e[4] = {meta->extent[0], meta->extent[1], meta->extent[2], meta->extent[3]};
-      key = exact_key(x, y, e);
-    }
+    double e[4] = {meta->extent[0], meta->extent[1], meta->extent[2], meta->extent[3]};
+    key = exact_key(x, y, e);
   }
   if (meta->m == n) {
     keys[i] = key;
@@ -906,7 +871,7 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
   if (box_blocks > (uint32_t)BOX_BLOCKS) box_blocks = BOX_BLOCKS;
   hipLaunchKernelGGL(k_box, dim3(box_blocks), dim3(BOX_THREADS), 0, stream, x, y, n,
                      (BoxPartial*)t.box_partials, t.meta);
-  hipLaunchKernelGGL(k_keys<false>, dim3(nb), dim3(EW_THREADS), 0, stream, x, y, n, t.meta, t.keys, t.perm, t.keys_all,
+  hipLaunchKernelGGL(k_keys, dim3(nb), dim3(EW_THREADS), 0, stream, x, y, n, t.meta, t.keys, t.perm, t.keys_all,
                      t.flags);
   // Compaction of NaN points; skipped on the device when m == n.
   const uint32_t* nptr = &t.meta->n;
@@ -950,24 +915,5 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
   return hipGetLastError();
 }
 
-hipError_t tree_prepare_fast(TreeBuffers& t, const double* x, const double* y, uint32_t n, hipStream_t stream) {
-  HIPCHECK(tree_reserve(t, n > 0 ? n : 1));
-  const uint32_t nb = ceil_div(n > 0 ? n : 1, EW_THREADS);
-  uint32_t box_blocks = ceil_div(n > 0 ? n : 1, BOX_THREADS * 4);
-  if (box_blocks > (uint32_t)BOX_BLOCKS) box_blocks = BOX_BLOCKS;
-  hipLaunchKernelGGL(k_box, dim3(box_blocks), dim3(BOX_THREADS), 0, stream, x, y, n,
-                     (BoxPartial*)t.box_partials, t.meta);
-  hipLaunchKernelGGL(k_keys<true>, dim3(nb), dim3(EW_THREADS), 0, stream, x, y, n, t.meta, t.keys, t.perm,
-                     t.keys_all, t.flags);
-  const uint32_t* nptr = &t.meta->n;
-  const uint32_t* skip = &t.meta->n;
-  HIPCHECK(scan_exclusive(t.flags, t.offs, n, nptr, t.partials, nullptr, skip, stream));
-  hipLaunchKernelGGL(k_compact, dim3(nb), dim3(EW_THREADS), 0, stream, n, t.meta, t.keys_all, t.flags, t.offs,
-                     t.keys, t.perm, t.rest);
-  hipLaunchKernelGGL(k_rest_all, dim3(nb), dim3(EW_THREADS), 0, stream, n, t.meta, t.rest);
-  // 32-bit keys: 4 passes (sort.zig via tree.zig:363).
-  return sort_digits(t.hist, t.rs_glob, t.keys, t.perm, t.keys_tmp, t.perm_tmp, n > 0 ? n : 1, &t.meta->m, 0, 4,
-                     stream);
-}
 
 }  // namespace d3f
