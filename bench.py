@@ -325,7 +325,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the tick on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+              "(sharding the tick over several GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import d3force_amd as d3

@@ -14,7 +14,7 @@ It is trained on layouts the simulator itself produces at alpha ~ 0.1 (fast
 mode from phyllotaxis), with the stress loss over the graph's links and random
 node pairs, so the rotation and translation of the prediction are free.
 
-Energy of a layout (the quantity the warm start is judged by; the reference
+Energy of a layout (the quantity the warm start is measured by; the reference
 defines none): the potential of d3's forces with their default parameters,
   E = sum over links 1/2 k (r - d)^2 + sum over node pairs |s| log(1 / r'),
 with k the link strength, d = 30, s = -30 and r' = max(r, 1) (distanceMin
@@ -221,7 +221,7 @@ WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_we
 # ---------------------------------------------------------------- pipeline
 
 def fast_simulation(n, src, tgt, core=None, x=None, y=None):
-    """The fast-mode simulation the warm start is trained on and judged by:
+    """The fast-mode simulation the warm start is trained on and measured by:
     manyBody + link + center with d3's defaults, FMM order 8, opening 1.5."""
     from . import forceSimulation, forceManyBody, forceLink, forceCenter, Nodes
     nodes = Nodes(n) if x is None else Nodes(n, x=x, y=y)

@@ -4,7 +4,7 @@ the oracle and the GPU core see identical inputs.
 * ``generator``: test/parity/scenarios.js:7-16 (a 32-bit integer mixer).
 * fast-mode layouts: test/threads/scenarios.js:13-56 (inputs built from
   correctly rounded arithmetic only, so the digests are portable).
-* BASELINE workloads: random graph with m = 2n distinct-pair links from the
+* benchmark workloads: random graph with m = 2n distinct-pair links from the
   d3 LCG, phyllotaxis start, optional pinned nodes.
 """
 
