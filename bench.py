@@ -52,10 +52,11 @@ def make_workload(core, n, m, pin_fraction, seed=1):
     return sim, nodes
 
 
-def make_fast_workload(core, n, seed=1):
+def make_fast_workload(core, n, seed=1, order=8, opening=1.5):
     """The fast-mode workload: phyllotaxis start, Barabasi-Albert links (m0 = 3,
-    3 per new node), forceManyBody (FMM order 8, opening 1.5), forceLink
-    (adjacency sweep), forceCollide with radius uniform in [2, 6]."""
+    3 per new node), forceManyBody (FMM of the given order and opening; 8 and
+    1.5 by default), forceLink (adjacency sweep), forceCollide with radius
+    uniform in [2, 6]."""
     import numpy as np
     import d3force_amd as d3
     from scenarios import ba_graph
@@ -67,12 +68,12 @@ def make_fast_workload(core, n, seed=1):
            .force("charge", d3.forceManyBody())
            .force("link", d3.forceLink((src, tgt)))
            .force("collide", d3.forceCollide(radii)))
-    sim.precision("fast", {"order": 8, "opening": 1.5})
+    sim.precision("fast", {"order": order, "opening": opening})
     log("fast workload %d nodes, %d links ready in %.1fs" % (n, len(src), time.time() - t0))
     return sim, nodes, len(src)
 
 
-def fast_rms(core, nodes, sample_size=10000, seed=3):
+def fast_rms(core, nodes, sample_size=10000, seed=3, order=8, opening=1.5):
     """Relative RMS error of the fast many-body force on the current layout
     against the library's FP64 direct sum, over a fixed sample of nodes: one
     manyBody-only tick with velocityDecay 0 from v = 0 (the form of the
@@ -84,7 +85,7 @@ def fast_rms(core, nodes, sample_size=10000, seed=3):
     probe = d3.Nodes(n, x=nodes.x.copy(), y=nodes.y.copy(), vx=np.zeros(n), vy=np.zeros(n))
     charge = d3.forceManyBody()
     sim = d3.forceSimulation(probe, core=core).velocity_decay(0).force("charge", charge).stop()
-    sim.precision("fast", {"order": 8, "opening": 1.5})
+    sim.precision("fast", {"order": order, "opening": opening})
     sample = np.sort(np.random.default_rng(seed).choice(n, min(n, sample_size), replace=False)).astype(np.uint32)
     ref = np.empty(2 * len(sample))
     ctx = sim.begin()
@@ -100,8 +101,8 @@ def fast_rms(core, nodes, sample_size=10000, seed=3):
     return float(np.sqrt(np.sum(ex * ex + ey * ey) / np.sum(fx * fx + fy * fy))), len(sample)
 
 
-def run_fast(core, n, steps, warmup):
-    sim, nodes, m = make_fast_workload(core, n)
+def run_fast(core, n, steps, warmup, order=8, opening=1.5):
+    sim, nodes, m = make_fast_workload(core, n, order=order, opening=opening)
     ctx = sim.begin()
     log("fast: warmup %d ticks" % warmup)
     sim.iterate(warmup, ctx)
@@ -118,9 +119,9 @@ def run_fast(core, n, steps, warmup):
     st = core.stats(ctx.h)
     sim.end(ctx)
     log("fast: %.3f ms/tick, phases %s" % (1e3 * dt / steps, st["ms"]))
-    err, k = fast_rms(core, nodes)
+    err, k = fast_rms(core, nodes, order=order, opening=opening)
     log("fast: rms vs FP64 direct sum %.3e on %d nodes" % (err, k))
-    p = 8
+    p = order
     launches = max(st["launches"]["fmm"], 1)
     t_fmm = st["ms"]["fmm"] / 1e3 / launches
     flops = (8 * (p + 1) ** 2 * st["m2l"] + 11 * st["p2p_pairs"]) / launches
@@ -128,8 +129,9 @@ def run_fast(core, n, steps, warmup):
         "value": steps / dt, "unit": "ticks/s", "ms_per_step": 1e3 * dt / steps, "steps": steps, "warmup": warmup,
         "dtype": "f64 positions and centres, f32 expansions / M2L / P2P",
         "config": {"workload": "fast mode, %d nodes, Barabasi-Albert graph (m0 = 3, 3 links per new node, %d links), "
-                               "phyllotaxis start, manyBody FMM order 8 opening 1.5 + link sweep + grid collide "
-                               "radius U[2, 6]" % (n, m), "nodes": n, "links": m, "mode": "fast",
+                               "phyllotaxis start, manyBody FMM order %d opening %g + link sweep + grid collide "
+                               "radius U[2, 6]" % (n, m, order, opening), "nodes": n, "links": m, "mode": "fast",
+                   "order": order, "opening": opening,
                    "ticks": "%d-%d" % (warmup + 1, warmup + steps)},
         "rms_force_error_vs_fp64_direct_sum": err, "rms_sample_nodes": k,
         "phases_ms_per_tick": {ph: v / steps for ph, v in st["ms"].items() if v > 0},

@@ -30,6 +30,7 @@ namespace {
 enum BatchStatus : int32_t { B_OK = 0, B_BAD_HANDLE = -1, B_OUT_OF_MEMORY = -2, B_BAD_ARGUMENT = -3, B_FAILED = -4 };
 
 int32_t bstatus(hipError_t e) {
+  publish_failure(e);
   if (e == hipSuccess) return B_OK;
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return B_OUT_OF_MEMORY;
   return B_FAILED;

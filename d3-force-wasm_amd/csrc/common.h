@@ -60,10 +60,27 @@ struct Frame {
 };
 constexpr uint32_t DEEP_STACK = 3 * MAX_DEPTH + 64;
 
-#define HIPCHECK(expr)                          \
-  do {                                          \
-    hipError_t e_ = (expr);                     \
-    if (e_ != hipSuccess) return e_;            \
+// Notes the first failure since the last status the library returned: the
+// HIP error, what failed and its source line (reported by core_last_error).
+void note_failure(hipError_t e, const char* what, const char* file, int line);
+// Publishes the noted failure, or names e alone when none was noted; clears
+// the note on success.
+void publish_failure(hipError_t e);
+
+#define HIPCHECK(expr)                                         \
+  do {                                                         \
+    hipError_t e_ = (expr);                                    \
+    if (e_ != hipSuccess) {                                    \
+      ::d3f::note_failure(e_, #expr, __FILE__, __LINE__);      \
+      return e_;                                               \
+    }                                                          \
+  } while (0)
+
+// Returns err after noting `why` as its cause.
+#define HIPFAIL(err, why)                                      \
+  do {                                                         \
+    ::d3f::note_failure((err), (why), __FILE__, __LINE__);     \
+    return (err);                                              \
   } while (0)
 
 __host__ __device__ inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }

@@ -513,7 +513,10 @@ __device__ inline uint32_t block_inclusive(uint32_t v, uint32_t* wsum, uint32_t&
 
 // One round of fmm.zig:400-453 over the frontier fp->front[step]. A block
 // classifies a tile of STEP_TILE pairs, scans its output counts and reserves
-// room in each list with one atomic per list and tile.
+// room in each list with one atomic per list and tile. A frontier that
+// overflowed its buffer voids the pass (it is redone after the buffers grow,
+// or the tick aborts): its counts include pairs that were never written, so
+// the rounds after it read nothing.
 __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cells, FmmParams* __restrict__ fp,
                                                    uint32_t step, const uint2* __restrict__ in,
                                                    uint2* __restrict__ out, uint32_t front_cap,
@@ -522,7 +525,7 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
   __shared__ uint32_t wsum[3][EWF / 64];
   __shared__ uint32_t base_sh[3];
   __shared__ unsigned long long pairs_sh[EWF / 64];
-  if (fp->abort) return;
+  if (fp->abort || (fp->list_overflow & 1u)) return;
   uint32_t cnt = fp->front[step];
   if (cnt > front_cap) cnt = front_cap;
   for (uint32_t tile = blockIdx.x * STEP_TILE; tile < cnt; tile += gridDim.x * STEP_TILE) {
@@ -1383,7 +1386,8 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
     maxd = 0;
     for (uint32_t d = 0; d <= FMM_LEVELS; ++d)
       if (P.level_n[d]) maxd = d;
-    if (nc > f.cell_cap || nc > f.cellarr_cap || (size_t)nc * Q > f.exp_cap) return hipErrorInvalidValue;
+    if (nc > f.cell_cap || nc > f.cellarr_cap || (size_t)nc * Q > f.exp_cap)
+      HIPFAIL(hipErrorInvalidValue, "cells beyond the reserved arrays");
   }
   auto grid = [&](uint32_t count, uint32_t per, uint32_t cap_blocks) -> uint32_t {
     if (g) return cap_blocks;
@@ -1416,8 +1420,10 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
     if (g) break;
     HIPCHECK(hipMemcpyAsync(f.params_host, f.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
-    if (P.front[steps] != 0) return hipErrorUnknown;  // the rounds must drain the recursion
-    if (!P.list_overflow) break;
+    if (!P.list_overflow) {
+      if (P.front[steps] != 0) HIPFAIL(hipErrorUnknown, "dual traversal not drained");  // the rounds must drain the recursion
+      break;
+    }
     uint32_t fmax = 0;
     for (uint32_t s = 0; s <= steps; ++s) fmax = std::max(fmax, P.front[s]);
     if (P.list_overflow & 1u) {
@@ -1592,7 +1598,7 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
   const uint32_t n = a.n;
   uint32_t p = a.order;  // fmm.zig:258-261
   if (p < 1 || p > FMM_MAX_ORDER) p = 8;
-  if (a.graphs == 0 || a.node_off[a.graphs] != n) return hipErrorInvalidValue;
+  if (a.graphs == 0 || a.node_off[a.graphs] != n) HIPFAIL(hipErrorInvalidValue, "graph offsets do not cover n");
   HIPCHECK(tree_reserve(t, n > 0 ? n : 1));
   HIPCHECK(reserve_points(f, n));
   HIPCHECK(reserve_forest(f, a, stream));

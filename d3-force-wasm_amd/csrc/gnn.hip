@@ -457,6 +457,7 @@ std::vector<Net*> g_nets;
 Net* nget(uint32_t h) { return h == 0 || h > g_nets.size() ? nullptr : g_nets[h - 1]; }
 
 int32_t nstatus(hipError_t e) {
+  publish_failure(e);
   if (e == hipSuccess) return 0;
   return e == hipErrorOutOfMemory ? -2 : -4;
 }

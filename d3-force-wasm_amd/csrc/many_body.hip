@@ -331,7 +331,7 @@ hipError_t mb_accumulate(TreeBuffers& t, const double* x, const double* y, const
   const uint32_t len = t.meta_host->len;
   if (len == 0) return hipSuccess;
   const uint32_t nbins = t.meta_host->depth + 1;
-  if (nbins + 1 > (uint32_t)DEPTH_BINS) return hipErrorInvalidValue;
+  if (nbins + 1 > (uint32_t)DEPTH_BINS) HIPFAIL(hipErrorInvalidValue, "tree deeper than the depth bins");
   const size_t lds = (size_t)nbins * sizeof(uint32_t);
   HIPCHECK(hipMemsetAsync(t.dbins, 0, lds, stream));
   uint32_t hist_blocks = ceil_div(len, ACC_THREADS * 8);

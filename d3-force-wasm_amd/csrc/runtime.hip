@@ -9,6 +9,7 @@
 // written them (any force_ptr or force_set_count call).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -20,6 +21,30 @@
 #define EXPORT extern "C" __attribute__((visibility("default")))
 
 namespace d3f {
+
+namespace {
+thread_local char g_noted[256];
+thread_local bool g_has_note = false;
+thread_local char g_last_error[256];
+}  // namespace
+
+void note_failure(hipError_t e, const char* what, const char* file, int line) {
+  if (g_has_note) return;  // the first failure is the cause; its callers pass it on
+  const char* base = std::strrchr(file, '/');
+  std::snprintf(g_noted, sizeof g_noted, "%s at %s:%d: %s", hipGetErrorName(e), base ? base + 1 : file, line, what);
+  g_has_note = true;
+}
+
+void publish_failure(hipError_t e) {
+  if (e != hipSuccess) {
+    if (g_has_note)
+      std::memcpy(g_last_error, g_noted, sizeof g_last_error);
+    else
+      std::snprintf(g_last_error, sizeof g_last_error, "%s", hipGetErrorName(e));
+  }
+  g_has_note = false;
+}
+
 namespace {
 
 enum Status : int32_t { OK = 0, GROWN = 1, BAD_HANDLE = -1, OUT_OF_MEMORY = -2, BAD_ARGUMENT = -3, WORKER_FAILED = -4 };
@@ -36,6 +61,7 @@ double draw() { return g_random ? g_random(g_random_ctx) : 0.5; }
 double jiggle_host(double r) { return (r - 0.5) * 1e-6; }  // rng.zig:7-11
 
 int32_t status_of(hipError_t e) {
+  publish_failure(e);
   if (e == hipSuccess) return OK;
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return OUT_OF_MEMORY;
   return WORKER_FAILED;
@@ -726,6 +752,8 @@ EXPORT void core_set_random(d3f_random_fn fn, void* ctx) {
 EXPORT void core_set_timing(int on) { g_timing = on != 0; }
 
 EXPORT int32_t core_synchronize(void) { return status_of(hipDeviceSynchronize()); }
+
+EXPORT const char* core_last_error(void) { return g_last_error; }
 
 EXPORT uint32_t sim_create(void) {  // root.zig:65-78
   int devices = 0;

@@ -768,7 +768,7 @@ __global__ __launch_bounds__(EW_THREADS) void k_emit(const uint64_t* __restrict_
 hipError_t sort_digits(uint32_t* hist, uint32_t* glob, uint64_t* keys, uint32_t* vals, uint64_t* keys_tmp,
                        uint32_t* vals_tmp, uint32_t n_max, const uint32_t* count, int first_digit, int digits,
                        hipStream_t stream) {
-  if (digits < 0 || digits > 8) return hipErrorInvalidValue;
+  if (digits < 0 || digits > 8) HIPFAIL(hipErrorInvalidValue, "radix digits out of range");
   const uint32_t ntiles = ceil_div(n_max > 0 ? n_max : 1, RS_TILE);
   uint64_t *ka = keys, *kb = keys_tmp;
   uint32_t *va = vals, *vb = vals_tmp;
@@ -896,7 +896,7 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipMemcpyAsync(t.meta_host, t.meta, sizeof(TreeMeta), hipMemcpyDeviceToHost, stream));
   HIPCHECK(hipStreamSynchronize(stream));
-  if (t.meta_host->overflow) return hipErrorOutOfMemory;
+  if (t.meta_host->overflow) HIPFAIL(hipErrorOutOfMemory, "tree arrays overflowed");
   if (!t.meta_host->valid) {
     t.meta_host->len = 0;
     return hipSuccess;

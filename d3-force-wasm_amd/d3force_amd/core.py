@@ -50,9 +50,15 @@ class CoreError(RuntimeError):
 
 
 def check(status):
-    """core.js:45-48: throw on a negative status, else pass it through."""
+    """core.js:45-48: throw on a negative status, else pass it through. A GPU
+    failure carries the library's account of its cause (core_last_error)."""
     if status < 0:
-        raise CoreError(status)
+        err = CoreError(status)
+        if status in (OUT_OF_MEMORY, WORKER_FAILED) and _gpu_core is not None and _gpu_core.last_error:
+            cause = _gpu_core.last_error()
+            if cause:
+                err.args = ("%s: %s" % (err.args[0], cause.decode(errors="replace")),)
+        raise err
     return status
 
 
@@ -127,6 +133,9 @@ class Core:
             self.sim_stats = self.bind("sim_stats", ctypes.c_int32, [ctypes.c_uint32, ctypes.POINTER(Stats)])
             self.core_set_timing = self.bind("core_set_timing", None, [ctypes.c_int])
             self.core_synchronize = self.bind("core_synchronize", ctypes.c_int32, [])
+        self.last_error = None
+        if hasattr(self.lib, prefix + "core_last_error"):
+            self.last_error = self.bind("core_last_error", ctypes.c_char_p, [])
         if hasattr(self.lib, prefix + "force_direct_sum"):
             self.force_direct_sum = self.bind("force_direct_sum", ctypes.c_int32,
                                               [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_void_p,

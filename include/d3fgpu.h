@@ -131,6 +131,11 @@ int32_t force_direct_sum(uint32_t h, uint32_t fh, double alpha, const uint32_t* 
 void core_set_timing(int on);
 /* Waits for all work the library queued on the device (benchmark brackets). */
 int32_t core_synchronize(void);
+/* The cause of the calling thread's last OUT_OF_MEMORY or WORKER_FAILED
+ * status (no reference counterpart; the wasm core has no such errors): the
+ * HIP error name, the library source line that failed and what failed there.
+ * Empty before the first failure. The string is owned by the library. */
+const char* core_last_error(void);
 
 /* ---- Batched fast mode (no reference counterpart) ----
  * Many independent simulations ("graphs") in one node arena and one link

@@ -251,6 +251,23 @@ def test_fmm_orders_and_openings(gpu_core, oracle_core, options):
     assert err < FP32_RMS
 
 
+@pytest.mark.parametrize("opening", [2.0, 4.0])
+def test_fmm_frontier_outgrows_its_first_buffer(gpu_core, oracle_core, opening):
+    # A wide opening keeps more cell pairs open per round than the first
+    # frontier buffer holds (16 per cell; about 25 and 100 same-level
+    # neighbours per cell stay open at openings 2 and 4): the pass that
+    # overflows is void, the buffers grow and the traversal is redone, with
+    # the oracle's lists.
+    print("fmm frontier, opening", opening, flush=True)
+    xs, ys = d3.simulation.phyllotaxis(100000)
+    a, sa = many_body_once(oracle_core, xs, ys, order=6, opening=opening)
+    b, sb = many_body_once(gpu_core, xs, ys, order=6, opening=opening)
+    assert fmm_counts(oracle_core, sa) == fmm_counts(gpu_core, sb)
+    err = rms(b, a)
+    print("  rms vs FP64 FMM %.2e" % err, flush=True)
+    assert err < FP32_RMS
+
+
 def test_fmm_distance_ring(gpu_core, oracle_core):
     # distanceMax finite and distanceMin 0 both change the opening test
     # (fmm.zig:423-431) and the pair term.
