@@ -75,30 +75,12 @@ def make_fast_workload(core, n, seed=1, order=8, opening=1.5):
 
 def fast_rms(core, nodes, sample_size=10000, seed=3, order=8, opening=1.5):
     """Relative RMS error of the fast many-body force on the current layout
-    against the library's FP64 direct sum, over a fixed sample of nodes: one
-    manyBody-only tick with velocityDecay 0 from v = 0 (the form of the
-    reference's test/fast/error-test.js), so v after it is the force."""
-    import ctypes
-    import numpy as np
-    import d3force_amd as d3
-    n = len(nodes)
-    probe = d3.Nodes(n, x=nodes.x.copy(), y=nodes.y.copy(), vx=np.zeros(n), vy=np.zeros(n))
-    charge = d3.forceManyBody()
-    sim = d3.forceSimulation(probe, core=core).velocity_decay(0).force("charge", charge).stop()
-    sim.precision("fast", {"order": order, "opening": opening})
-    sample = np.sort(np.random.default_rng(seed).choice(n, min(n, sample_size), replace=False)).astype(np.uint32)
-    ref = np.empty(2 * len(sample))
-    ctx = sim.begin()
-    charge.push_scalars()
-    from d3force_amd.core import check
-    check(core.force_direct_sum(ctx.h, charge.slot.fh, 1.0, sample.ctypes.data_as(ctypes.c_void_p), len(sample),
-                                ref.ctypes.data_as(ctypes.c_void_p)))
-    sim.iterate(1, ctx)
-    sim.end(ctx)
-    ref *= sim.alpha()  # the sum is linear in alpha; the tick used this alpha
-    fx, fy = ref[0::2], ref[1::2]
-    ex, ey = probe.vx[sample] - fx, probe.vy[sample] - fy
-    return float(np.sqrt(np.sum(ex * ex + ey * ey) / np.sum(fx * fx + fy * fy))), len(sample)
+    against the library's FP64 direct sum, over a fixed sample of nodes
+    (d3force_amd.accuracy; the form of the reference's
+    test/fast/error-test.js)."""
+    from d3force_amd.accuracy import fast_force_error
+    return fast_force_error(nodes.x, nodes.y, sample_size=sample_size, seed=seed, order=order, opening=opening,
+                            core=core)
 
 
 def run_fast(core, n, steps, warmup, order=8, opening=1.5):
@@ -320,6 +302,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fast-nodes", type=int, default=1000000, help="nodes of the fast-mode leg (0: skip it)")
     ap.add_argument("--fast-only", action="store_true", help="run only the fast-mode leg (prints its object)")
+    ap.add_argument("--fast-order", type=int, default=8, help="FMM order of the fast-only leg")
+    ap.add_argument("--fast-opening", type=float, default=1.5, help="FMM opening of the fast-only leg")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs of the batched leg (0: skip it)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched leg (prints its object)")
     ap.add_argument("--warm-nodes", type=int, default=100000, help="nodes of the warm-start graphs (0: skip)")
@@ -332,7 +316,8 @@ def main():
 
     import d3force_amd as d3
     if args.fast_only:
-        print(json.dumps(run_fast(d3.gpu_core(), args.fast_nodes, args.steps, args.warmup)))
+        print(json.dumps(run_fast(d3.gpu_core(), args.fast_nodes, args.steps, args.warmup, order=args.fast_order,
+                                  opening=args.fast_opening)))
         return
     if args.batch_only:
         print(json.dumps(run_batch(d3.gpu_core(), args.batch_graphs, args.steps, args.warmup)))
@@ -410,6 +395,13 @@ def main():
     roof = dict(kernels[dominant])
 
     fast = run_fast(core, args.fast_nodes, args.steps, args.warmup) if args.fast_nodes else None
+    # Order 8 measures ~1.7e-6 against the direct sum on this workload; order
+    # 10 is the lowest order within the metric's 1e-6.
+    fast_1e6 = None
+    if args.fast_nodes:
+        fast_1e6 = run_fast(core, args.fast_nodes, args.steps, args.warmup, order=10)
+        fast_1e6["note"] = ("the same workload at FMM order 10, the lowest order whose RMS force error against "
+                            "the FP64 direct sum is within 1e-6")
     batch = run_batch(core, args.batch_graphs, args.steps, args.warmup) if args.batch_graphs else None
     warm = run_warm(core, args.warm_nodes) if args.warm_nodes else None
 
@@ -442,6 +434,7 @@ def main():
         "rooflines": kernels,
         "cpu_baseline": cpu,
         "fast": fast,
+        "fast_rms_within_1e-6": fast_1e6,
         "batch": batch,
         "warm_start": warm,
     }
