@@ -167,15 +167,27 @@ hipError_t mb_traverse(const TravArgs& a, int mode, hipStream_t stream);
 hipError_t mb_nan_nodes(const TreeBuffers& t, uint32_t n, double* vx, double* vy, hipStream_t stream);
 
 // ---- link.hip ----
+// A node's position and velocity side by side, so an edge reads each endpoint
+// from one 32-byte record (the link pass's working copy).
+struct alignas(32) LinkNode {
+  double x, y, vx, vy;
+};
 struct LinkLevelArgs {
   const uint32_t *src, *tgt;
   const double *strength, *distance, *bias;
   const double *x, *y;
   double *vx, *vy;
+  LinkNode* nodes;  // n records
+  uint32_t n;
   double alpha;
   uint32_t* zero_flag;
 };
+// Copies x, y, vx, vy into the records.
+hipError_t link_pack(const LinkLevelArgs& a, hipStream_t stream);
 hipError_t link_level(const LinkLevelArgs& a, uint32_t lo, uint32_t hi, hipStream_t stream);
+// Copies the records' velocities back unless a level raised the zero flag
+// (then vx, vy keep their values from before the pass).
+hipError_t link_unpack(const LinkLevelArgs& a, hipStream_t stream);
 struct LinkSerialState {
   uint32_t edge;    // next edge; m when done
   uint32_t need;    // bit 0: x needs a draw, bit 1: y needs a draw

@@ -7,12 +7,15 @@
 // order gives every node its edges in user order with the same operands as
 // the serial pass. The schedule depends only on source/target and is built
 // when the links change (runtime.hip); edge arrays are stored level-major so
-// each level is one coalesced launch.
+// each level is one coalesced launch. The levels run on a packed copy of the
+// node state (x, y, vx, vy in one 32-byte record per node): an edge gathers
+// two records and writes two velocity pairs instead of touching twelve
+// separate words.
 //
 // A zero or NaN delta makes d3 draw from random in edge order
-// (link.zig:23-27). The level pass flags it; the host then restores the
-// velocities and runs k_link_serial, which stops before each such edge so the
-// host can draw in order and resume.
+// (link.zig:23-27). The level pass flags it and the packed velocities are not
+// copied back; the host then runs k_link_serial on the untouched velocities,
+// which stops before each such edge so the host can draw in order and resume.
 #include "common.h"
 
 namespace d3f {
@@ -20,16 +23,35 @@ namespace {
 
 constexpr int LINK_THREADS = 256;
 
+__global__ __launch_bounds__(LINK_THREADS) void k_link_pack(const double* __restrict__ xs,
+                                                             const double* __restrict__ ys,
+                                                             const double* __restrict__ vxs,
+                                                             const double* __restrict__ vys, uint32_t n,
+                                                             LinkNode* __restrict__ nodes) {
+  const uint32_t i = blockIdx.x * LINK_THREADS + threadIdx.x;
+  if (i >= n) return;
+  LinkNode r;
+  r.x = xs[i];
+  r.y = ys[i];
+  r.vx = vxs[i];
+  r.vy = vys[i];
+  nodes[i] = r;
+}
+
+// One dependency level (link.zig:14-38 per edge): the edges of a level share
+// no node. A self-link's delta, (x + vx) - x - vx, is a rounding residue that
+// need not be zero; its source update then applies to the updated target
+// velocity, as in the serial pass.
 __global__ __launch_bounds__(LINK_THREADS) void k_link_level(
     const uint32_t* __restrict__ src, const uint32_t* __restrict__ tgt, const double* __restrict__ strength,
     const double* __restrict__ distance, const double* __restrict__ bias, uint32_t lo, uint32_t hi,
-    const double* __restrict__ xs, const double* __restrict__ ys, double* __restrict__ vxs,
-    double* __restrict__ vys, double alpha, uint32_t* __restrict__ zero_flag) {
+    LinkNode* __restrict__ nodes, double alpha, uint32_t* __restrict__ zero_flag) {
   const uint32_t e = lo + blockIdx.x * LINK_THREADS + threadIdx.x;
   if (e >= hi) return;
   const uint32_t s = src[e], t = tgt[e];
-  double x = xs[t] + vxs[t] - xs[s] - vxs[s];
-  double y = ys[t] + vys[t] - ys[s] - vys[s];
+  const LinkNode a = nodes[s], c = nodes[t];
+  double x = c.x + c.vx - a.x - a.vx;
+  double y = c.y + c.vy - a.y - a.vy;
   if (x == 0 || x != x || y == 0 || y != y) {
     atomicOr(zero_flag, 1u);
     return;
@@ -39,11 +61,22 @@ __global__ __launch_bounds__(LINK_THREADS) void k_link_level(
   x *= l;
   y *= l;
   double b = bias[e];
-  vxs[t] -= x * b;
-  vys[t] -= y * b;
+  const double2 vt = make_double2(c.vx - x * b, c.vy - y * b);
   b = 1 - b;
-  vxs[s] += x * b;
-  vys[s] += y * b;
+  const double2 v0 = s == t ? vt : make_double2(a.vx, a.vy);
+  const double2 vs = make_double2(v0.x + x * b, v0.y + y * b);
+  if (s != t) *reinterpret_cast<double2*>(&nodes[t].vx) = vt;
+  *reinterpret_cast<double2*>(&nodes[s].vx) = vs;
+}
+
+__global__ __launch_bounds__(LINK_THREADS) void k_link_unpack(const LinkNode* __restrict__ nodes, uint32_t n,
+                                                               const uint32_t* __restrict__ zero_flag,
+                                                               double* __restrict__ vxs, double* __restrict__ vys) {
+  const uint32_t i = blockIdx.x * LINK_THREADS + threadIdx.x;
+  if (i >= n || *zero_flag) return;
+  const double2 v = *reinterpret_cast<const double2*>(&nodes[i].vx);
+  vxs[i] = v.x;
+  vys[i] = v.y;
 }
 
 // The serial pass from state->edge, stopping before an edge that needs a
@@ -90,10 +123,24 @@ __global__ void k_link_serial(const uint32_t* __restrict__ src, const uint32_t* 
 
 }  // namespace
 
+hipError_t link_pack(const LinkLevelArgs& a, hipStream_t stream) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_link_pack, dim3(ceil_div(a.n, LINK_THREADS)), dim3(LINK_THREADS), 0, stream, a.x, a.y, a.vx,
+                     a.vy, a.n, a.nodes);
+  return hipGetLastError();
+}
+
 hipError_t link_level(const LinkLevelArgs& a, uint32_t lo, uint32_t hi, hipStream_t stream) {
   if (hi <= lo) return hipSuccess;
   hipLaunchKernelGGL(k_link_level, dim3(ceil_div(hi - lo, LINK_THREADS)), dim3(LINK_THREADS), 0, stream, a.src,
-                     a.tgt, a.strength, a.distance, a.bias, lo, hi, a.x, a.y, a.vx, a.vy, a.alpha, a.zero_flag);
+                     a.tgt, a.strength, a.distance, a.bias, lo, hi, a.nodes, a.alpha, a.zero_flag);
+  return hipGetLastError();
+}
+
+hipError_t link_unpack(const LinkLevelArgs& a, hipStream_t stream) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_link_unpack, dim3(ceil_div(a.n, LINK_THREADS)), dim3(LINK_THREADS), 0, stream, a.nodes, a.n,
+                     a.zero_flag, a.vx, a.vy);
   return hipGetLastError();
 }
 

@@ -159,7 +159,8 @@ struct Sim {
   uint32_t n = 0, cap = 0;
   Pinned<double> x, y, vx, vy, fx, fy, snap;
   Pinned<uint8_t> fixed;
-  Device<double> dx, dy, dvx, dvy, dfx, dfy, dpx, dpy, dsvx, dsvy;
+  Device<double> dx, dy, dvx, dvy, dfx, dfy, dpx, dpy;
+  Device<LinkNode> dlnodes;  // the exact link pass's packed node state
   Device<uint8_t> dfixed;
   TreeBuffers tree;
   FmmBuffers fmm;
@@ -208,7 +209,8 @@ struct Sim {
     for (Force* f : forces) delete f;
     for (auto* a : {&x, &y, &vx, &vy, &fx, &fy, &snap}) a->release();
     fixed.release();
-    for (auto* a : {&dx, &dy, &dvx, &dvy, &dfx, &dfy, &dpx, &dpy, &dsvx, &dsvy}) a->release();
+    for (auto* a : {&dx, &dy, &dvx, &dvy, &dfx, &dfy, &dpx, &dpy}) a->release();
+    dlnodes.release();
     dfixed.release();
     tree_free(tree);
     fmm_free(fmm);
@@ -283,7 +285,7 @@ struct Sim {
     for (auto* a : {&x, &y, &vx, &vy, &fx, &fy}) HIPCHECK(a->reserve(c, old_n));
     HIPCHECK(fixed.reserve(c, old_n));
     for (auto* a : {&dx, &dy, &dvx, &dvy, &dfx, &dfy}) HIPCHECK(a->reserve(c, old_n, stream));
-    for (auto* a : {&dpx, &dpy, &dsvx, &dsvy}) HIPCHECK(a->reserve(c, 0, stream));
+    for (auto* a : {&dpx, &dpy}) HIPCHECK(a->reserve(c, 0, stream));
     HIPCHECK(dfixed.reserve(c, old_n, stream));
     HIPCHECK(flagged.reserve(c, 0, stream));
     HIPCHECK(order.reserve(c, 0, stream));
@@ -556,28 +558,27 @@ int32_t run_link_exact(Sim& s, Force& f, double alpha) {
   const double iterations = f.scalar(S_ITERATIONS);
   if (m == 0 || !(0 < iterations)) return OK;
   hipError_t e;
-  const size_t bytes = (size_t)s.n * sizeof(double);
-  if ((e = hipMemcpyAsync(s.dsvx.p, s.dvx.p, bytes, hipMemcpyDeviceToDevice, s.stream)) != hipSuccess) return status_of(e);
-  if ((e = hipMemcpyAsync(s.dsvy.p, s.dvy.p, bytes, hipMemcpyDeviceToDevice, s.stream)) != hipSuccess) return status_of(e);
+  if ((e = s.dlnodes.reserve(s.n > 0 ? s.n : 1, 0, s.stream)) != hipSuccess) return status_of(e);
   if ((e = hipMemsetAsync(s.dscalar_u + 1, 0, sizeof(uint32_t), s.stream)) != hipSuccess) return status_of(e);
-  LinkLevelArgs a{f.lsrc.p, f.ltgt.p, f.lf[0].p, f.lf[1].p, f.lf[2].p, s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, alpha,
-                  s.dscalar_u + 1};
+  LinkLevelArgs a{f.lsrc.p, f.ltgt.p, f.lf[0].p, f.lf[1].p, f.lf[2].p, s.dx.p, s.dy.p, s.dvx.p, s.dvy.p,
+                  s.dlnodes.p, s.n, alpha, s.dscalar_u + 1};
   const uint32_t levels = (uint32_t)f.level_off.size() - 2;
   s.timer_start(D3F_PHASE_LINK);
+  if ((e = link_pack(a, s.stream)) != hipSuccess) return status_of(e);
   for (double k = 0; k < iterations; k += 1) {
     for (uint32_t l = 1; l <= levels; ++l) {
       if ((e = link_level(a, f.level_off[l], f.level_off[l + 1], s.stream)) != hipSuccess) return status_of(e);
     }
   }
+  if ((e = link_unpack(a, s.stream)) != hipSuccess) return status_of(e);
   if ((e = hipMemcpyAsync(s.hscalar_u + 1, s.dscalar_u + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream)) !=
       hipSuccess)
     return status_of(e);
   s.timer_stop();
   if ((e = hipStreamSynchronize(s.stream)) != hipSuccess) return status_of(e);
   if (s.hscalar_u[1] == 0) return OK;
-  // A zero or NaN delta: d3 draws in edge order. Restore and run serially.
-  if ((e = hipMemcpyAsync(s.dvx.p, s.dsvx.p, bytes, hipMemcpyDeviceToDevice, s.stream)) != hipSuccess) return status_of(e);
-  if ((e = hipMemcpyAsync(s.dvy.p, s.dsvy.p, bytes, hipMemcpyDeviceToDevice, s.stream)) != hipSuccess) return status_of(e);
+  // A zero or NaN delta: d3 draws in edge order. vx, vy still hold their
+  // values from before the pass; run it serially.
   for (double k = 0; k < iterations; k += 1) {
     LinkSerialState* st = s.hlink;
     st->edge = 0;
