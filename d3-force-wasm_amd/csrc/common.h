@@ -141,9 +141,12 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t n_max, const uin
                     uint32_t* total_out, hipStream_t stream);
 
 // ---- many_body.hip ----
-// visitAfter charge sums (many_body.zig:101-135) into cells and tcells.
+// visitAfter charge sums (many_body.zig:101-135) into tcells (qx, qy, value).
 hipError_t mb_accumulate(TreeBuffers& t, const double* x, const double* y, const double* strengths, double theta2,
                          hipStream_t stream);
+// Copies the accumulated charges of every cell from tcells into the 88-byte
+// cells' x, y, value (what the tree export shows after a many-body run).
+hipError_t mb_publish_charges(TreeBuffers& t, hipStream_t stream);
 enum TravMode { TRAV_RECORD = 0, TRAV_COUNT = 1, TRAV_REPLAY = 2 };
 struct TravArgs {
   const TCell* tc;

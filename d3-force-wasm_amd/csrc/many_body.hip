@@ -32,8 +32,10 @@ constexpr int TRAV_WAVES = TRAV_THREADS / 64;
 
 __device__ inline bool truthy(double v) { return v != 0 && v == v; }
 
-__device__ inline void put(TCell* __restrict__ tc, Cell* __restrict__ cells, uint32_t c, const Cell& cell, double qx,
-                           double qy, double value, double theta2, uint32_t head, uint32_t flags) {
+// A cell's traversal record. The 88-byte cell's x, y, value receive the same
+// charges only when the tree is exported (k_publish_charges).
+__device__ inline void put(TCell* __restrict__ tc, uint32_t c, const Cell& cell, double qx, double qy, double value,
+                           double theta2, uint32_t head, uint32_t flags) {
   TCell t;
   t.qx = qx;
   t.qy = qy;
@@ -46,9 +48,15 @@ __device__ inline void put(TCell* __restrict__ tc, Cell* __restrict__ cells, uin
   t.head = head;
   t.flags = flags;
   tc[c] = t;
-  cells[c].x = qx;
-  cells[c].y = qy;
-  cells[c].value = value;
+}
+
+__global__ __launch_bounds__(ACC_THREADS) void k_publish_charges(Cell* __restrict__ cells,
+                                                                 const TCell* __restrict__ tc, uint32_t len) {
+  const uint32_t c = blockIdx.x * ACC_THREADS + threadIdx.x;
+  if (c >= len) return;
+  cells[c].x = tc[c].qx;
+  cells[c].y = tc[c].qy;
+  cells[c].value = tc[c].value;
 }
 
 // Cells grouped by depth (a counting sort): histogram, scan, scatter. The
@@ -82,30 +90,44 @@ __global__ void k_depth_scan(uint32_t* __restrict__ bins, uint32_t nbins, uint32
   start[nbins] = run;
 }
 
+// A block places SCATTER_PER cells per thread with one global atomic per
+// depth bin (the bins are few, so per-cell-block atomics would contend).
+constexpr int SCATTER_PER = 16;
+
 __global__ __launch_bounds__(ACC_THREADS) void k_depth_scatter(const uint32_t* __restrict__ cdepth, uint32_t len,
                                                                uint32_t nbins, uint32_t* __restrict__ cursor,
                                                                uint32_t* __restrict__ order) {
   extern __shared__ uint32_t h[];  // [nbins] local counts, then local bases
-  const uint32_t c = blockIdx.x * ACC_THREADS + threadIdx.x;
+  const uint32_t base = blockIdx.x * ACC_THREADS * SCATTER_PER + threadIdx.x;
   for (uint32_t b = threadIdx.x; b < nbins; b += ACC_THREADS) h[b] = 0;
   __syncthreads();
-  uint32_t d = 0, local = 0;
-  if (c < len) {
-    d = cdepth[c];
-    local = atomicAdd(&h[d], 1u);
+  uint32_t d[SCATTER_PER], local[SCATTER_PER];
+#pragma unroll
+  for (int q = 0; q < SCATTER_PER; ++q) {
+    const uint32_t c = base + q * ACC_THREADS;
+    d[q] = 0;
+    local[q] = 0;
+    if (c < len) {
+      d[q] = cdepth[c];
+      local[q] = atomicAdd(&h[d[q]], 1u);
+    }
   }
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < nbins; b += ACC_THREADS)
     if (h[b]) h[b] = atomicAdd(&cursor[b], h[b]);
   __syncthreads();
-  if (c < len) order[h[d] + local] = c;
+#pragma unroll
+  for (int q = 0; q < SCATTER_PER; ++q) {
+    const uint32_t c = base + q * ACC_THREADS;
+    if (c < len) order[h[d[q]] + local[q]] = c;
+  }
 }
 
 // One depth of visitAfter: leaves sum their chain from the head down
 // (many_body.zig:125-132); internal cells combine children 0..3, whose depth
 // was done by the previous launch (many_body.zig:112-121).
 __global__ __launch_bounds__(ACC_THREADS) void k_accumulate_level(
-    Cell* __restrict__ cells, TCell* __restrict__ tc, const uint32_t* __restrict__ order,
+    const Cell* __restrict__ cells, TCell* __restrict__ tc, const uint32_t* __restrict__ order,
     const uint32_t* __restrict__ start, uint32_t depth, const uint32_t* __restrict__ perm,
     const double* __restrict__ xs, const double* __restrict__ ys, const double* __restrict__ s, double theta2) {
   const uint32_t lo = start[depth], hi = start[depth + 1];
@@ -119,7 +141,7 @@ __global__ __launch_bounds__(ACC_THREADS) void k_accumulate_level(
         --j;
         strength += s[perm[j]];
       }
-      put(tc, cells, c, cell, xs[head], ys[head], strength, theta2, head,
+      put(tc, c, cell, xs[head], ys[head], strength, theta2, head,
           TC_LEAF | (truthy(strength) ? TC_REACH : 0));
       continue;
     }
@@ -138,7 +160,7 @@ __global__ __launch_bounds__(ACC_THREADS) void k_accumulate_level(
       }
       reach = reach || (tc[ch].flags & TC_REACH);
     }
-    put(tc, cells, c, cell, x / weight, y / weight, st, theta2, NONE, (reach && truthy(st)) ? TC_REACH : 0);
+    put(tc, c, cell, x / weight, y / weight, st, theta2, NONE, (reach && truthy(st)) ? TC_REACH : 0);
   }
 }
 
@@ -338,14 +360,22 @@ hipError_t mb_accumulate(TreeBuffers& t, const double* x, const double* y, const
   if (hist_blocks > 1024) hist_blocks = 1024;
   hipLaunchKernelGGL(k_depth_hist, dim3(hist_blocks), dim3(ACC_THREADS), lds, stream, t.cdepth, len, nbins, t.dbins);
   hipLaunchKernelGGL(k_depth_scan, dim3(1), dim3(64), 0, stream, t.dbins, nbins, t.dstart);
-  hipLaunchKernelGGL(k_depth_scatter, dim3(ceil_div(len, ACC_THREADS)), dim3(ACC_THREADS), lds, stream, t.cdepth,
-                     len, nbins, t.dbins, t.dorder);
+  hipLaunchKernelGGL(k_depth_scatter, dim3(ceil_div(len, ACC_THREADS * SCATTER_PER)), dim3(ACC_THREADS), lds, stream,
+                     t.cdepth, len, nbins, t.dbins, t.dorder);
   uint32_t level_blocks = ceil_div(len, ACC_THREADS);
   if (level_blocks > 2048) level_blocks = 2048;
   for (uint32_t d = nbins; d-- > 0;) {
     hipLaunchKernelGGL(k_accumulate_level, dim3(level_blocks), dim3(ACC_THREADS), 0, stream, t.cells, t.tcells,
                        t.dorder, t.dstart, d, t.perm, x, y, strengths, theta2);
   }
+  return hipGetLastError();
+}
+
+hipError_t mb_publish_charges(TreeBuffers& t, hipStream_t stream) {
+  const uint32_t len = t.meta_host->len;
+  if (len == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_publish_charges, dim3(ceil_div(len, ACC_THREADS)), dim3(ACC_THREADS), 0, stream, t.cells,
+                     t.tcells, len);
   return hipGetLastError();
 }
 

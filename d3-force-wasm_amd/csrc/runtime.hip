@@ -186,6 +186,7 @@ struct Sim {
   hipEvent_t ev_x_ready = nullptr, ev_sums_done = nullptr;
   double* dsums_side = nullptr;
   CenterWork cw_side, cw_main;  // workspaces of the sums on each stream
+  bool charges_pending = false;  // tcells hold many-body charges the cells' x, y, value lack
   bool side_valid = false;   // dsums_side holds the sums of the current x, y
   bool side_busy = false;    // the side stream may still be reading x, y
   uint32_t centers = 0;      // center forces of this simulation
@@ -433,6 +434,7 @@ hipError_t upload(Sim& s, Force& f) {
 int32_t run_many_body_exact(Sim& s, Force& f, double alpha) {
   TreeBuffers& t = s.tree;
   s.timer_start(D3F_PHASE_TREE);
+  s.charges_pending = false;
   hipError_t e = tree_build_exact(t, s.dx.p, s.dy.p, s.n, s.stream);
   if (e != hipSuccess) return status_of(e);
   s.timer_stop();
@@ -440,6 +442,7 @@ int32_t run_many_body_exact(Sim& s, Force& f, double alpha) {
   s.timer_start(D3F_PHASE_ACCUMULATE);
   e = mb_accumulate(t, s.dx.p, s.dy.p, f.df[0].p, f.scalar(S_THETA2), s.stream);
   if (e != hipSuccess) return status_of(e);
+  s.charges_pending = true;
   s.timer_stop();
   TravArgs a{};
   a.tc = t.tcells;
@@ -727,8 +730,14 @@ hipError_t refresh_tree_host(Sim& s) {
     for (int k = 0; k < 4; ++k) s.hextent[k] = __builtin_nan("");
     return hipSuccess;
   }
+  HIPCHECK(hipStreamSynchronize(s.stream));
   HIPCHECK(hipMemcpy(t.meta_host, t.meta, sizeof(TreeMeta), hipMemcpyDeviceToHost));
   if (!t.meta_host->valid) t.meta_host->len = 0;
+  if (s.charges_pending) {  // the last many-body run's charges, as the reference's cells hold them
+    HIPCHECK(mb_publish_charges(t, s.stream));
+    HIPCHECK(hipStreamSynchronize(s.stream));
+    s.charges_pending = false;
+  }
   for (int k = 0; k < 4; ++k) s.hextent[k] = t.meta_host->extent[k];
   const uint32_t len = t.meta_host->len, m = t.meta_host->m;
   HIPCHECK(s.hcells.reserve(len > 0 ? len : 1, 0));
@@ -877,6 +886,7 @@ EXPORT int32_t tree_build(uint32_t h, uint32_t shifted) {  // root.zig:102-108
   const void* c0 = s->hcells.p;
   const void* p0 = s->hperm.p;
   hipError_t e;
+  s->charges_pending = false;
   if (shifted) {
     e = shift_positions(s->dx.p, s->dy.p, s->dvx.p, s->dvy.p, s->dpx.p, s->dpy.p, s->n, s->stream);
     if (e == hipSuccess) e = tree_build_exact(s->tree, s->dpx.p, s->dpy.p, s->n, s->stream);

@@ -75,3 +75,30 @@ def test_gpu_tree_bit_identical_to_oracle(gpu_core, oracle_core, name):
     _same_cells(a["cells"], b["cells"])
     assert np.array_equal(a["perm"], b["perm"])
     assert a["visit"] == b["visit"]
+
+
+@pytest.mark.parametrize("name", ["uniform", "clusters"])
+def test_gpu_tree_export_after_many_body_tick(gpu_core, oracle_core, name):
+    # After an exact many-body tick the exported cells carry the
+    # accumulated charges (x, y, value of many_body.zig:101-135), every
+    # 88-byte record bit-identical to the oracle's.
+    import d3force_amd as d3
+    from treeview import read_tree
+    print("tree export after a many-body tick", name, flush=True)
+    rng = np.random.default_rng(7)
+    if name == "uniform":
+        xs, ys = rng.uniform(-300, 300, 5000), rng.uniform(-300, 300, 5000)
+    else:
+        c = rng.integers(0, 5, 5000)
+        xs, ys = c * 100.0 + rng.normal(0, 3, 5000), c * -40.0 + rng.normal(0, 3, 5000)
+    out = []
+    for core in (oracle_core, gpu_core):
+        nodes = d3.Nodes(len(xs), x=xs.copy(), y=ys.copy())
+        sim = d3.forceSimulation(nodes, core=core).force("charge", d3.forceManyBody()).stop()
+        sim.tick(2)
+        out.append(read_tree(core, d3.core.context(nodes).h))
+    (ea, ca, pa, ma), (eb, cb, pb, mb) = out
+    assert ea == eb and ma == mb
+    assert np.array_equal(pa, pb)
+    assert np.any(ca["value"] != 0)
+    _same_cells(ca, cb)
