@@ -76,7 +76,8 @@ struct Batch {
   Pinned<double> x, y, vx, vy, charge, lstrength, ldistance, lbias;
   Pinned<uint32_t> source, target;
   Pinned<uint64_t> seed_ids, seeds_mb, seeds_link;
-  Device<double> dx, dy, dvx, dvy, dcharge, dls, dld, dlb, dpx, dpy, dzero, dsums;
+  Device<double> dx, dy, dvx, dvy, dcharge, dls, dld, dlb, dzero, dsums;
+  Device<double2> dpos;  // x + vx, y + vy as pairs (link sweep)
   Device<uint8_t> dfixed;
   Device<uint32_t> dsrc, dtgt, dnode_graph, dnode_off, dlink_off;
   Device<uint64_t> dseeds_link, dseeds_mb, dseed_ids;
@@ -108,7 +109,8 @@ struct Batch {
     for (auto* a : {&x, &y, &vx, &vy, &charge, &lstrength, &ldistance, &lbias}) a->release();
     for (auto* a : {&source, &target}) a->release();
     for (auto* a : {&seed_ids, &seeds_mb, &seeds_link}) a->release();
-    for (auto* a : {&dx, &dy, &dvx, &dvy, &dcharge, &dls, &dld, &dlb, &dpx, &dpy, &dzero, &dsums, &terms}) a->release();
+    for (auto* a : {&dx, &dy, &dvx, &dvy, &dcharge, &dls, &dld, &dlb, &dzero, &dsums, &terms}) a->release();
+    dpos.release();
     dfixed.release();
     for (auto* a : {&dsrc, &dtgt, &dnode_graph, &dnode_off, &dlink_off, &row, &adj, &owner, &rows_long, &sv0, &sv1,
                     &shist, &sglob, &scount})
@@ -170,7 +172,8 @@ struct Batch {
     for (auto* a : {&source, &target}) HIPCHECK(a->reserve(cm, 0));
     for (auto* a : {&seed_ids, &seeds_mb, &seeds_link}) HIPCHECK(a->reserve(cg, 0));
     for (uint32_t k = 0; k < g; ++k) seed_ids.p[k] = k;
-    for (auto* a : {&dx, &dy, &dvx, &dvy, &dcharge, &dpx, &dpy, &dzero}) HIPCHECK(a->reserve(cn, 0, stream));
+    for (auto* a : {&dx, &dy, &dvx, &dvy, &dcharge, &dzero}) HIPCHECK(a->reserve(cn, 0, stream));
+    HIPCHECK(dpos.reserve(cn, 0, stream));
     for (auto* a : {&dls, &dld, &dlb}) HIPCHECK(a->reserve(cm, 0, stream));
     HIPCHECK(dsums.reserve(2 * cg, 0, stream));
     HIPCHECK(dfixed.reserve(cn, 0, stream));
@@ -270,8 +273,8 @@ struct Batch {
     // link
     ev = timed ? mark() : nullptr;
     for (double k = 0; k < scalars[D3F_BS_LINK_ITERATIONS]; k += 1) {
-      HIPCHECK(shift_positions(dx.p, dy.p, dvx.p, dvy.p, dpx.p, dpy.p, n, stream));
-      HIPCHECK(link_sweep(csr_args(), long_count, dpx.p, dpy.p, dvx.p, dvy.p, scalars[D3F_BS_ALPHA], 0, stream,
+      HIPCHECK(shift_pack(dx.p, dy.p, dvx.p, dvy.p, dpos.p, n, stream));
+      HIPCHECK(link_sweep(csr_args(), long_count, dpos.p, dvx.p, dvy.p, scalars[D3F_BS_ALPHA], 0, stream,
                           &dstate->alpha));
     }
     timer(D3F_PHASE_LINK, ev);

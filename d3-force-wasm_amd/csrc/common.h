@@ -247,10 +247,10 @@ struct LinkCsrArgs {
 // entries; *long_count (host) receives the number of rows longer than the
 // per-thread limit. Synchronises.
 hipError_t link_csr(const LinkCsrArgs& a, const SortBuffers& sb, uint32_t* long_count, hipStream_t stream);
-// One Jacobi sweep over the rows on px = x + vx (link_fast.zig:60-99).
+// One Jacobi sweep over the rows on pos = (x + vx, y + vy) (link_fast.zig:60-99).
 // alpha_dev, when given, replaces alpha (read on the device, for captured graphs).
-hipError_t link_sweep(const LinkCsrArgs& a, uint32_t long_count, const double* px, const double* py, double* vx,
-                      double* vy, double alpha, uint64_t seed, hipStream_t stream, const double* alpha_dev = nullptr);
+hipError_t link_sweep(const LinkCsrArgs& a, uint32_t long_count, const double2* pos, double* vx, double* vy,
+                      double alpha, uint64_t seed, hipStream_t stream, const double* alpha_dev = nullptr);
 // One iteration of the uniform-grid collide (grid.zig:221-285).
 hipError_t grid_collide_iteration(const double* x, const double* y, double* vx, double* vy, const double* radii,
                                   uint32_t n, double strength, uint64_t seed, const GridBuffers& g,
@@ -403,6 +403,10 @@ hipError_t radial_force(const double* x, const double* y, double* vx, double* vy
                         const double* radius, uint32_t n, double cx, double cy, double alpha, hipStream_t stream);
 hipError_t shift_positions(const double* x, const double* y, const double* vx, const double* vy, double* px,
                            double* py, uint32_t n, hipStream_t stream);
+// pos[i] = (x + vx, y + vy): the shifted positions as pairs, so the fast link
+// sweep gathers an endpoint from one place.
+hipError_t shift_pack(const double* x, const double* y, const double* vx, const double* vy, double2* pos, uint32_t n,
+                      hipStream_t stream);
 // A batch of graphs: sums[2g], sums[2g + 1] = d3's serial sums of graph g's
 // x and y (nodes node_off[g] .. node_off[g + 1] - 1 in order), then
 // x -= (sx / n_g - cx) * strength per node (simple.zig:14-31 per graph).

@@ -52,12 +52,13 @@ __global__ __launch_bounds__(EW) void k_csr_rows(const uint64_t* __restrict__ ke
 // velocity adds: vx -= x * b on the target side is vx += -(x * b), which
 // rounds identically, so a row's terms can be formed in parallel and added
 // in row order. bf is b on the target side and 1 - b on the source side.
-__device__ inline void link_term(const LinkEntry& le, const double* __restrict__ px, const double* __restrict__ py,
-                                 double pxi, double pyi, double alpha, uint64_t seed, double& tx, double& ty) {
+__device__ inline void link_term(const LinkEntry& le, const double2* __restrict__ pos, double2 pi, double alpha,
+                                 uint64_t seed, double& tx, double& ty) {
   const uint32_t e = le.e & ~TARGET_BIT;
   const bool is_target = (le.e & TARGET_BIT) != 0;
-  double x = is_target ? pxi - px[le.j] : px[le.j] - pxi;
-  double y = is_target ? pyi - py[le.j] : py[le.j] - pyi;
+  const double2 pj = pos[le.j];
+  double x = is_target ? pi.x - pj.x : pj.x - pi.x;
+  double y = is_target ? pi.y - pj.y : pj.y - pi.y;
   if (x == 0 || x != x) x = hash_jiggle(seed, e, 0, 0);
   if (y == 0 || y != y) y = hash_jiggle(seed, e, 0, 1);
   double l = __builtin_sqrt(x * x + y * y);
@@ -98,11 +99,11 @@ __global__ __launch_bounds__(EW) void k_csr_pack(uint32_t total, const uint64_t*
   owner[a] = i;
 }
 
-// Phase 1 of a sweep (link_fast.zig:60-99 on the snapshot px = x + vx):
-// every adjacency entry's term for its row's node, all in parallel.
+// Phase 1 of a sweep (link_fast.zig:60-99 on the snapshot pos = x + vx,
+// y + vy): every adjacency entry's term for its row's node, all in parallel.
 __global__ __launch_bounds__(EW) void k_link_terms(uint32_t total, const uint32_t* __restrict__ owner,
                                                    const LinkEntry* __restrict__ entries,
-                                                   const double* __restrict__ px, const double* __restrict__ py,
+                                                   const double2* __restrict__ pos,
                                                    double2* __restrict__ terms, double alpha, uint64_t seed,
                                                    const uint32_t* __restrict__ node_graph,
                                                    const uint64_t* __restrict__ seeds,
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(EW) void k_link_terms(uint32_t total, const uint32_
   const uint32_t i = owner[a];
   if (node_graph) seed = seeds[node_graph[i]];
   double tx, ty;
-  link_term(entries[a], px, py, px[i], py[i], alpha, seed, tx, ty);
+  link_term(entries[a], pos, pos[i], alpha, seed, tx, ty);
   terms[a] = make_double2(tx, ty);
 }
 
@@ -421,11 +422,11 @@ hipError_t link_csr(const LinkCsrArgs& a, const SortBuffers& sb, uint32_t* long_
   return hipStreamSynchronize(stream);  // `total` and *long_count live on the host
 }
 
-hipError_t link_sweep(const LinkCsrArgs& a, uint32_t long_count, const double* px, const double* py, double* vx,
-                      double* vy, double alpha, uint64_t seed, hipStream_t stream, const double* alpha_dev) {
+hipError_t link_sweep(const LinkCsrArgs& a, uint32_t long_count, const double2* pos, double* vx, double* vy,
+                      double alpha, uint64_t seed, hipStream_t stream, const double* alpha_dev) {
   const uint32_t total = 2 * a.m;
   if (a.n == 0 || total == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_link_terms, dim3(ceil_div(total, EW)), dim3(EW), 0, stream, total, a.owner, a.entries, px, py,
+  hipLaunchKernelGGL(k_link_terms, dim3(ceil_div(total, EW)), dim3(EW), 0, stream, total, a.owner, a.entries, pos,
                      (double2*)a.terms, alpha, seed, a.node_graph, a.seeds, alpha_dev);
   hipLaunchKernelGGL(k_link_rows, dim3(ceil_div(a.n, EW)), dim3(EW), 0, stream, a.n, a.row, (const double2*)a.terms,
                      vx, vy);

@@ -160,6 +160,7 @@ struct Sim {
   Pinned<double> x, y, vx, vy, fx, fy, snap;
   Pinned<uint8_t> fixed;
   Device<double> dx, dy, dvx, dvy, dfx, dfy, dpx, dpy;
+  Device<double2> dpos;      // x + vx, y + vy as pairs (fast link sweep)
   Device<LinkNode> dlnodes;  // the exact link pass's packed node state
   Device<uint8_t> dfixed;
   TreeBuffers tree;
@@ -211,6 +212,7 @@ struct Sim {
     for (auto* a : {&x, &y, &vx, &vy, &fx, &fy, &snap}) a->release();
     fixed.release();
     for (auto* a : {&dx, &dy, &dvx, &dvy, &dfx, &dfy, &dpx, &dpy}) a->release();
+    dpos.release();
     dlnodes.release();
     dfixed.release();
     tree_free(tree);
@@ -287,6 +289,7 @@ struct Sim {
     HIPCHECK(fixed.reserve(c, old_n));
     for (auto* a : {&dx, &dy, &dvx, &dvy, &dfx, &dfy}) HIPCHECK(a->reserve(c, old_n, stream));
     for (auto* a : {&dpx, &dpy}) HIPCHECK(a->reserve(c, 0, stream));
+    HIPCHECK(dpos.reserve(c, 0, stream));
     HIPCHECK(dfixed.reserve(c, old_n, stream));
     HIPCHECK(flagged.reserve(c, 0, stream));
     HIPCHECK(order.reserve(c, 0, stream));
@@ -636,9 +639,8 @@ int32_t run_link_fast(Sim& s, Force& f, double alpha) {  // link_fast.zig:22-99
   const uint64_t seed = seed_of(f);
   s.timer_start(D3F_PHASE_LINK);
   for (double k = 0; k < iterations; k += 1) {
-    e = shift_positions(s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, s.dpx.p, s.dpy.p, n, s.stream);
-    if (e == hipSuccess)
-      e = link_sweep(f.csr_args(n), f.long_count, s.dpx.p, s.dpy.p, s.dvx.p, s.dvy.p, alpha, seed, s.stream);
+    e = shift_pack(s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, s.dpos.p, n, s.stream);
+    if (e == hipSuccess) e = link_sweep(f.csr_args(n), f.long_count, s.dpos.p, s.dvx.p, s.dvy.p, alpha, seed, s.stream);
     if (e != hipSuccess) return status_of(e);
   }
   s.timer_stop();

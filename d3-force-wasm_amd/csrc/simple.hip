@@ -559,6 +559,14 @@ __global__ __launch_bounds__(EW) void k_shift(const double* __restrict__ xs, con
   py[i] = ys[i] + vys[i];
 }
 
+__global__ __launch_bounds__(EW) void k_shift_pack(const double* __restrict__ xs, const double* __restrict__ ys,
+                                                   const double* __restrict__ vxs, const double* __restrict__ vys,
+                                                   double2* __restrict__ pos, uint32_t n) {
+  const uint32_t i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n) return;
+  pos[i] = make_double2(xs[i] + vxs[i], ys[i] + vys[i]);
+}
+
 }  // namespace
 
 hipError_t batch_center(double* x, double* y, const uint32_t* node_off, const uint32_t* node_graph, uint32_t graphs,
@@ -675,6 +683,13 @@ hipError_t shift_positions(const double* x, const double* y, const double* vx, c
                            double* py, uint32_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_shift, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, x, y, vx, vy, px, py, n);
+  return hipGetLastError();
+}
+
+hipError_t shift_pack(const double* x, const double* y, const double* vx, const double* vy, double2* pos, uint32_t n,
+                      hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_shift_pack, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, x, y, vx, vy, pos, n);
   return hipGetLastError();
 }
 
