@@ -137,49 +137,49 @@ __global__ __launch_bounds__(EW) void k_link_rows(uint32_t n, const uint32_t* __
   vys[i] = vy;
 }
 
-// A long row (a hub of the graph): the wave loads 64 terms at a time into
-// LDS, the next chunk's loads in flight while lane 0 adds the current one in
-// row order.
+// A long row (a hub of the graph): the wave stages 64 terms at a time in
+// LDS, x and y apart, the next chunk's loads in flight; lane 0 adds the x
+// terms and lane 1 the y terms, each in row order (one dependent add per
+// entry and lane, both lanes in one instruction).
 __global__ __launch_bounds__(64) void k_link_rows_long(const uint32_t* __restrict__ rows_long,
                                                        const uint32_t* __restrict__ row,
                                                        const double2* __restrict__ terms, double* __restrict__ vxs,
                                                        double* __restrict__ vys) {
-  __shared__ double2 buf[64];
+  __shared__ double buf[2][64];
   const uint32_t i = rows_long[blockIdx.x];
   const int lane = threadIdx.x;
   const uint32_t a0 = row[i], a1 = row[i + 1];
-  double vx = vxs[i], vy = vys[i];
+  const int comp = lane & 1;  // lane 0: x, lane 1: y
+  double v = comp ? vys[i] : vxs[i];
   double2 next = a0 + lane < a1 ? terms[a0 + lane] : make_double2(0, 0);
   for (uint32_t base = a0; base < a1; base += 64) {
-    buf[lane] = next;
+    buf[0][lane] = next.x;
+    buf[1][lane] = next.y;
     __syncthreads();
     const uint32_t nb = base + 64 + lane;
     if (nb < a1) next = terms[nb];
-    if (lane == 0) {
+    if (lane < 2) {
+      const double* b = buf[comp];
       const uint32_t k = a1 - base < 64 ? a1 - base : 64;
       uint32_t q = 0;
-      for (; q + 4 <= k; q += 4) {
-        const double2 t0 = buf[q], t1 = buf[q + 1], t2 = buf[q + 2], t3 = buf[q + 3];
-        vx += t0.x;
-        vy += t0.y;
-        vx += t1.x;
-        vy += t1.y;
-        vx += t2.x;
-        vy += t2.y;
-        vx += t3.x;
-        vy += t3.y;
+      for (; q + 8 <= k; q += 8) {
+        const double t0 = b[q], t1 = b[q + 1], t2 = b[q + 2], t3 = b[q + 3];
+        const double t4 = b[q + 4], t5 = b[q + 5], t6 = b[q + 6], t7 = b[q + 7];
+        v += t0;
+        v += t1;
+        v += t2;
+        v += t3;
+        v += t4;
+        v += t5;
+        v += t6;
+        v += t7;
       }
-      for (; q < k; ++q) {
-        vx += buf[q].x;
-        vy += buf[q].y;
-      }
+      for (; q < k; ++q) v += b[q];
     }
     __syncthreads();
   }
-  if (lane == 0) {
-    vxs[i] = vx;
-    vys[i] = vy;
-  }
+  if (lane == 0) vxs[i] = v;
+  if (lane == 1) vys[i] = v;
 }
 
 // Nodes whose row is longer than LONG_ROW (any order: each is one block).
