@@ -159,12 +159,28 @@ __global__ __launch_bounds__(256) void k_reduce(const float* __restrict__ part, 
 }
 
 // Bias gradient: db[c] = sum over rows of dz[row][c] (FP32, row order).
-__global__ __launch_bounds__(64) void k_colsum(const float* __restrict__ dz, uint32_t rows, int cols,
-                                               float* __restrict__ db) {
+// Column sums of dz (the bias gradients) in a fixed order: block y sums
+// rows [y COLSUM_ROWS, (y + 1) COLSUM_ROWS) of each column in row order,
+// then k_colsum_final adds the block sums in block order.
+constexpr uint32_t COLSUM_ROWS = 1024;
+
+__global__ __launch_bounds__(64) void k_colsum_part(const float* __restrict__ dz, uint32_t rows, int cols,
+                                                    float* __restrict__ part) {
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  if (c >= cols) return;
+  const uint32_t r0 = blockIdx.y * COLSUM_ROWS;
+  const uint32_t r1 = rows - r0 < COLSUM_ROWS ? rows : r0 + COLSUM_ROWS;
+  float s = 0.f;
+  for (uint32_t r = r0; r < r1; ++r) s += dz[(size_t)r * cols + c];
+  part[(size_t)blockIdx.y * cols + c] = s;
+}
+
+__global__ __launch_bounds__(64) void k_colsum_final(const float* __restrict__ part, uint32_t parts, int cols,
+                                                     float* __restrict__ db) {
   const int c = blockIdx.x * 64 + threadIdx.x;
   if (c >= cols) return;
   float s = 0.f;
-  for (uint32_t r = 0; r < rows; ++r) s += dz[(size_t)r * cols + c];
+  for (uint32_t b = 0; b < parts; ++b) s += part[(size_t)b * cols + c];
   db[c] = s;
 }
 
@@ -280,7 +296,7 @@ __global__ __launch_bounds__(256) void k_zero_padding(float* __restrict__ w) {
 struct Net {
   uint32_t n = 0, rows = 0, nnz = 0;  // rows: n padded to 64
   Pinned<float> hparams, hgrad;
-  Device<float> w, g, m, v, part, z[LAYERS + 2], dh, dzf, p, dp, loss;
+  Device<float> w, g, m, v, part, cpart, z[LAYERS + 2], dh, dzf, p, dp, loss;
   Device<uint16_t> wt, wn, x, hh[LAYERS + 2], agg[LAYERS], dz, dhb, tmp;
   Device<uint32_t> row, col, pi, pj;
   Device<float> pd;
@@ -369,6 +385,7 @@ struct Net {
     HIPCHECK(p.reserve(r * OUT_PAD));
     HIPCHECK(dp.reserve(r * OUT_PAD));
     HIPCHECK(part.reserve((size_t)(r / 64) * HID * HID));
+    HIPCHECK(cpart.reserve((size_t)ceil_div(r, COLSUM_ROWS) * HID));
     // features as BF16, padded with zeros to K_IN columns and to `rows` rows
     std::vector<uint16_t> xb(r * K_IN, 0);
     for (uint32_t i = 0; i < n; ++i)
@@ -419,6 +436,13 @@ struct Net {
     return hipGetLastError();
   }
 
+  // db[c] = sum over the n node rows of m[r][c] (m has `cols` <= HID columns).
+  void colsum(const float* m, int cols, float* db) {
+    const uint32_t parts = ceil_div(n > 0 ? n : 1, COLSUM_ROWS);
+    hipLaunchKernelGGL(k_colsum_part, dim3(ceil_div(cols, 64), parts), dim3(64), 0, stream, m, n, cols, cpart.p);
+    hipLaunchKernelGGL(k_colsum_final, dim3(ceil_div(cols, 64)), dim3(64), 0, stream, cpart.p, parts, cols, db);
+  }
+
   // Backward from dp (FP32, OUT_PAD columns) into g.
   hipError_t backward() {
     const uint32_t tiles = rows / 16, cnt = rows * HID;
@@ -427,7 +451,7 @@ struct Net {
     hipLaunchKernelGGL(k_to_bf16, dim3(ceil_div(rows * OUT_PAD, 256)), dim3(256), 0, stream, dp.p, rows * OUT_PAD,
                        tmp.p);
     HIPCHECK((wgrad<HID, OUT_PAD>(hh[LAYERS + 1].p, tmp.p, OFF_WO)));
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, stream, dp.p, n, OUT_PAD, g.p + OFF_BO);
+    colsum(dp.p, OUT_PAD, g.p + OFF_BO);
     hipLaunchKernelGGL((k_gemm<OUT_PAD, HID>), dim3(tiles), dim3(64), 0, stream, tmp.p, wn.p + OFF_WO, nullptr, nullptr,
                        nullptr, nullptr, n, 0, nullptr, nullptr, dh.p);
     for (int l = LAYERS - 1; l >= 0; --l) {
@@ -436,7 +460,7 @@ struct Net {
                          dzf.p);
       HIPCHECK((wgrad<HID, HID>(hh[l + 1].p, dz.p, o)));
       HIPCHECK((wgrad<HID, HID>(agg[l].p, dz.p, o + HID * HID)));
-      hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, stream, dzf.p, n, HID, g.p + o + 2 * HID * HID);
+      colsum(dzf.p, HID, g.p + o + 2 * HID * HID);
       // dH_l = dZ Ws^T + mean^T(dZ Wn^T)
       hipLaunchKernelGGL((k_gemm<HID, HID>), dim3(tiles), dim3(64), 0, stream, dz.p, wn.p + o + HID * HID, nullptr,
                          nullptr, nullptr, nullptr, n, 0, nullptr, dhb.p, nullptr);
@@ -447,7 +471,7 @@ struct Net {
     // input layer
     hipLaunchKernelGGL(k_relu_back, dim3(ceil_div(cnt, 256)), dim3(256), 0, stream, dh.p, z[0].p, cnt, dz.p, dzf.p);
     HIPCHECK((wgrad<K_IN, HID>(x.p, dz.p, OFF_W0)));
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, stream, dzf.p, n, HID, g.p + OFF_B0);
+    colsum(dzf.p, HID, g.p + OFF_B0);
     hipLaunchKernelGGL(k_zero_padding, dim3(ceil_div(K_IN * HID, 256)), dim3(256), 0, stream, g.p);
     return hipGetLastError();
   }

@@ -290,20 +290,40 @@ def target_layout(g, core=None, alpha=0.1):
     return np.stack([nodes.x, nodes.y], axis=1)
 
 
-def train(graphs=48, epochs=30, steps=4, lr=2e-3, seed=1, out=WEIGHTS, log=print, core=None):
+def _prepare(job):
+    k, n, src, tgt = job
+    return Prepared(n, src, tgt, seed=k)
+
+
+def train(graphs=48, epochs=30, steps=4, lr=2e-3, seed=1, out=WEIGHTS, log=print, core=None, n_max=20000,
+          workers=1, init=None):
     """Trains the network on simulator layouts at alpha = 0.1 of `graphs`
-    training graphs and saves its parameters to `out`."""
+    training graphs (n log-uniform in [1000, n_max]) and saves its parameters
+    to `out`; `init` names parameters to start from (fine-tuning) instead of
+    the seeded initialisation. With workers > 1 the host-side graph features
+    are computed in that many spawned processes, started before this process
+    uses the GPU."""
+    gs = training_graphs(graphs, seed=seed, n_max=n_max)
+    jobs = [(k, n, s, t) for k, (n, s, t) in enumerate(gs)]
+    if workers > 1:
+        import multiprocessing
+        from concurrent.futures import ProcessPoolExecutor
+        with ProcessPoolExecutor(workers, mp_context=multiprocessing.get_context("spawn")) as ex:
+            prepared = list(ex.map(_prepare, jobs, chunksize=4))
+    else:
+        prepared = [_prepare(j) for j in jobs]
+    log("warm start: features of %d graphs ready" % len(prepared))
     core = core or gpu_core()
-    gs = training_graphs(graphs, seed=seed)
-    prepared = []
-    for k, (n, s, t) in enumerate(gs):
-        g = Prepared(n, s, t, seed=k)
+    for k, g in enumerate(prepared):
         lay = target_layout(g, core=core)
         lay = (lay - lay.mean(axis=0)) / g.scale
         g.pairs = stress_pairs(g.row, g.col, lay, seed=k)
-        prepared.append(g)
+        if (k + 1) % 100 == 0:
+            log("warm start: %d training layouts ready" % (k + 1))
     log("warm start: %d training layouts ready" % len(prepared))
     net = Network(seed=seed, core=core)
+    if init:
+        net.load(init)
     rng = np.random.default_rng(seed)
     total = epochs * len(prepared) * steps
     done = 0
@@ -381,7 +401,7 @@ def graph_stress(x, y, row, col, sources=16, seed=5):
     return float(np.mean((q / c - 1) ** 2))
 
 
-def compare(net, n, src, tgt, core=None, seed=0, short=(50, 100)):
+def compare(net, n, src, tgt, core=None, seed=0, short=(50, 100), stress_all=True):
     """Plain fast mode from phyllotaxis (300 ticks, d3's schedule) against
     the warm start (the prediction refined from alpha = 0.1, alpha decaying to
     0.001 over K ticks) and against plain fast mode squeezed into the same K
@@ -390,7 +410,8 @@ def compare(net, n, src, tgt, core=None, seed=0, short=(50, 100)):
     allocated by a first, untimed run); seconds are GPU kernel time.
     Energies use one sample of node pairs; `gap` is (E - E_plain) /
     (E_start - E_plain): 0 is as good as 300 plain ticks, 1 no better than
-    the start."""
+    the start. With stress_all False the squeezed plain runs skip the
+    (host-side) stress measurement."""
     import time
     core = core or gpu_core()
     g = Prepared(n, src, tgt, seed=seed)
@@ -430,7 +451,7 @@ def compare(net, n, src, tgt, core=None, seed=0, short=(50, 100)):
         s_warm = graph_stress(nodes.x, nodes.y, g.row, g.col)
         t_short = run(x0, y0, 1.0, 1 - math.pow(0.001, 1 / k), k)
         e_short, _ = energy(nodes.x, nodes.y, g.src, g.tgt, k_link, pairs=pairs)
-        s_short = graph_stress(nodes.x, nodes.y, g.row, g.col)
+        s_short = graph_stress(nodes.x, nodes.y, g.row, g.col) if stress_all else None
         out["runs"].append({
             "ticks": k,
             "warm": {"energy": e_warm, "gap": (e_warm - e_plain) / span,
@@ -439,6 +460,56 @@ def compare(net, n, src, tgt, core=None, seed=0, short=(50, 100)):
                                  "relative_to_plain": (e_short - e_plain) / abs(e_plain), "seconds": t_short,
                                  "stress": s_short}})
     return out
+
+
+def heldout_graph(k, nodes=100000, seed=20000):
+    """Held-out graph k of the evaluation: stochastic block model for even k,
+    Barabasi-Albert (m0 = 3, 3 links per node) for odd k, seeds disjoint from
+    the training graphs'."""
+    from .graphs import ba_graph, sbm_batch
+    if k % 2 == 0:
+        return "sbm", sbm_batch(1, seed=seed + k, n_min=nodes, n_max=nodes)[0]
+    return "ba", (nodes,) + ba_graph(nodes, seed=seed + k)
+
+
+def evaluate(net, count, nodes=100000, short=(25, 50, 75), core=None, log=print):
+    """The warm start against 300 plain fast-mode ticks on `count` held-out
+    graphs (heldout_graph): per graph, the first K of `short` whose warm run
+    ends within 1% of the plain energy and its GPU-time ratio, the energy
+    gaps at equal ticks and the stresses; then a summary per graph family."""
+    rows = []
+    for k in range(count):
+        family, (n, src, tgt) = heldout_graph(k, nodes)
+        r = compare(net, n, src, tgt, core=core, seed=k, short=short, stress_all=False)
+        hit = next((run for run in r["runs"] if abs(run["warm"]["relative_to_plain"]) <= 0.01), None)
+        row = {"graph": k, "family": family, "nodes": n, "links": len(src),
+               "plain_seconds": r["plain_seconds"], "plain_stress": r["plain_stress"],
+               "predicted_energy_gap": r["predicted_energy_gap"],
+               "hit_ticks": hit["ticks"] if hit else None,
+               "speedup": r["plain_seconds"] / hit["warm"]["seconds"] if hit else None,
+               "hit_stress": hit["warm"]["stress"] if hit else None,
+               "gaps": {run["ticks"]: (run["warm"]["gap"], run["plain_same_ticks"]["gap"]) for run in r["runs"]}}
+        rows.append(row)
+        log("held-out %d (%s): within 1%% at %s ticks, speedup %s, predicted gap %.3f, gaps %s"
+            % (k, family, row["hit_ticks"], None if hit is None else round(row["speedup"], 2),
+               row["predicted_energy_gap"],
+               {t: (round(a, 4), round(b, 4)) for t, (a, b) in row["gaps"].items()}))
+    summary = {}
+    for family in sorted(set(r["family"] for r in rows)):
+        fr = [r for r in rows if r["family"] == family]
+        sp = sorted(r["speedup"] for r in fr if r["speedup"] is not None)
+        mid = short[len(short) // 2]
+        summary[family] = {
+            "graphs": len(fr), "within_1pct": len(sp),
+            "speedup_min": sp[0] if sp else None, "speedup_median": sp[len(sp) // 2] if sp else None,
+            "speedup_max": sp[-1] if sp else None,
+            "at_%d_ticks_warm_gap_median" % mid: float(np.median([r["gaps"][mid][0] for r in fr])),
+            "at_%d_ticks_plain_gap_median" % mid: float(np.median([r["gaps"][mid][1] for r in fr])),
+            "plain_stress_median": float(np.median([r["plain_stress"] for r in fr])),
+            "hit_stress_median": float(np.median([r["hit_stress"] for r in fr if r["hit_stress"] is not None]))
+            if sp else None}
+        log("summary %s: %s" % (family, summary[family]))
+    return {"graphs": rows, "summary": summary}
 
 
 def _link_strength(n, src, tgt):
@@ -451,6 +522,24 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser(description="train the warm-start network on the GPU")
     ap.add_argument("--graphs", type=int, default=48)
     ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=4, help="Adam steps per graph and epoch")
+    ap.add_argument("--n-max", type=int, default=20000, help="largest training graph (nodes)")
+    ap.add_argument("--workers", type=int, default=1, help="processes for the host-side graph features")
+    ap.add_argument("--lr", type=float, default=2e-3, help="peak Adam learning rate (cosine schedule)")
+    ap.add_argument("--init", default=None, help="parameters to fine-tune from")
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--evaluate", type=int, default=0,
+                    help="instead of training: evaluate the committed network on this many held-out graphs")
+    ap.add_argument("--nodes", type=int, default=100000, help="nodes per held-out graph (with --evaluate)")
+    ap.add_argument("--weights", default=WEIGHTS, help="network parameters to evaluate (with --evaluate)")
     a = ap.parse_args()
-    train(graphs=a.graphs, epochs=a.epochs, out=a.out)
+    if a.evaluate:
+        import json
+        net = Network()
+        net.load(a.weights)
+        res = evaluate(net, a.evaluate, nodes=a.nodes, log=lambda *m: print(*m, flush=True))
+        net.close()
+        print(json.dumps(res["summary"]))
+    else:
+        train(graphs=a.graphs, epochs=a.epochs, steps=a.steps, lr=a.lr, out=a.out, n_max=a.n_max,
+              workers=a.workers, init=a.init, log=lambda *m: print(*m, flush=True))
