@@ -227,8 +227,10 @@ __global__ __launch_bounds__(EW_THREADS) void k_rest_all(uint32_t n, const TreeM
 }
 
 // ---------------------------------------------------------------- scan
-// Exclusive scan of u32 over [0, *count) in three kernels. `skip` (when not
-// null) makes every kernel return at once when skip[0] == skip[1].
+// Exclusive scan of u32 over [0, *count) in two kernels: the sum of every
+// tile of SCAN_TILE items, then each tile adds the sums of the tiles before it
+// and scans itself. `skip` (when not null) makes every kernel return at once
+// when skip[0] == skip[1].
 
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
   constexpr int NW = SCAN_THREADS / 64;
@@ -280,22 +282,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __
   if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_partials(uint32_t* __restrict__ partials,
-                                                                const uint32_t* __restrict__ count,
-                                                                const uint32_t* __restrict__ skip) {
-  if (skip && skip[0] == skip[1]) return;
-  const uint32_t nb = ceil_div(*count, SCAN_TILE);
-  uint32_t carry = 0;
-  for (uint32_t b0 = 0; b0 < nb; b0 += SCAN_THREADS) {
-    uint32_t i = b0 + threadIdx.x;
-    uint32_t v = i < nb ? partials[i] : 0;
-    uint32_t total;
-    uint32_t ex = block_exclusive_scan(v, &total);
-    if (i < nb) partials[i] = carry + ex;
-    carry += total;
-  }
-}
-
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* __restrict__ in,
                                                              uint32_t* __restrict__ out,
                                                              const uint32_t* __restrict__ count,
@@ -309,6 +295,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* __r
     if (m == 0 && blockIdx.x == 0 && threadIdx.x == 0 && total_out) *total_out = 0;
     return;
   }
+  // The tile's offset: the sums of the tiles before it (a few thousand words
+  // at most), added by this block itself.
+  uint32_t pre = 0;
+  for (uint32_t j = threadIdx.x; j < blockIdx.x; j += SCAN_THREADS) pre += partials[j];
+  uint32_t offset;
+  block_exclusive_scan(pre, &offset);
   // Thread t owns items base + t*ITEMS .. +ITEMS (blocked, so order is kept).
   uint32_t v[SCAN_ITEMS];
   uint32_t s = 0;
@@ -319,7 +311,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* __r
     s += v[k];
   }
   uint32_t total;
-  uint32_t ex = block_exclusive_scan(s, &total) + partials[blockIdx.x];
+  uint32_t ex = block_exclusive_scan(s, &total) + offset;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     uint32_t i = base + threadIdx.x * SCAN_ITEMS + k;
@@ -333,7 +325,6 @@ hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n_max, con
                           uint32_t* partials, uint32_t* total_out, const uint32_t* skip, hipStream_t stream) {
   uint32_t nb = ceil_div(n_max > 0 ? n_max : 1, SCAN_TILE);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, stream, in, count_dev, partials, skip);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_THREADS), 0, stream, partials, count_dev, skip);
   hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_THREADS), 0, stream, in, out, count_dev, partials,
                      total_out, skip);
   return hipGetLastError();
