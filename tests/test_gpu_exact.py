@@ -2,6 +2,8 @@
 and the random stream's position) to the oracle, over the reference's parity
 corpus scenarios and the benchmark workloads."""
 
+import os
+
 import numpy as np
 import pytest
 
@@ -60,6 +62,24 @@ def test_gpu_exact_tick_batches_match(gpu_core, oracle_core):
         a.tick(k)
         b.tick(k)
         assert same(na.state(), nb.state()), "after tick(%d)" % k
+
+
+@pytest.mark.skipif(not os.environ.get("D3F_LONG_TESTS"),
+                    reason="about two minutes with the 16-thread oracle; set D3F_LONG_TESTS=1 to run it")
+def test_gpu_exact_1m_pinned_300_ticks_bit_identical(gpu_core, oracle_core):
+    # The 1M-node configuration's criterion: x, y, vx, vy identical (Object.is)
+    # to the reference after every one of 300 ticks.
+    print("gpu exact 1M pinned x 300", flush=True)
+    oracle_core.set_threads(16)
+    a, na = workload(oracle_core, 1000000, 2000000, pin_fraction=0.01)
+    b, nb = workload(gpu_core, 1000000, 2000000, pin_fraction=0.01)
+    for t in range(300):
+        a.tick()
+        b.tick()
+        assert same(na.state(), nb.state()), "tick %d" % (t + 1)
+        if (t + 1) % 25 == 0:
+            print("  tick %d identical" % (t + 1), flush=True)
+    oracle_core.set_threads(1)
 
 
 def test_gpu_exact_1m_pinned_bit_identical(gpu_core, oracle_core):
