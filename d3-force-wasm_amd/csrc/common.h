@@ -321,6 +321,9 @@ struct FmmBuffers {
   // launch shapes (no result depends on them): the dual traversal's grid, the
   // per-depth build grids, the blocks sharing near-field work items
   uint32_t dual_blocks = 1024, level_blocks = 256, near_extra = 256;
+  // depths the eager cell build runs (the last build's deepest level + 2;
+  // a deeper tree is rebuilt with every level)
+  uint32_t build_levels = FMM_LEVELS;
   uint32_t* long_list = nullptr;           // targets whose segments the long-segment sort orders
   uint4* near_items = nullptr;             // (leaf, target batch, chunk or ALL, slot)
   uint4* near_combines = nullptr;          // (leaf, target batch, first slot, chunks)

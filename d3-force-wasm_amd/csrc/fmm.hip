@@ -356,6 +356,14 @@ __global__ __launch_bounds__(EWF) void k_fmm_children(FCell* __restrict__ cells,
   }
 }
 
+// After a build that stopped at depth d_end: when no cell reached d_end, every
+// deeper level is empty and the offsets after it equal its end.
+__global__ void k_fmm_levels_close(FmmParams* __restrict__ fp, uint32_t d_end) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || fp->level_n[d_end] != 0) return;
+  for (uint32_t d = d_end + 1; d <= FMM_LEVELS + 1; ++d) fp->level_n[d] = 0;
+  for (uint32_t d = d_end + 2; d <= FMM_LEVELS + 2; ++d) fp->level_off[d] = fp->level_off[d_end + 1];
+}
+
 // fmm.zig:668-682: Morton-ordered raw positions, unit-frame positions and
 // strength * alpha (NaN -> 0), each point in its graph's frame.
 __global__ __launch_bounds__(EWF) void k_fmm_gather(const uint32_t* __restrict__ perm, const uint64_t* __restrict__ keys,
@@ -1625,10 +1633,13 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
   HIPCHECK(sort_digits(t.hist, t.rs_glob, t.keys, t.perm, t.keys_tmp, t.perm_tmp, n > 0 ? n : 1, &f.params->m_total,
                        0, digits, stream));
   // Cells, one depth at a time; grown and rebuilt if they outgrow the buffer.
+  // The eager form builds the depths the last tree needed plus two and
+  // rebuilds with every depth when cells reach the last one.
+  uint32_t d_end = a.graph ? FMM_LEVELS : std::min(FMM_LEVELS, f.build_levels);
   for (;;) {
     hipLaunchKernelGGL(k_forest_roots, dim3(1), dim3(1024), 0, stream, f.graphs, a.graphs, f.params, f.cells,
                        f.cell_cap, t.meta);
-    for (uint32_t d = 0; d < FMM_LEVELS; ++d) {
+    for (uint32_t d = 0; d < d_end; ++d) {
       hipLaunchKernelGGL(k_fmm_split, dim3(f.level_blocks), dim3(EWF), 0, stream, t.keys, f.cells, f.params, d, f.split,
                          f.bounds, f.cell_cap);
       HIPCHECK(scan_u32(f.split, f.split_off, n > 0 ? n : 1, &f.params->level_n[d], t.partials,
@@ -1637,8 +1648,13 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
                          f.split_off, f.bounds, f.cell_cap);
     }
     if (a.graph) break;  // sizes checked on the device (k_fmm_guard)
+    if (d_end < FMM_LEVELS) hipLaunchKernelGGL(k_fmm_levels_close, dim3(1), dim3(64), 0, stream, f.params, d_end);
     HIPCHECK(hipMemcpyAsync(f.params_host, f.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
+    if (!f.params_host->cell_overflow && d_end < FMM_LEVELS && f.params_host->level_n[d_end] != 0) {
+      d_end = FMM_LEVELS;  // the tree reaches deeper than the last one: rebuild with every level
+      continue;
+    }
     if (!f.params_host->cell_overflow) break;
     const uint32_t cap = std::max(2 * f.cell_cap, f.params_host->level_off[FMM_LEVELS + 1] + 4096);
     HIPCHECK(grow(f.cells, cap));
@@ -1647,6 +1663,10 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
   }
   if (!a.graph) {
     const FmmParams& P = *f.params_host;
+    uint32_t deepest = 0;
+    for (uint32_t d = 0; d <= FMM_LEVELS; ++d)
+      if (P.level_n[d]) deepest = d;
+    f.build_levels = std::min(FMM_LEVELS, deepest + 2);
     // fmm.zig:255-257: one graph with a non-finite extent runs the exact kernel.
     if (a.graphs == 1 && P.declined) return hipErrorNotSupported;
     if (P.roots == 0) return hipSuccess;  // no point in any tree (fmm.zig:251-252)

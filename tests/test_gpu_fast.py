@@ -491,3 +491,34 @@ def test_fast_mode_4m_nodes_200_ticks(gpu_core):
           "sum on %d nodes: order 8 %.2e, order 10 %.2e" % (len(src), dt, 1e3 * dt / 200, k, err, err10), flush=True)
     assert err < 5e-6
     assert err10 < 1e-6
+
+
+def test_fmm_tree_deepens_between_ticks(gpu_core, oracle_core):
+    # The cell build of a tick runs the depths the previous tree used plus
+    # two; when the layout's tree is deeper (here a spread layout, then half
+    # of the nodes in a cluster 1e-3 wide, whose tree reaches the depth
+    # limit) it rebuilds with every depth. Lists and forces on the second
+    # tick equal the oracle's.
+    print("fmm tree deepens between ticks", flush=True)
+    rng = np.random.default_rng(9)
+    n = 20000
+    sx, sy = rng.uniform(-500, 500, n), rng.uniform(-500, 500, n)
+    cx, cy = sx.copy(), sy.copy()
+    cx[: n // 2] = 100 + rng.uniform(0, 1e-3, n // 2)
+    cy[: n // 2] = -50 + rng.uniform(0, 1e-3, n // 2)
+    out = []
+    for core in (oracle_core, gpu_core):
+        nodes = nodes_of(sx, sy)
+        sim = fast(d3.forceSimulation(nodes, core=core).velocity_decay(0).force("charge", d3.forceManyBody()).stop())
+        sim.tick()
+        nodes.x[:] = cx
+        nodes.y[:] = cy
+        nodes.vx[:] = 0
+        nodes.vy[:] = 0
+        sim.tick()
+        out.append((nodes.state(), fmm_counts(core, sim)))
+    (a, ca), (b, cb) = out
+    assert ca == cb
+    err = rms(b, a)
+    print("  rms vs FP64 FMM %.2e" % err, flush=True)
+    assert err < FP32_RMS
