@@ -310,6 +310,8 @@ struct Batch {
     a.node_graph = dnode_graph.p;
     a.seeds = nullptr;
     a.seeds_dev = dseeds_mb.p;
+    const double limit = scalars[D3F_BS_GRAPH_CELLS];
+    a.cell_limit = limit >= 1 && limit < 4294967296.0 ? (uint32_t)limit : 0;
     return a;
   }
 
@@ -496,7 +498,7 @@ EXPORT int32_t batch_sync(uint32_t h, uint32_t direction) {
 EXPORT int32_t batch_set_scalar(uint32_t h, uint32_t which, double v) {
   Batch* b = bget(h);
   if (!b) return B_BAD_HANDLE;
-  if (which >= D3F_BS_COUNT) return B_BAD_ARGUMENT;
+  if (which >= D3F_BS_COUNT || which == D3F_BS_GRAPH_REPLAYS || which == D3F_BS_GRAPH_REDOS) return B_BAD_ARGUMENT;
   b->scalars[which] = v;
   if (which == D3F_BS_ALPHA || which == D3F_BS_TICK)
     b->state_dirty = true;   // read by the device each tick
@@ -508,6 +510,8 @@ EXPORT int32_t batch_set_scalar(uint32_t h, uint32_t which, double v) {
 EXPORT double batch_get_scalar(uint32_t h, uint32_t which) {
   Batch* b = bget(h);
   if (!b || which >= D3F_BS_COUNT) return 0;
+  if (which == D3F_BS_GRAPH_REPLAYS) return (double)b->graph_replays;
+  if (which == D3F_BS_GRAPH_REDOS) return (double)b->graph_redos;
   return b->scalars[which];
 }
 

@@ -356,6 +356,9 @@ struct FmmArgs {
   bool graph = false;
   const double* alpha_dev = nullptr;
   const uint64_t* seeds_dev = nullptr;  // device seeds, in place of the host ones
+  // Capturable form only: when nonzero, the size guard treats this many cells
+  // as the capacity (a tick with more cells aborts and is redone eagerly).
+  uint32_t cell_limit = 0;
 };
 // Fast many-body (fmm.zig:247-354) over every graph of the forest at once.
 // With one graph, returns hipErrorNotSupported when the FMM declines (a

@@ -1672,7 +1672,8 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
     if (P.roots == 0) return hipSuccess;  // no point in any tree (fmm.zig:251-252)
     HIPCHECK(reserve_cells(f, P.level_off[FMM_LEVELS + 1], p + 1));
   }
-  const uint32_t room = std::min<uint32_t>(std::min(f.cell_cap, f.cellarr_cap), (uint32_t)(f.exp_cap / (p + 1)));
+  uint32_t room = std::min<uint32_t>(std::min(f.cell_cap, f.cellarr_cap), (uint32_t)(f.exp_cap / (p + 1)));
+  if (a.graph && a.cell_limit && a.cell_limit < room) room = a.cell_limit;
   hipLaunchKernelGGL(k_fmm_guard, dim3(1), dim3(64), 0, stream, f.params, 0u, room, f.front_cap, 0u);
   hipLaunchKernelGGL(k_fmm_gather, dim3(nb), dim3(EWF), 0, stream, t.perm, t.keys, a.x, a.y, a.strength, f.graphs,
                      f.params, a.alpha, a.alpha_dev, n, f.px, f.py, f.ux, f.uy, f.ps);

@@ -17,7 +17,8 @@ from .core import check, gpu_core
 
 B_X, B_Y, B_VX, B_VY, B_CHARGE, B_SOURCE, B_TARGET, B_LINK_STRENGTH, B_LINK_DISTANCE, B_LINK_BIAS, B_SEED = range(11)
 (BS_ALPHA, BS_ALPHA_DECAY, BS_ALPHA_TARGET, BS_VELOCITY_DECAY, BS_DISTANCE_MIN2, BS_DISTANCE_MAX2, BS_ORDER,
- BS_OPENING, BS_CENTER_X, BS_CENTER_Y, BS_CENTER_STRENGTH, BS_LINK_ITERATIONS, BS_TICK, BS_GRAPH) = range(14)
+ BS_OPENING, BS_CENTER_X, BS_CENTER_Y, BS_CENTER_STRENGTH, BS_LINK_ITERATIONS, BS_TICK, BS_GRAPH,
+ BS_GRAPH_CELLS, BS_GRAPH_REPLAYS, BS_GRAPH_REDOS) = range(17)
 
 _SIGNATURES = {
     "batch_create": (ctypes.c_uint32, []),

@@ -158,11 +158,16 @@ enum { D3F_B_X = 0, D3F_B_Y = 1, D3F_B_VX = 2, D3F_B_VY = 3, D3F_B_CHARGE = 4, D
  * ticks run. GRAPH = 1 replays one captured HIP graph per tick (captured
  * after an eager tick, with buffers sized for twice the cells that tick had;
  * a replay whose sizes do not fit is undone and the tick redone eagerly, and
- * the graph recaptured); results are bit-identical to the eager ticks. */
+ * the graph recaptured); results are bit-identical to the eager ticks.
+ * GRAPH_CELLS, when nonzero, caps the cells a replay may hold (0: the
+ * buffers' capacity), so that replays take the redo path (for tests).
+ * GRAPH_REPLAYS and GRAPH_REDOS are read-only counts of the replays launched
+ * and of those redone eagerly. */
 enum { D3F_BS_ALPHA = 0, D3F_BS_ALPHA_DECAY = 1, D3F_BS_ALPHA_TARGET = 2, D3F_BS_VELOCITY_DECAY = 3,
        D3F_BS_DISTANCE_MIN2 = 4, D3F_BS_DISTANCE_MAX2 = 5, D3F_BS_ORDER = 6, D3F_BS_OPENING = 7, D3F_BS_CENTER_X = 8,
        D3F_BS_CENTER_Y = 9, D3F_BS_CENTER_STRENGTH = 10, D3F_BS_LINK_ITERATIONS = 11, D3F_BS_TICK = 12,
-       D3F_BS_GRAPH = 13, D3F_BS_COUNT = 14 };
+       D3F_BS_GRAPH = 13, D3F_BS_GRAPH_CELLS = 14, D3F_BS_GRAPH_REPLAYS = 15, D3F_BS_GRAPH_REDOS = 16,
+       D3F_BS_COUNT = 17 };
 uint32_t batch_create(void);
 int32_t batch_destroy(uint32_t b);
 /* node_off, link_off: graphs + 1 host entries each, starting at 0. */

@@ -145,3 +145,23 @@ def test_batch_graph_replays_equal_eager_ticks(gpu_core, graphs):
     assert a.scalar(BS_ALPHA) == b.scalar(BS_ALPHA)
     a.close()
     b.close()
+
+
+def test_batch_graph_redo_path_equals_eager_ticks(gpu_core, graphs):
+    # With the replay's cells capped below what any tick needs
+    # (BS_GRAPH_CELLS = 1), every replay aborts on the device, is undone from
+    # the snapshot and redone eagerly, and the tick is captured again: the
+    # path a replay takes when a layout outgrows the captured buffers.
+    from d3force_amd.batch import BS_GRAPH, BS_GRAPH_CELLS, BS_GRAPH_REDOS, BS_GRAPH_REPLAYS
+    a = Batch(graphs, core=gpu_core)
+    b = Batch(graphs, core=gpu_core)
+    a.scalar(BS_GRAPH, 1)
+    a.scalar(BS_GRAPH_CELLS, 1)
+    for t in range(5):
+        a.tick(1)
+        b.tick(1)
+        assert same(a.state(), b.state()), "tick %d" % (t + 1)
+    assert a.scalar(BS_ALPHA) == b.scalar(BS_ALPHA)
+    assert a.scalar(BS_GRAPH_REPLAYS) == 4 and a.scalar(BS_GRAPH_REDOS) == 4
+    a.close()
+    b.close()
