@@ -986,15 +986,43 @@ __device__ __forceinline__ void near_batch(uint32_t A, uint32_t tb, uint32_t chu
       __syncthreads();
     }
     float accx = 0.f, accy = 0.f;
-    for (uint32_t s = 0; s < wk; ++s) {
-      const uint32_t B = seg[s];
-      const uint32_t f0 = cells[B].first, cnt = cells[B].count;
-      for (uint32_t p0 = 0; p0 < cnt; p0 += 64) {
-        const uint32_t np = cnt - p0 < 64 ? cnt - p0 : 64;
-        if ((uint32_t)lane < np) {
-          const uint32_t j = f0 + p0 + lane;
-          src[lane] = make_float4((float)(px[j] - ax), (float)(py[j] - ay), ps[j], __uint_as_float(j));
+    // The sources in 64-point steps (leaf s of the segment, points p0 ..):
+    // each step's points are loaded into registers while the step before it
+    // is computed, then staged in LDS.
+    uint32_t s = 0, p0 = 0, f0 = 0, cnt = 0;
+    if (wk) {
+      const uint32_t B = seg[0];
+      f0 = cells[B].first;
+      cnt = cells[B].count;
+    }
+    double nx = 0, ny = 0;
+    float nsr = 0.f;
+    if (wk && (uint32_t)lane < cnt) {
+      nx = px[f0 + lane];
+      ny = py[f0 + lane];
+      nsr = ps[f0 + lane];
+    }
+    while (s < wk) {
+      const uint32_t np = cnt - p0 < 64 ? cnt - p0 : 64;
+      if ((uint32_t)lane < np)
+        src[lane] = make_float4((float)(nx - ax), (float)(ny - ay), nsr, __uint_as_float(f0 + p0 + lane));
+      // the next step, its loads in flight during this one
+      uint32_t s2 = s, q0 = p0 + 64, g0 = f0, cnt2 = cnt;
+      if (q0 >= cnt) {
+        s2 = s + 1;
+        q0 = 0;
+        if (s2 < wk) {
+          const uint32_t B = seg[s2];
+          g0 = cells[B].first;
+          cnt2 = cells[B].count;
         }
+      }
+      if (s2 < wk && q0 + (uint32_t)lane < cnt2) {
+        nx = px[g0 + q0 + lane];
+        ny = py[g0 + q0 + lane];
+        nsr = ps[g0 + q0 + lane];
+      }
+      {
         __syncthreads();
         if (active) {
           for (uint32_t q = sub; q < np; q += S) {
@@ -1023,6 +1051,10 @@ __device__ __forceinline__ void near_batch(uint32_t A, uint32_t tb, uint32_t chu
         }
         __syncthreads();
       }
+      s = s2;
+      p0 = q0;
+      f0 = g0;
+      cnt = cnt2;
     }
     part[lane] = make_float2(accx, accy);
     __syncthreads();
