@@ -714,8 +714,9 @@ __device__ inline uint32_t load_window(const uint32_t* __restrict__ src, uint32_
 
 // One wave orders the k <= SEG_MAX distinct sources src[off .. off + k) by
 // rank into out[0 .. k).
-__device__ inline void sort_segment(const uint32_t* __restrict__ src, uint32_t off, uint32_t k,
-                                    uint32_t (&in)[SEG_MAX], uint32_t (&out)[SEG_MAX]) {
+template <uint32_t CAP>
+__device__ inline void sort_segment(const uint32_t* __restrict__ src, uint32_t off, uint32_t k, uint32_t (&in)[CAP],
+                                    uint32_t (&out)[CAP]) {
   const int lane = threadIdx.x & 63;
   for (uint32_t q = lane; q < k; q += 64) in[q] = src[off + q];
   __syncthreads();
@@ -957,7 +958,9 @@ __device__ __forceinline__ void near_batch(uint32_t A, uint32_t tb, uint32_t chu
                                            float dmax2) {
   const FCell ca = cells[A];
   const int lane = threadIdx.x;
-  __shared__ uint32_t seg_in[SEG_MAX], seg[SEG_MAX];
+  // at most NEAR_CHUNK sources at a time: a longer segment is ordered in
+  // place beforehand and read chunk by chunk
+  __shared__ uint32_t seg_in[NEAR_CHUNK], seg[NEAR_CHUNK];
   __shared__ float4 src[64];
   __shared__ float2 part[64];
   const uint32_t K = cnts[A], off = offs[A];
