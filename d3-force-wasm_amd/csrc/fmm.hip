@@ -1028,26 +1028,31 @@ __device__ __forceinline__ void near_batch(uint32_t A, uint32_t tb, uint32_t chu
       {
         __syncthreads();
         if (active) {
+          // Branch-free pair step: a skipped pair (the target itself, or
+          // beyond distanceMax) adds dx * 0; the rare jiggle case runs only
+          // when a lane of the wave needs it; distanceMin's square root is
+          // the hardware one (FP32 like the rest of the near field).
           for (uint32_t q = sub; q < np; q += S) {
             const float4 cq = src[q];
             const uint32_t j = f0 + p0 + q;
-            if (j == ti) continue;
             float dx = cq.x - xi, dy = cq.y - yi;
             float l = fmaf(dx, dx, dy * dy);
-            if (!(l < dmax2)) continue;
-            if (dx == 0.f || dy == 0.f) {
-              const uint32_t jl = perm[j] - base;
-              if (dx == 0.f) {
-                dx = (float)hash_jiggle(seed, il, jl, 0);
-                l = fmaf(dx, dx, l);
-              }
-              if (dy == 0.f) {
-                dy = (float)hash_jiggle(seed, il, jl, 1);
-                l = fmaf(dy, dy, l);
+            const bool ok = j != ti && l < dmax2;
+            if (__ballot(ok && (dx == 0.f || dy == 0.f)) != 0) {
+              if (ok && (dx == 0.f || dy == 0.f)) {
+                const uint32_t jl = perm[j] - base;
+                if (dx == 0.f) {
+                  dx = (float)hash_jiggle(seed, il, jl, 0);
+                  l = fmaf(dx, dx, l);
+                }
+                if (dy == 0.f) {
+                  dy = (float)hash_jiggle(seed, il, jl, 1);
+                  l = fmaf(dy, dy, l);
+                }
               }
             }
-            if (l < dmin2) l = __builtin_sqrtf(dmin2 * l);
-            const float wj = cq.z * __builtin_amdgcn_rcpf(l);
+            if (l < dmin2) l = __builtin_amdgcn_sqrtf(dmin2 * l);
+            const float wj = ok ? cq.z * __builtin_amdgcn_rcpf(l) : 0.f;
             accx = fmaf(dx, wj, accx);
             accy = fmaf(dy, wj, accy);
           }
