@@ -258,17 +258,20 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     if (O == 0) continue;
     if (!(q->flags & TC_LEAF)) {
       if (sp + 4 > (int)stack_cap) break;  // cannot happen with cap = 3 depth + 4
-      // Descend: push children 3..0 so 0 is visited first.
-      for (int k = 3; k >= 0; --k) {
-        const uint32_t ch = q->child[k];
+      // Descend: push children 3..0 so 0 is visited first, in one store:
+      // lane L < 4 writes child 3 - L, if present, above the present
+      // children of higher index.
+      const uint32_t h0 = q->child[0], h1 = q->child[1], h2 = q->child[2], h3 = q->child[3];
+      const int e0 = h0 != NONE, e1 = h1 != NONE, e2 = h2 != NONE, e3 = h3 != NONE;
+      if (lane < 4) {
+        const uint32_t ch = lane == 0 ? h3 : lane == 1 ? h2 : lane == 2 ? h1 : h0;
+        const int before = lane == 0 ? 0 : lane == 1 ? e3 : lane == 2 ? e3 + e2 : e3 + e2 + e1;
         if (ch != NONE) {
-          if (lane == 0) {
-            st_cell[sp] = ch;
-            st_mask[sp] = O;
-          }
-          ++sp;
+          st_cell[sp + before] = ch;
+          st_mask[sp + before] = O;
         }
       }
+      sp += e0 + e1 + e2 + e3;
       continue;
     }
     // Leaf: points directly (many_body.zig:208-258).
