@@ -222,35 +222,40 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     const double value = q->value;
     if (!truthy(value)) continue;
     const bool in = (M >> lane) & 1;
-    double dx = 0, dy = 0, l = 0;
-    bool open = false;
-    if (in) {
-      ++n_visit;
-      dx = q->qx - nx;
-      dy = q->qy - ny;
-      l = dx * dx + dy * dy;
-      if (q->w2t < l) {  // Barnes-Hut approximation
-        if (l < dmax2) {
-          const bool jx = dx == 0, jy = dy == 0;
-          if (MODE == TRAV_RECORD && (jx || jy)) {
-            aborted = true;
-          } else {
-            if (jx) {
-              if (MODE == TRAV_COUNT) { ++ndraw; dx = 1; } else dx = jiggle(draws[dptr++]);
-              l += dx * dx;
-            }
-            if (jy) {
-              if (MODE == TRAV_COUNT) { ++ndraw; dy = 1; } else dy = jiggle(draws[dptr++]);
-              l += dy * dy;
-            }
-            if (l < dmin2) l = __builtin_sqrt(dmin2 * l);
-            vx += dx * value * alpha / l;
-            vy += dy * value * alpha / l;
-            ++n_accept;
+    // Geometry on every lane (used where `in`); a lane accepts the cell when
+    // the Barnes-Hut test passes and the cell lies within distanceMax.
+    double dx = q->qx - nx, dy = q->qy - ny;
+    double l = dx * dx + dy * dy;
+    const bool far = q->w2t < l;  // Barnes-Hut approximation
+    const bool open = in && !far;
+    const bool acc = in && far && l < dmax2;
+    n_visit += in ? 1u : 0u;
+    if (__ballot(acc) != 0) {
+      if (__ballot(acc && (dx == 0 || dy == 0 || l < dmin2)) == 0) {
+        // the common case, no lane needing a jiggle or distanceMin: d3's
+        // terms added on the accepting lanes by select
+        const double tx = dx * value * alpha / l, ty = dy * value * alpha / l;
+        vx = acc ? vx + tx : vx;
+        vy = acc ? vy + ty : vy;
+        n_accept += acc ? 1u : 0u;
+      } else if (acc) {
+        const bool jx = dx == 0, jy = dy == 0;
+        if (MODE == TRAV_RECORD && (jx || jy)) {
+          aborted = true;
+        } else {
+          if (jx) {
+            if (MODE == TRAV_COUNT) { ++ndraw; dx = 1; } else dx = jiggle(draws[dptr++]);
+            l += dx * dx;
           }
+          if (jy) {
+            if (MODE == TRAV_COUNT) { ++ndraw; dy = 1; } else dy = jiggle(draws[dptr++]);
+            l += dy * dy;
+          }
+          if (l < dmin2) l = __builtin_sqrt(dmin2 * l);
+          vx += dx * value * alpha / l;
+          vy += dy * value * alpha / l;
+          ++n_accept;
         }
-      } else {
-        open = true;
       }
     }
     if (MODE == TRAV_RECORD) alive &= ~__ballot(aborted);
