@@ -164,6 +164,15 @@ __global__ __launch_bounds__(ACC_THREADS) void k_accumulate_level(
   }
 }
 
+// A TCell as 16 dwords: qx 0-1, qy 2-3, value 4-5, w2t 6-7, child 8-11,
+// first 12, count 13, head 14, flags 15.
+typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
+static_assert(sizeof(u32x16) == sizeof(TCell), "traversal record");
+
+__device__ inline double rec_f64(const u32x16& r, int w) {
+  return __builtin_bit_cast(double, (uint64_t)r[w + 1] << 32 | r[w]);
+}
+
 // d3's jiggle: (random() - 0.5) * 1e-6 (rng.zig:7-11).
 __device__ inline double jiggle(double r) { return (r - 0.5) * 1e-6; }
 
@@ -218,15 +227,17 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     const uint32_t mhi = __builtin_amdgcn_readfirstlane((uint32_t)(mraw >> 32));
     const uint64_t M = ((uint64_t)mhi << 32 | mlo) & alive;
     if (M == 0) continue;
-    const TCell* q = tc + c;
-    const double value = q->value;
+    // The whole 64-byte record in one scalar load (fields as TCell lays them
+    // out), so a visit waits on one load rather than one per field group.
+    const u32x16 rec = *reinterpret_cast<const u32x16*>(tc + c);
+    const double value = rec_f64(rec, 4);
     if (!truthy(value)) continue;
     const bool in = (M >> lane) & 1;
     // Geometry on every lane (used where `in`); a lane accepts the cell when
     // the Barnes-Hut test passes and the cell lies within distanceMax.
-    double dx = q->qx - nx, dy = q->qy - ny;
+    double dx = rec_f64(rec, 0) - nx, dy = rec_f64(rec, 2) - ny;
     double l = dx * dx + dy * dy;
-    const bool far = q->w2t < l;  // Barnes-Hut approximation
+    const bool far = rec_f64(rec, 6) < l;  // Barnes-Hut approximation
     const bool open = in && !far;
     const bool acc = in && far && l < dmax2;
     n_visit += in ? 1u : 0u;
@@ -261,12 +272,12 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     if (MODE == TRAV_RECORD) alive &= ~__ballot(aborted);
     const uint64_t O = __ballot(open);
     if (O == 0) continue;
-    if (!(q->flags & TC_LEAF)) {
+    if (!(rec[15] & TC_LEAF)) {
       if (sp + 4 > (int)stack_cap) break;  // cannot happen with cap = 3 depth + 4
       // Descend: push children 3..0 so 0 is visited first, in one store:
       // lane L < 4 writes child 3 - L, if present, above the present
       // children of higher index.
-      const uint32_t h0 = q->child[0], h1 = q->child[1], h2 = q->child[2], h3 = q->child[3];
+      const uint32_t h0 = rec[8], h1 = rec[9], h2 = rec[10], h3 = rec[11];
       const int e0 = h0 != NONE, e1 = h1 != NONE, e2 = h2 != NONE, e3 = h3 != NONE;
       if (lane < 4) {
         const uint32_t ch = lane == 0 ? h3 : lane == 1 ? h2 : lane == 2 ? h1 : h0;
@@ -281,7 +292,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     }
     // Leaf: points directly (many_body.zig:208-258).
     if (open && !(l >= dmax2)) {
-      const uint32_t first = q->first, cnt = q->count, head = q->head;
+      const uint32_t first = rec[12], cnt = rec[13], head = rec[14];
       bool go = true;
       if (head != i || cnt > 1) {
         const bool jx = dx == 0, jy = dy == 0;
