@@ -50,7 +50,7 @@ namespace d3f {
 namespace {
 
 constexpr int EWF = 256;
-constexpr uint32_t SEG_MAX = 1024;      // sources of one target ordered in LDS
+constexpr uint32_t SEG_MAX = 256;       // M2L sources of one target ordered in LDS (longer: in place)
 constexpr uint32_t NEAR_CHUNK = 64;     // source leaves per near-field chunk
 constexpr uint32_t NEAR_ALL = 0xFFFFFFFFu;
 constexpr uint32_t LONG_IDS = 262144;   // cell ids per pass of the long-segment sort (32 KB of LDS bitmap)
