@@ -42,6 +42,7 @@ constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 keys per tile
+static_assert(RS_THREADS == 256, "one thread per radix digit");
 constexpr int EW_THREADS = 256;                 // element-wise kernels
 
 struct BoxPartial {
@@ -403,7 +404,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scan(uint32_t* __restrict__ h
 
 // Each wave ranks its 1024 consecutive keys in 16 slots of 64 with ballot
 // matching (stable: slot order, then lane order), the waves are stacked in
-// order, and the tile lands at its scanned digit offsets.
+// order, and the tile is placed in LDS in digit order; consecutive threads
+// then write consecutive keys of one digit's run to its scanned offset.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
                                                            uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
@@ -412,6 +414,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
                                                            int shift) {
   __shared__ uint32_t wcount[RS_THREADS / 64][256];
   __shared__ uint32_t wbase[RS_THREADS / 64][256];
+  __shared__ uint32_t tstart[256];      // first slot of each digit in the tile
+  __shared__ uint32_t wsum[RS_THREADS / 64];
+  __shared__ uint64_t skey[RS_TILE];    // the tile in digit order
+  __shared__ uint32_t sval[RS_TILE];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int d = threadIdx.x; d < 256 * (RS_THREADS / 64); d += RS_THREADS) (&wcount[0][0])[d] = 0;
   __syncthreads();
@@ -444,6 +450,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
   }
   __syncthreads();
   {
+    // thread d: the waves' bases within digit d, then the digit's first slot
+    // in the tile (exclusive scan of the tile's digit counts)
     uint32_t b = 0;
     const uint32_t d = threadIdx.x;
 #pragma unroll
@@ -451,6 +459,19 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
       wbase[w][d] = b;
       b += wcount[w][d];
     }
+    uint32_t inc = b;
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) {
+      const uint32_t t = __shfl_up(inc, k, 64);
+      if (lane >= k) inc += t;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+#pragma unroll
+    for (int w = 0; w < RS_THREADS / 64; ++w)
+      if (w < wave) before += wsum[w];
+    tstart[d] = before + inc - b;
   }
   __syncthreads();
 #pragma unroll
@@ -458,9 +479,23 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
     uint32_t k = base + j * 64 + lane;
     if (k < m) {
       uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
-      uint32_t pos = hist[d * ntiles + blockIdx.x] + wbase[wave][d] + rank[j];
-      kout[pos] = key[j];
-      vout[pos] = val[j];
+      const uint32_t slot = tstart[d] + wbase[wave][d] + rank[j];
+      skey[slot] = key[j];
+      sval[slot] = val[j];
+    }
+  }
+  __syncthreads();
+  const uint32_t tile0 = blockIdx.x * RS_TILE;  // tiles past the count hold no key
+  const uint32_t tile_n = tile0 >= m ? 0 : (m - tile0 < (uint32_t)RS_TILE ? m - tile0 : (uint32_t)RS_TILE);
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    const uint32_t t = j * RS_THREADS + threadIdx.x;
+    if (t < tile_n) {
+      const uint64_t kk = skey[t];
+      const uint32_t d = (uint32_t)(kk >> shift) & 255u;
+      const uint32_t pos = hist[d * ntiles + blockIdx.x] + (t - tstart[d]);
+      kout[pos] = kk;
+      vout[pos] = sval[t];
     }
   }
 }
