@@ -262,7 +262,8 @@ def device_sync(core):
 
 
 def read_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
+    """HBM bytes per tick of `kernel` (all its dispatches in one tick) from the
+    committed rocprofv3 PMC summary; the traversal is one dispatch per tick."""
     path = os.path.join(ROOT, "profiles", "exact_1m_tick_hbm_traffic.md")
     if not os.path.exists(path):
         return None
