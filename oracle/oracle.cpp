@@ -609,6 +609,75 @@ int32_t link_exact(Sim& s, Force& f, double alpha) {
   return OK;
 }
 
+// ---------------------------------------------------------------- collide.zig (exact)
+// Each iteration: tree on (x + vx, y + vy) (collide.zig:32-33), the radius of
+// every cell bottom-up (Prep, collide.zig:104-130: a leaf takes its chain
+// head's radius, an internal cell the largest child radius above 0; cells are
+// numbered in preorder, so descending index visits children first), then the
+// Gauss-Seidel pass over i (collide.zig:45-101).
+void collide_exact(Sim& s, Force& f) {
+  const double* radii = f.f0.data();
+  const double strength = f.scalar(S_STRENGTH);
+  const double iterations = f.scalar(S_ITERATIONS);
+  const double* xs = s.x.data(); const double* ys = s.y.data();
+  double* vxs = s.vx.data(); double* vys = s.vy.data();
+  std::vector<uint32_t> stack;
+  for (double k = 0; k < iterations; k += 1) {
+    s.shift();
+    s.tree.build(s.px.data(), s.py.data(), s.n);
+    Tree& t = s.tree;
+    if (t.root == NONE) continue;
+    Cell* cells = t.cells.data();
+    const uint32_t* perm = t.perm.data();
+    for (uint32_t c = t.len; c-- > 0;) {
+      Cell& q = cells[c];
+      if (q.leaf()) {
+        q.r = radii[perm[q.first + q.count - 1]];
+      } else {
+        q.r = 0;
+        for (uint32_t ch : q.child)
+          if (ch != NONE && cells[ch].r > q.r) q.r = cells[ch].r;
+      }
+    }
+    for (uint32_t i = 0; i < s.n; ++i) {
+      const double ri = radii[i], ri2 = ri * ri;
+      const double xi = xs[i] + vxs[i], yi = ys[i] + vys[i];
+      stack.assign(1, t.root);
+      while (!stack.empty()) {
+        const Cell& q = cells[stack.back()];
+        stack.pop_back();
+        double rj = q.r, r = ri + rj;
+        if (q.leaf()) {
+          const uint32_t j = perm[q.first + q.count - 1];
+          if (j > i) {
+            double x = xi - xs[j] - vxs[j], y = yi - ys[j] - vys[j];
+            double l = x * x + y * y;
+            if (l < r * r) {
+              if (x == 0) { x = jiggle(js_random()); l += x * x; }
+              if (y == 0) { y = jiggle(js_random()); l += y * y; }
+              l = std::sqrt(l);
+              l = (r - l) / l * strength;
+              x *= l;
+              y *= l;
+              rj *= rj;
+              r = rj / (ri2 + rj);
+              vxs[i] += x * r;
+              vys[i] += y * r;
+              r = 1 - r;
+              vxs[j] -= x * r;
+              vys[j] -= y * r;
+            }
+          }
+          continue;
+        }
+        if (q.x0 > xi + r || q.x1 < xi - r || q.y0 > yi + r || q.y1 < yi - r) continue;
+        for (unsigned c = 4; c-- > 0;)
+          if (q.child[c] != NONE) stack.push_back(q.child[c]);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- simple.zig
 void center(Sim& s, Force& f) {  // simple.zig:14-31
   const uint32_t n = s.n;
@@ -1425,7 +1494,8 @@ EXPORT int32_t oracle_force_run(uint32_t h, uint32_t fh, double alpha) {  // roo
       return OK;
     case K_COLLIDE:
       if (fast_mode(*s, *f)) { grid_collide(*s, *f); return OK; }
-      return BAD_ARGUMENT;  // exact collide (collide.zig) is not restated here
+      collide_exact(*s, *f);
+      return OK;
     default: return BAD_ARGUMENT;
   }
 }

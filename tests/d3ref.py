@@ -18,7 +18,7 @@ def jiggle(random):  # d3-force jiggle.js
 
 
 class Leaf:
-    __slots__ = ("data", "next", "x", "y", "value")
+    __slots__ = ("data", "next", "x", "y", "value", "r", "kx", "ky")
 
     def __init__(self, data):
         self.data = data
@@ -71,6 +71,7 @@ class Quadtree:
         if math.isnan(x) or math.isnan(y):
             return
         leaf = Leaf(d)
+        leaf.kx, leaf.ky = x, y  # the accessor's coordinates (add.js reads them through tree._x, tree._y)
         node = self.root
         x0, y0, x1, y1 = self.x0, self.y0, self.x1, self.y1
         if node is None:
@@ -96,7 +97,7 @@ class Quadtree:
             if node is None:
                 parent[i] = leaf
                 return
-        xp, yp = node.data["x"], node.data["y"]
+        xp, yp = node.kx, node.ky
         if x == xp and y == yp:
             leaf.next = node
             if parent is not None:
@@ -130,12 +131,13 @@ class Quadtree:
         parent[j] = node
         parent[i] = leaf
 
-    def add_all(self, data):  # d3-quadtree addAll
+    def add_all(self, data, fx=None, fy=None):  # d3-quadtree addAll
         x0 = y0 = INF
         x1 = y1 = -INF
         pts = []
         for d in data:
-            x, y = d["x"], d["y"]
+            x = fx(d) if fx else d["x"]
+            y = fy(d) if fy else d["y"]
             if math.isnan(x) or math.isnan(y):
                 pts.append(None)
                 continue
@@ -356,6 +358,78 @@ class Link:
                 source["vx"] += x * b
                 source["vy"] += y * b
             k += 1
+
+
+class Collide:
+    """d3-force collide.js: per iteration a quadtree on x + vx, y + vy with
+    each quad's radius from visitAfter(prepare), then for every node a visit
+    that pushes apart the overlapping later nodes (data.index > node.index)."""
+
+    def __init__(self, radius=1.0, strength=1.0, iterations=1):
+        self.radius = radius
+        self.strength = strength
+        self.iterations = iterations
+
+    def initialize(self, nodes, random):
+        self.nodes = nodes
+        self.random = random
+        self.radii = [0.0] * len(nodes)
+        for i, node in enumerate(nodes):
+            r = self.radius
+            self.radii[node["index"]] = float(r(node, i) if callable(r) else r)
+
+    def __call__(self, alpha):
+        radii, strength = self.radii, self.strength
+
+        def prepare(quad, *_):
+            if isinstance(quad, Leaf):
+                quad.r = radii[quad.data["index"]]
+                return
+            quad.r = 0.0
+            for q in quad:
+                if q is not None and q.r > quad.r:
+                    quad.r = q.r
+
+        for _ in range(self.iterations):
+            tree = Quadtree().add_all(self.nodes, lambda d: d["x"] + d["vx"], lambda d: d["y"] + d["vy"])
+            tree.visit_after(prepare)
+            for node in self.nodes:
+                ri = radii[node["index"]]
+                ri2 = ri * ri
+                xi = node["x"] + node["vx"]
+                yi = node["y"] + node["vy"]
+
+                def apply(quad, x0, y0, x1, y1):
+                    rj = quad.r
+                    r = ri + rj
+                    if isinstance(quad, Leaf):
+                        data = quad.data
+                        if data["index"] > node["index"]:
+                            x = xi - data["x"] - data["vx"]
+                            y = yi - data["y"] - data["vy"]
+                            l = x * x + y * y
+                            if l < r * r:
+                                if x == 0:
+                                    x = jiggle(self.random)
+                                    l += x * x
+                                if y == 0:
+                                    y = jiggle(self.random)
+                                    l += y * y
+                                l = _sqrt(l)
+                                l = _div(r - l, l) * strength
+                                x *= l
+                                y *= l
+                                rj *= rj
+                                r = _div(rj, ri2 + rj)
+                                node["vx"] += x * r
+                                node["vy"] += y * r
+                                r = 1 - r
+                                data["vx"] -= x * r
+                                data["vy"] -= y * r
+                        return False
+                    return x0 > xi + r or x1 < xi - r or y0 > yi + r or y1 < yi - r
+
+                tree.visit(apply)
 
 
 class Center:

@@ -38,6 +38,11 @@ def defaults(nodes, links):
     return nodes, [("charge", {}), ("link", {"links": links}), ("center", {})]
 
 
+def with_collide(spec, radius):
+    nodes, forces = spec
+    return nodes, forces + [("collide", {"radius": radius})]
+
+
 def _scenarios():
     S = {}
     for n in (10, 100, 300):
@@ -45,15 +50,16 @@ def _scenarios():
     S["grid"] = lambda r: defaults([{"x": float(i % 20), "y": float(i // 20)} for i in range(400)],
                                    [l for i in range(400) for l in
                                     ([(i, i + 1)] if i % 20 < 19 else []) + ([(i, i + 20)] if i < 380 else [])])
-    S["all-at-origin"] = lambda r: defaults([{"x": 0.0, "y": 0.0} for _ in range(60)], random_links(60, 60, r))
-    S["coincident-groups"] = lambda r: defaults([{"x": (i % 6) * 10.0, "y": (i % 4) * 10.0} for i in range(120)],
-                                                random_links(120, 100, r))
+    S["all-at-origin"] = lambda r: with_collide(
+        defaults([{"x": 0.0, "y": 0.0} for _ in range(60)], random_links(60, 60, r)), 2.0)
+    S["coincident-groups"] = lambda r: with_collide(
+        defaults([{"x": (i % 6) * 10.0, "y": (i % 4) * 10.0} for i in range(120)], random_links(120, 100, r)), 3.0)
     S["near-1e300"] = lambda r: defaults([{"x": 1e300 * (r() - 0.5), "y": 1e300 * (r() - 0.5)} for _ in range(50)],
                                          random_links(50, 40, r))
-    S["near-duplicates"] = lambda r: defaults(
+    S["near-duplicates"] = lambda r: with_collide(defaults(
         [{"x": i * 300.0, "y": i * 300.0} if i < 4 else
          {"x": 0.5 + math.floor(r() * 6) * 1e-12, "y": 0.25 + math.floor(r() * 6) * 3e-13} for i in range(80)],
-        random_links(80, 40, r))
+        random_links(80, 40, r)), 1.0)
 
     def single(force, n=200):
         def make(r):
@@ -62,6 +68,7 @@ def _scenarios():
         return make
 
     S["only-center"] = single(("center", lambda r: {"x": 30 * r(), "y": -20 * r(), "strength": 0.7}))
+    S["only-collide"] = single(("collide", {"radius": 12.0}))
     S["only-link"] = lambda r: (random_nodes(150, r), [("link", {"links": random_links(150, 300, r)})])
     S["only-many-body"] = single(("charge", {}))
     S["only-radial"] = single(("radial", lambda r: {"radius": 100.0, "x": 10 * r(), "y": 5.0}))
@@ -69,6 +76,7 @@ def _scenarios():
     S["only-y"] = single(("y", lambda r: {"target": -50 * r()}))
     S["nan-accessors"] = lambda r: (random_nodes(100, r), [
         ("charge", {"strength": lambda i: math.nan if i % 13 == 0 else -20.0}),
+        ("collide", {"radius": lambda i: math.nan if i % 11 == 0 else 3.0}),
         ("x", {"target": lambda i: math.nan if i % 7 == 0 else 0.0}),
         ("radial", {"radius": lambda i: math.nan if i % 5 == 0 else 80.0})])
     S["nan-link-distance"] = lambda r: (random_nodes(60, r), [
@@ -94,17 +102,18 @@ def _scenarios():
     S["iterations-and-parameters"] = lambda r: (random_nodes(200, r), [
         ("charge", {"strength": -15.0, "theta": 0.5}),
         ("link", {"links": random_links(200, 250, r), "iterations": 2, "distance": 15.0}),
+        ("collide", {"radius": 6.0, "iterations": 3, "strength": 0.7}),
         ("sim", {"alpha_decay": 0.01, "alpha_target": 0.2, "velocity_decay": 0.25})])
 
     def nan_initial(r):
         nodes = []
         for i, d in enumerate(random_nodes(100, r)):
             nodes.append({"y": d["y"]} if i % 4 == 0 else ({"x": d["x"], "vx": math.nan} if i % 4 == 1 else d))
-        return defaults(nodes, random_links(100, 120, r))
+        return with_collide(defaults(nodes, random_links(100, 120, r)), 2.0)
     S["nan-initial-positions"] = nan_initial
 
     def xs(r):
-        nodes, forces = defaults([{"x": 0.0, "y": 0.0} for _ in range(40)], random_links(40, 30, r))
+        nodes, forces = with_collide(defaults([{"x": 0.0, "y": 0.0} for _ in range(40)], random_links(40, 30, r)), 2.0)
         return nodes, forces + [("random", {"seed": math.floor(r() * 2 ** 31)})]
     S["random-source-xorshift"] = xs
     S["pinned-nodes"] = lambda r: defaults(
@@ -143,6 +152,8 @@ def build_d3ref(nodes, forces):
             sim.force(d3ref.Axis(name, acc(p.get("target", 0.0))))
         elif name == "radial":
             sim.force(d3ref.Radial(acc(p["radius"]), p.get("x", 0.0), p.get("y", 0.0)))
+        elif name == "collide":
+            sim.force(d3ref.Collide(acc(p["radius"]), p.get("strength", 1.0), p.get("iterations", 1)))
     return sim
 
 
@@ -170,6 +181,10 @@ def build_facade(core, nodes, forces):
             f = d3.forceY(p.get("target", 0.0))
         elif name == "radial":
             f = d3.forceRadial(p["radius"], p.get("x", 0.0), p.get("y", 0.0))
+        elif name == "collide":
+            rad = p["radius"]
+            f = d3.forceCollide([rad(i) for i in range(len(nodes))] if callable(rad) else rad)
+            f.strength(p.get("strength", 1.0)).iterations(p.get("iterations", 1))
         else:
             continue
         sim.force("%s%d" % (name, k), f)
