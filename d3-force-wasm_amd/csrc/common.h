@@ -140,7 +140,51 @@ size_t sort_hist_words(uint32_t n_max);  // hist size sort_digits needs
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t n_max, const uint32_t* count_dev, uint32_t* partials,
                     uint32_t* total_out, hipStream_t stream);
 
+// ---- collide.hip ----
+constexpr uint32_t COLLIDE_K = 64;        // candidates per node
+constexpr uint32_t COLLIDE_T = 32;        // undecided ancestor cells per node
+constexpr uint32_t COLLIDE_STACK = 160;   // walk stack of the candidate pass
+// Status bits of a parallel pass; any of them sends the iteration to the
+// serial kernel.
+constexpr uint32_t COLLIDE_FULL = 1;      // candidates, ancestors or stack beyond their bounds
+constexpr uint32_t COLLIDE_MOVED = 2;     // a position left its box
+constexpr uint32_t COLLIDE_JIGGLE = 4;    // an overlapping pair with a zero delta
+constexpr uint32_t COLLIDE_STALLED = 8;   // a wait made no progress for a second
+struct CollideArgs {
+  Cell* cells;
+  const TreeMeta* meta;
+  const uint32_t* perm;
+  const double *px, *py;      // tree positions x + vx, y + vy at the iteration's start
+  const double *x, *y;
+  double *vx, *vy;
+  const double* radii;
+  double strength;
+  double scale;               // box half-width in units of |strength| * largest radius
+  uint32_t n;
+  uint32_t key_shift;         // pair keys are j << key_shift | i
+  uint32_t *ccount, *coff, *cand, *clink, *tcell, *tpar;  // per node: count, pair offset, lists
+  uint64_t* pkeys;
+  uint32_t* pvals;
+  uint32_t *run_start, *rank, *indeg, *done;
+  uint32_t *status, *ticket;
+};
+struct CollideSerialState {
+  uint32_t node, sp, inside, need, has_jig, pad;
+  double xi, yi;
+  double jig[2];
+};
+// Cell radii for collide (collide.zig:104-130) into t.cells[].r.
+hipError_t collide_radii(TreeBuffers& t, const double* radii, hipStream_t stream);
+hipError_t collide_candidates(const CollideArgs& a, hipStream_t stream);
+hipError_t collide_pairs(const CollideArgs& a, hipStream_t stream);
+hipError_t collide_ranks(const CollideArgs& a, uint32_t pairs, hipStream_t stream);
+hipError_t collide_apply(const CollideArgs& a, hipStream_t stream);
+hipError_t collide_serial(const CollideArgs& a, uint32_t* stack, CollideSerialState* st, hipStream_t stream);
+
 // ---- many_body.hip ----
+// Cells grouped by depth: t.dorder[t.dstart[d] .. t.dstart[d + 1]) holds the
+// cells of depth d (after a tree build, whose meta_host gives len and depth).
+hipError_t tree_depth_order(TreeBuffers& t, hipStream_t stream);
 // visitAfter charge sums (many_body.zig:101-135) into tcells (qx, qy, value).
 hipError_t mb_accumulate(TreeBuffers& t, const double* x, const double* y, const double* strengths, double theta2,
                          hipStream_t stream);

@@ -367,8 +367,7 @@ __global__ void k_nan_nodes(const uint32_t* __restrict__ rest, uint32_t count, c
 
 }  // namespace
 
-hipError_t mb_accumulate(TreeBuffers& t, const double* x, const double* y, const double* strengths, double theta2,
-                         hipStream_t stream) {
+hipError_t tree_depth_order(TreeBuffers& t, hipStream_t stream) {
   const uint32_t len = t.meta_host->len;
   if (len == 0) return hipSuccess;
   const uint32_t nbins = t.meta_host->depth + 1;
@@ -381,6 +380,15 @@ hipError_t mb_accumulate(TreeBuffers& t, const double* x, const double* y, const
   hipLaunchKernelGGL(k_depth_scan, dim3(1), dim3(64), 0, stream, t.dbins, nbins, t.dstart);
   hipLaunchKernelGGL(k_depth_scatter, dim3(ceil_div(len, ACC_THREADS * SCATTER_PER)), dim3(ACC_THREADS), lds, stream,
                      t.cdepth, len, nbins, t.dbins, t.dorder);
+  return hipGetLastError();
+}
+
+hipError_t mb_accumulate(TreeBuffers& t, const double* x, const double* y, const double* strengths, double theta2,
+                         hipStream_t stream) {
+  const uint32_t len = t.meta_host->len;
+  if (len == 0) return hipSuccess;
+  const uint32_t nbins = t.meta_host->depth + 1;
+  HIPCHECK(tree_depth_order(t, stream));
   uint32_t level_blocks = ceil_div(len, ACC_THREADS);
   if (level_blocks > 2048) level_blocks = 2048;
   for (uint32_t d = nbins; d-- > 0;) {

@@ -78,6 +78,7 @@ struct Force {
   bool dirty = true;
   // exact link schedule, level-major
   bool sched_valid = false;
+  double collide_scale = 0.5;  // exact collide: box scale of the last parallel pass
   uint32_t sched_n = 0;
   std::vector<uint32_t> level_off;
   Device<uint32_t> lsrc, ltgt;
@@ -194,6 +195,13 @@ struct Sim {
   LinkSerialState* dlink = nullptr;
   uint32_t* hscalar_u = nullptr;   // pinned mirrors
   LinkSerialState* hlink = nullptr;
+  // exact collide: per-node candidate lists, pair ranks, dataflow counters,
+  // the velocities at the iteration's start and the serial pass's state
+  Device<uint32_t> c_count, c_off, c_cand, c_link, c_tcell, c_tpar, c_run, c_rank, c_indeg, c_done, c_part,
+      c_stack;
+  Device<double> c_svx, c_svy;
+  CollideSerialState* dcol = nullptr;
+  CollideSerialState* hcol = nullptr;
   // host copies for tree introspection
   Pinned<Cell> hcells;
   Pinned<uint32_t> hperm;
@@ -232,6 +240,13 @@ struct Sim {
     if (dlink) (void)hipFree(dlink);
     if (hscalar_u) (void)hipHostFree(hscalar_u);
     if (hlink) (void)hipHostFree(hlink);
+    for (auto* a : {&c_count, &c_off, &c_cand, &c_link, &c_tcell, &c_tpar, &c_run, &c_rank, &c_indeg, &c_done,
+                    &c_part, &c_stack})
+      a->release();
+    c_svx.release();
+    c_svy.release();
+    if (dcol) (void)hipFree(dcol);
+    if (hcol) (void)hipHostFree(hcol);
     hcells.release();
     hperm.release();
     for (auto& t : pending) {
@@ -258,6 +273,8 @@ struct Sim {
     HIPCHECK(hipMalloc(&dlink, sizeof(LinkSerialState)));
     HIPCHECK(hipHostMalloc((void**)&hscalar_u, 16 * sizeof(uint32_t), hipHostMallocDefault));
     HIPCHECK(hipHostMalloc((void**)&hlink, sizeof(LinkSerialState), hipHostMallocDefault));
+    HIPCHECK(hipMalloc(&dcol, sizeof(CollideSerialState)));
+    HIPCHECK(hipHostMalloc((void**)&hcol, sizeof(CollideSerialState), hipHostMallocDefault));
     HIPCHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     HIPCHECK(hipEventCreateWithFlags(&ev_x_ready, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&ev_sums_done, hipEventDisableTiming));
@@ -605,6 +622,134 @@ int32_t run_link_exact(Sim& s, Force& f, double alpha) {
       st->has_jig = 1;
     }
   }
+  return OK;
+}
+
+// ---------------------------------------------------------------- collide (exact)
+// One iteration of collide.zig:27-101: the tree on x + vx, the cell radii,
+// then the pass over i as a dataflow over the candidate conflicts
+// (collide.hip); when the parallel pass cannot guarantee the reference's
+// result (see collide.hip) the velocities are restored and the iteration
+// runs on the serial kernel, which takes jiggles from the host's random in
+// pass order.
+hipError_t collide_iteration(Sim& s, Force& f) {
+  const uint32_t n = s.n;
+  TreeBuffers& t = s.tree;
+  s.charges_pending = false;
+  HIPCHECK(shift_positions(s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, s.dpx.p, s.dpy.p, n, s.stream));
+  HIPCHECK(tree_build_exact(t, s.dpx.p, s.dpy.p, n, s.stream));
+  if (!t.meta_host->valid) return hipSuccess;  // collide.zig:34
+  HIPCHECK(collide_radii(t, f.df[0].p, s.stream));
+  const size_t c = n > 0 ? n : 1;
+  for (auto* b : {&s.c_count, &s.c_off, &s.c_run, &s.c_indeg, &s.c_done}) HIPCHECK(b->reserve(c, 0, s.stream));
+  HIPCHECK(s.c_cand.reserve(c * COLLIDE_K, 0, s.stream));
+  HIPCHECK(s.c_link.reserve(c * COLLIDE_K, 0, s.stream));
+  HIPCHECK(s.c_tcell.reserve(c * COLLIDE_T, 0, s.stream));
+  HIPCHECK(s.c_tpar.reserve(c * COLLIDE_T, 0, s.stream));
+  HIPCHECK(s.c_part.reserve(c / 4096 + 2, 0, s.stream));
+  HIPCHECK(s.c_svx.reserve(c, 0, s.stream));
+  HIPCHECK(s.c_svy.reserve(c, 0, s.stream));
+  uint32_t* w = s.dscalar_u;  // [4] pairs, [5] status, [6] ticket, [7] n
+  CollideArgs a{};
+  a.cells = t.cells;
+  a.meta = t.meta;
+  a.perm = t.perm;
+  a.px = s.dpx.p;
+  a.py = s.dpy.p;
+  a.x = s.dx.p;
+  a.y = s.dy.p;
+  a.vx = s.dvx.p;
+  a.vy = s.dvy.p;
+  a.radii = f.df[0].p;
+  a.strength = f.scalar(S_STRENGTH);
+  a.n = n;
+  a.key_shift = n <= 2 ? 1 : 32 - __builtin_clz(n - 1);
+  a.ccount = s.c_count.p;
+  a.coff = s.c_off.p;
+  a.cand = s.c_cand.p;
+  a.clink = s.c_link.p;
+  a.tcell = s.c_tcell.p;
+  a.tpar = s.c_tpar.p;
+  a.run_start = s.c_run.p;
+  a.indeg = s.c_indeg.p;
+  a.done = s.c_done.p;
+  a.status = w + 5;
+  a.ticket = w + 6;
+  HIPCHECK(hipMemsetD32Async((hipDeviceptr_t)(w + 7), (int)n, 1, s.stream));
+  HIPCHECK(hipMemcpyAsync(s.c_svx.p, s.dvx.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+  HIPCHECK(hipMemcpyAsync(s.c_svy.p, s.dvy.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+  // A pass whose pushes leave their boxes is tried once more with boxes
+  // three times wider; any other stop goes to the serial pass.
+  bool serial = !std::isfinite(a.strength);
+  for (int attempt = 0; attempt < 2 && !serial; ++attempt) {
+    a.scale = f.collide_scale;
+    HIPCHECK(hipMemsetAsync(w + 4, 0, 3 * sizeof(uint32_t), s.stream));
+    HIPCHECK(collide_candidates(a, s.stream));
+    HIPCHECK(scan_u32(a.ccount, a.coff, n, w + 7, s.c_part.p, w + 4, s.stream));
+    HIPCHECK(hipMemcpyAsync(s.hscalar_u + 4, w + 4, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream));
+    HIPCHECK(hipStreamSynchronize(s.stream));
+    const uint32_t pairs = s.hscalar_u[4];
+    if (s.hscalar_u[5] != 0) {
+      s.stats.collide_status |= s.hscalar_u[5];
+      serial = true;
+      break;
+    }
+    if (pairs == 0) return hipSuccess;  // no pair can overlap: the pass pushes nothing
+    SortBuffers sb;
+    HIPCHECK(s.sort_buffers(pairs, &sb));
+    HIPCHECK(s.c_rank.reserve(pairs, 0, s.stream));
+    a.pkeys = sb.k0;
+    a.pvals = sb.v0;
+    a.rank = s.c_rank.p;
+    HIPCHECK(collide_pairs(a, s.stream));
+    HIPCHECK(sort_digits(sb.hist, sb.glob, sb.k0, sb.v0, sb.k1, sb.v1, pairs, w + 4, 0, (2 * a.key_shift + 7) / 8,
+                         s.stream));
+    HIPCHECK(hipMemsetAsync(a.indeg, 0, (size_t)n * sizeof(uint32_t), s.stream));
+    HIPCHECK(hipMemsetAsync(a.done, 0, (size_t)n * sizeof(uint32_t), s.stream));
+    HIPCHECK(collide_ranks(a, pairs, s.stream));
+    HIPCHECK(collide_apply(a, s.stream));
+    HIPCHECK(hipMemcpyAsync(s.hscalar_u + 5, w + 5, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream));
+    HIPCHECK(hipStreamSynchronize(s.stream));
+    const uint32_t status = s.hscalar_u[5];
+    if (status == 0) {
+      f.collide_scale = std::max(0.25, 0.75 * f.collide_scale);
+      return hipSuccess;
+    }
+    s.stats.collide_status |= status;
+    HIPCHECK(hipMemcpyAsync(s.dvx.p, s.c_svx.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+    HIPCHECK(hipMemcpyAsync(s.dvy.p, s.c_svy.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+    if (status != COLLIDE_MOVED || f.collide_scale >= 8) {
+      serial = true;
+      break;
+    }
+    f.collide_scale *= 3;
+    serial = attempt == 1;
+  }
+  ++s.stats.collide_serial;
+  HIPCHECK(s.c_stack.reserve(3 * (size_t)t.meta_host->depth + 8, 0, s.stream));
+  CollideSerialState* st = s.hcol;
+  std::memset(st, 0, sizeof(*st));
+  for (;;) {
+    HIPCHECK(hipMemcpyAsync(s.dcol, st, sizeof(*st), hipMemcpyHostToDevice, s.stream));
+    HIPCHECK(collide_serial(a, s.c_stack.p, s.dcol, s.stream));
+    HIPCHECK(hipMemcpyAsync(st, s.dcol, sizeof(*st), hipMemcpyDeviceToHost, s.stream));
+    HIPCHECK(hipStreamSynchronize(s.stream));
+    if (st->node >= n) break;
+    if (st->need & 1) st->jig[0] = jiggle_host(draw());
+    if (st->need & 2) st->jig[1] = jiggle_host(draw());
+    st->has_jig = 1;
+  }
+  return hipSuccess;
+}
+
+int32_t run_collide_exact(Sim& s, Force& f) {  // collide.zig:17-102
+  const double iterations = f.scalar(S_ITERATIONS);
+  s.timer_start(D3F_PHASE_COLLIDE);
+  for (double k = 0; k < iterations; k += 1) {
+    hipError_t e = collide_iteration(s, f);
+    if (e != hipSuccess) return status_of(e);
+  }
+  s.timer_stop();
   return OK;
 }
 
@@ -1036,9 +1181,7 @@ EXPORT int32_t force_run(uint32_t h, uint32_t fh, double alpha) {  // root.zig:2
     case K_MANY_BODY:
       return fast_mode(*s, *f) ? run_many_body_fast(*s, *f, alpha) : run_many_body_exact(*s, *f, alpha);
     case K_COLLIDE:
-      // Exact collide (collide.zig, Gauss-Seidel pushes onto other nodes) is
-      // not on this build: only the fast grid collide runs.
-      return fast_mode(*s, *f) ? run_collide_fast(*s, *f) : BAD_ARGUMENT;
+      return fast_mode(*s, *f) ? run_collide_fast(*s, *f) : run_collide_exact(*s, *f);
     default:
       return BAD_ARGUMENT;
   }

@@ -73,14 +73,16 @@ class Stats(ctypes.Structure):
     _fields_ = [("ms", ctypes.c_double * len(PHASES)), ("launches", ctypes.c_uint64 * len(PHASES)),
                 ("visits", ctypes.c_uint64), ("accepts", ctypes.c_uint64), ("pairs", ctypes.c_uint64),
                 ("flagged", ctypes.c_uint64), ("link_levels", ctypes.c_uint64),
-                ("m2l", ctypes.c_uint64), ("p2p_pairs", ctypes.c_uint64), ("fmm_cells", ctypes.c_uint64)]
+                ("m2l", ctypes.c_uint64), ("p2p_pairs", ctypes.c_uint64), ("fmm_cells", ctypes.c_uint64),
+                ("collide_serial", ctypes.c_uint64), ("collide_status", ctypes.c_uint64)]
 
     def as_dict(self):
         return {"ms": {p: self.ms[k] for k, p in enumerate(PHASES)},
                 "launches": {p: int(self.launches[k]) for k, p in enumerate(PHASES)},
                 "visits": int(self.visits), "accepts": int(self.accepts), "pairs": int(self.pairs),
                 "flagged": int(self.flagged), "link_levels": int(self.link_levels),
-                "m2l": int(self.m2l), "p2p_pairs": int(self.p2p_pairs), "fmm_cells": int(self.fmm_cells)}
+                "m2l": int(self.m2l), "p2p_pairs": int(self.p2p_pairs), "fmm_cells": int(self.fmm_cells),
+                "collide_serial": int(self.collide_serial), "collide_status": int(self.collide_status)}
 
 _SIGNATURES = {
     "core_set_random": (None, [_RANDOM_FN, ctypes.c_void_p]),

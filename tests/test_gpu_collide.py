@@ -1,0 +1,96 @@
+"""GPU exact forceCollide (collide.zig:17-130): bit-identical (Object.is on x,
+y, vx, vy after every tick, and the random stream's position) to the oracle.
+Covers d3's own collide tests (test/d3/collide-test.js:5-55), layouts with
+shared coordinates (zero deltas that draw jiggles), NaN and negative radii,
+several iterations, and layouts of 20,000 and 200,000 nodes whose overlaps the
+parallel pass resolves without the serial kernel. The parity corpus scenarios
+that carry forceCollide run in test_gpu_exact.py."""
+
+import numpy as np
+import pytest
+
+import d3force_amd as d3
+from parity import build_facade, same
+from test_oracle_collide import layouts
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(gpu_core, oracle_core, make, ticks):
+    a, ta = make(oracle_core)
+    b, tb = make(gpu_core)
+    for t in range(ticks):
+        a.tick()
+        b.tick()
+        assert same(ta.state(), tb.state()), "tick %d" % (t + 1)
+    assert a.random_source()() == b.random_source()(), "random stream position"
+    return b
+
+
+def test_gpu_collide_d3_collides_nodes(gpu_core, oracle_core):  # collide-test.js:5-19
+    print("gpu collide d3 test 1", flush=True)
+    ta, tb = d3.Nodes(3), d3.Nodes(3)
+    ca, cb = d3.forceCollide(1), d3.forceCollide(1)
+    a = d3.forceSimulation(ta, core=oracle_core).force("collide", ca).stop()
+    b = d3.forceSimulation(tb, core=gpu_core).force("collide", cb).stop()
+    a.tick(10)
+    b.tick(10)
+    assert same(ta.state(), tb.state())
+    assert abs(tb.x[1] - -9.03088751750192) < 1e-6 and abs(tb.y[2] - -15.750847141167634) < 1e-6
+    ca.radius(100)
+    cb.radius(100)
+    a.tick(10)
+    b.tick(10)
+    assert same(ta.state(), tb.state())
+    assert abs(tb.x[0] - 174.08616723117228) < 1e-6 and abs(tb.vx[2] - -0.1472585564003713) < 1e-6
+
+
+def test_gpu_collide_d3_jiggles(gpu_core, oracle_core):  # collide-test.js:34-50
+    print("gpu collide d3 jiggle tests", flush=True)
+
+    def two(core):
+        t = d3.Nodes.from_dicts([{"x": 0.0, "y": 0.0}, {"x": 0.0, "y": 0.0}])
+        return d3.forceSimulation(t, core=core).force("collide", d3.forceCollide(1)).stop(), t
+
+    run_both(gpu_core, oracle_core, two, 1)
+
+    def ten(core):
+        t = d3.Nodes.from_dicts([{"x": 0.0, "y": 0.0} for _ in range(10)])
+        return d3.forceSimulation(t, core=core).force("collide", d3.forceCollide()).stop(), t
+
+    run_both(gpu_core, oracle_core, ten, 50)
+
+
+@pytest.mark.parametrize("name", list(layouts()))
+def test_gpu_collide_layouts(gpu_core, oracle_core, name):
+    print("gpu collide layout", name, flush=True)
+    nodes, radius, strength, iterations = layouts()[name]
+    spec = [("collide", {"radius": (lambda i: radius[i]) if isinstance(radius, list) else radius,
+                         "strength": strength, "iterations": iterations}), ("center", {})]
+    run_both(gpu_core, oracle_core, lambda core: build_facade(core, nodes, spec), 25)
+
+
+def scatter(n, seed, spread):
+    rng = np.random.default_rng(seed)
+    xy = rng.uniform(-spread, spread, size=(n, 2))
+    radii = rng.uniform(0.5, 1.5, size=n)
+    return xy, radii
+
+
+@pytest.mark.parametrize("n,spread,ticks", [(20000, 300.0, 8), (200000, 1000.0, 3)])
+def test_gpu_collide_parallel_pass(gpu_core, oracle_core, n, spread, ticks):
+    # Uniform points with radii U[0.5, 1.5]: most nodes overlap a neighbour,
+    # by less than a radius, so every iteration runs as the dataflow (no
+    # serial pass) and matches the oracle bit for bit.
+    print("gpu collide parallel", n, flush=True)
+    xy, radii = scatter(n, 5, spread)
+
+    def make(core):
+        t = d3.Nodes(n, x=xy[:, 0].copy(), y=xy[:, 1].copy())
+        sim = (d3.forceSimulation(t, core=core)
+               .force("collide", d3.forceCollide(radii).strength(0.7).iterations(2)).stop())
+        return sim, t
+
+    b = run_both(gpu_core, oracle_core, make, ticks)
+    st = gpu_core.stats(d3.core.context(b.nodes()).h)
+    assert st["collide_serial"] == 0, "stop reasons %d" % st["collide_status"]
