@@ -17,6 +17,7 @@ cores over a bounded sample of the same workload.
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -218,6 +219,56 @@ def run_batch(core, graphs, steps, warmup, seed=1):
     }
 
 
+def make_collide_workload(core, n, seed=5):
+    """Exact forceCollide alone: n uniform points at 0.0556 nodes per unit
+    area (mean nearest-neighbour distance 2.1), radii U[0.5, 1.5], strength
+    0.7, one iteration per tick."""
+    import numpy as np
+    import d3force_amd as d3
+    rng = np.random.default_rng(seed)
+    half = 0.5 * math.sqrt(n / 0.0556)
+    xy = rng.uniform(-half, half, size=(n, 2))
+    radii = rng.uniform(0.5, 1.5, size=n)
+    nodes = d3.Nodes(n, x=xy[:, 0].copy(), y=xy[:, 1].copy())
+    sim = d3.forceSimulation(nodes, core=core).force("collide", d3.forceCollide(radii).strength(0.7)).stop()
+    return sim
+
+
+def run_collide(core, n, steps, warmup, cpu_ticks=1):
+    sim = make_collide_workload(core, n)
+    ctx = sim.begin()
+    log("collide: warmup %d ticks" % warmup)
+    sim.iterate(warmup, ctx)
+    device_sync(core)
+    core.stats(ctx.h)
+    log("collide: timing %d ticks" % steps)
+    t0 = time.perf_counter()
+    sim.iterate(steps, ctx)
+    device_sync(core)
+    dt = time.perf_counter() - t0
+    st = core.stats(ctx.h)
+    sim.end(ctx)
+    out = {"value": steps / dt, "unit": "ticks/s", "ms_per_step": 1e3 * dt / steps, "steps": steps,
+           "warmup": warmup, "dtype": "f64",
+           "config": {"workload": "exact forceCollide alone, %d nodes uniform at 0.0556 per unit area, radius "
+                                  "U[0.5, 1.5], strength 0.7, 1 iteration" % n, "nodes": n, "mode": "exact"},
+           "serial_iterations": st["collide_serial"], "stop_reasons": st["collide_status"]}
+    if cpu_ticks:
+        from oracle import oracle
+        ocore = oracle.load()
+        osim = make_collide_workload(ocore, n)
+        octx = osim.begin()
+        osim.iterate(warmup, octx)
+        log("collide: oracle %d ticks" % cpu_ticks)
+        t0 = time.perf_counter()
+        osim.iterate(cpu_ticks, octx)
+        cdt = time.perf_counter() - t0
+        osim.end(octx)
+        out["cpu_baseline"] = {"value": cpu_ticks / cdt, "unit": "ticks/s", "cores": 1, "kind": "port",
+                               "sample": "%d ticks after the same %d warm-up ticks" % (cpu_ticks, warmup)}
+    return out
+
+
 def run_warm(core, nodes=100000):
     """The learned warm start with the committed network on held-out graphs of
     `nodes` nodes (two stochastic block models, one Barabasi-Albert graph):
@@ -309,6 +360,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="run only the batched leg (prints its object)")
     ap.add_argument("--warm-nodes", type=int, default=100000, help="nodes of the warm-start graphs (0: skip)")
     ap.add_argument("--warm-only", action="store_true", help="run only the warm-start leg (prints its object)")
+    ap.add_argument("--collide-nodes", type=int, default=1000000, help="nodes of the exact collide leg (0: skip)")
+    ap.add_argument("--collide-only", action="store_true", help="run only the exact collide leg (prints its object)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the tick on a single MI355X "
@@ -325,6 +378,10 @@ def main():
         return
     if args.warm_only:
         print(json.dumps(run_warm(d3.gpu_core(), args.warm_nodes)))
+        return
+    if args.collide_only:
+        print(json.dumps(run_collide(d3.gpu_core(), args.collide_nodes, args.steps, args.warmup,
+                                     0 if args.no_cpu else 1)))
         return
     n = args.nodes
     m = 2 * n
@@ -405,6 +462,8 @@ def main():
                             "the FP64 direct sum is within 1e-6")
     batch = run_batch(core, args.batch_graphs, args.steps, args.warmup) if args.batch_graphs else None
     warm = run_warm(core, args.warm_nodes) if args.warm_nodes else None
+    collide = (run_collide(core, args.collide_nodes, min(args.steps, 10), args.warmup, 0 if args.no_cpu else 1)
+               if args.collide_nodes else None)
 
     cpu = None
     if not args.no_cpu:
@@ -438,6 +497,7 @@ def main():
         "fast_rms_within_1e-6": fast_1e6,
         "batch": batch,
         "warm_start": warm,
+        "exact_collide": collide,
     }
     print(json.dumps(out))
 

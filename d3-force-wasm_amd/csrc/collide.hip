@@ -18,9 +18,9 @@
 //    the scale). It descends every cell that
 //    some position in that box would not prune, keeps the leaves j > i that
 //    some positions in the boxes of i and j would make overlap (the
-//    candidates, in the walk's preorder), and for each candidate the chain of
-//    "uncertain" ancestors, the cells whose prune test the box does not
-//    decide.
+//    candidates, in the walk's preorder), and for each candidate its leaf
+//    cell and its shallowest "uncertain" ancestor, the first cell on its
+//    path whose prune test the box does not decide.
 // 3. The (candidate j, node i) pairs are sorted by (j, i): a pair's rank is
 //    the number of earlier nodes with j among their candidates, and indeg(j)
 //    is the number of all such nodes.
@@ -29,8 +29,8 @@
 //    earlier push onto i has landed) and done[j] == rank for each candidate
 //    (every earlier toucher of j is done, no later one has started). It then
 //    does exactly the reference's arithmetic for node i: xi from the current
-//    vx_i, each candidate's visit decided by evaluating its uncertain
-//    ancestors' prune tests at xi, the overlap test and push at the current
+//    vx_i, each candidate's visit decided by the prune tests at xi on its
+//    path from the uncertain ancestor down, the overlap test and push at the current
 //    vx_j, and finally increments done[] of every node it touched. Leaves that
 //    are not candidates cannot overlap for any positions in the boxes, so
 //    their reads need no order.
@@ -141,16 +141,19 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_cand(const CollideA
   const double pad = 3 * delta;
   const double ri = a.radii[i];
   uint32_t* cand = a.cand + (size_t)i * COLLIDE_K;
-  uint32_t* clink = a.clink + (size_t)i * COLLIDE_K;
-  uint32_t* tcell = a.tcell + (size_t)i * COLLIDE_T;
-  uint32_t* tpar = a.tpar + (size_t)i * COLLIDE_T;
-  uint32_t stk[COLLIDE_STACK], slink[COLLIDE_STACK];
-  uint32_t sp = 1, cnt = 0, nt = 0, fail = 0;
+  uint32_t* cleaf = a.cleaf + (size_t)i * COLLIDE_K;
+  uint32_t* ctop = a.ctop + (size_t)i * COLLIDE_K;
+  // Stack entries carry the shallowest undecided ancestor of their cell
+  // (NONE when the box decides every test above it): a candidate's visit is
+  // then settled at run time by the tests from that ancestor down to its
+  // leaf.
+  uint32_t stk[COLLIDE_STACK], stop[COLLIDE_STACK];
+  uint32_t sp = 1, cnt = 0, fail = 0;
   stk[0] = 0;
-  slink[0] = NONE;
+  stop[0] = NONE;
   while (sp > 0) {
     --sp;
-    const uint32_t c = stk[sp], link = slink[sp];
+    const uint32_t c = stk[sp], top = stop[sp];
     const Cell q = a.cells[c];
     const double r = ri + q.r;
     if (is_leaf(q)) {
@@ -165,7 +168,8 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_cand(const CollideA
             break;
           }
           cand[cnt] = j;
-          clink[cnt] = link;
+          cleaf[cnt] = c;
+          ctop[cnt] = top;
           ++cnt;
         }
       }
@@ -176,24 +180,15 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_cand(const CollideA
     const bool always = q.x0 > xhi + r || q.x1 < xlo - r || q.y0 > yhi + r || q.y1 < ylo - r;
     if (always) continue;
     const bool never = !(q.x0 > xlo + r) && !(q.x1 < xhi - r) && !(q.y0 > ylo + r) && !(q.y1 < yhi - r);
-    uint32_t next = link;
-    if (!never) {
-      if (nt == COLLIDE_T) {
-        fail = COLLIDE_FULL;
-        break;
-      }
-      tcell[nt] = c;
-      tpar[nt] = link;
-      next = nt++;
-    }
+    const uint32_t next = (never || top != NONE) ? top : c;
     if (sp + 4 > COLLIDE_STACK) {
-      fail = COLLIDE_FULL;
+      fail = COLLIDE_FULL_STACK;
       break;
     }
     for (int k = 3; k >= 0; --k) {
       if (q.child[k] != NONE) {
         stk[sp] = q.child[k];
-        slink[sp] = next;
+        stop[sp] = next;
         ++sp;
       }
     }
@@ -278,13 +273,34 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_apply(const Collide
         if (!(fabs(xi - a.px[i]) <= delta) || !(fabs(yi - a.py[i]) <= delta)) fail = COLLIDE_MOVED;
         for (uint32_t k = 0; k < cnt && !fail; ++k) {
           const uint32_t j = a.cand[(size_t)i * COLLIDE_K + k];
+          // The reference visits the leaf when no cell on its root path is
+          // pruned; the cells above ctop pass for any xi in the box. Cells
+          // are numbered in preorder with children in ascending order, so
+          // the path follows the last child whose index is not above the
+          // leaf.
           bool visited = true;
-          for (uint32_t t = a.clink[(size_t)i * COLLIDE_K + k]; t != NONE; t = a.tpar[(size_t)i * COLLIDE_T + t]) {
-            if (pruned(a.cells[a.tcell[(size_t)i * COLLIDE_T + t]], ri, xi, yi)) {
+          const uint32_t leaf = a.cleaf[(size_t)i * COLLIDE_K + k];
+          uint32_t cc = a.ctop[(size_t)i * COLLIDE_K + k];
+          for (uint32_t step = 0; cc != NONE && cc != leaf; ++step) {
+            const Cell q = a.cells[cc];
+            if (step == COLLIDE_STACK || is_leaf(q)) {
+              fail = COLLIDE_FULL_STACK;
+              break;
+            }
+            if (pruned(q, ri, xi, yi)) {
               visited = false;
               break;
             }
+            uint32_t nx = NONE;
+            for (int ch = 0; ch < 4; ++ch)
+              if (q.child[ch] != NONE && q.child[ch] <= leaf) nx = q.child[ch];
+            if (nx == NONE) {
+              fail = COLLIDE_FULL_STACK;
+              break;
+            }
+            cc = nx;
           }
+          if (fail) break;
           if (!visited) continue;
           // collide.zig:58-82
           double vxj = ld(a.vx + j), vyj = ld(a.vy + j);

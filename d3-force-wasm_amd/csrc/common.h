@@ -142,14 +142,14 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t n_max, const uin
 
 // ---- collide.hip ----
 constexpr uint32_t COLLIDE_K = 64;        // candidates per node
-constexpr uint32_t COLLIDE_T = 32;        // undecided ancestor cells per node
 constexpr uint32_t COLLIDE_STACK = 160;   // walk stack of the candidate pass
 // Status bits of a parallel pass; any of them sends the iteration to the
 // serial kernel.
-constexpr uint32_t COLLIDE_FULL = 1;      // candidates, ancestors or stack beyond their bounds
+constexpr uint32_t COLLIDE_FULL = 1;      // more candidates than COLLIDE_K
 constexpr uint32_t COLLIDE_MOVED = 2;     // a position left its box
 constexpr uint32_t COLLIDE_JIGGLE = 4;    // an overlapping pair with a zero delta
 constexpr uint32_t COLLIDE_STALLED = 8;   // a wait made no progress for a second
+constexpr uint32_t COLLIDE_FULL_STACK = 32;  // a walk deeper than COLLIDE_STACK allows
 struct CollideArgs {
   Cell* cells;
   const TreeMeta* meta;
@@ -162,7 +162,8 @@ struct CollideArgs {
   double scale;               // box half-width in units of |strength| * largest radius
   uint32_t n;
   uint32_t key_shift;         // pair keys are j << key_shift | i
-  uint32_t *ccount, *coff, *cand, *clink, *tcell, *tpar;  // per node: count, pair offset, lists
+  uint32_t *ccount, *coff;    // per node: candidates, offset of its pairs
+  uint32_t *cand, *cleaf, *ctop;  // per candidate: node, leaf cell, first undecided ancestor (or NONE)
   uint64_t* pkeys;
   uint32_t* pvals;
   uint32_t *run_start, *rank, *indeg, *done;

@@ -197,7 +197,7 @@ struct Sim {
   LinkSerialState* hlink = nullptr;
   // exact collide: per-node candidate lists, pair ranks, dataflow counters,
   // the velocities at the iteration's start and the serial pass's state
-  Device<uint32_t> c_count, c_off, c_cand, c_link, c_tcell, c_tpar, c_run, c_rank, c_indeg, c_done, c_part,
+  Device<uint32_t> c_count, c_off, c_cand, c_leaf, c_top, c_run, c_rank, c_indeg, c_done, c_part,
       c_stack;
   Device<double> c_svx, c_svy;
   CollideSerialState* dcol = nullptr;
@@ -240,7 +240,7 @@ struct Sim {
     if (dlink) (void)hipFree(dlink);
     if (hscalar_u) (void)hipHostFree(hscalar_u);
     if (hlink) (void)hipHostFree(hlink);
-    for (auto* a : {&c_count, &c_off, &c_cand, &c_link, &c_tcell, &c_tpar, &c_run, &c_rank, &c_indeg, &c_done,
+    for (auto* a : {&c_count, &c_off, &c_cand, &c_leaf, &c_top, &c_run, &c_rank, &c_indeg, &c_done,
                     &c_part, &c_stack})
       a->release();
     c_svx.release();
@@ -643,9 +643,8 @@ hipError_t collide_iteration(Sim& s, Force& f) {
   const size_t c = n > 0 ? n : 1;
   for (auto* b : {&s.c_count, &s.c_off, &s.c_run, &s.c_indeg, &s.c_done}) HIPCHECK(b->reserve(c, 0, s.stream));
   HIPCHECK(s.c_cand.reserve(c * COLLIDE_K, 0, s.stream));
-  HIPCHECK(s.c_link.reserve(c * COLLIDE_K, 0, s.stream));
-  HIPCHECK(s.c_tcell.reserve(c * COLLIDE_T, 0, s.stream));
-  HIPCHECK(s.c_tpar.reserve(c * COLLIDE_T, 0, s.stream));
+  HIPCHECK(s.c_leaf.reserve(c * COLLIDE_K, 0, s.stream));
+  HIPCHECK(s.c_top.reserve(c * COLLIDE_K, 0, s.stream));
   HIPCHECK(s.c_part.reserve(c / 4096 + 2, 0, s.stream));
   HIPCHECK(s.c_svx.reserve(c, 0, s.stream));
   HIPCHECK(s.c_svy.reserve(c, 0, s.stream));
@@ -667,9 +666,8 @@ hipError_t collide_iteration(Sim& s, Force& f) {
   a.ccount = s.c_count.p;
   a.coff = s.c_off.p;
   a.cand = s.c_cand.p;
-  a.clink = s.c_link.p;
-  a.tcell = s.c_tcell.p;
-  a.tpar = s.c_tpar.p;
+  a.cleaf = s.c_leaf.p;
+  a.ctop = s.c_top.p;
   a.run_start = s.c_run.p;
   a.indeg = s.c_indeg.p;
   a.done = s.c_done.p;

@@ -119,8 +119,9 @@ typedef struct {
   uint64_t fmm_cells;           /* cells of the last fast tree */
   uint64_t collide_serial;      /* exact collide iterations run on the serial pass */
   uint64_t collide_status;      /* OR of the parallel collide passes' stop reasons:
-                                   1 list bounds, 2 a position left its box,
-                                   4 a zero delta (jiggle), 8 no progress */
+                                   1 candidate list full, 2 a position left its
+                                   box, 4 a zero delta (jiggle), 8 no progress,
+                                   32 walk stack full */
 } d3f_stats;
 int32_t sim_stats(uint32_t h, d3f_stats* out);
 /* FP64 direct sum for the fast mode's error measurement (no reference
