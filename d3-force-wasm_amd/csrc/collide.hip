@@ -118,9 +118,13 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_radius_level(Cell* 
   }
 }
 
+// Threads take the tree's points in tree (Morton) order, so the walks of a
+// wave cover nearby cells; nodes outside the tree (NaN positions) keep the
+// zero count the host wrote.
 __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_cand(const CollideArgs a) {
-  const uint32_t i = blockIdx.x * COLLIDE_THREADS + threadIdx.x;
-  if (i >= a.n) return;
+  const uint32_t t = blockIdx.x * COLLIDE_THREADS + threadIdx.x;
+  if (t >= a.m) return;
+  const uint32_t i = a.perm[t];
   const double xi = a.px[i], yi = a.py[i];
   if (!__builtin_isfinite(xi) || !__builtin_isfinite(yi)) {
     // A non-finite position makes every overlap test false (l is inf or NaN)
@@ -443,7 +447,9 @@ hipError_t collide_radii(TreeBuffers& t, const double* radii, hipStream_t stream
 
 hipError_t collide_candidates(const CollideArgs& a, hipStream_t stream) {
   if (a.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_collide_cand, dim3(ceil_div(a.n, COLLIDE_THREADS)), dim3(COLLIDE_THREADS), 0, stream, a);
+  HIPCHECK(hipMemsetAsync(a.ccount, 0, (size_t)a.n * sizeof(uint32_t), stream));
+  if (a.m == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_collide_cand, dim3(ceil_div(a.m, COLLIDE_THREADS)), dim3(COLLIDE_THREADS), 0, stream, a);
   return hipGetLastError();
 }
 

@@ -161,6 +161,7 @@ struct CollideArgs {
   double strength;
   double scale;               // box half-width in units of |strength| * largest radius
   uint32_t n;
+  uint32_t m;                 // points in the tree (perm[0 .. m))
   uint32_t key_shift;         // pair keys are j << key_shift | i
   uint32_t *ccount, *coff;    // per node: candidates, offset of its pairs
   uint32_t *cand, *cleaf, *ctop;  // per candidate: node, leaf cell, first undecided ancestor (or NONE)

@@ -662,6 +662,7 @@ hipError_t collide_iteration(Sim& s, Force& f) {
   a.radii = f.df[0].p;
   a.strength = f.scalar(S_STRENGTH);
   a.n = n;
+  a.m = t.meta_host->m;
   a.key_shift = n <= 2 ? 1 : 32 - __builtin_clz(n - 1);
   a.ccount = s.c_count.p;
   a.coff = s.c_off.p;
@@ -676,10 +677,10 @@ hipError_t collide_iteration(Sim& s, Force& f) {
   HIPCHECK(hipMemsetD32Async((hipDeviceptr_t)(w + 7), (int)n, 1, s.stream));
   HIPCHECK(hipMemcpyAsync(s.c_svx.p, s.dvx.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
   HIPCHECK(hipMemcpyAsync(s.c_svy.p, s.dvy.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
-  // A pass whose pushes leave their boxes is tried once more with boxes
-  // three times wider; any other stop goes to the serial pass.
+  // A pass whose pushes leave their boxes is tried again with boxes three
+  // times wider (up to three passes); any other stop goes to the serial pass.
   bool serial = !std::isfinite(a.strength);
-  for (int attempt = 0; attempt < 2 && !serial; ++attempt) {
+  for (int attempt = 0; attempt < 3 && !serial; ++attempt) {
     a.scale = f.collide_scale;
     HIPCHECK(hipMemsetAsync(w + 4, 0, 3 * sizeof(uint32_t), s.stream));
     HIPCHECK(collide_candidates(a, s.stream));
@@ -721,7 +722,7 @@ hipError_t collide_iteration(Sim& s, Force& f) {
       break;
     }
     f.collide_scale *= 3;
-    serial = attempt == 1;
+    serial = attempt == 2;
   }
   ++s.stats.collide_serial;
   HIPCHECK(s.c_stack.reserve(3 * (size_t)t.meta_host->depth + 8, 0, s.stream));

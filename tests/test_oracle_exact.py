@@ -4,7 +4,7 @@ against d3ref, an independent restatement of d3-force 3.0.0 as published,
 over scenarios of the reference's parity corpus (test/parity/scenarios.js),
 comparing x, y, vx, vy with Object.is semantics after every tick and the
 random stream's position at the end (test/parity/parity-test.js:41-58).
-Scenarios that need exact forceCollide are left out: it is not on this path.
+The scenarios carry forceCollide where scenarios.js gives them one.
 """
 
 import math
