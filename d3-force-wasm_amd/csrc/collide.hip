@@ -37,9 +37,12 @@
 //    The boxes are checked as the pass runs: every pushed position must stay
 //    within delta of p0 (per axis). A push outside the box, an exact zero
 //    delta (the reference then draws a jiggle from the host's random, in
-//    pass order), a full candidate list or a wait that makes no progress for
-//    a second stops the pass; the host then restores vx, vy and runs the
-//    iteration with k_collide_serial.
+//    pass order) parks that node before it writes anything: the nodes that
+//    do not depend on it finish, and k_collide_serial continues with the
+//    unfinished ones in index order. A push outside the box, a full
+//    candidate list or a wait that makes no progress for a second stops the
+//    pass; the host then restores vx, vy and retries with wider boxes or runs
+//    the whole iteration with k_collide_serial.
 // 5. k_collide_serial: the reference's pass on one thread, stopping before a
 //    pair that needs jiggles so that the host draws them in order.
 //
@@ -57,6 +60,7 @@ namespace {
 
 constexpr uint32_t COLLIDE_THREADS = 256;
 constexpr uint64_t WAIT_LIMIT = 100000000ull;  // wall clock ticks (100 MHz): 1 s without progress
+constexpr uint64_t PARKED_LIMIT = 200000ull;   // 2 ms, once a node has stopped for a jiggle
 
 __device__ inline bool is_leaf(const Cell& q) {
   return q.child[0] == NONE && q.child[1] == NONE && q.child[2] == NONE && q.child[3] == NONE;
@@ -249,6 +253,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_apply(const Collide
     if (i < a.n) {
       cnt = a.ccount[i];
       off = a.coff[i];
+      if (cnt == 0) a.fin[i] = 1;
     }
     bool pending = cnt != 0;
     uint64_t waiting = cnt ? (cnt == 64 ? ~0ull : (1ull << cnt) - 1) : 0;  // candidates not yet ready
@@ -269,13 +274,18 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_apply(const Collide
         t0 = wall_clock64();
       }
       uint32_t fail = 0;
+      bool parked = false;
       if (go) {
         // collide.zig:46-50: xi from the current vx_i.
         const double ri = a.radii[i], ri2 = ri * ri;
         double vxi = ld(a.vx + i), vyi = ld(a.vy + i);
         const double xi = a.x[i] + vxi, yi = a.y[i] + vyi;
         if (!(fabs(xi - a.px[i]) <= delta) || !(fabs(yi - a.py[i]) <= delta)) fail = COLLIDE_MOVED;
-        for (uint32_t k = 0; k < cnt && !fail; ++k) {
+        // First the visits and overlap tests (the node's pushes change no
+        // value these read: each j appears once and xi is fixed), so that a
+        // node needing a jiggle stops before writing anything.
+        uint64_t hit = 0;
+        for (uint32_t k = 0; k < cnt && !fail && !parked; ++k) {
           const uint32_t j = a.cand[(size_t)i * COLLIDE_K + k];
           // The reference visits the leaf when no cell on its root path is
           // pruned; the cells above ctop pass for any xi in the box. Cells
@@ -304,18 +314,25 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_apply(const Collide
             }
             cc = nx;
           }
-          if (fail) break;
-          if (!visited) continue;
-          // collide.zig:58-82
-          double vxj = ld(a.vx + j), vyj = ld(a.vy + j);
-          double x = xi - a.x[j] - vxj, y = yi - a.y[j] - vyj;
-          double l = x * x + y * y;
-          double rj = a.radii[j], r = ri + rj;
+          if (fail || !visited) continue;
+          const double x = xi - a.x[j] - ld(a.vx + j), y = yi - a.y[j] - ld(a.vy + j);
+          const double l = x * x + y * y, r = ri + a.radii[j];
           if (l < r * r) {
-            if (x == 0 || y == 0) {
-              fail = COLLIDE_JIGGLE;
-              break;
-            }
+            if (x == 0 || y == 0) parked = true;  // collide.zig:62-69 draws from the host's random
+            hit |= 1ull << k;
+          }
+        }
+        if (parked && !fail) {
+          atomicMin(a.stop, i);
+        } else if (!fail) {
+          // collide.zig:58-82, in candidate (walk) order
+          for (uint64_t m = hit; m; m &= m - 1) {
+            const uint32_t k = __builtin_ctzll(m);
+            const uint32_t j = a.cand[(size_t)i * COLLIDE_K + k];
+            double vxj = ld(a.vx + j), vyj = ld(a.vy + j);
+            double x = xi - a.x[j] - vxj, y = yi - a.y[j] - vyj;
+            double l = x * x + y * y;
+            double rj = a.radii[j], r = ri + rj;
             l = __builtin_sqrt(l);
             l = (r - l) / l * a.strength;
             x *= l;
@@ -332,9 +349,9 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_apply(const Collide
             if (!(fabs(a.x[j] + vxj - a.px[j]) <= delta) || !(fabs(a.y[j] + vyj - a.py[j]) <= delta))
               fail = COLLIDE_MOVED;
           }
+          st(a.vx + i, vxi);
+          st(a.vy + i, vyi);
         }
-        st(a.vx + i, vxi);
-        st(a.vy + i, vyi);
       }
       if (__ballot(fail != 0)) {
         if (fail) atomicOr(a.status, fail);
@@ -343,16 +360,23 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_apply(const Collide
       if (__ballot(go)) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (go) {
-          for (uint32_t k = 0; k < cnt; ++k)
-            __hip_atomic_fetch_add(a.done + a.cand[(size_t)i * COLLIDE_K + k], 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+          if (!parked) {
+            for (uint32_t k = 0; k < cnt; ++k)
+              __hip_atomic_fetch_add(a.done + a.cand[(size_t)i * COLLIDE_K + k], 1u, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+            a.fin[i] = 1;
+          }
           pending = false;
         }
       }
       if (__ballot(pending) == 0) break;
       if (__ballot(ldu(a.status) != 0)) return;
-      if (wall_clock64() - t0 > WAIT_LIMIT) {
-        if (lane == 0) atomicOr(a.status, COLLIDE_STALLED);
+      // Once a node has stopped for a jiggle, nodes that wait on it would
+      // wait for ever: a wave that makes no progress for 2 ms leaves its
+      // nodes to the serial continuation.
+      const uint64_t limit = ldu(a.stop) != NONE ? PARKED_LIMIT : WAIT_LIMIT;
+      if (wall_clock64() - t0 > limit) {
+        if (limit == WAIT_LIMIT && lane == 0) atomicOr(a.status, COLLIDE_STALLED);
         return;
       }
       __builtin_amdgcn_s_sleep(1);
@@ -360,16 +384,37 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_apply(const Collide
   }
 }
 
+// Nodes the parallel pass left unfinished, in index order.
+__global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_unfinished_flags(const CollideArgs a,
+                                                                              uint32_t* __restrict__ flag) {
+  const uint32_t i = blockIdx.x * COLLIDE_THREADS + threadIdx.x;
+  if (i < a.n) flag[i] = a.fin[i] == 0 ? 1u : 0u;
+}
+__global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_unfinished_list(const CollideArgs a,
+                                                                             const uint32_t* __restrict__ flag,
+                                                                             const uint32_t* __restrict__ pos,
+                                                                             uint32_t* __restrict__ list) {
+  const uint32_t i = blockIdx.x * COLLIDE_THREADS + threadIdx.x;
+  if (i < a.n && flag[i]) list[pos[i]] = i;
+}
+
 // collide.zig:45-101 on one thread, resumable: before a pair that needs a
 // jiggle it stores its walk and returns; the host draws (x first, then y, as
 // collide.zig:62-69) and launches again.
-__global__ void k_collide_serial(const CollideArgs a, uint32_t* __restrict__ stack, CollideSerialState* st) {
+// With a list it continues a parallel pass: it runs only the listed nodes
+// (those the pass left unfinished, in index order) and checks that every push
+// keeps its node inside the pass's box, which the finished nodes' results
+// assume; a push outside it stops the kernel with need = COLLIDE_NEED_MOVED.
+__global__ void k_collide_serial(const CollideArgs a, uint32_t* __restrict__ stack, CollideSerialState* st,
+                                 const uint32_t* __restrict__ list, uint32_t count) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t i = st->node, sp = st->sp;
+  uint32_t p = st->node, sp = st->sp;
   bool inside = st->inside != 0;
   double xi = st->xi, yi = st->yi;
   const Cell* cells = a.cells;
-  while (i < a.n) {
+  const double delta = list ? collide_delta(a) : 0.0;
+  while (p < count) {
+    const uint32_t i = list ? list[p] : p;
     const double ri = a.radii[i], ri2 = ri * ri;
     if (!inside) {
       xi = a.x[i] + a.vx[i];
@@ -391,7 +436,7 @@ __global__ void k_collide_serial(const CollideArgs a, uint32_t* __restrict__ sta
             if ((x == 0 || y == 0) && !st->has_jig) {
               st->need = (x == 0 ? 1u : 0u) | (y == 0 ? 2u : 0u);
               stack[sp++] = c;
-              st->node = i;
+              st->node = p;
               st->sp = sp;
               st->inside = 1;
               st->xi = xi;
@@ -418,6 +463,12 @@ __global__ void k_collide_serial(const CollideArgs a, uint32_t* __restrict__ sta
             r = 1 - r;
             a.vx[j] -= x * r;
             a.vy[j] -= y * r;
+            if (list && (!(fabs(a.x[j] + a.vx[j] - a.px[j]) <= delta) ||
+                         !(fabs(a.y[j] + a.vy[j] - a.py[j]) <= delta))) {
+              st->need = COLLIDE_NEED_MOVED;
+              st->node = p;
+              return;
+            }
           }
         }
         continue;
@@ -427,9 +478,9 @@ __global__ void k_collide_serial(const CollideArgs a, uint32_t* __restrict__ sta
         if (q.child[k] != NONE) stack[sp++] = q.child[k];
     }
     inside = false;
-    ++i;
+    ++p;
   }
-  st->node = a.n;
+  st->node = count;
   st->need = 0;
 }
 
@@ -475,8 +526,19 @@ hipError_t collide_apply(const CollideArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t collide_serial(const CollideArgs& a, uint32_t* stack, CollideSerialState* st, hipStream_t stream) {
-  hipLaunchKernelGGL(k_collide_serial, dim3(1), dim3(64), 0, stream, a, stack, st);
+hipError_t collide_serial(const CollideArgs& a, uint32_t* stack, CollideSerialState* st, const uint32_t* list,
+                          uint32_t count, hipStream_t stream) {
+  hipLaunchKernelGGL(k_collide_serial, dim3(1), dim3(64), 0, stream, a, stack, st, list, count);
+  return hipGetLastError();
+}
+
+hipError_t collide_unfinished(const CollideArgs& a, uint32_t* flag, uint32_t* pos, uint32_t* list,
+                              const uint32_t* n_dev, uint32_t* partials, uint32_t* count_out, hipStream_t stream) {
+  if (a.n == 0) return hipSuccess;
+  const uint32_t blocks = ceil_div(a.n, COLLIDE_THREADS);
+  hipLaunchKernelGGL(k_collide_unfinished_flags, dim3(blocks), dim3(COLLIDE_THREADS), 0, stream, a, flag);
+  HIPCHECK(scan_u32(flag, pos, a.n, n_dev, partials, count_out, stream));
+  hipLaunchKernelGGL(k_collide_unfinished_list, dim3(blocks), dim3(COLLIDE_THREADS), 0, stream, a, flag, pos, list);
   return hipGetLastError();
 }
 

@@ -150,6 +150,7 @@ constexpr uint32_t COLLIDE_MOVED = 2;     // a position left its box
 constexpr uint32_t COLLIDE_JIGGLE = 4;    // an overlapping pair with a zero delta
 constexpr uint32_t COLLIDE_STALLED = 8;   // a wait made no progress for a second
 constexpr uint32_t COLLIDE_FULL_STACK = 32;  // a walk deeper than COLLIDE_STACK allows
+constexpr uint32_t COLLIDE_NEED_MOVED = 0x80;  // serial continuation: a push left its box
 struct CollideArgs {
   Cell* cells;
   const TreeMeta* meta;
@@ -168,7 +169,9 @@ struct CollideArgs {
   uint64_t* pkeys;
   uint32_t* pvals;
   uint32_t *run_start, *rank, *indeg, *done;
+  uint32_t* fin;              // per node: 1 once its pass is done (or it has no candidates)
   uint32_t *status, *ticket;
+  uint32_t* stop;             // lowest node parked for a jiggle (NONE: none)
 };
 struct CollideSerialState {
   uint32_t node, sp, inside, need, has_jig, pad;
@@ -181,7 +184,13 @@ hipError_t collide_candidates(const CollideArgs& a, hipStream_t stream);
 hipError_t collide_pairs(const CollideArgs& a, hipStream_t stream);
 hipError_t collide_ranks(const CollideArgs& a, uint32_t pairs, hipStream_t stream);
 hipError_t collide_apply(const CollideArgs& a, hipStream_t stream);
-hipError_t collide_serial(const CollideArgs& a, uint32_t* stack, CollideSerialState* st, hipStream_t stream);
+// The reference's pass on one thread, over every node (list == nullptr,
+// count == n) or over the listed ones, checking the boxes.
+hipError_t collide_serial(const CollideArgs& a, uint32_t* stack, CollideSerialState* st, const uint32_t* list,
+                          uint32_t count, hipStream_t stream);
+// The nodes with fin == 0 in index order into list; their count to *count_out.
+hipError_t collide_unfinished(const CollideArgs& a, uint32_t* flag, uint32_t* pos, uint32_t* list,
+                              const uint32_t* n_dev, uint32_t* partials, uint32_t* count_out, hipStream_t stream);
 
 // ---- many_body.hip ----
 // Cells grouped by depth: t.dorder[t.dstart[d] .. t.dstart[d + 1]) holds the

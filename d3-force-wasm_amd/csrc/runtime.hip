@@ -198,7 +198,7 @@ struct Sim {
   // exact collide: per-node candidate lists, pair ranks, dataflow counters,
   // the velocities at the iteration's start and the serial pass's state
   Device<uint32_t> c_count, c_off, c_cand, c_leaf, c_top, c_run, c_rank, c_indeg, c_done, c_part,
-      c_stack;
+      c_stack, c_fin, c_list;
   Device<double> c_svx, c_svy;
   CollideSerialState* dcol = nullptr;
   CollideSerialState* hcol = nullptr;
@@ -241,7 +241,7 @@ struct Sim {
     if (hscalar_u) (void)hipHostFree(hscalar_u);
     if (hlink) (void)hipHostFree(hlink);
     for (auto* a : {&c_count, &c_off, &c_cand, &c_leaf, &c_top, &c_run, &c_rank, &c_indeg, &c_done,
-                    &c_part, &c_stack})
+                    &c_part, &c_stack, &c_fin, &c_list})
       a->release();
     c_svx.release();
     c_svy.release();
@@ -632,8 +632,44 @@ int32_t run_link_exact(Sim& s, Force& f, double alpha) {
 // result (see collide.hip) the velocities are restored and the iteration
 // runs on the serial kernel, which takes jiggles from the host's random in
 // pass order.
+// Jiggles drawn in one iteration, replayed in order if the iteration restarts.
+struct JiggleReplay {
+  std::vector<double> values;
+  size_t pos = 0;
+  double next() {
+    if (pos == values.size()) values.push_back(jiggle_host(draw()));
+    return values[pos++];
+  }
+};
+
+// Runs k_collide_serial to the end of the list (all nodes when list is
+// null), drawing jiggles on the host when it stops for them. *moved reports a
+// push outside the boxes of a continuation.
+hipError_t collide_serial_run(Sim& s, const CollideArgs& a, const uint32_t* list, uint32_t count,
+                              JiggleReplay& replay, bool* moved) {
+  HIPCHECK(s.c_stack.reserve(3 * (size_t)s.tree.meta_host->depth + 8, 0, s.stream));
+  CollideSerialState* st = s.hcol;
+  std::memset(st, 0, sizeof(*st));
+  if (moved) *moved = false;
+  for (;;) {
+    HIPCHECK(hipMemcpyAsync(s.dcol, st, sizeof(*st), hipMemcpyHostToDevice, s.stream));
+    HIPCHECK(collide_serial(a, s.c_stack.p, s.dcol, list, count, s.stream));
+    HIPCHECK(hipMemcpyAsync(st, s.dcol, sizeof(*st), hipMemcpyDeviceToHost, s.stream));
+    HIPCHECK(hipStreamSynchronize(s.stream));
+    if (st->node >= count) return hipSuccess;
+    if (st->need & COLLIDE_NEED_MOVED) {
+      if (moved) *moved = true;
+      return hipSuccess;
+    }
+    if (st->need & 1) st->jig[0] = replay.next();
+    if (st->need & 2) st->jig[1] = replay.next();
+    st->has_jig = 1;
+  }
+}
+
 hipError_t collide_iteration(Sim& s, Force& f) {
   const uint32_t n = s.n;
+  JiggleReplay replay;
   TreeBuffers& t = s.tree;
   s.charges_pending = false;
   HIPCHECK(shift_positions(s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, s.dpx.p, s.dpy.p, n, s.stream));
@@ -641,7 +677,8 @@ hipError_t collide_iteration(Sim& s, Force& f) {
   if (!t.meta_host->valid) return hipSuccess;  // collide.zig:34
   HIPCHECK(collide_radii(t, f.df[0].p, s.stream));
   const size_t c = n > 0 ? n : 1;
-  for (auto* b : {&s.c_count, &s.c_off, &s.c_run, &s.c_indeg, &s.c_done}) HIPCHECK(b->reserve(c, 0, s.stream));
+  for (auto* b : {&s.c_count, &s.c_off, &s.c_run, &s.c_indeg, &s.c_done, &s.c_fin, &s.c_list})
+    HIPCHECK(b->reserve(c, 0, s.stream));
   HIPCHECK(s.c_cand.reserve(c * COLLIDE_K, 0, s.stream));
   HIPCHECK(s.c_leaf.reserve(c * COLLIDE_K, 0, s.stream));
   HIPCHECK(s.c_top.reserve(c * COLLIDE_K, 0, s.stream));
@@ -672,8 +709,10 @@ hipError_t collide_iteration(Sim& s, Force& f) {
   a.run_start = s.c_run.p;
   a.indeg = s.c_indeg.p;
   a.done = s.c_done.p;
+  a.fin = s.c_fin.p;
   a.status = w + 5;
   a.ticket = w + 6;
+  a.stop = w + 8;
   HIPCHECK(hipMemsetD32Async((hipDeviceptr_t)(w + 7), (int)n, 1, s.stream));
   HIPCHECK(hipMemcpyAsync(s.c_svx.p, s.dvx.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
   HIPCHECK(hipMemcpyAsync(s.c_svy.p, s.dvy.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
@@ -705,14 +744,32 @@ hipError_t collide_iteration(Sim& s, Force& f) {
                          s.stream));
     HIPCHECK(hipMemsetAsync(a.indeg, 0, (size_t)n * sizeof(uint32_t), s.stream));
     HIPCHECK(hipMemsetAsync(a.done, 0, (size_t)n * sizeof(uint32_t), s.stream));
+    HIPCHECK(hipMemsetAsync(a.fin, 0, (size_t)n * sizeof(uint32_t), s.stream));
+    HIPCHECK(hipMemsetD32Async((hipDeviceptr_t)a.stop, (int)NONE, 1, s.stream));
     HIPCHECK(collide_ranks(a, pairs, s.stream));
     HIPCHECK(collide_apply(a, s.stream));
     HIPCHECK(hipMemcpyAsync(s.hscalar_u + 5, w + 5, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream));
+    HIPCHECK(hipMemcpyAsync(s.hscalar_u + 8, w + 8, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream));
     HIPCHECK(hipStreamSynchronize(s.stream));
     const uint32_t status = s.hscalar_u[5];
     if (status == 0) {
       f.collide_scale = std::max(0.25, 0.75 * f.collide_scale);
-      return hipSuccess;
+      if (s.hscalar_u[8] == NONE) return hipSuccess;
+      // A node stopped for a jiggle: the unfinished nodes run serially, in
+      // index order, after the finished ones.
+      s.stats.collide_status |= COLLIDE_JIGGLE;
+      HIPCHECK(collide_unfinished(a, s.c_run.p, s.c_indeg.p, s.c_list.p, w + 7, s.c_part.p, w + 9, s.stream));
+      HIPCHECK(hipMemcpyAsync(s.hscalar_u + 9, w + 9, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream));
+      HIPCHECK(hipStreamSynchronize(s.stream));
+      bool moved = false;
+      HIPCHECK(collide_serial_run(s, a, s.c_list.p, s.hscalar_u[9], replay, &moved));
+      if (!moved) return hipSuccess;
+      // A push left its box, which the finished nodes assumed: the whole
+      // iteration again, serially, replaying the jiggles already drawn.
+      HIPCHECK(hipMemcpyAsync(s.dvx.p, s.c_svx.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+      HIPCHECK(hipMemcpyAsync(s.dvy.p, s.c_svy.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+      serial = true;
+      break;
     }
     s.stats.collide_status |= status;
     HIPCHECK(hipMemcpyAsync(s.dvx.p, s.c_svx.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
@@ -725,20 +782,8 @@ hipError_t collide_iteration(Sim& s, Force& f) {
     serial = attempt == 2;
   }
   ++s.stats.collide_serial;
-  HIPCHECK(s.c_stack.reserve(3 * (size_t)t.meta_host->depth + 8, 0, s.stream));
-  CollideSerialState* st = s.hcol;
-  std::memset(st, 0, sizeof(*st));
-  for (;;) {
-    HIPCHECK(hipMemcpyAsync(s.dcol, st, sizeof(*st), hipMemcpyHostToDevice, s.stream));
-    HIPCHECK(collide_serial(a, s.c_stack.p, s.dcol, s.stream));
-    HIPCHECK(hipMemcpyAsync(st, s.dcol, sizeof(*st), hipMemcpyDeviceToHost, s.stream));
-    HIPCHECK(hipStreamSynchronize(s.stream));
-    if (st->node >= n) break;
-    if (st->need & 1) st->jig[0] = jiggle_host(draw());
-    if (st->need & 2) st->jig[1] = jiggle_host(draw());
-    st->has_jig = 1;
-  }
-  return hipSuccess;
+  replay.pos = 0;
+  return collide_serial_run(s, a, nullptr, n, replay, nullptr);
 }
 
 int32_t run_collide_exact(Sim& s, Force& f) {  // collide.zig:17-102

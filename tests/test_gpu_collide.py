@@ -94,3 +94,26 @@ def test_gpu_collide_parallel_pass(gpu_core, oracle_core, n, spread, ticks):
     b = run_both(gpu_core, oracle_core, make, ticks)
     st = gpu_core.stats(d3.core.context(b.nodes()).h)
     assert st["collide_serial"] == 0, "stop reasons %d" % st["collide_status"]
+
+
+def test_gpu_collide_jiggles_continue_the_parallel_pass(gpu_core, oracle_core):
+    # 20,000 uniform nodes with 20 pairs placed on the same point: those
+    # pairs draw jiggles (collide.zig:62-69), so their nodes stop; the rest of
+    # the pass finishes in parallel and the unfinished nodes run serially in
+    # index order, drawing from the host's random as the reference does.
+    print("gpu collide jiggle continuation", flush=True)
+    n = 20000
+    xy, radii = scatter(n, 9, 300.0)
+    rng = np.random.default_rng(4)
+    src = rng.choice(n, 40, replace=False)
+    xy[src[20:]] = xy[src[:20]]
+
+    def make(core):
+        t = d3.Nodes(n, x=xy[:, 0].copy(), y=xy[:, 1].copy())
+        sim = d3.forceSimulation(t, core=core).force("collide", d3.forceCollide(radii).strength(0.7)).stop()
+        return sim, t
+
+    b = run_both(gpu_core, oracle_core, make, 4)
+    st = gpu_core.stats(d3.core.context(b.nodes()).h)
+    assert st["collide_serial"] == 0, "stop reasons %d" % st["collide_status"]
+    assert st["collide_status"] & 4, "no jiggle was drawn"
