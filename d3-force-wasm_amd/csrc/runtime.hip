@@ -753,7 +753,7 @@ hipError_t collide_iteration(Sim& s, Force& f) {
     HIPCHECK(hipStreamSynchronize(s.stream));
     const uint32_t status = s.hscalar_u[5];
     if (status == 0) {
-      f.collide_scale = std::max(0.25, 0.75 * f.collide_scale);
+      f.collide_scale = std::max(0.25, 0.5 * f.collide_scale);
       if (s.hscalar_u[8] == NONE) return hipSuccess;
       // A node stopped for a jiggle: the unfinished nodes run serially, in
       // index order, after the finished ones.
