@@ -77,6 +77,11 @@ __device__ inline bool pruned(const Cell& q, double ri, double xi, double yi) {
 // pair's radius sum, so delta is a multiple (the host's scale) of
 // |strength| R, R the largest radius, plus 1e-9 of the extent's magnitude,
 // far above the rounding of the positions.
+__device__ inline bool pruned(const CRec& q, double ri, double xi, double yi) {
+  const double r = ri + q.r;
+  return q.x0 > xi + r || q.x1 < xi - r || q.y0 > yi + r || q.y1 < yi - r;
+}
+
 __device__ inline double collide_delta(const CollideArgs& a) {
   const double R = a.cells[0].r > 0 ? a.cells[0].r : 0.0;  // the root holds the largest radius (or 0)
   double S = 0;
@@ -125,6 +130,26 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_radius_level(Cell* 
 // Threads take the tree's points in tree (Morton) order, so the walks of a
 // wave cover nearby cells; nodes outside the tree (NaN positions) keep the
 // zero count the host wrote.
+// The walks' 64-byte view of a cell: bounds, radius, children and the
+// chain head of a leaf (NONE for an internal cell).
+__global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_records(const Cell* __restrict__ cells, uint32_t len,
+                                                                     const uint32_t* __restrict__ perm,
+                                                                     CRec* __restrict__ rec) {
+  const uint32_t c = blockIdx.x * COLLIDE_THREADS + threadIdx.x;
+  if (c >= len) return;
+  const Cell q = cells[c];
+  CRec o;
+  o.x0 = q.x0;
+  o.y0 = q.y0;
+  o.x1 = q.x1;
+  o.y1 = q.y1;
+  o.r = q.r;
+  for (int k = 0; k < 4; ++k) o.child[k] = q.child[k];
+  o.head = is_leaf(q) ? perm[q.first + q.count - 1] : NONE;
+  o.pad = 0;
+  rec[c] = o;
+}
+
 __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_cand(const CollideArgs a) {
   const uint32_t t = blockIdx.x * COLLIDE_THREADS + threadIdx.x;
   if (t >= a.m) return;
@@ -162,10 +187,10 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_cand(const CollideA
   while (sp > 0) {
     --sp;
     const uint32_t c = stk[sp], top = stop[sp];
-    const Cell q = a.cells[c];
+    const CRec q = a.rec[c];
     const double r = ri + q.r;
-    if (is_leaf(q)) {
-      const uint32_t j = a.perm[q.first + q.count - 1];
+    if (q.head != NONE) {
+      const uint32_t j = q.head;
       if (j > i) {
         const double dx = xi - a.px[j], dy = yi - a.py[j];
         const double l = dx * dx + dy * dy;
@@ -296,8 +321,8 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide_apply(const Collide
           const uint32_t leaf = a.cleaf[(size_t)i * COLLIDE_K + k];
           uint32_t cc = a.ctop[(size_t)i * COLLIDE_K + k];
           for (uint32_t step = 0; cc != NONE && cc != leaf; ++step) {
-            const Cell q = a.cells[cc];
-            if (step == COLLIDE_STACK || is_leaf(q)) {
+            const CRec q = a.rec[cc];
+            if (step == COLLIDE_STACK || q.head != NONE) {
               fail = COLLIDE_FULL_STACK;
               break;
             }
@@ -493,6 +518,13 @@ hipError_t collide_radii(TreeBuffers& t, const double* radii, hipStream_t stream
   for (uint32_t d = t.meta_host->depth + 1; d-- > 0;)
     hipLaunchKernelGGL(k_collide_radius_level, dim3(blocks), dim3(COLLIDE_THREADS), 0, stream, t.cells, t.dorder,
                        t.dstart, d, t.perm, radii);
+  return hipGetLastError();
+}
+
+hipError_t collide_records(const CollideArgs& a, uint32_t len, hipStream_t stream) {
+  if (len == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_collide_records, dim3(ceil_div(len, COLLIDE_THREADS)), dim3(COLLIDE_THREADS), 0, stream,
+                     a.cells, len, a.perm, a.rec);
   return hipGetLastError();
 }
 

@@ -151,8 +151,16 @@ constexpr uint32_t COLLIDE_JIGGLE = 4;    // an overlapping pair with a zero del
 constexpr uint32_t COLLIDE_STALLED = 8;   // a wait made no progress for a second
 constexpr uint32_t COLLIDE_FULL_STACK = 32;  // a walk deeper than COLLIDE_STACK allows
 constexpr uint32_t COLLIDE_NEED_MOVED = 0x80;  // serial continuation: a push left its box
+struct alignas(64) CRec {
+  double x0, y0, x1, y1, r;
+  uint32_t child[4];
+  uint32_t head;  // a leaf's chain head, NONE for an internal cell
+  uint32_t pad;
+};
+static_assert(sizeof(CRec) == 64, "collide walk record");
 struct CollideArgs {
   Cell* cells;
+  CRec* rec;                  // 64-byte walk records of the cells
   const TreeMeta* meta;
   const uint32_t* perm;
   const double *px, *py;      // tree positions x + vx, y + vy at the iteration's start
@@ -180,6 +188,8 @@ struct CollideSerialState {
 };
 // Cell radii for collide (collide.zig:104-130) into t.cells[].r.
 hipError_t collide_radii(TreeBuffers& t, const double* radii, hipStream_t stream);
+// The walk records of cells[0 .. len) (after collide_radii).
+hipError_t collide_records(const CollideArgs& a, uint32_t len, hipStream_t stream);
 hipError_t collide_candidates(const CollideArgs& a, hipStream_t stream);
 hipError_t collide_pairs(const CollideArgs& a, hipStream_t stream);
 hipError_t collide_ranks(const CollideArgs& a, uint32_t pairs, hipStream_t stream);

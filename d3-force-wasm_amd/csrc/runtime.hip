@@ -200,6 +200,7 @@ struct Sim {
   Device<uint32_t> c_count, c_off, c_cand, c_leaf, c_top, c_run, c_rank, c_indeg, c_done, c_part,
       c_stack, c_fin, c_list;
   Device<double> c_svx, c_svy;
+  Device<CRec> c_rec;
   CollideSerialState* dcol = nullptr;
   CollideSerialState* hcol = nullptr;
   // host copies for tree introspection
@@ -245,6 +246,7 @@ struct Sim {
       a->release();
     c_svx.release();
     c_svy.release();
+    c_rec.release();
     if (dcol) (void)hipFree(dcol);
     if (hcol) (void)hipHostFree(hcol);
     hcells.release();
@@ -687,7 +689,9 @@ hipError_t collide_iteration(Sim& s, Force& f) {
   HIPCHECK(s.c_svy.reserve(c, 0, s.stream));
   uint32_t* w = s.dscalar_u;  // [4] pairs, [5] status, [6] ticket, [7] n
   CollideArgs a{};
+  HIPCHECK(s.c_rec.reserve(t.meta_host->len, 0, s.stream));
   a.cells = t.cells;
+  a.rec = s.c_rec.p;
   a.meta = t.meta;
   a.perm = t.perm;
   a.px = s.dpx.p;
@@ -713,6 +717,7 @@ hipError_t collide_iteration(Sim& s, Force& f) {
   a.status = w + 5;
   a.ticket = w + 6;
   a.stop = w + 8;
+  HIPCHECK(collide_records(a, t.meta_host->len, s.stream));
   HIPCHECK(hipMemsetD32Async((hipDeviceptr_t)(w + 7), (int)n, 1, s.stream));
   HIPCHECK(hipMemcpyAsync(s.c_svx.p, s.dvx.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
   HIPCHECK(hipMemcpyAsync(s.c_svy.p, s.dvy.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
