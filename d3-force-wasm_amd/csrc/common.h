@@ -51,6 +51,7 @@ struct TreeMeta {
   uint32_t valid;   // 1 when the tree has a root
   uint32_t overflow;  // a deep walk exceeded its stack
   uint32_t ticket;  // block counter of the box reduction
+  uint32_t long_group;  // a run of equal high key halves longer than the in-place sort takes
 };
 
 // A depth-first frame of the below-key-resolution build (tree.zig:64-76).

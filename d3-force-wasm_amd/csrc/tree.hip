@@ -562,6 +562,43 @@ __device__ inline uint32_t group_start(const uint64_t* keys, uint32_t k) {
 }
 
 // Marks group starts and groups whose points are not all coincident.
+// The sort of the 64-bit Morton keys runs its radix passes on the high 32
+// bits only; keys whose high halves are equal (points within 2^-16 of the
+// extent of each other, few in practice) are then ordered by their low halves
+// with a stable insertion sort, one thread per run. A run longer than
+// LOW_SORT_MAX raises meta->long_group and the host sorts all 64 bits again.
+// The result equals the stable sort by the whole key either way.
+constexpr uint32_t LOW_SORT_MAX = 256;
+__global__ __launch_bounds__(EW_THREADS) void k_sort_low_halves(uint64_t* __restrict__ keys,
+                                                                uint32_t* __restrict__ perm,
+                                                                TreeMeta* __restrict__ meta) {
+  const uint32_t m = meta->m;
+  const uint32_t p = blockIdx.x * EW_THREADS + threadIdx.x;
+  if (p >= m) return;
+  const uint32_t hi = (uint32_t)(keys[p] >> 32);
+  if (p > 0 && (uint32_t)(keys[p - 1] >> 32) == hi) return;  // not the first of its run
+  uint32_t q = p + 1;
+  while (q < m && (uint32_t)(keys[q] >> 32) == hi) {
+    if (q - p >= LOW_SORT_MAX) {
+      atomicOr(&meta->long_group, 1u);
+      return;
+    }
+    ++q;
+  }
+  for (uint32_t k = p + 1; k < q; ++k) {
+    const uint64_t key = keys[k];
+    const uint32_t v = perm[k];
+    uint32_t h = k;
+    while (h > p && keys[h - 1] > key) {
+      keys[h] = keys[h - 1];
+      perm[h] = perm[h - 1];
+      --h;
+    }
+    keys[h] = key;
+    perm[h] = v;
+  }
+}
+
 __global__ __launch_bounds__(EW_THREADS) void k_group_a(const uint64_t* __restrict__ keys,
                                                         const uint32_t* __restrict__ perm,
                                                         const double* __restrict__ xs, const double* __restrict__ ys,
@@ -906,10 +943,20 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
   hipLaunchKernelGGL(k_compact, dim3(nb), dim3(EW_THREADS), 0, stream, n, t.meta, t.keys_all, t.flags, t.offs,
                      t.keys, t.perm, t.rest);
   hipLaunchKernelGGL(k_rest_all, dim3(nb), dim3(EW_THREADS), 0, stream, n, t.meta, t.rest);
-  // Radix sort over m = meta->m keys, 8 passes (even: the result ends in keys/perm).
+  // Radix sort over m = meta->m keys: 4 passes on the high halves (even: the
+  // result ends in keys/perm), then the runs of equal high halves by their
+  // low halves; all 8 passes when a run is too long for that.
   const uint32_t* mptr = &t.meta->m;
-  HIPCHECK(sort_digits(t.hist, t.rs_glob, t.keys, t.perm, t.keys_tmp, t.perm_tmp, n > 0 ? n : 1, mptr, 0, 8,
+  HIPCHECK(sort_digits(t.hist, t.rs_glob, t.keys, t.perm, t.keys_tmp, t.perm_tmp, n > 0 ? n : 1, mptr, 4, 4,
                        stream));
+  HIPCHECK(hipMemsetAsync(&t.meta->long_group, 0, sizeof(uint32_t), stream));
+  hipLaunchKernelGGL(k_sort_low_halves, dim3(nb), dim3(EW_THREADS), 0, stream, t.keys, t.perm, t.meta);
+  HIPCHECK(hipMemcpyAsync(&t.meta_host->long_group, &t.meta->long_group, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+  if (t.meta_host->long_group)
+    HIPCHECK(sort_digits(t.hist, t.rs_glob, t.keys, t.perm, t.keys_tmp, t.perm_tmp, n > 0 ? n : 1, mptr, 0, 8,
+                         stream));
   // Key groups.
   HIPCHECK(hipMemsetAsync(t.mixed, 0, (size_t)(n > 0 ? n : 1) * sizeof(uint32_t), stream));
   hipLaunchKernelGGL(k_group_a, dim3(nb), dim3(EW_THREADS), 0, stream, t.keys, t.perm, x, y, t.meta, t.flags,
