@@ -1,10 +1,12 @@
 // Drives the GPU core through d3-force-wasm_amd/js/core.js exactly as the
 // reference's facade does (src/simulation.js tick loop, src/manyBody.js,
 // src/link.js and src/center.js force calls), for the end-to-end N-API test.
-// Input: a JSON file {x, y, source, target, ticks}; output: x, y, vx, vy of
-// every node after the ticks, as 16-hex-digit float64 bit patterns.
+// With "radius" in the input, src/collide.js's force runs after the link
+// force. Input: a JSON file {x, y, source, target, ticks[, radius]}; output:
+// x, y, vx, vy of every node after the ticks, as 16-hex-digit float64 bit
+// patterns.
 import {readFileSync} from "fs";
-import {context, Slot, MANY_BODY, LINK, CENTER, THETA2, DISTANCE_MIN2, DISTANCE_MAX2, ITERATIONS, CX, CY,
+import {context, Slot, MANY_BODY, LINK, CENTER, COLLIDE, THETA2, DISTANCE_MIN2, DISTANCE_MAX2, ITERATIONS, CX, CY,
         STRENGTH, F0, F1, F2, SOURCE, TARGET} from "../../d3-force-wasm_amd/js/core.js";
 
 const inp = JSON.parse(readFileSync(process.argv[2], "utf8"));
@@ -28,6 +30,11 @@ for (let e = 0; e < m; ++e) {
 }
 const center = new Slot(CENTER);
 center.bind(nodes);
+let collide = null;
+if (inp.radius !== undefined) {
+  collide = new Slot(COLLIDE);
+  collide.bind(nodes), collide.count(n), collide.f64(F0, n).fill(inp.radius);
+}
 
 let alpha = 1;
 const alphaDecay = 1 - Math.pow(0.001, 1 / 300);
@@ -40,6 +47,7 @@ for (let t = 0; t < inp.ticks; ++t) {
   charge.run(nodes, alpha, random);
   link.scalar(ITERATIONS, 1);
   link.run(nodes, alpha, random);
+  if (collide) collide.scalar(STRENGTH, 1), collide.scalar(ITERATIONS, 1), collide.run(nodes, 0, random);
   center.scalar(CX, 0), center.scalar(CY, 0), center.scalar(STRENGTH, 1);
   center.run(nodes, 0);
   ctx.integrate(0.6);

@@ -33,20 +33,27 @@ def test_addon_loads_and_core_js_exports_the_reference_interface():
 
 @needs_node
 @pytest.mark.gpu
-def test_js_facade_ticks_bit_identical_to_oracle(oracle_core, tmp_path):
+@pytest.mark.parametrize("radius", [None, 12.0])
+def test_js_facade_ticks_bit_identical_to_oracle(oracle_core, tmp_path, radius):
+    # With a radius, exact forceCollide runs after the link force.
     n, ticks = 5000, 12
     src, tgt = random_graph(n, 2 * n, seed=9)
     x, y = d3.phyllotaxis(n)
     path = tmp_path / "in.json"
-    path.write_text(json.dumps({"x": x.tolist(), "y": y.tolist(), "source": src.tolist(), "target": tgt.tolist(),
-                                "ticks": ticks}))
+    spec = {"x": x.tolist(), "y": y.tolist(), "source": src.tolist(), "target": tgt.tolist(), "ticks": ticks}
+    if radius is not None:
+        spec["radius"] = radius
+    path.write_text(json.dumps(spec))
     out = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "tick.mjs"), str(path)],
                          capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr[-2000:]
     got = np.array([[struct.unpack(">d", bytes.fromhex(b))[0] for b in row] for row in json.loads(out.stdout)])
     nodes = d3.Nodes(n, x=x, y=y)
-    sim = (d3.forceSimulation(nodes, core=oracle_core).force("charge", d3.forceManyBody())
-           .force("link", d3.forceLink((src, tgt))).force("center", d3.forceCenter()))
+    sim = d3.forceSimulation(nodes, core=oracle_core).force("charge", d3.forceManyBody()).force(
+        "link", d3.forceLink((src, tgt)))
+    if radius is not None:
+        sim.force("collide", d3.forceCollide(radius))
+    sim.force("center", d3.forceCenter())
     for _ in range(ticks):
         sim.tick()
     ref = nodes.state()
