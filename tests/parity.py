@@ -38,6 +38,18 @@ def defaults(nodes, links):
     return nodes, [("charge", {}), ("link", {"links": links}), ("center", {})]
 
 
+def scale_free_links(n, r):  # test/parity/scenarios.js:38-47
+    links, ends = [], [0]
+    for i in range(1, n):
+        k = 2 if r() < 0.3 else 1
+        while k > 0:
+            j = ends[math.floor(r() * len(ends))]
+            links.append((i, j))
+            ends.extend([i, j])
+            k -= 1
+    return links
+
+
 def with_collide(spec, radius):
     nodes, forces = spec
     return nodes, forces + [("collide", {"radius": radius})]
@@ -47,6 +59,10 @@ def _scenarios():
     S = {}
     for n in (10, 100, 300):
         S["default-forces-%d" % n] = lambda r, n=n: defaults([{} for _ in range(n)], random_links(n, 2 * n, r))
+    def scale_free(r):
+        nodes = random_nodes(300, r)
+        return with_collide(defaults(nodes, scale_free_links(300, r)), 4.0)
+    S["scale-free-collide"] = scale_free
     S["grid"] = lambda r: defaults([{"x": float(i % 20), "y": float(i // 20)} for i in range(400)],
                                    [l for i in range(400) for l in
                                     ([(i, i + 1)] if i % 20 < 19 else []) + ([(i, i + 20)] if i < 380 else [])])

@@ -117,3 +117,38 @@ def test_gpu_collide_jiggles_continue_the_parallel_pass(gpu_core, oracle_core):
     st = gpu_core.stats(d3.core.context(b.nodes()).h)
     assert st["collide_serial"] == 0, "stop reasons %d" % st["collide_status"]
     assert st["collide_status"] & 4, "no jiggle was drawn"
+
+
+EDGE = {
+    "no-nodes": (0, None, 1.0, 1.0, 1),
+    "one-node": (1, None, 3.0, 1.0, 1),
+    "two-coincident": (2, "same", 3.0, 1.0, 2),
+    "all-nan-positions": (20, "nan", 3.0, 1.0, 1),
+    "zero-strength": (300, None, 8.0, 0.0, 1),
+    "negative-strength": (300, None, 8.0, -0.5, 1),
+    "zero-iterations": (300, None, 8.0, 1.0, 0),
+    "infinite-radius-one-node": (300, "inf", 4.0, 1.0, 1),
+}
+
+
+@pytest.mark.parametrize("name", list(EDGE))
+def test_gpu_collide_edge_cases(gpu_core, oracle_core, name):
+    print("gpu collide edge", name, flush=True)
+    n, kind, radius, strength, iterations = EDGE[name]
+    rng = np.random.default_rng(12)
+    xy = rng.uniform(-60, 60, size=(n, 2))
+    if kind == "same":
+        xy[:] = 1.5
+    if kind == "nan":
+        xy[:] = np.nan
+    radii = np.full(n, radius)
+    if kind == "inf":
+        radii[7] = np.inf
+
+    def make(core):
+        t = d3.Nodes(n, x=xy[:, 0].copy(), y=xy[:, 1].copy()) if n else d3.Nodes(0)
+        sim = (d3.forceSimulation(t, core=core)
+               .force("collide", d3.forceCollide(radii).strength(strength).iterations(iterations)).stop())
+        return sim, t
+
+    run_both(gpu_core, oracle_core, make, 3)
