@@ -429,7 +429,16 @@ __global__ __launch_bounds__(CK_THREADS) void k_cs_chain(const double* __restric
           const uint32_t j = __builtin_amdgcn_readlane(__double2loint(v[7]), l);
           const uint32_t len = __builtin_amdgcn_readlane(__double2hiint(v[7]), l);
           const uint32_t lo_e = j * CC, hi_e = n - lo_e < len * CC ? n : lo_e + len * CC;
-          for (uint32_t e = lo_e; e < hi_e; ++e) s += p[e];
+          // CC loads in flight at a time, then their adds in order (the
+          // adds alone are the dependent chain).
+          for (uint32_t e0 = lo_e; e0 < hi_e; e0 += CC) {
+            double a[CC];
+#pragma unroll
+            for (uint32_t e = 0; e < CC; ++e) a[e] = e0 + e < hi_e ? p[e0 + e] : 0.0;
+#pragma unroll
+            for (uint32_t e = 0; e < CC; ++e)
+              if (e0 + e < hi_e) s += a[e];
+          }
         }
       };
       const uint32_t m1 = cnt < 64 ? cnt : 64;
