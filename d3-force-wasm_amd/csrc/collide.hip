@@ -68,7 +68,7 @@ __device__ inline bool is_leaf(const Cell& q) {
 
 // collide.zig:89: the prune test of an internal cell for a node at (xi, yi)
 // with radius ri (r = ri + quad.r computed as the reference does).
-__device__ inline bool pruned(const Cell& q, double ri, double xi, double yi) {
+__device__ inline bool pruned(const CRec& q, double ri, double xi, double yi) {
   const double r = ri + q.r;
   return q.x0 > xi + r || q.x1 < xi - r || q.y0 > yi + r || q.y1 < yi - r;
 }
@@ -77,11 +77,6 @@ __device__ inline bool pruned(const Cell& q, double ri, double xi, double yi) {
 // pair's radius sum, so delta is a multiple (the host's scale) of
 // |strength| R, R the largest radius, plus 1e-9 of the extent's magnitude,
 // far above the rounding of the positions.
-__device__ inline bool pruned(const CRec& q, double ri, double xi, double yi) {
-  const double r = ri + q.r;
-  return q.x0 > xi + r || q.x1 < xi - r || q.y0 > yi + r || q.y1 < yi - r;
-}
-
 __device__ inline double collide_delta(const CollideArgs& a) {
   const double R = a.cells[0].r > 0 ? a.cells[0].r : 0.0;  // the root holds the largest radius (or 0)
   double S = 0;
