@@ -77,11 +77,12 @@ def scatter(n, seed, spread):
     return xy, radii
 
 
-@pytest.mark.parametrize("n,spread,ticks", [(20000, 300.0, 8), (200000, 1000.0, 3)])
+@pytest.mark.parametrize("n,spread,ticks", [(20000, 300.0, 8), (200000, 1000.0, 3), (1000000, 2120.6, 3)])
 def test_gpu_collide_parallel_pass(gpu_core, oracle_core, n, spread, ticks):
     # Uniform points with radii U[0.5, 1.5]: most nodes overlap a neighbour,
     # by less than a radius, so every iteration runs as the dataflow (no
-    # serial pass) and matches the oracle bit for bit.
+    # serial pass) and matches the oracle bit for bit. The 1M case is the
+    # benchmark's collide workload (0.0556 nodes per unit area).
     print("gpu collide parallel", n, flush=True)
     xy, radii = scatter(n, 5, spread)
 
