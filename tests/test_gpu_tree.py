@@ -56,6 +56,17 @@ def _sets():
         xs += [c * 100 + k * 2e-11 for k in range(40)]
         ys += [c * 50 + (k % 3) * 1e-12 for k in range(40)]
     out["many-mixed-groups"] = (np.array(xs), np.array(ys))
+    # 700 distinct points within 1e-6 of each other beside far points: their
+    # keys share the high 32 bits, a run longer than the in-place low-half
+    # sort takes, so the build sorts all 64 bits; and a run of 200 that it
+    # does sort in place. Index order shuffled so stability matters.
+    for size, name in ((700, "long-equal-high-run"), (200, "short-equal-high-run")):
+        cx = 5.0 + rng.uniform(0, 1e-6, size)
+        cy = -3.0 + rng.uniform(0, 1e-6, size)
+        fx = rng.uniform(-1000, 1000, 50)
+        fy = rng.uniform(-1000, 1000, 50)
+        order = rng.permutation(size + 50)
+        out[name] = (np.concatenate([cx, fx])[order], np.concatenate([cy, fy])[order])
     out["signed-zeros"] = (np.array([0.0, -0.0, 1.0, -0.0, 0.0, 2.0]), np.array([-0.0, 0.0, 1.0, 0.0, -0.0, 2.0]))
     out["huge"] = (rng.uniform(-1e300, 1e300, 3000), rng.uniform(-1e300, 1e300, 3000))
     out["tiny"] = (rng.uniform(0, 1e-300, 3000), rng.uniform(0, 1e-300, 3000))
