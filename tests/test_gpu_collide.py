@@ -11,6 +11,7 @@ import pytest
 
 import d3force_amd as d3
 from parity import build_facade, same
+from scenarios import random_graph
 from test_oracle_collide import layouts
 
 pytestmark = pytest.mark.gpu
@@ -153,3 +154,25 @@ def test_gpu_collide_edge_cases(gpu_core, oracle_core, name):
         return sim, t
 
     run_both(gpu_core, oracle_core, make, 3)
+
+
+def test_gpu_collide_in_batched_ticks(gpu_core, oracle_core):
+    # The reference's parity scenario "batched-ticks" (test/parity/scenarios.js):
+    # tick(7) batches, one copy in and out around 7 ticks, with forceCollide
+    # after the default forces on 200 nodes placed by the phyllotaxis.
+    print("gpu collide batched ticks", flush=True)
+    src, tgt = random_graph(200, 300, seed=21)
+    out = []
+    for core in (oracle_core, gpu_core):
+        nodes = d3.Nodes(200)
+        sim = (d3.forceSimulation(nodes, core=core).force("charge", d3.forceManyBody())
+               .force("link", d3.forceLink((src, tgt))).force("center", d3.forceCenter())
+               .force("collide", d3.forceCollide(3)).stop())
+        states = []
+        for _ in range(6):
+            sim.tick(7)
+            states.append(nodes.state())
+        out.append((states, sim.random_source()()))
+    for k, (a, b) in enumerate(zip(out[0][0], out[1][0])):
+        assert same(a, b), "batch %d" % (k + 1)
+    assert out[0][1] == out[1][1]
