@@ -97,8 +97,7 @@ __device__ void cover(double e[4], double x, double y) {
 }
 
 __global__ __launch_bounds__(BOX_THREADS) void k_box(const double* __restrict__ xs, const double* __restrict__ ys,
-                                                     uint32_t n, BoxPartial* __restrict__ partials,
-                                                     TreeMeta* __restrict__ meta) {
+                                                     uint32_t n, BoxPartial* __restrict__ partials) {
   const double inf = __builtin_huge_val();
   BoxPartial p{inf, inf, -inf, -inf, NONE, NONE, NONE, NONE, 0, 0};
   for (uint32_t i = blockIdx.x * BOX_THREADS + threadIdx.x; i < n; i += gridDim.x * BOX_THREADS) {
@@ -111,30 +110,24 @@ __global__ __launch_bounds__(BOX_THREADS) void k_box(const double* __restrict__ 
     p.m++;
   }
   __shared__ BoxPartial sh[BOX_THREADS];
-  __shared__ bool last;
   sh[threadIdx.x] = p;
   __syncthreads();
   for (int s = BOX_THREADS / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) combine(sh[threadIdx.x], sh[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = sh[0];
-    __threadfence();
-    uint32_t ticket = atomicAdd(&meta->ticket, 1u);
-    last = ticket == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+// The blocks' partials in block order, then d3's cover of the two corners;
+// a kernel of its own, so the partials are visible without a fence per block.
+__global__ __launch_bounds__(BOX_THREADS) void k_box_final(uint32_t n, uint32_t blocks,
+                                                           const BoxPartial* __restrict__ partials,
+                                                           TreeMeta* __restrict__ meta) {
+  const double inf = __builtin_huge_val();
+  __shared__ BoxPartial sh[BOX_THREADS];
   BoxPartial q{inf, inf, -inf, -inf, NONE, NONE, NONE, NONE, 0, 0};
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += BOX_THREADS) {
-    BoxPartial r;
-    const volatile BoxPartial* src = partials + b;
-    r.x0 = src->x0; r.y0 = src->y0; r.x1 = src->x1; r.y1 = src->y1;
-    r.ix0 = src->ix0; r.iy0 = src->iy0; r.ix1 = src->ix1; r.iy1 = src->iy1; r.m = src->m;
-    combine(q, r);
-  }
+  for (uint32_t b = threadIdx.x; b < blocks; b += BOX_THREADS) combine(q, partials[b]);
   sh[threadIdx.x] = q;
   __syncthreads();
   for (int s = BOX_THREADS / 2; s > 0; s >>= 1) {
@@ -933,7 +926,9 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
   uint32_t box_blocks = ceil_div(n > 0 ? n : 1, BOX_THREADS * 4);
   if (box_blocks > (uint32_t)BOX_BLOCKS) box_blocks = BOX_BLOCKS;
   hipLaunchKernelGGL(k_box, dim3(box_blocks), dim3(BOX_THREADS), 0, stream, x, y, n,
-                     (BoxPartial*)t.box_partials, t.meta);
+                     (BoxPartial*)t.box_partials);
+  hipLaunchKernelGGL(k_box_final, dim3(1), dim3(BOX_THREADS), 0, stream, n, box_blocks,
+                     (const BoxPartial*)t.box_partials, t.meta);
   hipLaunchKernelGGL(k_keys, dim3(nb), dim3(EW_THREADS), 0, stream, x, y, n, t.meta, t.keys, t.perm, t.keys_all,
                      t.flags);
   // Compaction of NaN points; skipped on the device when m == n.
