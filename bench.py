@@ -120,8 +120,9 @@ def run_fast(core, n, steps, warmup, order=8, opening=1.5):
         "phases_ms_per_tick": {ph: v / steps for ph, v in st["ms"].items() if v > 0},
         "fmm_work_per_tick": {"m2l_pairs": st["m2l"] / launches, "p2p_point_pairs": st["p2p_pairs"] / launches,
                               "cells": st["fmm_cells"]},
-        "fmm_roofline": {"kernel": "fast many-body (all FMM kernels)", "bound": "mfma",
-                         "pipe": "FP32 (MI355X FP32 matrix peak 157.3 TFLOP/s)",
+        "fmm_roofline": {"kernel": "fast many-body (all FMM kernels)", "bound": "fp32",
+                         "pipe": "FP32: M2L on the matrix cores, P2P and expansions on the vector ALUs "
+                                 "(MI355X FP32 peak 157.3 TFLOP/s, vector and matrix)",
                          "achieved": flops / t_fmm / 1e12, "peak": 157.3, "unit": "TFLOP/s",
                          "frac": flops / t_fmm / 1e12 / 157.3, "launch_ms": t_fmm * 1e3,
                          "work_per_launch": {"flops": flops, "m2l_flops_per_pair": 8 * (p + 1) ** 2,
@@ -328,29 +329,83 @@ def read_traffic(kernel):
     return None
 
 
-def cpu_baseline(n, m, pin_fraction, ticks, threads):
+def host_cpu():
+    """The host's CPU as the box reports it: nproc, the CPUs this process may
+    run on, and the model name."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    return {"nproc": os.cpu_count(), "affinity": affinity, "cpu_model": model}
+
+
+def cpu_baseline(n, m, pin_fraction, warmup, ticks, rounds, threads, single_ticks):
+    """The oracle (a C++ restatement of the reference core, the reference's
+    pool chunking for tree build, accumulate, traversal and integrate; link
+    and center serial as in the reference) on `threads` host threads, by the
+    reference's method (bench/threads.js:24-35): `warmup` ticks, then the
+    fastest of `rounds` rounds of `ticks` ticks. Every round restarts from the
+    state after the warm-up (positions, velocities, alpha, random source), so
+    each times ticks warmup+1 .. warmup+ticks, the range the GPU times. Then
+    a bounded single-thread sample of `single_ticks` ticks from that state."""
     from oracle import oracle
     core = oracle.load()
     core.set_threads(threads)
-    sim, _ = make_workload(core, n, m, pin_fraction)
-    ctx = sim.begin()
-    t0 = time.time()
-    sim.iterate(ticks, ctx)
-    dt = time.time() - t0
-    sim.end(ctx)
-    return {"value": ticks / dt, "unit": "ticks/s", "cores": threads, "kind": "port",
-            "sample": "ticks 1-%d of the same 1M-node workload from its start, C++ oracle "
-                      "(restatement of the reference core), Barnes-Hut traversal on %d threads, "
-                      "tree build, link, center and integrate serial as in the reference" % (ticks, threads)}
+    sim, nodes = make_workload(core, n, m, pin_fraction)
+    log("cpu baseline: %d warm-up ticks on %d threads" % (warmup, threads))
+    sim.tick(warmup)
+    state, alpha, rnd = nodes.state(), sim.alpha(), list(sim.random_source().state)
+
+    def restore():
+        nodes.x[:], nodes.y[:], nodes.vx[:], nodes.vy[:] = state
+        sim.alpha(alpha)
+        sim.random_source().state[:] = rnd
+
+    def timed(k):
+        restore()
+        ctx = sim.begin()
+        t0 = time.perf_counter()
+        sim.iterate(k, ctx)
+        dt = time.perf_counter() - t0
+        sim.end(ctx)
+        return dt
+
+    best = min(timed(ticks) for _ in range(rounds))
+    log("cpu baseline: pool %.3f ticks/s" % (ticks / best))
+    core.set_threads(1)
+    one = timed(single_ticks)
+    log("cpu baseline: single thread %.3f ticks/s" % (single_ticks / one))
+    span = "ticks %d-%d" % (warmup + 1, warmup + ticks)
+    return {"value": ticks / best, "unit": "ticks/s", "cores": threads, "kind": "port",
+            "sample": "%s of the same 1M-node workload (the GPU's timed ticks): %d warm-up ticks, then the fastest "
+                      "of %d rounds of %d ticks, each round restarted from the state after the warm-up "
+                      "(bench/threads.js:24-35's method); C++ oracle (restatement of the reference core) with the "
+                      "reference's pool chunking for tree build, accumulate, traversal and integrate, link and "
+                      "center serial as in the reference, on %d threads (the box's CPU share for one GPU)"
+                      % (span, warmup, rounds, ticks, threads),
+            "host": host_cpu(),
+            "single_thread": {"value": single_ticks / one, "unit": "ticks/s", "cores": 1,
+                              "sample": "ticks %d-%d from the same post-warm-up state, one round"
+                                        % (warmup + 1, warmup + single_ticks)}}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nodes", type=int, default=1000000)
-    ap.add_argument("--cpu-ticks", type=int, default=3)
+    ap.add_argument("--cpu-rounds", type=int, default=3, help="rounds of the CPU pool baseline (fastest kept)")
+    ap.add_argument("--cpu-single-ticks", type=int, default=3, help="ticks of the single-thread CPU sample")
+    ap.add_argument("--cpu-threads", type=int, default=16, help="CPU pool threads (the box's share for one GPU)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fast-nodes", type=int, default=1000000, help="nodes of the fast-mode leg (0: skip it)")
     ap.add_argument("--fast-only", action="store_true", help="run only the fast-mode leg (prints its object)")
@@ -403,25 +458,36 @@ def main():
     core.core_set_timing(0)
     st = core.stats(ctx.h)
     sim.end(ctx)
-    ticks_per_s = args.steps / dt
-    log("%.3f ms/tick, phases %s" % (1e3 * dt / args.steps, st["ms"]))
+    # The facade's tick(k) with the boundary copies (copy-in of 7 arrays and
+    # copy-out of 4 per call, over PCIe): k = 1 and k = steps, after the
+    # timed region.
+    with_copies = {}
+    for k, calls in ((1, 10), (args.steps, 1)):
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            sim.tick(k)
+        with_copies["tick(%d)" % k] = {"ms_per_tick": 1e3 * (time.perf_counter() - t0) / (k * calls),
+                                       "calls": calls}
+    log("with copies: %s" % with_copies)
+    cells = int(core.tree_info(ctx.h, 0))  # cells of the last tree (host copy, after the timing)
 
     # Per-kernel rooflines from the live event timing; the dominant kernel is
     # the one with the most device time per tick. Algorithmic work per tick
-    # (C = cells, measured 1.72 per node on this workload):
+    # (C = cells of the tree, read from the library after the timed region):
     #   traversal  8 V + 8 A + 6 P FP64 flops (V visits, A accepted cells,
     #              P leaf pairs, counted by the kernel itself)
     #   center_sum 16 B/node read; 2n dependent FP64 adds in d3's order
     #   tree       28 B/node + 88 B/cell;  accumulate 88 B/cell + 24 B/node
     #   link       128 B/link;  integrate 81 B/node
     phase_ms = st["ms"]
-    C = 1.72 * n
+    C = cells
     hbm_bytes = {"tree": 28 * n + 88 * C, "accumulate": 88 * C + 24 * n, "link": 128 * m,
                  "integrate": 81 * n, "center_sum": 16 * n}
     names = {"traversal": "k_traverse<RECORD> (Barnes-Hut traversal)", "center_sum": "k_cs_chain (center sums, side stream)",
-             "tree": "quadtree build (11 kernels)", "accumulate": "k_accumulate_level (all levels)",
+             "tree": "quadtree build (13 kernels)", "accumulate": "k_accumulate_level (all levels)",
              "link": "k_link_level (all levels)", "integrate": "k_integrate"}
     kernels = {}
+    floors = {}
     for ph, ms in phase_ms.items():
         if ph == "center" or ms <= 0:
             continue
@@ -430,19 +496,24 @@ def main():
         if ph == "traversal":
             flops = (8 * st["visits"] + 8 * st["accepts"] + 6 * st["pairs"]) / launches
             a = flops / t / 1e12
-            k = {"kernel": names[ph], "bound": "mfma",
-                 "pipe": "FP64 VALU (MI355X FP64 vector peak = FP64 matrix peak)",
+            k = {"kernel": names[ph], "bound": "valu",
+                 "pipe": "FP64 vector ALU (no matrix cores; MI355X FP64 peak 78.6 TFLOP/s counts an FMA as 2, "
+                         "so with contraction off the ceiling is 39.3 T add/mul per s)",
                  "achieved": a, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": a / FP64_PEAK_TFLOPS,
                  "work_per_launch": {"flops": flops, "visits": st["visits"] / launches,
                                      "accepts": st["accepts"] / launches, "pairs": st["pairs"] / launches}}
+            floors[ph] = flops / (FP64_PEAK_TFLOPS * 1e12) * launches
         else:
             a = hbm_bytes[ph] / t / 1e9
             k = {"kernel": names[ph], "bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": a / HBM_PEAK_GBS, "bytes_per_launch": hbm_bytes[ph]}
             if ph == "center_sum":
+                k["bound"] = "latency"
                 k["note"] = ("second stream, overlapped with the tick: d3's serial sums by exact binade segments "
                              "(chunk transforms composed in parallel, one serial pass over the segment records); "
                              "%.2f ns per node from the side stream's start to its end" % (t * 1e9 / n))
+            else:
+                floors[ph] = hbm_bytes[ph] / (HBM_PEAK_GBS * 1e9) * launches
         k["launch_ms"] = t * 1e3
         k["traffic"] = read_traffic(names[ph].split(" ")[0].split("<")[0])
         kernels[ph] = k
@@ -451,6 +522,12 @@ def main():
     dominant = max((ph for ph in kernels if ph != "center_sum"),
                    key=lambda ph: kernels[ph]["launch_ms"] * max(st["launches"][ph], 1))
     roof = dict(kernels[dominant])
+    floor_s = sum(floors.values()) / args.steps if floors else 0.0
+    tick_roofline = {"floors_ms_per_tick": {ph: 1e3 * v / args.steps for ph, v in floors.items()},
+                     "sum_of_floors_ms": 1e3 * floor_s, "tick_ms": 1e3 * dt / args.steps,
+                     "frac": floor_s / (dt / args.steps),
+                     "note": "per phase floor = algorithmic bytes / 8 TB/s (HBM phases) or flops / 78.6 TFLOP/s "
+                             "(traversal); the center sums overlap on a second stream and are left out"}
 
     fast = run_fast(core, args.fast_nodes, args.steps, args.warmup) if args.fast_nodes else None
     # Order 8 measures ~1.7e-6 against the direct sum on this workload; order
@@ -467,10 +544,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log("cpu baseline: oracle, %d ticks on %d threads" % (args.cpu_ticks, threads))
-        cpu = cpu_baseline(n, m, pin, args.cpu_ticks, threads)
-        log("cpu baseline %.3f ticks/s" % cpu["value"])
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        cpu = cpu_baseline(n, m, pin, args.warmup, args.steps, args.cpu_rounds, threads, args.cpu_single_ticks)
 
     out = {
         "metric": METRIC,
@@ -491,6 +566,9 @@ def main():
                    "ticks": "%d-%d" % (args.warmup + 1, args.warmup + args.steps)},
         "phases_ms_per_tick": {k: v / args.steps for k, v in phase_ms.items()},
         "roofline": roof,
+        "tick_roofline": tick_roofline,
+        "with_copies": with_copies,
+        "cells": cells,
         "rooflines": kernels,
         "cpu_baseline": cpu,
         "fast": fast,

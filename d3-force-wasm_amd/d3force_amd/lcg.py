@@ -10,4 +10,5 @@ def lcg(seed=1):
         state[0] = (1664525 * state[0] + 1013904223) % 4294967296
         return state[0] / 4294967296
 
+    random.state = state  # the generator's position, for saving and restoring it
     return random

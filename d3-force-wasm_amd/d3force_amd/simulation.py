@@ -97,7 +97,9 @@ class Simulation:
     def iterate(self, iterations, ctx):
         """``iterations`` ticks on the core's resident arrays (simulation.js:63-82)."""
         nodes = self._nodes
-        for _ in range(int(iterations)):
+        k = 0
+        while k < iterations:  # simulation.js:63: for (k = 0; k < iterations; ++k)
+            k += 1
             self._alpha += (self._alpha_target - self._alpha) * self._alpha_decay
             for force in list(self._forces.values()):
                 slot = getattr(force, "slot", None)

@@ -28,9 +28,12 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <thread>
 #include <vector>
@@ -89,6 +92,88 @@ inline uint64_t seed_from(double v) {  // hash.zig:28-30
 inline double zmin(double a, double b) { return (b < a || a != a) ? b : a; }
 inline double zmax(double a, double b) { return (b > a || a != a) ? b : a; }
 
+// ---------------------------------------------------------------- pool.zig
+// The CPU baseline's thread pool (pool.zig:50-80 parallelFor): g_threads - 1
+// persistent workers plus the calling thread take chunk indices from one
+// counter. The reference promises chunk-independent results (pool.zig:1-10),
+// and every threaded phase below writes each chunk's own slots, so the bits
+// do not depend on the thread count.
+int g_threads = 1;
+
+class Pool {
+ public:
+  ~Pool() { resize(0); }
+  // Runs fn(c) for every c < n on the pool; returns when all are done.
+  void run(uint32_t n, const std::function<void(uint32_t)>& fn) {
+    if ((int)workers_.size() != g_threads - 1) resize(g_threads - 1);
+    if (workers_.empty() || n <= 1) {
+      for (uint32_t c = 0; c < n; ++c) fn(c);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &fn;
+      total_ = n;
+      next_.store(0);
+      busy_ = (int)workers_.size();
+      ++generation_;
+    }
+    cv_.notify_all();
+    drain();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [&] { return busy_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void drain() {
+    for (;;) {
+      const uint32_t c = next_.fetch_add(1);
+      if (c >= total_) return;
+      (*job_)(c);
+    }
+  }
+  void resize(int k) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+      ++generation_;
+    }
+    cv_.notify_all();
+    for (auto& t : workers_) t.join();
+    workers_.clear();
+    stop_ = false;
+    // Workers start from the current generation (read here, on the only
+    // thread that calls run), so none can miss the next job.
+    for (int i = 0; i < k; ++i) workers_.emplace_back([this, g = generation_] { loop(g); });
+  }
+  void loop(uint64_t seen) {
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return generation_ != seen; });
+        seen = generation_;
+        if (stop_) return;
+      }
+      drain();
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--busy_ == 0) done_.notify_one();
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(uint32_t)>* job_ = nullptr;
+  std::atomic<uint32_t> next_{0};
+  uint32_t total_ = 0;
+  uint64_t generation_ = 0;
+  int busy_ = 0;
+  bool stop_ = false;
+};
+Pool g_pool;
+
+inline uint32_t chunks(uint32_t n, uint32_t size) { return (n + size - 1) / size; }  // pool.zig:38-41
+
 // ---------------------------------------------------------------- tree.zig
 struct Cell {  // tree.zig:46-62, 88 bytes
   uint32_t child[4];
@@ -110,7 +195,11 @@ struct Tree {
   std::vector<Cell> cells;
   uint32_t len = 0, root = NONE, m = 0, depth = 0;
   std::vector<uint32_t> perm, perm_tmp, rest, diff;
-  std::vector<uint64_t> keys;
+  std::vector<uint64_t> keys, keys_tmp;
+  // The last pooled build's job subtrees (contiguous cell ranges, tree.zig:18-22)
+  // and the top cells above the cut, for the pooled accumulate.
+  bool pooled = false;
+  std::vector<uint32_t> job_base, job_cells, tops;
 
   // tree.zig:148-187 (d3-quadtree cover)
   void cover(double x, double y) {
@@ -142,7 +231,7 @@ struct Tree {
 
   void reserve(uint32_t n) {
     if (perm.size() < n) {
-      perm.resize(n); perm_tmp.resize(n); rest.resize(n); diff.resize(n); keys.resize(n);
+      perm.resize(n); perm_tmp.resize(n); rest.resize(n); diff.resize(n); keys.resize(n); keys_tmp.resize(n);
     }
     if (cells.size() < 2 * (size_t)n + 64) cells.resize(2 * (size_t)n + 64);
   }
@@ -279,24 +368,26 @@ struct Tree {
   }
 
   // tree.zig:572-647 emit, as the serial preorder walk the reference's job
-  // decomposition reproduces (tree.zig:18-22, :621-646).
-  void emit(const double* xs, const double* ys) {
+  // decomposition reproduces (tree.zig:18-22, :621-646): the cells of the
+  // subtree of `top`, numbered from 0 in preorder, into out.
+  uint32_t emit_from(const Frame& top, const double* xs, const double* ys, std::vector<Cell>& out,
+                     uint32_t* max_depth) {
     std::vector<Frame> stack;
-    stack.push_back(Frame{0, m, 0, NONE, 0, true, extent[0], extent[1], extent[2], extent[3]});
+    stack.push_back(top);
+    stack.back().parent = NONE;
     uint32_t idx = 0;
-    depth = 0;
     while (!stack.empty()) {
       Frame f = stack.back();
       stack.pop_back();
-      if (cells.size() <= idx) cells.resize(cells.size() * 2 + 64);
-      Cell& c = cells[idx];
+      if (out.size() <= idx) out.resize(out.size() * 2 + 64);
+      Cell& c = out[idx];
       c.child[0] = c.child[1] = c.child[2] = c.child[3] = NONE;
       c.first = f.lo; c.count = f.hi - f.lo;
       c.x0 = f.x0; c.y0 = f.y0; c.x1 = f.x1; c.y1 = f.y1;
       c.x = c.y = c.value = c.r = 0;
-      if (f.parent != NONE) cells[f.parent].child[f.slot] = idx;
+      if (f.parent != NONE) out[f.parent].child[f.slot] = idx;
       uint32_t me = idx++;
-      depth = std::max(depth, f.depth);
+      *max_depth = std::max(*max_depth, f.depth);
       if (f.depth >= MAX_DEPTH || all_same(f, xs, ys)) continue;
       uint32_t sp5[5];
       split(f, xs, ys, sp5);
@@ -305,15 +396,268 @@ struct Tree {
         stack.push_back(child(f, (unsigned)q, sp5[q], sp5[q + 1], me));
       }
     }
+    return idx;
+  }
+
+  void emit(const double* xs, const double* ys) {
+    depth = 0;
+    len = emit_from(Frame{0, m, 0, NONE, 0, true, extent[0], extent[1], extent[2], extent[3]}, xs, ys, cells, &depth);
     root = 0;
-    len = idx;
   }
 
   // tree.zig:191-196
   void build(const double* xs, const double* ys, uint32_t n) {
+    pooled = false;
+    if (g_threads > 1) {
+      build_pooled(xs, ys, n);
+      return;
+    }
     if (!prepare(xs, ys, n)) return;
     run_boundaries(xs, ys);
     emit(xs, ys);
+  }
+
+  // ---- The same build on the pool, with the reference's chunking: box by
+  // BOX_ROWS x CHUNK points, keys by CHUNK points, the radix sort by
+  // SORT_CHUNK keys, run boundaries by RUN_CHUNK points, emit by job
+  // subtrees below the cut (tree.zig:18-39, sort.zig:1-13, pool.zig:50-54).
+  // Every chunk writes its own slots, so the result is the serial one.
+  static constexpr uint32_t CHUNK = 1024, BOX_ROWS = 16, RUN_CHUNK = 16384, SORT_CHUNK = 16384;
+
+  bool box_pooled(const double* xs, const double* ys, uint32_t n, double b[4]) {
+    const uint32_t size = BOX_ROWS * CHUNK, nc = chunks(n, size);
+    std::vector<double> part(4 * (size_t)nc);
+    std::vector<uint32_t> cnt(nc);
+    g_pool.run(nc, [&](uint32_t c) {
+      double bx0 = Inf, by0 = Inf, bx1 = -Inf, by1 = -Inf;
+      uint32_t k = 0;
+      const uint32_t lo = c * size, hi = std::min(n, lo + size);
+      for (uint32_t i = lo; i < hi; ++i) {
+        const double x = xs[i], y = ys[i];
+        if (x != x || y != y) continue;
+        if (x < bx0) bx0 = x;
+        if (x > bx1) bx1 = x;
+        if (y < by0) by0 = y;
+        if (y > by1) by1 = y;
+        ++k;
+      }
+      part[4 * c] = bx0; part[4 * c + 1] = by0; part[4 * c + 2] = bx1; part[4 * c + 3] = by1;
+      cnt[c] = k;
+    });
+    // Chunk partials in chunk order with the same strict compares: the first
+    // of equal-comparing values (-0 / +0) still wins.
+    double bx0 = Inf, by0 = Inf, bx1 = -Inf, by1 = -Inf;
+    uint32_t total = 0;
+    for (uint32_t c = 0; c < nc; ++c) {
+      if (part[4 * c] < bx0) bx0 = part[4 * c];
+      if (part[4 * c + 2] > bx1) bx1 = part[4 * c + 2];
+      if (part[4 * c + 1] < by0) by0 = part[4 * c + 1];
+      if (part[4 * c + 3] > by1) by1 = part[4 * c + 3];
+      total += cnt[c];
+    }
+    m = total;
+    if (bx0 > bx1 || by0 > by1) return false;
+    b[0] = bx0; b[1] = by0; b[2] = bx1; b[3] = by1;
+    return true;
+  }
+
+  // sort.zig:55-91: stable LSD radix sort of keys[0..m) carrying perm, 8-bit
+  // digits, a pass skipped when one digit covers every key.
+  void radix_sort_pooled(uint32_t bytes) {
+    if (m == 0) return;
+    const uint32_t nc = chunks(m, SORT_CHUNK);
+    std::vector<uint32_t> hist(256 * (size_t)nc);
+    for (uint32_t b = 0; b < bytes; ++b) {
+      const unsigned shift = 8 * b;
+      g_pool.run(nc, [&](uint32_t c) {
+        uint32_t* h = hist.data() + 256 * (size_t)c;
+        std::fill(h, h + 256, 0u);
+        const uint32_t lo = c * SORT_CHUNK, hi = std::min(m, lo + SORT_CHUNK);
+        for (uint32_t k = lo; k < hi; ++k) h[(keys[k] >> shift) & 255]++;
+      });
+      bool skip = false;
+      for (uint32_t d = 0; d < 256 && !skip; ++d) {
+        uint32_t t = 0;
+        for (uint32_t c = 0; c < nc; ++c) t += hist[256 * (size_t)c + d];
+        skip = t == m;
+      }
+      if (skip) continue;
+      uint32_t run = 0;
+      for (uint32_t d = 0; d < 256; ++d)
+        for (uint32_t c = 0; c < nc; ++c) {
+          uint32_t& h = hist[256 * (size_t)c + d];
+          const uint32_t v = h;
+          h = run;
+          run += v;
+        }
+      g_pool.run(nc, [&](uint32_t c) {
+        uint32_t* h = hist.data() + 256 * (size_t)c;
+        const uint32_t lo = c * SORT_CHUNK, hi = std::min(m, lo + SORT_CHUNK);
+        for (uint32_t k = lo; k < hi; ++k) {
+          const uint32_t d = (keys[k] >> shift) & 255;
+          keys_tmp[h[d]] = keys[k];
+          perm_tmp[h[d]] = perm[k];
+          h[d]++;
+        }
+      });
+      keys.swap(keys_tmp);
+      perm.swap(perm_tmp);
+    }
+  }
+
+  void build_pooled(const double* xs, const double* ys, uint32_t n) {
+    reserve(n);
+    extent[0] = extent[1] = extent[2] = extent[3] = NaN;
+    len = 0; root = NONE; m = 0;
+    double b[4];
+    if (!box_pooled(xs, ys, n, b)) return;
+    cover(b[0], b[1]);
+    cover(b[2], b[3]);
+    // exactKeys (tree.zig:252-284) by chunks: count the finite points of
+    // each chunk, then each chunk writes its keys and NaN nodes at its offsets.
+    const uint32_t nc = chunks(n, CHUNK);
+    std::vector<uint32_t> off(nc + 1, 0);
+    g_pool.run(nc, [&](uint32_t c) {
+      uint32_t k = 0;
+      const uint32_t lo = c * CHUNK, hi = std::min(n, lo + CHUNK);
+      for (uint32_t i = lo; i < hi; ++i) k += (xs[i] == xs[i] && ys[i] == ys[i]) ? 1u : 0u;
+      off[c + 1] = k;
+    });
+    for (uint32_t c = 0; c < nc; ++c) off[c + 1] += off[c];
+    const double e0 = extent[0], e1 = extent[1], e2 = extent[2], e3 = extent[3];
+    g_pool.run(nc, [&](uint32_t c) {
+      const uint32_t lo = c * CHUNK, hi = std::min(n, lo + CHUNK);
+      uint32_t k = off[c], r = lo - off[c];
+      for (uint32_t i = lo; i < hi; ++i) {
+        const double x = xs[i], y = ys[i];
+        if (x != x || y != y) { rest[r++] = i; continue; }
+        double x0 = e0, y0 = e1, x1 = e2, y1 = e3;
+        uint64_t key = 0;
+        for (int lev = 0; lev < KEY_LEVELS; ++lev) {
+          const double xm = (x0 + x1) / 2, ym = (y0 + y1) / 2;
+          const bool right = x >= xm;
+          if (right) x0 = xm; else x1 = xm;
+          const bool bottom = y >= ym;
+          if (bottom) y0 = ym; else y1 = ym;
+          key = key << 2 | (uint64_t)bottom << 1 | (uint64_t)right;
+        }
+        perm[k] = i;
+        keys[k] = key;
+        ++k;
+      }
+    });
+    radix_sort_pooled(8);
+    // runBoundaries (tree.zig:392-433) by chunks: differences per chunk, then
+    // each chunk writes its running count.
+    if (m) {
+      const uint32_t rc = chunks(m, RUN_CHUNK);
+      std::vector<uint32_t> d(rc + 1, 0);
+      auto differs = [&](uint32_t k) {
+        const uint32_t p = perm[k], q = perm[k - 1];
+        return (xs[p] != xs[q] || ys[p] != ys[q]) ? 1u : 0u;
+      };
+      g_pool.run(rc, [&](uint32_t c) {
+        uint32_t t = 0;
+        const uint32_t lo = std::max(1u, c * RUN_CHUNK), hi = std::min(m, c * RUN_CHUNK + RUN_CHUNK);
+        for (uint32_t k = lo; k < hi; ++k) t += differs(k);
+        d[c + 1] = t;
+      });
+      for (uint32_t c = 0; c < rc; ++c) d[c + 1] += d[c];
+      g_pool.run(rc, [&](uint32_t c) {
+        uint32_t t = d[c];
+        const uint32_t lo = c * RUN_CHUNK, hi = std::min(m, lo + RUN_CHUNK);
+        for (uint32_t k = lo; k < hi; ++k) {
+          if (k > 0) t += differs(k);
+          diff[k] = t;
+        }
+      });
+    }
+    emit_pooled(xs, ys);
+    pooled = true;
+  }
+
+  // Emit by subtrees (tree.zig:572-647, the cut at max(m / 64, 64) points,
+  // pool.zig:50-54): the top of the tree is walked in preorder on the calling
+  // thread; a frame of at most `limit` points becomes a job whose subtree is
+  // emitted on the pool; every cell then goes to the index the serial
+  // preorder walk gives it.
+  void emit_pooled(const double* xs, const double* ys) {
+    const uint32_t limit = std::max(m / 64, 64u);
+    std::vector<Cell> top_cells;
+    std::vector<Frame> job_frames;
+    std::vector<uint32_t> item_ref;  // per preorder item: its top cell or job
+    std::vector<uint8_t> item_job;
+    std::vector<Frame> stack;
+    depth = 0;
+    stack.push_back(Frame{0, m, 0, NONE, 0, true, extent[0], extent[1], extent[2], extent[3]});
+    while (!stack.empty()) {
+      Frame f = stack.back();
+      stack.pop_back();
+      const uint32_t item = (uint32_t)item_ref.size();
+      if (f.parent != NONE) top_cells[item_ref[f.parent]].child[f.slot] = item;
+      if (f.hi - f.lo <= limit) {
+        item_job.push_back(1);
+        item_ref.push_back((uint32_t)job_frames.size());
+        job_frames.push_back(f);
+        continue;
+      }
+      Cell c;
+      c.child[0] = c.child[1] = c.child[2] = c.child[3] = NONE;
+      c.first = f.lo; c.count = f.hi - f.lo;
+      c.x0 = f.x0; c.y0 = f.y0; c.x1 = f.x1; c.y1 = f.y1;
+      c.x = c.y = c.value = c.r = 0;
+      item_job.push_back(0);
+      item_ref.push_back((uint32_t)top_cells.size());
+      top_cells.push_back(c);
+      depth = std::max(depth, f.depth);
+      if (f.depth >= MAX_DEPTH || all_same(f, xs, ys)) continue;
+      uint32_t sp5[5];
+      split(f, xs, ys, sp5);
+      for (int q = 3; q >= 0; --q) {
+        if (sp5[q] == sp5[q + 1]) continue;
+        stack.push_back(child(f, (unsigned)q, sp5[q], sp5[q + 1], item));
+      }
+    }
+    const uint32_t nj = (uint32_t)job_frames.size();
+    std::vector<std::vector<Cell>> local(nj);
+    std::vector<uint32_t> local_n(nj), local_depth(nj, 0);
+    g_pool.run(nj, [&](uint32_t j) { local_n[j] = emit_from(job_frames[j], xs, ys, local[j], &local_depth[j]); });
+    // Preorder indices: the items in order, a job taking its subtree's cells.
+    std::vector<uint32_t> index(item_ref.size());
+    job_base.assign(nj, 0);
+    job_cells.assign(local_n.begin(), local_n.end());
+    tops.clear();
+    uint32_t idx = 0;
+    for (size_t it = 0; it < item_ref.size(); ++it) {
+      index[it] = idx;
+      if (item_job[it]) {
+        job_base[item_ref[it]] = idx;
+        idx += local_n[item_ref[it]];
+      } else {
+        tops.push_back(idx);
+        idx += 1;
+      }
+    }
+    if (cells.size() < idx) cells.resize(idx);
+    for (uint32_t j = 0; j < nj; ++j) depth = std::max(depth, local_depth[j]);
+    for (size_t it = 0; it < item_ref.size(); ++it) {
+      if (item_job[it]) continue;
+      Cell c = top_cells[item_ref[it]];
+      for (int q = 0; q < 4; ++q)
+        if (c.child[q] != NONE) c.child[q] = index[c.child[q]];
+      cells[index[it]] = c;
+    }
+    g_pool.run(nj, [&](uint32_t j) {
+      const uint32_t base = job_base[j];
+      for (uint32_t k = 0; k < local_n[j]; ++k) {
+        Cell c = local[j][k];
+        for (int q = 0; q < 4; ++q)
+          if (c.child[q] != NONE) c.child[q] += base;
+        cells[base + k] = c;
+      }
+    });
+    root = 0;
+    len = idx;
   }
 
   // tree.zig:652-684: d3-quadtree visit (preorder) / visitAfter (post-order) cell order.
@@ -437,41 +781,52 @@ struct Sim {  // sim.zig:16-43
 struct Counters { uint64_t visits = 0, accepts = 0, pairs = 0; };
 Counters g_count;
 bool g_counting = false;
-int g_threads = 1;
 
-void accumulate(Tree& t, const double* xs, const double* ys, const double* strengths) {
-  // many_body.zig:40-46 + Acc.cell :101-135. Reverse preorder is a valid
-  // post-order: every child has a larger index than its parent.
-  for (uint32_t c = t.len; c-- > 0;) {
-    Cell& quad = t.cells[c];
-    double strength = 0;
-    if (!quad.leaf()) {
-      double weight = 0, x = 0, y = 0;
-      for (int k = 0; k < 4; ++k) {
-        uint32_t ch = quad.child[k];
-        if (ch == NONE) continue;
-        const Cell& q = t.cells[ch];
-        double w = std::fabs(q.value);
-        if (w > 0) {
-          strength += q.value;
-          weight += w;
-          x += w * q.x;
-          y += w * q.y;
-        }
-      }
-      quad.x = x / weight;
-      quad.y = y / weight;
-    } else {
-      uint32_t head = t.perm[quad.first + quad.count - 1];
-      quad.x = xs[head];
-      quad.y = ys[head];
-      for (uint32_t k = quad.first + quad.count; k > quad.first;) {
-        --k;
-        strength += strengths[t.perm[k]];
+// many_body.zig:101-135 Acc.cell.
+void accumulate_cell(Tree& t, uint32_t c, const double* xs, const double* ys, const double* strengths) {
+  Cell& quad = t.cells[c];
+  double strength = 0;
+  if (!quad.leaf()) {
+    double weight = 0, x = 0, y = 0;
+    for (int k = 0; k < 4; ++k) {
+      uint32_t ch = quad.child[k];
+      if (ch == NONE) continue;
+      const Cell& q = t.cells[ch];
+      double w = std::fabs(q.value);
+      if (w > 0) {
+        strength += q.value;
+        weight += w;
+        x += w * q.x;
+        y += w * q.y;
       }
     }
-    quad.value = strength;
+    quad.x = x / weight;
+    quad.y = y / weight;
+  } else {
+    uint32_t head = t.perm[quad.first + quad.count - 1];
+    quad.x = xs[head];
+    quad.y = ys[head];
+    for (uint32_t k = quad.first + quad.count; k > quad.first;) {
+      --k;
+      strength += strengths[t.perm[k]];
+    }
   }
+  quad.value = strength;
+}
+
+void accumulate(Tree& t, const double* xs, const double* ys, const double* strengths) {
+  // many_body.zig:40-46. Reverse preorder is a valid post-order: every child
+  // has a larger index than its parent. After a pooled build, each job's
+  // subtree (one contiguous range) runs on the pool, then the top cells, as
+  // the reference does; only the 0..3 order within a cell affects the sums.
+  if (t.pooled) {
+    g_pool.run((uint32_t)t.job_base.size(), [&](uint32_t j) {
+      for (uint32_t c = t.job_base[j] + t.job_cells[j]; c-- > t.job_base[j];) accumulate_cell(t, c, xs, ys, strengths);
+    });
+    for (size_t k = t.tops.size(); k-- > 0;) accumulate_cell(t, t.tops[k], xs, ys, strengths);
+    return;
+  }
+  for (uint32_t c = t.len; c-- > 0;) accumulate_cell(t, c, xs, ys, strengths);
 }
 
 struct TravParams {
@@ -557,25 +912,15 @@ void many_body_exact(Sim& s, Force& f, double alpha) {  // many_body.zig:31-84
   // Threaded: record pass in chunks of Morton order, then replay flagged
   // nodes in ascending index with the real random (many_body.zig:65-83).
   std::vector<uint8_t> flags(n, 0);
-  std::atomic<uint32_t> next{0};
-  const uint32_t CHUNK = 32;
-  auto worker = [&]() {
-    std::vector<uint32_t> stack;
-    for (;;) {
-      uint32_t c = next.fetch_add(1);
-      uint32_t lo = c * CHUNK;
-      if (lo >= n) break;
-      uint32_t hi = std::min(n, lo + CHUNK);
-      for (uint32_t pos = lo; pos < hi; ++pos) {
-        uint32_t i = pos < t.m ? t.perm[pos] : t.rest[pos - t.m];
-        if (!trav_node(p, true, i, stack, nullptr)) flags[i] = 1;
-      }
+  const uint32_t CHUNK = 32;  // many_body.zig:25
+  g_pool.run(chunks(n, CHUNK), [&](uint32_t c) {
+    thread_local std::vector<uint32_t> stack;
+    const uint32_t lo = c * CHUNK, hi = std::min(n, lo + CHUNK);
+    for (uint32_t pos = lo; pos < hi; ++pos) {
+      uint32_t i = pos < t.m ? t.perm[pos] : t.rest[pos - t.m];
+      if (!trav_node(p, true, i, stack, nullptr)) flags[i] = 1;
     }
-  };
-  std::vector<std::thread> pool;
-  for (int k = 1; k < g_threads; ++k) pool.emplace_back(worker);
-  worker();
-  for (auto& th : pool) th.join();
+  });
   std::vector<uint32_t> stack;
   for (uint32_t i = 0; i < n; ++i) if (flags[i]) trav_node(p, false, i, stack, nullptr);
 }
@@ -710,14 +1055,18 @@ void radial(Sim& s, Force& f, double alpha) {  // simple.zig:64-86
   }
 }
 
-void integrate(Sim& s, double decay) {  // simple.zig:90-123
-  for (uint32_t i = 0; i < s.n; ++i) {
-    uint8_t fixed = s.fixed[i];
-    if ((fixed & 1) == 0) { s.vx[i] *= decay; s.x[i] += s.vx[i]; }
-    else { s.x[i] = s.fx[i]; s.vx[i] = 0; }
-    if ((fixed & 2) == 0) { s.vy[i] *= decay; s.y[i] += s.vy[i]; }
-    else { s.y[i] = s.fy[i]; s.vy[i] = 0; }
-  }
+void integrate(Sim& s, double decay) {  // simple.zig:90-123, by chunks on the pool
+  const uint32_t size = 16384;
+  g_pool.run(chunks(s.n, size), [&](uint32_t c) {
+    const uint32_t lo = c * size, hi = std::min(s.n, lo + size);
+    for (uint32_t i = lo; i < hi; ++i) {
+      uint8_t fixed = s.fixed[i];
+      if ((fixed & 1) == 0) { s.vx[i] *= decay; s.x[i] += s.vx[i]; }
+      else { s.x[i] = s.fx[i]; s.vx[i] = 0; }
+      if ((fixed & 2) == 0) { s.vy[i] *= decay; s.y[i] += s.vy[i]; }
+      else { s.y[i] = s.fy[i]; s.vy[i] = 0; }
+    }
+  });
 }
 
 // ---------------------------------------------------------------- link_fast.zig

@@ -36,3 +36,65 @@ def test_alpha_decay_bits():
     import struct
     v = 1 - math.pow(0.001, 1 / 300)
     assert struct.pack(">d", v).hex() == "3f974f203e3e5400"
+
+
+def test_pooled_oracle_matches_serial(oracle_core):
+    # The CPU baseline's pooled oracle (tree build, accumulate, traversal and
+    # integrate on 8 threads with the reference's chunking) gives the serial
+    # oracle's bits: a 200k-node graph with pinned nodes, coincident groups
+    # and points closer than 2^-32 of the extent (deep splits inside jobs).
+    import numpy as np
+    import d3force_amd as d3
+    from scenarios import pinned, random_graph
+
+    n = 200000
+    src, tgt = random_graph(n, 2 * n, seed=5)
+
+    def make():
+        nodes = d3.Nodes(n)
+        idx, fx, fy = pinned(n, 0.01)
+        nodes.fix(fx=fx, fy=fy, where=idx)
+        sim = (d3.forceSimulation(nodes, core=oracle_core)
+               .force("charge", d3.forceManyBody())
+               .force("link", d3.forceLink((src, tgt)))
+               .force("center", d3.forceCenter()))
+        nodes.x[1000:1400] = nodes.x[2000:2400]  # coincident pairs
+        nodes.y[1000:1400] = nodes.y[2000:2400]
+        nodes.x[3000:3010] = nodes.x[3010] + np.arange(10) * 1e-13
+        nodes.y[3000:3010] = nodes.y[3010]
+        return sim, nodes
+
+    a, na = make()
+    b, nb = make()
+    try:
+        for t in range(3):
+            oracle_core.set_threads(1)
+            a.tick()
+            oracle_core.set_threads(8)
+            b.tick()
+            assert same(na.state(), nb.state()), "tick %d" % (t + 1)
+    finally:
+        oracle_core.set_threads(1)
+
+
+def test_fractional_tick_count_runs_like_d3(oracle_core):
+    # simulation.js:63 loops `for (k = 0; k < iterations; ++k)`: tick(2.5)
+    # runs 3 ticks, tick(0.1) one, tick(0) and tick(-1) none.
+    import d3force_amd as d3
+    from scenarios import random_graph
+
+    def make():
+        src, tgt = random_graph(300, 600, seed=2)
+        nodes = d3.Nodes(300)
+        sim = (d3.forceSimulation(nodes, core=oracle_core).force("charge", d3.forceManyBody())
+               .force("link", d3.forceLink((src, tgt))))
+        return sim, nodes
+
+    for k, ticks in ((2.5, 3), (0.1, 1), (0, 0), (-1, 0)):
+        a, na = make()
+        b, nb = make()
+        a.tick(k)
+        for _ in range(ticks):
+            b.tick()
+        assert a.alpha() == b.alpha()
+        assert same(na.state(), nb.state()), "tick(%r)" % k
