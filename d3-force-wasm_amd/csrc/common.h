@@ -119,6 +119,11 @@ struct TreeBuffers {
   uint32_t *dbins = nullptr, *dstart = nullptr, *dorder = nullptr;  // cells by depth
   TreeMeta* meta = nullptr;       // device
   TreeMeta* meta_host = nullptr;  // pinned mirror
+  // The traversal's stacks in device memory, when a wave's stack outgrows
+  // LDS (deep trees): (cell, lane mask) entries per wave, this simulation's own.
+  uint32_t* tstack_cells = nullptr;
+  uint64_t* tstack_masks = nullptr;
+  size_t tstack_entries = 0;
 };
 
 hipError_t tree_reserve(TreeBuffers& t, uint32_t n);
@@ -229,6 +234,8 @@ struct TravArgs {
   const double* draws;     // REPLAY: the random() values
   uint32_t stack_cap;      // entries per wave
   unsigned long long* counters;  // visits, accepts, pairs (summed)
+  uint32_t* error;         // set nonzero when a wave's stack would overflow
+  TreeBuffers* stacks;     // owner of the device-memory stacks
 };
 hipError_t mb_traverse(const TravArgs& a, int mode, hipStream_t stream);
 // Nodes with a NaN coordinate (tree.rest): their traversal reaches every
@@ -475,6 +482,10 @@ hipError_t radial_force(const double* x, const double* y, double* vx, double* vy
                         const double* radius, uint32_t n, double cx, double cy, double alpha, hipStream_t stream);
 hipError_t shift_positions(const double* x, const double* y, const double* vx, const double* vy, double* px,
                            double* py, uint32_t n, hipStream_t stream);
+// out[0..n) = vx, out[n..2n) = vy and, when x is not null, out[2n..3n) = x,
+// out[3n..4n) = y (sim_snapshot).
+hipError_t snapshot_copy(const double* vx, const double* vy, const double* x, const double* y, uint32_t n,
+                         double* out, hipStream_t stream);
 // pos[i] = (x + vx, y + vy): the shifted positions as pairs, so the fast link
 // sweep gathers an endpoint from one place.
 hipError_t shift_pack(const double* x, const double* y, const double* vx, const double* vy, double2* pos, uint32_t n,

@@ -184,7 +184,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     double alpha, uint32_t* __restrict__ flagged, uint32_t* __restrict__ n_flagged,
     uint32_t* __restrict__ draws_count, const uint32_t* __restrict__ draw_off, const double* __restrict__ draws,
     uint32_t stack_cap, uint32_t* __restrict__ gstack_cells, uint64_t* __restrict__ gstack_masks,
-    unsigned long long* __restrict__ counters) {
+    unsigned long long* __restrict__ counters, uint32_t* __restrict__ err) {
   extern __shared__ uint64_t lds[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint64_t* st_mask;
@@ -273,7 +273,10 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     const uint64_t O = __ballot(open);
     if (O == 0) continue;
     if (!(rec[15] & TC_LEAF)) {
-      if (sp + 4 > (int)stack_cap) break;  // cannot happen with cap = 3 depth + 4
+      if (sp + 4 > (int)stack_cap) {  // cannot happen with cap = 3 depth + 4; if it does, say so
+        if (lane == 0) atomicOr(err, 1u);
+        break;
+      }
       // Descend: push children 3..0 so 0 is visited first, in one store:
       // lane L < 4 writes child 3 - L, if present, above the present
       // children of higher index.
@@ -406,32 +409,30 @@ hipError_t mb_publish_charges(TreeBuffers& t, hipStream_t stream) {
   return hipGetLastError();
 }
 
-static uint32_t* g_stack_cells = nullptr;
-static uint64_t* g_stack_masks = nullptr;
-static size_t g_stack_entries = 0;
-
 hipError_t mb_traverse(const TravArgs& a, int mode, hipStream_t stream) {
   if (a.count == 0) return hipSuccess;
   const uint32_t blocks = ceil_div(a.count, TRAV_THREADS);
   const size_t lds = (size_t)TRAV_WAVES * a.stack_cap * (sizeof(uint64_t) + sizeof(uint32_t));
   const bool global = lds > 48 * 1024;
+  TreeBuffers& st = *a.stacks;
   if (global) {
     const size_t need = (size_t)blocks * TRAV_WAVES * a.stack_cap;
-    if (need > g_stack_entries) {
-      if (g_stack_cells) HIPCHECK(hipFree(g_stack_cells));
-      if (g_stack_masks) HIPCHECK(hipFree(g_stack_masks));
-      g_stack_cells = nullptr;
-      g_stack_masks = nullptr;
-      HIPCHECK(hipMalloc(&g_stack_cells, need * sizeof(uint32_t)));
-      HIPCHECK(hipMalloc(&g_stack_masks, need * sizeof(uint64_t)));
-      g_stack_entries = need;
+    if (need > st.tstack_entries) {
+      if (st.tstack_cells) HIPCHECK(hipFree(st.tstack_cells));
+      if (st.tstack_masks) HIPCHECK(hipFree(st.tstack_masks));
+      st.tstack_cells = nullptr;
+      st.tstack_masks = nullptr;
+      st.tstack_entries = 0;
+      HIPCHECK(hipMalloc(&st.tstack_cells, need * sizeof(uint32_t)));
+      HIPCHECK(hipMalloc(&st.tstack_masks, need * sizeof(uint64_t)));
+      st.tstack_entries = need;
     }
   }
 #define TRAV_LAUNCH(M, G)                                                                                        \
   hipLaunchKernelGGL((k_traverse<M, G>), dim3(blocks), dim3(TRAV_THREADS), G ? 0 : lds, stream, a.tc, a.order,   \
                      a.count, a.perm, a.x, a.y, a.s, a.vx, a.vy, a.dmin2, a.dmax2, a.alpha, a.flagged,          \
-                     a.n_flagged, a.draws_count, a.draw_off, a.draws, a.stack_cap, g_stack_cells, g_stack_masks,   \
-                     a.counters)
+                     a.n_flagged, a.draws_count, a.draw_off, a.draws, a.stack_cap, st.tstack_cells,              \
+                     st.tstack_masks, a.counters, a.error)
   if (global) {
     if (mode == TRAV_RECORD) TRAV_LAUNCH(TRAV_RECORD, true);
     else if (mode == TRAV_COUNT) TRAV_LAUNCH(TRAV_COUNT, true);

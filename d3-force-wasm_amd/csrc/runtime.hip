@@ -67,6 +67,31 @@ int32_t status_of(hipError_t e) {
   return WORKER_FAILED;
 }
 
+// A traversal whose stack would have overflowed stopped early: its result
+// is wrong, so the step fails (the facade's failover reruns it).
+int32_t traversal_overflow() {
+  std::snprintf(g_last_error, sizeof g_last_error,
+                "traversal stack overflow in k_traverse (many_body.hip): the step's velocities are incomplete");
+  return WORKER_FAILED;
+}
+
+// Failure injection for tests of the failover path: after g_fail_after more
+// steps (force_run, sim_integrate), the next g_fail_count steps return
+// g_fail_status before any device work.
+uint32_t g_fail_after = 0, g_fail_count = 0;
+int32_t g_fail_status = WORKER_FAILED;
+bool injected_failure(int32_t* status) {
+  if (g_fail_count == 0) return false;
+  if (g_fail_after > 0) {
+    --g_fail_after;
+    return false;
+  }
+  --g_fail_count;
+  std::snprintf(g_last_error, sizeof g_last_error, "injected failure (core_fail_next): no device work was done");
+  *status = g_fail_status;
+  return true;
+}
+
 struct Force {
   Kind kind;
   double scalars[S_COUNT] = {0};
@@ -159,6 +184,9 @@ struct Force {
 struct Sim {
   uint32_t n = 0, cap = 0;
   Pinned<double> x, y, vx, vy, fx, fy, snap;
+  Device<double> dsnap;     // sim_snapshot's copy (vx, vy, x, y), brought to `snap` on demand
+  bool snap_pending = false;  // dsnap is newer than snap
+  bool snap_positions = false;
   Pinned<uint8_t> fixed;
   Device<double> dx, dy, dvx, dvy, dfx, dfy, dpx, dpy;
   Device<double2> dpos;      // x + vx, y + vy as pairs (fast link sweep)
@@ -220,7 +248,7 @@ struct Sim {
     for (Force* f : forces) delete f;
     for (auto* a : {&x, &y, &vx, &vy, &fx, &fy, &snap}) a->release();
     fixed.release();
-    for (auto* a : {&dx, &dy, &dvx, &dvy, &dfx, &dfy, &dpx, &dpy}) a->release();
+    for (auto* a : {&dx, &dy, &dvx, &dvy, &dfx, &dfy, &dpx, &dpy, &dsnap}) a->release();
     dpos.release();
     dlnodes.release();
     dfixed.release();
@@ -483,14 +511,19 @@ int32_t run_many_body_exact(Sim& s, Force& f, double alpha) {
   a.n_flagged = s.dscalar_u;
   a.stack_cap = 3 * t.meta_host->depth + 4;
   a.counters = s.dcounters;
-  if ((e = hipMemsetAsync(s.dscalar_u, 0, sizeof(uint32_t), s.stream)) != hipSuccess) return status_of(e);
+  a.error = s.dscalar_u + 2;
+  a.stacks = &t;
+  // [0] flagged nodes, [2] the traversal's stack-overflow flag
+  if ((e = hipMemsetAsync(s.dscalar_u, 0, 3 * sizeof(uint32_t), s.stream)) != hipSuccess) return status_of(e);
   s.timer_start(D3F_PHASE_TRAVERSAL);
   if ((e = mb_traverse(a, TRAV_RECORD, s.stream)) != hipSuccess) return status_of(e);
   s.timer_stop();
   if ((e = mb_nan_nodes(t, s.n, s.dvx.p, s.dvy.p, s.stream)) != hipSuccess) return status_of(e);
-  if ((e = hipMemcpyAsync(s.hscalar_u, s.dscalar_u, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream)) != hipSuccess)
+  if ((e = hipMemcpyAsync(s.hscalar_u, s.dscalar_u, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream)) !=
+      hipSuccess)
     return status_of(e);
   if ((e = hipStreamSynchronize(s.stream)) != hipSuccess) return status_of(e);
+  if (s.hscalar_u[2]) return traversal_overflow();
   const uint32_t nf = s.hscalar_u[0];
   s.stats.flagged += nf;
   if (nf == 0) return OK;
@@ -524,7 +557,11 @@ int32_t run_many_body_exact(Sim& s, Force& f, double alpha) {
   a.draw_off = s.draw_off.p;
   a.draws = s.draws.p;
   if ((e = mb_traverse(a, TRAV_REPLAY, s.stream)) != hipSuccess) return status_of(e);
-  return status_of(hipStreamSynchronize(s.stream));
+  if ((e = hipMemcpyAsync(s.hscalar_u + 2, s.dscalar_u + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream)) !=
+      hipSuccess)
+    return status_of(e);
+  if ((e = hipStreamSynchronize(s.stream)) != hipSuccess) return status_of(e);
+  return s.hscalar_u[2] ? traversal_overflow() : OK;
 }
 
 // ---------------------------------------------------------------- link (exact)
@@ -920,6 +957,17 @@ int32_t run_center(Sim& s, Force& f) {  // simple.zig:14-31
   return status_of(e);
 }
 
+// The snapshot's host copy (sim_ptr(h, D3F_PSNAP)): vx, vy, and x, y when
+// the snapshot took positions, n entries each.
+hipError_t fetch_snapshot(Sim& s) {
+  if (!s.snap_pending) return hipSuccess;
+  const size_t b = (size_t)s.n * sizeof(double) * (s.snap_positions ? 4 : 2);
+  HIPCHECK(hipMemcpyAsync(s.snap.p, s.dsnap.p, b, hipMemcpyDeviceToHost, s.stream));
+  HIPCHECK(hipStreamSynchronize(s.stream));
+  s.snap_pending = false;
+  return hipSuccess;
+}
+
 hipError_t refresh_tree_host(Sim& s) {
   TreeBuffers& t = s.tree;
   if (!t.meta) {
@@ -960,6 +1008,14 @@ EXPORT void core_set_timing(int on) { g_timing = on != 0; }
 EXPORT int32_t core_synchronize(void) { return status_of(hipDeviceSynchronize()); }
 
 EXPORT const char* core_last_error(void) { return g_last_error; }
+
+EXPORT int32_t core_fail_next(int32_t status, uint32_t after, uint32_t count) {
+  if (status != WORKER_FAILED && status != OUT_OF_MEMORY) return BAD_ARGUMENT;
+  g_fail_status = status;
+  g_fail_after = after;
+  g_fail_count = count;
+  return OK;
+}
 
 EXPORT uint32_t sim_create(void) {  // root.zig:65-78
   int devices = 0;
@@ -1012,7 +1068,7 @@ EXPORT void* sim_ptr(uint32_t h, uint32_t which) {  // root.zig:94-98, sim.zig:7
     case D3F_PPERM:
       if (refresh_tree_host(*s) != hipSuccess) return nullptr;
       return s->tree.meta_host && s->tree.meta_host->m ? s->hperm.p : nullptr;
-    case D3F_PSNAP: return s->snap.cap ? s->snap.p : nullptr;
+    case D3F_PSNAP: return fetch_snapshot(*s) == hipSuccess && s->snap.cap ? s->snap.p : nullptr;
     default: return nullptr;
   }
 }
@@ -1045,27 +1101,26 @@ EXPORT int32_t sim_sync(uint32_t h, uint32_t direction) {
 }
 
 EXPORT int32_t sim_snapshot(uint32_t h, uint32_t positions) {  // root.zig:205-209, sim.zig:91-100
+  // GPU: the copy stays in HBM (one copy kernel on the stream); sim_ptr(h,
+  // D3F_PSNAP) brings it to the host array when asked for it.
   Sim* s = get(h);
   if (!s) return BAD_HANDLE;
   const uint32_t n = s->n;
-  if (s->snap.reserve(4 * (size_t)(s->cap > 0 ? s->cap : 1), 0) != hipSuccess) return OUT_OF_MEMORY;
-  const size_t b = (size_t)n * sizeof(double);
+  const size_t c = 4 * (size_t)(s->cap > 0 ? s->cap : 1);
+  if (s->snap.reserve(c, 0) != hipSuccess || s->dsnap.reserve(c, 0, s->stream) != hipSuccess) return OUT_OF_MEMORY;
   hipError_t e = hipSuccess;
-  if (n) {
-    e = hipMemcpyAsync(s->snap.p, s->dvx.p, b, hipMemcpyDeviceToHost, s->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->snap.p + n, s->dvy.p, b, hipMemcpyDeviceToHost, s->stream);
-    if (positions) {
-      if (e == hipSuccess) e = hipMemcpyAsync(s->snap.p + 2 * n, s->dx.p, b, hipMemcpyDeviceToHost, s->stream);
-      if (e == hipSuccess) e = hipMemcpyAsync(s->snap.p + 3 * n, s->dy.p, b, hipMemcpyDeviceToHost, s->stream);
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  }
+  if (n) e = snapshot_copy(s->dvx.p, s->dvy.p, positions ? s->dx.p : nullptr, positions ? s->dy.p : nullptr, n,
+                           s->dsnap.p, s->stream);
+  s->snap_pending = n > 0;
+  s->snap_positions = positions != 0;
   return status_of(e);
 }
 
 EXPORT int32_t sim_integrate(uint32_t h, double velocity_decay) {  // root.zig:250-254
   Sim* s = get(h);
   if (!s) return BAD_HANDLE;
+  int32_t injected;
+  if (injected_failure(&injected)) return injected;
   hipError_t e = s->before_x_write();
   s->timer_start(D3F_PHASE_INTEGRATE);
   if (e == hipSuccess)
@@ -1214,6 +1269,8 @@ EXPORT int32_t force_run(uint32_t h, uint32_t fh, double alpha) {  // root.zig:2
   Force* f = get_force(s, fh);
   if (!f) return BAD_HANDLE;
   if (f->node_sized() && f->count < s->n) return BAD_ARGUMENT;
+  int32_t injected;
+  if (injected_failure(&injected)) return injected;
   hipError_t e = upload(*s, *f);
   if (e != hipSuccess) return status_of(e);
   switch (f->kind) {

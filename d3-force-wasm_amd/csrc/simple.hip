@@ -576,7 +576,28 @@ __global__ __launch_bounds__(EW) void k_shift_pack(const double* __restrict__ xs
   pos[i] = make_double2(xs[i] + vxs[i], ys[i] + vys[i]);
 }
 
+// sim_snapshot's copy: out = vx[0..n) vy[0..n) and, when x is given, x, y.
+__global__ __launch_bounds__(EW) void k_snapshot(const double* __restrict__ vxs, const double* __restrict__ vys,
+                                                 const double* __restrict__ xs, const double* __restrict__ ys,
+                                                 double* __restrict__ out, uint32_t n) {
+  const uint32_t i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n) return;
+  out[i] = vxs[i];
+  out[(size_t)n + i] = vys[i];
+  if (xs) {
+    out[2 * (size_t)n + i] = xs[i];
+    out[3 * (size_t)n + i] = ys[i];
+  }
+}
+
 }  // namespace
+
+hipError_t snapshot_copy(const double* vx, const double* vy, const double* x, const double* y, uint32_t n,
+                         double* out, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_snapshot, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, vx, vy, x, y, out, n);
+  return hipGetLastError();
+}
 
 hipError_t batch_center(double* x, double* y, const uint32_t* node_off, const uint32_t* node_graph, uint32_t graphs,
                         uint32_t n, double* sums, double cx, double cy, double strength, hipStream_t stream) {

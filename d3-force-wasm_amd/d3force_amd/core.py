@@ -138,6 +138,12 @@ class Core:
         self.last_error = None
         if hasattr(self.lib, prefix + "core_last_error"):
             self.last_error = self.bind("core_last_error", ctypes.c_char_p, [])
+        # A core whose steps can fail (the GPU library) snapshots each step's
+        # inputs so the facade can rerun it (pool.js:73-84).
+        self.can_fail = hasattr(self.lib, prefix + "core_fail_next")
+        if self.can_fail:
+            self.core_fail_next = self.bind("core_fail_next", ctypes.c_int32,
+                                            [ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32])
         if hasattr(self.lib, prefix + "force_direct_sum"):
             self.force_direct_sum = self.bind("force_direct_sum", ctypes.c_int32,
                                               [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_void_p,
@@ -192,7 +198,9 @@ class Context:
         self.live = False  # true while a tick owns the arrays
         self.ptrs = None
         self.precision = None  # None = exact; dict(order, opening) = fast
-        self.slots = set()
+        self.slots = []
+        self.failover = True  # pool.js:73-84: rerun a failed step (on cores whose steps can fail)
+        self.failovers = 0
         self._finalizer = weakref.finalize(self, core.sim_destroy, self.h)
 
     def resize(self, n):  # core.js:102-104
@@ -238,11 +246,84 @@ class Context:
             nodes.vx[:] = self.vx
             nodes.vy[:] = self.vy
 
-    def step(self, slot, arg):  # core.js:155-159
+    def step(self, slot, arg):
+        """core.js:155-159 with pool.js:73-84's failover: on a core whose steps
+        can fail, the step's vx, vy (and x, y for integrate) are snapshotted
+        first; if the step returns WORKER_FAILED or OUT_OF_MEMORY, the
+        simulation moves to a fresh simulation of the same core with the
+        snapshot restored and the step runs again there, so the tick still
+        returns d3's bits. A second failure raises, naming its cause."""
+        core = self.core
+        guard = self.failover and getattr(core, "can_fail", False)
+        if guard:
+            check(core.sim_snapshot(self.h, 0 if slot is not None else 1))
+        s = self._run(slot, arg)
+        if guard and s in (WORKER_FAILED, OUT_OF_MEMORY):
+            self.failovers += 1
+            self.move(restore=1 if slot is not None else 2)
+            s = self._run(slot, arg)
+        check(s)
+
+    def _run(self, slot, arg):
         if slot is not None:
-            check(self.core.force_run(self.h, slot.fh, arg))
-        else:
-            check(self.core.sim_integrate(self.h, arg))
+            return self.core.force_run(self.h, slot.fh, arg)
+        return self.core.sim_integrate(self.h, arg)
+
+    def move(self, restore=0):
+        """pool.js:93-150 moveTo: a fresh simulation in the same core takes
+        over this one's node arrays and every bound force's arrays and
+        scalars verbatim; `restore` takes vx, vy (and x, y when 2) from the
+        snapshot of the failed step."""
+        core, old, n = self.core, self.h, self.n
+        snap = None
+        if restore:
+            snap = _view(core.sim_ptr(old, P_SNAP), np.float64, (4 if restore == 2 else 2) * n).copy()
+        if restore != 2:
+            check(core.sim_sync(old, SYNC_D2H))  # the step's x, y (integrate restores them too)
+        self.views()
+        x, y, vx, vy = self.x.copy(), self.y.copy(), self.vx.copy(), self.vy.copy()
+        fx, fy, fixed = self.fx.copy(), self.fy.copy(), self.fixed.copy()
+        if snap is not None and n:
+            vx, vy = snap[:n], snap[n:2 * n]
+            if restore == 2:
+                x, y = snap[2 * n:3 * n], snap[3 * n:4 * n]
+        h = core.sim_create()
+        if not h:
+            raise CoreError(OUT_OF_MEMORY)
+        handles = []
+        try:
+            check(core.sim_set_node_count(h, n))
+            for slot in self.slots:
+                fh = core.force_create(h, slot.kind)
+                if not fh:
+                    raise CoreError(OUT_OF_MEMORY)
+                m = slot.m if slot.kind == LINK else n
+                if slot.kind == LINK:
+                    check(core.force_set_count(h, fh, m))
+                for which in range(FLOATS[slot.kind]):
+                    _view(core.force_ptr(h, fh, which), np.float64, m)[:] = \
+                        _view(core.force_ptr(old, slot.fh, which), np.float64, m)
+                if slot.kind == LINK:
+                    for which in (SOURCE, TARGET):
+                        _view(core.force_ptr(h, fh, which), np.uint32, m)[:] = \
+                            _view(core.force_ptr(old, slot.fh, which), np.uint32, m)
+                for which in range(SCALARS):
+                    core.force_set_scalar(h, fh, which, core.force_get_scalar(old, slot.fh, which))
+                handles.append(fh)
+        except Exception:
+            core.sim_destroy(h)
+            raise
+        for slot, fh in zip(self.slots, handles):
+            slot.fh = fh
+        self._finalizer()  # destroys the old simulation
+        self.h = h
+        self._finalizer = weakref.finalize(self, core.sim_destroy, h)
+        self.ptrs = None
+        self.views()
+        if n:
+            self.x[:], self.y[:], self.vx[:], self.vy[:] = x, y, vx, vy
+            self.fx[:], self.fy[:], self.fixed[:] = fx, fy, fixed
+        check(core.sim_sync(h, SYNC_H2D))
 
     def integrate(self, velocity_decay):  # core.js:161-163
         self.step(None, velocity_decay)
@@ -274,12 +355,13 @@ class Slot:
         if ctx is not self.ctx:
             if self.ctx is not None:
                 self.ctx.core.force_destroy(self.ctx.h, self.fh)
-                self.ctx.slots.discard(self)
+                if self in self.ctx.slots:
+                    self.ctx.slots.remove(self)
             self.fh = ctx.core.force_create(ctx.h, self.kind)
             if not self.fh:
                 raise CoreError(OUT_OF_MEMORY)
             self.ctx, self.m = ctx, 0
-            ctx.slots.add(self)
+            ctx.slots.append(self)
         ctx.resize(len(nodes))
 
     def count(self, m):  # core.js:193-196

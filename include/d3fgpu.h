@@ -64,7 +64,9 @@ void* sim_ptr(uint32_t h, uint32_t which);
  * counterpart; core.js copyIn / copyOut at core.js:121-140 call it). */
 int32_t sim_sync(uint32_t h, uint32_t direction);
 /* root.zig:205-209: copies vx, vy (and x, y when positions != 0) into the
- * snapshot array, sim_ptr(h, D3F_PSNAP). */
+ * snapshot array, sim_ptr(h, D3F_PSNAP): vx, vy, x, y, n entries each.
+ * GPU: the copy is taken in HBM on the simulation's stream; sim_ptr(h,
+ * D3F_PSNAP) brings it to the host array. */
 int32_t sim_snapshot(uint32_t h, uint32_t positions);
 /* root.zig:250-254: the end of d3's tick (velocity decay, fixed nodes). */
 int32_t sim_integrate(uint32_t h, double velocity_decay);
@@ -141,6 +143,13 @@ int32_t core_synchronize(void);
  * HIP error name, the library source line that failed and what failed there.
  * Empty before the first failure. The string is owned by the library. */
 const char* core_last_error(void);
+/* Failure injection for tests of the failover path (no reference
+ * counterpart; it stands in for a worker that dies, pool.js:73-84): after
+ * `after` more steps (force_run or sim_integrate, on any simulation) run
+ * normally, the next `count` steps return `status` (WORKER_FAILED -4 or
+ * OUT_OF_MEMORY -2) before any device work, and core_last_error names the
+ * injection. Returns BAD_ARGUMENT for any other status. */
+int32_t core_fail_next(int32_t status, uint32_t after, uint32_t count);
 
 /* ---- Batched fast mode (no reference counterpart) ----
  * Many independent simulations ("graphs") in one node arena and one link

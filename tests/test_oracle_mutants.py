@@ -43,7 +43,7 @@ MUTANTS = [
     ("charge weight without abs",
      "double w = std::fabs(q.value);", "double w = q.value;", ["cell_charges", "theta_infinite_sees_only_root"]),
     ("leaf charge of the head only",
-     "        strength += strengths[t.perm[k]];", "        strength = strengths[t.perm[k]];", ["cell_charges"]),
+     "strength += strengths[t.perm[k]];", "strength = strengths[t.perm[k]];", ["cell_charges"]),
     ("leaf head alone skips its jiggle",
      "if (head != i || quad.count > 1) {", "if (head != i) {", ["coincident_jiggle_order"]),
     ("jiggle draws y before x",
