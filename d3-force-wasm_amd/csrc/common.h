@@ -29,13 +29,15 @@ struct Cell {
 };
 static_assert(sizeof(Cell) == 88, "reference cell layout");
 
-// The traversal's 64-byte view of a cell, one scalar load per visit:
-// centre of charge, charge, the opening threshold w*w/theta2, children, the
-// leaf chain and flags (bit 0 leaf, bit 1 "a truthy leaf is reachable").
+// The traversal's view of a cell: centre of charge, charge, the opening
+// threshold w*w/theta2, the number of cells in its subtree (cells are numbered
+// in d3's visit preorder, so a cell's first child is the next cell and its
+// subtree ends `skip` cells on), its leaf chain, and flags (bit 0 leaf, bit 1
+// "a truthy leaf is reachable"). A visit reads the first 48 bytes.
 struct TCell {
   double qx, qy, value, w2t;
-  uint32_t child[4];
-  uint32_t first, count, head, flags;
+  uint32_t skip, first, count, head;
+  uint32_t flags, pad[3];
 };
 static_assert(sizeof(TCell) == 64, "traversal cell layout");
 constexpr uint32_t TC_LEAF = 1, TC_REACH = 2;
@@ -119,11 +121,6 @@ struct TreeBuffers {
   uint32_t *dbins = nullptr, *dstart = nullptr, *dorder = nullptr;  // cells by depth
   TreeMeta* meta = nullptr;       // device
   TreeMeta* meta_host = nullptr;  // pinned mirror
-  // The traversal's stacks in device memory, when a wave's stack outgrows
-  // LDS (deep trees): (cell, lane mask) entries per wave, this simulation's own.
-  uint32_t* tstack_cells = nullptr;
-  uint64_t* tstack_masks = nullptr;
-  size_t tstack_entries = 0;
 };
 
 hipError_t tree_reserve(TreeBuffers& t, uint32_t n);
@@ -232,10 +229,8 @@ struct TravArgs {
   uint32_t* draws_count;   // COUNT: draws per slot
   const uint32_t* draw_off;  // REPLAY: first draw per slot
   const double* draws;     // REPLAY: the random() values
-  uint32_t stack_cap;      // entries per wave
+  uint32_t len;            // cells of the tree
   unsigned long long* counters;  // visits, accepts, pairs (summed)
-  uint32_t* error;         // set nonzero when a wave's stack would overflow
-  TreeBuffers* stacks;     // owner of the device-memory stacks
 };
 hipError_t mb_traverse(const TravArgs& a, int mode, hipStream_t stream);
 // Nodes with a NaN coordinate (tree.rest): their traversal reaches every

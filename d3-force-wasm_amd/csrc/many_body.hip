@@ -8,13 +8,16 @@
 // d3's order. Kernel boundaries order the levels, with no inter-workgroup
 // atomics (whose cache maintenance spans all eight XCDs' L2s).
 //
-// Traversal (visit): one lane per node, 64 nodes per wave in Morton order.
-// The wave walks the union of its lanes' d3 traversals depth first with one
-// shared LDS stack of (cell, lane mask) entries: a lane processes a cell only
-// if every ancestor on the path was opened by that lane, so each lane sees
-// exactly its own serial visit sequence (children 0..3, many_body.zig:190-259)
-// and accumulates its velocity in d3's order. Cells are read with uniform
-// (scalar) loads of one 64-byte record.
+// Traversal (visit): one lane per node, nodes in Morton order. Cells are
+// numbered in d3's visit preorder (children 0..3), so a node's visit sequence
+// needs no stack: from cell c the walk goes on to c + 1 when it opens c (its
+// first child) and to c + skip (the cell after c's subtree) when it does not
+// (an empty cell, an accepted cell, a leaf). Each lane walks its own node's
+// sequence (many_body.zig:190-259) and accumulates its velocity in d3's
+// order; c only grows, so every walk ends within the cell count. Lanes of a
+// wave move through the cells in the same direction, so the records they read
+// stay close in the caches; blocks are mapped to the eight XCDs in
+// contiguous ranges of nodes, so each XCD's L2 serves one region of the tree.
 //
 // Jiggle (many_body.zig:65-83): with RECORD a lane that meets a zero delta
 // stops without writing and is listed; the host then sorts the list, COUNT
@@ -28,25 +31,25 @@ namespace {
 
 constexpr int ACC_THREADS = 256;
 constexpr int TRAV_THREADS = 256;
-constexpr int TRAV_WAVES = TRAV_THREADS / 64;
 
 __device__ inline bool truthy(double v) { return v != 0 && v == v; }
 
 // A cell's traversal record. The 88-byte cell's x, y, value receive the same
 // charges only when the tree is exported (k_publish_charges).
 __device__ inline void put(TCell* __restrict__ tc, uint32_t c, const Cell& cell, double qx, double qy, double value,
-                           double theta2, uint32_t head, uint32_t flags) {
+                           double theta2, uint32_t skip, uint32_t head, uint32_t flags) {
   TCell t;
   t.qx = qx;
   t.qy = qy;
   t.value = value;
   const double w = cell.x1 - cell.x0;
   t.w2t = w * w / theta2;  // many_body.zig:202, identical operands per cell
-  for (int k = 0; k < 4; ++k) t.child[k] = cell.child[k];
+  t.skip = skip;
   t.first = cell.first;
   t.count = cell.count;
   t.head = head;
   t.flags = flags;
+  t.pad[0] = t.pad[1] = t.pad[2] = 0;
   tc[c] = t;
 }
 
@@ -141,15 +144,17 @@ __global__ __launch_bounds__(ACC_THREADS) void k_accumulate_level(
         --j;
         strength += s[perm[j]];
       }
-      put(tc, c, cell, xs[head], ys[head], strength, theta2, head,
+      put(tc, c, cell, xs[head], ys[head], strength, theta2, 1, head,
           TC_LEAF | (truthy(strength) ? TC_REACH : 0));
       continue;
     }
     double st = 0, weight = 0, x = 0, y = 0;
     bool reach = false;
+    uint32_t skip = 1;
     for (int q = 0; q < 4; ++q) {
       const uint32_t ch = cell.child[q];
       if (ch == NONE) continue;
+      skip += tc[ch].skip;
       const double v = tc[ch].value;
       const double w = fabs(v);
       if (w > 0) {  // `c = Math.abs(q.value)` is truthy
@@ -160,179 +165,164 @@ __global__ __launch_bounds__(ACC_THREADS) void k_accumulate_level(
       }
       reach = reach || (tc[ch].flags & TC_REACH);
     }
-    put(tc, c, cell, x / weight, y / weight, st, theta2, NONE, (reach && truthy(st)) ? TC_REACH : 0);
+    put(tc, c, cell, x / weight, y / weight, st, theta2, skip, NONE, (reach && truthy(st)) ? TC_REACH : 0);
   }
 }
 
-// A TCell as 16 dwords: qx 0-1, qy 2-3, value 4-5, w2t 6-7, child 8-11,
-// first 12, count 13, head 14, flags 15.
-typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
-static_assert(sizeof(u32x16) == sizeof(TCell), "traversal record");
-
-__device__ inline double rec_f64(const u32x16& r, int w) {
-  return __builtin_bit_cast(double, (uint64_t)r[w + 1] << 32 | r[w]);
+__device__ inline double f64_of(uint32_t lo, uint32_t hi) {
+  return __builtin_bit_cast(double, (uint64_t)hi << 32 | lo);
 }
 
 // d3's jiggle: (random() - 0.5) * 1e-6 (rng.zig:7-11).
 __device__ inline double jiggle(double r) { return (r - 0.5) * 1e-6; }
 
-template <int MODE, bool GLOBAL_STACK>
-__global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
-    const TCell* __restrict__ tc, const uint32_t* __restrict__ order, uint32_t count,
-    const uint32_t* __restrict__ perm, const double* __restrict__ xs, const double* __restrict__ ys,
-    const double* __restrict__ s, double* __restrict__ vxs, double* __restrict__ vys, double dmin2, double dmax2,
-    double alpha, uint32_t* __restrict__ flagged, uint32_t* __restrict__ n_flagged,
-    uint32_t* __restrict__ draws_count, const uint32_t* __restrict__ draw_off, const double* __restrict__ draws,
-    uint32_t stack_cap, uint32_t* __restrict__ gstack_cells, uint64_t* __restrict__ gstack_masks,
-    unsigned long long* __restrict__ counters, uint32_t* __restrict__ err) {
-  extern __shared__ uint64_t lds[];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint64_t* st_mask;
-  uint32_t* st_cell;
-  if (GLOBAL_STACK) {
-    const size_t w = (size_t)blockIdx.x * TRAV_WAVES + wave;
-    st_mask = gstack_masks + w * stack_cap;
-    st_cell = gstack_cells + w * stack_cap;
-  } else {
-    st_mask = lds + (size_t)wave * stack_cap;
-    st_cell = (uint32_t*)(lds + (size_t)TRAV_WAVES * stack_cap) + (size_t)wave * stack_cap;
+constexpr uint32_t XCDS = 8;  // MI355X: workgroups go to the XCDs round-robin
+constexpr int WALKS = 2;      // nodes per lane, walked side by side
+
+// One node's walk: its position, its velocity being summed, the cell it is
+// at, its draws (COUNT / REPLAY) and its work counts.
+struct Walk {
+  double nx, ny, vx, vy;
+  uint32_t i, c, dptr, ndraw, n_visit, n_accept, n_pair;
+  bool aborted;
+};
+
+// The jiggle and distanceMin rules of a visit (many_body.zig:203-207,
+// 213-218): in RECORD mode a zero delta aborts the walk (false), else dx, dy
+// take their draws and l its new value.
+template <int MODE>
+__device__ inline bool special(Walk& w, double& dx, double& dy, double& l, double dmin2,
+                               const double* __restrict__ draws) {
+  const bool jx = dx == 0, jy = dy == 0;
+  if (MODE == TRAV_RECORD && (jx || jy)) {
+    w.aborted = true;  // the first point where d3 would call random
+    return false;
   }
-  const uint32_t slot = blockIdx.x * TRAV_THREADS + threadIdx.x;
-  const bool active = slot < count;
-  const uint32_t i = active ? order[slot] : 0;
-  double nx = 0, ny = 0, vx = 0, vy = 0;
-  if (active) {
-    nx = xs[i];
-    ny = ys[i];
-    vx = vxs[i];
-    vy = vys[i];
+  if (jx) {
+    if (MODE == TRAV_COUNT) { ++w.ndraw; dx = 1; } else dx = jiggle(draws[w.dptr++]);
+    l += dx * dx;
   }
-  uint64_t alive = __ballot(active);
-  if (alive == 0) return;
-  uint32_t ndraw = 0;
-  uint32_t n_visit = 0, n_accept = 0, n_pair = 0;
-  uint32_t dptr = (MODE == TRAV_REPLAY && active) ? draw_off[slot] : 0;
-  bool aborted = false;
-  if (lane == 0) {
-    st_cell[0] = 0;
-    st_mask[0] = alive;
+  if (jy) {
+    if (MODE == TRAV_COUNT) { ++w.ndraw; dy = 1; } else dy = jiggle(draws[w.dptr++]);
+    l += dy * dy;
   }
-  int sp = 1;
-  while (sp > 0) {
-    --sp;
-    __builtin_amdgcn_wave_barrier();
-    const uint32_t c = __builtin_amdgcn_readfirstlane(st_cell[sp]);
-    const uint64_t mraw = st_mask[sp];
-    const uint32_t mlo = __builtin_amdgcn_readfirstlane((uint32_t)mraw);
-    const uint32_t mhi = __builtin_amdgcn_readfirstlane((uint32_t)(mraw >> 32));
-    const uint64_t M = ((uint64_t)mhi << 32 | mlo) & alive;
-    if (M == 0) continue;
-    // The whole 64-byte record in one scalar load (fields as TCell lays them
-    // out), so a visit waits on one load rather than one per field group.
-    const u32x16 rec = *reinterpret_cast<const u32x16*>(tc + c);
-    const double value = rec_f64(rec, 4);
-    if (!truthy(value)) continue;
-    const bool in = (M >> lane) & 1;
-    // Geometry on every lane (used where `in`); a lane accepts the cell when
-    // the Barnes-Hut test passes and the cell lies within distanceMax.
-    double dx = rec_f64(rec, 0) - nx, dy = rec_f64(rec, 2) - ny;
+  if (l < dmin2) l = __builtin_sqrt(dmin2 * l);
+  return true;
+}
+
+// One visit of walk w at its cell, whose first 48 bytes are g0 g1 g2; moves
+// w.c on to the next cell of its sequence (len when it aborts).
+template <int MODE>
+__device__ inline void visit(Walk& w, const uint4& g0, const uint4& g1, const uint4& g2, uint32_t len,
+                             const uint32_t* __restrict__ perm, const double* __restrict__ s, double dmin2,
+                             double dmax2, double alpha, const double* __restrict__ draws) {
+  const double value = f64_of(g1.x, g1.y);
+  const uint32_t skip = g2.x;
+  uint32_t next = w.c + (skip > 0 ? skip : 1);
+  if (truthy(value)) {  // many_body.zig:191: `if (!quad.value) return true`
+    ++w.n_visit;
+    double dx = f64_of(g0.x, g0.y) - w.nx, dy = f64_of(g0.z, g0.w) - w.ny;
     double l = dx * dx + dy * dy;
-    const bool far = rec_f64(rec, 6) < l;  // Barnes-Hut approximation
-    const bool open = in && !far;
-    const bool acc = in && far && l < dmax2;
-    n_visit += in ? 1u : 0u;
-    if (__ballot(acc) != 0) {
-      if (__ballot(acc && (dx == 0 || dy == 0 || l < dmin2)) == 0) {
-        // the common case, no lane needing a jiggle or distanceMin: d3's
-        // terms added on the accepting lanes by select
-        const double tx = dx * value * alpha / l, ty = dy * value * alpha / l;
-        vx = acc ? vx + tx : vx;
-        vy = acc ? vy + ty : vy;
-        n_accept += acc ? 1u : 0u;
-      } else if (acc) {
-        const bool jx = dx == 0, jy = dy == 0;
-        if (MODE == TRAV_RECORD && (jx || jy)) {
-          aborted = true;
+    if (f64_of(g1.z, g1.w) < l) {  // w * w / theta2 < l: the Barnes-Hut approximation
+      if (l < dmax2) {
+        bool add = true;
+        if (dx == 0 || dy == 0 || l < dmin2) add = special<MODE>(w, dx, dy, l, dmin2, draws);
+        if (add) {
+          w.vx += dx * value * alpha / l;
+          w.vy += dy * value * alpha / l;
+          ++w.n_accept;
         } else {
-          if (jx) {
-            if (MODE == TRAV_COUNT) { ++ndraw; dx = 1; } else dx = jiggle(draws[dptr++]);
-            l += dx * dx;
-          }
-          if (jy) {
-            if (MODE == TRAV_COUNT) { ++ndraw; dy = 1; } else dy = jiggle(draws[dptr++]);
-            l += dy * dy;
-          }
-          if (l < dmin2) l = __builtin_sqrt(dmin2 * l);
-          vx += dx * value * alpha / l;
-          vy += dy * value * alpha / l;
-          ++n_accept;
+          next = len;
         }
       }
-    }
-    if (MODE == TRAV_RECORD) alive &= ~__ballot(aborted);
-    const uint64_t O = __ballot(open);
-    if (O == 0) continue;
-    if (!(rec[15] & TC_LEAF)) {
-      if (sp + 4 > (int)stack_cap) {  // cannot happen with cap = 3 depth + 4; if it does, say so
-        if (lane == 0) atomicOr(err, 1u);
-        break;
-      }
-      // Descend: push children 3..0 so 0 is visited first, in one store:
-      // lane L < 4 writes child 3 - L, if present, above the present
-      // children of higher index.
-      const uint32_t h0 = rec[8], h1 = rec[9], h2 = rec[10], h3 = rec[11];
-      const int e0 = h0 != NONE, e1 = h1 != NONE, e2 = h2 != NONE, e3 = h3 != NONE;
-      if (lane < 4) {
-        const uint32_t ch = lane == 0 ? h3 : lane == 1 ? h2 : lane == 2 ? h1 : h0;
-        const int before = lane == 0 ? 0 : lane == 1 ? e3 : lane == 2 ? e3 + e2 : e3 + e2 + e1;
-        if (ch != NONE) {
-          st_cell[sp + before] = ch;
-          st_mask[sp + before] = O;
-        }
-      }
-      sp += e0 + e1 + e2 + e3;
-      continue;
-    }
-    // Leaf: points directly (many_body.zig:208-258).
-    if (open && !(l >= dmax2)) {
-      const uint32_t first = rec[12], cnt = rec[13], head = rec[14];
+    } else if (skip > 1) {
+      next = w.c + 1;  // opened: its first child
+    } else if (!(l >= dmax2)) {
+      // Leaf: points directly (many_body.zig:208-258).
+      const uint32_t first = g2.y, cnt = g2.z, head = g2.w;
       bool go = true;
-      if (head != i || cnt > 1) {
-        const bool jx = dx == 0, jy = dy == 0;
-        if (MODE == TRAV_RECORD && (jx || jy)) {
-          aborted = true;
-          go = false;
-        } else {
-          if (jx) {
-            if (MODE == TRAV_COUNT) { ++ndraw; dx = 1; } else dx = jiggle(draws[dptr++]);
-            l += dx * dx;
-          }
-          if (jy) {
-            if (MODE == TRAV_COUNT) { ++ndraw; dy = 1; } else dy = jiggle(draws[dptr++]);
-            l += dy * dy;
-          }
-          if (l < dmin2) l = __builtin_sqrt(dmin2 * l);
-        }
-      }
+      if (head != w.i || cnt > 1) go = special<MODE>(w, dx, dy, l, dmin2, draws);
       if (go) {
         for (uint32_t k = first + cnt; k > first;) {
           --k;
           const uint32_t j = perm[k];
-          if (j != i) {
+          if (j != w.i) {
             const double wj = s[j] * alpha / l;
-            vx += dx * wj;
-            vy += dy * wj;
-            ++n_pair;
+            w.vx += dx * wj;
+            w.vy += dy * wj;
+            ++w.n_pair;
           }
         }
+      } else {
+        next = len;
       }
     }
-    if (MODE == TRAV_RECORD) alive &= ~__ballot(aborted);
+  }
+  w.c = next;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
+    const TCell* __restrict__ tc, uint32_t len, const uint32_t* __restrict__ order, uint32_t count,
+    const uint32_t* __restrict__ perm, const double* __restrict__ xs, const double* __restrict__ ys,
+    const double* __restrict__ s, double* __restrict__ vxs, double* __restrict__ vys, double dmin2, double dmax2,
+    double alpha, uint32_t* __restrict__ flagged, uint32_t* __restrict__ n_flagged,
+    uint32_t* __restrict__ draws_count, const uint32_t* __restrict__ draw_off, const double* __restrict__ draws,
+    uint32_t per_xcd, uint32_t blocks, unsigned long long* __restrict__ counters) {
+  // Block b runs on XCD b % 8; give each XCD a contiguous range of blocks.
+  const uint32_t block = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
+  if (block >= blocks) return;
+  const int lane = threadIdx.x & 63;
+  // A wave takes WALKS * 64 consecutive slots (Morton-ordered nodes); lane L
+  // walks slots base + L, base + 64 + L, ...
+  const uint32_t base = (block * TRAV_THREADS + (threadIdx.x & ~63u)) * WALKS + lane;
+  Walk w[WALKS];
+#pragma unroll
+  for (int k = 0; k < WALKS; ++k) {
+    const uint32_t slot = base + 64 * k;
+    const bool active = slot < count;
+    Walk& a = w[k];
+    a.i = active ? order[slot] : 0;
+    a.nx = a.ny = a.vx = a.vy = 0;
+    if (active) {
+      a.nx = xs[a.i];
+      a.ny = ys[a.i];
+      a.vx = vxs[a.i];
+      a.vy = vys[a.i];
+    }
+    a.c = active ? 0 : len;  // cell 0 is the root
+    a.dptr = (MODE == TRAV_REPLAY && active) ? draw_off[slot] : 0;
+    a.ndraw = a.n_visit = a.n_accept = a.n_pair = 0;
+    a.aborted = false;
+  }
+  for (;;) {
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < WALKS; ++k) any = any || w[k].c < len;
+    if (!any) break;
+    // Every walk's record first, so their loads are in flight together.
+    uint4 g[WALKS][3];
+#pragma unroll
+    for (int k = 0; k < WALKS; ++k) {
+      const uint4* r = reinterpret_cast<const uint4*>(tc + (w[k].c < len ? w[k].c : 0));
+      g[k][0] = r[0];
+      g[k][1] = r[1];
+      g[k][2] = r[2];
+    }
+#pragma unroll
+    for (int k = 0; k < WALKS; ++k)
+      if (w[k].c < len) visit<MODE>(w[k], g[k][0], g[k][1], g[k][2], len, perm, s, dmin2, dmax2, alpha, draws);
   }
   if (MODE != TRAV_COUNT) {
-    // Work of the lanes whose results stand (an aborted lane is redone by REPLAY).
-    const bool keep = active && !(MODE == TRAV_RECORD && aborted);
-    uint32_t cv = keep ? n_visit : 0, ca = keep ? n_accept : 0, cp = keep ? n_pair : 0;
+    // Work of the walks whose results stand (an aborted walk is redone by REPLAY).
+    uint32_t cv = 0, ca = 0, cp = 0;
+#pragma unroll
+    for (int k = 0; k < WALKS; ++k) {
+      const bool keep = base + 64 * k < count && !(MODE == TRAV_RECORD && w[k].aborted);
+      cv += keep ? w[k].n_visit : 0;
+      ca += keep ? w[k].n_accept : 0;
+      cp += keep ? w[k].n_pair : 0;
+    }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
       cv += __shfl_xor(cv, d, 64);
@@ -345,15 +335,19 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
       atomicAdd(&counters[2], (unsigned long long)cp);
     }
   }
-  if (!active) return;
-  if (MODE == TRAV_RECORD && aborted) {
-    const uint32_t k = atomicAdd(n_flagged, 1u);
-    flagged[k] = i;
-  } else if (MODE == TRAV_COUNT) {
-    draws_count[slot] = ndraw;
-  } else {
-    vxs[i] = vx;
-    vys[i] = vy;
+#pragma unroll
+  for (int k = 0; k < WALKS; ++k) {
+    const uint32_t slot = base + 64 * k;
+    if (slot >= count) continue;
+    if (MODE == TRAV_RECORD && w[k].aborted) {
+      const uint32_t q = atomicAdd(n_flagged, 1u);
+      flagged[q] = w[k].i;
+    } else if (MODE == TRAV_COUNT) {
+      draws_count[slot] = w[k].ndraw;
+    } else {
+      vxs[w[k].i] = w[k].vx;
+      vys[w[k].i] = w[k].vy;
+    }
   }
 }
 
@@ -411,37 +405,15 @@ hipError_t mb_publish_charges(TreeBuffers& t, hipStream_t stream) {
 
 hipError_t mb_traverse(const TravArgs& a, int mode, hipStream_t stream) {
   if (a.count == 0) return hipSuccess;
-  const uint32_t blocks = ceil_div(a.count, TRAV_THREADS);
-  const size_t lds = (size_t)TRAV_WAVES * a.stack_cap * (sizeof(uint64_t) + sizeof(uint32_t));
-  const bool global = lds > 48 * 1024;
-  TreeBuffers& st = *a.stacks;
-  if (global) {
-    const size_t need = (size_t)blocks * TRAV_WAVES * a.stack_cap;
-    if (need > st.tstack_entries) {
-      if (st.tstack_cells) HIPCHECK(hipFree(st.tstack_cells));
-      if (st.tstack_masks) HIPCHECK(hipFree(st.tstack_masks));
-      st.tstack_cells = nullptr;
-      st.tstack_masks = nullptr;
-      st.tstack_entries = 0;
-      HIPCHECK(hipMalloc(&st.tstack_cells, need * sizeof(uint32_t)));
-      HIPCHECK(hipMalloc(&st.tstack_masks, need * sizeof(uint64_t)));
-      st.tstack_entries = need;
-    }
-  }
-#define TRAV_LAUNCH(M, G)                                                                                        \
-  hipLaunchKernelGGL((k_traverse<M, G>), dim3(blocks), dim3(TRAV_THREADS), G ? 0 : lds, stream, a.tc, a.order,   \
-                     a.count, a.perm, a.x, a.y, a.s, a.vx, a.vy, a.dmin2, a.dmax2, a.alpha, a.flagged,          \
-                     a.n_flagged, a.draws_count, a.draw_off, a.draws, a.stack_cap, st.tstack_cells,              \
-                     st.tstack_masks, a.counters, a.error)
-  if (global) {
-    if (mode == TRAV_RECORD) TRAV_LAUNCH(TRAV_RECORD, true);
-    else if (mode == TRAV_COUNT) TRAV_LAUNCH(TRAV_COUNT, true);
-    else TRAV_LAUNCH(TRAV_REPLAY, true);
-  } else {
-    if (mode == TRAV_RECORD) TRAV_LAUNCH(TRAV_RECORD, false);
-    else if (mode == TRAV_COUNT) TRAV_LAUNCH(TRAV_COUNT, false);
-    else TRAV_LAUNCH(TRAV_REPLAY, false);
-  }
+  const uint32_t blocks = ceil_div(a.count, TRAV_THREADS * WALKS);
+  const uint32_t per_xcd = ceil_div(blocks, XCDS);
+#define TRAV_LAUNCH(M)                                                                                            \
+  hipLaunchKernelGGL((k_traverse<M>), dim3(per_xcd * XCDS), dim3(TRAV_THREADS), 0, stream, a.tc, a.len, a.order,  \
+                     a.count, a.perm, a.x, a.y, a.s, a.vx, a.vy, a.dmin2, a.dmax2, a.alpha, a.flagged,           \
+                     a.n_flagged, a.draws_count, a.draw_off, a.draws, per_xcd, blocks, a.counters)
+  if (mode == TRAV_RECORD) TRAV_LAUNCH(TRAV_RECORD);
+  else if (mode == TRAV_COUNT) TRAV_LAUNCH(TRAV_COUNT);
+  else TRAV_LAUNCH(TRAV_REPLAY);
 #undef TRAV_LAUNCH
   return hipGetLastError();
 }

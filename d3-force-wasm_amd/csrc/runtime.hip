@@ -67,14 +67,6 @@ int32_t status_of(hipError_t e) {
   return WORKER_FAILED;
 }
 
-// A traversal whose stack would have overflowed stopped early: its result
-// is wrong, so the step fails (the facade's failover reruns it).
-int32_t traversal_overflow() {
-  std::snprintf(g_last_error, sizeof g_last_error,
-                "traversal stack overflow in k_traverse (many_body.hip): the step's velocities are incomplete");
-  return WORKER_FAILED;
-}
-
 // Failure injection for tests of the failover path: after g_fail_after more
 // steps (force_run, sim_integrate), the next g_fail_count steps return
 // g_fail_status before any device work.
@@ -509,21 +501,16 @@ int32_t run_many_body_exact(Sim& s, Force& f, double alpha) {
   a.alpha = alpha;
   a.flagged = s.flagged.p;
   a.n_flagged = s.dscalar_u;
-  a.stack_cap = 3 * t.meta_host->depth + 4;
+  a.len = t.meta_host->len;
   a.counters = s.dcounters;
-  a.error = s.dscalar_u + 2;
-  a.stacks = &t;
-  // [0] flagged nodes, [2] the traversal's stack-overflow flag
-  if ((e = hipMemsetAsync(s.dscalar_u, 0, 3 * sizeof(uint32_t), s.stream)) != hipSuccess) return status_of(e);
+  if ((e = hipMemsetAsync(s.dscalar_u, 0, sizeof(uint32_t), s.stream)) != hipSuccess) return status_of(e);
   s.timer_start(D3F_PHASE_TRAVERSAL);
   if ((e = mb_traverse(a, TRAV_RECORD, s.stream)) != hipSuccess) return status_of(e);
   s.timer_stop();
   if ((e = mb_nan_nodes(t, s.n, s.dvx.p, s.dvy.p, s.stream)) != hipSuccess) return status_of(e);
-  if ((e = hipMemcpyAsync(s.hscalar_u, s.dscalar_u, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream)) !=
-      hipSuccess)
+  if ((e = hipMemcpyAsync(s.hscalar_u, s.dscalar_u, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream)) != hipSuccess)
     return status_of(e);
   if ((e = hipStreamSynchronize(s.stream)) != hipSuccess) return status_of(e);
-  if (s.hscalar_u[2]) return traversal_overflow();
   const uint32_t nf = s.hscalar_u[0];
   s.stats.flagged += nf;
   if (nf == 0) return OK;
@@ -557,11 +544,7 @@ int32_t run_many_body_exact(Sim& s, Force& f, double alpha) {
   a.draw_off = s.draw_off.p;
   a.draws = s.draws.p;
   if ((e = mb_traverse(a, TRAV_REPLAY, s.stream)) != hipSuccess) return status_of(e);
-  if ((e = hipMemcpyAsync(s.hscalar_u + 2, s.dscalar_u + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream)) !=
-      hipSuccess)
-    return status_of(e);
-  if ((e = hipStreamSynchronize(s.stream)) != hipSuccess) return status_of(e);
-  return s.hscalar_u[2] ? traversal_overflow() : OK;
+  return status_of(hipStreamSynchronize(s.stream));
 }
 
 // ---------------------------------------------------------------- link (exact)

@@ -914,7 +914,7 @@ void tree_free(TreeBuffers& t) {
   void* ptrs[] = {t.keys, t.keys_tmp, t.keys_all, t.perm, t.perm_tmp, t.rest, t.flags, t.offs, t.count,
                   t.base, t.dleft, t.gend, t.leafdepth, t.mixed, t.mixed_list, t.hist, t.partials,
                   t.box_partials, t.deep_stack, t.cells, t.tcells, t.cdepth, t.dbins, t.dstart, t.dorder,
-                  t.rs_glob, t.cell_s, t.cell_d, t.meta, t.tstack_cells, t.tstack_masks};
+                  t.rs_glob, t.cell_s, t.cell_d, t.meta};
   for (void* p : ptrs) if (p) (void)hipFree(p);
   if (t.meta_host) (void)hipHostFree(t.meta_host);
   t = TreeBuffers();
