@@ -1300,6 +1300,85 @@ EXPORT int32_t force_direct_sum(uint32_t h, uint32_t fh, double alpha, const uin
   return status_of(e);
 }
 
+// ---- force_direct_sum without allocation: the staging of the sample
+// indices and the outputs lives in `scratch`, a file-scope device array of
+// fixed size (each device holds its own copy until the process ends). Host
+// code never passes `scratch` itself to the device: its device address is
+// taken with hipGetSymbolAddress once per device and kept in g_scratch_dev.
+constexpr uint32_t STATIC_SAMPLES = 16384;  // the most samples one call takes
+constexpr size_t SCRATCH_OUT = (size_t)STATIC_SAMPLES * 2 * sizeof(double);  // outputs first (256-aligned)
+static __device__ __align__(256) unsigned char scratch[SCRATCH_OUT + (size_t)STATIC_SAMPLES * sizeof(uint32_t)];
+constexpr int MAX_DEVICES = 64;
+unsigned char* g_scratch_dev[MAX_DEVICES] = {nullptr};
+
+// The scratch array's device address on the current device (set up on the
+// first call there).
+hipError_t scratch_address(unsigned char** out) {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) HIPFAIL(hipErrorInvalidDevice, "device index beyond the scratch table");
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch_dev[dev];
+  return hipSuccess;
+}
+
+// upload() without allocation: the force's device arrays must already hold
+// `count` entries (a force_run of the force sizes them).
+int32_t upload_in_place(Sim& s, Force& f) {
+  if (!f.dirty) return OK;
+  for (int i = 0; i < f.float_arrays(); ++i)
+    if (f.df[i].cap < f.count || !f.df[i].p) {
+      std::snprintf(g_last_error, sizeof g_last_error,
+                    "force_direct_sum_static: the force's parameters are not on the device yet (run the force "
+                    "once, or call force_direct_sum, after changing its arrays)");
+      return BAD_ARGUMENT;
+    }
+  for (int i = 0; i < f.float_arrays(); ++i) {
+    if (!f.count) continue;
+    hipError_t e = hipMemcpyAsync(f.df[i].p, f.f[i].p, f.count * sizeof(double), hipMemcpyHostToDevice, s.stream);
+    if (e != hipSuccess) return status_of(e);
+  }
+  hipError_t e = hipStreamSynchronize(s.stream);
+  if (e != hipSuccess) return status_of(e);
+  f.dirty = false;
+  f.sched_valid = false;
+  return OK;
+}
+
+EXPORT int32_t force_direct_sum_static(uint32_t h, uint32_t fh, double alpha, const uint32_t* sample, uint32_t k,
+                                       double* out) {
+  Sim* s = get(h);
+  Force* f = get_force(s, fh);
+  if (!f) return BAD_HANDLE;
+  if (f->kind != K_MANY_BODY || f->count < s->n || (k && (!sample || !out))) return BAD_ARGUMENT;
+  if (k > STATIC_SAMPLES) {
+    std::snprintf(g_last_error, sizeof g_last_error,
+                  "force_direct_sum_static: %u samples exceed its fixed scratch of %u (use force_direct_sum)", k,
+                  STATIC_SAMPLES);
+    return BAD_ARGUMENT;
+  }
+  for (uint32_t q = 0; q < k; ++q)
+    if (sample[q] >= s->n) return BAD_ARGUMENT;
+  const int32_t st = upload_in_place(*s, *f);
+  if (st != OK) return st;
+  if (k == 0) return OK;
+  unsigned char* dev = nullptr;
+  hipError_t e = scratch_address(&dev);
+  double* dout = reinterpret_cast<double*>(dev);
+  uint32_t* dsample = reinterpret_cast<uint32_t*>(dev + SCRATCH_OUT);
+  if (e == hipSuccess) e = hipMemcpyAsync(dsample, sample, k * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess)
+    e = direct_sum(s->dx.p, s->dy.p, f->df[0].p, s->n, dsample, k, alpha, f->scalar(S_DMIN2), f->scalar(S_DMAX2),
+                   dout, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, 2 * k * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  return status_of(e);
+}
+
 EXPORT int32_t sim_stats(uint32_t h, d3f_stats* out) {
   Sim* s = get(h);
   if (!s || !out) return BAD_HANDLE;

@@ -135,6 +135,18 @@ int32_t sim_stats(uint32_t h, d3f_stats* out);
  * skipped) on the current positions, using many-body force fh's strengths
  * and distance scalars. sample and out are host pointers. */
 int32_t force_direct_sum(uint32_t h, uint32_t fh, double alpha, const uint32_t* sample, uint32_t k, double* out);
+/* force_direct_sum without any device allocation (no reference counterpart;
+ * for callers that cannot allocate: code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). Same arguments and results, bit for
+ * bit; the sample indices and outputs are staged in a fixed device array of
+ * the library (STATIC_SAMPLES = 16384 samples at most; more return
+ * BAD_ARGUMENT, with core_last_error saying so), held by each device until the
+ * process ends. The force's parameter arrays must already be on the device
+ * (any force_run of the force, or force_direct_sum, puts them there);
+ * otherwise BAD_ARGUMENT. Every call on one device shares that array: calls
+ * that overlap on one GPU (several host threads) race. */
+int32_t force_direct_sum_static(uint32_t h, uint32_t fh, double alpha, const uint32_t* sample, uint32_t k,
+                                double* out);
 void core_set_timing(int on);
 /* Waits for all work the library queued on the device (benchmark brackets). */
 int32_t core_synchronize(void);
