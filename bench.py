@@ -385,12 +385,12 @@ def cpu_baseline(n, m, pin_fraction, warmup, ticks, rounds, threads, single_tick
     log("cpu baseline: single thread %.3f ticks/s" % (single_ticks / one))
     span = "ticks %d-%d" % (warmup + 1, warmup + ticks)
     return {"value": ticks / best, "unit": "ticks/s", "cores": threads, "kind": "port",
-            "sample": "%s of the same 1M-node workload (the GPU's timed ticks): %d warm-up ticks, then the fastest "
+            "sample": "%s of the same %d-node workload (the GPU's timed ticks): %d warm-up ticks, then the fastest "
                       "of %d rounds of %d ticks, each round restarted from the state after the warm-up "
                       "(bench/threads.js:24-35's method); C++ oracle (restatement of the reference core) with the "
                       "reference's pool chunking for tree build, accumulate, traversal and integrate, link and "
                       "center serial as in the reference, on %d threads (the box's CPU share for one GPU)"
-                      % (span, warmup, rounds, ticks, threads),
+                      % (span, n, warmup, rounds, ticks, threads),
             "host": host_cpu(),
             "single_thread": {"value": single_ticks / one, "unit": "ticks/s", "cores": 1,
                               "sample": "ticks %d-%d from the same post-warm-up state, one round"
@@ -458,6 +458,8 @@ def main():
     core.core_set_timing(0)
     st = core.stats(ctx.h)
     sim.end(ctx)
+    ticks_per_s = args.steps / dt
+    log("%.3f ms/tick, phases %s" % (1e3 * dt / args.steps, st["ms"]))
     # The facade's tick(k) with the boundary copies (copy-in of 7 arrays and
     # copy-out of 4 per call, over PCIe): k = 1 and k = steps, after the
     # timed region.
