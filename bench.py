@@ -175,7 +175,7 @@ def run_batch(core, graphs, steps, warmup, seed=1):
     # to 300 nodes of each sampled graph.
     rng = np.random.default_rng(7)
     pick = sorted(rng.choice(graphs, min(graphs, 16), replace=False).tolist())
-    probe = Batch([gs[g] for g in pick], core=core, seed_ids=pick)
+    probe = Batch([gs[g] for g in pick], core=core)
     probe.scalar(BS_LINK_ITERATIONS, 0).scalar(BS_CENTER_STRENGTH, 0).scalar(BS_VELOCITY_DECAY, 1)
     s0 = probe.state()
     probe.tick(1)

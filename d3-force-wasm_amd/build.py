@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBRARY = os.path.join(HERE, "libd3fgpu.so")
-SOURCES = ["runtime.hip", "tree.hip", "many_body.hip", "link.hip", "collide.hip", "simple.hip", "fast.hip", "fmm.hip", "batch.hip", "gnn.hip"]
+SOURCES = ["runtime.hip", "tree.hip", "many_body.hip", "link.hip", "collide.hip", "simple.hip", "fast.hip", "fmm.hip", "batch.hip", "batch_exact.hip", "gnn.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIPFLAGS = [

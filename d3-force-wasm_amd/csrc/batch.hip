@@ -10,9 +10,13 @@
 //  - link: the adjacency sweep (link_fast.zig) over the arena's rows;
 //  - center: each graph's serial sums (simple.zig:14-31), one wave a graph;
 //  - integrate: per node.
-// Jiggle seeds are per graph and per tick, mixed from the graph's seed id, and
-// hash indices within the graph, so a graph's result does not depend on the
-// other graphs of the batch.
+// A graph below the fast-mode crossover (n < 2000, root.zig:246-248) runs
+// manyBody and link with the exact kernels instead, as the reference does,
+// one workgroup per graph (batch_exact.hip). Each graph has its own random
+// source, d3's lcg (lcg.js), from which the facade's draws are taken: per
+// tick the manyBody seed and the link seed (core.js:218-222) and, for a small
+// graph, the exact kernels' jiggle draws; so a graph's result does not depend
+// on the other graphs of the batch.
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -36,12 +40,12 @@ int32_t bstatus(hipError_t e) {
   return B_FAILED;
 }
 
-// splitmix64 of (id, tick, salt): the graph's seed for one force and tick.
-__host__ __device__ inline uint64_t mix_seed(uint64_t id, uint64_t tick, uint64_t salt) {
-  uint64_t h = id * 0x9E3779B97F4A7C15ull ^ (tick << 8 | salt);
-  h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
-  h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
-  return h ^ (h >> 31);
+constexpr uint32_t BATCH_MIN_N = 2000;  // fmm.zig:43, the fast-mode crossover
+
+// d3's lcg (lcg.js): s = (1664525 s + 1013904223) mod 2^32, random() = s / 2^32.
+__device__ inline double batch_lcg(uint32_t& s) {
+  s = 1664525u * s + 1013904223u;
+  return (double)s / 4294967296.0;
 }
 
 // Per-tick scalars on the device (so that a captured tick reads them):
@@ -60,14 +64,19 @@ __global__ void k_batch_step(BatchState* __restrict__ st, double target, double 
   st->tick += 1;
 }
 
-__global__ __launch_bounds__(256) void k_batch_seeds(const BatchState* __restrict__ st,
-                                                     const uint64_t* __restrict__ ids, uint32_t graphs,
-                                                     uint64_t* __restrict__ mb, uint64_t* __restrict__ link) {
+// The fast-mode seeds of one tick for the graphs at or above the crossover:
+// manyBody's then link's, each one draw of the graph's random source taken as
+// the seed's bits (core.js:218-222, hash.zig:28-30). The small graphs' draws
+// are taken by the exact kernels, in the facade's order.
+__global__ __launch_bounds__(256) void k_batch_seeds(const uint32_t* __restrict__ node_off, uint32_t graphs,
+                                                     uint32_t* __restrict__ lcg, uint64_t* __restrict__ mb,
+                                                     uint64_t* __restrict__ link) {
   const uint32_t g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= graphs) return;
-  const uint64_t t = st->used;
-  mb[g] = mix_seed(ids[g], t, 0);
-  link[g] = mix_seed(ids[g], t, 1);
+  if (g >= graphs || node_off[g + 1] - node_off[g] < BATCH_MIN_N) return;
+  uint32_t s = lcg[g];
+  mb[g] = __builtin_bit_cast(uint64_t, batch_lcg(s));
+  link[g] = __builtin_bit_cast(uint64_t, batch_lcg(s));
+  lcg[g] = s;
 }
 
 struct Batch {
@@ -80,7 +89,18 @@ struct Batch {
   Device<double2> dpos;  // x + vx, y + vy as pairs (link sweep)
   Device<uint8_t> dfixed;
   Device<uint32_t> dsrc, dtgt, dnode_graph, dnode_off, dlink_off;
-  Device<uint64_t> dseeds_link, dseeds_mb, dseed_ids;
+  Device<uint64_t> dseeds_link, dseeds_mb;
+  Device<uint32_t> dlcg, snap_lcg;  // per graph: the random state, and its copy from before the tick
+  // the graphs below the crossover, ticked by the exact kernels
+  std::vector<SmallJob> small;
+  Device<SmallJob> dsmall;
+  Device<uint8_t> scells;
+  Device<TCell> stcells;
+  Device<double> vout_x, vout_y;
+  Device<SmallLink> slinks;     // their link schedules (k_small_schedule)
+  Device<uint32_t> slevels, sorder;
+  uint32_t* derror = nullptr;  // device: a small graph's tree outgrew its kernel
+  uint32_t* herror = nullptr;  // pinned mirror
   BatchState* dstate = nullptr;
   bool state_dirty = true;     // the host's alpha / tick must be uploaded before the next tick
   // captured tick (D3F_BS_GRAPH)
@@ -115,7 +135,16 @@ struct Batch {
     for (auto* a : {&dsrc, &dtgt, &dnode_graph, &dnode_off, &dlink_off, &row, &adj, &owner, &rows_long, &sv0, &sv1,
                     &shist, &sglob, &scount})
       a->release();
-    for (auto* a : {&dseeds_link, &dseeds_mb, &dseed_ids, &sk0, &sk1}) a->release();
+    for (auto* a : {&dseeds_link, &dseeds_mb, &sk0, &sk1}) a->release();
+    for (auto* a : {&dlcg, &snap_lcg}) a->release();
+    dsmall.release();
+    scells.release();
+    stcells.release();
+    for (auto* a : {&vout_x, &vout_y}) a->release();
+    slinks.release();
+    for (auto* a : {&slevels, &sorder}) a->release();
+    if (derror) (void)hipFree(derror);
+    if (herror) (void)hipHostFree(herror);
     snap.release();
     drop_graph();
     if (dstate) (void)hipFree(dstate);
@@ -140,6 +169,10 @@ struct Batch {
   hipError_t init() {
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHECK(hipMalloc(&dstate, sizeof(BatchState)));
+    HIPCHECK(hipMalloc(&derror, sizeof(uint32_t)));
+    HIPCHECK(hipHostMalloc((void**)&herror, sizeof(uint32_t), hipHostMallocDefault));
+    HIPCHECK(hipMemset(derror, 0, sizeof(uint32_t)));
+    scalars[D3F_BS_THETA2] = 0.81;  // manyBody.js:9, theta 0.9
     HIPCHECK(sglob.reserve(8 * 256));
     HIPCHECK(scount.reserve(4));
     // d3's defaults (simulation.js:18-31, manyBody.js:5-12, center.js:4-7)
@@ -171,7 +204,7 @@ struct Batch {
     for (auto* a : {&lstrength, &ldistance, &lbias}) HIPCHECK(a->reserve(cm, 0));
     for (auto* a : {&source, &target}) HIPCHECK(a->reserve(cm, 0));
     for (auto* a : {&seed_ids, &seeds_mb, &seeds_link}) HIPCHECK(a->reserve(cg, 0));
-    for (uint32_t k = 0; k < g; ++k) seed_ids.p[k] = k;
+    for (uint32_t k = 0; k < g; ++k) seed_ids.p[k] = 1;  // lcg.js starts at s = 1
     for (auto* a : {&dx, &dy, &dvx, &dvy, &dcharge, &dzero}) HIPCHECK(a->reserve(cn, 0, stream));
     HIPCHECK(dpos.reserve(cn, 0, stream));
     for (auto* a : {&dls, &dld, &dlb}) HIPCHECK(a->reserve(cm, 0, stream));
@@ -184,13 +217,42 @@ struct Batch {
     HIPCHECK(dlink_off.reserve(cg + 1, 0, stream));
     HIPCHECK(dseeds_link.reserve(cg, 0, stream));
     HIPCHECK(dseeds_mb.reserve(cg, 0, stream));
-    HIPCHECK(dseed_ids.reserve(cg, 0, stream));
+    HIPCHECK(dlcg.reserve(cg, 0, stream));
+    HIPCHECK(snap_lcg.reserve(cg, 0, stream));
+    // the exact kernels' jobs: graphs below the crossover, 3 n + 128 cells each
+    // and their link schedules (built with the link rows): m records and
+    // m + 1 level starts each
+    small.clear();
+    uint32_t cells = 0, recs = 0, levs = 0;
+    for (uint32_t k = 0; k < g; ++k) {
+      const uint32_t nk = node_off[k + 1] - node_off[k];
+      if (nk >= BATCH_MIN_N) continue;
+      const uint32_t mk = link_off[k + 1] - link_off[k];
+      SmallJob j{node_off[k], nk, link_off[k], mk, cells, 3 * nk + 128, k, recs, levs, 0, {0, 0}};
+      small.push_back(j);
+      cells += j.cell_cap;
+      recs += mk;
+      levs += mk + 1;
+    }
+    HIPCHECK(dsmall.reserve(small.empty() ? 1 : small.size(), 0, stream));
+    HIPCHECK(scells.reserve((size_t)(cells > 0 ? cells : 1) * small_cell_bytes(), 0, stream));
+    HIPCHECK(stcells.reserve(cells > 0 ? cells : 1, 0, stream));
+    HIPCHECK(slinks.reserve(recs > 0 ? recs : 1, 0, stream));
+    HIPCHECK(sorder.reserve(recs > 0 ? recs : 1, 0, stream));
+    HIPCHECK(slevels.reserve(levs > 0 ? levs : 1, 0, stream));
+    for (auto* a : {&vout_x, &vout_y}) HIPCHECK(a->reserve(cn, 0, stream));
+    if (!small.empty())
+      HIPCHECK(hipMemcpyAsync(dsmall.p, small.data(), small.size() * sizeof(SmallJob), hipMemcpyHostToDevice, stream));
     HIPCHECK(hipMemsetAsync(dfixed.p, 0, cn, stream));
     HIPCHECK(hipMemsetAsync(dzero.p, 0, cn * sizeof(double), stream));
     if (n) HIPCHECK(hipMemcpyAsync(dnode_graph.p, node_graph.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     HIPCHECK(hipMemcpyAsync(dnode_off.p, node_off.data(), (g + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     HIPCHECK(hipMemcpyAsync(dlink_off.p, link_off.data(), (g + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
-    HIPCHECK(hipMemcpyAsync(dseed_ids.p, seed_ids.p, (size_t)g * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+    {
+      std::vector<uint32_t> s1(g, 1u);
+      HIPCHECK(hipMemcpyAsync(dlcg.p, s1.data(), (size_t)g * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+    }
     csr_valid = false;
     drop_graph();
     return hipStreamSynchronize(stream);
@@ -238,6 +300,8 @@ struct Batch {
     sb.count = scount.p;
     sb.cap = (uint32_t)(2 * c);
     HIPCHECK(link_csr(csr_args(), sb, &long_count, stream));
+    HIPCHECK(small_schedule(dsmall.p, (uint32_t)small.size(), dsrc.p, dtgt.p, dls.p, dld.p, dlb.p, sorder.p,
+                            slevels.p, slinks.p, stream));
     csr_valid = true;
     return hipSuccess;
   }
@@ -255,19 +319,76 @@ struct Batch {
     return a;
   }
 
+  // A small graph's tree outgrew the exact kernel (it wrote only vout):
+  // the random states go back to their copy from before the tick, the
+  // host's alpha and tick count are uploaded again, the flag is cleared.
+  hipError_t small_overflow() {
+    HIPCHECK(hipMemcpyAsync(dlcg.p, snap_lcg.p, (size_t)graphs * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemsetAsync(derror, 0, sizeof(uint32_t), stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    state_dirty = true;
+    note_failure(hipErrorNotSupported,
+                 "batch_tick: a graph below 2000 nodes needs a quadtree deeper than the batched exact kernel "
+                 "holds (about 60 levels) or more than 3 n + 128 cells; the tick did not run",
+                 __FILE__, __LINE__);
+    return hipErrorNotSupported;
+  }
+
   // The device part of a tick. graph: the capturable form (no host
-  // synchronisation; nothing the host reads back).
+  // synchronisation; nothing the host reads back). Eagerly, the small
+  // graphs' kernel runs first and its overflow flag is read before anything
+  // writes the node arrays, so a tick that fails leaves them as they were.
   hipError_t device_tick(bool graph) {
     hipError_t e;
+    if (!graph)
+      HIPCHECK(hipMemcpyAsync(snap_lcg.p, dlcg.p, (size_t)graphs * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
     hipLaunchKernelGGL(k_batch_step, dim3(1), dim3(64), 0, stream, dstate, scalars[D3F_BS_ALPHA_TARGET],
                        scalars[D3F_BS_ALPHA_DECAY]);
-    hipLaunchKernelGGL(k_batch_seeds, dim3(ceil_div(graphs, 256u)), dim3(256), 0, stream, dstate, dseed_ids.p, graphs,
+    hipLaunchKernelGGL(k_batch_seeds, dim3(ceil_div(graphs, 256u)), dim3(256), 0, stream, dnode_off.p, graphs, dlcg.p,
                        dseeds_mb.p, dseeds_link.p);
+    const bool timed = g_batch_timing && !graph;
+    hipEvent_t ev = timed ? mark() : nullptr;
+    // the small graphs' exact manyBody and link, from the tick's starting state
+    if (!small.empty()) {
+      SmallExactArgs sa;
+      sa.x = dx.p;
+      sa.y = dy.p;
+      sa.vx = dvx.p;
+      sa.vy = dvy.p;
+      sa.charge = dcharge.p;
+      sa.src = dsrc.p;
+      sa.tgt = dtgt.p;
+      sa.lstrength = dls.p;
+      sa.ldistance = dld.p;
+      sa.lbias = dlb.p;
+      sa.lrec = slinks.p;
+      sa.levels = slevels.p;
+      sa.alpha_dev = &dstate->alpha;
+      sa.theta2 = scalars[D3F_BS_THETA2];
+      sa.dmin2 = scalars[D3F_BS_DISTANCE_MIN2];
+      sa.dmax2 = scalars[D3F_BS_DISTANCE_MAX2];
+      sa.link_iterations = scalars[D3F_BS_LINK_ITERATIONS];
+      sa.lcg = dlcg.p;
+      sa.cells = scells.p;
+      sa.tcells = stcells.p;
+      sa.vout_x = vout_x.p;
+      sa.vout_y = vout_y.p;
+      sa.error = derror;
+      HIPCHECK(small_exact(dsmall.p, (uint32_t)small.size(), sa, stream));
+      if (!graph) {
+        HIPCHECK(hipMemcpyAsync(herror, derror, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        if (*herror) {
+          if (ev) (void)hipEventDestroy(ev);
+          return small_overflow();
+        }
+      }
+    }
+    timer(D3F_PHASE_TRAVERSAL, ev);
     // many-body
     FmmArgs a = fmm_args();
     a.graph = graph;
-    const bool timed = g_batch_timing && !graph;
-    hipEvent_t ev = timed ? mark() : nullptr;
+    ev = timed ? mark() : nullptr;
     if ((e = fmm_run(fmm, tree, a, stream)) != hipSuccess) return e;
     timer(D3F_PHASE_FMM, ev);
     // link
@@ -277,6 +398,8 @@ struct Batch {
       HIPCHECK(link_sweep(csr_args(), long_count, dpos.p, dvx.p, dvy.p, scalars[D3F_BS_ALPHA], 0, stream,
                           &dstate->alpha));
     }
+    // the small graphs take their exact velocities
+    HIPCHECK(small_scatter(dsmall.p, (uint32_t)small.size(), vout_x.p, vout_y.p, dvx.p, dvy.p, stream));
     timer(D3F_PHASE_LINK, ev);
     // center
     ev = timed ? mark() : nullptr;
@@ -329,6 +452,8 @@ struct Batch {
     const double* from[4] = {dx.p, dy.p, dvx.p, dvy.p};
     for (int k = 0; k < 4 && e == hipSuccess; ++k)
       e = hipMemcpyAsync(snap.p + (size_t)k * n, from[k], bn, hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(snap_lcg.p, dlcg.p, (size_t)graphs * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
     if (e == hipSuccess) e = device_tick(true);
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(stream, &g);
@@ -360,22 +485,31 @@ struct Batch {
       if ((e = hipMemcpyAsync(fmm.params_host, fmm.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream)) !=
           hipSuccess)
         return bstatus(e);
+      *herror = 0;
+      if (!small.empty() &&
+          (e = hipMemcpyAsync(herror, derror, sizeof(uint32_t), hipMemcpyDeviceToHost, stream)) != hipSuccess)
+        return bstatus(e);
       if ((e = hipStreamSynchronize(stream)) != hipSuccess) return bstatus(e);
       ++graph_replays;
-      if (!fmm.params_host->abort) {
+      if (!fmm.params_host->abort && !*herror) {
         done = true;
-      } else {  // sizes outgrew the captured buffers: undo the replay, redo the tick eagerly
+      } else {  // sizes outgrew the captured buffers, or a small graph its kernel: undo the replay
+        const bool failed = *herror != 0;
         ++graph_redos;
         const size_t bn = (size_t)n * sizeof(double);
         double* to[4] = {dx.p, dy.p, dvx.p, dvy.p};
         for (int k = 0; k < 4; ++k)
           if ((e = hipMemcpyAsync(to[k], snap.p + (size_t)k * n, bn, hipMemcpyDeviceToDevice, stream)) != hipSuccess)
             return bstatus(e);
+        if ((e = hipMemcpyAsync(dlcg.p, snap_lcg.p, (size_t)graphs * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                stream)) != hipSuccess)
+          return bstatus(e);
         const BatchState hs{scalars[D3F_BS_ALPHA], (unsigned long long)scalars[D3F_BS_TICK], 0ull};
         if ((e = hipMemcpyAsync(dstate, &hs, sizeof(hs), hipMemcpyHostToDevice, stream)) != hipSuccess ||
             (e = hipStreamSynchronize(stream)) != hipSuccess)
           return bstatus(e);
         drop_graph();
+        if (failed) return bstatus(small_overflow());  // the eager tick would fail the same way
       }
     }
     if (!done) {
@@ -478,16 +612,25 @@ EXPORT int32_t batch_sync(uint32_t h, uint32_t direction) {
     struct { void* d; const void* p; size_t bytes; } copies[] = {
         {b->dx.p, b->x.p, bn}, {b->dy.p, b->y.p, bn}, {b->dvx.p, b->vx.p, bn}, {b->dvy.p, b->vy.p, bn},
         {b->dcharge.p, b->charge.p, bn}, {b->dls.p, b->lstrength.p, bm}, {b->dld.p, b->ldistance.p, bm},
-        {b->dlb.p, b->lbias.p, bm}, {b->dsrc.p, b->source.p, (size_t)b->m * 4}, {b->dtgt.p, b->target.p, (size_t)b->m * 4},
-        {b->dseed_ids.p, b->seed_ids.p, (size_t)b->graphs * 8}};
+        {b->dlb.p, b->lbias.p, bm}, {b->dsrc.p, b->source.p, (size_t)b->m * 4}, {b->dtgt.p, b->target.p, (size_t)b->m * 4}};
     for (auto& c : copies)
       if (e == hipSuccess && c.bytes) e = hipMemcpyAsync(c.d, c.p, c.bytes, hipMemcpyHostToDevice, s);
+    std::vector<uint32_t> states(b->graphs);  // the random states, 32 bits each
+    for (uint32_t g = 0; g < b->graphs; ++g) states[g] = (uint32_t)b->seed_ids.p[g];
+    if (e == hipSuccess && b->graphs)
+      e = hipMemcpyAsync(b->dlcg.p, states.data(), (size_t)b->graphs * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     b->csr_valid = false;
   } else if (direction == D3F_SYNC_D2H) {
     struct { void* p; const void* d; } copies[] = {{b->x.p, b->dx.p}, {b->y.p, b->dy.p}, {b->vx.p, b->dvx.p},
                                                    {b->vy.p, b->dvy.p}};
     for (auto& c : copies)
       if (e == hipSuccess && bn) e = hipMemcpyAsync(c.p, c.d, bn, hipMemcpyDeviceToHost, s);
+    std::vector<uint32_t> states(b->graphs);
+    if (e == hipSuccess && b->graphs)
+      e = hipMemcpyAsync(states.data(), b->dlcg.p, (size_t)b->graphs * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (uint32_t g = 0; e == hipSuccess && g < b->graphs; ++g) b->seed_ids.p[g] = states[g];
   } else {
     return B_BAD_ARGUMENT;
   }

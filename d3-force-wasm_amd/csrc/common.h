@@ -237,6 +237,55 @@ hipError_t mb_traverse(const TravArgs& a, int mode, hipStream_t stream);
 // truthy leaf through truthy cells, and any reached leaf makes v NaN.
 hipError_t mb_nan_nodes(const TreeBuffers& t, uint32_t n, double* vx, double* vy, hipStream_t stream);
 
+// ---- batch_exact.hip ----
+// One small graph of a batch (fewer nodes than the fast-mode crossover),
+// ticked with the exact kernels by one workgroup: its node and link ranges
+// in the arenas, its cell range in the small-graph cell arrays (cell_cap
+// cells) and its graph index (random state). The link schedule: the graph's
+// m links in level order at rec0 of the record array, level L's links at
+// [levels[lev0 + L], levels[lev0 + L + 1]) of that range, nlev levels (the
+// levels array holds m + 1 entries per graph from lev0); links of one level
+// share no node, and every link's level is above those of the earlier links
+// at its ends, so a level at a time gives the bits of the serial order.
+struct SmallJob {
+  uint32_t node0, n, link0, m, cell0, cell_cap, graph, rec0, lev0, nlev, pad[2];
+};
+static_assert(sizeof(SmallJob) == 48, "small-graph job");
+// One link of a small graph in the level schedule: its ends (local node
+// indices) and parameters.
+struct SmallLink {
+  uint16_t a, b;
+  uint32_t pad;
+  double distance, strength, bias;
+};
+static_assert(sizeof(SmallLink) == 32, "small-graph link record");
+struct SmallExactArgs {
+  const double *x, *y, *vx, *vy, *charge;
+  const uint32_t *src, *tgt;
+  const double *lstrength, *ldistance, *lbias;
+  const SmallLink* lrec;     // the link schedule (small_schedule)
+  const uint32_t* levels;
+  const double* alpha_dev;   // this tick's alpha
+  double theta2, dmin2, dmax2, link_iterations;
+  uint32_t* lcg;             // per graph: d3's lcg state (lcg.js)
+  void* cells;               // small_cell_bytes() x cells
+  TCell* tcells;
+  double *vout_x, *vout_y;   // the small graphs' velocities after manyBody and link
+  uint32_t* error;           // set nonzero when a graph's tree outgrows the kernel
+};
+size_t small_cell_bytes();
+// manyBody + link, exact, for every job; reads x, y, vx, vy, writes vout.
+hipError_t small_exact(const SmallJob* jobs, uint32_t count, const SmallExactArgs& a, hipStream_t stream);
+// The link schedule of every job from the link arrays (src, tgt global node
+// indices): the levels, then the records in level order; writes each job's
+// nlev. order: scratch, one u32 per scheduled link.
+hipError_t small_schedule(SmallJob* jobs, uint32_t count, const uint32_t* src, const uint32_t* tgt,
+                          const double* lstrength, const double* ldistance, const double* lbias, uint32_t* order,
+                          uint32_t* levels, SmallLink* lrec, hipStream_t stream);
+// vx, vy of every job's nodes from vout.
+hipError_t small_scatter(const SmallJob* jobs, uint32_t count, const double* vout_x, const double* vout_y, double* vx,
+                         double* vy, hipStream_t stream);
+
 // ---- link.hip ----
 // A node's position and velocity side by side, so an edge reads each endpoint
 // from one 32-byte record (the link pass's working copy).

@@ -18,7 +18,8 @@ from .core import check, gpu_core
 B_X, B_Y, B_VX, B_VY, B_CHARGE, B_SOURCE, B_TARGET, B_LINK_STRENGTH, B_LINK_DISTANCE, B_LINK_BIAS, B_SEED = range(11)
 (BS_ALPHA, BS_ALPHA_DECAY, BS_ALPHA_TARGET, BS_VELOCITY_DECAY, BS_DISTANCE_MIN2, BS_DISTANCE_MAX2, BS_ORDER,
  BS_OPENING, BS_CENTER_X, BS_CENTER_Y, BS_CENTER_STRENGTH, BS_LINK_ITERATIONS, BS_TICK, BS_GRAPH,
- BS_GRAPH_CELLS, BS_GRAPH_REPLAYS, BS_GRAPH_REDOS) = range(17)
+ BS_GRAPH_CELLS, BS_GRAPH_REPLAYS, BS_GRAPH_REDOS, BS_THETA2) = range(18)
+MIN_N = 2000  # fmm.zig:43: graphs below this many nodes run the exact kernels
 
 _SIGNATURES = {
     "batch_create": (ctypes.c_uint32, []),
@@ -63,10 +64,10 @@ def link_defaults(n, src, tgt):
 class Batch:
     """graphs: a list of (n, source, target) with graph-local node indices.
     positions: optional list of (x, y) per graph (d3's phyllotaxis start
-    otherwise). seed_ids: optional jiggle seed id per graph (default its
-    index)."""
+    otherwise). random_states: optional starting state of each graph's random
+    source, d3's lcg (default 1, as every d3 simulation starts)."""
 
-    def __init__(self, graphs, core=None, charge=-30.0, distance=30.0, positions=None, seed_ids=None):
+    def __init__(self, graphs, core=None, charge=-30.0, distance=30.0, positions=None, random_states=None):
         self.core = _bind(core or gpu_core())
         c = self.core
         self.h = c.batch_create()
@@ -100,11 +101,18 @@ class Batch:
         vy[:] = 0
         q[:] = charge
         ld[:] = distance
-        if seed_ids is not None:
-            seeds = np.ctypeslib.as_array(ctypes.cast(c.batch_ptr(self.h, B_SEED), ctypes.POINTER(ctypes.c_uint64)),
-                                          shape=(self.graphs,))
-            seeds[:] = seed_ids
+        if random_states is not None:
+            self._states()[:] = random_states
         self.sync_in()
+
+    def _states(self):
+        return np.ctypeslib.as_array(ctypes.cast(self.core.batch_ptr(self.h, B_SEED), ctypes.POINTER(ctypes.c_uint64)),
+                                     shape=(self.graphs,))
+
+    def random_states(self):
+        """Each graph's random state (d3's lcg) after the ticks so far."""
+        self.sync_out()
+        return self._states().copy()
 
     def _f64(self, which, count):
         p = self.core.batch_ptr(self.h, which)

@@ -165,18 +165,22 @@ int32_t core_fail_next(int32_t status, uint32_t after, uint32_t count);
 
 /* ---- Batched fast mode (no reference counterpart) ----
  * Many independent simulations ("graphs") in one node arena and one link
- * arena, ticked together on one GPU: per graph, a tick is the facade's fast
- * tick with forceManyBody (FMM), forceLink (adjacency sweep) and forceCenter
- * in that order, then integrate (simulation.js:49-89, root.zig:211-254 for one
+ * arena, ticked together on one GPU: per graph, a tick is the facade's tick
+ * in fast precision with forceManyBody, forceLink and forceCenter in that
+ * order, then integrate (simulation.js:49-89, root.zig:211-254 for one
  * simulation). Graph g owns nodes [node_off[g], node_off[g+1]) and links
  * [link_off[g], link_off[g+1]); link endpoints are arena node indices inside
- * their graph (checked at batch_sync). Jiggle seeds are mixed per graph and
- * tick from the graph's seed id (D3F_B_SEED, default g) and hash indices
- * within the graph, so a graph's result does not depend on the rest of the
- * batch. Every graph runs the FMM whatever its size (the reference's facade
- * runs the exact kernels below 2000 nodes); a graph whose extent is not
- * finite makes batch_tick return BAD_ARGUMENT. Status codes are the
- * simulation's. */
+ * their graph (checked at batch_sync). As in the reference (root.zig:246-248),
+ * a graph of at least 2000 nodes runs fast mode (FMM, adjacency-sweep link)
+ * and a smaller one the exact kernels (Barnes-Hut with D3F_BS_THETA2, the
+ * Gauss-Seidel link), bit-identical to the reference's exact tick. Each graph
+ * has its own random source, d3's lcg (lcg.js) with state D3F_B_SEED
+ * (1 at the start, as every d3 simulation): per tick the manyBody and link
+ * seeds are drawn from it, and a small graph's jiggle draws, in the facade's
+ * order (core.js:213-231). A graph whose extent is not finite makes
+ * batch_tick return BAD_ARGUMENT; a graph below 2000 nodes whose quadtree is
+ * deeper than about 60 levels makes it return WORKER_FAILED. Status codes are
+ * the simulation's. */
 enum { D3F_B_X = 0, D3F_B_Y = 1, D3F_B_VX = 2, D3F_B_VY = 3, D3F_B_CHARGE = 4, D3F_B_SOURCE = 5, D3F_B_TARGET = 6,
        D3F_B_LINK_STRENGTH = 7, D3F_B_LINK_DISTANCE = 8, D3F_B_LINK_BIAS = 9, D3F_B_SEED = 10 };
 /* Batch scalars; VELOCITY_DECAY is the multiplier (1 - d3's velocityDecay,
@@ -193,13 +197,15 @@ enum { D3F_BS_ALPHA = 0, D3F_BS_ALPHA_DECAY = 1, D3F_BS_ALPHA_TARGET = 2, D3F_BS
        D3F_BS_DISTANCE_MIN2 = 4, D3F_BS_DISTANCE_MAX2 = 5, D3F_BS_ORDER = 6, D3F_BS_OPENING = 7, D3F_BS_CENTER_X = 8,
        D3F_BS_CENTER_Y = 9, D3F_BS_CENTER_STRENGTH = 10, D3F_BS_LINK_ITERATIONS = 11, D3F_BS_TICK = 12,
        D3F_BS_GRAPH = 13, D3F_BS_GRAPH_CELLS = 14, D3F_BS_GRAPH_REPLAYS = 15, D3F_BS_GRAPH_REDOS = 16,
-       D3F_BS_COUNT = 17 };
+       D3F_BS_THETA2 = 17, D3F_BS_COUNT = 18 };
 uint32_t batch_create(void);
 int32_t batch_destroy(uint32_t b);
 /* node_off, link_off: graphs + 1 host entries each, starting at 0. */
 int32_t batch_set_graphs(uint32_t b, uint32_t graphs, const uint32_t* node_off, const uint32_t* link_off);
 /* Pinned host array D3F_B_*: f64 per node (X..CHARGE), u32 per link
- * (SOURCE, TARGET), f64 per link (LINK_*), u64 per graph (SEED). */
+ * (SOURCE, TARGET), f64 per link (LINK_*), u64 per graph (SEED: the random
+ * state, low 32 bits used; batch_sync D2H brings the current states back,
+ * H2D sets them). */
 void* batch_ptr(uint32_t b, uint32_t which);
 int32_t batch_sync(uint32_t b, uint32_t direction);
 int32_t batch_set_scalar(uint32_t b, uint32_t which, double v);
