@@ -107,7 +107,7 @@ struct Batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool graph_valid = false;
-  Device<double> snap;         // x, y, vx, vy before a replayed tick
+  Device<double> snap;         // x, y, vx, vy from before the tick (replays; eager ticks with small graphs)
   uint64_t graph_replays = 0, graph_redos = 0;
   // link rows
   bool csr_valid = false;
@@ -120,6 +120,8 @@ struct Batch {
   TreeBuffers tree;
   FmmBuffers fmm;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // the small graphs' kernel, beside the FMM
+  hipEvent_t fork = nullptr, join = nullptr;
   double scalars[D3F_BS_COUNT] = {0};
   d3f_stats stats{};
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -155,6 +157,9 @@ struct Batch {
       (void)hipEventDestroy(p.second.first);
       (void)hipEventDestroy(p.second.second);
     }
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -168,6 +173,9 @@ struct Batch {
 
   hipError_t init() {
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
     HIPCHECK(hipMalloc(&dstate, sizeof(BatchState)));
     HIPCHECK(hipMalloc(&derror, sizeof(uint32_t)));
     HIPCHECK(hipHostMalloc((void**)&herror, sizeof(uint32_t), hipHostMallocDefault));
@@ -306,27 +314,36 @@ struct Batch {
     return hipSuccess;
   }
 
-  void timer(int phase, hipEvent_t a) {
+  void timer(int phase, hipEvent_t a, hipStream_t on = nullptr) {
     hipEvent_t b = nullptr;
     if (!a || hipEventCreate(&b) != hipSuccess) return;
-    (void)hipEventRecord(b, stream);
+    (void)hipEventRecord(b, on ? on : stream);
     pending.push_back({phase, {a, b}});
   }
-  hipEvent_t mark() {
+  hipEvent_t mark(hipStream_t on = nullptr) {
     hipEvent_t a = nullptr;
     if (hipEventCreate(&a) != hipSuccess) return nullptr;
-    (void)hipEventRecord(a, stream);
+    (void)hipEventRecord(a, on ? on : stream);
     return a;
   }
 
-  // A small graph's tree outgrew the exact kernel (it wrote only vout):
-  // the random states go back to their copy from before the tick, the
-  // host's alpha and tick count are uploaded again, the flag is cleared.
-  hipError_t small_overflow() {
+  // x, y, vx, vy and the random states back to their copies from before the
+  // tick; the host's alpha and tick count are uploaded again.
+  hipError_t restore() {
+    const size_t bn = (size_t)n * sizeof(double);
+    double* to[4] = {dx.p, dy.p, dvx.p, dvy.p};
+    for (int k = 0; k < 4; ++k) HIPCHECK(hipMemcpyAsync(to[k], snap.p + (size_t)k * n, bn, hipMemcpyDeviceToDevice, stream));
     HIPCHECK(hipMemcpyAsync(dlcg.p, snap_lcg.p, (size_t)graphs * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+    const BatchState hs{scalars[D3F_BS_ALPHA], (unsigned long long)scalars[D3F_BS_TICK], 0ull};
+    HIPCHECK(hipMemcpyAsync(dstate, &hs, sizeof(hs), hipMemcpyHostToDevice, stream));
+    return hipStreamSynchronize(stream);
+  }
+
+  // A small graph's tree outgrew the exact kernel: after restore(), the flag
+  // is cleared and the tick fails with its cause.
+  hipError_t small_overflow() {
     HIPCHECK(hipMemsetAsync(derror, 0, sizeof(uint32_t), stream));
     HIPCHECK(hipStreamSynchronize(stream));
-    state_dirty = true;
     note_failure(hipErrorNotSupported,
                  "batch_tick: a graph below 2000 nodes needs a quadtree deeper than the batched exact kernel "
                  "holds (about 60 levels) or more than 3 n + 128 cells; the tick did not run",
@@ -334,22 +351,32 @@ struct Batch {
     return hipErrorNotSupported;
   }
 
+  // Copies of x, y, vx, vy and the random states from before the tick.
+  hipError_t snapshot() {
+    const size_t bn = (size_t)n * sizeof(double);
+    const double* from[4] = {dx.p, dy.p, dvx.p, dvy.p};
+    for (int k = 0; k < 4; ++k)
+      HIPCHECK(hipMemcpyAsync(snap.p + (size_t)k * n, from[k], bn, hipMemcpyDeviceToDevice, stream));
+    return hipMemcpyAsync(snap_lcg.p, dlcg.p, (size_t)graphs * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
+  }
+
   // The device part of a tick. graph: the capturable form (no host
-  // synchronisation; nothing the host reads back). Eagerly, the small
-  // graphs' kernel runs first and its overflow flag is read before anything
-  // writes the node arrays, so a tick that fails leaves them as they were.
+  // synchronisation; nothing the host reads back). The small graphs' exact
+  // kernel runs on the side stream while the FMM (which leaves their nodes
+  // alone) runs on the main one; the link sweep waits for both.
   hipError_t device_tick(bool graph) {
     hipError_t e;
-    if (!graph)
-      HIPCHECK(hipMemcpyAsync(snap_lcg.p, dlcg.p, (size_t)graphs * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
     hipLaunchKernelGGL(k_batch_step, dim3(1), dim3(64), 0, stream, dstate, scalars[D3F_BS_ALPHA_TARGET],
                        scalars[D3F_BS_ALPHA_DECAY]);
     hipLaunchKernelGGL(k_batch_seeds, dim3(ceil_div(graphs, 256u)), dim3(256), 0, stream, dnode_off.p, graphs, dlcg.p,
                        dseeds_mb.p, dseeds_link.p);
     const bool timed = g_batch_timing && !graph;
-    hipEvent_t ev = timed ? mark() : nullptr;
+    hipEvent_t ev = nullptr;
     // the small graphs' exact manyBody and link, from the tick's starting state
     if (!small.empty()) {
+      HIPCHECK(hipEventRecord(fork, stream));
+      HIPCHECK(hipStreamWaitEvent(side, fork, 0));
+      ev = timed ? mark(side) : nullptr;
       SmallExactArgs sa;
       sa.x = dx.p;
       sa.y = dy.p;
@@ -374,23 +401,17 @@ struct Batch {
       sa.vout_x = vout_x.p;
       sa.vout_y = vout_y.p;
       sa.error = derror;
-      HIPCHECK(small_exact(dsmall.p, (uint32_t)small.size(), sa, stream));
-      if (!graph) {
-        HIPCHECK(hipMemcpyAsync(herror, derror, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-        HIPCHECK(hipStreamSynchronize(stream));
-        if (*herror) {
-          if (ev) (void)hipEventDestroy(ev);
-          return small_overflow();
-        }
-      }
+      HIPCHECK(small_exact(dsmall.p, (uint32_t)small.size(), sa, side));
+      timer(D3F_PHASE_TRAVERSAL, ev, side);
+      HIPCHECK(hipEventRecord(join, side));
     }
-    timer(D3F_PHASE_TRAVERSAL, ev);
     // many-body
     FmmArgs a = fmm_args();
     a.graph = graph;
     ev = timed ? mark() : nullptr;
     if ((e = fmm_run(fmm, tree, a, stream)) != hipSuccess) return e;
     timer(D3F_PHASE_FMM, ev);
+    if (!small.empty()) HIPCHECK(hipStreamWaitEvent(stream, join, 0));
     // link
     ev = timed ? mark() : nullptr;
     for (double k = 0; k < scalars[D3F_BS_LINK_ITERATIONS]; k += 1) {
@@ -435,6 +456,7 @@ struct Batch {
     a.seeds_dev = dseeds_mb.p;
     const double limit = scalars[D3F_BS_GRAPH_CELLS];
     a.cell_limit = limit >= 1 && limit < 4294967296.0 ? (uint32_t)limit : 0;
+    a.min_graph_n = BATCH_MIN_N;  // the small graphs run the exact kernels
     return a;
   }
 
@@ -442,18 +464,12 @@ struct Batch {
   // tick): the state snapshot, then the device part of a tick.
   hipError_t capture() {
     drop_graph();
-    const size_t bn = (size_t)n * sizeof(double);
     HIPCHECK(snap.reserve(4 * (size_t)(n > 0 ? n : 1), 0, stream));
     const uint32_t cells = 2 * fmm.params_host->level_off[FMM_LEVELS + 1] + 4096;
     HIPCHECK(fmm_reserve_capacity(fmm, tree, fmm_args(), cells, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = hipSuccess;
-    const double* from[4] = {dx.p, dy.p, dvx.p, dvy.p};
-    for (int k = 0; k < 4 && e == hipSuccess; ++k)
-      e = hipMemcpyAsync(snap.p + (size_t)k * n, from[k], bn, hipMemcpyDeviceToDevice, stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(snap_lcg.p, dlcg.p, (size_t)graphs * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
+    hipError_t e = snapshot();
     if (e == hipSuccess) e = device_tick(true);
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(stream, &g);
@@ -496,24 +512,26 @@ struct Batch {
       } else {  // sizes outgrew the captured buffers, or a small graph its kernel: undo the replay
         const bool failed = *herror != 0;
         ++graph_redos;
-        const size_t bn = (size_t)n * sizeof(double);
-        double* to[4] = {dx.p, dy.p, dvx.p, dvy.p};
-        for (int k = 0; k < 4; ++k)
-          if ((e = hipMemcpyAsync(to[k], snap.p + (size_t)k * n, bn, hipMemcpyDeviceToDevice, stream)) != hipSuccess)
-            return bstatus(e);
-        if ((e = hipMemcpyAsync(dlcg.p, snap_lcg.p, (size_t)graphs * sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                                stream)) != hipSuccess)
-          return bstatus(e);
-        const BatchState hs{scalars[D3F_BS_ALPHA], (unsigned long long)scalars[D3F_BS_TICK], 0ull};
-        if ((e = hipMemcpyAsync(dstate, &hs, sizeof(hs), hipMemcpyHostToDevice, stream)) != hipSuccess ||
-            (e = hipStreamSynchronize(stream)) != hipSuccess)
-          return bstatus(e);
+        if ((e = restore()) != hipSuccess) return bstatus(e);
         drop_graph();
         if (failed) return bstatus(small_overflow());  // the eager tick would fail the same way
       }
     }
     if (!done) {
+      // with small graphs, the copy that a failed tick goes back to
+      if (!small.empty() && ((e = snap.reserve(4 * (size_t)(n > 0 ? n : 1), 0, stream)) != hipSuccess ||
+                             (e = snapshot()) != hipSuccess))
+        return bstatus(e);
       if ((e = device_tick(false)) != hipSuccess) return bstatus(e);
+      if (!small.empty()) {
+        if ((e = hipMemcpyAsync(herror, derror, sizeof(uint32_t), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(stream)) != hipSuccess)
+          return bstatus(e);
+        if (*herror) {
+          if ((e = restore()) != hipSuccess) return bstatus(e);
+          return bstatus(small_overflow());
+        }
+      }
       if (want_graph && !graph_valid && !fmm.params_host->declined && (e = capture()) != hipSuccess)
         return bstatus(e);
     }

@@ -476,6 +476,9 @@ struct FmmArgs {
   // Capturable form only: when nonzero, the size guard treats this many cells
   // as the capacity (a tick with more cells aborts and is redone eagerly).
   uint32_t cell_limit = 0;
+  // Graphs with fewer nodes than this get no tree: their nodes are left as
+  // they are (the batch ticks them with the exact kernels).
+  uint32_t min_graph_n = 0;
 };
 // Fast many-body (fmm.zig:247-354) over every graph of the forest at once.
 // With one graph, returns hipErrorNotSupported when the FMM declines (a

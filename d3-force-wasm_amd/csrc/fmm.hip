@@ -146,15 +146,17 @@ __global__ __launch_bounds__(EWF) void k_forest_box(const double* __restrict__ x
 
 // One thread per graph: its box, the fast extent (tree.zig:357-359), the unit
 // frame (fmm.zig:253-270) and the scaled distance limits (fmm.zig:272-281).
+// A graph of fewer than min_n nodes is given no point (no tree).
 __global__ __launch_bounds__(64) void k_forest_graphs(const BoxPart* __restrict__ parts,
                                                       const uint32_t* __restrict__ chunk_first, uint32_t graphs,
                                                       double dmin2, double dmax2, const uint64_t* __restrict__ seeds,
-                                                      const uint32_t* __restrict__ node_off,
+                                                      const uint32_t* __restrict__ node_off, uint32_t min_n,
                                                       FmmGraph* __restrict__ gp, FmmParams* __restrict__ fp) {
   const uint32_t g = blockIdx.x * 64 + threadIdx.x;
   if (g >= graphs) return;
   BoxPart b = bempty();
-  for (uint32_t c = chunk_first[g]; c < chunk_first[g + 1]; ++c) bcombine(b, parts[c]);
+  if (node_off[g + 1] - node_off[g] >= min_n)
+    for (uint32_t c = chunk_first[g]; c < chunk_first[g + 1]; ++c) bcombine(b, parts[c]);
   FmmGraph r;
   memset(&r, 0, sizeof(r));
   const double nan = __builtin_nan(""), inf = __builtin_huge_val();
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(EWF) void k_forest_keys(const double* __restrict__ 
   if (i >= n) return;
   const uint32_t g = node_graph ? node_graph[i] : 0;
   const double x = xs[i], y = ys[i];
-  const bool valid = !(x != x || y != y);
+  const bool valid = !(x != x || y != y) && gp[g].m != 0;  // a graph without points: NaN or left out
   uint64_t key = 0;
   if (valid) {
     const FmmGraph& G = gp[g];
@@ -1662,7 +1664,7 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
                        (BoxPart*)f.box_parts);
   hipLaunchKernelGGL(k_forest_graphs, dim3(ceil_div(a.graphs, 64)), dim3(64), 0, stream, (const BoxPart*)f.box_parts,
                      f.chunk_first, a.graphs, a.dmin2, a.dmax2, a.seeds_dev ? a.seeds_dev : f.seeds, f.node_off_dev,
-                     f.graphs, f.params);
+                     a.min_graph_n, f.graphs, f.params);
   hipLaunchKernelGGL(k_forest_keys, dim3(nb), dim3(EWF), 0, stream, a.x, a.y, n, a.node_graph, f.graphs, f.params,
                      t.keys, t.perm, t.keys_all, t.flags);
   HIPCHECK(scan_u32(t.flags, t.offs, n > 0 ? n : 1, &f.params->n, t.partials, nullptr, stream));
