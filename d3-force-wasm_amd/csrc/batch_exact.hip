@@ -155,6 +155,71 @@ __device__ bool walk(uint32_t i, double nx, double ny, double& vx, double& vy, c
   return true;
 }
 
+// A walk of the first pass, several in flight per lane so that their cell
+// loads overlap: a walk that meets a point where d3 would call random stops
+// there (drew) and is redone with the draws.
+constexpr int SX_WALKS = 2;
+struct SWalk {
+  double nx, ny, vx, vy;
+  uint32_t i, c;
+  bool drew;
+};
+
+__device__ inline double f64_at(uint32_t lo, uint32_t hi) {
+  return __builtin_bit_cast(double, (uint64_t)hi << 32 | lo);
+}
+
+// One visit of the first pass (walk without draws) at cell w.c, whose first 48
+// bytes are g0 g1 g2; moves w.c on.
+__device__ inline void step(SWalk& w, const uint4& g0, const uint4& g1, const uint4& g2, uint32_t len,
+                            const uint16_t* __restrict__ perm, uint32_t node0, const double* __restrict__ s,
+                            double dmin2, double dmax2, double alpha) {
+  const double value = f64_at(g1.x, g1.y);
+  const uint32_t skip = g2.x;
+  uint32_t next = w.c + (skip > 0 ? skip : 1);
+  if (truthy(value)) {
+    const double dx = f64_at(g0.x, g0.y) - w.nx, dy = f64_at(g0.z, g0.w) - w.ny;
+    double l = dx * dx + dy * dy;
+    if (f64_at(g1.z, g1.w) < l) {
+      if (l < dmax2) {
+        if (dx == 0 || dy == 0) {
+          w.drew = true;
+          next = len;
+        } else {
+          if (l < dmin2) l = __builtin_sqrt(dmin2 * l);
+          w.vx += dx * value * alpha / l;
+          w.vy += dy * value * alpha / l;
+        }
+      }
+    } else if (skip > 1) {
+      next = w.c + 1;
+    } else if (!(l >= dmax2)) {
+      const uint32_t first = g2.y, cnt = g2.z, head = g2.w;
+      bool go = true;
+      if (head != w.i || cnt > 1) {
+        if (dx == 0 || dy == 0) {
+          w.drew = true;
+          next = len;
+          go = false;
+        } else if (l < dmin2) {
+          l = __builtin_sqrt(dmin2 * l);
+        }
+      }
+      if (go)
+        for (uint32_t k = first + cnt; k > first;) {
+          --k;
+          const uint32_t j = node0 + perm[k];
+          if (j != w.i) {
+            const double wj = s[j] * alpha / l;
+            w.vx += dx * wj;
+            w.vy += dy * wj;
+          }
+        }
+    }
+  }
+  w.c = next;
+}
+
 __global__ __launch_bounds__(SX_THREADS) void k_small_exact(
     const SmallJob* __restrict__ jobs, const double* __restrict__ xs, const double* __restrict__ ys,
     const double* __restrict__ vxs, const double* __restrict__ vys, const double* __restrict__ charge,
@@ -685,17 +750,52 @@ __global__ __launch_bounds__(SX_THREADS) void k_small_exact(
     __syncthreads();
   }
 
-  // 5. the walks: nodes in Morton order, one per lane; a walk that would draw stops
+  // 5. the walks: nodes in Morton order, SX_WALKS per lane at a time; a walk
+  // that would draw stops and its node is flagged
   for (uint32_t k = tid; k < SX_MAX / 32; k += SX_THREADS) flagged[k] = 0;
   __syncthreads();
-  for (uint32_t k = tid; k < m; k += SX_THREADS) {
-    const uint32_t local = perm[k], i = node0 + local;
-    double vx = vxs[i], vy = vys[i];
-    if (walk<0>(i, xs[i], ys[i], vx, vy, tc, len, perm, node0, charge, dmin2, dmax2, alpha, nullptr)) {
-      vout_x[i] = vx;
-      vout_y[i] = vy;
-    } else {
-      atomicOr(&flagged[local >> 5], 1u << (local & 31));
+  for (uint32_t k0 = tid; k0 < m; k0 += SX_THREADS * SX_WALKS) {
+    SWalk w[SX_WALKS];
+#pragma unroll
+    for (int q = 0; q < SX_WALKS; ++q) {
+      const uint32_t slot = k0 + q * SX_THREADS;
+      const bool active = slot < m;
+      SWalk& a = w[q];
+      a.i = node0 + (active ? perm[slot] : 0u);
+      a.nx = active ? xs[a.i] : 0;
+      a.ny = active ? ys[a.i] : 0;
+      a.vx = active ? vxs[a.i] : 0;
+      a.vy = active ? vys[a.i] : 0;
+      a.c = active ? 0 : len;
+      a.drew = false;
+    }
+    for (;;) {
+      bool any = false;
+#pragma unroll
+      for (int q = 0; q < SX_WALKS; ++q) any = any || w[q].c < len;
+      if (!any) break;
+      uint4 g[SX_WALKS][3];
+#pragma unroll
+      for (int q = 0; q < SX_WALKS; ++q) {
+        const uint4* r = reinterpret_cast<const uint4*>(tc + (w[q].c < len ? w[q].c : 0));
+        g[q][0] = r[0];
+        g[q][1] = r[1];
+        g[q][2] = r[2];
+      }
+#pragma unroll
+      for (int q = 0; q < SX_WALKS; ++q)
+        if (w[q].c < len) step(w[q], g[q][0], g[q][1], g[q][2], len, perm, node0, charge, dmin2, dmax2, alpha);
+    }
+#pragma unroll
+    for (int q = 0; q < SX_WALKS; ++q) {
+      if (k0 + q * SX_THREADS >= m) continue;
+      const uint32_t local = w[q].i - node0;
+      if (!w[q].drew) {
+        vout_x[w[q].i] = w[q].vx;
+        vout_y[w[q].i] = w[q].vy;
+      } else {
+        atomicOr(&flagged[local >> 5], 1u << (local & 31));
+      }
     }
   }
   // nodes with a NaN coordinate reach every truthy leaf (v becomes NaN when

@@ -441,6 +441,7 @@ struct FmmBuffers {
   // depths the eager cell build runs (the last build's deepest level + 2;
   // a deeper tree is rebuilt with every level)
   uint32_t build_levels = FMM_LEVELS;
+  uint32_t graph_levels = FMM_LEVELS;  // depths of the last build (the capturable form's bound)
   uint32_t* long_list = nullptr;           // targets whose segments the long-segment sort orders
   uint4* near_items = nullptr;             // (leaf, target batch, chunk or ALL, slot)
   uint4* near_combines = nullptr;          // (leaf, target batch, first slot, chunks)

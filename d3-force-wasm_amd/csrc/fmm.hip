@@ -1282,8 +1282,11 @@ __global__ void k_fmm_guard(FmmParams* __restrict__ fp, uint32_t stage, uint32_t
                             uint32_t last_step) {
   if (threadIdx.x != 0) return;
   const uint32_t nc = fp->level_off[FMM_LEVELS + 1];
-  if (stage == 0) {
-    fp->abort = (fp->cell_overflow || nc > cell_room || fp->roots > front_cap) ? 1u : 0u;
+  if (stage == 0) {  // last_step: the depths built (cells at that depth would need more)
+    fp->abort = (fp->cell_overflow || nc > cell_room || fp->roots > front_cap ||
+                 (last_step < FMM_LEVELS && fp->level_n[last_step] != 0))
+                    ? 1u
+                    : 0u;
   } else if (fp->list_overflow || fp->front[last_step] != 0) {
     fp->abort |= 2u;
   }
@@ -1422,6 +1425,7 @@ hipError_t reserve_cells(FmmBuffers& f, uint32_t nc, uint32_t q) {
 // Grids of the capturable form (grid-stride kernels over device counts).
 constexpr uint32_t CAP_CELL_BLOCKS = 4096;  // per-cell and per-pair elementwise kernels
 constexpr uint32_t CAP_TARGET_BLOCKS = 65536;  // one wave per target, looping (many, for load balance)
+constexpr uint32_t CAP_NEAR_BLOCKS = 1u << 22;   // the near field's blocks (one a cell, up to this)
 
 template <int Q>
 hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t stream) {
@@ -1431,7 +1435,8 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   // build; the capturable form from the capacities, every kernel reading its
   // count on the device (both run the same kernels on the same data).
   const uint32_t nc = g ? f.cell_cap : P.level_off[FMM_LEVELS + 1];
-  uint32_t maxd = FMM_LEVELS;
+  // the capturable form: every cell is above depth graph_levels (k_fmm_guard)
+  uint32_t maxd = f.graph_levels > 0 ? f.graph_levels - 1 : 0;
   if (!g) {
     maxd = 0;
     for (uint32_t d = 0; d <= FMM_LEVELS; ++d)
@@ -1457,7 +1462,7 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   mac.mac_far = 2 * kappa - 1;
   mac.nbits = nbits;
   mac.graphs = f.graphs;
-  const uint32_t steps = g ? FMM_MAX_STEPS : 2 * maxd + 1;
+  const uint32_t steps = 2 * maxd + 1;  // a traversal not drained by then aborts the replay (k_fmm_guard)
   const uint32_t root_blocks = ceil_div(a.graphs > 0 ? a.graphs : 1, EWF);  // one root per graph at most
   for (;;) {
     hipLaunchKernelGGL(k_fmm_lists_reset, dim3(root_blocks), dim3(EWF), 0, stream, f.params, f.front_a, f.front_cap);
@@ -1555,7 +1560,9 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   hipLaunchKernelGGL(k_near_items, dim3(grid(nc, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.cells, ncp,
                      f.p2p_cnt, f.params, f.near_items, f.near_combines, f.cursor, f.slot_cap);
   const uint32_t extra = g ? f.near_extra : (P.m_total / 64 < f.near_extra ? P.m_total / 64 + 1 : f.near_extra);
-  hipLaunchKernelGGL(k_near<Q>, dim3(extra + grid(nc, 1, CAP_TARGET_BLOCKS)), dim3(64), 0, stream, f.cells,
+  // one block a cell, as eagerly, up to a bound (then the blocks loop)
+  const uint32_t near_blocks = g ? std::min(f.cell_cap, CAP_NEAR_BLOCKS) : grid(nc, 1, 0);
+  hipLaunchKernelGGL(k_near<Q>, dim3(extra + near_blocks), dim3(64), 0, stream, f.cells,
                      f.p2p_src, f.p2p_off, f.p2p_cnt, f.params, (const float2*)f.lexp, f.px, f.py, f.ps, f.ux, f.uy,
                      t.perm, a.vx, a.vy, f.graphs, dmin2, dmax2, f.near_items, f.cursor, f.near_partial, extra);
   hipLaunchKernelGGL(k_near_combine<Q>, dim3(extra), dim3(64), 0, stream, f.cells, f.params, f.near_combines,
@@ -1677,7 +1684,8 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
   // Cells, one depth at a time; grown and rebuilt if they outgrow the buffer.
   // The eager form builds the depths the last tree needed plus two and
   // rebuilds with every depth when cells reach the last one.
-  uint32_t d_end = a.graph ? FMM_LEVELS : std::min(FMM_LEVELS, f.build_levels);
+  uint32_t d_end = std::min(FMM_LEVELS, f.build_levels);
+  f.graph_levels = d_end;  // the capturable form builds these depths; a deeper tree aborts the replay
   for (;;) {
     hipLaunchKernelGGL(k_forest_roots, dim3(1), dim3(1024), 0, stream, f.graphs, a.graphs, f.params, f.cells,
                        f.cell_cap, t.meta);
@@ -1689,8 +1697,8 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
       hipLaunchKernelGGL(k_fmm_children, dim3(f.level_blocks), dim3(EWF), 0, stream, f.cells, f.params, d, f.split,
                          f.split_off, f.bounds, f.cell_cap);
     }
-    if (a.graph) break;  // sizes checked on the device (k_fmm_guard)
     if (d_end < FMM_LEVELS) hipLaunchKernelGGL(k_fmm_levels_close, dim3(1), dim3(64), 0, stream, f.params, d_end);
+    if (a.graph) break;  // sizes checked on the device (k_fmm_guard)
     HIPCHECK(hipMemcpyAsync(f.params_host, f.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     if (!f.params_host->cell_overflow && d_end < FMM_LEVELS && f.params_host->level_n[d_end] != 0) {
@@ -1716,7 +1724,8 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
   }
   uint32_t room = std::min<uint32_t>(std::min(f.cell_cap, f.cellarr_cap), (uint32_t)(f.exp_cap / (p + 1)));
   if (a.graph && a.cell_limit && a.cell_limit < room) room = a.cell_limit;
-  hipLaunchKernelGGL(k_fmm_guard, dim3(1), dim3(64), 0, stream, f.params, 0u, room, f.front_cap, 0u);
+  hipLaunchKernelGGL(k_fmm_guard, dim3(1), dim3(64), 0, stream, f.params, 0u, room, f.front_cap,
+                     a.graph ? d_end : FMM_LEVELS);
   hipLaunchKernelGGL(k_fmm_gather, dim3(nb), dim3(EWF), 0, stream, t.perm, t.keys, a.x, a.y, a.strength, f.graphs,
                      f.params, a.alpha, a.alpha_dev, n, f.px, f.py, f.ux, f.uy, f.ps);
   return Orders::fns[p - 1](f, t, a, stream);
