@@ -4,12 +4,16 @@ start at alpha = 1 (the reference always starts there, simulation.js:91-105).
 
 The network (csrc/gnn.hip) has 8 input features per node, 4 message-passing
 layers of width 64 and a 2-D position head; it runs and trains on the GPU with
-BF16 matrix-core kernels. Inputs:
-  - log(1 + degree) and 1 / sqrt(1 + degree);
-  - 6 random-walk features: Ritz vectors of the lazy random walk
-    (I + D^-1 A) / 2 after subspace iteration from seeded Gaussian vectors
-    (the graph's smoothest non-constant modes), each with unit variance and
-    its sign fixed by its third moment.
+BF16 matrix-core kernels. Inputs, computed on the GPU from the links alone
+(gnn_set_links):
+  - log(1 + degree) and 1 / sqrt(1 + degree) (degree: distinct neighbours);
+  - 6 random-walk features: the smoothest non-constant modes of the lazy
+    random walk (I + D^-1 A) / 2, from 40 steps of subspace iteration on the
+    similar symmetric walk from seeded Gaussian vectors and Rayleigh-Ritz,
+    each with mean 0, unit variance and its sign fixed by its third moment.
+A prediction (gnn_predict) is the network's output centred, scaled to the size
+that minimises the energy below over uniform scalings, plus a small seeded
+jitter; everything from the links to the layout runs on the GPU.
 It is trained on layouts the simulator itself produces at alpha ~ 0.1 (fast
 mode from phyllotaxis), with the stress loss over the graph's links and random
 node pairs, so the rotation and translation of the prediction are free.
@@ -45,42 +49,6 @@ def csr_undirected(n, src, tgt):
     np.add.at(row, a + 1, 1)
     row = np.cumsum(row).astype(np.uint32)
     return row, b.astype(np.uint32)
-
-
-def features(n, row, col, seed=0, modes=6, iterations=40):
-    """(n, 8) float32 node features (see the module docstring)."""
-    import scipy.sparse as sp
-    deg = np.diff(row.astype(np.int64)).astype(np.float64)
-    f = np.zeros((n, F_IN), dtype=np.float64)
-    f[:, 0] = np.log1p(deg)
-    f[:, 1] = 1.0 / np.sqrt(1.0 + deg)
-    if n > modes + 1:
-        data = np.ones(len(col))
-        rows = np.repeat(np.arange(n), np.diff(row.astype(np.int64)))
-        a = sp.csr_matrix((data, (rows, col.astype(np.int64))), shape=(n, n))
-        dinv = np.where(deg > 0, 1.0 / np.maximum(deg, 1), 0.0)
-        walk = sp.diags(dinv) @ a
-        rng = np.random.default_rng(seed)
-        x = rng.standard_normal((n, modes + 1))
-        for it in range(iterations):
-            x = 0.5 * (x + walk @ x)
-            if it % 5 == 4 or it == iterations - 1:
-                x -= x.mean(axis=0)
-                x, _ = np.linalg.qr(x)
-        # Rayleigh-Ritz on the lazy walk, smoothest modes first
-        mx = 0.5 * (x + walk @ x)
-        h = x.T @ mx
-        w, v = np.linalg.eig(h)
-        order = np.argsort(-w.real)
-        ritz = (x @ v[:, order].real)[:, :modes]
-        for k in range(modes):
-            c = ritz[:, k] - ritz[:, k].mean()
-            sd = c.std()
-            c = c / sd if sd > 0 else c
-            if np.mean(c ** 3) < 0:
-                c = -c
-            f[:, 2 + k] = c
-    return f.astype(np.float32)
 
 
 def stress_pairs(row, col, layout, random_pairs=4, seed=0):
@@ -136,6 +104,9 @@ _SIGNATURES = {
                                        ctypes.c_void_p]),
     "gnn_gradient": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]),
     "gnn_train_step": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_float, ctypes.c_void_p]),
+    "gnn_set_links": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
+    "gnn_predict": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, ctypes.c_void_p]),
 }
 
 
@@ -186,6 +157,24 @@ class Network:
         check(self.core.gnn_set_graph(self.h, self.n, _ptr(row), _ptr(col) if len(col) else None, _ptr(feats)))
         self._keep = (row, col, feats)
 
+    def set_links(self, n, src, tgt, seed=0, features=False):
+        """The graph from its links (rows and features on the GPU); returns
+        the (n, 8) FP32 features when `features`."""
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        tgt = np.ascontiguousarray(tgt, dtype=np.uint32)
+        f = np.empty((n, F_IN), dtype=np.float32) if features else None
+        check(self.core.gnn_set_links(self.h, n, len(src), _ptr(src) if len(src) else None,
+                                      _ptr(tgt) if len(tgt) else None, seed, _ptr(f) if f is not None else None))
+        self.n = n
+        self._keep = (src, tgt)
+        return f
+
+    def predict(self, jitter=0.02, seed=0):
+        """(n, 2) FP64 layout of the graph given to set_links (module docstring)."""
+        out = np.empty((self.n, 2))
+        check(self.core.gnn_predict(self.h, float(jitter), seed, _ptr(out)))
+        return out
+
     def forward(self):
         out = np.empty((self.n, 2), dtype=np.float32)
         check(self.core.gnn_forward(self.h, _ptr(out)))
@@ -234,19 +223,15 @@ def fast_simulation(n, src, tgt, core=None, x=None, y=None):
 
 
 def timed_ticks(sim, core, ticks):
-    """Runs `ticks` ticks (one copy in and out); returns the GPU seconds of
-    their kernels, from the library's event timing of each phase (the center
-    sums overlap the tick on a second stream and are not added)."""
-    from .core import context
-    h = context(sim.nodes()).h
+    """Runs `ticks` ticks through the facade (one copy in and out); returns
+    their wall-clock seconds, from an idle device to the layout back on the
+    host."""
+    import time
     check(core.core_synchronize())
-    core.stats(h)
-    core.core_set_timing(1)
+    t0 = time.perf_counter()
     sim.tick(ticks)
     check(core.core_synchronize())
-    core.core_set_timing(0)
-    ms = core.stats(h)["ms"]
-    return sum(v for k, v in ms.items() if k != "center_sum") / 1e3
+    return time.perf_counter() - t0
 
 
 def ticks_to_alpha(alpha0, alpha1, decay):
@@ -273,12 +258,14 @@ def training_graphs(count, seed=1, n_min=1000, n_max=20000):
 
 
 class Prepared:
-    """One graph's rows, features and (for training) normalised target layout."""
+    """One graph's rows, features (computed by `net` on the GPU, as a
+    prediction computes them) and, for training, normalised target layout."""
 
-    def __init__(self, n, src, tgt, seed=0):
+    def __init__(self, n, src, tgt, net, seed=0):
         self.n, self.src, self.tgt = n, np.asarray(src), np.asarray(tgt)
         self.row, self.col = csr_undirected(n, src, tgt)
-        self.x = features(n, self.row, self.col, seed=seed)
+        self.seed = seed
+        self.x = net.set_links(n, src, tgt, seed=seed, features=True)
         self.scale = math.sqrt(n)
 
 
@@ -290,30 +277,19 @@ def target_layout(g, core=None, alpha=0.1):
     return np.stack([nodes.x, nodes.y], axis=1)
 
 
-def _prepare(job):
-    k, n, src, tgt = job
-    return Prepared(n, src, tgt, seed=k)
-
-
 def train(graphs=48, epochs=30, steps=4, lr=2e-3, seed=1, out=WEIGHTS, log=print, core=None, n_max=20000,
-          workers=1, init=None):
+          init=None):
     """Trains the network on simulator layouts at alpha = 0.1 of `graphs`
     training graphs (n log-uniform in [1000, n_max]) and saves its parameters
     to `out`; `init` names parameters to start from (fine-tuning) instead of
-    the seeded initialisation. With workers > 1 the host-side graph features
-    are computed in that many spawned processes, started before this process
-    uses the GPU."""
-    gs = training_graphs(graphs, seed=seed, n_max=n_max)
-    jobs = [(k, n, s, t) for k, (n, s, t) in enumerate(gs)]
-    if workers > 1:
-        import multiprocessing
-        from concurrent.futures import ProcessPoolExecutor
-        with ProcessPoolExecutor(workers, mp_context=multiprocessing.get_context("spawn")) as ex:
-            prepared = list(ex.map(_prepare, jobs, chunksize=4))
-    else:
-        prepared = [_prepare(j) for j in jobs]
-    log("warm start: features of %d graphs ready" % len(prepared))
+    the seeded initialisation."""
     core = core or gpu_core()
+    net = Network(seed=seed, core=core)
+    if init:
+        net.load(init)
+    gs = training_graphs(graphs, seed=seed, n_max=n_max)
+    prepared = [Prepared(n, s, t, net, seed=k) for k, (n, s, t) in enumerate(gs)]
+    log("warm start: features of %d graphs ready" % len(prepared))
     for k, g in enumerate(prepared):
         lay = target_layout(g, core=core)
         lay = (lay - lay.mean(axis=0)) / g.scale
@@ -321,9 +297,6 @@ def train(graphs=48, epochs=30, steps=4, lr=2e-3, seed=1, out=WEIGHTS, log=print
         if (k + 1) % 100 == 0:
             log("warm start: %d training layouts ready" % (k + 1))
     log("warm start: %d training layouts ready" % len(prepared))
-    net = Network(seed=seed, core=core)
-    if init:
-        net.load(init)
     rng = np.random.default_rng(seed)
     total = epochs * len(prepared) * steps
     done = 0
@@ -343,38 +316,18 @@ def train(graphs=48, epochs=30, steps=4, lr=2e-3, seed=1, out=WEIGHTS, log=print
     return net
 
 
-def energy_scale(p, src, tgt, link_strength, charge=-30.0, distance=30.0):
-    """The factor s minimising the energy of s p (module docstring) over
-    uniform scalings, with every node pair farther apart than distanceMin:
-    dE/ds = sum k (s r - d) r - |charge| n (n - 1) / 2 / s = 0."""
-    n = len(p)
-    r = np.hypot(*(p[src] - p[tgt]).T)
-    a = np.sum(link_strength * r * r)
-    b = np.sum(link_strength * distance * r)
-    c = abs(charge) * n * (n - 1) / 2
-    return (b + math.sqrt(b * b + 4 * a * c)) / (2 * a) if a > 0 else 1.0
-
-
-def predict(net, g, jitter=0.02, seed=0):
-    """Predicted layout (n x 2) in the simulation's units: the network's shape,
-    centred and scaled to the energy-minimising size (the network's global
-    scale is its least reliable output, and a layout much larger or smaller
-    than the simulation's costs refinement ticks), plus a seeded uniform
-    jitter of +-`jitter` times the median predicted link length. Nodes with
-    identical features and neighbourhoods (leaves of one hub, isolated
-    nodes) get identical predictions; the jitter spreads them on the
-    layout's own scale, where a fixed jitter would leave them stacked (far
-    inside distanceMin, where their repulsion is largest)."""
-    net.set_graph(g.row, g.col, g.x)
-    p = net.forward().astype(np.float64)
-    p -= p.mean(axis=0)
-    spread = jitter
-    if len(g.src):
-        p *= energy_scale(p, g.src, g.tgt, _link_strength(g.n, g.src, g.tgt))
-        spread = jitter * float(np.median(np.hypot(*(p[g.src] - p[g.tgt]).T)))
-    if spread:
-        p += np.random.default_rng(seed).uniform(-spread, spread, p.shape)
-    return p
+def predict(net, n, src, tgt, jitter=0.02, seed=0):
+    """Predicted layout (n x 2) in the simulation's units, all on the GPU:
+    rows and features from the links, the network's shape centred and scaled
+    to the energy-minimising size (the network's global scale is its least
+    reliable output, and a layout much larger or smaller than the
+    simulation's costs refinement ticks), plus a seeded uniform jitter of
+    +-`jitter` times the mean predicted link length. Nodes with identical
+    features and neighbourhoods (leaves of one hub, isolated nodes) get
+    identical predictions; the jitter spreads them on the layout's own
+    scale."""
+    net.set_links(n, src, tgt, seed=seed)
+    return net.predict(jitter=jitter, seed=seed)
 
 
 def graph_stress(x, y, row, col, sources=16, seed=5):
@@ -407,15 +360,20 @@ def compare(net, n, src, tgt, core=None, seed=0, short=(50, 100), stress_all=Tru
     0.001 over K ticks) and against plain fast mode squeezed into the same K
     ticks (alpha 1 -> 0.001 over K ticks: the method without the network),
     for each K of `short`. All runs reuse one simulation (its buffers are
-    allocated by a first, untimed run); seconds are GPU kernel time.
+    allocated by a first, untimed run). Seconds are wall-clock on both sides:
+    a plain run from its start layout on the host to its layout back on the
+    host; a warm run from the links on the host (rows, features, network,
+    scaling and jitter on the GPU, the layout to the host) through its
+    refinement ticks to the layout back on the host.
     Energies use one sample of node pairs; `gap` is (E - E_plain) /
     (E_start - E_plain): 0 is as good as 300 plain ticks, 1 no better than
     the start. With stress_all False the squeezed plain runs skip the
     (host-side) stress measurement."""
     import time
     core = core or gpu_core()
-    g = Prepared(n, src, tgt, seed=seed)
-    k_link = _link_strength(n, g.src, g.tgt)
+    src, tgt = np.asarray(src), np.asarray(tgt)
+    row, col = csr_undirected(n, src, tgt)
+    k_link = _link_strength(n, src, tgt)
     sim, nodes = fast_simulation(n, src, tgt, core=core)
     x0, y0 = nodes.x.copy(), nodes.y.copy()
     plain_decay = sim.alpha_decay()
@@ -429,29 +387,28 @@ def compare(net, n, src, tgt, core=None, seed=0, short=(50, 100), stress_all=Tru
         return timed_ticks(sim, core, ticks)
 
     run(x0, y0, 1.0, plain_decay, 5)  # allocations, untimed
-    e_start, pairs = energy(x0, y0, g.src, g.tgt, k_link)
+    e_start, pairs = energy(x0, y0, src, tgt, k_link)
     t_plain = run(x0, y0, 1.0, plain_decay, 300)
-    e_plain, _ = energy(nodes.x, nodes.y, g.src, g.tgt, k_link, pairs=pairs)
+    e_plain, _ = energy(nodes.x, nodes.y, src, tgt, k_link, pairs=pairs)
     span = e_start - e_plain
     out = {"nodes": n, "links": len(src), "start_energy": e_start, "plain_energy": e_plain, "plain_ticks": 300,
-           "plain_seconds": t_plain, "plain_stress": graph_stress(nodes.x, nodes.y, g.row, g.col), "runs": []}
-    predict(net, g)  # first call allocates
+           "plain_seconds": t_plain, "plain_stress": graph_stress(nodes.x, nodes.y, row, col), "runs": []}
+    predict(net, n, src, tgt, seed=seed)  # first call allocates
     check(core.core_synchronize())
     t0 = time.perf_counter()
-    p = predict(net, g)
-    check(core.core_synchronize())
-    t_predict = time.perf_counter() - t0  # wall time of the network's kernels and copies
-    e_pred, _ = energy(p[:, 0], p[:, 1], g.src, g.tgt, k_link, pairs=pairs)
+    p = predict(net, n, src, tgt, seed=seed)
+    t_predict = time.perf_counter() - t0
+    e_pred, _ = energy(p[:, 0], p[:, 1], src, tgt, k_link, pairs=pairs)
     out["predict_seconds"] = t_predict
     out["predicted_energy_gap"] = (e_pred - e_plain) / span
     run(p[:, 0], p[:, 1], 0.1, 1 - math.pow(0.001 / 0.1, 1 / max(short)), 3)  # buffers for this layout, untimed
     for k in short:
         t_warm = t_predict + run(p[:, 0], p[:, 1], 0.1, 1 - math.pow(0.001 / 0.1, 1 / k), k)
-        e_warm, _ = energy(nodes.x, nodes.y, g.src, g.tgt, k_link, pairs=pairs)
-        s_warm = graph_stress(nodes.x, nodes.y, g.row, g.col)
+        e_warm, _ = energy(nodes.x, nodes.y, src, tgt, k_link, pairs=pairs)
+        s_warm = graph_stress(nodes.x, nodes.y, row, col)
         t_short = run(x0, y0, 1.0, 1 - math.pow(0.001, 1 / k), k)
-        e_short, _ = energy(nodes.x, nodes.y, g.src, g.tgt, k_link, pairs=pairs)
-        s_short = graph_stress(nodes.x, nodes.y, g.row, g.col) if stress_all else None
+        e_short, _ = energy(nodes.x, nodes.y, src, tgt, k_link, pairs=pairs)
+        s_short = graph_stress(nodes.x, nodes.y, row, col) if stress_all else None
         out["runs"].append({
             "ticks": k,
             "warm": {"energy": e_warm, "gap": (e_warm - e_plain) / span,
@@ -524,7 +481,6 @@ if __name__ == "__main__":
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--steps", type=int, default=4, help="Adam steps per graph and epoch")
     ap.add_argument("--n-max", type=int, default=20000, help="largest training graph (nodes)")
-    ap.add_argument("--workers", type=int, default=1, help="processes for the host-side graph features")
     ap.add_argument("--lr", type=float, default=2e-3, help="peak Adam learning rate (cosine schedule)")
     ap.add_argument("--init", default=None, help="parameters to fine-tune from")
     ap.add_argument("--out", default=WEIGHTS)
@@ -542,4 +498,4 @@ if __name__ == "__main__":
         print(json.dumps(res["summary"]))
     else:
         train(graphs=a.graphs, epochs=a.epochs, steps=a.steps, lr=a.lr, out=a.out, n_max=a.n_max,
-              workers=a.workers, init=a.init, log=lambda *m: print(*m, flush=True))
+              init=a.init, log=lambda *m: print(*m, flush=True))

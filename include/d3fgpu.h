@@ -237,6 +237,17 @@ int32_t gnn_sync_params(uint32_t h, uint32_t direction); /* H2D also resets the 
 /* row[n + 1], col[row[n]]: symmetric adjacency rows; features: n x 8 FP32. */
 int32_t gnn_set_graph(uint32_t h, uint32_t n, const uint32_t* row, const uint32_t* col, const float* features);
 int32_t gnn_forward(uint32_t h, float* positions); /* n x 2 */
+/* The graph from its m links alone, on the GPU: symmetric adjacency rows
+ * without self-links or repeats, and the 8 input features (log(1 + degree),
+ * 1 / sqrt(1 + degree), six smooth random-walk modes from seeded subspace
+ * iteration); features, when not null, receives them (n x 8 FP32). */
+int32_t gnn_set_links(uint32_t h, uint32_t n, uint32_t m, const uint32_t* source, const uint32_t* target,
+                      uint64_t seed, float* features);
+/* Predicted layout of the graph set by gnn_set_links (n x 2 FP64): the
+ * network's output centred, scaled to the size minimising d3's energy over
+ * uniform scalings, plus a seeded jitter of +-jitter times the mean link
+ * length. */
+int32_t gnn_predict(uint32_t h, double jitter, uint64_t seed, double* positions);
 /* Stress pairs for training: |p_i - p_j| should be d. */
 int32_t gnn_set_pairs(uint32_t h, uint32_t pairs, const uint32_t* i, const uint32_t* j, const float* d);
 /* Gradient of sum(dp * positions) into grad (gnn_param_count() floats). */

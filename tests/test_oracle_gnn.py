@@ -5,7 +5,7 @@ BF16 rounding against exact values."""
 import numpy as np
 
 from oracle import gnn_ref as ref
-from d3force_amd.warmstart import csr_undirected, features
+from d3force_amd.warmstart import csr_undirected
 
 
 def small_case(n=40, seed=3):
@@ -57,14 +57,3 @@ def test_bf16_rounding():
     assert r[2] == 1.0 + 2 ** -6    # halfway: to even (1 + 4 * 2^-8)
     assert r[3] == -2.5
     assert ref.bf16(np.float32(1.0 + 2 ** -7)) == 1.0 + 2 ** -7
-
-
-def test_features_are_deterministic_and_normalised():
-    rng = np.random.default_rng(1)
-    n = 500
-    row, col = csr_undirected(n, rng.integers(0, n, 1500), rng.integers(0, n, 1500))
-    f1 = features(n, row, col, seed=4)
-    f2 = features(n, row, col, seed=4)
-    assert np.array_equal(f1, f2)
-    assert f1.shape == (n, 8)
-    assert np.allclose(f1[:, 2:].std(axis=0), 1, atol=1e-3)
