@@ -387,26 +387,36 @@ __global__ __launch_bounds__(256) void k_in_init(const uint32_t* __restrict__ ro
   v[8 * (size_t)i + 7] = sqrt(deg);
 }
 
-// y = L v on columns 0..6 (column 7 copied: L keeps sqrt(degree)).
+// y = L v on columns 0..6 (column 7 copied: L keeps sqrt(degree)). A row
+// is summed by a group of IN_GROUP lanes (neighbour a by lane a mod
+// IN_GROUP, then a fixed-order shuffle reduction), so hub rows of
+// thousands of neighbours do not hold one lane for the whole pass.
+constexpr int IN_GROUP = 16;
 __global__ __launch_bounds__(256) void k_in_walk(const uint32_t* __restrict__ row, const uint32_t* __restrict__ col,
                                                  const double* __restrict__ dis, const double* __restrict__ v,
                                                  double* __restrict__ y, uint32_t n) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  const uint32_t i = (blockIdx.x * 256 + threadIdx.x) / IN_GROUP;
+  const int g = threadIdx.x % IN_GROUP;
+  if (i >= n) return;  // whole groups leave together
   double s[IN_COLS];
 #pragma unroll
   for (int k = 0; k < IN_COLS; ++k) s[k] = 0;
-  for (uint32_t a = row[i]; a < row[i + 1]; ++a) {
+  for (uint32_t a = row[i] + g; a < row[i + 1]; a += IN_GROUP) {
     const uint32_t j = col[a];
     const double w = dis[j];
     const double* vj = v + 8 * (size_t)j;
 #pragma unroll
     for (int k = 0; k < IN_COLS; ++k) s[k] += w * vj[k];
   }
-  const double di = dis[i];
 #pragma unroll
-  for (int k = 0; k < IN_COLS; ++k) y[8 * (size_t)i + k] = 0.5 * (v[8 * (size_t)i + k] + di * s[k]);
-  y[8 * (size_t)i + 7] = v[8 * (size_t)i + 7];
+  for (int k = 0; k < IN_COLS; ++k)
+    for (int d = IN_GROUP / 2; d > 0; d >>= 1) s[k] += __shfl_xor(s[k], d, IN_GROUP);
+  if (g == 0) {
+    const double di = dis[i];
+#pragma unroll
+    for (int k = 0; k < IN_COLS; ++k) y[8 * (size_t)i + k] = 0.5 * (v[8 * (size_t)i + k] + di * s[k]);
+    y[8 * (size_t)i + 7] = v[8 * (size_t)i + 7];
+  }
 }
 
 // Per-block partial sums of V^T W (8 x 8; W = V when null) into part[block][64].
@@ -866,18 +876,19 @@ struct Net {
     HIPCHECK(fpart.reserve((size_t)IN_BLOCKS * 64));
     HIPCHECK(fred.reserve(64));
     const uint32_t nb = ceil_div(n > 0 ? n : 1, 256u);
+    const uint32_t wb = (uint32_t)(((uint64_t)(n > 0 ? n : 1) * IN_GROUP + 255) / 256);
     hipLaunchKernelGGL(k_in_init, dim3(nb), dim3(256), 0, stream, row.p, n, seed, fv.p, dis.p, ff.p);
     bool ok = n > (uint32_t)IN_COLS;  // warmstart.py's features: modes only above 7 nodes
     Mat7 t{};
     constexpr int ITERATIONS = 40;
     for (int it = 0; it < ITERATIONS && ok; ++it) {
-      hipLaunchKernelGGL(k_in_walk, dim3(nb), dim3(256), 0, stream, row.p, col.p, dis.p, fv.p, fy.p, n);
+      hipLaunchKernelGGL(k_in_walk, dim3(wb), dim3(256), 0, stream, row.p, col.p, dis.p, fv.p, fy.p, n);
       std::swap(fv.p, fy.p);
       if (it % 5 == 4 || it == ITERATIONS - 1)
         for (int pass = 0; pass < 2 && ok; ++pass) HIPCHECK(orthonormalise(&ok));
     }
     if (ok) {  // Rayleigh-Ritz: H = V^T L V, smoothest modes first
-      hipLaunchKernelGGL(k_in_walk, dim3(nb), dim3(256), 0, stream, row.p, col.p, dis.p, fv.p, fy.p, n);
+      hipLaunchKernelGGL(k_in_walk, dim3(wb), dim3(256), 0, stream, row.p, col.p, dis.p, fv.p, fy.p, n);
       double g[8][8], h[IN_COLS][IN_COLS], w[IN_COLS], q[IN_COLS][IN_COLS];
       HIPCHECK(gram(fy.p, g));
       for (int r = 0; r < IN_COLS; ++r)

@@ -1,6 +1,7 @@
 """Learned warm start: a message-passing network predicts a layout, and fast
-mode refines it from alpha = 0.1 instead of running from d3's phyllotaxis
-start at alpha = 1 (the reference always starts there, simulation.js:91-105).
+mode refines it from alpha = REFINE_ALPHA (0.5) instead of running from d3's
+phyllotaxis start at alpha = 1 (the reference always starts there,
+simulation.js:91-105).
 
 The network (csrc/gnn.hip) has 8 input features per node, 4 message-passing
 layers of width 64 and a 2-D position head; it runs and trains on the GPU with
@@ -34,6 +35,7 @@ import numpy as np
 from .core import check, gpu_core
 
 F_IN = 8
+REFINE_ALPHA = 0.5  # where the refinement of a prediction starts (alpha decays to 0.001)
 
 
 def csr_undirected(n, src, tgt):
@@ -356,8 +358,8 @@ def graph_stress(x, y, row, col, sources=16, seed=5):
 
 def compare(net, n, src, tgt, core=None, seed=0, short=(50, 100), stress_all=True):
     """Plain fast mode from phyllotaxis (300 ticks, d3's schedule) against
-    the warm start (the prediction refined from alpha = 0.1, alpha decaying to
-    0.001 over K ticks) and against plain fast mode squeezed into the same K
+    the warm start (the prediction refined from alpha = REFINE_ALPHA, alpha
+    decaying to 0.001 over K ticks) and against plain fast mode squeezed into the same K
     ticks (alpha 1 -> 0.001 over K ticks: the method without the network),
     for each K of `short`. All runs reuse one simulation (its buffers are
     allocated by a first, untimed run). Seconds are wall-clock on both sides:
@@ -401,9 +403,10 @@ def compare(net, n, src, tgt, core=None, seed=0, short=(50, 100), stress_all=Tru
     e_pred, _ = energy(p[:, 0], p[:, 1], src, tgt, k_link, pairs=pairs)
     out["predict_seconds"] = t_predict
     out["predicted_energy_gap"] = (e_pred - e_plain) / span
-    run(p[:, 0], p[:, 1], 0.1, 1 - math.pow(0.001 / 0.1, 1 / max(short)), 3)  # buffers for this layout, untimed
+    a0 = REFINE_ALPHA
+    run(p[:, 0], p[:, 1], a0, 1 - math.pow(0.001 / a0, 1 / max(short)), 3)  # buffers for this layout, untimed
     for k in short:
-        t_warm = t_predict + run(p[:, 0], p[:, 1], 0.1, 1 - math.pow(0.001 / 0.1, 1 / k), k)
+        t_warm = t_predict + run(p[:, 0], p[:, 1], a0, 1 - math.pow(0.001 / a0, 1 / k), k)
         e_warm, _ = energy(nodes.x, nodes.y, src, tgt, k_link, pairs=pairs)
         s_warm = graph_stress(nodes.x, nodes.y, row, col)
         t_short = run(x0, y0, 1.0, 1 - math.pow(0.001, 1 / k), k)
