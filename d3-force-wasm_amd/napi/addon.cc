@@ -2,8 +2,12 @@
 // uses in place of its wasm instance (src/core.js:24-30). Each export wraps
 // one C entry point of include/d3fgpu.h one to one; arrays the wasm core
 // exposed as offsets into its linear memory are handed to JS as external
-// ArrayBuffers over the library's pinned host memory (valid until the next
-// GROWN status for that simulation or force). The random callback is the
+// ArrayBuffers over the library's pinned host memory. Before a call that may
+// free or move that memory (a resize, a count change, a destroy, a tree
+// build) the addon detaches every buffer it handed out over it, so a view
+// kept across such a call reads as empty instead of reaching freed memory
+// (the wasm core's views are detached by memory growth in the same way).
+// The random callback is the
 // JS function given to set_random, called synchronously on the JS thread
 // inside force_run, as the wasm import env.random was (core.js:11-14).
 #include <node_api.h>
@@ -66,13 +70,45 @@ napi_value num(napi_env env, double v) {
   return r;
 }
 
-napi_value external(napi_env env, void* p, size_t bytes) {
+// Buffers handed out, by owner: the simulation, and the force (0: the
+// simulation's own arrays, TREE: its exported tree); weak references.
+constexpr uint32_t TREE = 0xffffffffu;
+struct Held {
+  uint32_t sim, force;
+  napi_ref ref;
+};
+std::vector<Held> g_held;
+
+// Detaches the held buffers of simulation `sim` whose owner matches `force`
+// (ALL: every owner of the simulation) and forgets them.
+constexpr uint32_t ALL = 0xfffffffeu;
+void detach(napi_env env, uint32_t sim, uint32_t force) {
+  size_t keep = 0;
+  for (size_t k = 0; k < g_held.size(); ++k) {
+    Held& h = g_held[k];
+    if (h.sim == sim && (force == ALL || h.force == force)) {
+      napi_value ab;
+      if (napi_get_reference_value(env, h.ref, &ab) == napi_ok && ab) napi_detach_arraybuffer(env, ab);
+      napi_delete_reference(env, h.ref);
+    } else {
+      g_held[keep++] = h;
+    }
+  }
+  g_held.resize(keep);
+}
+
+napi_value external(napi_env env, void* p, size_t bytes, uint32_t sim, uint32_t force) {
   napi_value r;
   if (!p || bytes == 0) {
     napi_get_null(env, &r);
     return r;
   }
-  if (napi_create_external_arraybuffer(env, p, bytes, nullptr, nullptr, &r) != napi_ok) napi_get_null(env, &r);
+  if (napi_create_external_arraybuffer(env, p, bytes, nullptr, nullptr, &r) != napi_ok) {
+    napi_get_null(env, &r);
+    return r;
+  }
+  napi_ref ref;
+  if (napi_create_reference(env, r, 0, &ref) == napi_ok) g_held.push_back(Held{sim, force, ref});
   return r;
 }
 
@@ -104,13 +140,42 @@ napi_value SimCreate(napi_env env, napi_callback_info) { return num(env, sim_cre
     return num(env, fn(u32(env, argv[0]), u32(env, argv[1])));         \
   }
 
-WRAP1(SimDestroy, sim_destroy)
-WRAP2(SimSetNodeCount, sim_set_node_count)
 WRAP2(SimSync, sim_sync)
 WRAP2(SimSnapshot, sim_snapshot)
-WRAP2(TreeBuild, tree_build)
 WRAP2(ForceCreate, force_create)
-WRAP2(ForceDestroy, force_destroy)
+
+napi_value SimDestroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  const uint32_t h = u32(env, argv[0]);
+  detach(env, h, ALL);
+  return num(env, sim_destroy(h));
+}
+
+// A new node count may grow the simulation's arrays and its forces'.
+napi_value SimSetNodeCount(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  const uint32_t h = u32(env, argv[0]);
+  detach(env, h, ALL);
+  return num(env, sim_set_node_count(h, u32(env, argv[1])));
+}
+
+napi_value TreeBuild(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  const uint32_t h = u32(env, argv[0]);
+  detach(env, h, TREE);
+  return num(env, tree_build(h, u32(env, argv[1])));
+}
+
+napi_value ForceDestroy(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  const uint32_t h = u32(env, argv[0]), fh = u32(env, argv[1]);
+  detach(env, h, fh);
+  return num(env, force_destroy(h, fh));
+}
 
 napi_value TreeInfo(napi_env env, napi_callback_info info) {
   napi_value argv[2];
@@ -128,7 +193,8 @@ napi_value SimIntegrate(napi_env env, napi_callback_info info) {
 napi_value SimBuffer(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   if (!args(env, info, 3, argv)) return nullptr;
-  return external(env, sim_ptr(u32(env, argv[0]), u32(env, argv[1])), u32(env, argv[2]));
+  const uint32_t h = u32(env, argv[0]);
+  return external(env, sim_ptr(h, u32(env, argv[1])), u32(env, argv[2]), h, 0);
 }
 
 // tree_buffer(h, which): 2 cells (len x 88 bytes), 3 extent (32), 4 perm (m x 4).
@@ -140,7 +206,7 @@ napi_value TreeBuffer(napi_env env, napi_callback_info info) {
   if (which == 2) bytes = (size_t)tree_info(h, 0) * (size_t)tree_info(h, 5);
   else if (which == 3) bytes = 32;
   else if (which == 4) bytes = (size_t)tree_info(h, 6) * 4;
-  return external(env, (void*)(uintptr_t)tree_info(h, which), bytes);
+  return external(env, (void*)(uintptr_t)tree_info(h, which), bytes, h, TREE);
 }
 
 // tree_order(h, after) -> Uint32Array of the cell count.
@@ -161,14 +227,17 @@ napi_value TreeOrder(napi_env env, napi_callback_info info) {
 napi_value ForceSetCount(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   if (!args(env, info, 3, argv)) return nullptr;
-  return num(env, force_set_count(u32(env, argv[0]), u32(env, argv[1]), u32(env, argv[2])));
+  const uint32_t h = u32(env, argv[0]), fh = u32(env, argv[1]);
+  detach(env, h, fh);
+  return num(env, force_set_count(h, fh, u32(env, argv[2])));
 }
 
 // force_buffer(h, fh, which, bytes): the array force_ptr(h, fh, which) points at.
 napi_value ForceBuffer(napi_env env, napi_callback_info info) {
   napi_value argv[4];
   if (!args(env, info, 4, argv)) return nullptr;
-  return external(env, force_ptr(u32(env, argv[0]), u32(env, argv[1]), u32(env, argv[2])), u32(env, argv[3]));
+  const uint32_t h = u32(env, argv[0]), fh = u32(env, argv[1]);
+  return external(env, force_ptr(h, fh, u32(env, argv[2])), u32(env, argv[3]), h, fh);
 }
 
 napi_value ForceSetScalar(napi_env env, napi_callback_info info) {

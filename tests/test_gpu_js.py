@@ -58,3 +58,40 @@ def test_js_facade_ticks_bit_identical_to_oracle(oracle_core, tmp_path, radius):
         sim.tick()
     ref = nodes.state()
     assert np.array_equal(ref.view(np.uint64), got.view(np.uint64))
+
+
+@needs_node
+@pytest.mark.gpu
+def test_views_kept_across_a_resize_are_detached():
+    # A typed array over the library's pinned memory, kept across a call that
+    # may free or move that memory (a count change, a node resize, a destroy),
+    # reads as empty afterwards instead of reaching freed memory; views made
+    # after the call work.
+    core_js = os.path.join(ROOT, "d3-force-wasm_amd", "js", "core.js")
+    code = """
+import('%s').then(({context, Slot, LINK, F0}) => {
+  const nodes = Array.from({length: 100}, (_, i) => ({index: i, x: i, y: 0, vx: 0, vy: 0}));
+  const ctx = context(nodes);
+  ctx.resize(100), ctx.views();
+  const x = ctx.x;
+  const link = new Slot(LINK);
+  link.bind(nodes), link.count(10);
+  const f = link.f64(F0, 10);
+  f.fill(3);
+  const before = [x.length, f.length, f[9]];
+  link.count(20);
+  const afterCount = [f.length, x.length];
+  ctx.resize(5000);
+  const afterResize = x.length;
+  ctx.views();
+  const g = link.f64(F0, 20);
+  console.log(JSON.stringify({before, afterCount, afterResize, fresh: [ctx.x.length, g.length]}));
+});
+""" % core_js
+    out = subprocess.run(["node", "-e", code], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr[-2000:]
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    assert r["before"] == [100, 10, 3]
+    assert r["afterCount"] == [0, 100]  # the force's view detached, the nodes' kept
+    assert r["afterResize"] == 0
+    assert r["fresh"] == [5000, 20]
