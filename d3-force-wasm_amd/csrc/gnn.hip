@@ -450,14 +450,21 @@ __global__ __launch_bounds__(256) void k_in_gram(const double* __restrict__ v, c
         red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
 }
 
-// out[k] = sum over blocks of part[block][k], k < width.
-__global__ __launch_bounds__(64) void k_in_sum(const double* __restrict__ part, uint32_t blocks, uint32_t width,
-                                               double* __restrict__ out) {
-  const uint32_t k = threadIdx.x;
-  if (k >= width) return;
+// out[k] = sum over blocks of part[block][k]: one block per k, the
+// partials added by a fixed tree in LDS.
+__global__ __launch_bounds__(256) void k_in_sum(const double* __restrict__ part, uint32_t blocks, uint32_t width,
+                                                double* __restrict__ out) {
+  __shared__ double red[256];
+  const uint32_t k = blockIdx.x;
   double s = 0;
-  for (uint32_t b = 0; b < blocks; ++b) s += part[(size_t)b * width + k];
-  out[k] = s;
+  for (uint32_t b = threadIdx.x; b < blocks; b += 256) s += part[(size_t)b * width + k];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = 128; d > 0; d >>= 1) {
+    if ((int)threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[k] = red[0];
 }
 
 // V[0..6] <- (V[0..6] - V[7] c^T) M, column 7 kept.
@@ -767,7 +774,7 @@ struct Net {
 
   // Sum of `width` values over the IN_BLOCKS partials in fpart, to the host.
   hipError_t reduce_host(uint32_t width, double* out) {
-    hipLaunchKernelGGL(k_in_sum, dim3(1), dim3(64), 0, stream, fpart.p, IN_BLOCKS, width, fred.p);
+    hipLaunchKernelGGL(k_in_sum, dim3(width), dim3(256), 0, stream, fpart.p, IN_BLOCKS, width, fred.p);
     HIPCHECK(hipMemcpyAsync(out, fred.p, width * sizeof(double), hipMemcpyDeviceToHost, stream));
     return hipStreamSynchronize(stream);
   }
