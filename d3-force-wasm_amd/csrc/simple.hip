@@ -441,9 +441,39 @@ __global__ __launch_bounds__(CK_THREADS) void k_cs_chain(const double* __restric
           }
         }
       };
+      // The round first runs speculatively: a record's kind (needing or
+      // segment) does not depend on s, so only a segment's transform choice
+      // (by the parity of s) and its add sit on the chain; its interval test
+      // is noted, not branched on. If any test failed, the round is redone
+      // from its start by `record`, which branches on each test.
+      auto quick = [&](const double (&v)[CD], uint32_t l, bool& bad) {
+        if (__builtin_amdgcn_readlane(__double2hiint(v[6]), l)) {
+          const uint32_t fi = __builtin_amdgcn_readlane(__double2loint(v[6]), l);
+          const double* x = sbuf[b] + (fi - f0) * CC;
+          double a[CC];
+#pragma unroll
+          for (uint32_t e = 0; e < CC; ++e) a[e] = x[e];
+#pragma unroll
+          for (uint32_t e = 0; e < CC; ++e) s += a[e];
+        } else {
+          const double lo0 = rl(v[0], l), hi0 = rl(v[1], l), d0 = rl(v[2], l);
+          const double lo1 = rl(v[3], l), hi1 = rl(v[4], l), d1 = rl(v[5], l);
+          const bool odd = __double2loint(s) & 1;
+          const double lo = odd ? lo1 : lo0, hi = odd ? hi1 : hi0, dd = odd ? d1 : d0;
+          bad = bad | !(lo <= s && s <= hi);
+          s += dd;
+        }
+      };
       const uint32_t m1 = cnt < 64 ? cnt : 64;
-      for (uint32_t k = 0; k < m1; ++k) record(da, k);
-      for (uint32_t k = 64; k < cnt; ++k) record(db, k - 64);
+      const double s0 = s;
+      bool bad = false;
+      for (uint32_t k = 0; k < m1; ++k) quick(da, k, bad);
+      for (uint32_t k = 64; k < cnt; ++k) quick(db, k - 64, bad);
+      if (__builtin_amdgcn_ballot_w64(bad)) {
+        s = s0;
+        for (uint32_t k = 0; k < m1; ++k) record(da, k);
+        for (uint32_t k = 64; k < cnt; ++k) record(db, k - 64);
+      }
     }
     __syncthreads();
   }
