@@ -54,7 +54,15 @@ struct TreeMeta {
   uint32_t overflow;  // a deep walk exceeded its stack
   uint32_t ticket;  // block counter of the box reduction
   uint32_t long_group;  // a run of equal high key halves longer than the in-place sort takes
+  uint32_t ncut;      // subtrees of at most ACC_SUB cells whose parent's subtree is larger (or the root's)
+  uint32_t ntop;      // cells whose subtree exceeds ACC_SUB cells
+  uint32_t topdepth;  // deepest of those cells
 };
+
+// Subtrees of at most this many cells are accumulated by one workgroup in LDS.
+// Their list entries are (root, size | depth << CUT_SIZE_BITS).
+constexpr uint32_t ACC_SUB = 1024;
+constexpr uint32_t CUT_SIZE_BITS = 11, CUT_SIZE_MASK = (1u << CUT_SIZE_BITS) - 1;
 
 // A depth-first frame of the below-key-resolution build (tree.zig:64-76).
 struct Frame {
@@ -116,6 +124,9 @@ struct TreeBuffers {
   Cell* cells = nullptr;
   TCell* tcells = nullptr;
   uint32_t* cdepth = nullptr;                        // depth of each cell
+  uint32_t* cskip = nullptr;                         // cells in each cell's subtree (itself included)
+  uint2* cut = nullptr;                              // subtree roots (TreeMeta ncut), see ACC_SUB
+  uint32_t* top = nullptr;                           // cells above them (TreeMeta ntop)
   uint32_t *cell_s = nullptr, *cell_d = nullptr;     // each cell's key group and depth
   uint32_t* rs_glob = nullptr;                       // radix digit totals, 8 passes
   uint32_t *dbins = nullptr, *dstart = nullptr, *dorder = nullptr;  // cells by depth

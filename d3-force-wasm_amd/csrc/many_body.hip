@@ -169,6 +169,277 @@ __global__ __launch_bounds__(ACC_THREADS) void k_accumulate_level(
   }
 }
 
+// visitAfter over the subtrees in `cut` (at most ACC_SUB cells each, a
+// contiguous preorder range [r, r + skip)), one subtree per workgroup at a
+// time: leaves first (many_body.zig:125-132), then the internal cells one
+// depth at a time from the deepest, children read from LDS (many_body.zig:
+// 112-121). Each thread keeps its cells' children, depth and record fields
+// in registers; the sums have d3's operands in d3's order.
+constexpr int SUB_THREADS = 256;
+
+// A subtree ends inside the tree (1 <= skip <= len - c by construction); the
+// clamp keeps every walk moving forward whatever the arrays hold.
+__device__ inline uint32_t clamp_skip(uint32_t skip, uint32_t c, uint32_t len) {
+  return skip >= 1 && skip <= len - c ? skip : len - c;
+}
+constexpr int SUB_PER = (int)ACC_SUB / SUB_THREADS;
+
+__global__ __launch_bounds__(SUB_THREADS) void k_acc_subtrees(
+    const Cell* __restrict__ cells, TCell* __restrict__ tc, const uint2* __restrict__ cut,
+    const uint32_t* __restrict__ cskip, const uint32_t* __restrict__ cdepth, const TreeMeta* __restrict__ meta,
+    const uint32_t* __restrict__ perm, const double* __restrict__ xs, const double* __restrict__ ys,
+    const double* __restrict__ s, double theta2) {
+  __shared__ double sx[ACC_SUB], sy[ACC_SUB], sv[ACC_SUB];
+  __shared__ uint8_t sreach[ACC_SUB];
+  __shared__ uint32_t smaxd;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t ncut = meta->ncut, len = meta->len;
+  if (tid == 0) smaxd = 0;
+  __syncthreads();
+  for (uint32_t j = blockIdx.x; j < ncut; j += gridDim.x) {
+    const uint2 e = cut[j];  // root, size | depth << CUT_SIZE_BITS (k_emit)
+    const uint32_t r = e.x;
+    const uint32_t size = min(min(e.y & CUT_SIZE_MASK, ACC_SUB), len - r);  // <= ACC_SUB by construction
+    const uint32_t d0 = e.y >> CUT_SIZE_BITS;
+    // Per cell: its children as 10-bit offsets in the subtree (0 = none: the
+    // subtree's root, at offset 0, is no cell's child here), its depth, or
+    // NONE for a leaf or a slot past the subtree. The loads are issued for
+    // the four slots together, with no branch between them (a slot past the
+    // subtree loads the root's again): records, then leaf heads, then
+    // positions and strengths.
+    uint4 ch4[SUB_PER];
+    uint2 fc[SUB_PER];
+    double x0[SUB_PER], x1[SUB_PER];
+    uint32_t dep[SUB_PER], skp[SUB_PER];
+#pragma unroll
+    for (int q = 0; q < SUB_PER; ++q) {
+      const uint32_t l = tid + (uint32_t)q * SUB_THREADS;
+      const Cell* cp = cells + r + (l < size ? l : 0);
+      ch4[q] = *reinterpret_cast<const uint4*>(cp->child);
+      fc[q] = *reinterpret_cast<const uint2*>(&cp->first);
+      x0[q] = cp->x0;
+      x1[q] = cp->x1;
+      dep[q] = cdepth[cp - cells];
+      skp[q] = cskip[cp - cells];
+    }
+    uint64_t kids[SUB_PER];
+    double w2t[SUB_PER];
+    uint32_t head[SUB_PER];
+    bool leaf[SUB_PER];
+#pragma unroll
+    for (int q = 0; q < SUB_PER; ++q) {
+      const double w = x1[q] - x0[q];
+      w2t[q] = w * w / theta2;  // many_body.zig:202, identical operands per cell
+      const uint32_t k4[4] = {ch4[q].x, ch4[q].y, ch4[q].z, ch4[q].w};
+      leaf[q] = k4[0] == NONE && k4[1] == NONE && k4[2] == NONE && k4[3] == NONE;
+      kids[q] = 0;
+      for (int k = 0; k < 4; ++k) {  // children lie inside the subtree's range
+        const uint32_t o = k4[k] == NONE || k4[k] - r >= size ? 0u : k4[k] - r;
+        kids[q] |= (uint64_t)o << (10 * k);
+      }
+      head[q] = perm[fc[q].x + fc[q].y - 1];
+    }
+    double hx[SUB_PER], hy[SUB_PER], hs[SUB_PER];
+#pragma unroll
+    for (int q = 0; q < SUB_PER; ++q) {
+      hx[q] = xs[head[q]];
+      hy[q] = ys[head[q]];
+      hs[q] = s[head[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < SUB_PER; ++q) {
+      const uint32_t l = tid + (uint32_t)q * SUB_THREADS;
+      if (l >= size) {
+        dep[q] = NONE;
+        continue;
+      }
+      if (!leaf[q]) {
+        atomicMax(&smaxd, dep[q]);
+        continue;
+      }
+      // A leaf: its chain from the head down (many_body.zig:125-132); the
+      // head's strength is loaded, a longer chain loads the rest here.
+      double strength = 0;
+      strength += hs[q];
+      for (uint32_t p = fc[q].x + fc[q].y - 1; p > fc[q].x;) {
+        --p;
+        strength += s[perm[p]];
+      }
+      dep[q] = NONE;
+      sx[l] = hx[q];
+      sy[l] = hy[q];
+      sv[l] = strength;
+      sreach[l] = truthy(strength) ? 1 : 0;
+      TCell t;
+      t.qx = hx[q];
+      t.qy = hy[q];
+      t.value = strength;
+      t.w2t = w2t[q];
+      t.skip = 1;
+      t.first = fc[q].x;
+      t.count = fc[q].y;
+      t.head = head[q];
+      t.flags = TC_LEAF | (truthy(strength) ? TC_REACH : 0);
+      t.pad[0] = t.pad[1] = t.pad[2] = 0;
+      tc[r + l] = t;
+    }
+    __syncthreads();
+    const uint32_t maxd = max(smaxd, d0);
+    for (uint32_t d = maxd + 1; d-- > d0;) {
+#pragma unroll
+      for (int q = 0; q < SUB_PER; ++q) {
+        if (dep[q] != d) continue;
+        const uint32_t l = tid + (uint32_t)q * SUB_THREADS;
+        // The four children's charges read together (slot 0 stands in for
+        // an absent child), then summed in order.
+        double cv[4], cx[4], cy[4];
+        uint32_t ck[4];
+        bool cr[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          ck[k] = (uint32_t)(kids[q] >> (10 * k)) & 1023u;
+          cv[k] = sv[ck[k]];
+          cx[k] = sx[ck[k]];
+          cy[k] = sy[ck[k]];
+          cr[k] = sreach[ck[k]] != 0;
+        }
+        double st = 0, weight = 0, x = 0, y = 0;
+        bool reach = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (ck[k] == 0) continue;
+          const double v = cv[k];
+          const double w = fabs(v);
+          if (w > 0) {  // `c = Math.abs(q.value)` is truthy
+            st += v;
+            weight += w;
+            x += w * cx[k];
+            y += w * cy[k];
+          }
+          reach = reach || cr[k];
+        }
+        const double qx = x / weight, qy = y / weight;
+        const bool rr = reach && truthy(st);
+        sx[l] = qx;
+        sy[l] = qy;
+        sv[l] = st;
+        sreach[l] = rr ? 1 : 0;
+        TCell t;
+        t.qx = qx;
+        t.qy = qy;
+        t.value = st;
+        t.w2t = w2t[q];
+        t.skip = clamp_skip(skp[q], r + l, len);
+        t.first = fc[q].x;
+        t.count = fc[q].y;
+        t.head = NONE;
+        t.flags = rr ? TC_REACH : 0;
+        t.pad[0] = t.pad[1] = t.pad[2] = 0;
+        tc[r + l] = t;
+      }
+      __syncthreads();
+    }
+    if (tid == 0) smaxd = 0;  // every thread read it before the last barrier
+    __syncthreads();
+  }
+}
+
+// The cells above the subtrees (their subtrees exceed ACC_SUB cells), one
+// workgroup, one depth at a time from the deepest; children come from the
+// subtrees' records (the previous launch) or from a deeper pass of this one.
+// Up to TOP_THREADS x TOP_PER cells keep their records in registers, so a
+// depth costs one round of child loads.
+constexpr int TOP_THREADS = 1024;
+constexpr int TOP_PER = 4;
+
+__device__ inline void acc_internal(TCell* __restrict__ tc, uint32_t c, const uint32_t* child, double w2t,
+                                    uint32_t skip, uint32_t first, uint32_t count) {
+  // The four children's charges are loaded together (an absent child loads
+  // the cell's own record and is skipped), then summed in order.
+  double cv[4], cx[4], cy[4];
+  uint32_t cf[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const TCell& k = tc[child[q] == NONE ? c : child[q]];
+    cv[q] = k.value;
+    cx[q] = k.qx;
+    cy[q] = k.qy;
+    cf[q] = k.flags;
+  }
+  double st = 0, weight = 0, x = 0, y = 0;
+  bool reach = false;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (child[q] == NONE) continue;
+    const double v = cv[q];
+    const double w = fabs(v);
+    if (w > 0) {  // `c = Math.abs(q.value)` is truthy (many_body.zig:112-121)
+      st += v;
+      weight += w;
+      x += w * cx[q];
+      y += w * cy[q];
+    }
+    reach = reach || (cf[q] & TC_REACH);
+  }
+  TCell t;
+  t.qx = x / weight;
+  t.qy = y / weight;
+  t.value = st;
+  t.w2t = w2t;
+  t.skip = skip;
+  t.first = first;
+  t.count = count;
+  t.head = NONE;
+  t.flags = (reach && truthy(st)) ? TC_REACH : 0;
+  t.pad[0] = t.pad[1] = t.pad[2] = 0;
+  tc[c] = t;
+}
+
+__global__ __launch_bounds__(TOP_THREADS) void k_acc_top(const Cell* __restrict__ cells, TCell* __restrict__ tc,
+                                                         const uint32_t* __restrict__ top,
+                                                         const uint32_t* __restrict__ cskip,
+                                                         const uint32_t* __restrict__ cdepth,
+                                                         const TreeMeta* __restrict__ meta, double theta2) {
+  const uint32_t ntop = meta->ntop, len = meta->len;
+  if (ntop == 0) return;
+  const uint32_t tid = threadIdx.x;
+  const bool held = ntop <= (uint32_t)(TOP_THREADS * TOP_PER);
+  uint32_t cc[TOP_PER], dep[TOP_PER], child[TOP_PER][4], skip[TOP_PER], first[TOP_PER], count[TOP_PER];
+  double w2t[TOP_PER];
+#pragma unroll
+  for (int q = 0; q < TOP_PER; ++q) {
+    const uint32_t k = tid + (uint32_t)q * TOP_THREADS;
+    dep[q] = NONE;
+    if (!held || k >= ntop) continue;
+    const uint32_t c = top[k];
+    const Cell cell = cells[c];
+    cc[q] = c;
+    dep[q] = cdepth[c];
+    for (int j = 0; j < 4; ++j) child[q][j] = cell.child[j];
+    const double w = cell.x1 - cell.x0;
+    w2t[q] = w * w / theta2;
+    skip[q] = clamp_skip(cskip[c], c, len);
+    first[q] = cell.first;
+    count[q] = cell.count;
+  }
+  for (uint32_t d = meta->topdepth + 1; d-- > 0;) {
+    if (held) {
+#pragma unroll
+      for (int q = 0; q < TOP_PER; ++q)
+        if (dep[q] == d) acc_internal(tc, cc[q], child[q], w2t[q], skip[q], first[q], count[q]);
+    } else {
+      for (uint32_t k = tid; k < ntop; k += TOP_THREADS) {
+        const uint32_t c = top[k];
+        if (cdepth[c] != d) continue;
+        const Cell cell = cells[c];
+        const double w = cell.x1 - cell.x0;
+        acc_internal(tc, c, cell.child, w * w / theta2, clamp_skip(cskip[c], c, len), cell.first, cell.count);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 __device__ inline double f64_of(uint32_t lo, uint32_t hi) {
   return __builtin_bit_cast(double, (uint64_t)hi << 32 | lo);
 }
@@ -384,6 +655,19 @@ hipError_t mb_accumulate(TreeBuffers& t, const double* x, const double* y, const
                          hipStream_t stream) {
   const uint32_t len = t.meta_host->len;
   if (len == 0) return hipSuccess;
+  if (t.meta_host->mixed == 0) {
+    // Subtree sizes and lists from the build (cells below key resolution,
+    // written by the serial split, have none: those trees go level by level).
+    // Enough blocks for one subtree each (subtrees average a few hundred
+    // cells); blocks past the count return at once.
+    uint32_t blocks = ceil_div(len, 128);
+    if (blocks > 65535) blocks = 65535;
+    hipLaunchKernelGGL(k_acc_subtrees, dim3(blocks), dim3(SUB_THREADS), 0, stream, t.cells, t.tcells, t.cut, t.cskip,
+                       t.cdepth, t.meta, t.perm, x, y, strengths, theta2);
+    hipLaunchKernelGGL(k_acc_top, dim3(1), dim3(TOP_THREADS), 0, stream, t.cells, t.tcells, t.top, t.cskip, t.cdepth,
+                       t.meta, theta2);
+    return hipGetLastError();
+  }
   const uint32_t nbins = t.meta_host->depth + 1;
   HIPCHECK(tree_depth_order(t, stream));
   uint32_t level_blocks = ceil_div(len, ACC_THREADS);

@@ -144,6 +144,9 @@ __global__ __launch_bounds__(BOX_THREADS) void k_box_final(uint32_t n, uint32_t 
     meta->depth = 0;
     meta->mixed = 0;
     meta->overflow = 0;
+    meta->ncut = 0;
+    meta->ntop = 0;
+    meta->topdepth = 0;
     if (r.x0 > r.x1 || r.y0 > r.y1) {  // no finite point (tree.zig:338)
       meta->m = 0;
       meta->valid = 0;
@@ -761,17 +764,38 @@ __global__ __launch_bounds__(EW_THREADS) void k_assign(const uint32_t* __restric
   }
 }
 
+// A slot in a list for each lane with `want`, one atomic per wave: the
+// wave's count is added once and each lane takes its rank among the wanting
+// lanes. Called by every active lane of the wave alike.
+__device__ inline uint32_t wave_append(uint32_t* counter, bool want) {
+  const uint64_t mask = __ballot(want);
+  if (mask == 0) return 0;
+  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+  const uint32_t lane = __lane_id();
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = (uint32_t)__shfl((int)base, (int)leader);
+  return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+}
+
 // Phase 2, one thread per cell: bounds by d3's midpoint walk along the key,
 // the point range, and children 0..3 from the digit boundaries inside it.
+// A cell's subtree ends where the group after its point range starts its
+// cells (preorder is the order of (first point, depth), and that group's
+// first cell is the one at the depth where it leaves the previous group's
+// path): skip = base[hi] - idx, or len - idx for the last range. Cells whose
+// subtree exceeds ACC_SUB go to `top`; their children within it, and a root
+// within it, go to `cut` (the subtrees accumulate sums in LDS).
 __global__ __launch_bounds__(EW_THREADS) void k_emit(const uint64_t* __restrict__ keys,
                                                      const uint32_t* __restrict__ mixed,
-                                                     const TreeMeta* __restrict__ meta,
+                                                     TreeMeta* __restrict__ meta,
                                                      const uint32_t* __restrict__ gend,
                                                      const uint32_t* __restrict__ leafdepth,
                                                      const uint32_t* __restrict__ base,
                                                      const uint32_t* __restrict__ cell_s,
                                                      const uint32_t* __restrict__ cell_d, Cell* __restrict__ cells,
-                                                     uint32_t* __restrict__ cdepth) {
+                                                     uint32_t* __restrict__ cdepth, uint32_t* __restrict__ cskip,
+                                                     uint2* __restrict__ cut, uint32_t* __restrict__ top) {
   const uint32_t idx = blockIdx.x * EW_THREADS + threadIdx.x;
   if (idx >= meta->len) return;
   const uint32_t s = cell_s[idx];
@@ -798,6 +822,10 @@ __global__ __launch_bounds__(EW_THREADS) void k_emit(const uint64_t* __restrict_
   c.count = hi - s;
   c.x0 = x0; c.y0 = y0; c.x1 = x1; c.y1 = y1;
   c.x = c.y = c.value = c.r = 0;
+  const uint32_t len = meta->len;
+  const uint32_t skip = (hi < m ? base[hi] : len) - idx;
+  const bool is_top = skip > ACC_SUB;
+  uint32_t child_skip[4] = {0, 0, 0, 0};
   if (internal) {
     const int dsh = 62 - 2 * d;
     uint32_t bnd[5];
@@ -807,10 +835,33 @@ __global__ __launch_bounds__(EW_THREADS) void k_emit(const uint64_t* __restrict_
     for (int q = 0; q < 4; ++q) {
       if (bnd[q] == bnd[q + 1]) continue;
       c.child[q] = bnd[q] == s ? idx + 1 : base[bnd[q]];
+      child_skip[q] = (bnd[q + 1] < m ? base[bnd[q + 1]] : len) - c.child[q];
     }
   }
   cells[idx] = c;
   cdepth[idx] = (uint32_t)d;
+  cskip[idx] = skip;
+  // List appends, one atomic per wave and list.
+  for (int q = 0; q < 4; ++q) {
+    const bool want = is_top && c.child[q] != NONE && child_skip[q] <= ACC_SUB;
+    const uint32_t slot = wave_append(&meta->ncut, want);
+    if (want) cut[slot] = make_uint2(c.child[q], child_skip[q] | (uint32_t)(d + 1) << CUT_SIZE_BITS);
+  }
+  const bool small_root = d == 0 && !is_top;
+  const uint32_t rslot = wave_append(&meta->ncut, small_root);
+  if (small_root) cut[rslot] = make_uint2(idx, skip);  // the root, depth 0
+  const uint32_t tslot = wave_append(&meta->ntop, is_top);
+  if (is_top) top[tslot] = idx;
+  if (__ballot(is_top)) {
+    // The wave's deepest top cell, bit by bit (top cells lie above depth
+    // KEY_LEVELS); the first active lane posts it.
+    uint32_t dm = 0;
+    for (int b = 5; b >= 0; --b) {
+      const uint32_t cand = dm | (1u << b);
+      if (__ballot(is_top && (uint32_t)d >= cand)) dm = cand;
+    }
+    if (__lane_id() == (uint32_t)__ffsll((unsigned long long)__ballot(true)) - 1) atomicMax(&meta->topdepth, dm);
+  }
 }
 
 }  // namespace
@@ -903,6 +954,9 @@ hipError_t tree_reserve_cells(TreeBuffers& t, uint32_t cells) {
   HIPCHECK(grow(t.cells, cap));
   HIPCHECK(grow(t.tcells, cap));
   HIPCHECK(grow(t.cdepth, cap));
+  HIPCHECK(grow(t.cskip, cap));
+  HIPCHECK(grow(t.cut, cap));
+  HIPCHECK(grow(t.top, cap));
   HIPCHECK(grow(t.cell_s, cap));
   HIPCHECK(grow(t.cell_d, cap));
   HIPCHECK(grow(t.dorder, cap));
@@ -913,7 +967,7 @@ hipError_t tree_reserve_cells(TreeBuffers& t, uint32_t cells) {
 void tree_free(TreeBuffers& t) {
   void* ptrs[] = {t.keys, t.keys_tmp, t.keys_all, t.perm, t.perm_tmp, t.rest, t.flags, t.offs, t.count,
                   t.base, t.dleft, t.gend, t.leafdepth, t.mixed, t.mixed_list, t.hist, t.partials,
-                  t.box_partials, t.deep_stack, t.cells, t.tcells, t.cdepth, t.dbins, t.dstart, t.dorder,
+                  t.box_partials, t.deep_stack, t.cells, t.tcells, t.cdepth, t.cskip, t.cut, t.top, t.dbins, t.dstart, t.dorder,
                   t.rs_glob, t.cell_s, t.cell_d, t.meta};
   for (void* p : ptrs) if (p) (void)hipFree(p);
   if (t.meta_host) (void)hipHostFree(t.meta_host);
@@ -975,7 +1029,7 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
   hipLaunchKernelGGL(k_assign, dim3(nb), dim3(EW_THREADS), 0, stream, t.flags, t.mixed, t.meta, t.dleft,
                      t.leafdepth, t.base, t.cell_s, t.cell_d);
   hipLaunchKernelGGL(k_emit, dim3(ceil_div(len, EW_THREADS)), dim3(EW_THREADS), 0, stream, t.keys, t.mixed, t.meta,
-                     t.gend, t.leafdepth, t.base, t.cell_s, t.cell_d, t.cells, t.cdepth);
+                     t.gend, t.leafdepth, t.base, t.cell_s, t.cell_d, t.cells, t.cdepth, t.cskip, t.cut, t.top);
   if (t.meta_host->mixed) {
     hipLaunchKernelGGL(k_deep<true>, dim3(1), dim3(64), 0, stream, t.keys, t.perm, t.perm_tmp, x, y, t.meta,
                        t.mixed_list, t.dleft, t.gend, t.count, t.base, t.deep_stack, t.cells, t.cdepth);
