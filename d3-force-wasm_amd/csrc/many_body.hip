@@ -1,12 +1,17 @@
 // forceManyBody, exact mode, on gfx950 (reference: src/core/many_body.zig,
 // itself d3-force's manyBody.js over the quadtree).
 //
-// Accumulate (visitAfter): cells are bucketed by depth and processed from
-// the deepest level up, one launch per level; a leaf sums its chain from the
-// head down (many_body.zig:125-132) and an internal cell combines children
-// 0..3 in order (many_body.zig:112-121), so every sum has d3's operands in
-// d3's order. Kernel boundaries order the levels, with no inter-workgroup
-// atomics (whose cache maintenance spans all eight XCDs' L2s).
+// Accumulate (visitAfter): a leaf sums its chain from the head down
+// (many_body.zig:125-132) and an internal cell combines children 0..3 in
+// order (many_body.zig:112-121), so every sum has d3's operands in d3's
+// order. Cells are numbered in preorder, so a subtree is a contiguous range
+// whose size the tree build records; subtrees of at most ACC_SUB cells are
+// summed by one workgroup each in LDS, depth by depth under workgroup
+// barriers (k_acc_subtrees), and the few cells above them by one workgroup
+// (k_acc_top). Kernel boundaries order the two, with no inter-workgroup
+// atomics (whose cache maintenance spans all eight XCDs' L2s). Trees with
+// cells below key resolution (whose serial split records no subtree sizes)
+// go level by level, one launch per depth (k_accumulate_level).
 //
 // Traversal (visit): one lane per node, nodes in Morton order. Cells are
 // numbered in d3's visit preorder (children 0..3), so a node's visit sequence
