@@ -386,6 +386,10 @@ hipError_t grid_collide_iteration(const double* x, const double* y, double* vx, 
 
 // ---- fmm.hip ----
 constexpr uint32_t FMM_LEAF = 32, FMM_LEVELS = 16;  // fmm.zig:43-47
+// Cells of at most this many points get their multipole straight from their
+// points (one wave each); the larger ones (at most m / FMM_DIRECT per depth)
+// by M2M from their children.
+constexpr uint32_t FMM_DIRECT = 2048;
 constexpr uint32_t FMM_MAX_ORDER = 24;
 constexpr uint32_t FMM_MAX_STEPS = 2 * FMM_LEVELS + 2;  // dual-tree rounds
 
@@ -424,6 +428,7 @@ struct FmmParams {
   uint32_t near_items, near_slots, near_combines;  // near-field work items, partial-sum slots, combines
   uint32_t abort;                          // a device-side size check failed: later kernels skip the tick
   uint32_t ncg;                            // cells, or 0 when aborted (the count the per-cell kernels use)
+  uint32_t big_n[FMM_LEVELS + 1];          // cells of more than FMM_DIRECT points per depth
 };
 
 struct FmmBuffers {
@@ -439,20 +444,25 @@ struct FmmBuffers {
   size_t exp_cap = 0;                      // complex coefficients mexp and lexp hold
   FCell* cells = nullptr;
   double* bbox = nullptr;                  // 4 per cell
-  uint32_t *split = nullptr, *split_off = nullptr;  // per level: child counts, offsets
-  uint32_t* bounds = nullptr;              // 5 per cell of a level
+  uint32_t* parent = nullptr;              // per cell (cell_cap): its parent, NONE for a root
+  uint16_t* fl = nullptr;                  // per point: first depth it starts a cell | (leaf depth + 1) << 8
+  uint32_t blk_cap = 0;                    // blocks of the cell build the per-block arrays hold
+  uint32_t *blk_cnt = nullptr, *blk_fb = nullptr, *blk_off = nullptr, *blk_nb = nullptr;  // depth-major
+  uint32_t big_cap = 0;                    // per depth
+  uint32_t* big = nullptr;                 // cells of more than FMM_DIRECT points, per depth
   float *mexp = nullptr, *lexp = nullptr;  // 2(p+1) per cell
   double *px = nullptr, *py = nullptr, *ux = nullptr, *uy = nullptr;
   float* ps = nullptr;
   uint2 *front_a = nullptr, *front_b = nullptr;
   uint32_t long_cap = 0, slot_cap = 0, item_cap = 0;
   // launch shapes (no result depends on them): the dual traversal's grid, the
-  // per-depth build grids, the blocks sharing near-field work items
-  uint32_t dual_blocks = 1024, level_blocks = 256, near_extra = 256;
-  // depths the eager cell build runs (the last build's deepest level + 2;
-  // a deeper tree is rebuilt with every level)
+  // grid of the per-cell upward pass (4 waves a block), the blocks sharing
+  // near-field work items
+  uint32_t dual_blocks = 1024, level_blocks = 1024, near_extra = 256;
+  // the last eager tree's deepest level + 2: the depths the capturable form's
+  // loops cover (a deeper tree aborts the replay)
   uint32_t build_levels = FMM_LEVELS;
-  uint32_t graph_levels = FMM_LEVELS;  // depths of the last build (the capturable form's bound)
+  uint32_t graph_levels = FMM_LEVELS;
   uint32_t* long_list = nullptr;           // targets whose segments the long-segment sort orders
   uint4* near_items = nullptr;             // (leaf, target batch, chunk or ALL, slot)
   uint4* near_combines = nullptr;          // (leaf, target batch, first slot, chunks)

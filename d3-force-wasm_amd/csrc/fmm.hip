@@ -244,11 +244,11 @@ __global__ __launch_bounds__(EWF) void k_forest_compact(uint32_t n, const FmmPar
 }
 
 // One block: each graph's first point in Morton order (the sorted keys hold
-// the graphs in order) and a root cell for every graph with points and a
-// finite extent, roots numbered in graph order (level 0 of the forest).
+// the graphs in order) and the index of its root cell for every graph with
+// points and a finite extent, roots numbered in graph order (level 0 of the
+// forest; k_cells_emit writes them).
 __global__ __launch_bounds__(1024) void k_forest_roots(FmmGraph* __restrict__ gp, uint32_t graphs,
-                                                       FmmParams* __restrict__ fp, FCell* __restrict__ cells,
-                                                       uint32_t cell_cap, TreeMeta* __restrict__ meta) {
+                                                       FmmParams* __restrict__ fp, TreeMeta* __restrict__ meta) {
   __shared__ uint32_t wsum[2][16];
   __shared__ uint32_t carry[2];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -279,7 +279,6 @@ __global__ __launch_bounds__(1024) void k_forest_roots(FmmGraph* __restrict__ gp
       const uint32_t first = before[0] + inc[0] - m, idx = before[1] + inc[1] - root;
       gp[g].point_off = first;
       gp[g].root = root ? idx : NONE;
-      if (root && idx < cell_cap) cells[idx] = FCell{0, 0, 0, first, m, 0, 0, 0, g};
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -290,9 +289,6 @@ __global__ __launch_bounds__(1024) void k_forest_roots(FmmGraph* __restrict__ gp
   }
   if (threadIdx.x == 0) {
     fp->roots = carry[1];
-    fp->level_n[0] = carry[1];
-    fp->level_off[0] = 0;
-    fp->level_off[1] = carry[1];
     // prepareFast leaves no exact tree behind (tree.zig:351-356).
     for (int k = 0; k < 4; ++k) meta->extent[k] = gp[0].extent[k];
     meta->len = 0;
@@ -300,71 +296,279 @@ __global__ __launch_bounds__(1024) void k_forest_roots(FmmGraph* __restrict__ gp
   }
 }
 
-// Quadrant boundaries of each splitting cell of depth d (fmm.zig:694-708):
-// a cell of more than FMM_LEAF points above the last level splits into its
-// non-empty quadrants, in digit order.
-__global__ __launch_bounds__(EWF) void k_fmm_split(const uint64_t* __restrict__ keys, const FCell* __restrict__ cells,
-                                                   const FmmParams* __restrict__ fp, uint32_t d,
-                                                   uint32_t* __restrict__ split, uint32_t* __restrict__ bounds,
-                                                   uint32_t cell_cap) {
-  const uint32_t lo = fp->level_off[d], cnt = fp->level_n[d];
-  const int shift = 2 * (int)(FMM_LEVELS - 1 - d);
-  for (uint32_t t = blockIdx.x * EWF + threadIdx.x; t < cnt; t += gridDim.x * EWF) {
-    uint32_t nc = 0;
-    if (lo + t < cell_cap) {
-      const FCell c = cells[lo + t];
-      if (c.count > FMM_LEAF && d < FMM_LEVELS) {
-        uint32_t b[5];
-        b[0] = c.first;
-        b[4] = c.first + c.count;
-        for (uint32_t q = 1; q < 4; ++q) {
-          uint32_t l = b[q - 1], h = b[4];
-          while (l < h) {
-            const uint32_t mid = l + (h - l) / 2;
-            if (((uint32_t)(keys[mid] >> shift) & 3u) < q) l = mid + 1; else h = mid;
-          }
-          b[q] = l;
-        }
-        for (int q = 0; q < 5; ++q) bounds[5 * (size_t)t + q] = b[q];
-        for (int q = 0; q < 4; ++q) nc += b[q + 1] > b[q];
+// ---- cells (fmm.zig:685-711) from the sorted keys, every depth at once
+//
+// The reference builds level-order cells breadth first: a cell of more than
+// FMM_LEAF points above depth FMM_LEVELS splits into its non-empty quadrants.
+// With the keys sorted, the cells follow from per-point quantities:
+//   b(k)  = the quad digits keys k - 1 and k share (-1 at a graph's first
+//           point, where a new root begins); point k starts cells at depths
+//           b(k) + 1 and deeper;
+//   W(j)  = the digits keys j and j + FMM_LEAF share (-1 past the end or
+//           across graphs): the depth-d cell holding k has more than FMM_LEAF
+//           points exactly when W(j) >= d for some j in [k - FMM_LEAF, k];
+//   L(k)  = min(FMM_LEVELS, 1 + max of those W(j)): the depth of the leaf
+//           holding k.
+// So point k starts one cell at every depth d with b(k) + 1 <= d <= L(k),
+// and a cell's level-order index is the number of cells of its depth that
+// start at earlier points. The build is three launches: per-point b and L
+// with per-block counts and first breaks per depth (k_cells_mark); per depth
+// a scan of the counts and a suffix minimum of the breaks over the blocks
+// (k_cells_scan); then every cell written with its count, parent and first
+// child (k_cells_emit). Children after the first add themselves to their
+// parent's nchild in k_fmm_up_direct.
+constexpr uint32_t CB_THREADS = 256, CB_PTS = 1024;  // points per block of the cell build (4 per thread)
+constexpr uint32_t CB_DEPTHS = FMM_LEVELS + 1;       // depths 0 .. FMM_LEVELS
+constexpr uint32_t CB_HALO = FMM_LEAF + 1;           // keys kept before the block's first point
+
+// Shared quad digits of two keys of one graph (FMM_LEVELS when equal), -1
+// for keys of different graphs.
+__device__ inline int key_lcp(uint64_t a, uint64_t b) {
+  if ((a >> 32) != (b >> 32)) return -1;
+  const uint32_t x = (uint32_t)a ^ (uint32_t)b;
+  return x == 0 ? (int)FMM_LEVELS : (int)(__clz(x) >> 1);
+}
+
+// The key prefix that names a depth-d cell (the graph id above it).
+__device__ inline uint64_t key_prefix(uint64_t key, uint32_t d) { return key >> (2 * (FMM_LEVELS - d)); }
+
+__global__ __launch_bounds__(CB_THREADS) void k_cells_mark(const uint64_t* __restrict__ keys,
+                                                           const FmmGraph* __restrict__ gp,
+                                                           const FmmParams* __restrict__ fp, uint32_t nblk,
+                                                           uint16_t* __restrict__ fl, uint32_t* __restrict__ blk_cnt,
+                                                           uint32_t* __restrict__ blk_fb) {
+  __shared__ uint64_t sk[CB_HALO + CB_PTS + FMM_LEAF];  // keys [base - CB_HALO, base + CB_PTS + FMM_LEAF)
+  __shared__ int8_t sw[FMM_LEAF + CB_PTS];              // W(j), j in [base - FMM_LEAF, base + CB_PTS)
+  __shared__ uint32_t cnt[CB_DEPTHS], fb[CB_DEPTHS];
+  const uint32_t m = fp->m_total;
+  const int64_t base = (int64_t)blockIdx.x * CB_PTS;
+  for (uint32_t i = threadIdx.x; i < CB_HALO + CB_PTS + FMM_LEAF; i += CB_THREADS) {
+    const int64_t k = base - CB_HALO + i;
+    sk[i] = (k >= 0 && k < (int64_t)m) ? keys[k] : 0;
+  }
+  if (threadIdx.x < CB_DEPTHS) {
+    cnt[threadIdx.x] = 0;
+    fb[threadIdx.x] = NONE;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < FMM_LEAF + CB_PTS; i += CB_THREADS) {
+    const int64_t j = base - FMM_LEAF + i;  // sk index of key j: i + 1
+    sw[i] = (int8_t)((j >= 0 && j + FMM_LEAF < (int64_t)m) ? key_lcp(sk[i + 1], sk[i + 1 + FMM_LEAF]) : -1);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t t = q * CB_THREADS + threadIdx.x;
+    const int64_t k = base + t;
+    uint32_t first = 1, leafp1 = 0;  // no cell
+    if (k < (int64_t)m) {
+      const uint64_t key = sk[CB_HALO + t];
+      const int b = k == 0 ? -1 : key_lcp(sk[CB_HALO + t - 1], key);
+      for (uint32_t d = (uint32_t)(b + 1); d < CB_DEPTHS; ++d) atomicMin(&fb[d], (uint32_t)k);
+      if (gp[key >> 32].ok) {
+        int mw = -1;
+        for (uint32_t s = 0; s <= FMM_LEAF; ++s) mw = max(mw, (int)sw[t + s]);
+        first = (uint32_t)(b + 1);
+        leafp1 = (uint32_t)min((int)FMM_LEVELS, mw + 1) + 1;
+        for (uint32_t d = first; d < leafp1; ++d) atomicAdd(&cnt[d], 1u);
       }
+      fl[k] = (uint16_t)(first | leafp1 << 8);
     }
-    split[t] = nc;
+  }
+  __syncthreads();
+  if (threadIdx.x < CB_DEPTHS) {
+    blk_cnt[(size_t)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+    blk_fb[(size_t)threadIdx.x * nblk + blockIdx.x] = fb[threadIdx.x];
   }
 }
 
-__global__ __launch_bounds__(EWF) void k_fmm_children(FCell* __restrict__ cells, FmmParams* __restrict__ fp,
-                                                      uint32_t d, const uint32_t* __restrict__ split,
-                                                      const uint32_t* __restrict__ off,
-                                                      const uint32_t* __restrict__ bounds, uint32_t cell_cap) {
-  const uint32_t lo = fp->level_off[d], cnt = fp->level_n[d];
-  const uint32_t base = fp->level_off[d + 1];
-  if (blockIdx.x == 0 && threadIdx.x == 0) fp->level_off[d + 2] = base + fp->level_n[d + 1];
-  for (uint32_t t = blockIdx.x * EWF + threadIdx.x; t < cnt; t += gridDim.x * EWF) {
-    const uint32_t nc = split[t];
-    if (nc == 0) continue;
-    const uint32_t c0 = base + off[t];
-    cells[lo + t].child0 = c0;
-    cells[lo + t].nchild = nc;
-    const uint32_t graph = cells[lo + t].graph;
-    uint32_t k = 0;
+// Block d: the exclusive scan of depth d's per-block cell counts (the
+// level-order rank of each block's first depth-d cell) and the suffix
+// minimum of its first breaks (the first point at or after each block that
+// ends a depth-d cell). Block 0 also clears the big-cell counts.
+__global__ __launch_bounds__(1024) void k_cells_scan(const uint32_t* __restrict__ blk_cnt,
+                                                     const uint32_t* __restrict__ blk_fb, uint32_t nblk,
+                                                     uint32_t* __restrict__ blk_off, uint32_t* __restrict__ blk_nb,
+                                                     FmmParams* __restrict__ fp, uint32_t cell_cap) {
+  __shared__ uint32_t ws[16];
+  __shared__ uint32_t carry;
+  const uint32_t d = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t* cnt = blk_cnt + (size_t)d * nblk;
+  uint32_t* off = blk_off + (size_t)d * nblk;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < nblk; b0 += 1024) {
+    const uint32_t b = b0 + threadIdx.x;
+    const uint32_t v = b < nblk ? cnt[b] : 0;
+    uint32_t inc = v;
+    for (int s = 1; s < 64; s <<= 1) {
+      const uint32_t u = __shfl_up(inc, s, 64);
+      if (lane >= s) inc += u;
+    }
+    if (lane == 63) ws[wave] = inc;
+    __syncthreads();
+    uint32_t before = carry, total = 0;
+    for (int w = 0; w < 16; ++w) {
+      if (w < wave) before += ws[w];
+      total += ws[w];
+    }
+    if (b < nblk) off[b] = before + inc - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) fp->level_n[d] = carry;
+  // suffix minimum, right to left
+  const uint32_t* fb = blk_fb + (size_t)d * nblk;
+  uint32_t* nb = blk_nb + (size_t)d * nblk;
+  if (threadIdx.x == 0) carry = NONE;
+  __syncthreads();
+  for (uint32_t e = nblk; e > 0;) {
+    const uint32_t s0 = e > 1024 ? e - 1024 : 0;
+    const uint32_t b = s0 + threadIdx.x;  // this pass covers [s0, e)
+    uint32_t v = b < e ? fb[b] : NONE;
+    for (int s = 1; s < 64; s <<= 1) {
+      const uint32_t u = __shfl_down(v, s, 64);
+      if (lane + s < 64) v = min(v, u);
+    }
+    if (lane == 0) ws[wave] = v;
+    __syncthreads();
+    uint32_t after = carry;
+    for (int w = wave + 1; w < 16; ++w) after = min(after, ws[w]);
+    v = min(v, after);
+    if (b < e) nb[b] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t lo = carry;
+      for (int w = 0; w < 16; ++w) lo = min(lo, ws[w]);
+      carry = lo;
+    }
+    __syncthreads();
+    e = s0;
+  }
+  if (d == 0 && threadIdx.x < CB_DEPTHS) fp->big_n[threadIdx.x] = 0;
+  if (d == 0 && threadIdx.x == 0) fp->level_n[FMM_LEVELS + 1] = 0;
+}
+
+// The cells. Block 0 publishes the level offsets; every block ranks its
+// points' cells per depth with wave ballots (points in the order q, wave,
+// lane) and writes them: count from the block's keys when the cell ends in
+// the block, else from the next break after it; the parent from this
+// thread's previous cell (a first child) or from the rank of the cell of the
+// depth above holding the point (a later child); cells of more than
+// FMM_DIRECT points also go to their depth's list for k_fmm_up_big.
+__global__ __launch_bounds__(CB_THREADS) void k_cells_emit(const uint64_t* __restrict__ keys,
+                                                           const uint16_t* __restrict__ fl, uint32_t nblk,
+                                                           const uint32_t* __restrict__ blk_off,
+                                                           const uint32_t* __restrict__ blk_nb,
+                                                           FmmParams* __restrict__ fp, FCell* __restrict__ cells,
+                                                           uint32_t* __restrict__ parent, uint32_t cell_cap,
+                                                           uint32_t* __restrict__ big, uint32_t big_cap) {
+  __shared__ uint64_t sk[CB_PTS];
+  __shared__ uint32_t loff[CB_DEPTHS + 1];
+  __shared__ uint32_t woff[CB_DEPTHS][16];  // per depth: exclusive offsets of the (q, wave) groups
+  const uint32_t m = fp->m_total;
+  const uint32_t base = blockIdx.x * CB_PTS;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t lt = (1ull << lane) - 1;
+  if (threadIdx.x == 0) {
+    uint32_t o = 0;
+    for (uint32_t d = 0; d < CB_DEPTHS; ++d) {
+      loff[d] = o;
+      o += fp->level_n[d];
+    }
+    loff[CB_DEPTHS] = o;
+    if (blockIdx.x == 0) {
+      for (uint32_t d = 0; d <= CB_DEPTHS; ++d) fp->level_off[d] = loff[d];
+      fp->level_off[FMM_LEVELS + 2] = o;
+      if (o > cell_cap) fp->cell_overflow = 1;
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < CB_PTS; i += CB_THREADS) sk[i] = base + i < m ? keys[base + i] : 0;
+  uint32_t first[4], leafp1[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t k = base + q * CB_THREADS + threadIdx.x;
+    const uint32_t v = k < m ? fl[k] : 1u;
+    first[q] = v & 0xffu;
+    leafp1[q] = v >> 8;
+  }
+  for (uint32_t d = 0; d < CB_DEPTHS; ++d)
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const uint32_t b0 = bounds[5 * (size_t)t + q], b1 = bounds[5 * (size_t)t + q + 1];
-      if (b1 == b0) continue;
-      const uint32_t idx = c0 + k++;
-      if (idx < cell_cap) cells[idx] = FCell{0, 0, 0, b0, b1 - b0, 0, 0, d + 1, graph};
-      else fp->cell_overflow = 1;
+      const uint64_t bal = __ballot(first[q] <= d && d < leafp1[q]);
+      if (lane == 0) woff[d][q * 4 + wave] = (uint32_t)__popcll(bal);
+    }
+  __syncthreads();
+  if (threadIdx.x < CB_DEPTHS) {
+    uint32_t o = blk_off[(size_t)threadIdx.x * nblk + blockIdx.x];
+    for (int g = 0; g < 16; ++g) {
+      const uint32_t c = woff[threadIdx.x][g];
+      woff[threadIdx.x][g] = o;
+      o += c;
+    }
+  }
+  __syncthreads();
+  const uint32_t total = loff[CB_DEPTHS];
+  uint32_t prev_idx[4], prev_excl[4];
+  for (uint32_t d = 0; d < CB_DEPTHS; ++d) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const bool has = first[q] <= d && d < leafp1[q];
+      const uint64_t bal = __ballot(has);
+      const uint32_t excl = woff[d][q * 4 + wave] + (uint32_t)__popcll(bal & lt);  // depth-d cells before the point
+      if (has) {
+        const uint32_t t = q * CB_THREADS + threadIdx.x;
+        const uint32_t k = base + t;
+        const uint32_t idx = loff[d] + excl;
+        const uint64_t key = sk[t];
+        const uint64_t P = key_prefix(key, d);
+        // the first point after k outside the cell: the block's keys, else
+        // the first break at or after the next block
+        uint32_t end;
+        if (t + 1 >= CB_PTS || base + t + 1 >= m || key_prefix(sk[t + 1], d) != P) {
+          end = k + 1;
+          if (t + 1 >= CB_PTS && base + CB_PTS < m)
+            end = blockIdx.x + 1 < nblk ? blk_nb[(size_t)d * nblk + blockIdx.x + 1] : NONE;
+        } else {
+          uint32_t a = t + 1, step = 1, b;
+          for (;;) {
+            b = a + step;
+            if (b >= CB_PTS || base + b >= m || key_prefix(sk[b], d) != P) break;
+            a = b;
+            step <<= 1;
+          }
+          if (b > CB_PTS) b = CB_PTS;
+          if (base + b > m) b = m - base;
+          while (b - a > 1) {
+            const uint32_t mid = a + (b - a) / 2;
+            if (key_prefix(sk[mid], d) == P) a = mid; else b = mid;
+          }
+          end = base + b;
+          if (b == CB_PTS && base + CB_PTS < m)  // the cell runs into the next block
+            end = blockIdx.x + 1 < nblk ? blk_nb[(size_t)d * nblk + blockIdx.x + 1] : NONE;
+        }
+        if (end > m) end = m;
+        uint32_t par = NONE;
+        if (d > 0) par = d > first[q] ? prev_idx[q] : loff[d - 1] + prev_excl[q] - 1;
+        const uint32_t split = d + 1 < leafp1[q] ? 1u : 0u;
+        if (total <= cell_cap) {
+          cells[idx] = FCell{0, 0, 0, k, end - k, 0, split, d, (uint32_t)(key >> 32)};
+          parent[idx] = par;
+          if (d > first[q]) cells[par].child0 = idx;  // this thread wrote the parent one depth above
+          if (end - k > FMM_DIRECT) {
+            const uint32_t s = atomicAdd(&fp->big_n[d], 1u);
+            if (s < big_cap) big[(size_t)d * big_cap + s] = idx;
+          }
+        }
+        prev_idx[q] = idx;
+      }
+      prev_excl[q] = excl + (has ? 1u : 0u);  // depth-d cells up to and including the point
     }
   }
 }
 
-// After a build that stopped at depth d_end: when no cell reached d_end, every
-// deeper level is empty and the offsets after it equal its end.
-__global__ void k_fmm_levels_close(FmmParams* __restrict__ fp, uint32_t d_end) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || fp->level_n[d_end] != 0) return;
-  for (uint32_t d = d_end + 1; d <= FMM_LEVELS + 1; ++d) fp->level_n[d] = 0;
-  for (uint32_t d = d_end + 2; d <= FMM_LEVELS + 2; ++d) fp->level_off[d] = fp->level_off[d_end + 1];
-}
 
 // fmm.zig:668-682: Morton-ordered raw positions, unit-frame positions and
 // strength * alpha (NaN -> 0), each point in its graph's frame.
@@ -475,17 +679,104 @@ __device__ __forceinline__ void fmm_up_cell(uint32_t c, FCell* __restrict__ cell
   for (int k = 0; k < Q; ++k) mexp[(size_t)c * Q + k] = m[k];
 }
 
-// P2M / M2M of the cells of depth d (counts read on the device).
+// Geometry (fmm.zig:721-754) and radius-scaled multipole of a cell of up to
+// FMM_DIRECT points, one wave per cell, straight from its points: the bounds
+// of its points are the union of its children's (min and max are exact), and
+// the multipole of its points is what M2M from its children sums to. Lanes
+// take every 64th point; the lane sums are combined in a fixed butterfly.
+// Lane 0 also adds a later child to its parent's nchild (k_cells_emit gave
+// each splitting cell its first child).
 template <int Q>
-__global__ __launch_bounds__(EWF) void k_fmm_up(FCell* __restrict__ cells, double* __restrict__ bbox,
-                                                const double* __restrict__ ux, const double* __restrict__ uy,
-                                                const float* __restrict__ ps, float2* __restrict__ mexp,
-                                                const FmmParams* __restrict__ fp, uint32_t d) {
+__global__ __launch_bounds__(256) void k_fmm_up_direct(FCell* __restrict__ cells, const uint32_t* __restrict__ parent,
+                                                       double* __restrict__ bbox, const double* __restrict__ ux,
+                                                       const double* __restrict__ uy, const float* __restrict__ ps,
+                                                       float2* __restrict__ mexp, const FmmParams* __restrict__ fp) {
   if (fp->abort) return;
-  const uint32_t lo = fp->level_off[d], cnt = fp->level_n[d];
-  for (uint32_t t = blockIdx.x * EWF + threadIdx.x; t < cnt; t += gridDim.x * EWF)
-    fmm_up_cell<Q>(lo + t, cells, bbox, ux, uy, ps, mexp);
+  const uint32_t nc = fp->ncg;
+  const int lane = threadIdx.x & 63;
+  const uint32_t waves = gridDim.x * 4;
+  for (uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6); c < nc; c += waves) {
+    const uint32_t first = cells[c].first, count = cells[c].count;
+    if (lane == 0) {
+      const uint32_t p = parent[c];
+      if (p != NONE && cells[p].first != first) atomicAdd(&cells[p].nchild, 1u);
+    }
+    if (count > FMM_DIRECT) continue;
+    const double inf = __builtin_huge_val();
+    double x0 = inf, y0 = inf, x1 = -inf, y1 = -inf;
+    for (uint32_t k = first + lane; k < first + count; k += 64) {
+      const double u = ux[k], v = uy[k];
+      x0 = zmin(x0, u);
+      x1 = zmax(x1, u);
+      y0 = zmin(y0, v);
+      y1 = zmax(y1, v);
+    }
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+      x0 = zmin(x0, __shfl_xor(x0, s, 64));
+      x1 = zmax(x1, __shfl_xor(x1, s, 64));
+      y0 = zmin(y0, __shfl_xor(y0, s, 64));
+      y1 = zmax(y1, __shfl_xor(y1, s, 64));
+    }
+    const double cx = (x0 + x1) / 2, cy = (y0 + y1) / 2;
+    const double dx = x1 - x0, dy = y1 - y0;
+    const double r = 0.5 * __builtin_sqrt(dx * dx + dy * dy);
+    if (lane == 0) {
+      bbox[4 * (size_t)c + 0] = x0;
+      bbox[4 * (size_t)c + 1] = y0;
+      bbox[4 * (size_t)c + 2] = x1;
+      bbox[4 * (size_t)c + 3] = y1;
+      cells[c].cx = cx;
+      cells[c].cy = cy;
+      cells[c].r = r;
+    }
+    const double inv = r > 0 ? 1 / r : 0;
+    float2 mm[Q];
+#pragma unroll
+    for (int o = 0; o < Q; ++o) mm[o] = make_float2(0.f, 0.f);
+    for (uint32_t k = first + lane; k < first + count; k += 64) {
+      const float s = ps[k];
+      const float2 z = make_float2((float)((ux[k] - cx) * inv), (float)((uy[k] - cy) * inv));
+      float2 zp = make_float2(s, 0.f);
+      mm[0].x += s;
+#pragma unroll
+      for (int o = 1; o < Q; ++o) {
+        zp = cmul(zp, z);
+        mm[o].x += zp.x;
+        mm[o].y += zp.y;
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < Q; ++o)
+#pragma unroll
+      for (int s = 32; s > 0; s >>= 1) {
+        mm[o].x += __shfl_xor(mm[o].x, s, 64);
+        mm[o].y += __shfl_xor(mm[o].y, s, 64);
+      }
+#pragma unroll
+    for (int o = 0; o < Q; ++o)
+      if (lane == o) mexp[(size_t)c * Q + o] = mm[o];
+  }
 }
+
+// The cells of more than FMM_DIRECT points (few: at most m / FMM_DIRECT per
+// depth), one workgroup, deepest depth first: M2M from their children
+// (fmm_up_cell), or P2M when such a cell is a leaf of the last depth.
+template <int Q>
+__global__ __launch_bounds__(1024) void k_fmm_up_big(FCell* __restrict__ cells, double* __restrict__ bbox,
+                                                     const double* __restrict__ ux, const double* __restrict__ uy,
+                                                     const float* __restrict__ ps, float2* __restrict__ mexp,
+                                                     const uint32_t* __restrict__ big, uint32_t big_cap,
+                                                     const FmmParams* __restrict__ fp) {
+  if (fp->abort) return;
+  for (int d = (int)FMM_LEVELS; d >= 0; --d) {
+    const uint32_t cnt = min(fp->big_n[d], big_cap);
+    for (uint32_t i = threadIdx.x; i < cnt; i += 1024)
+      fmm_up_cell<Q>(big[(size_t)d * big_cap + i], cells, bbox, ux, uy, ps, mexp);
+    __syncthreads();
+  }
+}
+
 
 // ---------------------------------------------------------------- dual tree
 
@@ -874,46 +1165,60 @@ __global__ __launch_bounds__(64) void k_m2l(const FCell* __restrict__ cells, con
 
 // ---------------------------------------------------------------- downward
 
-// L2L (fmm.zig:823-849) from each cell of one depth into its children.
+// L2L (fmm.zig:823-849) straight from every ancestor into each leaf, one
+// thread per leaf: the local expansion a leaf evaluates is its own M2L sum
+// plus each ancestor's M2L sum shifted to the leaf's centre, which is what
+// the reference's level-by-level L2L chain adds up to (shifting a polynomial
+// is exact). With d the offset of the leaf's centre from the ancestor's,
+//   L^_m(leaf) += (rho_leaf / rho_A)^m sum_{l >= m} C(l, m) L^_l(A) (d / rho_A)^(l-m).
+// Only leaves are written, so the ancestors' sums are read unchanged.
 template <int Q>
-__device__ __forceinline__ void fmm_down_cell(uint32_t c, const FCell* __restrict__ cells,
-                                              float2* __restrict__ lexp) {
-  const FCell cell = cells[c];
-  if (cell.nchild == 0) return;
-  float2 lp[Q];
+__global__ __launch_bounds__(EWF) void k_fmm_down_leaf(const FCell* __restrict__ cells,
+                                                       const uint32_t* __restrict__ parent,
+                                                       const uint32_t* __restrict__ m2l_cnt,
+                                                       float2* __restrict__ lexp, const FmmParams* __restrict__ fp) {
+  if (fp->abort) return;
+  const uint32_t nc = fp->ncg;
+  for (uint32_t c = blockIdx.x * EWF + threadIdx.x; c < nc; c += gridDim.x * EWF) {
+    const FCell cc = cells[c];
+    if (cc.nchild != 0) continue;
+    float2 acc[Q];
 #pragma unroll
-  for (int l = 0; l < Q; ++l) lp[l] = lexp[(size_t)c * Q + l];
-  const double inv = cell.r > 0 ? 1 / cell.r : 0;
-  for (uint32_t ch = cell.child0; ch < cell.child0 + cell.nchild; ++ch) {
-    const FCell cc = cells[ch];
-    const float tr = (float)(cc.r * inv);
-    const float2 e = make_float2((float)((cc.cx - cell.cx) * inv), (float)((cc.cy - cell.cy) * inv));
-    float2 ep[Q];
-    ep[0] = make_float2(1.f, 0.f);
+    for (int k = 0; k < Q; ++k) acc[k] = make_float2(0.f, 0.f);
+    bool any = false;
+    for (uint32_t a = parent[c]; a != NONE; a = parent[a]) {
+      if (m2l_cnt[a] == 0) continue;  // no M2L sum of its own
+      any = true;
+      const FCell ca = cells[a];
+      const double inv = ca.r > 0 ? 1 / ca.r : 0;
+      const float tr = (float)(cc.r * inv);
+      const float2 e = make_float2((float)((cc.cx - ca.cx) * inv), (float)((cc.cy - ca.cy) * inv));
+      float2 lp[Q], ep[Q];
 #pragma unroll
-    for (int k = 1; k < Q; ++k) ep[k] = cmul(ep[k - 1], e);
-    float tm = 1.f;
-    // L^_m(c) += (rho_c / rho_P)^m sum_{l >= m} C(l, m) L^_l(P) (d / rho_P)^(l-m)
+      for (int l = 0; l < Q; ++l) lp[l] = lexp[(size_t)a * Q + l];
+      ep[0] = make_float2(1.f, 0.f);
 #pragma unroll
-    for (int mm = 0; mm < Q; ++mm) {
-      float2 acc = make_float2(0.f, 0.f);
+      for (int k = 1; k < Q; ++k) ep[k] = cmul(ep[k - 1], e);
+      float tm = 1.f;
 #pragma unroll
-      for (int l = mm; l < Q; ++l) acc = cfma(cscale(lp[l], binom(l, mm)), ep[l - mm], acc);
-      float2 old = lexp[(size_t)ch * Q + mm];
-      old.x = fmaf(acc.x, tm, old.x);
-      old.y = fmaf(acc.y, tm, old.y);
-      lexp[(size_t)ch * Q + mm] = old;
-      tm *= tr;
+      for (int mm = 0; mm < Q; ++mm) {
+        float2 s = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int l = mm; l < Q; ++l) s = cfma(cscale(lp[l], binom(l, mm)), ep[l - mm], s);
+        acc[mm].x = fmaf(s.x, tm, acc[mm].x);
+        acc[mm].y = fmaf(s.y, tm, acc[mm].y);
+        tm *= tr;
+      }
+    }
+    if (!any) continue;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      float2 v = lexp[(size_t)c * Q + k];
+      v.x += acc[k].x;
+      v.y += acc[k].y;
+      lexp[(size_t)c * Q + k] = v;
     }
   }
-}
-
-template <int Q>
-__global__ __launch_bounds__(EWF) void k_fmm_down(const FCell* __restrict__ cells, float2* __restrict__ lexp,
-                                                  const FmmParams* __restrict__ fp, uint32_t d) {
-  if (fp->abort) return;
-  const uint32_t lo = fp->level_off[d], cnt = fp->level_n[d];
-  for (uint32_t t = blockIdx.x * EWF + threadIdx.x; t < cnt; t += gridDim.x * EWF) fmm_down_cell<Q>(lo + t, cells, lexp);
 }
 
 // ---------------------------------------------------------------- near field + L2P
@@ -1349,6 +1654,14 @@ hipError_t grow(T*& p, size_t n) {
   return hipMalloc((void**)&p, (n > 0 ? n : 1) * sizeof(T));
 }
 
+// The cell array and its per-cell parents (contents not kept).
+hipError_t grow_cells(FmmBuffers& f, uint32_t cap) {
+  HIPCHECK(grow(f.cells, cap));
+  HIPCHECK(grow(f.parent, cap));
+  f.cell_cap = cap;
+  return hipSuccess;
+}
+
 hipError_t upload_binomials() {
   static bool done = false;
   if (done) return hipSuccess;
@@ -1375,14 +1688,17 @@ hipError_t reserve_points(FmmBuffers& f, uint32_t n) {
   HIPCHECK(grow(f.ux, cap));
   HIPCHECK(grow(f.uy, cap));
   HIPCHECK(grow(f.ps, cap));
-  HIPCHECK(grow(f.split, cap));
-  HIPCHECK(grow(f.split_off, cap));
-  HIPCHECK(grow(f.bounds, (size_t)5 * cap));
+  HIPCHECK(grow(f.fl, cap));
+  const uint32_t blk = ceil_div(cap, CB_PTS);
+  HIPCHECK(grow(f.blk_cnt, (size_t)CB_DEPTHS * blk));
+  HIPCHECK(grow(f.blk_fb, (size_t)CB_DEPTHS * blk));
+  HIPCHECK(grow(f.blk_off, (size_t)CB_DEPTHS * blk));
+  HIPCHECK(grow(f.blk_nb, (size_t)CB_DEPTHS * blk));
+  f.blk_cap = blk;
+  f.big_cap = cap / FMM_DIRECT + 1;
+  HIPCHECK(grow(f.big, (size_t)CB_DEPTHS * f.big_cap));
   f.point_cap = cap;
-  if (f.cell_cap < cap / 4 + 4096) {
-    HIPCHECK(grow(f.cells, cap / 4 + 4096));
-    f.cell_cap = cap / 4 + 4096;
-  }
+  if (f.cell_cap < cap / 4 + 4096) HIPCHECK(grow_cells(f, cap / 4 + 4096));
   return hipSuccess;
 }
 
@@ -1449,10 +1765,14 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
     const uint32_t b = ceil_div(count, per);
     return b > 0 ? b : 1;
   };
-  // upward
-  for (int d = (int)maxd; d >= 0; --d)
-    hipLaunchKernelGGL(k_fmm_up<Q>, dim3(grid(g ? 0 : P.level_n[d], EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream,
-                       f.cells, f.bbox, f.ux, f.uy, f.ps, (float2*)f.mexp, f.params, (uint32_t)d);
+  // upward: cells of up to FMM_DIRECT points from their points (one wave
+  // each, level_blocks blocks of 4 waves at most), then the larger ones by
+  // M2M in one workgroup
+  const uint32_t up_blocks = g ? f.level_blocks : std::max(1u, std::min(f.level_blocks, ceil_div(nc, 4)));
+  hipLaunchKernelGGL(k_fmm_up_direct<Q>, dim3(up_blocks), dim3(256), 0, stream, f.cells, f.parent, f.bbox, f.ux, f.uy,
+                     f.ps, (float2*)f.mexp, f.params);
+  hipLaunchKernelGGL(k_fmm_up_big<Q>, dim3(1), dim3(1024), 0, stream, f.cells, f.bbox, f.ux, f.uy, f.ps,
+                     (float2*)f.mexp, f.big, f.big_cap, f.params);
   // interaction lists (pair keys: target << nbits | source)
   uint32_t nbits = 1;
   while (((uint64_t)1 << nbits) < f.cell_cap) ++nbits;
@@ -1552,9 +1872,8 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   // far field
   hipLaunchKernelGGL(k_m2l<Q>, dim3(grid(nc, 1, CAP_TARGET_BLOCKS)), dim3(64), 0, stream, f.cells, f.m2l_src,
                      f.m2l_off, f.m2l_cnt, (const float2*)f.mexp, (float2*)f.lexp, f.params);
-  for (uint32_t d = 0; d < maxd; ++d)
-    hipLaunchKernelGGL(k_fmm_down<Q>, dim3(grid(g ? 0 : P.level_n[d], EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream,
-                       f.cells, (float2*)f.lexp, f.params, d);
+  hipLaunchKernelGGL(k_fmm_down_leaf<Q>, dim3(grid(nc, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.cells, f.parent,
+                     f.m2l_cnt, (float2*)f.lexp, f.params);
   // near field and evaluation
   const float dmin2 = (float)a.dmin2, dmax2 = (float)a.dmax2;
   hipLaunchKernelGGL(k_near_items, dim3(grid(nc, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.cells, ncp,
@@ -1584,7 +1903,7 @@ using Orders = OrderTable<2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17
 }  // namespace
 
 void fmm_free(FmmBuffers& f) {
-  void* ptrs[] = {f.graphs, f.seeds, f.node_off_dev, f.chunk_lo, f.chunk_hi, f.chunk_first, f.box_parts, f.cells, f.bbox, f.split, f.split_off, f.bounds, f.mexp, f.lexp, f.px, f.py, f.ux, f.uy, f.ps,
+  void* ptrs[] = {f.graphs, f.seeds, f.node_off_dev, f.chunk_lo, f.chunk_hi, f.chunk_first, f.box_parts, f.cells, f.parent, f.bbox, f.fl, f.blk_cnt, f.blk_fb, f.blk_off, f.blk_nb, f.big, f.mexp, f.lexp, f.px, f.py, f.ux, f.uy, f.ps,
                   f.front_a, f.front_b, f.long_list, f.near_items, f.near_combines, f.near_partial, f.m2l, f.p2p, f.m2l_src, f.p2p_src, f.m2l_off, f.m2l_cnt, f.p2p_off,
                   f.p2p_cnt, f.cursor, f.scan_partials, f.params};
   for (void* p : ptrs)
@@ -1659,10 +1978,7 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
   HIPCHECK(tree_reserve(t, n > 0 ? n : 1));
   HIPCHECK(reserve_points(f, n));
   HIPCHECK(reserve_forest(f, a, stream));
-  if (f.cell_cap < a.graphs + 4096) {
-    HIPCHECK(grow(f.cells, a.graphs + 4096));
-    f.cell_cap = a.graphs + 4096;
-  }
+  if (f.cell_cap < a.graphs + 4096) HIPCHECK(grow_cells(f, a.graphs + 4096));
   const uint32_t nb = ceil_div(n > 0 ? n : 1, EWF);
   // Boxes, frames and keys of every graph; NaN points compacted out; one sort.
   hipLaunchKernelGGL(k_forest_reset, dim3(1), dim3(64), 0, stream, f.params, n);
@@ -1681,34 +1997,26 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
   for (uint32_t v = a.graphs - 1; v > 0; v >>= 8) ++digits;
   HIPCHECK(sort_digits(t.hist, t.rs_glob, t.keys, t.perm, t.keys_tmp, t.perm_tmp, n > 0 ? n : 1, &f.params->m_total,
                        0, digits, stream));
-  // Cells, one depth at a time; grown and rebuilt if they outgrow the buffer.
-  // The eager form builds the depths the last tree needed plus two and
-  // rebuilds with every depth when cells reach the last one.
-  uint32_t d_end = std::min(FMM_LEVELS, f.build_levels);
-  f.graph_levels = d_end;  // the capturable form builds these depths; a deeper tree aborts the replay
+  // Cells of every depth in three launches; grown and rebuilt if they
+  // outgrow the buffer. The capturable form's loops cover the last eager
+  // tree's depths plus two (a deeper tree aborts the replay).
+  const uint32_t d_end = std::min(FMM_LEVELS, f.build_levels);
+  f.graph_levels = d_end;
+  const uint32_t nblk = ceil_div(n > 0 ? n : 1, CB_PTS);
+  if (nblk > f.blk_cap) HIPFAIL(hipErrorInvalidValue, "cell build blocks beyond the reserved arrays");
+  hipLaunchKernelGGL(k_forest_roots, dim3(1), dim3(1024), 0, stream, f.graphs, a.graphs, f.params, t.meta);
   for (;;) {
-    hipLaunchKernelGGL(k_forest_roots, dim3(1), dim3(1024), 0, stream, f.graphs, a.graphs, f.params, f.cells,
-                       f.cell_cap, t.meta);
-    for (uint32_t d = 0; d < d_end; ++d) {
-      hipLaunchKernelGGL(k_fmm_split, dim3(f.level_blocks), dim3(EWF), 0, stream, t.keys, f.cells, f.params, d, f.split,
-                         f.bounds, f.cell_cap);
-      HIPCHECK(scan_u32(f.split, f.split_off, n > 0 ? n : 1, &f.params->level_n[d], t.partials,
-                        &f.params->level_n[d + 1], stream));
-      hipLaunchKernelGGL(k_fmm_children, dim3(f.level_blocks), dim3(EWF), 0, stream, f.cells, f.params, d, f.split,
-                         f.split_off, f.bounds, f.cell_cap);
-    }
-    if (d_end < FMM_LEVELS) hipLaunchKernelGGL(k_fmm_levels_close, dim3(1), dim3(64), 0, stream, f.params, d_end);
+    hipLaunchKernelGGL(k_cells_mark, dim3(nblk), dim3(CB_THREADS), 0, stream, t.keys, f.graphs, f.params, nblk, f.fl,
+                       f.blk_cnt, f.blk_fb);
+    hipLaunchKernelGGL(k_cells_scan, dim3(CB_DEPTHS), dim3(1024), 0, stream, f.blk_cnt, f.blk_fb, nblk, f.blk_off,
+                       f.blk_nb, f.params, f.cell_cap);
+    hipLaunchKernelGGL(k_cells_emit, dim3(nblk), dim3(CB_THREADS), 0, stream, t.keys, f.fl, nblk, f.blk_off,
+                       f.blk_nb, f.params, f.cells, f.parent, f.cell_cap, f.big, f.big_cap);
     if (a.graph) break;  // sizes checked on the device (k_fmm_guard)
     HIPCHECK(hipMemcpyAsync(f.params_host, f.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
-    if (!f.params_host->cell_overflow && d_end < FMM_LEVELS && f.params_host->level_n[d_end] != 0) {
-      d_end = FMM_LEVELS;  // the tree reaches deeper than the last one: rebuild with every level
-      continue;
-    }
     if (!f.params_host->cell_overflow) break;
-    const uint32_t cap = std::max(2 * f.cell_cap, f.params_host->level_off[FMM_LEVELS + 1] + 4096);
-    HIPCHECK(grow(f.cells, cap));
-    f.cell_cap = cap;
+    HIPCHECK(grow_cells(f, std::max(2 * f.cell_cap, f.params_host->level_off[FMM_LEVELS + 1] + 4096)));
     hipLaunchKernelGGL(k_forest_reset_overflow, dim3(1), dim3(64), 0, stream, f.params);
   }
   if (!a.graph) {
@@ -1738,10 +2046,7 @@ hipError_t fmm_reserve_capacity(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a,
   HIPCHECK(tree_reserve(t, a.n > 0 ? a.n : 1));
   HIPCHECK(reserve_points(f, a.n));
   HIPCHECK(reserve_forest(f, a, stream));
-  if (f.cell_cap < cells) {
-    HIPCHECK(grow(f.cells, cells));
-    f.cell_cap = cells;
-  }
+  if (f.cell_cap < cells) HIPCHECK(grow_cells(f, cells));
   HIPCHECK(reserve_cells(f, f.cell_cap, p + 1));
   const uint32_t long_need = f.p2p_cap / NEAR_CHUNK + f.m2l_cap / SEG_MAX + 2;
   if (long_need > f.long_cap) {

@@ -141,7 +141,7 @@ class Batch:
     def autotune(self, ticks=2):
         """Times `ticks` ticks per candidate launch shape (include/d3fgpu.h,
         batch_autotune) and keeps the fastest; returns the shapes chosen
-        (dual-traversal blocks, build blocks, near-field item blocks) and the
+        (dual-traversal blocks, upward-pass blocks, near-field item blocks) and the
         per-tick milliseconds of the candidates. The ticks (one untimed, then
         `ticks` per candidate) advance the batch."""
         chosen = (ctypes.c_uint32 * 3)()

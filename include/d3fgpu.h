@@ -211,7 +211,7 @@ int32_t batch_sync(uint32_t b, uint32_t direction);
 int32_t batch_set_scalar(uint32_t b, uint32_t which, double v);
 double batch_get_scalar(uint32_t b, uint32_t which);
 int32_t batch_tick(uint32_t b, uint32_t ticks);
-/* Autotunes the FMM's launch shapes (dual-traversal grid, per-depth build
+/* Autotunes the FMM's launch shapes (dual-traversal grid, upward-pass
  * grid, blocks sharing near-field items; no result depends on them) by timing
  * `ticks` real ticks per candidate (after one untimed tick), one shape at a
  * time; keeps the fastest.
