@@ -700,7 +700,7 @@ EXPORT int32_t batch_autotune(uint32_t h, uint32_t ticks, uint32_t* chosen, floa
   if (b->graphs == 0 || ticks == 0) return B_BAD_ARGUMENT;
   b->drop_graph();  // a captured tick holds the launch shapes
   static const uint32_t dual[] = {256, 512, 1024, 2048, 4096};
-  static const uint32_t level[] = {256, 512, 1024, 2048, 4096};
+  static const uint32_t level[] = {4096, 8192, 16384, 32768, 65536};
   static const uint32_t extra[] = {64, 128, 256, 512};
   struct Knob { uint32_t* field; const uint32_t* values; int count; };
   Knob knobs[3] = {{&b->fmm.dual_blocks, dual, 5}, {&b->fmm.level_blocks, level, 5}, {&b->fmm.near_extra, extra, 4}};

@@ -429,6 +429,7 @@ struct FmmParams {
   uint32_t abort;                          // a device-side size check failed: later kernels skip the tick
   uint32_t ncg;                            // cells, or 0 when aborted (the count the per-cell kernels use)
   uint32_t big_n[FMM_LEVELS + 1];          // cells of more than FMM_DIRECT points per depth
+  uint32_t mid_n;                          // cells of the upward pass's wave-per-cell list
 };
 
 struct FmmBuffers {
@@ -450,6 +451,8 @@ struct FmmBuffers {
   uint32_t *blk_cnt = nullptr, *blk_fb = nullptr, *blk_off = nullptr, *blk_nb = nullptr;  // depth-major
   uint32_t big_cap = 0;                    // per depth
   uint32_t* big = nullptr;                 // cells of more than FMM_DIRECT points, per depth
+  uint32_t mid_cap = 0;
+  uint32_t* mid = nullptr;                 // cells the upward pass gives a wave each
   float *mexp = nullptr, *lexp = nullptr;  // 2(p+1) per cell
   double *px = nullptr, *py = nullptr, *ux = nullptr, *uy = nullptr;
   float* ps = nullptr;
@@ -458,7 +461,7 @@ struct FmmBuffers {
   // launch shapes (no result depends on them): the dual traversal's grid, the
   // grid of the per-cell upward pass (4 waves a block), the blocks sharing
   // near-field work items
-  uint32_t dual_blocks = 1024, level_blocks = 1024, near_extra = 256;
+  uint32_t dual_blocks = 1024, level_blocks = 16384, near_extra = 256;
   // the last eager tree's deepest level + 2: the depths the capturable form's
   // loops cover (a deeper tree aborts the replay)
   uint32_t build_levels = FMM_LEVELS;

@@ -144,19 +144,30 @@ __global__ __launch_bounds__(EWF) void k_forest_box(const double* __restrict__ x
   if (threadIdx.x == 0) parts[blockIdx.x] = sh[0];
 }
 
-// One thread per graph: its box, the fast extent (tree.zig:357-359), the unit
-// frame (fmm.zig:253-270) and the scaled distance limits (fmm.zig:272-281).
-// A graph of fewer than min_n nodes is given no point (no tree).
+// One wave per graph: its box from its chunks' boxes (the lanes combine
+// every 64th chunk, then a tree in LDS; the combine is order-free, ties going
+// to the lower node), the fast extent (tree.zig:357-359), the unit frame
+// (fmm.zig:253-270) and the scaled distance limits (fmm.zig:272-281). A
+// graph of fewer than min_n nodes is given no point (no tree).
 __global__ __launch_bounds__(64) void k_forest_graphs(const BoxPart* __restrict__ parts,
                                                       const uint32_t* __restrict__ chunk_first, uint32_t graphs,
                                                       double dmin2, double dmax2, const uint64_t* __restrict__ seeds,
                                                       const uint32_t* __restrict__ node_off, uint32_t min_n,
                                                       FmmGraph* __restrict__ gp, FmmParams* __restrict__ fp) {
-  const uint32_t g = blockIdx.x * 64 + threadIdx.x;
+  __shared__ BoxPart sh[64];
+  const uint32_t g = blockIdx.x;
   if (g >= graphs) return;
   BoxPart b = bempty();
   if (node_off[g + 1] - node_off[g] >= min_n)
-    for (uint32_t c = chunk_first[g]; c < chunk_first[g + 1]; ++c) bcombine(b, parts[c]);
+    for (uint32_t c = chunk_first[g] + threadIdx.x; c < chunk_first[g + 1]; c += 64) bcombine(b, parts[c]);
+  sh[threadIdx.x] = b;
+  __syncthreads();
+  for (int k = 32; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) bcombine(sh[threadIdx.x], sh[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  b = sh[0];
   FmmGraph r;
   memset(&r, 0, sizeof(r));
   const double nan = __builtin_nan(""), inf = __builtin_huge_val();
@@ -316,10 +327,15 @@ __global__ __launch_bounds__(1024) void k_forest_roots(FmmGraph* __restrict__ gp
 // a scan of the counts and a suffix minimum of the breaks over the blocks
 // (k_cells_scan); then every cell written with its count, parent and first
 // child (k_cells_emit). Children after the first add themselves to their
-// parent's nchild in k_fmm_up_direct.
+// parent's nchild in k_fmm_gather.
 constexpr uint32_t CB_THREADS = 256, CB_PTS = 1024;  // points per block of the cell build (4 per thread)
 constexpr uint32_t CB_DEPTHS = FMM_LEVELS + 1;       // depths 0 .. FMM_LEVELS
 constexpr uint32_t CB_HALO = FMM_LEAF + 1;           // keys kept before the block's first point
+constexpr uint32_t UP_SMALL = 64;                    // upward pass: cells of at most this many points get 8 lanes
+// cells of more than UP_SMALL points starting in one block: they are
+// disjoint, so at most CB_PTS / (UP_SMALL + 1) lie inside it and one more
+// runs past its end, per depth
+constexpr uint32_t CB_MIDS = CB_DEPTHS * (CB_PTS / (UP_SMALL + 1) + 1);
 
 // Shared quad digits of two keys of one graph (FMM_LEVELS when equal), -1
 // for keys of different graphs.
@@ -385,7 +401,7 @@ __global__ __launch_bounds__(CB_THREADS) void k_cells_mark(const uint64_t* __res
 // Block d: the exclusive scan of depth d's per-block cell counts (the
 // level-order rank of each block's first depth-d cell) and the suffix
 // minimum of its first breaks (the first point at or after each block that
-// ends a depth-d cell). Block 0 also clears the big-cell counts.
+// ends a depth-d cell). Block 0 also clears the counts of the cell lists.
 __global__ __launch_bounds__(1024) void k_cells_scan(const uint32_t* __restrict__ blk_cnt,
                                                      const uint32_t* __restrict__ blk_fb, uint32_t nblk,
                                                      uint32_t* __restrict__ blk_off, uint32_t* __restrict__ blk_nb,
@@ -448,7 +464,10 @@ __global__ __launch_bounds__(1024) void k_cells_scan(const uint32_t* __restrict_
     e = s0;
   }
   if (d == 0 && threadIdx.x < CB_DEPTHS) fp->big_n[threadIdx.x] = 0;
-  if (d == 0 && threadIdx.x == 0) fp->level_n[FMM_LEVELS + 1] = 0;
+  if (d == 0 && threadIdx.x == 0) {
+    fp->level_n[FMM_LEVELS + 1] = 0;
+    fp->mid_n = 0;
+  }
 }
 
 // The cells. Block 0 publishes the level offsets; every block ranks its
@@ -457,17 +476,20 @@ __global__ __launch_bounds__(1024) void k_cells_scan(const uint32_t* __restrict_
 // the block, else from the next break after it; the parent from this
 // thread's previous cell (a first child) or from the rank of the cell of the
 // depth above holding the point (a later child); cells of more than
-// FMM_DIRECT points also go to their depth's list for k_fmm_up_big.
+// FMM_DIRECT points also go to their depth's list for k_fmm_up_big, cells of
+// UP_SMALL + 1 .. FMM_DIRECT points to the list for k_fmm_up_mid.
 __global__ __launch_bounds__(CB_THREADS) void k_cells_emit(const uint64_t* __restrict__ keys,
                                                            const uint16_t* __restrict__ fl, uint32_t nblk,
                                                            const uint32_t* __restrict__ blk_off,
                                                            const uint32_t* __restrict__ blk_nb,
                                                            FmmParams* __restrict__ fp, FCell* __restrict__ cells,
                                                            uint32_t* __restrict__ parent, uint32_t cell_cap,
-                                                           uint32_t* __restrict__ big, uint32_t big_cap) {
+                                                           uint32_t* __restrict__ big, uint32_t big_cap,
+                                                           uint32_t* __restrict__ mid, uint32_t mid_cap) {
   __shared__ uint64_t sk[CB_PTS];
   __shared__ uint32_t loff[CB_DEPTHS + 1];
   __shared__ uint32_t woff[CB_DEPTHS][16];  // per depth: exclusive offsets of the (q, wave) groups
+  __shared__ uint32_t mids[CB_MIDS], n_mid, mid_base;  // the block's part of the mid-size list
   const uint32_t m = fp->m_total;
   const uint32_t base = blockIdx.x * CB_PTS;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -479,6 +501,7 @@ __global__ __launch_bounds__(CB_THREADS) void k_cells_emit(const uint64_t* __res
       o += fp->level_n[d];
     }
     loff[CB_DEPTHS] = o;
+    n_mid = 0;
     if (blockIdx.x == 0) {
       for (uint32_t d = 0; d <= CB_DEPTHS; ++d) fp->level_off[d] = loff[d];
       fp->level_off[FMM_LEVELS + 2] = o;
@@ -560,6 +583,8 @@ __global__ __launch_bounds__(CB_THREADS) void k_cells_emit(const uint64_t* __res
           if (end - k > FMM_DIRECT) {
             const uint32_t s = atomicAdd(&fp->big_n[d], 1u);
             if (s < big_cap) big[(size_t)d * big_cap + s] = idx;
+          } else if (end - k > UP_SMALL) {
+            mids[atomicAdd(&n_mid, 1u)] = idx;
           }
         }
         prev_idx[q] = idx;
@@ -567,6 +592,11 @@ __global__ __launch_bounds__(CB_THREADS) void k_cells_emit(const uint64_t* __res
       prev_excl[q] = excl + (has ? 1u : 0u);  // depth-d cells up to and including the point
     }
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && n_mid) mid_base = atomicAdd(&fp->mid_n, n_mid);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n_mid; i += CB_THREADS)
+    if (mid_base + i < mid_cap) mid[mid_base + i] = mids[i];
 }
 
 
@@ -579,20 +609,30 @@ __global__ __launch_bounds__(EWF) void k_fmm_gather(const uint32_t* __restrict__
                                                     const double* __restrict__ alpha_dev, uint32_t n,
                                                     double* __restrict__ px, double* __restrict__ py,
                                                     double* __restrict__ ux, double* __restrict__ uy,
-                                                    float* __restrict__ ps) {
+                                                    float* __restrict__ ps, FCell* __restrict__ cells,
+                                                    const uint32_t* __restrict__ parent) {
   if (alpha_dev) alpha = *alpha_dev;
   const uint32_t j = blockIdx.x * EWF + threadIdx.x;
-  if (j >= n || j >= fp->m_total) return;
-  const FmmGraph& G = gp[keys[j] >> 32];
-  if (!G.ok) return;
-  const uint32_t i = perm[j];
-  const double x = xs[i], y = ys[i];
-  px[j] = x;
-  py[j] = y;
-  const double s = st[i] * alpha;
-  ps[j] = s != s ? 0.0f : (float)s;
-  ux[j] = (x - G.ox) * G.inv_w;
-  uy[j] = (y - G.oy) * G.inv_w;
+  if (j < n && j < fp->m_total) {
+    const FmmGraph& G = gp[keys[j] >> 32];
+    if (G.ok) {
+      const uint32_t i = perm[j];
+      const double x = xs[i], y = ys[i];
+      px[j] = x;
+      py[j] = y;
+      const double s = st[i] * alpha;
+      ps[j] = s != s ? 0.0f : (float)s;
+      ux[j] = (x - G.ox) * G.inv_w;
+      uy[j] = (y - G.oy) * G.inv_w;
+    }
+  }
+  // k_cells_emit gave each splitting cell its first child; every later child
+  // (one starting after its parent's first point) adds itself.
+  const uint32_t nc = fp->ncg;
+  for (uint32_t c = j; c < nc; c += gridDim.x * EWF) {
+    const uint32_t p = parent[c];
+    if (p != NONE && cells[p].first != cells[c].first) atomicAdd(&cells[p].nchild, 1u);
+  }
 }
 
 // ---------------------------------------------------------------- upward
@@ -680,82 +720,104 @@ __device__ __forceinline__ void fmm_up_cell(uint32_t c, FCell* __restrict__ cell
 }
 
 // Geometry (fmm.zig:721-754) and radius-scaled multipole of a cell of up to
-// FMM_DIRECT points, one wave per cell, straight from its points: the bounds
-// of its points are the union of its children's (min and max are exact), and
-// the multipole of its points is what M2M from its children sums to. Lanes
-// take every 64th point; the lane sums are combined in a fixed butterfly.
-// Lane 0 also adds a later child to its parent's nchild (k_cells_emit gave
-// each splitting cell its first child).
+// FMM_DIRECT points straight from its points, by a group of G lanes: the
+// bounds of its points are the union of its children's (min and max are
+// exact), and the multipole of its points is what M2M from its children sums
+// to. Lane l of the group takes points l, l + G, ...; the lane sums are
+// combined in a fixed butterfly.
+template <int Q, int G>
+__device__ __forceinline__ void up_direct(uint32_t c, uint32_t first, uint32_t count, int lane,
+                                          FCell* __restrict__ cells, double* __restrict__ bbox,
+                                          const double* __restrict__ ux, const double* __restrict__ uy,
+                                          const float* __restrict__ ps, float2* __restrict__ mexp) {
+  const double inf = __builtin_huge_val();
+  double x0 = inf, y0 = inf, x1 = -inf, y1 = -inf;
+  for (uint32_t k = first + lane; k < first + count; k += G) {
+    const double u = ux[k], v = uy[k];
+    x0 = zmin(x0, u);
+    x1 = zmax(x1, u);
+    y0 = zmin(y0, v);
+    y1 = zmax(y1, v);
+  }
+#pragma unroll
+  for (int s = G / 2; s > 0; s >>= 1) {
+    x0 = zmin(x0, __shfl_xor(x0, s, 64));
+    x1 = zmax(x1, __shfl_xor(x1, s, 64));
+    y0 = zmin(y0, __shfl_xor(y0, s, 64));
+    y1 = zmax(y1, __shfl_xor(y1, s, 64));
+  }
+  const double cx = (x0 + x1) / 2, cy = (y0 + y1) / 2;
+  const double dx = x1 - x0, dy = y1 - y0;
+  const double r = 0.5 * __builtin_sqrt(dx * dx + dy * dy);
+  if (lane == 0) {
+    bbox[4 * (size_t)c + 0] = x0;
+    bbox[4 * (size_t)c + 1] = y0;
+    bbox[4 * (size_t)c + 2] = x1;
+    bbox[4 * (size_t)c + 3] = y1;
+    cells[c].cx = cx;
+    cells[c].cy = cy;
+    cells[c].r = r;
+  }
+  const double inv = r > 0 ? 1 / r : 0;
+  float2 mm[Q];
+#pragma unroll
+  for (int o = 0; o < Q; ++o) mm[o] = make_float2(0.f, 0.f);
+  for (uint32_t k = first + lane; k < first + count; k += G) {
+    const float s = ps[k];
+    const float2 z = make_float2((float)((ux[k] - cx) * inv), (float)((uy[k] - cy) * inv));
+    float2 zp = make_float2(s, 0.f);
+    mm[0].x += s;
+#pragma unroll
+    for (int o = 1; o < Q; ++o) {
+      zp = cmul(zp, z);
+      mm[o].x += zp.x;
+      mm[o].y += zp.y;
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < Q; ++o)
+#pragma unroll
+    for (int s = G / 2; s > 0; s >>= 1) {
+      mm[o].x += __shfl_xor(mm[o].x, s, 64);
+      mm[o].y += __shfl_xor(mm[o].y, s, 64);
+    }
+#pragma unroll
+  for (int o = 0; o < Q; ++o)
+    if (o % G == lane) mexp[(size_t)c * Q + o] = mm[o];
+}
+
+// Every cell of at most UP_SMALL points (the leaves and the smallest
+// splitting cells), 8 lanes a cell, 8 cells a wave.
 template <int Q>
-__global__ __launch_bounds__(256) void k_fmm_up_direct(FCell* __restrict__ cells, const uint32_t* __restrict__ parent,
-                                                       double* __restrict__ bbox, const double* __restrict__ ux,
-                                                       const double* __restrict__ uy, const float* __restrict__ ps,
-                                                       float2* __restrict__ mexp, const FmmParams* __restrict__ fp) {
+__global__ __launch_bounds__(256) void k_fmm_up_small(FCell* __restrict__ cells, double* __restrict__ bbox,
+                                                      const double* __restrict__ ux, const double* __restrict__ uy,
+                                                      const float* __restrict__ ps, float2* __restrict__ mexp,
+                                                      const FmmParams* __restrict__ fp) {
   if (fp->abort) return;
   const uint32_t nc = fp->ncg;
-  const int lane = threadIdx.x & 63;
-  const uint32_t waves = gridDim.x * 4;
-  for (uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6); c < nc; c += waves) {
-    const uint32_t first = cells[c].first, count = cells[c].count;
-    if (lane == 0) {
-      const uint32_t p = parent[c];
-      if (p != NONE && cells[p].first != first) atomicAdd(&cells[p].nchild, 1u);
-    }
-    if (count > FMM_DIRECT) continue;
-    const double inf = __builtin_huge_val();
-    double x0 = inf, y0 = inf, x1 = -inf, y1 = -inf;
-    for (uint32_t k = first + lane; k < first + count; k += 64) {
-      const double u = ux[k], v = uy[k];
-      x0 = zmin(x0, u);
-      x1 = zmax(x1, u);
-      y0 = zmin(y0, v);
-      y1 = zmax(y1, v);
-    }
-#pragma unroll
-    for (int s = 32; s > 0; s >>= 1) {
-      x0 = zmin(x0, __shfl_xor(x0, s, 64));
-      x1 = zmax(x1, __shfl_xor(x1, s, 64));
-      y0 = zmin(y0, __shfl_xor(y0, s, 64));
-      y1 = zmax(y1, __shfl_xor(y1, s, 64));
-    }
-    const double cx = (x0 + x1) / 2, cy = (y0 + y1) / 2;
-    const double dx = x1 - x0, dy = y1 - y0;
-    const double r = 0.5 * __builtin_sqrt(dx * dx + dy * dy);
-    if (lane == 0) {
-      bbox[4 * (size_t)c + 0] = x0;
-      bbox[4 * (size_t)c + 1] = y0;
-      bbox[4 * (size_t)c + 2] = x1;
-      bbox[4 * (size_t)c + 3] = y1;
-      cells[c].cx = cx;
-      cells[c].cy = cy;
-      cells[c].r = r;
-    }
-    const double inv = r > 0 ? 1 / r : 0;
-    float2 mm[Q];
-#pragma unroll
-    for (int o = 0; o < Q; ++o) mm[o] = make_float2(0.f, 0.f);
-    for (uint32_t k = first + lane; k < first + count; k += 64) {
-      const float s = ps[k];
-      const float2 z = make_float2((float)((ux[k] - cx) * inv), (float)((uy[k] - cy) * inv));
-      float2 zp = make_float2(s, 0.f);
-      mm[0].x += s;
-#pragma unroll
-      for (int o = 1; o < Q; ++o) {
-        zp = cmul(zp, z);
-        mm[o].x += zp.x;
-        mm[o].y += zp.y;
-      }
-    }
-#pragma unroll
-    for (int o = 0; o < Q; ++o)
-#pragma unroll
-      for (int s = 32; s > 0; s >>= 1) {
-        mm[o].x += __shfl_xor(mm[o].x, s, 64);
-        mm[o].y += __shfl_xor(mm[o].y, s, 64);
-      }
-#pragma unroll
-    for (int o = 0; o < Q; ++o)
-      if (lane == o) mexp[(size_t)c * Q + o] = mm[o];
+  const int lane = threadIdx.x & 7;
+  const uint32_t groups = gridDim.x * 32;
+  for (uint32_t c0 = blockIdx.x * 32; c0 < nc; c0 += groups) {
+    const uint32_t c = c0 + (threadIdx.x >> 3);  // the groups of a wave stay together
+    const bool mine = c < nc;
+    const uint32_t first = mine ? cells[c].first : 0, count = mine ? cells[c].count : 0;
+    if (mine && count <= UP_SMALL) up_direct<Q, 8>(c, first, count, lane, cells, bbox, ux, uy, ps, mexp);
+  }
+}
+
+// The cells of UP_SMALL + 1 .. FMM_DIRECT points (listed by k_cells_emit),
+// one wave each.
+template <int Q>
+__global__ __launch_bounds__(256) void k_fmm_up_mid(FCell* __restrict__ cells, double* __restrict__ bbox,
+                                                    const double* __restrict__ ux, const double* __restrict__ uy,
+                                                    const float* __restrict__ ps, float2* __restrict__ mexp,
+                                                    const uint32_t* __restrict__ mid, uint32_t mid_cap,
+                                                    const FmmParams* __restrict__ fp) {
+  if (fp->abort) return;
+  const uint32_t cnt = min(fp->mid_n, mid_cap);
+  for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < cnt; i += gridDim.x * 4) {
+    const uint32_t c = mid[i];
+    up_direct<Q, 64>(c, cells[c].first, cells[c].count, threadIdx.x & 63, cells, bbox, ux, uy, ps, mexp);
   }
 }
 
@@ -768,10 +830,13 @@ __global__ __launch_bounds__(1024) void k_fmm_up_big(FCell* __restrict__ cells, 
                                                      const float* __restrict__ ps, float2* __restrict__ mexp,
                                                      const uint32_t* __restrict__ big, uint32_t big_cap,
                                                      const FmmParams* __restrict__ fp) {
+  __shared__ uint32_t cnt[FMM_LEVELS + 1];
   if (fp->abort) return;
+  if (threadIdx.x <= FMM_LEVELS) cnt[threadIdx.x] = min(fp->big_n[threadIdx.x], big_cap);
+  __syncthreads();
   for (int d = (int)FMM_LEVELS; d >= 0; --d) {
-    const uint32_t cnt = min(fp->big_n[d], big_cap);
-    for (uint32_t i = threadIdx.x; i < cnt; i += 1024)
+    if (cnt[d] == 0) continue;
+    for (uint32_t i = threadIdx.x; i < cnt[d]; i += 1024)
       fmm_up_cell<Q>(big[(size_t)d * big_cap + i], cells, bbox, ux, uy, ps, mexp);
     __syncthreads();
   }
@@ -822,7 +887,9 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
                                                    uint32_t step, const uint2* __restrict__ in,
                                                    uint2* __restrict__ out, uint32_t front_cap,
                                                    uint64_t* __restrict__ m2l, uint32_t m2l_cap,
-                                                   uint64_t* __restrict__ p2p, uint32_t p2p_cap, MacArgs mac) {
+                                                   uint64_t* __restrict__ p2p, uint32_t p2p_cap,
+                                                   uint32_t* __restrict__ m2l_cnt, uint32_t* __restrict__ p2p_cnt,
+                                                   MacArgs mac) {
   __shared__ uint32_t wsum[3][EWF / 64];
   __shared__ uint32_t base_sh[3];
   __shared__ unsigned long long pairs_sh[EWF / 64];
@@ -930,10 +997,12 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
         if (o_m2l < m2l_cap) m2l[o_m2l] = key;
         else atomicOr(&fp->list_overflow, 2u);
         ++o_m2l;
+        atomicAdd(&m2l_cnt[A], 1u);
       } else if (kind == 2) {
         if (o_p2p < p2p_cap) p2p[o_p2p] = key;
         else atomicOr(&fp->list_overflow, 4u);
         ++o_p2p;
+        atomicAdd(&p2p_cnt[A], 1u);
       }
       if (push) {
         const uint32_t np = push == 1 ? nch * nch : nch;
@@ -975,18 +1044,11 @@ __global__ __launch_bounds__(EWF) void k_fmm_lists_reset(FmmParams* __restrict__
   }
 }
 
-// Interaction pairs grouped by target: a count per target, a scan, and a
-// scatter through per-target cursors. Within a target the scatter's order is
-// arbitrary; the kernels that read a group first order it by source
-// (sort_segment), so every sum runs in one fixed order.
-__global__ __launch_bounds__(EWF) void k_target_count(const uint64_t* __restrict__ list,
-                                                      const uint32_t* __restrict__ count, uint32_t nbits,
-                                                      uint32_t* __restrict__ cnt, const FmmParams* __restrict__ fp) {
-  if (fp->abort) return;
-  for (uint32_t k = blockIdx.x * EWF + threadIdx.x; k < *count; k += gridDim.x * EWF)
-    atomicAdd(&cnt[list[k] >> nbits], 1u);
-}
-
+// Interaction pairs grouped by target: the dual-tree rounds count each
+// target's pairs, a scan places the groups, and a scatter through per-target
+// cursors fills them. Within a target the scatter's order is arbitrary; the
+// kernels that read a group first order it by source (sort_segment), so
+// every sum runs in one fixed order.
 __global__ __launch_bounds__(EWF) void k_target_scatter(const uint64_t* __restrict__ list,
                                                         const uint32_t* __restrict__ count, uint32_t nbits,
                                                         const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
@@ -1697,6 +1759,8 @@ hipError_t reserve_points(FmmBuffers& f, uint32_t n) {
   f.blk_cap = blk;
   f.big_cap = cap / FMM_DIRECT + 1;
   HIPCHECK(grow(f.big, (size_t)CB_DEPTHS * f.big_cap));
+  f.mid_cap = (uint32_t)((uint64_t)CB_DEPTHS * cap / (UP_SMALL + 1)) + 64;  // at most cap / (UP_SMALL + 1) a depth
+  HIPCHECK(grow(f.mid, f.mid_cap));
   f.point_cap = cap;
   if (f.cell_cap < cap / 4 + 4096) HIPCHECK(grow_cells(f, cap / 4 + 4096));
   return hipSuccess;
@@ -1765,12 +1829,16 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
     const uint32_t b = ceil_div(count, per);
     return b > 0 ? b : 1;
   };
-  // upward: cells of up to FMM_DIRECT points from their points (one wave
-  // each, level_blocks blocks of 4 waves at most), then the larger ones by
-  // M2M in one workgroup
-  const uint32_t up_blocks = g ? f.level_blocks : std::max(1u, std::min(f.level_blocks, ceil_div(nc, 4)));
-  hipLaunchKernelGGL(k_fmm_up_direct<Q>, dim3(up_blocks), dim3(256), 0, stream, f.cells, f.parent, f.bbox, f.ux, f.uy,
-                     f.ps, (float2*)f.mexp, f.params);
+  // upward: cells of up to FMM_DIRECT points from their points (8 lanes a
+  // cell up to UP_SMALL points, else a wave; the capturable form's
+  // level_blocks blocks loop over them), then the larger ones by M2M in one
+  // workgroup
+  const uint32_t small_blocks = g ? f.level_blocks : std::max(1u, std::min(1u << 20, ceil_div(nc, 32)));
+  hipLaunchKernelGGL(k_fmm_up_small<Q>, dim3(small_blocks), dim3(256), 0, stream, f.cells, f.bbox, f.ux, f.uy, f.ps,
+                     (float2*)f.mexp, f.params);
+  const uint32_t mid_blocks = g ? f.level_blocks : std::max(1u, std::min(f.level_blocks, ceil_div(nc, 16)));
+  hipLaunchKernelGGL(k_fmm_up_mid<Q>, dim3(mid_blocks), dim3(256), 0, stream, f.cells, f.bbox, f.ux, f.uy, f.ps,
+                     (float2*)f.mexp, f.mid, f.mid_cap, f.params);
   hipLaunchKernelGGL(k_fmm_up_big<Q>, dim3(1), dim3(1024), 0, stream, f.cells, f.bbox, f.ux, f.uy, f.ps,
                      (float2*)f.mexp, f.big, f.big_cap, f.params);
   // interaction lists (pair keys: target << nbits | source)
@@ -1784,12 +1852,16 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   mac.graphs = f.graphs;
   const uint32_t steps = 2 * maxd + 1;  // a traversal not drained by then aborts the replay (k_fmm_guard)
   const uint32_t root_blocks = ceil_div(a.graphs > 0 ? a.graphs : 1, EWF);  // one root per graph at most
+  // per-target list counts, taken by the rounds (cleared by kernels: no memset node)
+  const uint32_t* ncp = &f.params->ncg;
   for (;;) {
+    hipLaunchKernelGGL(k_fmm_zero, dim3(grid(nc, 256, CAP_CELL_BLOCKS)), dim3(256), 0, stream, f.m2l_cnt, ncp);
+    hipLaunchKernelGGL(k_fmm_zero, dim3(grid(nc, 256, CAP_CELL_BLOCKS)), dim3(256), 0, stream, f.p2p_cnt, ncp);
     hipLaunchKernelGGL(k_fmm_lists_reset, dim3(root_blocks), dim3(EWF), 0, stream, f.params, f.front_a, f.front_cap);
     uint2 *in = f.front_a, *out = f.front_b;
     for (uint32_t s = 0; s < steps; ++s) {
       hipLaunchKernelGGL(k_dual_step, dim3(f.dual_blocks), dim3(EWF), 0, stream, f.cells, f.params, s, in, out,
-                         f.front_cap, f.m2l, f.m2l_cap, f.p2p, f.p2p_cap, mac);
+                         f.front_cap, f.m2l, f.m2l_cap, f.p2p, f.p2p_cap, f.m2l_cnt, f.p2p_cnt, mac);
       std::swap(in, out);
     }
     if (g) break;
@@ -1841,17 +1913,8 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
       f.item_cap = item_need;
     }
   }
-  // lists grouped by target (counters cleared by kernels: no memset node)
-  const uint32_t* ncp = &f.params->ncg;
+  // lists grouped by target: the rounds' counts scanned, then a scatter
   const uint32_t scan_max = g ? f.cellarr_cap : nc;
-  hipLaunchKernelGGL(k_fmm_zero, dim3(grid(nc, 256, CAP_CELL_BLOCKS)), dim3(256), 0, stream, f.m2l_cnt, ncp);
-  hipLaunchKernelGGL(k_fmm_zero, dim3(grid(nc, 256, CAP_CELL_BLOCKS)), dim3(256), 0, stream, f.p2p_cnt, ncp);
-  if (g || P.m2l)
-    hipLaunchKernelGGL(k_target_count, dim3(grid(P.m2l, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.m2l,
-                       &f.params->m2l, nbits, f.m2l_cnt, f.params);
-  if (g || P.p2p)
-    hipLaunchKernelGGL(k_target_count, dim3(grid(P.p2p, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.p2p,
-                       &f.params->p2p, nbits, f.p2p_cnt, f.params);
   HIPCHECK(scan_u32(f.m2l_cnt, f.m2l_off, scan_max, ncp, f.scan_partials, nullptr, stream));
   HIPCHECK(scan_u32(f.p2p_cnt, f.p2p_off, scan_max, ncp, f.scan_partials, nullptr, stream));
   if (g || P.m2l) {
@@ -1903,7 +1966,7 @@ using Orders = OrderTable<2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17
 }  // namespace
 
 void fmm_free(FmmBuffers& f) {
-  void* ptrs[] = {f.graphs, f.seeds, f.node_off_dev, f.chunk_lo, f.chunk_hi, f.chunk_first, f.box_parts, f.cells, f.parent, f.bbox, f.fl, f.blk_cnt, f.blk_fb, f.blk_off, f.blk_nb, f.big, f.mexp, f.lexp, f.px, f.py, f.ux, f.uy, f.ps,
+  void* ptrs[] = {f.graphs, f.seeds, f.node_off_dev, f.chunk_lo, f.chunk_hi, f.chunk_first, f.box_parts, f.cells, f.parent, f.bbox, f.fl, f.blk_cnt, f.blk_fb, f.blk_off, f.blk_nb, f.big, f.mid, f.mexp, f.lexp, f.px, f.py, f.ux, f.uy, f.ps,
                   f.front_a, f.front_b, f.long_list, f.near_items, f.near_combines, f.near_partial, f.m2l, f.p2p, f.m2l_src, f.p2p_src, f.m2l_off, f.m2l_cnt, f.p2p_off,
                   f.p2p_cnt, f.cursor, f.scan_partials, f.params};
   for (void* p : ptrs)
@@ -1921,7 +1984,7 @@ hipError_t direct_sum(const double* x, const double* y, const double* strength, 
 
 // Chunks of at most FOREST_CHUNK nodes that never straddle a graph, for the
 // per-graph boxes; rebuilt when the forest changes.
-constexpr uint32_t FOREST_CHUNK = 16384;
+constexpr uint32_t FOREST_CHUNK = 2048;
 
 hipError_t reserve_forest(FmmBuffers& f, const FmmArgs& a, hipStream_t stream) {
   const uint32_t G = a.graphs;
@@ -1985,7 +2048,7 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
   if (f.chunks)
     hipLaunchKernelGGL(k_forest_box, dim3(f.chunks), dim3(EWF), 0, stream, a.x, a.y, f.chunk_lo, f.chunk_hi,
                        (BoxPart*)f.box_parts);
-  hipLaunchKernelGGL(k_forest_graphs, dim3(ceil_div(a.graphs, 64)), dim3(64), 0, stream, (const BoxPart*)f.box_parts,
+  hipLaunchKernelGGL(k_forest_graphs, dim3(a.graphs), dim3(64), 0, stream, (const BoxPart*)f.box_parts,
                      f.chunk_first, a.graphs, a.dmin2, a.dmax2, a.seeds_dev ? a.seeds_dev : f.seeds, f.node_off_dev,
                      a.min_graph_n, f.graphs, f.params);
   hipLaunchKernelGGL(k_forest_keys, dim3(nb), dim3(EWF), 0, stream, a.x, a.y, n, a.node_graph, f.graphs, f.params,
@@ -2011,7 +2074,7 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
     hipLaunchKernelGGL(k_cells_scan, dim3(CB_DEPTHS), dim3(1024), 0, stream, f.blk_cnt, f.blk_fb, nblk, f.blk_off,
                        f.blk_nb, f.params, f.cell_cap);
     hipLaunchKernelGGL(k_cells_emit, dim3(nblk), dim3(CB_THREADS), 0, stream, t.keys, f.fl, nblk, f.blk_off,
-                       f.blk_nb, f.params, f.cells, f.parent, f.cell_cap, f.big, f.big_cap);
+                       f.blk_nb, f.params, f.cells, f.parent, f.cell_cap, f.big, f.big_cap, f.mid, f.mid_cap);
     if (a.graph) break;  // sizes checked on the device (k_fmm_guard)
     HIPCHECK(hipMemcpyAsync(f.params_host, f.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
@@ -2035,7 +2098,7 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
   hipLaunchKernelGGL(k_fmm_guard, dim3(1), dim3(64), 0, stream, f.params, 0u, room, f.front_cap,
                      a.graph ? d_end : FMM_LEVELS);
   hipLaunchKernelGGL(k_fmm_gather, dim3(nb), dim3(EWF), 0, stream, t.perm, t.keys, a.x, a.y, a.strength, f.graphs,
-                     f.params, a.alpha, a.alpha_dev, n, f.px, f.py, f.ux, f.uy, f.ps);
+                     f.params, a.alpha, a.alpha_dev, n, f.px, f.py, f.ux, f.uy, f.ps, f.cells, f.parent);
   return Orders::fns[p - 1](f, t, a, stream);
 }
 
