@@ -877,6 +877,61 @@ __device__ inline uint32_t block_inclusive(uint32_t v, uint32_t* wsum, uint32_t&
   return before + inc;
 }
 
+// fmm.zig:400-453 for one pair (A, B): kind 0 none, 1 M2L, 2 P2P; push 0
+// none, 1 children x children, 2 (A, children of B), 3 (children of A, B),
+// the children being c0 .. c0 + nch; pairs counts the P2P point pairs.
+__device__ __forceinline__ void dual_pair(const FCell* __restrict__ cells, uint32_t A, uint32_t B, const MacArgs& mac,
+                                          uint32_t& kind, uint32_t& push, uint32_t& c0, uint32_t& nch,
+                                          unsigned long long& pairs) {
+  const FCell ca = cells[A];
+  if (A == B) {
+    if (ca.nchild == 0) {
+      kind = 2;
+      pairs += (unsigned long long)ca.count * ca.count;
+    } else {
+      push = 1;
+      c0 = ca.child0;
+      nch = ca.nchild;
+    }
+    return;
+  }
+  const FCell cb = cells[B];
+  const double ra = ca.r, rb = cb.r;
+  const double dx = ca.cx - cb.cx, dy = ca.cy - cb.cy;
+  const double d2 = dx * dx + dy * dy;
+  const double rmax = zmax(ra, rb), rmin = zmin(ra, rb);
+  const double thr = mac.mac_far * rmax + rmin;
+  bool done = false;
+  if (d2 > MIN_SEP2 && d2 >= thr * thr) {
+    bool separated = true;
+    const FmmGraph& G = mac.graphs[ca.graph];
+    if (G.ring) {
+      const double gap = __builtin_sqrt(d2) - ra - rb;
+      if (gap >= G.dmax_s) done = true;
+      else if (gap + 2 * (ra + rb) > G.dmax_s || gap < G.dmin_s) separated = false;
+    }
+    if (!done && separated) {
+      kind = 1;
+      done = true;
+    }
+  }
+  if (!done) {
+    const bool la = ca.nchild == 0, lb = cb.nchild == 0;
+    if (la && lb) {
+      kind = 2;
+      pairs += (unsigned long long)ca.count * cb.count;
+    } else if (la || (!lb && rb > ra)) {
+      push = 2;
+      c0 = cb.child0;
+      nch = cb.nchild;
+    } else {
+      push = 3;
+      c0 = ca.child0;
+      nch = ca.nchild;
+    }
+  }
+}
+
 // One round of fmm.zig:400-453 over the frontier fp->front[step]. A block
 // classifies a tile of STEP_TILE pairs, scans its output counts and reserves
 // room in each list with one atomic per list and tile. A frontier that
@@ -910,53 +965,7 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
         const uint2 pr = in[idx];
         A = pr.x;
         B = pr.y;
-        const FCell ca = cells[A];
-        if (A == B) {
-          if (ca.nchild == 0) {
-            kind = 2;
-            pairs += (unsigned long long)ca.count * ca.count;
-          } else {
-            push = 1;
-            c0 = ca.child0;
-            nch = ca.nchild;
-          }
-        } else {
-          const FCell cb = cells[B];
-          const double ra = ca.r, rb = cb.r;
-          const double dx = ca.cx - cb.cx, dy = ca.cy - cb.cy;
-          const double d2 = dx * dx + dy * dy;
-          const double rmax = zmax(ra, rb), rmin = zmin(ra, rb);
-          const double thr = mac.mac_far * rmax + rmin;
-          bool done = false;
-          if (d2 > MIN_SEP2 && d2 >= thr * thr) {
-            bool separated = true;
-            const FmmGraph& G = mac.graphs[ca.graph];
-            if (G.ring) {
-              const double gap = __builtin_sqrt(d2) - ra - rb;
-              if (gap >= G.dmax_s) done = true;
-              else if (gap + 2 * (ra + rb) > G.dmax_s || gap < G.dmin_s) separated = false;
-            }
-            if (!done && separated) {
-              kind = 1;
-              done = true;
-            }
-          }
-          if (!done) {
-            const bool la = ca.nchild == 0, lb = cb.nchild == 0;
-            if (la && lb) {
-              kind = 2;
-              pairs += (unsigned long long)ca.count * cb.count;
-            } else if (la || (!lb && rb > ra)) {
-              push = 2;
-              c0 = cb.child0;
-              nch = cb.nchild;
-            } else {
-              push = 3;
-              c0 = ca.child0;
-              nch = ca.nchild;
-            }
-          }
-        }
+        dual_pair(cells, A, B, mac, kind, push, c0, nch, pairs);
       }
       pa[q] = A;
       pb[q] = B;
@@ -1022,7 +1031,8 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
   }
 }
 
-// The first frontier: every root paired with itself.
+// The first frontier: every root paired with itself; the list counters
+// cleared.
 __global__ __launch_bounds__(EWF) void k_fmm_lists_reset(FmmParams* __restrict__ fp, uint2* __restrict__ front,
                                                          uint32_t front_cap) {
   if (fp->abort) return;
