@@ -130,6 +130,43 @@ def run_fast(core, n, steps, warmup, order=8, opening=1.5):
     }
 
 
+def run_fast_same(core, n, steps, warmup, order=10, opening=1.5):
+    """Fast precision on the exact leg's workload (the same graph, pins and
+    forces: manyBody + link + center), at the lowest FMM order whose RMS force
+    error against the FP64 direct sum is within 1e-6 (the metric's fast form).
+    The sums of forceCenter run on the second stream as in exact mode."""
+    m = 2 * n
+    sim, nodes = make_workload(core, n, m, 0.01)
+    sim.precision("fast", {"order": order, "opening": opening})
+    ctx = sim.begin()
+    log("fast (exact workload): warmup %d ticks" % warmup)
+    sim.iterate(warmup, ctx)
+    device_sync(core)
+    core.stats(ctx.h)
+    core.core_set_timing(1)
+    device_sync(core)
+    t0 = time.perf_counter()
+    sim.iterate(steps, ctx)
+    device_sync(core)
+    dt = time.perf_counter() - t0
+    core.core_set_timing(0)
+    st = core.stats(ctx.h)
+    sim.end(ctx)
+    log("fast (exact workload): %.3f ms/tick, phases %s" % (1e3 * dt / steps, st["ms"]))
+    err, k = fast_rms(core, nodes, order=order, opening=opening)
+    log("fast (exact workload): rms vs FP64 direct sum %.3e on %d nodes" % (err, k))
+    return {
+        "value": steps / dt, "unit": "ticks/s", "ms_per_step": 1e3 * dt / steps, "steps": steps, "warmup": warmup,
+        "dtype": "f64 positions and centres, f32 expansions / M2L / P2P",
+        "config": {"workload": "fast mode on the exact leg's workload: %d nodes, %d links (distinct pairs), 1%% "
+                               "pinned, phyllotaxis start, manyBody FMM order %d opening %g + link sweep + center"
+                               % (n, m, order, opening), "nodes": n, "links": m, "mode": "fast", "order": order,
+                   "opening": opening, "ticks": "%d-%d" % (warmup + 1, warmup + steps)},
+        "rms_force_error_vs_fp64_direct_sum": err, "rms_sample_nodes": k,
+        "phases_ms_per_tick": {ph: v / steps for ph, v in st["ms"].items() if v > 0},
+    }
+
+
 def run_batch(core, graphs, steps, warmup, seed=1):
     """The batched workload: `graphs` stochastic-block-model graphs (n
     log-uniform in [50, 20000], 2-16 blocks, p_in 0.05 / p_out 0.002 scaled to
@@ -415,6 +452,8 @@ def main():
     ap.add_argument("--fast-only", action="store_true", help="run only the fast-mode leg (prints its object)")
     ap.add_argument("--fast-order", type=int, default=8, help="FMM order of the fast-only leg")
     ap.add_argument("--fast-opening", type=float, default=1.5, help="FMM opening of the fast-only leg")
+    ap.add_argument("--fast-same-only", action="store_true",
+                    help="run only fast precision on the exact leg's workload (prints its object)")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs of the batched leg (0: skip it)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched leg (prints its object)")
     ap.add_argument("--warm-nodes", type=int, default=100000, help="nodes of the warm-start graphs (0: skip)")
@@ -431,6 +470,9 @@ def main():
     if args.fast_only:
         print(json.dumps(run_fast(d3.gpu_core(), args.fast_nodes, args.steps, args.warmup, order=args.fast_order,
                                   opening=args.fast_opening)))
+        return
+    if args.fast_same_only:
+        print(json.dumps(run_fast_same(d3.gpu_core(), args.nodes, args.steps, args.warmup)))
         return
     if args.batch_only:
         print(json.dumps(run_batch(d3.gpu_core(), args.batch_graphs, args.steps, args.warmup)))
@@ -543,6 +585,7 @@ def main():
         fast_1e6 = run_fast(core, args.fast_nodes, args.steps, args.warmup, order=10)
         fast_1e6["note"] = ("the same workload at FMM order 10, the lowest order whose RMS force error against "
                             "the FP64 direct sum is within 1e-6")
+    fast_same = run_fast_same(core, n, args.steps, args.warmup) if args.fast_nodes else None
     batch = run_batch(core, args.batch_graphs, args.steps, args.warmup) if args.batch_graphs else None
     warm = run_warm(core, args.warm_nodes) if args.warm_nodes else None
     collide = (run_collide(core, args.collide_nodes, min(args.steps, 10), args.warmup, 0 if args.no_cpu else 1)
@@ -577,6 +620,7 @@ def main():
         "cells": cells,
         "rooflines": kernels,
         "cpu_baseline": cpu,
+        "fast_exact_workload": fast_same,
         "fast": fast,
         "fast_rms_within_1e-6": fast_1e6,
         "batch": batch,

@@ -398,10 +398,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scan(uint32_t* __restrict__ h
   }
 }
 
-// Each wave ranks its 1024 consecutive keys in 16 slots of 64 with ballot
-// matching (stable: slot order, then lane order), the waves are stacked in
-// order, and the tile is placed in LDS in digit order; consecutive threads
-// then write consecutive keys of one digit's run to its scanned offset.
+// Each wave ranks its RS_TILE / 4 consecutive keys in RS_ITEMS slots of 64
+// with ballot matching (stable: slot order, then lane order), the waves are
+// stacked in order, and the tile is placed in LDS in digit order;
+// consecutive threads then write consecutive keys of one digit's run to its
+// scanned offset.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
                                                            uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
