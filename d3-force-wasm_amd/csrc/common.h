@@ -134,6 +134,13 @@ struct TreeBuffers {
   TreeMeta* meta_host = nullptr;  // pinned mirror
 };
 
+// The streams every simulation on the current device runs on: its main
+// stream and a side stream for work that overlaps it (forceCenter's sums).
+// Created once, back to back, so the two get hardware queues of their own;
+// streams created and destroyed with each simulation could come to share one
+// queue, and the side stream's work would then wait for the main stream's.
+hipError_t shared_streams(hipStream_t* main, hipStream_t* side);
+
 hipError_t tree_reserve(TreeBuffers& t, uint32_t n);
 hipError_t tree_reserve_cells(TreeBuffers& t, uint32_t cells);
 void tree_free(TreeBuffers& t);

@@ -237,6 +237,10 @@ struct Sim {
   int open_phase = 0;
 
   ~Sim() {
+    // the streams are shared with the device's other simulations: wait for
+    // this one's work, keep the streams
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (side) (void)hipStreamSynchronize(side);
     for (Force* f : forces) delete f;
     for (auto* a : {&x, &y, &vx, &vy, &fx, &fy, &snap}) a->release();
     fixed.release();
@@ -284,12 +288,10 @@ struct Sim {
     center_work_free(cw_main);
     if (ev_x_ready) (void)hipEventDestroy(ev_x_ready);
     if (ev_sums_done) (void)hipEventDestroy(ev_sums_done);
-    if (side) (void)hipStreamDestroy(side);
-    if (stream) (void)hipStreamDestroy(stream);
   }
 
   hipError_t init() {
-    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(shared_streams(&stream, &side));
     HIPCHECK(hipMalloc(&dscalar_u, 16 * sizeof(uint32_t)));
     HIPCHECK(hipMalloc(&dscalar_d, 16 * sizeof(double)));
     HIPCHECK(hipMalloc(&dlink, sizeof(LinkSerialState)));
@@ -297,7 +299,6 @@ struct Sim {
     HIPCHECK(hipHostMalloc((void**)&hlink, sizeof(LinkSerialState), hipHostMallocDefault));
     HIPCHECK(hipMalloc(&dcol, sizeof(CollideSerialState)));
     HIPCHECK(hipHostMalloc((void**)&hcol, sizeof(CollideSerialState), hipHostMallocDefault));
-    HIPCHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     HIPCHECK(hipEventCreateWithFlags(&ev_x_ready, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&ev_sums_done, hipEventDisableTiming));
     HIPCHECK(hipMalloc(&dsums_side, 2 * sizeof(double)));

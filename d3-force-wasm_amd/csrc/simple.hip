@@ -3,6 +3,9 @@
 // are per node. forceCenter's mean is d3's serial sum over nodes 0..n-1
 // (simple.zig:18-22), which no reassociation may change: one workgroup stages
 // x and y through LDS and one lane per axis adds them in index order.
+#include <mutex>
+#include <vector>
+
 #include "common.h"
 
 namespace d3f {
@@ -110,6 +113,7 @@ constexpr uint32_t CD = 8;         // doubles per record: lo0 hi0 D0 lo1 hi1 D1 
 constexpr uint32_t CR = 128;       // records per staging round (two per lane); every needing one is staged
 static_assert(CR == 128, "the chain reads two records per lane");
 constexpr int CK_THREADS = 256;    // wave 0 adds, waves 1-3 stage
+constexpr size_t CS_CHAIN_LDS = 65536;  // LDS a chain block takes in all (see center_raw_sum)
 constexpr uint32_t CS_RTAB = 4096; // rounds whose staging indices are kept in LDS
 constexpr uint32_t META_VALID = 1u << 13, META_NEG = 1u << 12;
 
@@ -691,16 +695,16 @@ hipError_t center_raw_sum(const double* x, const double* y, uint32_t n, double* 
   hipLaunchKernelGGL(k_cs_seglist, dim3(g), dim3(EW), 0, stream, nch, w.segflag, w.segidx, w.need, w.fidx, w.segtmp,
                      w.seglist, w.round_f, w.segrange, rstride);
   hipLaunchKernelGGL(k_cs_stage, dim3(g), dim3(EW), 0, stream, x, y, n, nch, w.need, w.fidx, w.staged);
-  // Each chain block takes its CU's whole LDS, so no other kernel's
-  // workgroups (the tick's traversal runs meanwhile) share its SIMDs.
+  // Each chain block asks for CS_CHAIN_LDS of LDS in all, so few other
+  // workgroups share its SIMDs, yet it still fits on a CU beside the small
+  // workgroups of the tick's other kernels (a block that needed a whole CU's
+  // LDS would wait for a CU to drain completely, which the fast-mode kernels
+  // of the tick may not leave until they end).
   static size_t pad = 0;
   if (!pad) {
     hipFuncAttributes fa;
     HIPCHECK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_cs_chain)));
-    int dev = 0, lds = 0;
-    HIPCHECK(hipGetDevice(&dev));
-    HIPCHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
-    pad = lds > (int)fa.sharedSizeBytes ? (size_t)lds - fa.sharedSizeBytes : 1;
+    pad = CS_CHAIN_LDS > fa.sharedSizeBytes ? CS_CHAIN_LDS - fa.sharedSizeBytes : 1;
     if (pad > 1)
       HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cs_chain),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)pad));
@@ -708,6 +712,25 @@ hipError_t center_raw_sum(const double* x, const double* y, uint32_t n, double* 
   hipLaunchKernelGGL(k_cs_chain, dim3(2), dim3(CK_THREADS), pad > 1 ? pad : 0, stream, x, y, n, w.seglist,
                      w.staged, w.round_f, w.segrange, rstride, sums);
   return hipGetLastError();
+}
+
+hipError_t shared_streams(hipStream_t* main, hipStream_t* side) {
+  static std::mutex lock;
+  static hipStream_t pairs[64][2];
+  std::lock_guard<std::mutex> hold(lock);
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) HIPFAIL(hipErrorInvalidDevice, "device ordinal beyond the stream table");
+  if (!pairs[dev][0]) {
+    hipStream_t m = nullptr, t = nullptr;
+    HIPCHECK(hipStreamCreateWithFlags(&m, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+    pairs[dev][0] = m;
+    pairs[dev][1] = t;
+  }
+  *main = pairs[dev][0];
+  *side = pairs[dev][1];
+  return hipSuccess;
 }
 
 hipError_t center_apply(double* x, double* y, uint32_t n, const double* sums, double cx, double cy, double strength,
