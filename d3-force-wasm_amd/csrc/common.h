@@ -135,11 +135,12 @@ struct TreeBuffers {
 };
 
 // The streams every simulation on the current device runs on: its main
-// stream and a side stream for work that overlaps it (forceCenter's sums).
-// Created once, back to back, so the two get hardware queues of their own;
-// streams created and destroyed with each simulation could come to share one
-// queue, and the side stream's work would then wait for the main stream's.
-hipError_t shared_streams(hipStream_t* main, hipStream_t* side);
+// stream, a side stream for work that overlaps the tick (forceCenter's sums)
+// and an auxiliary one for work that overlaps a force's own launches (the
+// FMM's multipoles). Created once, back to back, so each gets a hardware
+// queue of its own; streams created and destroyed with each simulation could
+// come to share one queue, and their work would then run one after the other.
+hipError_t shared_streams(hipStream_t* main, hipStream_t* side, hipStream_t* aux);
 
 hipError_t tree_reserve(TreeBuffers& t, uint32_t n);
 hipError_t tree_reserve_cells(TreeBuffers& t, uint32_t cells);
@@ -483,6 +484,7 @@ struct FmmBuffers {
   uint32_t* scan_partials = nullptr;
   FmmParams* params = nullptr;       // device
   FmmParams* params_host = nullptr;  // pinned
+  hipEvent_t ev_geom = nullptr, ev_mp = nullptr;  // the upward pass's fork to and join from an auxiliary stream
 };
 void fmm_free(FmmBuffers& f);
 struct FmmArgs {
@@ -511,6 +513,9 @@ struct FmmArgs {
   // Graphs with fewer nodes than this get no tree: their nodes are left as
   // they are (the batch ticks them with the exact kernels).
   uint32_t min_graph_n = 0;
+  // Eager form only: when set, the multipoles run on this stream beside the
+  // dual traversal (forked after the geometry, joined before M2L).
+  hipStream_t aux = nullptr;
 };
 // Fast many-body (fmm.zig:247-354) over every graph of the forest at once.
 // With one graph, returns hipErrorNotSupported when the FMM declines (a

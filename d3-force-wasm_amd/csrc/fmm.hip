@@ -637,12 +637,25 @@ __global__ __launch_bounds__(EWF) void k_fmm_gather(const uint32_t* __restrict__
 
 // ---------------------------------------------------------------- upward
 
-// Geometry (fmm.zig:721-754) and the radius-scaled multipole of the cells
-// [lo, lo + cnt) of one depth: P2M for a leaf, M2M from the children.
-template <int Q>
-__device__ __forceinline__ void fmm_up_cell(uint32_t c, FCell* __restrict__ cells, double* __restrict__ bbox,
-                                            const double* __restrict__ ux, const double* __restrict__ uy,
-                                            const float* __restrict__ ps, float2* __restrict__ mexp) {
+// Geometry (fmm.zig:721-754) of one cell: the bounding box of its points
+// (a leaf) or of its children's boxes, centre = the box's midpoint, radius =
+// half its diagonal, in FP64 with the reference's operations.
+__device__ __forceinline__ void geom_finish(uint32_t c, double x0, double y0, double x1, double y1,
+                                            FCell* __restrict__ cells, double* __restrict__ bbox) {
+  bbox[4 * (size_t)c + 0] = x0;
+  bbox[4 * (size_t)c + 1] = y0;
+  bbox[4 * (size_t)c + 2] = x1;
+  bbox[4 * (size_t)c + 3] = y1;
+  const double dx = x1 - x0, dy = y1 - y0;
+  cells[c].cx = (x0 + x1) / 2;
+  cells[c].cy = (y0 + y1) / 2;
+  cells[c].r = 0.5 * __builtin_sqrt(dx * dx + dy * dy);
+}
+
+// A large cell's geometry, one thread: its points' box for a leaf of the last
+// depth, else the union of its children's boxes.
+__device__ __forceinline__ void geom_cell(uint32_t c, FCell* __restrict__ cells, double* __restrict__ bbox,
+                                          const double* __restrict__ ux, const double* __restrict__ uy) {
   const FCell cell = cells[c];
   const double inf = __builtin_huge_val();
   double x0 = inf, y0 = inf, x1 = -inf, y1 = -inf;
@@ -661,22 +674,23 @@ __device__ __forceinline__ void fmm_up_cell(uint32_t c, FCell* __restrict__ cell
       y1 = zmax(y1, bbox[4 * (size_t)ch + 3]);
     }
   }
-  bbox[4 * (size_t)c + 0] = x0;
-  bbox[4 * (size_t)c + 1] = y0;
-  bbox[4 * (size_t)c + 2] = x1;
-  bbox[4 * (size_t)c + 3] = y1;
-  const double cx = (x0 + x1) / 2, cy = (y0 + y1) / 2;
-  const double dx = x1 - x0, dy = y1 - y0;
-  const double r = 0.5 * __builtin_sqrt(dx * dx + dy * dy);
-  cells[c].cx = cx;
-  cells[c].cy = cy;
-  cells[c].r = r;
+  geom_finish(c, x0, y0, x1, y1, cells, bbox);
+}
 
+// A large cell's radius-scaled multipole (fmm.zig:757-806), one thread: P2M
+// for a leaf of the last depth, else M2M from its children, whose multipoles
+// and geometry are done.
+template <int Q>
+__device__ __forceinline__ void mp_cell(uint32_t c, const FCell* __restrict__ cells, const double* __restrict__ ux,
+                                        const double* __restrict__ uy, const float* __restrict__ ps,
+                                        float2* __restrict__ mexp) {
+  const FCell cell = cells[c];
+  const double cx = cell.cx, cy = cell.cy, r = cell.r;
   float2 m[Q];
 #pragma unroll
   for (int k = 0; k < Q; ++k) m[k] = make_float2(0.f, 0.f);
+  const double inv = r > 0 ? 1 / r : 0;
   if (cell.nchild == 0) {
-    const double inv = r > 0 ? 1 / r : 0;
     for (uint32_t k = cell.first; k < cell.first + cell.count; ++k) {
       const float s = ps[k];
       const float2 z = make_float2((float)((ux[k] - cx) * inv), (float)((uy[k] - cy) * inv));
@@ -690,7 +704,6 @@ __device__ __forceinline__ void fmm_up_cell(uint32_t c, FCell* __restrict__ cell
       }
     }
   } else {
-    const double inv = r > 0 ? 1 / r : 0;
     for (uint32_t ch = cell.child0; ch < cell.child0 + cell.nchild; ++ch) {
       const FCell cc = cells[ch];
       const float tr = (float)(cc.r * inv);
@@ -719,17 +732,16 @@ __device__ __forceinline__ void fmm_up_cell(uint32_t c, FCell* __restrict__ cell
   for (int k = 0; k < Q; ++k) mexp[(size_t)c * Q + k] = m[k];
 }
 
-// Geometry (fmm.zig:721-754) and radius-scaled multipole of a cell of up to
-// FMM_DIRECT points straight from its points, by a group of G lanes: the
-// bounds of its points are the union of its children's (min and max are
-// exact), and the multipole of its points is what M2M from its children sums
-// to. Lane l of the group takes points l, l + G, ...; the lane sums are
-// combined in a fixed butterfly.
-template <int Q, int G>
-__device__ __forceinline__ void up_direct(uint32_t c, uint32_t first, uint32_t count, int lane,
-                                          FCell* __restrict__ cells, double* __restrict__ bbox,
-                                          const double* __restrict__ ux, const double* __restrict__ uy,
-                                          const float* __restrict__ ps, float2* __restrict__ mexp) {
+// A cell of up to FMM_DIRECT points straight from its points, by a group of
+// G lanes (lane l takes points l, l + G, ...; lane results combined in a
+// fixed butterfly): its geometry (the bounds of its points are the union of
+// its children's: min and max are exact), or, with the geometry done, its
+// multipole (the multipole of its points is what M2M from its children sums
+// to).
+template <int G>
+__device__ __forceinline__ void geom_direct(uint32_t c, uint32_t first, uint32_t count, int lane,
+                                            FCell* __restrict__ cells, double* __restrict__ bbox,
+                                            const double* __restrict__ ux, const double* __restrict__ uy) {
   const double inf = __builtin_huge_val();
   double x0 = inf, y0 = inf, x1 = -inf, y1 = -inf;
   for (uint32_t k = first + lane; k < first + count; k += G) {
@@ -746,18 +758,13 @@ __device__ __forceinline__ void up_direct(uint32_t c, uint32_t first, uint32_t c
     y0 = zmin(y0, __shfl_xor(y0, s, 64));
     y1 = zmax(y1, __shfl_xor(y1, s, 64));
   }
-  const double cx = (x0 + x1) / 2, cy = (y0 + y1) / 2;
-  const double dx = x1 - x0, dy = y1 - y0;
-  const double r = 0.5 * __builtin_sqrt(dx * dx + dy * dy);
-  if (lane == 0) {
-    bbox[4 * (size_t)c + 0] = x0;
-    bbox[4 * (size_t)c + 1] = y0;
-    bbox[4 * (size_t)c + 2] = x1;
-    bbox[4 * (size_t)c + 3] = y1;
-    cells[c].cx = cx;
-    cells[c].cy = cy;
-    cells[c].r = r;
-  }
+  if (lane == 0) geom_finish(c, x0, y0, x1, y1, cells, bbox);
+}
+
+template <int Q, int G>
+__device__ __forceinline__ void mp_direct(uint32_t c, uint32_t first, uint32_t count, int lane, double cx, double cy,
+                                          double r, const double* __restrict__ ux, const double* __restrict__ uy,
+                                          const float* __restrict__ ps, float2* __restrict__ mexp) {
   const double inv = r > 0 ? 1 / r : 0;
   float2 mm[Q];
 #pragma unroll
@@ -786,9 +793,14 @@ __device__ __forceinline__ void up_direct(uint32_t c, uint32_t first, uint32_t c
     if (o % G == lane) mexp[(size_t)c * Q + o] = mm[o];
 }
 
-// Every cell of at most UP_SMALL points (the leaves and the smallest
-// splitting cells), 8 lanes a cell, 8 cells a wave.
-template <int Q>
+// The upward pass in two halves: geometry (what the dual traversal needs)
+// and multipoles (what M2L needs), each by the cells' sizes: up to UP_SMALL
+// points 8 lanes a cell, 8 cells a wave (every cell is visited and the larger
+// ones skipped); UP_SMALL + 1 .. FMM_DIRECT points (listed by k_cells_emit) a
+// wave each; the larger ones (at most m / FMM_DIRECT a depth, listed per
+// depth) by one workgroup, deepest depth first, from their children.
+// MP = false runs the geometry, MP = true the multipoles.
+template <int Q, bool MP>
 __global__ __launch_bounds__(256) void k_fmm_up_small(FCell* __restrict__ cells, double* __restrict__ bbox,
                                                       const double* __restrict__ ux, const double* __restrict__ uy,
                                                       const float* __restrict__ ps, float2* __restrict__ mexp,
@@ -799,15 +811,15 @@ __global__ __launch_bounds__(256) void k_fmm_up_small(FCell* __restrict__ cells,
   const uint32_t groups = gridDim.x * 32;
   for (uint32_t c0 = blockIdx.x * 32; c0 < nc; c0 += groups) {
     const uint32_t c = c0 + (threadIdx.x >> 3);  // the groups of a wave stay together
-    const bool mine = c < nc;
-    const uint32_t first = mine ? cells[c].first : 0, count = mine ? cells[c].count : 0;
-    if (mine && count <= UP_SMALL) up_direct<Q, 8>(c, first, count, lane, cells, bbox, ux, uy, ps, mexp);
+    if (c >= nc) continue;
+    const FCell cell = cells[c];
+    if (cell.count > UP_SMALL) continue;
+    if (MP) mp_direct<Q, 8>(c, cell.first, cell.count, lane, cell.cx, cell.cy, cell.r, ux, uy, ps, mexp);
+    else geom_direct<8>(c, cell.first, cell.count, lane, cells, bbox, ux, uy);
   }
 }
 
-// The cells of UP_SMALL + 1 .. FMM_DIRECT points (listed by k_cells_emit),
-// one wave each.
-template <int Q>
+template <int Q, bool MP>
 __global__ __launch_bounds__(256) void k_fmm_up_mid(FCell* __restrict__ cells, double* __restrict__ bbox,
                                                     const double* __restrict__ ux, const double* __restrict__ uy,
                                                     const float* __restrict__ ps, float2* __restrict__ mexp,
@@ -817,14 +829,13 @@ __global__ __launch_bounds__(256) void k_fmm_up_mid(FCell* __restrict__ cells, d
   const uint32_t cnt = min(fp->mid_n, mid_cap);
   for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < cnt; i += gridDim.x * 4) {
     const uint32_t c = mid[i];
-    up_direct<Q, 64>(c, cells[c].first, cells[c].count, threadIdx.x & 63, cells, bbox, ux, uy, ps, mexp);
+    const FCell cell = cells[c];
+    if (MP) mp_direct<Q, 64>(c, cell.first, cell.count, threadIdx.x & 63, cell.cx, cell.cy, cell.r, ux, uy, ps, mexp);
+    else geom_direct<64>(c, cell.first, cell.count, threadIdx.x & 63, cells, bbox, ux, uy);
   }
 }
 
-// The cells of more than FMM_DIRECT points (few: at most m / FMM_DIRECT per
-// depth), one workgroup, deepest depth first: M2M from their children
-// (fmm_up_cell), or P2M when such a cell is a leaf of the last depth.
-template <int Q>
+template <int Q, bool MP>
 __global__ __launch_bounds__(1024) void k_fmm_up_big(FCell* __restrict__ cells, double* __restrict__ bbox,
                                                      const double* __restrict__ ux, const double* __restrict__ uy,
                                                      const float* __restrict__ ps, float2* __restrict__ mexp,
@@ -836,12 +847,14 @@ __global__ __launch_bounds__(1024) void k_fmm_up_big(FCell* __restrict__ cells, 
   __syncthreads();
   for (int d = (int)FMM_LEVELS; d >= 0; --d) {
     if (cnt[d] == 0) continue;
-    for (uint32_t i = threadIdx.x; i < cnt[d]; i += 1024)
-      fmm_up_cell<Q>(big[(size_t)d * big_cap + i], cells, bbox, ux, uy, ps, mexp);
+    for (uint32_t i = threadIdx.x; i < cnt[d]; i += 1024) {
+      const uint32_t c = big[(size_t)d * big_cap + i];
+      if (MP) mp_cell<Q>(c, cells, ux, uy, ps, mexp);
+      else geom_cell(c, cells, bbox, ux, uy);
+    }
     __syncthreads();
   }
 }
-
 
 // ---------------------------------------------------------------- dual tree
 
@@ -1839,18 +1852,36 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
     const uint32_t b = ceil_div(count, per);
     return b > 0 ? b : 1;
   };
-  // upward: cells of up to FMM_DIRECT points from their points (8 lanes a
-  // cell up to UP_SMALL points, else a wave; the capturable form's
-  // level_blocks blocks loop over them), then the larger ones by M2M in one
-  // workgroup
+  // upward: the geometry first (the dual traversal reads it), then the
+  // multipoles, which M2L reads: with an auxiliary stream (the eager form)
+  // they run there beside the dual traversal, joined before M2L. Cells of up
+  // to FMM_DIRECT points from their points (8 lanes a cell up to UP_SMALL
+  // points, else a wave; the capturable form's level_blocks blocks loop over
+  // them), the larger ones from their children in one workgroup.
   const uint32_t small_blocks = g ? f.level_blocks : std::max(1u, std::min(1u << 20, ceil_div(nc, 32)));
-  hipLaunchKernelGGL(k_fmm_up_small<Q>, dim3(small_blocks), dim3(256), 0, stream, f.cells, f.bbox, f.ux, f.uy, f.ps,
-                     (float2*)f.mexp, f.params);
   const uint32_t mid_blocks = g ? f.level_blocks : std::max(1u, std::min(f.level_blocks, ceil_div(nc, 16)));
-  hipLaunchKernelGGL(k_fmm_up_mid<Q>, dim3(mid_blocks), dim3(256), 0, stream, f.cells, f.bbox, f.ux, f.uy, f.ps,
-                     (float2*)f.mexp, f.mid, f.mid_cap, f.params);
-  hipLaunchKernelGGL(k_fmm_up_big<Q>, dim3(1), dim3(1024), 0, stream, f.cells, f.bbox, f.ux, f.uy, f.ps,
+  hipLaunchKernelGGL((k_fmm_up_small<Q, false>), dim3(small_blocks), dim3(256), 0, stream, f.cells, f.bbox, f.ux,
+                     f.uy, f.ps, (float2*)f.mexp, f.params);
+  hipLaunchKernelGGL((k_fmm_up_mid<Q, false>), dim3(mid_blocks), dim3(256), 0, stream, f.cells, f.bbox, f.ux, f.uy,
+                     f.ps, (float2*)f.mexp, f.mid, f.mid_cap, f.params);
+  hipLaunchKernelGGL((k_fmm_up_big<Q, false>), dim3(1), dim3(1024), 0, stream, f.cells, f.bbox, f.ux, f.uy, f.ps,
                      (float2*)f.mexp, f.big, f.big_cap, f.params);
+  const bool aux = !g && a.aux;
+  hipStream_t mps = stream;
+  if (aux) {
+    if (!f.ev_geom) HIPCHECK(hipEventCreateWithFlags(&f.ev_geom, hipEventDisableTiming));
+    if (!f.ev_mp) HIPCHECK(hipEventCreateWithFlags(&f.ev_mp, hipEventDisableTiming));
+    HIPCHECK(hipEventRecord(f.ev_geom, stream));
+    HIPCHECK(hipStreamWaitEvent(a.aux, f.ev_geom, 0));
+    mps = a.aux;
+  }
+  hipLaunchKernelGGL((k_fmm_up_small<Q, true>), dim3(small_blocks), dim3(256), 0, mps, f.cells, f.bbox, f.ux, f.uy,
+                     f.ps, (float2*)f.mexp, f.params);
+  hipLaunchKernelGGL((k_fmm_up_mid<Q, true>), dim3(mid_blocks), dim3(256), 0, mps, f.cells, f.bbox, f.ux, f.uy,
+                     f.ps, (float2*)f.mexp, f.mid, f.mid_cap, f.params);
+  hipLaunchKernelGGL((k_fmm_up_big<Q, true>), dim3(1), dim3(1024), 0, mps, f.cells, f.bbox, f.ux, f.uy, f.ps,
+                     (float2*)f.mexp, f.big, f.big_cap, f.params);
+  if (aux) HIPCHECK(hipEventRecord(f.ev_mp, a.aux));
   // interaction lists (pair keys: target << nbits | source)
   uint32_t nbits = 1;
   while (((uint64_t)1 << nbits) < f.cell_cap) ++nbits;
@@ -1943,6 +1974,7 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   hipLaunchKernelGGL(k_sort_long, dim3(LONG_BLOCKS), dim3(256), 0, stream, f.long_list, f.params, ncp, f.m2l_src,
                      f.m2l_off, f.m2l_cnt, (uint32_t*)f.m2l, f.p2p_src, f.p2p_off, f.p2p_cnt, (uint32_t*)f.p2p);
   // far field
+  if (aux) HIPCHECK(hipStreamWaitEvent(stream, f.ev_mp, 0));  // the multipoles are done
   hipLaunchKernelGGL(k_m2l<Q>, dim3(grid(nc, 1, CAP_TARGET_BLOCKS)), dim3(64), 0, stream, f.cells, f.m2l_src,
                      f.m2l_off, f.m2l_cnt, (const float2*)f.mexp, (float2*)f.lexp, f.params);
   hipLaunchKernelGGL(k_fmm_down_leaf<Q>, dim3(grid(nc, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.cells, f.parent,
@@ -1982,6 +2014,8 @@ void fmm_free(FmmBuffers& f) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (f.params_host) (void)hipHostFree(f.params_host);
+  if (f.ev_geom) (void)hipEventDestroy(f.ev_geom);
+  if (f.ev_mp) (void)hipEventDestroy(f.ev_mp);
   f = FmmBuffers();
 }
 

@@ -205,6 +205,7 @@ struct Sim {
   // a center force exists) the sums for the next center run are started on a
   // side stream, where they overlap the many-body and link work of the tick.
   hipStream_t side = nullptr;
+  hipStream_t aux = nullptr;  // a force's own concurrent work (the FMM's multipoles)
   hipEvent_t ev_x_ready = nullptr, ev_sums_done = nullptr;
   double* dsums_side = nullptr;
   CenterWork cw_side, cw_main;  // workspaces of the sums on each stream
@@ -241,6 +242,7 @@ struct Sim {
     // this one's work, keep the streams
     if (stream) (void)hipStreamSynchronize(stream);
     if (side) (void)hipStreamSynchronize(side);
+    if (aux) (void)hipStreamSynchronize(aux);
     for (Force* f : forces) delete f;
     for (auto* a : {&x, &y, &vx, &vy, &fx, &fy, &snap}) a->release();
     fixed.release();
@@ -291,7 +293,7 @@ struct Sim {
   }
 
   hipError_t init() {
-    HIPCHECK(shared_streams(&stream, &side));
+    HIPCHECK(shared_streams(&stream, &side, &aux));
     HIPCHECK(hipMalloc(&dscalar_u, 16 * sizeof(uint32_t)));
     HIPCHECK(hipMalloc(&dscalar_d, 16 * sizeof(double)));
     HIPCHECK(hipMalloc(&dlink, sizeof(LinkSerialState)));
@@ -910,6 +912,7 @@ int32_t run_many_body_fast(Sim& s, Force& f, double alpha) {  // fmm.zig:247-354
   a.node_off = node_off;
   a.node_graph = nullptr;
   a.seeds = &seed;
+  a.aux = s.aux;
   s.timer_start(D3F_PHASE_FMM);
   hipError_t e = fmm_run(s.fmm, s.tree, a, s.stream);
   s.timer_stop();

@@ -714,22 +714,21 @@ hipError_t center_raw_sum(const double* x, const double* y, uint32_t n, double* 
   return hipGetLastError();
 }
 
-hipError_t shared_streams(hipStream_t* main, hipStream_t* side) {
+hipError_t shared_streams(hipStream_t* main, hipStream_t* side, hipStream_t* aux) {
   static std::mutex lock;
-  static hipStream_t pairs[64][2];
+  static hipStream_t sets[64][3];
   std::lock_guard<std::mutex> hold(lock);
   int dev = 0;
   HIPCHECK(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64) HIPFAIL(hipErrorInvalidDevice, "device ordinal beyond the stream table");
-  if (!pairs[dev][0]) {
-    hipStream_t m = nullptr, t = nullptr;
-    HIPCHECK(hipStreamCreateWithFlags(&m, hipStreamNonBlocking));
-    HIPCHECK(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
-    pairs[dev][0] = m;
-    pairs[dev][1] = t;
+  if (!sets[dev][0]) {
+    hipStream_t k[3] = {nullptr, nullptr, nullptr};
+    for (auto& q : k) HIPCHECK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+    for (int i = 0; i < 3; ++i) sets[dev][i] = k[i];
   }
-  *main = pairs[dev][0];
-  *side = pairs[dev][1];
+  *main = sets[dev][0];
+  *side = sets[dev][1];
+  *aux = sets[dev][2];
   return hipSuccess;
 }
 
