@@ -57,6 +57,7 @@ struct TreeMeta {
   uint32_t ncut;      // subtrees of at most ACC_SUB cells whose parent's subtree is larger (or the root's)
   uint32_t ntop;      // cells whose subtree exceeds ACC_SUB cells
   uint32_t topdepth;  // deepest of those cells
+  uint32_t stop;      // capturable build: a tree its buffers or kernels cannot finish (see tree_build_exact)
 };
 
 // Subtrees of at most this many cells are accumulated by one workgroup in LDS.
@@ -148,7 +149,13 @@ void tree_free(TreeBuffers& t);
 // Builds the exact quadtree over (x, y)[0..n) on `stream`; on return
 // meta_host holds the tree's scalars (the call synchronises once, after the
 // cell count is known, to size the cell buffers).
-hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, uint32_t n, hipStream_t stream);
+// graph: the capturable form. Nothing is read back; launches are sized by n
+// and cell_cap, and meta->stop is set, with no cell written, for a tree this
+// form does not build: no finite point, a run of equal high key halves too
+// long for the in-place sort, points distinct below the key resolution, or
+// more cells than cell_cap. The kernels that use the tree then do nothing.
+hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, uint32_t n, hipStream_t stream,
+                            bool graph = false);
 // Stable LSD radix sort of (keys, vals)[0 .. *count) on the 8-bit digits
 // [first_digit, first_digit + digits); n_max bounds *count. The result is in
 // keys / vals (after an odd number of passes it is copied back from the
@@ -229,8 +236,9 @@ hipError_t collide_unfinished(const CollideArgs& a, uint32_t* flag, uint32_t* po
 // cells of depth d (after a tree build, whose meta_host gives len and depth).
 hipError_t tree_depth_order(TreeBuffers& t, hipStream_t stream);
 // visitAfter charge sums (many_body.zig:101-135) into tcells (qx, qy, value).
+// graph: after a capturable tree build (sizes from cell_cap, nothing read back).
 hipError_t mb_accumulate(TreeBuffers& t, const double* x, const double* y, const double* strengths, double theta2,
-                         hipStream_t stream);
+                         hipStream_t stream, bool graph = false);
 // Copies the accumulated charges of every cell from tcells into the 88-byte
 // cells' x, y, value (what the tree export shows after a many-body run).
 hipError_t mb_publish_charges(TreeBuffers& t, hipStream_t stream);
@@ -250,11 +258,17 @@ struct TravArgs {
   const double* draws;     // REPLAY: the random() values
   uint32_t len;            // cells of the tree
   unsigned long long* counters;  // visits, accepts, pairs (summed)
+  // Captured ticks: count and len are read from the tree's meta (m, len; no
+  // walk when stop is set) and alpha from alpha_dev. count then bounds the grid.
+  const TreeMeta* meta = nullptr;
+  const double* alpha_dev = nullptr;
 };
 hipError_t mb_traverse(const TravArgs& a, int mode, hipStream_t stream);
 // Nodes with a NaN coordinate (tree.rest): their traversal reaches every
 // truthy leaf through truthy cells, and any reached leaf makes v NaN.
 hipError_t mb_nan_nodes(const TreeBuffers& t, uint32_t n, double* vx, double* vy, hipStream_t stream);
+// The same after a capturable build (the count n - m read on the device).
+hipError_t mb_nan_nodes_graph(const TreeBuffers& t, uint32_t n, double* vx, double* vy, hipStream_t stream);
 
 // ---- batch_exact.hip ----
 // One small graph of a batch (fewer nodes than the fast-mode crossover),
@@ -320,6 +334,7 @@ struct LinkLevelArgs {
   uint32_t n;
   double alpha;
   uint32_t* zero_flag;
+  const double* alpha_dev = nullptr;  // replaces alpha when set (captured ticks)
 };
 // Copies x, y, vx, vy into the records.
 hipError_t link_pack(const LinkLevelArgs& a, hipStream_t stream);
@@ -385,8 +400,10 @@ struct LinkCsrArgs {
 hipError_t link_csr(const LinkCsrArgs& a, const SortBuffers& sb, uint32_t* long_count, hipStream_t stream);
 // One Jacobi sweep over the rows on pos = (x + vx, y + vy) (link_fast.zig:60-99).
 // alpha_dev, when given, replaces alpha (read on the device, for captured graphs).
+// seed_dev, when given, replaces seed the same way.
 hipError_t link_sweep(const LinkCsrArgs& a, uint32_t long_count, const double2* pos, double* vx, double* vy,
-                      double alpha, uint64_t seed, hipStream_t stream, const double* alpha_dev = nullptr);
+                      double alpha, uint64_t seed, hipStream_t stream, const double* alpha_dev = nullptr,
+                      const uint64_t* seed_dev = nullptr);
 // One iteration of the uniform-grid collide (grid.zig:221-285).
 hipError_t grid_collide_iteration(const double* x, const double* y, double* vx, double* vy, const double* radii,
                                   uint32_t n, double strength, uint64_t seed, const GridBuffers& g,
@@ -496,10 +513,10 @@ struct FmmArgs {
   // array of graphs + 1 entries); node_graph (device, n entries) maps a node
   // to its graph and may be null for one graph; seeds (host) holds each
   // graph's jiggle seed.
-  uint32_t graphs;
-  const uint32_t* node_off;
-  const uint32_t* node_graph;
-  const uint64_t* seeds;
+  uint32_t graphs = 1;
+  const uint32_t* node_off = nullptr;
+  const uint32_t* node_graph = nullptr;
+  const uint64_t* seeds = nullptr;
   // Capturable form: no host synchronisation, launches sized by the buffers'
   // capacities (reserved beforehand by fmm_reserve_capacity) and by counts
   // read on the device; alpha read from alpha_dev. The caller reads
@@ -561,15 +578,49 @@ hipError_t center_apply(double* x, double* y, uint32_t n, const double* sums, do
 hipError_t integrate(double* x, double* y, double* vx, double* vy, const double* fx, const double* fy,
                      const uint8_t* fixed, uint32_t n, double decay, hipStream_t stream);
 hipError_t axis_force(const double* pos, double* v, const double* strength, const double* target, uint32_t n,
-                      double alpha, hipStream_t stream);
+                      double alpha, hipStream_t stream, const double* alpha_dev = nullptr);
 hipError_t radial_force(const double* x, const double* y, double* vx, double* vy, const double* strength,
-                        const double* radius, uint32_t n, double cx, double cy, double alpha, hipStream_t stream);
+                        const double* radius, uint32_t n, double cx, double cy, double alpha, hipStream_t stream,
+                        const double* alpha_dev = nullptr);
 hipError_t shift_positions(const double* x, const double* y, const double* vx, const double* vy, double* px,
                            double* py, uint32_t n, hipStream_t stream);
 // out[0..n) = vx, out[n..2n) = vy and, when x is not null, out[2n..3n) = x,
 // out[3n..4n) = y (sim_snapshot).
 hipError_t snapshot_copy(const double* vx, const double* vy, const double* x, const double* y, uint32_t n,
                          double* out, hipStream_t stream);
+
+// ---- captured ticks (simple.hip) ----
+// A simulation's tick captured once as a HIP graph and replayed: what
+// changes between ticks (alpha, the fast forces' seeds) is read on the
+// device from TickParams[tick], the tick's position in the uploaded run.
+constexpr int TICK_SEEDS = 8;        // seeded forces a captured tick holds
+constexpr uint32_t TICK_RUN = 256;   // ticks one upload of TickParams covers
+struct TickParams {
+  double alpha;
+  uint64_t seed[TICK_SEEDS];
+};
+struct TickState {
+  double alpha;                 // the running tick's
+  uint64_t seed[TICK_SEEDS];
+  uint32_t tick;                // index of the running tick in the TickParams run
+  uint32_t first_fail;          // the first tick whose replay could not finish it (NONE: none)
+  uint32_t status;              // that tick's flags
+  uint32_t pad;
+};
+// Flags a replay raises (any non-zero word) for work only the eager tick does.
+struct TickNotes {
+  const uint32_t* flag[8];
+  uint32_t bit[8];
+  uint32_t count;
+};
+// Start of a replay: the tick's alpha and seeds, and `nzero` words cleared.
+hipError_t tick_begin(TickState* st, const TickParams* params, uint32_t* zero, uint32_t nzero, hipStream_t stream);
+// The tick's starting x, y, vx, vy, kept until a replay fails: snap holds
+// the state at the start of the first failing tick.
+hipError_t tick_snapshot(const TickState* st, const double* x, const double* y, const double* vx, const double* vy,
+                         uint32_t n, double* snap, hipStream_t stream);
+// End of a replay: the flags noted, the tick counted.
+hipError_t tick_end(TickState* st, const TickNotes& notes, hipStream_t stream);
 // pos[i] = (x + vx, y + vy): the shifted positions as pairs, so the fast link
 // sweep gathers an endpoint from one place.
 hipError_t shift_pack(const double* x, const double* y, const double* vx, const double* vy, double2* pos, uint32_t n,

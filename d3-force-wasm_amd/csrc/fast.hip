@@ -107,10 +107,12 @@ __global__ __launch_bounds__(EW) void k_link_terms(uint32_t total, const uint32_
                                                    double2* __restrict__ terms, double alpha, uint64_t seed,
                                                    const uint32_t* __restrict__ node_graph,
                                                    const uint64_t* __restrict__ seeds,
-                                                   const double* __restrict__ alpha_dev) {
+                                                   const double* __restrict__ alpha_dev,
+                                                   const uint64_t* __restrict__ seed_dev) {
   const uint32_t a = blockIdx.x * EW + threadIdx.x;
   if (a >= total) return;
   if (alpha_dev) alpha = *alpha_dev;
+  if (seed_dev) seed = *seed_dev;
   const uint32_t i = owner[a];
   if (node_graph) seed = seeds[node_graph[i]];
   double tx, ty;
@@ -423,11 +425,12 @@ hipError_t link_csr(const LinkCsrArgs& a, const SortBuffers& sb, uint32_t* long_
 }
 
 hipError_t link_sweep(const LinkCsrArgs& a, uint32_t long_count, const double2* pos, double* vx, double* vy,
-                      double alpha, uint64_t seed, hipStream_t stream, const double* alpha_dev) {
+                      double alpha, uint64_t seed, hipStream_t stream, const double* alpha_dev,
+                      const uint64_t* seed_dev) {
   const uint32_t total = 2 * a.m;
   if (a.n == 0 || total == 0) return hipSuccess;
   hipLaunchKernelGGL(k_link_terms, dim3(ceil_div(total, EW)), dim3(EW), 0, stream, total, a.owner, a.entries, pos,
-                     (double2*)a.terms, alpha, seed, a.node_graph, a.seeds, alpha_dev);
+                     (double2*)a.terms, alpha, seed, a.node_graph, a.seeds, alpha_dev, seed_dev);
   hipLaunchKernelGGL(k_link_rows, dim3(ceil_div(a.n, EW)), dim3(EW), 0, stream, a.n, a.row, (const double2*)a.terms,
                      vx, vy);
   if (long_count)

@@ -2072,7 +2072,8 @@ hipError_t reserve_forest(FmmBuffers& f, const FmmArgs& a, hipStream_t stream) {
     f.node_off_host.assign(a.node_off, a.node_off + G + 1);
     f.chunks = C;
   }
-  if (!a.seeds_dev) HIPCHECK(hipMemcpyAsync(f.seeds, a.seeds, G * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+  if (!a.seeds_dev && a.seeds)
+    HIPCHECK(hipMemcpyAsync(f.seeds, a.seeds, G * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
   return hipSuccess;
 }
 

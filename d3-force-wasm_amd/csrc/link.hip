@@ -45,9 +45,10 @@ __global__ __launch_bounds__(LINK_THREADS) void k_link_pack(const double* __rest
 __global__ __launch_bounds__(LINK_THREADS) void k_link_level(
     const uint32_t* __restrict__ src, const uint32_t* __restrict__ tgt, const double* __restrict__ strength,
     const double* __restrict__ distance, const double* __restrict__ bias, uint32_t lo, uint32_t hi,
-    LinkNode* __restrict__ nodes, double alpha, uint32_t* __restrict__ zero_flag) {
+    LinkNode* __restrict__ nodes, double alpha, uint32_t* __restrict__ zero_flag, const double* __restrict__ alpha_dev) {
   const uint32_t e = lo + blockIdx.x * LINK_THREADS + threadIdx.x;
   if (e >= hi) return;
+  if (alpha_dev) alpha = *alpha_dev;
   const uint32_t s = src[e], t = tgt[e];
   const LinkNode a = nodes[s], c = nodes[t];
   double x = c.x + c.vx - a.x - a.vx;
@@ -133,7 +134,7 @@ hipError_t link_pack(const LinkLevelArgs& a, hipStream_t stream) {
 hipError_t link_level(const LinkLevelArgs& a, uint32_t lo, uint32_t hi, hipStream_t stream) {
   if (hi <= lo) return hipSuccess;
   hipLaunchKernelGGL(k_link_level, dim3(ceil_div(hi - lo, LINK_THREADS)), dim3(LINK_THREADS), 0, stream, a.src,
-                     a.tgt, a.strength, a.distance, a.bias, lo, hi, a.nodes, a.alpha, a.zero_flag);
+                     a.tgt, a.strength, a.distance, a.bias, lo, hi, a.nodes, a.alpha, a.zero_flag, a.alpha_dev);
   return hipGetLastError();
 }
 

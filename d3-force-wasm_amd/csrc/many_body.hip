@@ -544,10 +544,17 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     const double* __restrict__ s, double* __restrict__ vxs, double* __restrict__ vys, double dmin2, double dmax2,
     double alpha, uint32_t* __restrict__ flagged, uint32_t* __restrict__ n_flagged,
     uint32_t* __restrict__ draws_count, const uint32_t* __restrict__ draw_off, const double* __restrict__ draws,
-    uint32_t per_xcd, uint32_t blocks, unsigned long long* __restrict__ counters) {
+    uint32_t per_xcd, uint32_t blocks, unsigned long long* __restrict__ counters, const TreeMeta* __restrict__ meta,
+    const double* __restrict__ alpha_dev) {
   // Block b runs on XCD b % 8; give each XCD a contiguous range of blocks.
   const uint32_t block = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
   if (block >= blocks) return;
+  if (meta) {  // a captured tick: the tree's sizes from the device
+    if (meta->stop) return;
+    count = meta->m;
+    len = meta->len;
+  }
+  if (alpha_dev) alpha = *alpha_dev;
   const int lane = threadIdx.x & 63;
   // A wave takes WALKS * 64 consecutive slots (Morton-ordered nodes); lane L
   // walks slots base + L, base + 64 + L, ...
@@ -638,6 +645,20 @@ __global__ void k_nan_nodes(const uint32_t* __restrict__ rest, uint32_t count, c
   vy[i] = vy[i] + nan;
 }
 
+// k_nan_nodes after a capturable build: the count n - m from the tree's meta.
+__global__ void k_nan_nodes_dev(const uint32_t* __restrict__ rest, uint32_t n, const TreeMeta* __restrict__ meta,
+                                const TCell* __restrict__ tc, double* __restrict__ vx, double* __restrict__ vy) {
+  if (meta->stop || !meta->valid) return;
+  const uint32_t m = meta->m;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n || k >= n - m) return;
+  if (!(tc[0].flags & TC_REACH)) return;
+  const uint32_t i = rest[k];
+  const double nan = __builtin_nan("");
+  vx[i] = vx[i] + nan;
+  vy[i] = vy[i] + nan;
+}
+
 }  // namespace
 
 hipError_t tree_depth_order(TreeBuffers& t, hipStream_t stream) {
@@ -657,10 +678,12 @@ hipError_t tree_depth_order(TreeBuffers& t, hipStream_t stream) {
 }
 
 hipError_t mb_accumulate(TreeBuffers& t, const double* x, const double* y, const double* strengths, double theta2,
-                         hipStream_t stream) {
-  const uint32_t len = t.meta_host->len;
+                         hipStream_t stream, bool graph) {
+  // A capturable build left no mixed group (else it stopped, and the lists
+  // these kernels read are empty).
+  const uint32_t len = graph ? t.cell_cap : t.meta_host->len;
   if (len == 0) return hipSuccess;
-  if (t.meta_host->mixed == 0) {
+  if (graph || t.meta_host->mixed == 0) {
     // Subtree sizes and lists from the build (cells below key resolution,
     // written by the serial split, have none: those trees go level by level).
     // Enough blocks for one subtree each (subtrees average a few hundred
@@ -699,11 +722,19 @@ hipError_t mb_traverse(const TravArgs& a, int mode, hipStream_t stream) {
 #define TRAV_LAUNCH(M)                                                                                            \
   hipLaunchKernelGGL((k_traverse<M>), dim3(per_xcd * XCDS), dim3(TRAV_THREADS), 0, stream, a.tc, a.len, a.order,  \
                      a.count, a.perm, a.x, a.y, a.s, a.vx, a.vy, a.dmin2, a.dmax2, a.alpha, a.flagged,           \
-                     a.n_flagged, a.draws_count, a.draw_off, a.draws, per_xcd, blocks, a.counters)
+                     a.n_flagged, a.draws_count, a.draw_off, a.draws, per_xcd, blocks, a.counters, a.meta,        \
+                     a.alpha_dev)
   if (mode == TRAV_RECORD) TRAV_LAUNCH(TRAV_RECORD);
   else if (mode == TRAV_COUNT) TRAV_LAUNCH(TRAV_COUNT);
   else TRAV_LAUNCH(TRAV_REPLAY);
 #undef TRAV_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t mb_nan_nodes_graph(const TreeBuffers& t, uint32_t n, double* vx, double* vy, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_nan_nodes_dev, dim3(ceil_div(n, 256)), dim3(256), 0, stream, t.rest, n, t.meta, t.tcells, vx,
+                     vy);
   return hipGetLastError();
 }
 

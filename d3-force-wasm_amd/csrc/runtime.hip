@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <new>
 #include <vector>
 
@@ -57,7 +58,18 @@ d3f_random_fn g_random = nullptr;
 void* g_random_ctx = nullptr;
 bool g_timing = false;
 
-double draw() { return g_random ? g_random(g_random_ctx) : 0.5; }
+// Values a run of captured ticks drew ahead of the eager order (seeds of
+// ticks it then had to redo eagerly): the next draws take them first.
+std::deque<double> g_pending;
+
+double draw() {
+  if (!g_pending.empty()) {
+    const double v = g_pending.front();
+    g_pending.pop_front();
+    return v;
+  }
+  return g_random ? g_random(g_random_ctx) : 0.5;
+}
 double jiggle_host(double r) { return (r - 0.5) * 1e-6; }  // rng.zig:7-11
 
 int32_t status_of(hipError_t e) {
@@ -236,6 +248,28 @@ struct Sim {
   std::vector<hipEvent_t> spare;
   hipEvent_t open_event = nullptr;
   int open_phase = 0;
+  // captured ticks (sim_ticks): one tick as a HIP graph, replayed with
+  // alpha and the seeds from tparams[tick]
+  uint64_t epoch = 1;  // advanced by anything that may change what a captured tick holds
+  TickState* dtick = nullptr;
+  TickState* htick = nullptr;  // pinned
+  Device<TickParams> tparams;
+  Pinned<TickParams> hparams;
+  Device<double> tsnap;  // the state at the start of the first failing replay
+  hipGraph_t tgraph = nullptr;
+  hipGraphExec_t texec = nullptr;
+  hipEvent_t ev_tfork = nullptr, ev_tjoin = nullptr;
+  std::vector<uint32_t> tforces;  // what the captured tick was captured for
+  double tdecay = 0;
+  uint64_t tepoch = 0;
+  uint32_t tlevels = 0;
+  uint64_t graph_ticks = 0, graph_redos = 0, graph_captures = 0;
+  void drop_graph() {
+    if (texec) (void)hipGraphExecDestroy(texec);
+    if (tgraph) (void)hipGraphDestroy(tgraph);
+    texec = nullptr;
+    tgraph = nullptr;
+  }
 
   ~Sim() {
     // the streams are shared with the device's other simulations: wait for
@@ -290,6 +324,14 @@ struct Sim {
     center_work_free(cw_main);
     if (ev_x_ready) (void)hipEventDestroy(ev_x_ready);
     if (ev_sums_done) (void)hipEventDestroy(ev_sums_done);
+    drop_graph();
+    if (ev_tfork) (void)hipEventDestroy(ev_tfork);
+    if (ev_tjoin) (void)hipEventDestroy(ev_tjoin);
+    if (dtick) (void)hipFree(dtick);
+    if (htick) (void)hipHostFree(htick);
+    tparams.release();
+    hparams.release();
+    tsnap.release();
   }
 
   hipError_t init() {
@@ -315,6 +357,7 @@ struct Sim {
 
   // sim.zig:59-73
   hipError_t set_node_count(uint32_t nn, bool* grown) {
+    ++epoch;
     HIPCHECK(before_x_write());
     side_valid = false;
     const uint32_t old_n = n;
@@ -456,6 +499,7 @@ bool fast_mode(const Sim& s, const Force& f) {  // root.zig:246-248
 // Uploads a force's parameter arrays when the facade may have written them.
 hipError_t upload(Sim& s, Force& f) {
   if (!f.dirty) return hipSuccess;
+  ++s.epoch;
   const size_t c = f.count > 0 ? f.count : 1;
   for (int i = 0; i < f.float_arrays(); ++i) {
     HIPCHECK(f.df[i].reserve(c, 0, s.stream));
@@ -944,6 +988,38 @@ int32_t run_center(Sim& s, Force& f) {  // simple.zig:14-31
   return status_of(e);
 }
 
+// One force's run (root.zig:211-242 after the argument checks).
+int32_t run_force(Sim& s, Force& f, double alpha) {
+  hipError_t e = upload(s, f);
+  if (e != hipSuccess) return status_of(e);
+  switch (f.kind) {
+    case K_CENTER: return run_center(s, f);
+    case K_X:
+      return status_of(axis_force(s.dx.p, s.dvx.p, f.df[0].p, f.df[1].p, s.n, alpha, s.stream));
+    case K_Y:
+      return status_of(axis_force(s.dy.p, s.dvy.p, f.df[0].p, f.df[1].p, s.n, alpha, s.stream));
+    case K_RADIAL:
+      return status_of(radial_force(s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, f.df[0].p, f.df[1].p, s.n, f.scalar(S_X),
+                                    f.scalar(S_Y), alpha, s.stream));
+    case K_LINK:
+      return fast_mode(s, f) ? run_link_fast(s, f, alpha) : run_link_exact(s, f, alpha);
+    case K_MANY_BODY:
+      return fast_mode(s, f) ? run_many_body_fast(s, f, alpha) : run_many_body_exact(s, f, alpha);
+    case K_COLLIDE:
+      return fast_mode(s, f) ? run_collide_fast(s, f) : run_collide_exact(s, f);
+    default:
+      return BAD_ARGUMENT;
+  }
+}
+
+hipError_t integrate_step(Sim& s, double velocity_decay) {  // simple.zig:90-123
+  HIPCHECK(s.before_x_write());
+  s.timer_start(D3F_PHASE_INTEGRATE);
+  HIPCHECK(integrate(s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, s.dfx.p, s.dfy.p, s.dfixed.p, s.n, velocity_decay, s.stream));
+  s.timer_stop();
+  return s.speculate_center();
+}
+
 // The snapshot's host copy (sim_ptr(h, D3F_PSNAP)): vx, vy, and x, y when
 // the snapshot took positions, n entries each.
 hipError_t fetch_snapshot(Sim& s) {
@@ -976,6 +1052,351 @@ hipError_t refresh_tree_host(Sim& s) {
   if (len) HIPCHECK(hipMemcpy(s.hcells.p, t.cells, (size_t)len * sizeof(Cell), hipMemcpyDeviceToHost));
   if (m) HIPCHECK(hipMemcpy(s.hperm.p, t.perm, (size_t)m * sizeof(uint32_t), hipMemcpyDeviceToHost));
   return hipSuccess;
+}
+
+// ---------------------------------------------------------------- captured ticks
+// sim_ticks runs whole ticks (simulation.js:63-82 with the facade's forces
+// in order, then integrate) as one captured HIP graph per tick. What a
+// replay cannot finish (jiggle draws in the exact traversal or link pass, a
+// tree or FMM beyond the captured buffers, an FMM that declines) is noted on
+// the device; the host then restores the state from the start of the first
+// such tick and runs that tick eagerly, drawing from the random source in
+// the eager order.
+
+enum TickFlag : uint32_t { TF_FMM_ABORT = 1, TF_FMM_DECLINED = 2, TF_TREE_STOP = 4, TF_JIGGLE = 8, TF_LINK_ZERO = 16 };
+
+// Seeds are drawn per run for the seeded kinds in fast precision (core.js:213-231).
+bool seeded(const Force& f) {
+  return f.scalar(S_PRECISION) == 1 && (f.kind == K_MANY_BODY || f.kind == K_LINK || f.kind == K_COLLIDE);
+}
+
+// The eager tick: each force's run as force_run does it, then integrate.
+int32_t eager_tick(Sim& s, const std::vector<Force*>& fs, double alpha, double decay) {
+  for (Force* f : fs) {
+    if (seeded(*f)) f->scalars[S_SEED] = draw();
+    const int32_t r = run_force(s, *f, alpha);
+    if (r < 0) return r;
+  }
+  return status_of(integrate_step(s, decay));
+}
+
+FmmArgs fast_args(Sim& s, Force& f, double alpha) {
+  FmmArgs a;
+  a.x = s.dx.p;
+  a.y = s.dy.p;
+  a.strength = f.df[0].p;
+  a.vx = s.dvx.p;
+  a.vy = s.dvy.p;
+  a.n = s.n;
+  const double order = f.scalar(S_ORDER);
+  a.order = order >= 1 && order <= (double)FMM_MAX_ORDER ? (uint32_t)order : 8;
+  a.alpha = alpha;
+  a.opening = f.scalar(S_OPENING);
+  a.dmin2 = f.scalar(S_DMIN2);
+  a.dmax2 = f.scalar(S_DMAX2);
+  a.graphs = 1;
+  a.node_graph = nullptr;
+  return a;
+}
+
+// The device part of a tick in capturable form (no host synchronisation,
+// nothing read back), on s.stream with forceCenter's sums of the starting
+// positions on the side stream.
+hipError_t graph_tick(Sim& s, const std::vector<Force*>& fs, double decay) {
+  hipStream_t st = s.stream;
+  const uint32_t n = s.n;
+  const double* alpha_dev = &s.dtick->alpha;
+  TickNotes notes{};
+  auto note = [&](const uint32_t* flag, uint32_t bit) {
+    notes.flag[notes.count] = flag;
+    notes.bit[notes.count] = bit;
+    ++notes.count;
+  };
+  HIPCHECK(tick_begin(s.dtick, s.tparams.p, s.dscalar_u, 2, st));
+  HIPCHECK(tick_snapshot(s.dtick, s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, n, s.tsnap.p, st));
+  // Only center and integrate write x, y: the first center's sums are those
+  // of the tick's starting positions.
+  bool side_sums = false;
+  for (Force* f : fs) side_sums = side_sums || f->kind == K_CENTER;
+  if (side_sums) {
+    HIPCHECK(hipEventRecord(s.ev_tfork, st));
+    HIPCHECK(hipStreamWaitEvent(s.side, s.ev_tfork, 0));
+    HIPCHECK(center_raw_sum(s.dx.p, s.dy.p, n, s.dsums_side, s.cw_side, s.side));
+    HIPCHECK(hipEventRecord(s.ev_tjoin, s.side));
+  }
+  int seed_slot = 0;
+  for (Force* fp : fs) {
+    Force& f = *fp;
+    const uint64_t* seed_dev = seeded(f) ? &s.dtick->seed[seed_slot++] : nullptr;
+    const bool fast = fast_mode(s, f);
+    switch (f.kind) {
+      case K_MANY_BODY:
+        if (fast) {
+          FmmArgs a = fast_args(s, f, 0);
+          const uint32_t node_off[2] = {0, n};
+          a.node_off = node_off;
+          a.alpha_dev = alpha_dev;
+          a.seeds_dev = seed_dev;
+          a.graph = true;
+          HIPCHECK(fmm_run(s.fmm, s.tree, a, st));
+          note(&s.fmm.params->abort, TF_FMM_ABORT);
+          note(&s.fmm.params->declined, TF_FMM_DECLINED);
+        } else {
+          TreeBuffers& t = s.tree;
+          HIPCHECK(tree_build_exact(t, s.dx.p, s.dy.p, n, st, true));
+          HIPCHECK(mb_accumulate(t, s.dx.p, s.dy.p, f.df[0].p, f.scalar(S_THETA2), st, true));
+          TravArgs a{};
+          a.tc = t.tcells;
+          a.order = t.perm;
+          a.count = n;  // bounds the grid; the walks take m from the tree
+          a.perm = t.perm;
+          a.x = s.dx.p;
+          a.y = s.dy.p;
+          a.s = f.df[0].p;
+          a.vx = s.dvx.p;
+          a.vy = s.dvy.p;
+          a.dmin2 = f.scalar(S_DMIN2);
+          a.dmax2 = f.scalar(S_DMAX2);
+          a.alpha = 0;
+          a.flagged = s.flagged.p;
+          a.n_flagged = s.dscalar_u;
+          a.len = t.cell_cap;
+          a.counters = s.dcounters;
+          a.meta = t.meta;
+          a.alpha_dev = alpha_dev;
+          HIPCHECK(mb_traverse(a, TRAV_RECORD, st));
+          HIPCHECK(mb_nan_nodes_graph(t, n, s.dvx.p, s.dvy.p, st));
+          note(&t.meta->stop, TF_TREE_STOP);
+          note(s.dscalar_u, TF_JIGGLE);
+        }
+        break;
+      case K_LINK: {
+        const double iterations = f.scalar(S_ITERATIONS);
+        if (f.count == 0 || !(0 < iterations)) break;
+        if (fast) {
+          for (double k = 0; k < iterations; k += 1) {
+            HIPCHECK(shift_pack(s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, s.dpos.p, n, st));
+            HIPCHECK(link_sweep(f.csr_args(n), f.long_count, s.dpos.p, s.dvx.p, s.dvy.p, 0, 0, st, alpha_dev,
+                                seed_dev));
+          }
+        } else {
+          LinkLevelArgs a{f.lsrc.p, f.ltgt.p, f.lf[0].p, f.lf[1].p, f.lf[2].p, s.dx.p, s.dy.p, s.dvx.p, s.dvy.p,
+                          s.dlnodes.p, n, 0, s.dscalar_u + 1};
+          a.alpha_dev = alpha_dev;
+          const uint32_t levels = (uint32_t)f.level_off.size() - 2;
+          HIPCHECK(link_pack(a, st));
+          for (double k = 0; k < iterations; k += 1)
+            for (uint32_t l = 1; l <= levels; ++l) HIPCHECK(link_level(a, f.level_off[l], f.level_off[l + 1], st));
+          HIPCHECK(link_unpack(a, st));
+          note(s.dscalar_u + 1, TF_LINK_ZERO);
+        }
+        break;
+      }
+      case K_CENTER: {
+        const double* sums = s.dsums_side;
+        if (side_sums) {
+          HIPCHECK(hipStreamWaitEvent(st, s.ev_tjoin, 0));
+          side_sums = false;
+        } else {
+          sums = s.dscalar_d;
+          HIPCHECK(center_raw_sum(s.dx.p, s.dy.p, n, s.dscalar_d, s.cw_main, st));
+        }
+        HIPCHECK(center_apply(s.dx.p, s.dy.p, n, sums, f.scalar(S_X), f.scalar(S_Y), f.scalar(S_STRENGTH), st));
+        break;
+      }
+      case K_X:
+        HIPCHECK(axis_force(s.dx.p, s.dvx.p, f.df[0].p, f.df[1].p, n, 0, st, alpha_dev));
+        break;
+      case K_Y:
+        HIPCHECK(axis_force(s.dy.p, s.dvy.p, f.df[0].p, f.df[1].p, n, 0, st, alpha_dev));
+        break;
+      case K_RADIAL:
+        HIPCHECK(radial_force(s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, f.df[0].p, f.df[1].p, n, f.scalar(S_X),
+                              f.scalar(S_Y), 0, st, alpha_dev));
+        break;
+      default:
+        HIPFAIL(hipErrorInvalidValue, "a force kind the captured tick does not hold");
+    }
+  }
+  HIPCHECK(integrate(s.dx.p, s.dy.p, s.dvx.p, s.dvy.p, s.dfx.p, s.dfy.p, s.dfixed.p, n, decay, st));
+  return tick_end(s.dtick, notes, st);
+}
+
+// Sizes every buffer a replay uses from the eager tick just run (with room
+// to grow), then captures the tick.
+hipError_t capture_tick(Sim& s, const std::vector<Force*>& fs, const std::vector<uint32_t>& fhs, double decay) {
+  s.drop_graph();
+  const uint32_t n = s.n;
+  if (!s.dtick) {
+    HIPCHECK(hipMalloc(&s.dtick, sizeof(TickState)));
+    HIPCHECK(hipHostMalloc((void**)&s.htick, sizeof(TickState), hipHostMallocDefault));
+    HIPCHECK(hipEventCreateWithFlags(&s.ev_tfork, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&s.ev_tjoin, hipEventDisableTiming));
+  }
+  HIPCHECK(s.tparams.reserve(TICK_RUN, 0, s.stream));
+  HIPCHECK(s.hparams.reserve(TICK_RUN, 0));
+  HIPCHECK(s.tsnap.reserve(4 * (size_t)s.cap, 0, s.stream));
+  for (Force* f : fs) {
+    if (f->kind != K_MANY_BODY) continue;
+    if (fast_mode(s, *f)) {
+      FmmArgs a = fast_args(s, *f, 0);
+      const uint32_t node_off[2] = {0, n};
+      a.node_off = node_off;
+      a.seeds_dev = &s.dtick->seed[0];  // the replays' seeds (nothing taken from the host)
+      const uint32_t cells = 2 * s.fmm.params_host->level_off[FMM_LEVELS + 1] + 4096;
+      HIPCHECK(fmm_reserve_capacity(s.fmm, s.tree, a, cells, s.stream));
+    } else {
+      const uint32_t len = s.tree.meta_host ? s.tree.meta_host->len : 0;
+      HIPCHECK(tree_reserve_cells(s.tree, std::max(len + len / 2 + 4096, 2 * n + 64)));
+    }
+  }
+  // both center workspaces sized (the capture may use either)
+  if (s.centers) {
+    HIPCHECK(center_raw_sum(s.dx.p, s.dy.p, n, s.dscalar_d, s.cw_main, s.stream));
+    HIPCHECK(center_raw_sum(s.dx.p, s.dy.p, n, s.dscalar_d, s.cw_side, s.stream));
+  }
+  HIPCHECK(hipStreamSynchronize(s.stream));
+  HIPCHECK(hipStreamSynchronize(s.side));
+  s.side_busy = false;
+  s.side_valid = false;
+  HIPCHECK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = graph_tick(s, fs, decay);
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(s.stream, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return e != hipSuccess ? e : e2;
+  }
+  s.tgraph = g;
+  HIPCHECK(hipGraphInstantiate(&s.texec, s.tgraph, nullptr, nullptr, 0));
+  s.tforces = fhs;
+  s.tdecay = decay;
+  s.tepoch = s.epoch;
+  s.tlevels = s.fmm.build_levels;
+  ++s.graph_captures;
+  return hipSuccess;
+}
+
+bool graph_matches(const Sim& s, const std::vector<uint32_t>& fhs, double decay) {
+  return s.texec && s.tepoch == s.epoch && s.tforces == fhs && std::memcmp(&s.tdecay, &decay, sizeof decay) == 0 &&
+         s.tlevels == s.fmm.build_levels;
+}
+
+// Runs `ticks` ticks; returns NOT_CAPTURED before doing anything when the
+// forces or the library's state call for the per-force path.
+constexpr int32_t NOT_CAPTURED = 2;
+int32_t run_ticks(Sim& s, const uint32_t* fhs_in, uint32_t nf, uint32_t ticks, const double* alphas, double decay,
+                  uint32_t* done) {
+  if (g_timing || g_fail_count || s.n < 2 || nf == 0 || nf > 16) return NOT_CAPTURED;
+  std::vector<uint32_t> fhs(fhs_in, fhs_in + nf);
+  std::vector<Force*> fs;
+  uint32_t bodies = 0, links = 0, seeds = 0;
+  for (uint32_t fh : fhs) {
+    Force* f = get_force(&s, fh);
+    if (!f) return BAD_HANDLE;
+    if (f->node_sized() && f->count < s.n) return BAD_ARGUMENT;
+    if (f->kind == K_COLLIDE) return NOT_CAPTURED;
+    bodies += f->kind == K_MANY_BODY;
+    links += f->kind == K_LINK;
+    seeds += seeded(*f);
+    fs.push_back(f);
+  }
+  if (bodies > 1 || links > 1 || seeds > (uint32_t)TICK_SEEDS) return NOT_CAPTURED;
+  int32_t r;
+  uint32_t t = 0;
+  bool use_graph = true;
+  bool fresh = false;  // captured after the last eager tick, not yet replayed through
+  std::vector<double> drawn;
+  auto finish = [&](int32_t status) {
+    g_pending.clear();
+    if (done) *done = t;
+    return status;
+  };
+  // After an eager tick: a capture sized by it (also after the run's last
+  // tick, for the next call), unless this run has gone eager.
+  auto recapture = [&]() -> int32_t {
+    if (!use_graph) return OK;
+    const hipError_t e = capture_tick(s, fs, fhs, decay);
+    fresh = true;
+    if (e == hipSuccess) return OK;
+    s.drop_graph();
+    use_graph = false;  // the rest of this run goes eagerly
+    if (e == hipErrorOutOfMemory) return status_of(e);
+    (void)hipGetLastError();
+    return OK;
+  };
+  while (t < ticks) {
+    if (!use_graph || !graph_matches(s, fhs, decay)) {
+      if ((r = eager_tick(s, fs, alphas[t], decay)) < 0) return finish(r);
+      ++t;
+      if ((r = recapture()) < 0) return finish(r);
+      continue;
+    }
+    const uint32_t chunk = std::min(ticks - t, TICK_RUN);
+    drawn.clear();
+    for (uint32_t k = 0; k < chunk; ++k) {
+      TickParams& p = s.hparams.p[k];
+      p.alpha = alphas[t + k];
+      int slot = 0;
+      for (Force* f : fs)
+        if (seeded(*f)) {
+          const double v = draw();
+          drawn.push_back(v);
+          std::memcpy(&p.seed[slot++], &v, sizeof v);
+        }
+    }
+    TickState& h = *s.htick;
+    h.tick = 0;
+    h.first_fail = NONE;
+    h.status = 0;
+    hipError_t e = hipMemcpyAsync(s.tparams.p, s.hparams.p, chunk * sizeof(TickParams), hipMemcpyHostToDevice,
+                                  s.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.dtick, s.htick, sizeof(TickState), hipMemcpyHostToDevice, s.stream);
+    for (uint32_t k = 0; k < chunk && e == hipSuccess; ++k) e = hipGraphLaunch(s.texec, s.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.htick, s.dtick, sizeof(TickState), hipMemcpyDeviceToHost, s.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s.stream);
+    if (e != hipSuccess) return finish(status_of(e));
+    if (h.first_fail == NONE) {
+      s.graph_ticks += chunk;
+      t += chunk;
+      fresh = false;
+      continue;
+    }
+    // A replay could not finish its tick: the state from its start, then
+    // that tick eagerly. The seeds drawn for it and the ticks after it go
+    // back in front of the random source, so the eager draws see them in
+    // the eager order.
+    const uint32_t tf = h.first_fail;
+    s.graph_ticks += tf;
+    ++s.graph_redos;
+    for (size_t k = drawn.size(); k-- > (size_t)tf * seeds;) g_pending.push_front(drawn[k]);
+    const size_t bn = (size_t)s.n * sizeof(double);
+    double* to[4] = {s.dvx.p, s.dvy.p, s.dx.p, s.dy.p};
+    for (int k = 0; k < 4 && e == hipSuccess; ++k)
+      e = hipMemcpyAsync(to[k], s.tsnap.p + (size_t)k * s.n, bn, hipMemcpyDeviceToDevice, s.stream);
+    if (e != hipSuccess) return finish(status_of(e));
+    t += tf;
+    s.side_busy = false;
+    s.side_valid = false;
+    s.drop_graph();  // the eager tick may grow buffers the graph holds
+    if ((r = eager_tick(s, fs, alphas[t], decay)) < 0) return finish(r);
+    ++t;
+    // A tick a fresh capture could not replay at once would likely fail
+    // again: the rest of the run goes eagerly.
+    if (fresh && tf == 0) use_graph = false;
+    if ((r = recapture()) < 0) return finish(r);
+  }
+  if (s.texec && graph_matches(s, fhs, decay)) {
+    // as after an eager integrate: the next center's sums start now
+    s.side_busy = false;
+    s.side_valid = false;
+    const hipError_t e = s.speculate_center();
+    if (e != hipSuccess) return finish(status_of(e));
+    for (Force* f : fs)
+      if (f->kind == K_MANY_BODY && !fast_mode(s, *f)) s.charges_pending = true;
+  }
+  return finish(OK);
 }
 
 }  // namespace
@@ -1103,19 +1524,33 @@ EXPORT int32_t sim_snapshot(uint32_t h, uint32_t positions) {  // root.zig:205-2
   return status_of(e);
 }
 
+EXPORT int32_t sim_ticks(uint32_t h, const uint32_t* forces, uint32_t nforces, uint32_t ticks, const double* alphas,
+                         double velocity_decay, uint32_t* done) {  // simulation.js:63-82
+  Sim* s = get(h);
+  if (done) *done = 0;
+  if (!s) return BAD_HANDLE;
+  if (ticks == 0) return OK;
+  if (!forces || !alphas) return BAD_ARGUMENT;
+  return run_ticks(*s, forces, nforces, ticks, alphas, velocity_decay, done);
+}
+
+EXPORT int32_t sim_tick_counters(uint32_t h, uint64_t* out) {
+  Sim* s = get(h);
+  if (!s) return BAD_HANDLE;
+  if (!out) return BAD_ARGUMENT;
+  out[0] = s->graph_captures;
+  out[1] = s->graph_ticks;
+  out[2] = s->graph_redos;
+  return OK;
+}
+
 EXPORT int32_t sim_integrate(uint32_t h, double velocity_decay) {  // root.zig:250-254
   Sim* s = get(h);
   if (!s) return BAD_HANDLE;
   int32_t injected;
   if (injected_failure(&injected)) return injected;
-  hipError_t e = s->before_x_write();
-  s->timer_start(D3F_PHASE_INTEGRATE);
-  if (e == hipSuccess)
-    e = integrate(s->dx.p, s->dy.p, s->dvx.p, s->dvy.p, s->dfx.p, s->dfy.p, s->dfixed.p, s->n, velocity_decay,
-                  s->stream);
-  s->timer_stop();
-  if (e == hipSuccess) e = s->speculate_center();
-  return status_of(e);
+  ++s->epoch;
+  return status_of(integrate_step(*s, velocity_decay));
 }
 
 EXPORT int32_t tree_build(uint32_t h, uint32_t shifted) {  // root.zig:102-108
@@ -1124,6 +1559,7 @@ EXPORT int32_t tree_build(uint32_t h, uint32_t shifted) {  // root.zig:102-108
   const void* c0 = s->hcells.p;
   const void* p0 = s->hperm.p;
   hipError_t e;
+  ++s->epoch;
   s->charges_pending = false;
   if (shifted) {
     e = shift_positions(s->dx.p, s->dy.p, s->dvx.p, s->dvy.p, s->dpx.p, s->dpy.p, s->n, s->stream);
@@ -1190,6 +1626,7 @@ EXPORT uint32_t force_create(uint32_t h, uint32_t kind) {  // root.zig:147-161
     return 0;
   }
   if (kind == K_CENTER) s->centers++;
+  ++s->epoch;
   for (size_t i = 0; i < s->forces.size(); ++i)
     if (!s->forces[i]) {
       s->forces[i] = f;
@@ -1205,6 +1642,7 @@ EXPORT int32_t force_destroy(uint32_t h, uint32_t fh) {  // root.zig:163-170
   if (!f) return BAD_HANDLE;
   (void)hipStreamSynchronize(s->stream);
   if (f->kind == K_CENTER) s->centers--;
+  ++s->epoch;
   delete f;
   s->forces[fh - 1] = nullptr;
   return OK;
@@ -1214,6 +1652,7 @@ EXPORT int32_t force_set_count(uint32_t h, uint32_t fh, uint32_t count) {  // ro
   Sim* s = get(h);
   Force* f = get_force(s, fh);
   if (!f) return BAD_HANDLE;
+  ++s->epoch;
   if (f->set_count(count) != hipSuccess) return OUT_OF_MEMORY;
   return GROWN;
 }
@@ -1222,6 +1661,7 @@ EXPORT void* force_ptr(uint32_t h, uint32_t fh, uint32_t which) {  // root.zig:1
   Sim* s = get(h);
   Force* f = get_force(s, fh);
   if (!f || f->count == 0) return nullptr;
+  ++s->epoch;
   f->dirty = true;
   f->sched_valid = false;
   f->csr_valid = false;
@@ -1240,6 +1680,8 @@ EXPORT int32_t force_set_scalar(uint32_t h, uint32_t fh, uint32_t which, double 
   Force* f = get_force(s, fh);
   if (!f) return BAD_HANDLE;
   if (which >= S_COUNT) return BAD_ARGUMENT;
+  // a captured tick holds every scalar but the seed (it reads the seed on the device)
+  if (which != S_SEED && std::memcmp(&f->scalars[which], &v, sizeof v) != 0) ++s->epoch;
   f->scalars[which] = v;
   return OK;
 }
@@ -1258,26 +1700,8 @@ EXPORT int32_t force_run(uint32_t h, uint32_t fh, double alpha) {  // root.zig:2
   if (f->node_sized() && f->count < s->n) return BAD_ARGUMENT;
   int32_t injected;
   if (injected_failure(&injected)) return injected;
-  hipError_t e = upload(*s, *f);
-  if (e != hipSuccess) return status_of(e);
-  switch (f->kind) {
-    case K_CENTER: return run_center(*s, *f);
-    case K_X:
-      return status_of(axis_force(s->dx.p, s->dvx.p, f->df[0].p, f->df[1].p, s->n, alpha, s->stream));
-    case K_Y:
-      return status_of(axis_force(s->dy.p, s->dvy.p, f->df[0].p, f->df[1].p, s->n, alpha, s->stream));
-    case K_RADIAL:
-      return status_of(radial_force(s->dx.p, s->dy.p, s->dvx.p, s->dvy.p, f->df[0].p, f->df[1].p, s->n,
-                                    f->scalar(S_X), f->scalar(S_Y), alpha, s->stream));
-    case K_LINK:
-      return fast_mode(*s, *f) ? run_link_fast(*s, *f, alpha) : run_link_exact(*s, *f, alpha);
-    case K_MANY_BODY:
-      return fast_mode(*s, *f) ? run_many_body_fast(*s, *f, alpha) : run_many_body_exact(*s, *f, alpha);
-    case K_COLLIDE:
-      return fast_mode(*s, *f) ? run_collide_fast(*s, *f) : run_collide_exact(*s, *f);
-    default:
-      return BAD_ARGUMENT;
-  }
+  ++s->epoch;
+  return run_force(*s, *f, alpha);
 }
 
 EXPORT int32_t force_direct_sum(uint32_t h, uint32_t fh, double alpha, const uint32_t* sample, uint32_t k,

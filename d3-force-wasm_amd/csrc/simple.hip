@@ -569,9 +569,11 @@ __global__ __launch_bounds__(EW) void k_integrate(double* __restrict__ xs, doubl
 // simple.zig:51-62: v += (target - p) * strength * alpha.
 __global__ __launch_bounds__(EW) void k_axis(const double* __restrict__ pos, double* __restrict__ v,
                                              const double* __restrict__ strength,
-                                             const double* __restrict__ target, uint32_t n, double alpha) {
+                                             const double* __restrict__ target, uint32_t n, double alpha,
+                                             const double* __restrict__ alpha_dev) {
   const uint32_t i = blockIdx.x * EW + threadIdx.x;
   if (i >= n) return;
+  if (alpha_dev) alpha = *alpha_dev;
   v[i] += (target[i] - pos[i]) * strength[i] * alpha;
 }
 
@@ -580,9 +582,10 @@ __global__ __launch_bounds__(EW) void k_radial(const double* __restrict__ xs, co
                                                double* __restrict__ vxs, double* __restrict__ vys,
                                                const double* __restrict__ strength,
                                                const double* __restrict__ radius, uint32_t n, double cx, double cy,
-                                               double alpha) {
+                                               double alpha, const double* __restrict__ alpha_dev) {
   const uint32_t i = blockIdx.x * EW + threadIdx.x;
   if (i >= n) return;
+  if (alpha_dev) alpha = *alpha_dev;
   double dx = xs[i] - cx;
   if (dx == 0 || dx != dx) dx = 1e-6;
   double dy = ys[i] - cy;
@@ -622,6 +625,44 @@ __global__ __launch_bounds__(EW) void k_snapshot(const double* __restrict__ vxs,
     out[2 * (size_t)n + i] = xs[i];
     out[3 * (size_t)n + i] = ys[i];
   }
+}
+
+// Captured ticks: the start of a replay (one thread).
+__global__ void k_tick_begin(TickState* __restrict__ st, const TickParams* __restrict__ params,
+                             uint32_t* __restrict__ zero, uint32_t nzero) {
+  if (threadIdx.x != 0) return;
+  const TickParams p = params[st->tick];
+  st->alpha = p.alpha;
+  for (int k = 0; k < TICK_SEEDS; ++k) st->seed[k] = p.seed[k];
+  for (uint32_t k = 0; k < nzero; ++k) zero[k] = 0;
+}
+
+// The tick's starting state into snap (vx, vy, x, y), unless an earlier
+// replay of the run failed: snap then keeps the state that replay started from.
+__global__ __launch_bounds__(EW) void k_tick_snapshot(const TickState* __restrict__ st, const double* __restrict__ xs,
+                                                      const double* __restrict__ ys, const double* __restrict__ vxs,
+                                                      const double* __restrict__ vys, uint32_t n,
+                                                      double* __restrict__ out) {
+  if (st->first_fail != NONE) return;
+  for (uint32_t i = blockIdx.x * EW + threadIdx.x; i < n; i += gridDim.x * EW) {
+    out[i] = vxs[i];
+    out[(size_t)n + i] = vys[i];
+    out[2 * (size_t)n + i] = xs[i];
+    out[3 * (size_t)n + i] = ys[i];
+  }
+}
+
+// The end of a replay: the first tick with a raised flag is kept.
+__global__ void k_tick_end(TickState* __restrict__ st, TickNotes notes) {
+  if (threadIdx.x != 0) return;
+  uint32_t status = 0;
+  for (uint32_t k = 0; k < notes.count; ++k)
+    if (*notes.flag[k]) status |= notes.bit[k];
+  if (status && st->first_fail == NONE) {
+    st->first_fail = st->tick;
+    st->status = status;
+  }
+  st->tick += 1;
 }
 
 }  // namespace
@@ -674,6 +715,24 @@ static hipError_t center_reserve(CenterWork& w, uint32_t nch) {
   HIPCHECK(hipMalloc(&w.count, sizeof(uint32_t)));
   w.cap = cap;
   return hipSuccess;
+}
+
+hipError_t tick_begin(TickState* st, const TickParams* params, uint32_t* zero, uint32_t nzero, hipStream_t stream) {
+  hipLaunchKernelGGL(k_tick_begin, dim3(1), dim3(64), 0, stream, st, params, zero, nzero);
+  return hipGetLastError();
+}
+
+hipError_t tick_snapshot(const TickState* st, const double* x, const double* y, const double* vx, const double* vy,
+                         uint32_t n, double* snap, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tick_snapshot, dim3(std::min(ceil_div(n, (uint32_t)EW), 2048u)), dim3(EW), 0, stream, st, x, y,
+                     vx, vy, n, snap);
+  return hipGetLastError();
+}
+
+hipError_t tick_end(TickState* st, const TickNotes& notes, hipStream_t stream) {
+  hipLaunchKernelGGL(k_tick_end, dim3(1), dim3(64), 0, stream, st, notes);
+  return hipGetLastError();
 }
 
 hipError_t center_raw_sum(const double* x, const double* y, uint32_t n, double* sums, CenterWork& w,
@@ -747,17 +806,19 @@ hipError_t integrate(double* x, double* y, double* vx, double* vy, const double*
 }
 
 hipError_t axis_force(const double* pos, double* v, const double* strength, const double* target, uint32_t n,
-                      double alpha, hipStream_t stream) {
+                      double alpha, hipStream_t stream, const double* alpha_dev) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_axis, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, pos, v, strength, target, n, alpha);
+  hipLaunchKernelGGL(k_axis, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, pos, v, strength, target, n, alpha,
+                     alpha_dev);
   return hipGetLastError();
 }
 
 hipError_t radial_force(const double* x, const double* y, double* vx, double* vy, const double* strength,
-                        const double* radius, uint32_t n, double cx, double cy, double alpha, hipStream_t stream) {
+                        const double* radius, uint32_t n, double cx, double cy, double alpha, hipStream_t stream,
+                        const double* alpha_dev) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_radial, dim3(ceil_div(n, EW)), dim3(EW), 0, stream, x, y, vx, vy, strength, radius, n, cx,
-                     cy, alpha);
+                     cy, alpha, alpha_dev);
   return hipGetLastError();
 }
 

@@ -147,6 +147,8 @@ __global__ __launch_bounds__(BOX_THREADS) void k_box_final(uint32_t n, uint32_t 
     meta->ncut = 0;
     meta->ntop = 0;
     meta->topdepth = 0;
+    meta->long_group = 0;
+    meta->stop = 0;
     if (r.x0 > r.x1 || r.y0 > r.y1) {  // no finite point (tree.zig:338)
       meta->m = 0;
       meta->valid = 0;
@@ -754,7 +756,7 @@ __global__ __launch_bounds__(EW_THREADS) void k_assign(const uint32_t* __restric
                                                        const uint32_t* __restrict__ base,
                                                        uint32_t* __restrict__ cell_s, uint32_t* __restrict__ cell_d) {
   const uint32_t s = blockIdx.x * EW_THREADS + threadIdx.x;
-  if (s >= meta->m || !start[s]) return;
+  if (s >= meta->m || !start[s] || meta->stop) return;
   const int dl = dleft[s];
   const int lmax = mixed[s] ? KEY_LEVELS - 1 : (int)leafdepth[s];
   const uint32_t b = base[s];
@@ -798,7 +800,7 @@ __global__ __launch_bounds__(EW_THREADS) void k_emit(const uint64_t* __restrict_
                                                      uint32_t* __restrict__ cdepth, uint32_t* __restrict__ cskip,
                                                      uint2* __restrict__ cut, uint32_t* __restrict__ top) {
   const uint32_t idx = blockIdx.x * EW_THREADS + threadIdx.x;
-  if (idx >= meta->len) return;
+  if (idx >= meta->len || meta->stop) return;
   const uint32_t s = cell_s[idx];
   if (s == NONE) return;  // a cell below key resolution (k_deep)
   const int d = (int)cell_d[idx];
@@ -914,6 +916,26 @@ static hipError_t grow(T*& p, size_t n) {
   return hipMalloc(&p, n * sizeof(T));
 }
 
+namespace {
+
+// Capturable build, once the cell count is known: the trees it leaves to the
+// eager build (tree_build_exact) stop here.
+__global__ void k_tree_gate(TreeMeta* __restrict__ meta, uint32_t cell_cap) {
+  if (threadIdx.x != 0) return;
+  if (!meta->valid || meta->mixed || meta->overflow || meta->long_group || meta->len > cell_cap) meta->stop = 1;
+}
+
+// cell_s[0 .. len) = NONE (the cells k_assign does not name), with the grid
+// sized by the capacity.
+__global__ __launch_bounds__(EW_THREADS) void k_fill_cells(uint32_t* __restrict__ cell_s,
+                                                           const TreeMeta* __restrict__ meta) {
+  if (meta->stop) return;
+  const uint32_t len = meta->len;
+  for (uint32_t i = blockIdx.x * EW_THREADS + threadIdx.x; i < len; i += gridDim.x * EW_THREADS) cell_s[i] = NONE;
+}
+
+}  // namespace
+
 hipError_t tree_reserve(TreeBuffers& t, uint32_t n) {
   if (!t.meta) {
     HIPCHECK(hipMalloc(&t.meta, sizeof(TreeMeta)));
@@ -975,7 +997,8 @@ void tree_free(TreeBuffers& t) {
   t = TreeBuffers();
 }
 
-hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, uint32_t n, hipStream_t stream) {
+hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, uint32_t n, hipStream_t stream,
+                            bool graph) {
   HIPCHECK(tree_reserve(t, n > 0 ? n : 1));
   const uint32_t nb = ceil_div(n > 0 ? n : 1, EW_THREADS);
   uint32_t box_blocks = ceil_div(n > 0 ? n : 1, BOX_THREADS * 4);
@@ -999,16 +1022,18 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
   const uint32_t* mptr = &t.meta->m;
   HIPCHECK(sort_digits(t.hist, t.rs_glob, t.keys, t.perm, t.keys_tmp, t.perm_tmp, n > 0 ? n : 1, mptr, 4, 4,
                        stream));
-  HIPCHECK(hipMemsetAsync(&t.meta->long_group, 0, sizeof(uint32_t), stream));
   hipLaunchKernelGGL(k_sort_low_halves, dim3(nb), dim3(EW_THREADS), 0, stream, t.keys, t.perm, t.meta);
-  HIPCHECK(hipMemcpyAsync(&t.meta_host->long_group, &t.meta->long_group, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                          stream));
-  HIPCHECK(hipStreamSynchronize(stream));
-  if (t.meta_host->long_group)
-    HIPCHECK(sort_digits(t.hist, t.rs_glob, t.keys, t.perm, t.keys_tmp, t.perm_tmp, n > 0 ? n : 1, mptr, 0, 8,
-                         stream));
+  const uint32_t zero_blocks = std::min(ceil_div(n > 0 ? n : 1, 256u), 4096u);
+  if (!graph) {
+    HIPCHECK(hipMemcpyAsync(&t.meta_host->long_group, &t.meta->long_group, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (t.meta_host->long_group)
+      HIPCHECK(sort_digits(t.hist, t.rs_glob, t.keys, t.perm, t.keys_tmp, t.perm_tmp, n > 0 ? n : 1, mptr, 0, 8,
+                           stream));
+  }
   // Key groups.
-  HIPCHECK(hipMemsetAsync(t.mixed, 0, (size_t)(n > 0 ? n : 1) * sizeof(uint32_t), stream));
+  hipLaunchKernelGGL(k_zero_words, dim3(zero_blocks), dim3(256), 0, stream, t.mixed, n > 0 ? n : 1);
   hipLaunchKernelGGL(k_group_a, dim3(nb), dim3(EW_THREADS), 0, stream, t.keys, t.perm, x, y, t.meta, t.flags,
                      t.mixed);
   hipLaunchKernelGGL(k_group_b, dim3(nb), dim3(EW_THREADS), 0, stream, t.keys, t.flags, t.mixed, t.meta, t.count,
@@ -1017,6 +1042,18 @@ hipError_t tree_build_exact(TreeBuffers& t, const double* x, const double* y, ui
                      t.mixed_list, t.dleft, t.gend, t.count, t.base, t.deep_stack, t.cells, t.cdepth);
   HIPCHECK(scan_exclusive(t.count, t.base, n, mptr, t.partials, &t.meta->len, nullptr, stream));
   HIPCHECK(hipGetLastError());
+  if (graph) {
+    const uint32_t cap = t.cell_cap;
+    hipLaunchKernelGGL(k_tree_gate, dim3(1), dim3(64), 0, stream, t.meta, cap);
+    hipLaunchKernelGGL(k_fill_cells, dim3(std::min(ceil_div(cap, (uint32_t)EW_THREADS), 4096u)), dim3(EW_THREADS), 0,
+                       stream, t.cell_s, t.meta);
+    hipLaunchKernelGGL(k_assign, dim3(nb), dim3(EW_THREADS), 0, stream, t.flags, t.mixed, t.meta, t.dleft,
+                       t.leafdepth, t.base, t.cell_s, t.cell_d);
+    hipLaunchKernelGGL(k_emit, dim3(ceil_div(cap, (uint32_t)EW_THREADS)), dim3(EW_THREADS), 0, stream, t.keys,
+                       t.mixed, t.meta, t.gend, t.leafdepth, t.base, t.cell_s, t.cell_d, t.cells, t.cdepth, t.cskip,
+                       t.cut, t.top);
+    return hipGetLastError();
+  }
   HIPCHECK(hipMemcpyAsync(t.meta_host, t.meta, sizeof(TreeMeta), hipMemcpyDeviceToHost, stream));
   HIPCHECK(hipStreamSynchronize(stream));
   if (t.meta_host->overflow) HIPFAIL(hipErrorOutOfMemory, "tree arrays overflowed");

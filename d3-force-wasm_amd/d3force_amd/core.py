@@ -32,6 +32,7 @@ SCALARS = 11
 FLOATS = [0, 1, 3, 1, 2, 2, 2]
 
 OK, GROWN = 0, 1
+NOT_CAPTURED = 2  # sim_ticks: nothing ran, use the per-force loop
 BAD_HANDLE, OUT_OF_MEMORY, BAD_ARGUMENT, WORKER_FAILED = -1, -2, -3, -4
 _STATUSES = {-1: "BAD_HANDLE", -2: "OUT_OF_MEMORY", -3: "BAD_ARGUMENT", -4: "WORKER_FAILED"}
 
@@ -144,6 +145,14 @@ class Core:
         if self.can_fail:
             self.core_fail_next = self.bind("core_fail_next", ctypes.c_int32,
                                             [ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32])
+        # Whole ticks captured as a HIP graph (a GPU-library extension).
+        self.sim_ticks = None
+        if hasattr(self.lib, prefix + "sim_ticks"):
+            self.sim_ticks = self.bind("sim_ticks", ctypes.c_int32,
+                                       [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_uint32)])
+            self.sim_tick_counters = self.bind("sim_tick_counters", ctypes.c_int32,
+                                               [ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)])
         if hasattr(self.lib, prefix + "force_direct_sum"):
             self.force_direct_sum = self.bind("force_direct_sum", ctypes.c_int32,
                                               [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_void_p,
@@ -328,6 +337,45 @@ class Context:
     def integrate(self, velocity_decay):  # core.js:161-163
         self.step(None, velocity_decay)
 
+    def ticks(self, forces, alphas, velocity_decay, rand):
+        """Whole ticks in one library call (sim_ticks): for each alpha, every
+        force of `forces` (native slots of this context, in order), then
+        integrate, as the per-force loop runs them, bit for bit and draw for
+        draw of `rand`. Returns the ticks run, or None when the library asks
+        for the per-force loop (nothing ran then)."""
+        core = self.core
+        if core.sim_ticks is None:
+            return None
+        for f in forces:
+            f.prepare()
+            f.slot.prepare(self.precision)
+        fhs = (ctypes.c_uint32 * len(forces))(*[f.slot.fh for f in forces])
+        a = np.ascontiguousarray(alphas, dtype=np.float64)
+        done = ctypes.c_uint32(0)
+        previous = core.random
+        core.random = rand
+        try:
+            s = core.sim_ticks(self.h, fhs, len(forces), len(a), a.ctypes.data, velocity_decay, ctypes.byref(done))
+        finally:
+            core.random = previous
+        if s == NOT_CAPTURED:
+            return None
+        if s < 0:
+            err = None
+            try:
+                check(s)
+            except CoreError as e:
+                err = e
+            err.ticks_done = int(done.value)
+            raise err
+        return int(done.value)
+
+    def tick_counters(self):
+        """(captures, replayed ticks, ticks rolled back and run eagerly)."""
+        out = (ctypes.c_uint64 * 3)()
+        check(self.core.sim_tick_counters(self.h, out))
+        return tuple(int(v) for v in out)
+
 
 def context(nodes):
     """core.js:166-170: the node table's Context in the table's core."""
@@ -376,6 +424,13 @@ class Slot:
 
     def scalar(self, which, v):  # core.js:209-211
         self.ctx.core.force_set_scalar(self.ctx.h, self.fh, which, float(v))
+
+    def prepare(self, fast):
+        """The precision scalars run() sets before a run (core.js:216-221)."""
+        self.scalar(PRECISION, 1 if fast else 0)
+        if fast:
+            self.scalar(ORDER, fast["order"])
+            self.scalar(OPENING, fast["opening"])
 
     def run(self, nodes, alpha, rand=None):  # core.js:213-231
         ctx = self.ctx

@@ -51,6 +51,9 @@ class _Force:
     def _initialize(self):
         pass
 
+    def prepare(self):
+        """The scalars a run sets before the core runs the force."""
+
 
 class ManyBody(_Force):
     """manyBody.js:4-54."""
@@ -65,6 +68,9 @@ class ManyBody(_Force):
     def __call__(self, alpha):  # manyBody.js:14-17
         self.push_scalars()
         self.slot.run(self.nodes, alpha, self.random)
+
+    def prepare(self):
+        self.push_scalars()
 
     def push_scalars(self):
         """theta^2, distanceMin^2 and distanceMax^2 into the core's force."""
@@ -113,8 +119,11 @@ class Link(_Force):
         self.size = 0
 
     def __call__(self, alpha):  # link.js:34-37
-        self.slot.scalar(ITERATIONS, self._iterations)
+        self.prepare()
         self.slot.run(self.nodes, alpha, self.random)
+
+    def prepare(self):
+        self.slot.scalar(ITERATIONS, self._iterations)
 
     def _initialize(self):  # link.js:39-67
         if self.nodes is None:
@@ -187,11 +196,14 @@ class Center(_Force):
         self._strength = 1.0
 
     def __call__(self, alpha=None):  # center.js:9-12
+        self.prepare()
+        self.slot.run(self.nodes, 0.0)
+
+    def prepare(self):
         s = self.slot
         s.scalar(CX, self._x)
         s.scalar(CY, self._y)
         s.scalar(STRENGTH, self._strength)
-        s.run(self.nodes, 0.0)
 
     def initialize(self, nodes, random=None):  # center.js:16-19
         self.nodes = nodes
@@ -285,10 +297,13 @@ class Radial(_Force):
         self._strength = 0.1
 
     def __call__(self, alpha):
+        self.prepare()
+        self.slot.run(self.nodes, alpha)
+
+    def prepare(self):
         s = self.slot
         s.scalar(CX, self._x)
         s.scalar(CY, self._y)
-        s.run(self.nodes, alpha)
 
     def initialize(self, nodes, random=None):
         self.nodes = nodes

@@ -7,11 +7,15 @@ native slot sees the nodes through a copy-out / copy-in around it.
 """
 
 import math
+import os
 
 import numpy as np
 
 from . import core as _core
 from .lcg import lcg
+
+# Ticks per sim_ticks call (their alphas are computed ahead on the host).
+TICK_RUN = 1024
 
 INITIAL_RADIUS = 10.0
 INITIAL_ANGLE = math.pi * (3 - math.sqrt(5))  # simulation.js:14-15
@@ -49,6 +53,9 @@ class Simulation:
         self._random = lcg()
         self._precision = "exact"
         self._fast = None
+        # Whole ticks as replays of a captured HIP graph where the core offers
+        # it (sim_ticks); D3F_TICK_GRAPH=0 keeps the per-force loop.
+        self._captured = os.environ.get("D3F_TICK_GRAPH", "1") != "0"
         self._initialize_nodes()
 
     # simulation.js:91-105
@@ -94,11 +101,63 @@ class Simulation:
         ctx.live = True
         return ctx
 
+    def captured_ticks(self, on=None):
+        """Whether tick() runs whole ticks as replays of a captured HIP graph
+        (the same bits as the per-force loop) when every force is native."""
+        if on is None:
+            return self._captured
+        self._captured = bool(on)
+        return self
+
+    def _native_forces(self, ctx):
+        """The forces in order when all run in the core and the random source
+        they draw from is the simulation's, else None."""
+        forces = list(self._forces.values())
+        for f in forces:
+            slot = getattr(f, "slot", None)
+            if slot is None or slot.ctx is not ctx or not hasattr(f, "prepare"):
+                return None
+            if getattr(f, "random", self._random) not in (None, self._random):
+                return None
+        return forces
+
+    def _run_captured(self, iterations, k, ctx):
+        """Ticks k.. of the loop through sim_ticks; returns the new k, or None
+        when the per-force loop must run them."""
+        forces = self._native_forces(ctx)
+        if not forces:
+            return None
+        count = 0
+        alphas = []
+        a = self._alpha
+        while k + count < iterations and count < TICK_RUN:  # simulation.js:63-64, ahead
+            count += 1
+            a += (self._alpha_target - a) * self._alpha_decay
+            alphas.append(a)
+        try:
+            done = ctx.ticks(forces, alphas, self._velocity_decay, self._random)
+        except Exception as e:
+            d = getattr(e, "ticks_done", 0)
+            if d:
+                self._alpha = alphas[d - 1]
+            raise
+        if done is None:
+            return None
+        self._alpha = alphas[-1]
+        return k + count
+
     def iterate(self, iterations, ctx):
         """``iterations`` ticks on the core's resident arrays (simulation.js:63-82)."""
         nodes = self._nodes
         k = 0
+        captured = self._captured
         while k < iterations:  # simulation.js:63: for (k = 0; k < iterations; ++k)
+            if captured:
+                nk = self._run_captured(iterations, k, ctx)
+                if nk is not None:
+                    k = nk
+                    continue
+                captured = False  # this call's ticks run force by force
             k += 1
             self._alpha += (self._alpha_target - self._alpha) * self._alpha_decay
             for force in list(self._forces.values()):

@@ -70,6 +70,26 @@ int32_t sim_sync(uint32_t h, uint32_t direction);
 int32_t sim_snapshot(uint32_t h, uint32_t positions);
 /* root.zig:250-254: the end of d3's tick (velocity decay, fixed nodes). */
 int32_t sim_integrate(uint32_t h, double velocity_decay);
+/* GPU: `ticks` whole ticks at once (no reference counterpart; it replaces the
+ * facade's per-tick loop, simulation.js:63-82, of force_run for each force in
+ * `forces` order with alpha = alphas[k], then sim_integrate). The result is
+ * the loop's, bit for bit, including the random source's draws (a SEED =
+ * random() per seeded force per tick in fast precision, as core.js:213-231
+ * draws it, and every jiggle in the eager order); the caller sets each
+ * force's PRECISION, ORDER and OPENING scalars first. The tick is captured
+ * once as a HIP graph and replayed; a replay that meets work only the eager
+ * tick does (jiggles, buffers outgrown, a declined FMM) is rolled back to its
+ * start and that tick runs eagerly. Returns D3F_NOT_CAPTURED (2), having run
+ * nothing, when the per-force loop must be used instead: timing on, a
+ * failure injection pending, fewer than 2 nodes, a forceCollide among the
+ * forces, or more than one manyBody or link force. *done receives the ticks
+ * run (all of them unless a status < 0 is returned). */
+#define D3F_NOT_CAPTURED 2
+int32_t sim_ticks(uint32_t h, const uint32_t* forces, uint32_t nforces, uint32_t ticks, const double* alphas,
+                  double velocity_decay, uint32_t* done);
+/* GPU: out[0] ticks captured, out[1] ticks replayed, out[2] replays rolled
+ * back and run eagerly, since sim_create (no reference counterpart). */
+int32_t sim_tick_counters(uint32_t h, uint64_t* out);
 
 /* root.zig:102-108: builds the exact quadtree on x, y (shifted = 0) or on
  * x + vx, y + vy (1). Returns GROWN when tree arrays moved. */
