@@ -1866,7 +1866,7 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
                      f.ps, (float2*)f.mexp, f.mid, f.mid_cap, f.params);
   hipLaunchKernelGGL((k_fmm_up_big<Q, false>), dim3(1), dim3(1024), 0, stream, f.cells, f.bbox, f.ux, f.uy, f.ps,
                      (float2*)f.mexp, f.big, f.big_cap, f.params);
-  const bool aux = !g && a.aux;
+  const bool aux = a.aux != nullptr;  // captured too: the fork and join become graph edges
   hipStream_t mps = stream;
   if (aux) {
     if (!f.ev_geom) HIPCHECK(hipEventCreateWithFlags(&f.ev_geom, hipEventDisableTiming));

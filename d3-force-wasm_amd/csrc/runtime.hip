@@ -1138,6 +1138,7 @@ hipError_t graph_tick(Sim& s, const std::vector<Force*>& fs, double decay) {
           a.alpha_dev = alpha_dev;
           a.seeds_dev = seed_dev;
           a.graph = true;
+          a.aux = s.aux;
           HIPCHECK(fmm_run(s.fmm, s.tree, a, st));
           note(&s.fmm.params->abort, TF_FMM_ABORT);
           note(&s.fmm.params->declined, TF_FMM_DECLINED);
