@@ -536,14 +536,15 @@ def main():
     #   traversal  8 V + 8 A + 6 P FP64 flops (V visits, A accepted cells,
     #              P leaf pairs, counted by the kernel itself)
     #   center_sum 16 B/node read; 2n dependent FP64 adds in d3's order
-    #   tree       28 B/node + 88 B/cell;  accumulate 88 B/cell + 24 B/node
+    #   tree       28 B/node + 88 B/cell;  accumulate 152 B/cell (88-byte cells read,
+    #              64-byte walk records written) + 24 B/node
     #   link       128 B/link;  integrate 81 B/node
     phase_ms = st["ms"]
     C = cells
-    hbm_bytes = {"tree": 28 * n + 88 * C, "accumulate": 88 * C + 24 * n, "link": 128 * m,
+    hbm_bytes = {"tree": 28 * n + 88 * C, "accumulate": 152 * C + 24 * n, "link": 128 * m,
                  "integrate": 81 * n, "center_sum": 16 * n}
     names = {"traversal": "k_traverse<RECORD> (Barnes-Hut traversal)", "center_sum": "k_cs_chain (center sums, side stream)",
-             "tree": "quadtree build (13 kernels)", "accumulate": "k_accumulate_level (all levels)",
+             "tree": "quadtree build (13 kernels)", "accumulate": "k_acc_subtrees+k_acc_top (two launches)",
              "link": "k_link_level (all levels)", "integrate": "k_integrate"}
     kernels = {}
     floors = {}
@@ -574,7 +575,8 @@ def main():
             else:
                 floors[ph] = hbm_bytes[ph] / (HBM_PEAK_GBS * 1e9) * launches
         k["launch_ms"] = t * 1e3
-        k["traffic"] = read_traffic(names[ph].split(" ")[0].split("<")[0])
+        parts = [read_traffic(kn.split("<")[0]) for kn in names[ph].split(" ")[0].split("+")]
+        k["traffic"] = None if None in parts else sum(parts)
         kernels[ph] = k
     # The dominant kernel is taken on the tick's critical path: the center sums
     # run on a second stream, overlapped with the tree build and traversal.
