@@ -1847,10 +1847,16 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
     if (nc > f.cell_cap || nc > f.cellarr_cap || (size_t)nc * Q > f.exp_cap)
       HIPFAIL(hipErrorInvalidValue, "cells beyond the reserved arrays");
   }
+  // Per-cell launches: count bounds the cells (the capacity in the
+  // capturable form, whose kernels loop over the device's count).
   auto grid = [&](uint32_t count, uint32_t per, uint32_t cap_blocks) -> uint32_t {
-    if (g) return cap_blocks;
-    const uint32_t b = ceil_div(count, per);
-    return b > 0 ? b : 1;
+    const uint32_t b = std::max(1u, ceil_div(count, per));
+    return g ? std::min(cap_blocks, b) : b;
+  };
+  // Per-pair launches: the capturable form cannot bound the pairs by the
+  // last eager count, so its blocks loop over them.
+  auto grid_pairs = [&](uint32_t count, uint32_t per, uint32_t cap_blocks) -> uint32_t {
+    return g ? cap_blocks : std::max(1u, ceil_div(count, per));
   };
   // upward: the geometry first (the dual traversal reads it), then the
   // multipoles, which M2L reads: with an auxiliary stream (the eager form)
@@ -1858,8 +1864,8 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   // to FMM_DIRECT points from their points (8 lanes a cell up to UP_SMALL
   // points, else a wave; the capturable form's level_blocks blocks loop over
   // them), the larger ones from their children in one workgroup.
-  const uint32_t small_blocks = g ? f.level_blocks : std::max(1u, std::min(1u << 20, ceil_div(nc, 32)));
-  const uint32_t mid_blocks = g ? f.level_blocks : std::max(1u, std::min(f.level_blocks, ceil_div(nc, 16)));
+  const uint32_t small_blocks = std::max(1u, std::min(g ? f.level_blocks : 1u << 20, ceil_div(nc, 32)));
+  const uint32_t mid_blocks = std::max(1u, std::min(f.level_blocks, ceil_div(nc, 16)));
   hipLaunchKernelGGL((k_fmm_up_small<Q, false>), dim3(small_blocks), dim3(256), 0, stream, f.cells, f.bbox, f.ux,
                      f.uy, f.ps, (float2*)f.mexp, f.params);
   hipLaunchKernelGGL((k_fmm_up_mid<Q, false>), dim3(mid_blocks), dim3(256), 0, stream, f.cells, f.bbox, f.ux, f.uy,
@@ -1960,12 +1966,12 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   HIPCHECK(scan_u32(f.p2p_cnt, f.p2p_off, scan_max, ncp, f.scan_partials, nullptr, stream));
   if (g || P.m2l) {
     hipLaunchKernelGGL(k_fmm_zero, dim3(grid(nc, 256, CAP_CELL_BLOCKS)), dim3(256), 0, stream, f.cursor, ncp);
-    hipLaunchKernelGGL(k_target_scatter, dim3(grid(P.m2l, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.m2l,
+    hipLaunchKernelGGL(k_target_scatter, dim3(grid_pairs(P.m2l, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.m2l,
                        &f.params->m2l, nbits, f.m2l_off, f.cursor, f.m2l_src, f.params);
   }
   if (g || P.p2p) {
     hipLaunchKernelGGL(k_fmm_zero, dim3(grid(nc, 256, CAP_CELL_BLOCKS)), dim3(256), 0, stream, f.cursor, ncp);
-    hipLaunchKernelGGL(k_target_scatter, dim3(grid(P.p2p, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.p2p,
+    hipLaunchKernelGGL(k_target_scatter, dim3(grid_pairs(P.p2p, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.p2p,
                        &f.params->p2p, nbits, f.p2p_off, f.cursor, f.p2p_src, f.params);
   }
   // segments too long for the in-LDS order, ordered in place
