@@ -372,9 +372,17 @@ struct Batch {
                        dseeds_mb.p, dseeds_link.p);
     const bool timed = g_batch_timing && !graph;
     hipEvent_t ev = nullptr;
-    // the small graphs' exact manyBody and link, from the tick's starting state
+    // many-body; the small graphs' exact manyBody and link (from the tick's
+    // starting state, which the FMM leaves alone for their nodes) start on
+    // the side stream once the FMM's cells are built, beside its traversal
+    // and field passes rather than beside its sort
+    FmmArgs a = fmm_args();
+    a.graph = graph;
+    if (!small.empty()) a.built = fork;
+    hipEvent_t evf = timed ? mark() : nullptr;
+    if ((e = fmm_run(fmm, tree, a, stream)) != hipSuccess) return e;
+    timer(D3F_PHASE_FMM, evf);
     if (!small.empty()) {
-      HIPCHECK(hipEventRecord(fork, stream));
       HIPCHECK(hipStreamWaitEvent(side, fork, 0));
       ev = timed ? mark(side) : nullptr;
       SmallExactArgs sa;
@@ -405,12 +413,6 @@ struct Batch {
       timer(D3F_PHASE_TRAVERSAL, ev, side);
       HIPCHECK(hipEventRecord(join, side));
     }
-    // many-body
-    FmmArgs a = fmm_args();
-    a.graph = graph;
-    ev = timed ? mark() : nullptr;
-    if ((e = fmm_run(fmm, tree, a, stream)) != hipSuccess) return e;
-    timer(D3F_PHASE_FMM, ev);
     if (!small.empty()) HIPCHECK(hipStreamWaitEvent(stream, join, 0));
     // link
     ev = timed ? mark() : nullptr;

@@ -533,6 +533,9 @@ struct FmmArgs {
   // Eager form only: when set, the multipoles run on this stream beside the
   // dual traversal (forked after the geometry, joined before M2L).
   hipStream_t aux = nullptr;
+  // when set, recorded on the stream once the cells are built (a caller's
+  // own work that reads only x, y can start there)
+  hipEvent_t built = nullptr;
 };
 // Fast many-body (fmm.zig:247-354) over every graph of the forest at once.
 // With one graph, returns hipErrorNotSupported when the FMM declines (a

@@ -2133,6 +2133,7 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
     HIPCHECK(grow_cells(f, std::max(2 * f.cell_cap, f.params_host->level_off[FMM_LEVELS + 1] + 4096)));
     hipLaunchKernelGGL(k_forest_reset_overflow, dim3(1), dim3(64), 0, stream, f.params);
   }
+  if (a.built) HIPCHECK(hipEventRecord(a.built, stream));
   if (!a.graph) {
     const FmmParams& P = *f.params_host;
     uint32_t deepest = 0;
