@@ -1162,7 +1162,7 @@ hipError_t graph_tick(Sim& s, const std::vector<Force*>& fs, double decay) {
           a.flagged = s.flagged.p;
           a.n_flagged = s.dscalar_u;
           a.len = t.cell_cap;
-          a.counters = s.dcounters;
+          a.counters = nullptr;  // the work counters are taken by the ticks run force by force
           a.meta = t.meta;
           a.alpha_dev = alpha_dev;
           HIPCHECK(mb_traverse(a, TRAV_RECORD, st));

@@ -341,8 +341,7 @@ __global__ __launch_bounds__(256) void k_zero_words(uint32_t* __restrict__ p, ui
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ keys,
                                                         const uint32_t* __restrict__ count,
-                                                        uint32_t* __restrict__ hist, uint32_t* __restrict__ glob,
-                                                        uint32_t ntiles, int shift) {
+                                                        uint32_t* __restrict__ hist, uint32_t ntiles, int shift) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -355,28 +354,18 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restri
   }
   __syncthreads();
   hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
-  if (h[threadIdx.x]) atomicAdd(&glob[threadIdx.x], h[threadIdx.x]);
 }
 
-// Digit d's tile offsets: (keys of smaller digits) + exclusive scan of its
-// tile counts, one block per digit.
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scan(uint32_t* __restrict__ hist,
-                                                        const uint32_t* __restrict__ glob, uint32_t ntiles) {
+// Digit d's tile offsets within the digit (exclusive scan of its tile
+// counts) and the digit's total in glob[d], one block per digit (the digit's
+// first slot is added by the scatter: no atomics on the totals).
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scan(uint32_t* __restrict__ hist, uint32_t* __restrict__ glob,
+                                                        uint32_t ntiles) {
   __shared__ uint32_t wsum[RS_THREADS / 64];
   __shared__ uint32_t carry;
   const uint32_t d = blockIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  // offset of digit d = sum of glob[0..d)
-  uint32_t v = threadIdx.x < d ? glob[threadIdx.x] : 0;
-#pragma unroll
-  for (int k = 32; k > 0; k >>= 1) v += __shfl_xor(v, k, 64);
-  if (lane == 0) wsum[wave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t o = 0;
-    for (int w = 0; w < RS_THREADS / 64; ++w) o += wsum[w];
-    carry = o;
-  }
+  if (threadIdx.x == 0) carry = 0;
   __syncthreads();
   uint32_t* row = hist + (size_t)d * ntiles;
   for (uint32_t b0 = 0; b0 < ntiles; b0 += RS_THREADS) {
@@ -398,6 +387,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scan(uint32_t* __restrict__ h
     if (threadIdx.x == RS_THREADS - 1) carry = before + inc;
     __syncthreads();
   }
+  if (threadIdx.x == 0) glob[d] = carry;
 }
 
 // Each wave ranks its RS_TILE / 4 consecutive keys in RS_ITEMS slots of 64
@@ -409,12 +399,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
                                                            const uint32_t* __restrict__ vin,
                                                            uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                            const uint32_t* __restrict__ hist,
+                                                           const uint32_t* __restrict__ glob,
                                                            const uint32_t* __restrict__ count, uint32_t ntiles,
                                                            int shift) {
+  __shared__ uint32_t dbase[256];       // first output slot of each digit
   __shared__ uint32_t wcount[RS_THREADS / 64][256];
   __shared__ uint32_t wbase[RS_THREADS / 64][256];
   __shared__ uint32_t tstart[256];      // first slot of each digit in the tile
-  __shared__ uint32_t wsum[RS_THREADS / 64];
+  __shared__ uint32_t wsum[RS_THREADS / 64], gsum[RS_THREADS / 64];
   __shared__ uint64_t skey[RS_TILE];    // the tile in digit order
   __shared__ uint32_t sval[RS_TILE];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -458,19 +450,32 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
       wbase[w][d] = b;
       b += wcount[w][d];
     }
-    uint32_t inc = b;
+    // two exclusive scans over the digits: the tile's counts (each digit's
+    // first slot in the tile) and the digit totals (its first output slot)
+    const uint32_t gt = glob[d];
+    uint32_t inc = b, ginc = gt;
 #pragma unroll
     for (int k = 1; k < 64; k <<= 1) {
-      const uint32_t t = __shfl_up(inc, k, 64);
-      if (lane >= k) inc += t;
+      const uint32_t t = __shfl_up(inc, k, 64), gu = __shfl_up(ginc, k, 64);
+      if (lane >= k) {
+        inc += t;
+        ginc += gu;
+      }
     }
-    if (lane == 63) wsum[wave] = inc;
+    if (lane == 63) {
+      wsum[wave] = inc;
+      gsum[wave] = ginc;
+    }
     __syncthreads();
-    uint32_t before = 0;
+    uint32_t before = 0, gbefore = 0;
 #pragma unroll
     for (int w = 0; w < RS_THREADS / 64; ++w)
-      if (w < wave) before += wsum[w];
+      if (w < wave) {
+        before += wsum[w];
+        gbefore += gsum[w];
+      }
     tstart[d] = before + inc - b;
+    dbase[d] = gbefore + ginc - gt;
   }
   __syncthreads();
 #pragma unroll
@@ -492,7 +497,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
     if (t < tile_n) {
       const uint64_t kk = skey[t];
       const uint32_t d = (uint32_t)(kk >> shift) & 255u;
-      const uint32_t pos = hist[d * ntiles + blockIdx.x] + (t - tstart[d]);
+      const uint32_t pos = dbase[d] + hist[d * ntiles + blockIdx.x] + (t - tstart[d]);
       kout[pos] = kk;
       vout[pos] = sval[t];
     }
@@ -649,7 +654,12 @@ __global__ __launch_bounds__(EW_THREADS) void k_group_b(const uint64_t* __restri
   atomicMax(&bmax, L);
   }
   __syncthreads();
-  if (threadIdx.x == 0 && bmax) atomicMax(&meta->depth, bmax);
+  // One block's atomic on the shared word only when it could raise it:
+  // atomics on one address serialize, and the depth stops rising after the
+  // first blocks (a stale read only costs an atomic).
+  if (threadIdx.x == 0 && bmax &&
+      bmax > __hip_atomic_load(&meta->depth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(&meta->depth, bmax);
 }
 
 // ---------------------------------------------------------------- below key resolution
@@ -882,13 +892,12 @@ hipError_t sort_digits(uint32_t* hist, uint32_t* glob, uint64_t* keys, uint32_t*
   const uint32_t ntiles = ceil_div(n_max > 0 ? n_max : 1, RS_TILE);
   uint64_t *ka = keys, *kb = keys_tmp;
   uint32_t *va = vals, *vb = vals_tmp;
-  hipLaunchKernelGGL(k_zero_words, dim3(8), dim3(256), 0, stream, glob, (uint32_t)(digits * 256));
   for (int pass = 0; pass < digits; ++pass) {
     uint32_t* g = glob + 256 * pass;
     const int shift = 8 * (first_digit + pass);
-    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, count, hist, g, ntiles, shift);
+    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, count, hist, ntiles, shift);
     hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(RS_THREADS), 0, stream, hist, g, ntiles);
-    hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, va, kb, vb, hist, count,
+    hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, va, kb, vb, hist, g, count,
                        ntiles, shift);
     std::swap(ka, kb);
     std::swap(va, vb);
