@@ -549,7 +549,7 @@ int32_t run_many_body_exact(Sim& s, Force& f, double alpha) {
   a.flagged = s.flagged.p;
   a.n_flagged = s.dscalar_u;
   a.len = t.meta_host->len;
-  a.counters = s.dcounters;
+  a.counters = g_timing ? s.dcounters : nullptr;  // work counters while timing (atomics on three words)
   if ((e = hipMemsetAsync(s.dscalar_u, 0, sizeof(uint32_t), s.stream)) != hipSuccess) return status_of(e);
   s.timer_start(D3F_PHASE_TRAVERSAL);
   if ((e = mb_traverse(a, TRAV_RECORD, s.stream)) != hipSuccess) return status_of(e);

@@ -122,9 +122,10 @@ int32_t force_run(uint32_t h, uint32_t fh, double alpha);
 /* Library introspection (no reference counterpart), for the benchmark's
  * roofline: with timing on, every phase launch is bracketed by HIP events on
  * the simulation's stream (no extra synchronisation); sim_stats waits for the
- * stream, adds the elapsed times up per phase and resets them. The traversal
- * counts its algorithmic work always: non-empty cell visits, accepted
- * far-field cells and leaf pair interactions, summed over nodes. */
+ * stream, adds the elapsed times up per phase and resets them. While timing
+ * is on, the traversal also counts its algorithmic work: non-empty cell
+ * visits, accepted far-field cells and leaf pair interactions, summed over
+ * nodes. */
 enum { D3F_PHASE_TREE = 0, D3F_PHASE_ACCUMULATE = 1, D3F_PHASE_TRAVERSAL = 2, D3F_PHASE_LINK = 3,
        D3F_PHASE_CENTER = 4, D3F_PHASE_INTEGRATE = 5, D3F_PHASE_CENTER_SUM = 6, D3F_PHASE_FMM = 7,
        D3F_PHASE_COLLIDE = 8, D3F_PHASES = 9 };
@@ -134,7 +135,7 @@ enum { D3F_PHASE_TREE = 0, D3F_PHASE_ACCUMULATE = 1, D3F_PHASE_TRAVERSAL = 2, D3
 typedef struct {
   double ms[D3F_PHASES];        /* summed kernel time per phase */
   uint64_t launches[D3F_PHASES];
-  uint64_t visits, accepts, pairs;  /* Barnes-Hut work since the last call */
+  uint64_t visits, accepts, pairs;  /* Barnes-Hut work since the last call (counted while timing is on) */
   uint64_t flagged;             /* many-body nodes replayed for jiggle */
   uint64_t link_levels;         /* levels of the last exact link schedule */
   uint64_t m2l, p2p_pairs;      /* fast many-body: M2L cell pairs, P2P point pairs */
