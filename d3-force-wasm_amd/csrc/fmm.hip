@@ -1010,21 +1010,33 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
     o_m2l += base_sh[1];
     o_p2p += base_sh[2];
     __syncthreads();
+    const uint32_t lane = threadIdx.x & 63;
 #pragma unroll
     for (int q = 0; q < STEP_PER; ++q) {
       const uint32_t kind = code[q] & 3, push = (code[q] >> 2) & 3, nch = code[q] >> 4;
       const uint32_t A = pa[q], B = pb[q], c0 = pc0[q];
       const uint64_t key = (uint64_t)A << mac.nbits | B;
+      // Per-target counts, one atomic per run of lanes holding the same A
+      // (the pairs a split pushes are adjacent, so such runs are common).
+      const uint32_t prev = __shfl_up(A, 1, 64);
+      const bool head = lane == 0 || prev != A;
+      const uint64_t heads = __ballot(head), is_m2l = __ballot(kind == 1), is_p2p = __ballot(kind == 2);
+      if (head) {
+        const uint64_t above = heads & ~((2ull << lane) - 1ull);  // heads after this lane
+        const uint64_t upto = above ? (above & (0ull - above)) - 1ull : ~0ull;  // lanes before the next head
+        const uint64_t run = upto & ~((1ull << lane) - 1ull);
+        const uint32_t nm = (uint32_t)__popcll(is_m2l & run), np = (uint32_t)__popcll(is_p2p & run);
+        if (nm) atomicAdd(&m2l_cnt[A], nm);
+        if (np) atomicAdd(&p2p_cnt[A], np);
+      }
       if (kind == 1) {
         if (o_m2l < m2l_cap) m2l[o_m2l] = key;
         else atomicOr(&fp->list_overflow, 2u);
         ++o_m2l;
-        atomicAdd(&m2l_cnt[A], 1u);
       } else if (kind == 2) {
         if (o_p2p < p2p_cap) p2p[o_p2p] = key;
         else atomicOr(&fp->list_overflow, 4u);
         ++o_p2p;
-        atomicAdd(&p2p_cnt[A], 1u);
       }
       if (push) {
         const uint32_t np = push == 1 ? nch * nch : nch;
