@@ -964,6 +964,7 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
   if (fp->abort || (fp->list_overflow & 1u)) return;
   uint32_t cnt = fp->front[step];
   if (cnt > front_cap) cnt = front_cap;
+  unsigned long long block_pairs = 0;  // thread 0: the block's P2P point pairs, added once at the end
   for (uint32_t tile = blockIdx.x * STEP_TILE; tile < cnt; tile += gridDim.x * STEP_TILE) {
     // kind: 0 none, 1 M2L, 2 P2P; push: 0 none, 1 children x children,
     // 2 (A, children of B), 3 (children of A, B)
@@ -1001,11 +1002,8 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
       base_sh[2] = t_p2p ? atomicAdd(&fp->p2p, t_p2p) : 0;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long ps = 0;
-      for (int w = 0; w < EWF / 64; ++w) ps += pairs_sh[w];
-      if (ps) atomicAdd(&fp->pairs, ps);
-    }
+    if (threadIdx.x == 0)
+      for (int w = 0; w < EWF / 64; ++w) block_pairs += pairs_sh[w];
     o_front += base_sh[0];
     o_m2l += base_sh[1];
     o_p2p += base_sh[2];
@@ -1054,6 +1052,7 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
       }
     }
   }
+  if (threadIdx.x == 0 && block_pairs) atomicAdd(&fp->pairs, block_pairs);
 }
 
 // The first frontier: every root paired with itself; the list counters
@@ -1089,10 +1088,23 @@ __global__ __launch_bounds__(EWF) void k_target_scatter(const uint64_t* __restri
                                                         const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
                                                         uint32_t* __restrict__ src, const FmmParams* __restrict__ fp) {
   if (fp->abort) return;
-  for (uint32_t k = blockIdx.x * EWF + threadIdx.x; k < *count; k += gridDim.x * EWF) {
-    const uint64_t key = list[k];
-    const uint32_t a = (uint32_t)(key >> nbits);
-    src[off[a] + atomicAdd(&cursor[a], 1u)] = (uint32_t)(key & (((uint64_t)1 << nbits) - 1));
+  // Consecutive keys often share a target: one cursor atomic per run of
+  // lanes with the same target, each lane placed by its rank in the run.
+  const uint32_t n = *count, lane = threadIdx.x & 63;
+  for (uint32_t k0 = blockIdx.x * EWF; k0 < n; k0 += gridDim.x * EWF) {
+    const uint32_t k = k0 + threadIdx.x;
+    const bool valid = k < n;
+    const uint64_t key = valid ? list[k] : 0;
+    const uint32_t a = valid ? (uint32_t)(key >> nbits) : 0xFFFFFFFFu;  // no cell has that index
+    const uint32_t prev = __shfl_up(a, 1, 64);
+    const uint64_t heads = __ballot(lane == 0 || prev != a);
+    const uint64_t below = (2ull << lane) - 1ull;  // lanes 0 .. lane (all 64 at lane 63)
+    const uint32_t head = 63u - (uint32_t)__builtin_clzll(heads & below);
+    const uint64_t above = heads & ~below;
+    uint32_t base = 0;
+    if (valid && head == lane) base = atomicAdd(&cursor[a], (above ? (uint32_t)__builtin_ctzll(above) : 64u) - lane);
+    base = __shfl(base, (int)head, 64);
+    if (valid) src[off[a] + base + (lane - head)] = (uint32_t)(key & (((uint64_t)1 << nbits) - 1));
   }
 }
 
