@@ -374,11 +374,16 @@ struct Batch {
     hipEvent_t ev = nullptr;
     // many-body; the small graphs' exact manyBody and link (from the tick's
     // starting state, which the FMM leaves alone for their nodes) start on
-    // the side stream once the FMM's cells are built, beside its traversal
-    // and field passes rather than beside its sort
+    // the side stream once the FMM's dual traversal is done, beside its field
+    // passes rather than beside its sort and its latency-bound rounds. fork is
+    // recorded here first, so a FMM that returns before its own record still
+    // orders them after the snapshot and the last tick's integrate.
     FmmArgs a = fmm_args();
     a.graph = graph;
-    if (!small.empty()) a.built = fork;
+    if (!small.empty()) {
+      a.built = fork;
+      HIPCHECK(hipEventRecord(fork, stream));
+    }
     hipEvent_t evf = timed ? mark() : nullptr;
     if ((e = fmm_run(fmm, tree, a, stream)) != hipSuccess) return e;
     timer(D3F_PHASE_FMM, evf);

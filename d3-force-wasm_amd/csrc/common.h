@@ -533,8 +533,9 @@ struct FmmArgs {
   // Eager form only: when set, the multipoles run on this stream beside the
   // dual traversal (forked after the geometry, joined before M2L).
   hipStream_t aux = nullptr;
-  // when set, recorded on the stream once the cells are built (a caller's
-  // own work that reads only x, y can start there)
+  // when set, recorded on the stream once the dual traversal is done (a
+  // caller's own work on nodes the FMM does not write can start there; the
+  // FMM may return before recording it, so the caller records it first)
   hipEvent_t built = nullptr;
 };
 // Fast many-body (fmm.zig:247-354) over every graph of the forest at once.

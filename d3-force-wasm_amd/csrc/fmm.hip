@@ -1963,6 +1963,7 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
     }
   }
   hipLaunchKernelGGL(k_fmm_guard, dim3(1), dim3(64), 0, stream, f.params, 1u, 0u, 0u, steps);
+  if (a.built) HIPCHECK(hipEventRecord(a.built, stream));  // the dual traversal is done
   if (!g) {
     // long-segment list, near-field items and partial-sum slots for these lists
     const uint32_t long_need = P.p2p / NEAR_CHUNK + P.m2l / SEG_MAX + 2;
@@ -2014,8 +2015,11 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
   hipLaunchKernelGGL(k_near_items, dim3(grid(nc, EWF, CAP_CELL_BLOCKS)), dim3(EWF), 0, stream, f.cells, ncp,
                      f.p2p_cnt, f.params, f.near_items, f.near_combines, f.cursor, f.slot_cap);
   const uint32_t extra = g ? f.near_extra : (P.m_total / 64 < f.near_extra ? P.m_total / 64 + 1 : f.near_extra);
-  // one block a cell, as eagerly, up to a bound (then the blocks loop)
-  const uint32_t near_blocks = g ? std::min(f.cell_cap, CAP_NEAR_BLOCKS) : grid(nc, 1, 0);
+  // one block a cell, as eagerly; captured, for the cells of the eager tick
+  // before the capture and an eighth more (the blocks loop over any others)
+  const uint32_t seen = P.level_off[FMM_LEVELS + 1];
+  const uint32_t near_blocks =
+      g ? std::min(std::min(f.cell_cap, CAP_NEAR_BLOCKS), std::max(4096u, seen + seen / 8 + 1024)) : grid(nc, 1, 0);
   hipLaunchKernelGGL(k_near<Q>, dim3(extra + near_blocks), dim3(64), 0, stream, f.cells,
                      f.p2p_src, f.p2p_off, f.p2p_cnt, f.params, (const float2*)f.lexp, f.px, f.py, f.ps, f.ux, f.uy,
                      t.perm, a.vx, a.vy, f.graphs, dmin2, dmax2, f.near_items, f.cursor, f.near_partial, extra);
@@ -2157,7 +2161,6 @@ hipError_t fmm_run(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_t 
     HIPCHECK(grow_cells(f, std::max(2 * f.cell_cap, f.params_host->level_off[FMM_LEVELS + 1] + 4096)));
     hipLaunchKernelGGL(k_forest_reset_overflow, dim3(1), dim3(64), 0, stream, f.params);
   }
-  if (a.built) HIPCHECK(hipEventRecord(a.built, stream));
   if (!a.graph) {
     const FmmParams& P = *f.params_host;
     uint32_t deepest = 0;
