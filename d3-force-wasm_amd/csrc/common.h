@@ -446,8 +446,12 @@ struct FmmParams {
   uint32_t level_n[FMM_LEVELS + 2];        // cells per depth
   uint32_t level_off[FMM_LEVELS + 3];      // first cell of each depth
   uint32_t front[FMM_MAX_STEPS + 2];       // dual-tree frontier sizes
-  uint32_t m2l, p2p;                       // list lengths
-  uint32_t list_overflow;
+  // list lengths, each on a cache line of its own: every tile of a dual-tree
+  // round reserves its output with one atomic on each of front[step + 1],
+  // m2l and p2p
+  alignas(128) uint32_t m2l;
+  alignas(128) uint32_t p2p;
+  alignas(128) uint32_t list_overflow;
   uint32_t long_segs;                      // target segments ordered by the long-segment sort
   unsigned long long pairs;                // point pairs of the P2P list
   uint32_t near_items, near_slots, near_combines;  // near-field work items, partial-sum slots, combines
