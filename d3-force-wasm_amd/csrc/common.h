@@ -454,10 +454,14 @@ struct FmmParams {
   alignas(128) uint32_t list_overflow;
   uint32_t long_segs;                      // target segments ordered by the long-segment sort
   unsigned long long pairs;                // point pairs of the P2P list
-  uint32_t near_items, near_slots, near_combines;  // near-field work items, partial-sum slots, combines
-  uint32_t abort;                          // a device-side size check failed: later kernels skip the tick
+  // near-field work items, partial-sum slots, combines (taken by atomics)
+  alignas(128) uint32_t near_items;
+  alignas(128) uint32_t near_slots;
+  alignas(128) uint32_t near_combines;
+  // read by every kernel, apart from the counters above
+  alignas(128) uint32_t abort;             // a device-side size check failed: later kernels skip the tick
   uint32_t ncg;                            // cells, or 0 when aborted (the count the per-cell kernels use)
-  uint32_t big_n[FMM_LEVELS + 1];          // cells of more than FMM_DIRECT points per depth
+  alignas(128) uint32_t big_n[FMM_LEVELS + 1];  // cells of more than FMM_DIRECT points per depth
   uint32_t mid_n;                          // cells of the upward pass's wave-per-cell list
 };
 
