@@ -339,6 +339,9 @@ struct LinkLevelArgs {
 // Copies x, y, vx, vy into the records.
 hipError_t link_pack(const LinkLevelArgs& a, hipStream_t stream);
 hipError_t link_level(const LinkLevelArgs& a, uint32_t lo, uint32_t hi, hipStream_t stream);
+// Levels 1 .. levels, level l being edges off[l] .. off[l + 1] (host offsets):
+// large levels one launch each, runs of small ones in one workgroup.
+hipError_t link_levels(const LinkLevelArgs& a, const uint32_t* off, uint32_t levels, hipStream_t stream);
 // Copies the records' velocities back unless a level raised the zero flag
 // (then vx, vy keep their values from before the pass).
 hipError_t link_unpack(const LinkLevelArgs& a, hipStream_t stream);

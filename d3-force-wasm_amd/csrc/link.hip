@@ -42,13 +42,10 @@ __global__ __launch_bounds__(LINK_THREADS) void k_link_pack(const double* __rest
 // no node. A self-link's delta, (x + vx) - x - vx, is a rounding residue that
 // need not be zero; its source update then applies to the updated target
 // velocity, as in the serial pass.
-__global__ __launch_bounds__(LINK_THREADS) void k_link_level(
-    const uint32_t* __restrict__ src, const uint32_t* __restrict__ tgt, const double* __restrict__ strength,
-    const double* __restrict__ distance, const double* __restrict__ bias, uint32_t lo, uint32_t hi,
-    LinkNode* __restrict__ nodes, double alpha, uint32_t* __restrict__ zero_flag, const double* __restrict__ alpha_dev) {
-  const uint32_t e = lo + blockIdx.x * LINK_THREADS + threadIdx.x;
-  if (e >= hi) return;
-  if (alpha_dev) alpha = *alpha_dev;
+__device__ __forceinline__ void link_edge(const uint32_t* __restrict__ src, const uint32_t* __restrict__ tgt,
+                                          const double* __restrict__ strength, const double* __restrict__ distance,
+                                          const double* __restrict__ bias, uint32_t e, LinkNode* __restrict__ nodes,
+                                          double alpha, uint32_t* __restrict__ zero_flag) {
   const uint32_t s = src[e], t = tgt[e];
   const LinkNode a = nodes[s], c = nodes[t];
   double x = c.x + c.vx - a.x - a.vx;
@@ -68,6 +65,40 @@ __global__ __launch_bounds__(LINK_THREADS) void k_link_level(
   const double2 vs = make_double2(v0.x + x * b, v0.y + y * b);
   if (s != t) *reinterpret_cast<double2*>(&nodes[t].vx) = vt;
   *reinterpret_cast<double2*>(&nodes[s].vx) = vs;
+}
+
+__global__ __launch_bounds__(LINK_THREADS) void k_link_level(
+    const uint32_t* __restrict__ src, const uint32_t* __restrict__ tgt, const double* __restrict__ strength,
+    const double* __restrict__ distance, const double* __restrict__ bias, uint32_t lo, uint32_t hi,
+    LinkNode* __restrict__ nodes, double alpha, uint32_t* __restrict__ zero_flag, const double* __restrict__ alpha_dev) {
+  const uint32_t e = lo + blockIdx.x * LINK_THREADS + threadIdx.x;
+  if (e >= hi) return;
+  if (alpha_dev) alpha = *alpha_dev;
+  link_edge(src, tgt, strength, distance, bias, e, nodes, alpha, zero_flag);
+}
+
+// A run of consecutive small levels in one workgroup, a barrier between
+// levels (the workgroup's waves share one CU and its cache, so a level's
+// writes are seen by the next after the barrier): one launch instead of one
+// per level for the deep levels, which hold few links each.
+constexpr int RUN_THREADS = 1024;
+constexpr uint32_t RUN_MAX_LEVELS = 62;
+constexpr uint32_t RUN_MAX_EDGES = 2 * RUN_THREADS;  // levels of at most this many links join a run
+struct LevelRun {
+  uint32_t count;
+  uint32_t off[RUN_MAX_LEVELS + 1];
+};
+
+__global__ __launch_bounds__(RUN_THREADS) void k_link_run(
+    const uint32_t* __restrict__ src, const uint32_t* __restrict__ tgt, const double* __restrict__ strength,
+    const double* __restrict__ distance, const double* __restrict__ bias, LevelRun run,
+    LinkNode* __restrict__ nodes, double alpha, uint32_t* __restrict__ zero_flag, const double* __restrict__ alpha_dev) {
+  if (alpha_dev) alpha = *alpha_dev;
+  for (uint32_t l = 0; l < run.count; ++l) {
+    for (uint32_t e = run.off[l] + threadIdx.x; e < run.off[l + 1]; e += RUN_THREADS)
+      link_edge(src, tgt, strength, distance, bias, e, nodes, alpha, zero_flag);
+    __syncthreads();
+  }
 }
 
 __global__ __launch_bounds__(LINK_THREADS) void k_link_unpack(const LinkNode* __restrict__ nodes, uint32_t n,
@@ -136,6 +167,32 @@ hipError_t link_level(const LinkLevelArgs& a, uint32_t lo, uint32_t hi, hipStrea
   hipLaunchKernelGGL(k_link_level, dim3(ceil_div(hi - lo, LINK_THREADS)), dim3(LINK_THREADS), 0, stream, a.src,
                      a.tgt, a.strength, a.distance, a.bias, lo, hi, a.nodes, a.alpha, a.zero_flag, a.alpha_dev);
   return hipGetLastError();
+}
+
+hipError_t link_levels(const LinkLevelArgs& a, const uint32_t* off, uint32_t levels, hipStream_t stream) {
+  uint32_t l = 1;
+  while (l <= levels) {
+    if (off[l + 1] - off[l] > RUN_MAX_EDGES) {
+      HIPCHECK(link_level(a, off[l], off[l + 1], stream));
+      ++l;
+      continue;
+    }
+    LevelRun run;
+    run.count = 0;
+    run.off[0] = off[l];
+    while (l <= levels && run.count < RUN_MAX_LEVELS && off[l + 1] - off[l] <= RUN_MAX_EDGES) {
+      run.off[++run.count] = off[l + 1];
+      ++l;
+    }
+    if (run.count == 1) {
+      HIPCHECK(link_level(a, run.off[0], run.off[1], stream));
+      continue;
+    }
+    hipLaunchKernelGGL(k_link_run, dim3(1), dim3(RUN_THREADS), 0, stream, a.src, a.tgt, a.strength, a.distance,
+                       a.bias, run, a.nodes, a.alpha, a.zero_flag, a.alpha_dev);
+    HIPCHECK(hipGetLastError());
+  }
+  return hipSuccess;
 }
 
 hipError_t link_unpack(const LinkLevelArgs& a, hipStream_t stream) {

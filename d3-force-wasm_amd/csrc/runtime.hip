@@ -657,11 +657,8 @@ int32_t run_link_exact(Sim& s, Force& f, double alpha) {
   const uint32_t levels = (uint32_t)f.level_off.size() - 2;
   s.timer_start(D3F_PHASE_LINK);
   if ((e = link_pack(a, s.stream)) != hipSuccess) return status_of(e);
-  for (double k = 0; k < iterations; k += 1) {
-    for (uint32_t l = 1; l <= levels; ++l) {
-      if ((e = link_level(a, f.level_off[l], f.level_off[l + 1], s.stream)) != hipSuccess) return status_of(e);
-    }
-  }
+  for (double k = 0; k < iterations; k += 1)
+    if ((e = link_levels(a, f.level_off.data(), levels, s.stream)) != hipSuccess) return status_of(e);
   if ((e = link_unpack(a, s.stream)) != hipSuccess) return status_of(e);
   if ((e = hipMemcpyAsync(s.hscalar_u + 1, s.dscalar_u + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream)) !=
       hipSuccess)
@@ -1187,7 +1184,7 @@ hipError_t graph_tick(Sim& s, const std::vector<Force*>& fs, double decay) {
           const uint32_t levels = (uint32_t)f.level_off.size() - 2;
           HIPCHECK(link_pack(a, st));
           for (double k = 0; k < iterations; k += 1)
-            for (uint32_t l = 1; l <= levels; ++l) HIPCHECK(link_level(a, f.level_off[l], f.level_off[l + 1], st));
+            HIPCHECK(link_levels(a, f.level_off.data(), levels, st));
           HIPCHECK(link_unpack(a, st));
           note(s.dscalar_u + 1, TF_LINK_ZERO);
         }
