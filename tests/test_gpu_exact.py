@@ -90,3 +90,27 @@ def test_gpu_exact_1m_pinned_bit_identical(gpu_core, oracle_core):
         b.tick()
         assert same(na.state(), nb.state()), "tick %d" % (t + 1)
     oracle_core.set_threads(1)
+
+
+def test_gpu_exact_hub_links_many_levels(gpu_core, oracle_core):
+    # A hub linked to 700 nodes puts each of its links in a level of its own
+    # (link.zig:14-38 runs the links in order): hundreds of small levels,
+    # which run in one workgroup per 62 levels, with ordinary levels between
+    print("gpu exact hub links", flush=True)
+    n = 4000
+    src, tgt = random_graph(n, 6000, seed=3)
+    hub = np.arange(1, 701, dtype=src.dtype)
+    src = np.concatenate([src[:3000], np.zeros(700, src.dtype), src[3000:], hub[::-1]])
+    tgt = np.concatenate([tgt[:3000], hub, tgt[3000:], np.full(700, 2, tgt.dtype)])
+    states = []
+    for core in (oracle_core, gpu_core):
+        nodes = d3.Nodes(n)
+        sim = (d3.forceSimulation(nodes, core=core).force("charge", d3.forceManyBody())
+               .force("link", d3.forceLink((src, tgt)).iterations(2)).force("center", d3.forceCenter()))
+        out = []
+        for k in (1, 1, 5, 5):
+            sim.tick(k)
+            out.append(nodes.state())
+        states.append(out)
+    for c, (a, b) in enumerate(zip(*states)):
+        assert same(a, b), "after call %d" % (c + 1)
