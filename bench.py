@@ -545,7 +545,7 @@ def main():
                  "integrate": 81 * n, "center_sum": 16 * n}
     names = {"traversal": "k_traverse<RECORD> (Barnes-Hut traversal)", "center_sum": "k_cs_chain (center sums, side stream)",
              "tree": "quadtree build (13 kernels)", "accumulate": "k_acc_subtrees+k_acc_top (two launches)",
-             "link": "k_link_level (all levels)", "integrate": "k_integrate"}
+             "link": "k_link_level+k_link_run (all levels)", "integrate": "k_integrate"}
     kernels = {}
     floors = {}
     for ph, ms in phase_ms.items():
