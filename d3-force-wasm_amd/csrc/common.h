@@ -443,12 +443,15 @@ struct FmmGraph {
 };
 
 // Device-resident scalars of one fast many-body run.
+constexpr uint32_t FRONT_STRIDE = 32;  // words per frontier count (128 bytes)
 struct FmmParams {
   uint32_t n, m_total, roots, declined;    // nodes, points in trees, graphs with a tree, non-finite extents
   uint32_t cell_overflow;
   uint32_t level_n[FMM_LEVELS + 2];        // cells per depth
   uint32_t level_off[FMM_LEVELS + 3];      // first cell of each depth
-  uint32_t front[FMM_MAX_STEPS + 2];       // dual-tree frontier sizes
+  // dual-tree frontier sizes, one a cache line (round s at s * FRONT_STRIDE):
+  // a round's blocks read its size while they add to the next round's
+  alignas(128) uint32_t front[(FMM_MAX_STEPS + 2) * FRONT_STRIDE];
   // list lengths, each on a cache line of its own: every tile of a dual-tree
   // round reserves its output with one atomic on each of front[step + 1],
   // m2l and p2p

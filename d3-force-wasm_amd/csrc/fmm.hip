@@ -945,7 +945,7 @@ __device__ __forceinline__ void dual_pair(const FCell* __restrict__ cells, uint3
   }
 }
 
-// One round of fmm.zig:400-453 over the frontier fp->front[step]. A block
+// One round of fmm.zig:400-453 over the frontier of round step (its size at fp->front[step * FRONT_STRIDE]). A block
 // classifies a tile of STEP_TILE pairs, scans its output counts and reserves
 // room in each list with one atomic per list and tile. A frontier that
 // overflowed its buffer voids the pass (it is redone after the buffers grow,
@@ -962,7 +962,7 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
   __shared__ uint32_t base_sh[3];
   __shared__ unsigned long long pairs_sh[EWF / 64];
   if (fp->abort || (fp->list_overflow & 1u)) return;
-  uint32_t cnt = fp->front[step];
+  uint32_t cnt = fp->front[step * FRONT_STRIDE];
   if (cnt > front_cap) cnt = front_cap;
   unsigned long long block_pairs = 0;  // thread 0: the block's P2P point pairs, added once at the end
   for (uint32_t tile = blockIdx.x * STEP_TILE; tile < cnt; tile += gridDim.x * STEP_TILE) {
@@ -997,7 +997,7 @@ __global__ __launch_bounds__(EWF) void k_dual_step(const FCell* __restrict__ cel
     for (int d = 32; d > 0; d >>= 1) pairs += __shfl_xor(pairs, d, 64);
     if ((threadIdx.x & 63) == 0) pairs_sh[threadIdx.x >> 6] = pairs;
     if (threadIdx.x == 0) {
-      base_sh[0] = t_front ? atomicAdd(&fp->front[step + 1], t_front) : 0;
+      base_sh[0] = t_front ? atomicAdd(&fp->front[(step + 1) * FRONT_STRIDE], t_front) : 0;
       base_sh[1] = t_m2l ? atomicAdd(&fp->m2l, t_m2l) : 0;
       base_sh[2] = t_p2p ? atomicAdd(&fp->p2p, t_p2p) : 0;
     }
@@ -1064,7 +1064,7 @@ __global__ __launch_bounds__(EWF) void k_fmm_lists_reset(FmmParams* __restrict__
   const uint32_t t = blockIdx.x * EWF + threadIdx.x;
   if (t < roots) front[t] = make_uint2(t, t);
   if (blockIdx.x == 0) {
-    if (threadIdx.x < FMM_MAX_STEPS + 2) fp->front[threadIdx.x] = threadIdx.x == 0 ? roots : 0u;
+    if (threadIdx.x < FMM_MAX_STEPS + 2) fp->front[threadIdx.x * FRONT_STRIDE] = threadIdx.x == 0 ? roots : 0u;
     if (threadIdx.x == 0) {
       fp->m2l = 0;
       fp->p2p = 0;
@@ -1701,7 +1701,7 @@ __global__ void k_fmm_guard(FmmParams* __restrict__ fp, uint32_t stage, uint32_t
                  (last_step < FMM_LEVELS && fp->level_n[last_step] != 0))
                     ? 1u
                     : 0u;
-  } else if (fp->list_overflow || fp->front[last_step] != 0) {
+  } else if (fp->list_overflow || fp->front[last_step * FRONT_STRIDE] != 0) {
     fp->abort |= 2u;
   }
   fp->ncg = fp->abort ? 0u : nc;
@@ -1939,11 +1939,11 @@ hipError_t run_order(FmmBuffers& f, TreeBuffers& t, const FmmArgs& a, hipStream_
     HIPCHECK(hipMemcpyAsync(f.params_host, f.params, sizeof(FmmParams), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     if (!P.list_overflow) {
-      if (P.front[steps] != 0) HIPFAIL(hipErrorUnknown, "dual traversal not drained");  // the rounds must drain the recursion
+      if (P.front[steps * FRONT_STRIDE] != 0) HIPFAIL(hipErrorUnknown, "dual traversal not drained");  // the rounds must drain the recursion
       break;
     }
     uint32_t fmax = 0;
-    for (uint32_t s = 0; s <= steps; ++s) fmax = std::max(fmax, P.front[s]);
+    for (uint32_t s = 0; s <= steps; ++s) fmax = std::max(fmax, P.front[s * FRONT_STRIDE]);
     if (P.list_overflow & 1u) {
       const uint32_t c = std::max(2 * f.front_cap, fmax + fmax / 4);
       HIPCHECK(grow(f.front_a, c));
